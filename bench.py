@@ -1,0 +1,182 @@
+#!/usr/bin/env python
+"""Headline benchmark: closest-hit traversal throughput on one MI355X.
+
+Workload (the raw traversal microbench): 2^20 random triangles (centroids
+U[-1,1]^3, vertices offset by N(0, 0.01^2)), a binned-SAH BVH (32 bins,
+leaf <= 4) built on the GPU, and 2^24 rays (origins U[-1.5,1.5]^3, directions
+uniform on the sphere): the first half closest-hit with t_max = inf, the
+second half any-hit shadow rays with t_max ~ U[0.1, 3]. Seeds 1 (triangles)
+and 2 (rays).
+
+A step is one closest-hit pass over the 2^23 closest-hit rays, inputs
+resident in HBM; value = closest-hit Mrays/s over K timed steps. The any-hit
+pass is timed the same way in its own block and reported beside it.
+
+Roofline: algorithmic bytes per closest-hit ray =
+32 (ray) + 8 (hit) + 32 * N_node + 36 * N_tri, with N_node / N_tri the node
+fetches and triangle tests of the reference traversal (geometry/bvh.rs:86-143)
+on the reference build (32 bins, leaf 4) of the same triangles, measured by
+the oracle on the first 2^20 closest-hit rays.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+METRIC = "Mrays/s closest-hit; seconds to relMSE 0.01 at 1080p, plain vs NRC"
+N_TRIS = 1 << 20
+N_RAYS = 1 << 24
+N_CLOSEST = 1 << 23
+BINS, MAX_LEAF = 32, 4
+# Reference traversal statistics on this workload (oracle, 2^20 closest rays).
+N_NODE_CLOSEST = 61.5237
+N_TRI_CLOSEST = 2.5797
+N_NODE_ANY = 57.9721
+N_TRI_ANY = 2.4079
+BYTES_CLOSEST = 32 + 8 + 32 * N_NODE_CLOSEST + 36 * N_TRI_CLOSEST
+BYTES_ANY = 32 + 4 + 32 * N_NODE_ANY + 36 * N_TRI_ANY
+HBM_PEAK_GBS = 8000.0
+CPU_SAMPLE = 1 << 22
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(tris, rays36, sample):
+    """Oracle (C++ restatement of the reference) on the host cores."""
+    from oracle import oracle as O
+    if not os.path.exists(O.LIB_PATH):
+        O.build()
+    threads = O.nthreads()
+    sc = O.Scene()
+    sc.add_material(O.material(0, (0.5, 0.5, 0.5)))
+    t0 = time.perf_counter()
+    sc.add_mesh(tris.reshape(-1, 3), np.arange(3 * len(tris), dtype=np.uint32).reshape(-1, 3), None, 0, BINS, MAX_LEAF)
+    build_s = time.perf_counter() - t0
+    sub = rays36[:sample]
+    t0 = time.perf_counter()
+    out = sc.intersect(sub, threads=threads)
+    dt = time.perf_counter() - t0
+    return sc, out, dict(value=round(sample / dt / 1e6, 3), unit="Mrays/s", cores=threads, kind="port",
+                         sample=f"first {sample} closest-hit rays of the workload, oracle BVH "
+                                f"(reference build, {BINS} bins, leaf {MAX_LEAF}; build {build_s:.1f} s untimed)")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity spot check")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(rays and pixels shard with no exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import pyre_amd
+
+    ctx = pyre_amd.Context(0)
+    dev_name, cus = ctx.device_info()
+    log(f"device {dev_name} cus={cus}")
+    tris = pyre_amd.random_triangles(1, N_TRIS, 0.01)
+    rays32 = pyre_amd.random_rays(2, N_RAYS, N_CLOSEST, "ray32")
+    scene = pyre_amd.Scene.from_arrays(
+        ctx, [dict(positions=tris.reshape(-1, 3), indices=np.arange(3 * N_TRIS, dtype=np.uint32).reshape(-1, 3),
+                   material_id=0)], [pyre_amd.Material(pyre_amd.MAT_LAMBERTIAN, (0.5, 0.5, 0.5))])
+    info = scene.build_bvh(bins=BINS, max_leaf=MAX_LEAF)
+    build_ms = [info["build_ms"]]
+    for _ in range(2):
+        build_ms.append(scene.build_bvh(bins=BINS, max_leaf=MAX_LEAF)["build_ms"])
+    info = scene.bvh_info()
+    log(f"bvh nodes={info['n_nodes']} leaves={info['n_leaves']} depth={info['depth']} build_ms={build_ms}")
+
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        d_rays = torch.from_numpy(rays32).to("cuda")
+        closest = d_rays[:N_CLOSEST]
+        shadow = d_rays[N_CLOSEST:]
+        hits = torch.empty((N_CLOSEST, 2), dtype=torch.int32, device="cuda")
+        occ = torch.empty(N_RAYS - N_CLOSEST, dtype=torch.int32, device="cuda")
+    sh = stream.cuda_stream
+    torch.cuda.synchronize()
+
+    def closest_pass():
+        scene.trace_closest_dev(closest.data_ptr(), N_CLOSEST, hits.data_ptr(), sh)
+
+    def any_pass():
+        scene.trace_any_dev(shadow.data_ptr(), N_RAYS - N_CLOSEST, occ.data_ptr(), sh)
+
+    for _ in range(args.warmup):
+        closest_pass()
+        any_pass()
+    torch.cuda.synchronize()
+
+    def timed(fn, k):
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(k)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for a, b in evs:
+            a.record(stream)
+            fn()
+            b.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        return wall, float(np.mean([a.elapsed_time(b) for a, b in evs]))
+
+    wall_c, kern_c = timed(closest_pass, args.steps)
+    log(f"closest: wall {wall_c:.3f}s kernel {kern_c:.3f} ms/launch")
+    wall_a, kern_a = timed(any_pass, args.steps)
+    log(f"any: wall {wall_a:.3f}s kernel {kern_a:.3f} ms/launch")
+
+    value = N_CLOSEST * args.steps / wall_c / 1e6
+    achieved = N_CLOSEST * BYTES_CLOSEST / (kern_c * 1e-3) / 1e9
+    line = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall_c / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded triangle soup and rays; no dataset)",
+        "config": {"workload": "raw traversal microbench: 2^20 random triangles, GPU binned-SAH BVH (32 bins, "
+                               "leaf<=4), 2^24 rays: 2^23 closest-hit + 2^23 any-hit",
+                   "n_triangles": N_TRIS, "n_rays": N_RAYS, "bins": BINS, "max_leaf": MAX_LEAF},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_ray": round(BYTES_CLOSEST, 1), "kernel_ms": round(kern_c, 4)},
+        "anyhit_mrays_s": round((N_RAYS - N_CLOSEST) * args.steps / wall_a / 1e6, 2),
+        "anyhit_kernel_ms": round(kern_a, 4),
+        "bvh_build_ms": round(min(build_ms), 2),
+        "bvh": {"nodes": info["n_nodes"], "leaves": info["n_leaves"], "depth": info["depth"]},
+        "device": dev_name,
+    }
+    if not args.no_cpu:
+        rays36 = pyre_amd.random_rays(2, N_CLOSEST, N_CLOSEST)
+        osc, o, cb = cpu_baseline(tris, rays36, CPU_SAMPLE)
+        line["cpu_baseline"] = cb
+        # parity spot check on the first 2^16 rays: hit ids and t bit-exact
+        h = hits[: 1 << 16].cpu().numpy()
+        g_prim = h[:, 0].view(np.uint32)
+        g_t = h[:, 1].view(np.uint32)
+        o_prim = np.where(o["kind"][: 1 << 16] == 1, o["tri"][: 1 << 16], 0xFFFFFFFF).astype(np.uint32)
+        o_t = np.where(o["kind"][: 1 << 16] == 1, o["t"][: 1 << 16], 0).astype(np.float32).view(np.uint32)
+        line["parity_spot_check"] = {"rays": 1 << 16, "prim_mismatch": int((g_prim != o_prim).sum()),
+                                     "t_mismatch": int(((g_t != o_t) & (g_prim == o_prim)).sum())}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

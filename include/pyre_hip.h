@@ -1,0 +1,253 @@
+/*
+ * pyre_hip.h -- C ABI of the MI355X (gfx950) path-tracing backend for Pyre.
+ *
+ * The backend replaces the body of the reference CLI's render closure
+ * (crates/pyre-cli/src/main.rs:253-287) and the hot-path traits it calls:
+ *   Shape::intersect          crates/pyre/src/geometry/mod.rs:19-21
+ *   Bvh::build / intersect    crates/pyre/src/geometry/bvh.rs:45-64, 86-143
+ *   Scene::intersect/occluded crates/pyre/src/scene.rs:57-140
+ *   Bsdf eval/pdf/sample      crates/pyre/src/material.rs:23-36
+ *   Light / EnvironmentLight  crates/pyre/src/light.rs:36-42, 185-193
+ *   PathIntegrator::integrate crates/pyre/src/integrator.rs:57-231
+ *   Film::render              crates/pyre/src/film.rs:96-109
+ *
+ * Conventions
+ *   - Every entry point returns PYRE_OK (0) or a negative PYRE_E* status and
+ *     never unwinds; pyre_last_error() returns the text of the last failure.
+ *     The reference panics where this library returns a status
+ *     (bvh.rs:46 length assert, bvh.rs:99 stack overflow,
+ *     distribution.rs:131 and light.rs:215 buffer-size asserts).
+ *   - Host pointers are owned by the caller and copied in/out. Entry points
+ *     whose name ends in _dev take device pointers (memory on the context's
+ *     device) and enqueue on the given HIP stream (NULL = the context stream)
+ *     without synchronising.
+ *   - A context is used by one host thread at a time. A scene is read-only
+ *     after upload + build.
+ *   - All arithmetic on the hot path is IEEE FP32 without contraction
+ *     (-ffp-contract=off), matching the reference's x86-64 build.
+ */
+#ifndef PYRE_HIP_H
+#define PYRE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PYRE_OK 0
+#define PYRE_EINVAL (-1)    /* bad argument or descriptor */
+#define PYRE_EHIP (-2)      /* HIP runtime failure */
+#define PYRE_ENOMEM (-3)    /* device allocation failed */
+#define PYRE_EOVERFLOW (-4) /* traversal stack or queue could not be grown */
+#define PYRE_ESTATE (-5)    /* call out of order (e.g. trace before build) */
+
+#define PYRE_MISS 0xFFFFFFFFu
+
+typedef struct pyre_ctx pyre_ctx;
+typedef struct pyre_scene pyre_scene;
+
+/* Ray (crates/pyre/src/math.rs:3-13), 36 bytes. */
+typedef struct pyre_ray {
+  float o[3];
+  float d[3];
+  float t_min;
+  float t_max;
+  float time;
+} pyre_ray;
+
+/* Device ray record for the _dev trace entry points, 32 bytes:
+ * {o.x, o.y, o.z, t_min, d.x, d.y, d.z, t_max}. */
+typedef struct pyre_ray32 {
+  float o[3];
+  float t_min;
+  float d[3];
+  float t_max;
+} pyre_ray32;
+
+/* Closest hit. inst = primitive index in the scene (scene.rs:37-40
+ * HitKind::Surface primitive_id), tri = triangle index within that
+ * primitive's mesh (mesh.rs:124, dropped by the reference), t/u/v as
+ * returned by moller_trumbore (mesh.rs:153-178). inst = PYRE_MISS on a miss. */
+typedef struct pyre_hit {
+  uint32_t inst;
+  uint32_t tri;
+  float t;
+  float u;
+  float v;
+} pyre_hit;
+
+/* Compact device hit for the _dev closest-hit entry point, 8 bytes:
+ * prim = scene-global triangle id (PYRE_MISS on a miss), t. */
+typedef struct pyre_hit8 {
+  uint32_t prim;
+  float t;
+} pyre_hit8;
+
+#define PYRE_MAT_LAMBERTIAN 0u /* material.rs:42-77 (base_color = albedo) */
+#define PYRE_MAT_DISNEY 1u     /* material.rs:83-226 */
+
+typedef struct pyre_material {
+  uint32_t kind;
+  float base_color[3];
+  float metallic;
+  float roughness;
+  float specular;
+} pyre_material;
+
+/* DiffuseAreaQuadLight::new arguments (light.rs:61-76). */
+typedef struct pyre_quad_light {
+  float p0[3];
+  float edge_u[3];
+  float edge_v[3];
+  float emission[3];
+} pyre_quad_light;
+
+/* One scene primitive (scene.rs:19-22): an indexed triangle mesh in world
+ * space (TriangleMesh, geometry/mesh.rs:5-11) and its material id.
+ * normals may be NULL (face normals are used, mesh.rs:131-136). */
+typedef struct pyre_mesh_desc {
+  const float* positions;  /* 3 * n_vertices */
+  const uint32_t* indices; /* 3 * n_triangles */
+  const float* normals;    /* 3 * n_vertices or NULL */
+  uint32_t n_vertices;
+  uint32_t n_triangles;
+  uint32_t material_id;
+} pyre_mesh_desc;
+
+/* HdriEnvironmentLight::new arguments (light.rs:214-242): row-major RGB32F. */
+typedef struct pyre_env_desc {
+  uint32_t width;
+  uint32_t height;
+  const float* rgb; /* 3 * width * height */
+  float intensity;
+} pyre_env_desc;
+
+typedef struct pyre_scene_desc {
+  const pyre_mesh_desc* meshes;
+  uint32_t n_meshes;
+  const pyre_material* materials;
+  uint32_t n_materials;
+  const pyre_quad_light* lights;
+  uint32_t n_lights;
+  const pyre_env_desc* env; /* NULL = no environment (scene.rs:33) */
+} pyre_scene_desc;
+
+/* Binned-SAH parameters (bvh.rs:13-16): bins (2..64), max primitives per
+ * leaf, traversal and intersection costs. The reference uses 12/8/1/2. */
+typedef struct pyre_bvh_params {
+  uint32_t bins;
+  uint32_t max_leaf;
+  float traversal_cost;
+  float intersect_cost;
+} pyre_bvh_params;
+
+/* Camera (camera.rs:29-143). aperture_radius > 0 selects the thin lens
+ * (main.rs:206-224); focus_distance <= 0 means |target - origin|
+ * (main.rs:203-205). */
+typedef struct pyre_camera_desc {
+  float origin[3];
+  float target[3];
+  float up[3];
+  float vfov_deg;
+  float aspect;
+  float aperture_radius;
+  float focus_distance;
+} pyre_camera_desc;
+
+/* PathIntegrator (integrator.rs:36-48). */
+typedef struct pyre_integrator_params {
+  uint32_t max_depth;
+  uint32_t min_rr_depth;
+} pyre_integrator_params;
+
+/* PixelSample (integrator.rs:14-20), 40 bytes, same field order. */
+typedef struct pyre_pixel_sample {
+  float radiance[3];
+  float albedo[3];
+  float normal[3];
+  float depth;
+} pyre_pixel_sample;
+
+/* ---------------------------------------------------------------- context */
+int pyre_ctx_create(int device, pyre_ctx** out);
+void pyre_ctx_destroy(pyre_ctx* ctx);
+const char* pyre_last_error(const pyre_ctx* ctx);
+/* Device name and CU count of the context's GPU (for reports). */
+int pyre_ctx_device_info(pyre_ctx* ctx, char* name, size_t name_len, int* cu_count);
+
+/* ------------------------------------------------------------------ scene */
+/* Copies a flattened Scene (scene.rs:24-34) to the device. The triangle
+ * arrays become one scene-global triangle list (primitive by primitive, in
+ * order); the environment's marginal/conditional CDFs
+ * (distribution.rs:39-62, 130-147) are built on the device in the
+ * reference's sequential f32 order. */
+int pyre_scene_upload(pyre_ctx* ctx, const pyre_scene_desc* desc, pyre_scene** out);
+void pyre_scene_destroy(pyre_scene* scene);
+/* Scene::bounds (scene.rs:146-152) over the uploaded geometry. */
+int pyre_scene_bounds(pyre_ctx* ctx, pyre_scene* scene, float out_min[3], float out_max[3]);
+
+/* GPU binned-SAH build over all triangles of the scene (replaces
+ * Bvh::build, bvh.rs:45-64, called per mesh by MeshInstance::build,
+ * mesh.rs:77-87). Split decisions follow bvh.rs:180-346; nodes are emitted
+ * breadth-first. */
+int pyre_bvh_build(pyre_ctx* ctx, pyre_scene* scene, const pyre_bvh_params* params);
+int pyre_bvh_info(pyre_ctx* ctx, pyre_scene* scene, uint32_t* n_nodes, uint32_t* n_leaves, uint32_t* depth,
+                  double* build_ms);
+/* Exports the built tree for checking against the reference build:
+ * per node bounds (6 floats: min, max), left child (interior) or first
+ * primitive slot (leaf), right child (interior) or primitive count (leaf),
+ * is_leaf, split axis; prim_order = scene-global triangle ids in leaf order. */
+int pyre_bvh_export(pyre_ctx* ctx, pyre_scene* scene, float* bounds6, uint32_t* a, uint32_t* b, uint32_t* is_leaf,
+                    uint32_t* axis, uint32_t* prim_order);
+
+/* ---------------------------------------------------------------- tracing */
+/* Batched Scene::intersect over the scene's triangles (scene.rs:57-94 minus
+ * the light loop): closest hit with t in (t_min, t_max); exact ties in t
+ * resolve to the lower primitive index like scene.rs:65. */
+int pyre_intersect_closest(pyre_ctx* ctx, pyre_scene* scene, const pyre_ray* rays, size_t n, pyre_hit* hits);
+/* Batched occlusion (scene.rs:102-140): occluded[i] = 1 iff any triangle has
+ * a hit with t in (t_min, t_max). */
+int pyre_intersect_any(pyre_ctx* ctx, pyre_scene* scene, const pyre_ray* rays, size_t n, uint32_t* occluded);
+/* Device-resident forms: rays = pyre_ray32[n], hits = pyre_hit8[n] /
+ * occluded = uint32_t[n]; stream = hipStream_t or NULL. */
+int pyre_trace_closest_dev(pyre_ctx* ctx, pyre_scene* scene, const void* rays, size_t n, void* hits, void* stream);
+int pyre_trace_any_dev(pyre_ctx* ctx, pyre_scene* scene, const void* rays, size_t n, void* occluded, void* stream);
+
+/* -------------------------------------------------- shade-stage probes */
+/* Bsdf::eval + Bsdf::pdf (material.rs:23-36) on local-frame directions:
+ * wo, wi = 3n floats; f = 3n, pdf = n. */
+int pyre_bsdf_eval_batch(pyre_ctx* ctx, const pyre_material* m, const float* wo, const float* wi, size_t n, float* f,
+                         float* pdf);
+/* Bsdf::sample: u = 2n floats; ok[i] = 0 where the reference returns None. */
+int pyre_bsdf_sample_batch(pyre_ctx* ctx, const pyre_material* m, const float* wo, const float* u, size_t n,
+                           float* wi, float* f, float* pdf, uint32_t* ok);
+/* EnvironmentLight::sample / pdf / le of the scene's HDRI (light.rs:317-350). */
+int pyre_env_sample_batch(pyre_ctx* ctx, pyre_scene* scene, const float* u, size_t n, float* wi, float* li,
+                          float* pdf);
+int pyre_env_pdf_batch(pyre_ctx* ctx, pyre_scene* scene, const float* wi, size_t n, float* pdf);
+int pyre_env_le_batch(pyre_ctx* ctx, pyre_scene* scene, const float* wi, size_t n, float* le);
+/* Light::sample / pdf of quad light `light` (light.rs:110-144):
+ * from = 3n, u = 2n; outputs pos 3n, wi 3n, dist n, pdf n, ok n. */
+int pyre_light_sample_batch(pyre_ctx* ctx, pyre_scene* scene, uint32_t light, const float* from, const float* u,
+                            size_t n, float* pos, float* wi, float* dist, float* pdf, uint32_t* ok);
+int pyre_light_pdf_batch(pyre_ctx* ctx, pyre_scene* scene, uint32_t light, const float* from, const float* wi,
+                         size_t n, float* pdf);
+
+/* ----------------------------------------------------------------- render */
+/* Replaces main.rs:253-287: renders samples [spp_offset, spp_offset + spp)
+ * of every pixel with the per-sample streams pixel_seed(x, y, s)
+ * (sampler.rs:41-49) and writes the per-pixel mean (PixelSample) to out
+ * (width*height, row-major, (0,0) top-left) if out != NULL. If accum64 !=
+ * NULL it receives, added to its current contents, the FP64 sum of sample
+ * radiance (3 doubles per pixel) for reference-quality accumulation. */
+int pyre_render(pyre_ctx* ctx, pyre_scene* scene, const pyre_camera_desc* camera,
+                const pyre_integrator_params* integrator, uint32_t width, uint32_t height, uint32_t spp,
+                uint32_t spp_offset, pyre_pixel_sample* out, double* accum64);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYRE_HIP_H */

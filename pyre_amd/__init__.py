@@ -1,0 +1,407 @@
+"""pyre_amd -- MI355X (gfx950) path-tracing backend for Pyre.
+
+Python view of the C ABI in include/pyre_hip.h (libpyre_hip.so). The names
+mirror the reference's public surface (crates/pyre/src/lib.rs:22-35):
+Scene / PathIntegrator / Film / camera / Bsdf probes. Every call runs the HIP
+implementation; when the library is missing, import succeeds but the first
+call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import PyreError
+
+__all__ = [
+    "Context", "Scene", "Preset", "PathIntegrator", "Film", "Camera", "Material", "PyreError",
+    "random_triangles", "random_rays", "auto_frame", "MAT_LAMBERTIAN", "MAT_DISNEY", "MISS",
+]
+
+MAT_LAMBERTIAN = L.MAT_LAMBERTIAN
+MAT_DISNEY = L.MAT_DISNEY
+MISS = L.PYRE_MISS
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+class Context:
+    """pyre_ctx: one GPU, one stream."""
+
+    def __init__(self, device: int = 0):
+        self.lib = L.load()
+        h = C.c_void_p()
+        rc = self.lib.pyre_ctx_create(device, C.byref(h))
+        if rc != 0:
+            raise PyreError(rc, f"pyre_ctx_create({device}) failed")
+        self.h = h
+
+    def check(self, rc: int, what: str):
+        if rc != 0:
+            msg = self.lib.pyre_last_error(self.h)
+            raise PyreError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    def device_info(self):
+        buf = C.create_string_buffer(128)
+        cu = C.c_int()
+        self.check(self.lib.pyre_ctx_device_info(self.h, buf, 128, C.byref(cu)), "device_info")
+        return buf.value.decode(), cu.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.pyre_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class Material:
+    """pyre_material: LAMBERTIAN (material.rs:42-77) or DISNEY (:83-226)."""
+    kind: int
+    base_color: tuple
+    metallic: float = 0.0
+    roughness: float = 0.0
+    specular: float = 0.0
+
+    def to_c(self) -> L.Material:
+        return L.Material(self.kind, (C.c_float * 3)(*self.base_color), self.metallic, self.roughness, self.specular)
+
+
+@dataclass
+class Camera:
+    """pyre_camera_desc (camera.rs): aperture_radius > 0 -> thin lens."""
+    origin: tuple
+    target: tuple
+    up: tuple = (0.0, 1.0, 0.0)
+    vfov_deg: float = 45.0
+    aspect: float = 1.0
+    aperture_radius: float = 0.0
+    focus_distance: float = 0.0
+
+    def to_c(self) -> L.CameraDesc:
+        f3 = C.c_float * 3
+        return L.CameraDesc(f3(*self.origin), f3(*self.target), f3(*self.up), self.vfov_deg, self.aspect,
+                            self.aperture_radius, self.focus_distance)
+
+    @classmethod
+    def from_c(cls, c: L.CameraDesc) -> "Camera":
+        return cls(tuple(c.origin), tuple(c.target), tuple(c.up), c.vfov_deg, c.aspect, c.aperture_radius,
+                   c.focus_distance)
+
+
+@dataclass
+class PathIntegrator:
+    """PathIntegrator (integrator.rs:36-48)."""
+    max_depth: int = 8
+    min_rr_depth: int = 3
+
+
+class Preset:
+    """Host-built scenes (pyre_gen.h): the CLI's Cornell box and the
+    synthetic production scene."""
+
+    def __init__(self, handle, lib):
+        self.h = handle
+        self.lib = lib
+
+    @classmethod
+    def cornell(cls) -> "Preset":
+        lib = L.load()
+        h = C.c_void_p()
+        rc = lib.pyre_preset_cornell(0, C.byref(h))
+        if rc:
+            raise PyreError(rc, "pyre_preset_cornell")
+        return cls(h, lib)
+
+    @classmethod
+    def production(cls, seed=3, n_instances=64, level=7, env_w=2048, env_h=1024) -> "Preset":
+        lib = L.load()
+        h = C.c_void_p()
+        rc = lib.pyre_preset_production(seed, n_instances, level, env_w, env_h, C.byref(h))
+        if rc:
+            raise PyreError(rc, "pyre_preset_production")
+        return cls(h, lib)
+
+    def desc(self) -> L.SceneDesc:
+        d = L.SceneDesc()
+        self.lib.pyre_preset_desc(self.h, C.byref(d))
+        return d
+
+    def camera(self, aspect: float, aperture: float = 0.0) -> Camera:
+        c = L.CameraDesc()
+        self.lib.pyre_preset_camera(self.h, aspect, aperture, C.byref(c))
+        return Camera.from_c(c)
+
+    def arrays(self):
+        """Copies of the preset's meshes/materials/lights/env as numpy arrays."""
+        d = self.desc()
+        meshes = []
+        for i in range(d.n_meshes):
+            m = d.meshes[i]
+            pos = np.ctypeslib.as_array(m.positions, (m.n_vertices, 3)).copy()
+            idx = np.ctypeslib.as_array(m.indices, (m.n_triangles, 3)).copy()
+            nrm = np.ctypeslib.as_array(m.normals, (m.n_vertices, 3)).copy() if m.normals else None
+            meshes.append(dict(positions=pos, indices=idx, normals=nrm, material_id=m.material_id))
+        mats = [Material(d.materials[i].kind, tuple(d.materials[i].base_color), d.materials[i].metallic,
+                         d.materials[i].roughness, d.materials[i].specular) for i in range(d.n_materials)]
+        lights = [np.array([*d.lights[i].p0, *d.lights[i].edge_u, *d.lights[i].edge_v, *d.lights[i].emission],
+                           np.float32) for i in range(d.n_lights)]
+        env = None
+        if d.env:
+            e = d.env.contents
+            env = dict(width=e.width, height=e.height, intensity=e.intensity,
+                       rgb=np.ctypeslib.as_array(e.rgb, (e.height, e.width, 3)).copy())
+        return dict(meshes=meshes, materials=mats, lights=lights, env=env)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.pyre_preset_destroy(self.h)
+            self.h = None
+
+
+class Scene:
+    """A scene uploaded to the device (pyre_scene) plus its BVH."""
+
+    def __init__(self, ctx: Context, desc: L.SceneDesc, keep=None):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        h = C.c_void_p()
+        ctx.check(self.lib.pyre_scene_upload(ctx.h, C.byref(desc), C.byref(h)), "pyre_scene_upload")
+        self.h = h
+        self._keep = keep
+
+    @classmethod
+    def from_preset(cls, ctx: Context, preset: Preset) -> "Scene":
+        return cls(ctx, preset.desc(), keep=preset)
+
+    @classmethod
+    def from_arrays(cls, ctx: Context, meshes, materials, lights=(), env=None) -> "Scene":
+        """meshes: dicts with positions (n,3), indices (m,3), optional normals, material_id."""
+        keep = []
+        md = (L.MeshDesc * max(1, len(meshes)))()
+        for i, m in enumerate(meshes):
+            pos = _f32(m["positions"]).reshape(-1)
+            idx = _u32(m["indices"]).reshape(-1)
+            nrm = _f32(m["normals"]).reshape(-1) if m.get("normals") is not None else None
+            keep += [pos, idx, nrm]
+            md[i] = L.MeshDesc(pos.ctypes.data_as(L._fp), idx.ctypes.data_as(L._u32p),
+                               nrm.ctypes.data_as(L._fp) if nrm is not None else None,
+                               pos.size // 3, idx.size // 3, int(m.get("material_id", 0)))
+        mats = (L.Material * max(1, len(materials)))(*[m.to_c() for m in materials])
+        lts = (L.QuadLight * max(1, len(lights)))()
+        for i, l in enumerate(lights):
+            l = _f32(l)
+            lts[i] = L.QuadLight((C.c_float * 3)(*l[0:3]), (C.c_float * 3)(*l[3:6]), (C.c_float * 3)(*l[6:9]),
+                                 (C.c_float * 3)(*l[9:12]))
+        envp = None
+        if env is not None:
+            rgb = _f32(env["rgb"]).reshape(-1)
+            keep.append(rgb)
+            ed = L.EnvDesc(env["width"], env["height"], rgb.ctypes.data_as(L._fp), env.get("intensity", 1.0))
+            keep.append(ed)
+            envp = C.pointer(ed)
+        d = L.SceneDesc(md, len(meshes), mats, len(materials), lts, len(lights), envp)
+        keep += [md, mats, lts]
+        return cls(ctx, d, keep=keep)
+
+    def bounds(self):
+        lo = (C.c_float * 3)()
+        hi = (C.c_float * 3)()
+        self.ctx.check(self.lib.pyre_scene_bounds(self.ctx.h, self.h, lo, hi), "pyre_scene_bounds")
+        return np.array(lo, np.float32), np.array(hi, np.float32)
+
+    def build_bvh(self, bins=12, max_leaf=8, traversal_cost=1.0, intersect_cost=2.0):
+        p = L.BvhParams(bins, max_leaf, traversal_cost, intersect_cost)
+        self.ctx.check(self.lib.pyre_bvh_build(self.ctx.h, self.h, C.byref(p)), "pyre_bvh_build")
+        return self.bvh_info()
+
+    def bvh_info(self):
+        nn, nl, dep = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        ms = C.c_double()
+        self.ctx.check(self.lib.pyre_bvh_info(self.ctx.h, self.h, C.byref(nn), C.byref(nl), C.byref(dep),
+                                              C.byref(ms)), "pyre_bvh_info")
+        return dict(n_nodes=nn.value, n_leaves=nl.value, depth=dep.value, build_ms=ms.value)
+
+    def bvh_export(self, n_tris: int):
+        info = self.bvh_info()
+        n = info["n_nodes"]
+        b = np.empty((n, 6), np.float32)
+        a = np.empty(n, np.uint32)
+        bb = np.empty(n, np.uint32)
+        leaf = np.empty(n, np.uint32)
+        ax = np.empty(n, np.uint32)
+        prim = np.empty(n_tris, np.uint32)
+        self.ctx.check(self.lib.pyre_bvh_export(self.ctx.h, self.h, _ptr(b), _ptr(a), _ptr(bb), _ptr(leaf), _ptr(ax),
+                                                _ptr(prim)), "pyre_bvh_export")
+        return dict(bounds=b, a=a, b=bb, is_leaf=leaf, axis=ax, prim=prim)
+
+    def intersect(self, rays):
+        """rays: (n, 9) float32 (o, d, t_min, t_max, time). Returns dict of arrays."""
+        rays = _f32(rays).reshape(-1, 9)
+        out = np.empty((len(rays), 5), np.uint32)
+        self.ctx.check(self.lib.pyre_intersect_closest(self.ctx.h, self.h, _ptr(rays), len(rays), _ptr(out)),
+                       "pyre_intersect_closest")
+        f = out.view(np.float32)
+        return dict(inst=out[:, 0].copy(), tri=out[:, 1].copy(), t=f[:, 2].copy(), u=f[:, 3].copy(),
+                    v=f[:, 4].copy())
+
+    def occluded(self, rays):
+        rays = _f32(rays).reshape(-1, 9)
+        out = np.empty(len(rays), np.uint32)
+        self.ctx.check(self.lib.pyre_intersect_any(self.ctx.h, self.h, _ptr(rays), len(rays), _ptr(out)),
+                       "pyre_intersect_any")
+        return out
+
+    def trace_closest_dev(self, rays_ptr: int, n: int, hits_ptr: int, stream: int = 0):
+        self.ctx.check(self.lib.pyre_trace_closest_dev(self.ctx.h, self.h, C.c_void_p(rays_ptr), n,
+                                                       C.c_void_p(hits_ptr), C.c_void_p(stream or None)),
+                       "pyre_trace_closest_dev")
+
+    def trace_any_dev(self, rays_ptr: int, n: int, occ_ptr: int, stream: int = 0):
+        self.ctx.check(self.lib.pyre_trace_any_dev(self.ctx.h, self.h, C.c_void_p(rays_ptr), n,
+                                                   C.c_void_p(occ_ptr), C.c_void_p(stream or None)),
+                       "pyre_trace_any_dev")
+
+    # -- shade-stage probes
+    def env_sample(self, u):
+        u = _f32(u).reshape(-1, 2)
+        n = len(u)
+        wi, li, pdf = np.empty((n, 3), np.float32), np.empty((n, 3), np.float32), np.empty(n, np.float32)
+        self.ctx.check(self.lib.pyre_env_sample_batch(self.ctx.h, self.h, _ptr(u), n, _ptr(wi), _ptr(li), _ptr(pdf)),
+                       "pyre_env_sample_batch")
+        return wi, li, pdf
+
+    def env_pdf(self, wi):
+        wi = _f32(wi).reshape(-1, 3)
+        pdf = np.empty(len(wi), np.float32)
+        self.ctx.check(self.lib.pyre_env_pdf_batch(self.ctx.h, self.h, _ptr(wi), len(wi), _ptr(pdf)),
+                       "pyre_env_pdf_batch")
+        return pdf
+
+    def env_le(self, wi):
+        wi = _f32(wi).reshape(-1, 3)
+        le = np.empty_like(wi)
+        self.ctx.check(self.lib.pyre_env_le_batch(self.ctx.h, self.h, _ptr(wi), len(wi), _ptr(le)),
+                       "pyre_env_le_batch")
+        return le
+
+    def light_sample(self, light, frm, u):
+        frm = _f32(frm).reshape(-1, 3)
+        u = _f32(u).reshape(-1, 2)
+        n = len(frm)
+        pos, wi = np.empty((n, 3), np.float32), np.empty((n, 3), np.float32)
+        dist, pdf, ok = np.empty(n, np.float32), np.empty(n, np.float32), np.empty(n, np.uint32)
+        self.ctx.check(self.lib.pyre_light_sample_batch(self.ctx.h, self.h, light, _ptr(frm), _ptr(u), n, _ptr(pos),
+                                                        _ptr(wi), _ptr(dist), _ptr(pdf), _ptr(ok)),
+                       "pyre_light_sample_batch")
+        return pos, wi, dist, pdf, ok.astype(bool)
+
+    def light_pdf(self, light, frm, wi):
+        frm = _f32(frm).reshape(-1, 3)
+        wi = _f32(wi).reshape(-1, 3)
+        pdf = np.empty(len(frm), np.float32)
+        self.ctx.check(self.lib.pyre_light_pdf_batch(self.ctx.h, self.h, light, _ptr(frm), _ptr(wi), len(frm),
+                                                     _ptr(pdf)), "pyre_light_pdf_batch")
+        return pdf
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.pyre_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def bsdf_eval(ctx: Context, mat: Material, wo, wi):
+    wo = _f32(wo).reshape(-1, 3)
+    wi = _f32(wi).reshape(-1, 3)
+    f = np.empty_like(wo)
+    pdf = np.empty(len(wo), np.float32)
+    m = mat.to_c()
+    ctx.check(ctx.lib.pyre_bsdf_eval_batch(ctx.h, C.byref(m), _ptr(wo), _ptr(wi), len(wo), _ptr(f), _ptr(pdf)),
+              "pyre_bsdf_eval_batch")
+    return f, pdf
+
+
+def bsdf_sample(ctx: Context, mat: Material, wo, u):
+    wo = _f32(wo).reshape(-1, 3)
+    u = _f32(u).reshape(-1, 2)
+    n = len(wo)
+    wi, f = np.empty((n, 3), np.float32), np.empty((n, 3), np.float32)
+    pdf, ok = np.empty(n, np.float32), np.empty(n, np.uint32)
+    m = mat.to_c()
+    ctx.check(ctx.lib.pyre_bsdf_sample_batch(ctx.h, C.byref(m), _ptr(wo), _ptr(u), n, _ptr(wi), _ptr(f), _ptr(pdf),
+                                             _ptr(ok)), "pyre_bsdf_sample_batch")
+    return wi, f, pdf, ok.astype(bool)
+
+
+class Film:
+    """Film (film.rs:38-109): per-pixel PixelSample means, row-major, (0,0) top-left."""
+
+    def __init__(self, width: int, height: int):
+        self.width = width
+        self.height = height
+        self.samples = np.zeros((height * width, 10), np.float32)
+        self.accum64 = None
+
+    def render(self, scene: Scene, camera: Camera, integrator: PathIntegrator, spp: int, spp_offset: int = 0,
+               accum64: bool = False):
+        """Replaces main.rs:253-287 (pyre_render)."""
+        cam = camera.to_c()
+        ip = L.IntegratorParams(integrator.max_depth, integrator.min_rr_depth)
+        if accum64 and self.accum64 is None:
+            self.accum64 = np.zeros((self.height * self.width, 3), np.float64)
+        scene.ctx.check(scene.lib.pyre_render(scene.ctx.h, scene.h, C.byref(cam), C.byref(ip), self.width,
+                                              self.height, spp, spp_offset, _ptr(self.samples),
+                                              _ptr(self.accum64) if accum64 else None), "pyre_render")
+        return self.samples
+
+
+def random_triangles(seed: int, n: int, sigma: float = 0.01) -> np.ndarray:
+    out = np.empty((n, 9), np.float32)
+    rc = L.load().pyre_gen_random_triangles(seed, n, sigma, _ptr(out))
+    if rc:
+        raise PyreError(rc, "pyre_gen_random_triangles")
+    return out
+
+
+def random_rays(seed: int, n: int, n_closest: int, layout: str = "ray36") -> np.ndarray:
+    """layout 'ray36' -> (n, 9) host rays; 'ray32' -> (n, 8) device records."""
+    if layout == "ray36":
+        out = np.empty((n, 9), np.float32)
+        rc = L.load().pyre_gen_random_rays(seed, n, n_closest, _ptr(out), None)
+    else:
+        out = np.empty((n, 8), np.float32)
+        rc = L.load().pyre_gen_random_rays(seed, n, n_closest, None, _ptr(out))
+    if rc:
+        raise PyreError(rc, "pyre_gen_random_rays")
+    return out
+
+
+def auto_frame(bmin, bmax, vfov_deg=45.0, aspect=1.0):
+    lo, hi = _f32(bmin), _f32(bmax)
+    o, t = (C.c_float * 3)(), (C.c_float * 3)()
+    L.load().pyre_auto_frame(lo.ctypes.data_as(L._fp), hi.ctypes.data_as(L._fp), vfov_deg, aspect, o, t)
+    return np.array(o, np.float32), np.array(t, np.float32)

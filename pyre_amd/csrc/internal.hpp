@@ -1,0 +1,155 @@
+// Internal types of the gfx950 backend (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/pyre_hip.h"
+#include "pmath.hpp"
+
+struct pyre_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::string name;
+  int cus = 0;
+};
+
+namespace pyre {
+
+// Failure carried to the C ABI boundary as a status code.
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define PYRE_HIP_CHECK(expr)                                                                        \
+  do {                                                                                              \
+    hipError_t e_ = (expr);                                                                         \
+    if (e_ != hipSuccess)                                                                           \
+      throw ::pyre::Error(e_ == hipErrorOutOfMemory ? PYRE_ENOMEM : PYRE_EHIP,                      \
+                          std::string(#expr) + ": " + hipGetErrorString(e_));                      \
+  } while (0)
+
+inline void require(bool ok, const std::string& msg) {
+  if (!ok) throw Error(PYRE_EINVAL, msg);
+}
+
+template <class T>
+T* dev_alloc(size_t count) {
+  T* p = nullptr;
+  if (count == 0) count = 1;
+  PYRE_HIP_CHECK(hipMalloc((void**)&p, count * sizeof(T)));
+  return p;
+}
+template <class T>
+void dev_free(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+// Traversal node: both children's boxes live in the parent so one 64-byte
+// fetch (four dwordx4 loads) tests two boxes.
+//   x = (c0.lo.x, c0.hi.x, c1.lo.x, c1.hi.x), y and z likewise,
+//   meta = (c0 ref, c1 ref, c0 count, c1 count):
+//   count == 0 -> ref is a node index; count > 0 -> ref is the first slot of a
+//   leaf in the triangle array; ref < 0 -> no child.
+struct alignas(16) Node2 {
+  float4 x, y, z;
+  int4 meta;
+};
+
+// Build node (binned-SAH builder). Bounds are kept as order-preserving int
+// keys so they can be accumulated with integer atomics.
+struct BNode {
+  int32_t bkey[6];  // node bounds min xyz, max xyz
+  int32_t ckey[6];  // centroid bounds min xyz, max xyz
+  uint32_t start, count;
+  int32_t left, right;  // child build-node ids (interior), -1 for leaves
+  uint32_t axis;
+  uint32_t flags;  // bit0 leaf, bit1 used
+  float split_pos;
+  uint32_t mode;  // decide result for the wide-node path
+};
+static_assert(sizeof(BNode) == 80, "BNode layout");
+
+PH int32_t f2key(float f) {
+  int32_t i = __builtin_bit_cast(int32_t, f);
+  return i >= 0 ? i : (i ^ 0x7FFFFFFF);
+}
+PH float key2f(int32_t k) {
+  int32_t i = k >= 0 ? k : (k ^ 0x7FFFFFFF);
+  return __builtin_bit_cast(float, i);
+}
+
+// DiffuseAreaQuadLight with the fields its constructor derives (light.rs:61-76).
+struct QuadLightDev {
+  V3 p0, eu, ev, cross, normal, emission;
+  float inv_cross_len_sq, area;
+};
+
+struct MatDev {
+  uint32_t kind;
+  V3 base;
+  float metallic, roughness, specular;
+};
+
+// HdriEnvironmentLight on the device (light.rs:205-211): texels as float4,
+// conditional CDFs (height rows of width + 1), marginal CDF (height + 1).
+struct EnvDev {
+  uint32_t width = 0, height = 0;
+  float intensity = 1.0f;
+  const float4* texels = nullptr;
+  const float* cond_cdf = nullptr;
+  const float* marg_cdf = nullptr;
+};
+
+}  // namespace pyre
+
+struct pyre_scene {
+  pyre_ctx* ctx = nullptr;
+  uint32_t n_meshes = 0, n_tris = 0, n_materials = 0, n_lights = 0;
+  std::vector<uint32_t> h_tri_offset;  // n_meshes + 1
+  std::vector<uint32_t> h_mesh_info;   // material id | has_normals << 31
+  std::vector<pyre::MatDev> h_mats;
+  std::vector<pyre::QuadLightDev> h_lights;
+  float* d_verts = nullptr;    // 9 floats per scene triangle (v0, v1, v2)
+  float* d_normals = nullptr;  // 9 floats per triangle (vertex normals) or null
+  uint32_t* d_tri_offset = nullptr;
+  uint32_t* d_mesh_info = nullptr;
+  pyre::MatDev* d_mats = nullptr;
+  pyre::QuadLightDev* d_lights = nullptr;
+  bool has_env = false;
+  pyre::EnvDev env;
+  float4* d_env_texels = nullptr;
+  float* d_env_cond = nullptr;
+  float* d_env_marg = nullptr;
+  float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
+
+  // BVH
+  bool built = false;
+  pyre::Node2* d_nodes = nullptr;
+  uint32_t n_nodes2 = 0;
+  float4* d_tris = nullptr;        // 3 float4 per slot: (v0, gid bits), (e1, 0), (e2, 0)
+  uint32_t* d_prim_order = nullptr;  // scene-global triangle id per slot
+  pyre::BNode* d_bnodes = nullptr;
+  uint32_t n_bnodes = 0;
+  uint32_t n_leaves = 0, n_interior = 0, depth = 0;
+  double build_ms = 0.0;
+  uint32_t* d_flag = nullptr;  // traversal overflow flag (sticky)
+};
+
+namespace pyre {
+// bvh_build.hip
+void bvh_build(pyre_scene* s, const pyre_bvh_params& p, hipStream_t st);
+void bvh_release(pyre_scene* s);
+// trace.hip
+// check = synchronise and turn a stack overflow into PYRE_EOVERFLOW.
+void trace_closest(pyre_scene* s, const float4* rays, size_t n, pyre_hit8* hits, float2* uv, hipStream_t st,
+                   bool check);
+void trace_any(pyre_scene* s, const float4* rays, size_t n, uint32_t* occluded, hipStream_t st, bool check);
+}  // namespace pyre

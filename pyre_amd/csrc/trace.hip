@@ -1,0 +1,96 @@
+// Batched closest-hit / any-hit kernels (Scene::intersect / occluded over a
+// ray array) built on the traversal core in traverse.hpp.
+#include "traverse.hpp"
+
+namespace pyre {
+namespace {
+
+constexpr int kTraceThreads = 256;
+
+template <bool ANY, int STACK>
+__global__ __launch_bounds__(kTraceThreads) void k_trace(const Node2* __restrict__ nodes,
+                                                         const float4* __restrict__ tris,
+                                                         const float4* __restrict__ rays, uint32_t n,
+                                                         pyre_hit8* __restrict__ out_hit,
+                                                         float2* __restrict__ out_uv,
+                                                         uint32_t* __restrict__ out_occ, int32_t* gstack,
+                                                         int gdepth, uint32_t* overflow) {
+  __shared__ int32_t lds_stack[STACK > 0 ? STACK * kTraceThreads : 1];
+  const uint32_t i = blockIdx.x * kTraceThreads + threadIdx.x;
+  if (i >= n) return;
+  int32_t* stack;
+  size_t stride;
+  int cap;
+  if constexpr (STACK > 0) {
+    stack = lds_stack + threadIdx.x;
+    stride = kTraceThreads;
+    cap = STACK;
+  } else {
+    stack = gstack + i;
+    stride = n;
+    cap = gdepth;
+  }
+  const float4 ra = rays[2 * (size_t)i], rb = rays[2 * (size_t)i + 1];
+  const TRay r = make_tray(v3(ra.x, ra.y, ra.z), v3(rb.x, rb.y, rb.z), ra.w, rb.w);
+  THit h;
+  if (!traverse<ANY>(nodes, tris, r, stack, stride, cap, &h)) atomicOr(overflow, 1u);
+  if (ANY) {
+    out_occ[i] = h.gid != PYRE_MISS ? 1u : 0u;
+  } else {
+    pyre_hit8 o;
+    o.prim = h.gid;
+    o.t = h.t;
+    out_hit[i] = o;
+    if (out_uv) out_uv[i] = make_float2(h.u, h.v);
+  }
+}
+
+template <bool ANY>
+void launch(pyre_scene* s, const float4* rays, size_t n, pyre_hit8* hit, float2* uv, uint32_t* occ, hipStream_t st,
+            bool check) {
+  require(s->built, "trace before pyre_bvh_build");
+  require(n <= 0xFFFFFFFFull, "too many rays in one call");
+  if (n == 0) return;
+  dim3 grid((unsigned)((n + kTraceThreads - 1) / kTraceThreads)), block(kTraceThreads);
+  uint32_t* ovf = s->d_flag;
+  int32_t* gst = nullptr;
+  const uint32_t need = s->depth + 1;
+  if (need <= 16)
+    hipLaunchKernelGGL((k_trace<ANY, 16>), grid, block, 0, st, s->d_nodes, s->d_tris, rays, (uint32_t)n, hit, uv, occ,
+                       gst, 0, ovf);
+  else if (need <= 32)
+    hipLaunchKernelGGL((k_trace<ANY, 32>), grid, block, 0, st, s->d_nodes, s->d_tris, rays, (uint32_t)n, hit, uv, occ,
+                       gst, 0, ovf);
+  else if (need <= 48)
+    hipLaunchKernelGGL((k_trace<ANY, 48>), grid, block, 0, st, s->d_nodes, s->d_tris, rays, (uint32_t)n, hit, uv, occ,
+                       gst, 0, ovf);
+  else {
+    gst = dev_alloc<int32_t>((size_t)need * n);
+    hipLaunchKernelGGL((k_trace<ANY, 0>), grid, block, 0, st, s->d_nodes, s->d_tris, rays, (uint32_t)n, hit, uv, occ,
+                       gst, (int)need, ovf);
+  }
+  PYRE_HIP_CHECK(hipGetLastError());
+  if (gst) {
+    PYRE_HIP_CHECK(hipStreamSynchronize(st));
+    dev_free(gst);
+  }
+  if (check) {
+    uint32_t h_ovf = 0;
+    PYRE_HIP_CHECK(hipMemcpyAsync(&h_ovf, ovf, 4, hipMemcpyDeviceToHost, st));
+    PYRE_HIP_CHECK(hipStreamSynchronize(st));
+    if (h_ovf) throw Error(PYRE_EOVERFLOW, "traversal stack overflow (tree deeper than recorded depth)");
+  }
+}
+
+}  // namespace
+
+void trace_closest(pyre_scene* s, const float4* rays, size_t n, pyre_hit8* hits, float2* uv, hipStream_t st,
+                   bool check) {
+  launch<false>(s, rays, n, hits, uv, nullptr, st, check);
+}
+
+void trace_any(pyre_scene* s, const float4* rays, size_t n, uint32_t* occluded, hipStream_t st, bool check) {
+  launch<true>(s, rays, n, nullptr, nullptr, occluded, st, check);
+}
+
+}  // namespace pyre

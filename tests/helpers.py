@@ -1,0 +1,59 @@
+"""Shared helpers for the parity tests (oracle vs HIP path)."""
+from __future__ import annotations
+
+import numpy as np
+
+
+def canonical_oracle_tree(t, i=0):
+    """Oracle BVH (DFS, left = i + 1, right = offset) -> nested tuples."""
+    b = tuple(t["bounds"][i].tolist())
+    if t["count"][i] > 0:
+        s, n = int(t["offset"][i]), int(t["count"][i])
+        return ("L", b, tuple(sorted(t["prim"][s:s + n].tolist())))
+    return ("N", b, int(t["axis"][i]), canonical_oracle_tree(t, i + 1), canonical_oracle_tree(t, int(t["offset"][i])))
+
+
+def canonical_gpu_tree(t, i=0):
+    """Exported device BVH (explicit child ids) -> nested tuples."""
+    b = tuple(t["bounds"][i].tolist())
+    if t["is_leaf"][i] == 1:
+        s, n = int(t["a"][i]), int(t["b"][i])
+        return ("L", b, tuple(sorted(t["prim"][s:s + n].tolist())))
+    assert t["is_leaf"][i] == 0, f"node {i} unused"
+    return ("N", b, int(t["axis"][i]), canonical_gpu_tree(t, int(t["a"][i])), canonical_gpu_tree(t, int(t["b"][i])))
+
+
+def tree_stats(tree):
+    """(nodes, leaves, depth) of a canonical tree."""
+    if tree[0] == "L":
+        return 1, 1, 0
+    a = tree_stats(tree[3])
+    b = tree_stats(tree[4])
+    return 1 + a[0] + b[0], a[1] + b[1], 1 + max(a[2], b[2])
+
+
+def edge_distance(v0, v1, v2, u, v):
+    """Distance (float64) from the barycentric point to the nearest triangle edge."""
+    v0, v1, v2 = (np.asarray(x, np.float64) for x in (v0, v1, v2))
+    u = np.asarray(u, np.float64)
+    v = np.asarray(v, np.float64)
+    w = 1.0 - u - v
+    area2 = np.linalg.norm(np.cross(v1 - v0, v2 - v0), axis=-1)
+    e_opp0 = np.linalg.norm(v2 - v1, axis=-1)  # opposite v0 (weight w)
+    e_opp1 = np.linalg.norm(v2 - v0, axis=-1)  # opposite v1 (weight u)
+    e_opp2 = np.linalg.norm(v1 - v0, axis=-1)  # opposite v2 (weight v)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        d = np.minimum(np.minimum(w * area2 / e_opp0, u * area2 / e_opp1), v * area2 / e_opp2)
+    return np.nan_to_num(d, nan=0.0)
+
+
+def rel_err(a, b, floor=1e-30):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.abs(a - b) / np.maximum(np.maximum(np.abs(a), np.abs(b)), floor)
+
+
+def rel_mse(img, ref, eps=1e-2):
+    img = np.asarray(img, np.float64)
+    ref = np.asarray(ref, np.float64)
+    return float(np.mean((img - ref) ** 2 / (ref ** 2 + eps)))

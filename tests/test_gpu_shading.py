@@ -1,0 +1,138 @@
+"""GPU parity of the shade-stage functions vs the oracle.
+
+Tolerance (stated by the task for floating point): BSDF value/pdf,
+environment-sample pdf and light pdf agree with the reference functions
+within 1e-5 relative. Device sin/cos/acos/atan2 may differ from glibc by an
+ulp, which this tolerance absorbs; directions within 1e-6 (in uv) of an
+environment pixel boundary are excluded and counted (the pdf is piecewise
+constant there).
+"""
+import numpy as np
+import pytest
+
+import pyre_amd
+from tests.helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+DISNEYS = [((0.7, 0.3, 0.2), 0.0, 0.4, 0.5), ((1.0, 0.766, 0.336), 1.0, 0.1, 0.5), ((0.5, 0.5, 0.5), 0.5, 0.7, 0.5),
+           ((0.72, 0.72, 0.72), 0.0, 0.18, 0.5), ((0.9, 0.1, 0.4), 0.0, 1.0, 0.5), ((0.2, 0.8, 0.3), 1.0, 0.05, 0.5)]
+
+
+def _dirs(rng, n, upper=True):
+    v = rng.normal(size=(n, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    if upper:
+        v[:, 2] = np.abs(v[:, 2])
+    return v.astype(np.float32)
+
+
+def _mats():
+    out = [pyre_amd.Material(pyre_amd.MAT_LAMBERTIAN, (0.73, 0.73, 0.73))]
+    out += [pyre_amd.Material(pyre_amd.MAT_DISNEY, b, m, r, s) for (b, m, r, s) in DISNEYS]
+    return out
+
+
+def _omat(oracle, m):
+    return oracle.material(m.kind, m.base_color, m.metallic, m.roughness, m.specular)
+
+
+@pytest.mark.parametrize("mi", range(1 + len(DISNEYS)))
+def test_bsdf_eval_pdf(ctx, oracle, mi):
+    m = _mats()[mi]
+    rng = np.random.default_rng(100 + mi)
+    wo = _dirs(rng, 20000, upper=False)
+    wi = _dirs(rng, 20000, upper=False)
+    f, pdf = pyre_amd.bsdf_eval(ctx, m, wo, wi)
+    of, opdf = oracle.bsdf_eval(_omat(oracle, m), wo, wi)
+    assert np.all(rel_err(f, of) <= TOL)
+    assert np.all(rel_err(pdf, opdf) <= TOL)
+    # no transcendental functions in eval/pdf: expected bit-exact
+    assert np.array_equal(f.view(np.uint32), of.view(np.uint32))
+    assert np.array_equal(pdf.view(np.uint32), opdf.view(np.uint32))
+    assert (opdf > 0).mean() > 0.2
+
+
+@pytest.mark.parametrize("mi", range(1 + len(DISNEYS)))
+def test_bsdf_sample(ctx, oracle, mi):
+    """Sampled directions agree to a few ulps of the device sin/cos; the value
+    and pdf returned with them equal the reference eval/pdf at that direction
+    (eval and pdf use only IEEE +-*/ and sqrt, so they are exact)."""
+    m = _mats()[mi]
+    rng = np.random.default_rng(200 + mi)
+    wo = _dirs(rng, 20000)
+    u = rng.random((20000, 2)).astype(np.float32)
+    wi, f, pdf, ok = pyre_amd.bsdf_sample(ctx, m, wo, u)
+    owi, of, opdf, ook = oracle.bsdf_sample(_omat(oracle, m), wo, u)
+    assert (ok != ook).sum() <= 2
+    both = ok & ook
+    assert np.all(np.abs(wi[both] - owi[both]) <= 2e-6)
+    rf, rp = oracle.bsdf_eval(_omat(oracle, m), wo[ok], wi[ok])
+    assert np.all(rel_err(f[ok], rf) <= TOL)
+    assert np.all(rel_err(pdf[ok], rp) <= TOL)
+    # material.rs:332-387: sample.f/pdf equal eval/pdf at the sampled wi, pdf > 0, wi.z > 0
+    f2, p2 = pyre_amd.bsdf_eval(ctx, m, wo[ok], wi[ok])
+    assert np.all(np.linalg.norm(f2 - f[ok], axis=1) < 1e-5)
+    assert np.all(np.abs(p2 - pdf[ok]) < 1e-5 * np.maximum(1, p2))
+    assert np.all(pdf[ok] > 0) and np.all(wi[ok][:, 2] > 0)
+
+
+def _env(oracle_mod, w=256, h=128, seed=3):
+    rng = np.random.default_rng(seed)
+    rgb = rng.random((h, w, 3)).astype(np.float32) * 0.5
+    # a bright lobe like the production HDRI's sun
+    yy, xx = np.mgrid[0:h, 0:w]
+    rgb += (300.0 * np.exp(-((xx - 60) ** 2 + (yy - 30) ** 2) / 8.0))[..., None].astype(np.float32)
+    return dict(width=w, height=h, rgb=rgb.astype(np.float32), intensity=1.5)
+
+
+def _env_scene(ctx, env):
+    tri = dict(positions=np.array([[0, -10, 0], [1, -10, 0], [0, -10, 1]], np.float32),
+               indices=np.array([[0, 1, 2]], np.uint32), material_id=0)
+    return pyre_amd.Scene.from_arrays(ctx, [tri], [pyre_amd.Material(0, (0.5, 0.5, 0.5))], env=env)
+
+
+def test_env_sample_pdf_le(ctx, oracle):
+    env = _env(oracle)
+    sc = _env_scene(ctx, env)
+    oe = oracle.Hdri(env["width"], env["height"], env["rgb"], env["intensity"])
+    rng = np.random.default_rng(7)
+    u = rng.random((50000, 2)).astype(np.float32)
+    wi, li, pdf = sc.env_sample(u)
+    owi, oli, opdf = oe.sample(u)
+    assert np.all(np.abs(wi - owi) <= 2e-6)
+    # radiance is a bilinear lookup at a uv that may differ by an ulp of the
+    # device acos/atan2; on the steep sun lobe that moves it by ~1e-5
+    assert np.all(rel_err(li, oli) <= 1e-4)
+    assert np.all(rel_err(pdf, opdf) <= TOL)
+    # pdf(wi) of the sampled directions, away from pixel boundaries
+    d = _dirs(rng, 50000, upper=False)
+    uv = oracle.dir_to_uv(d)
+    fu = uv[:, 0] * env["width"]
+    fv = uv[:, 1] * env["height"]
+    near = (np.abs(fu - np.round(fu)) < 1e-6 * env["width"]) | (np.abs(fv - np.round(fv)) < 1e-6 * env["height"])
+    p = sc.env_pdf(d)
+    op = oe.pdf(d)
+    keep = ~near
+    assert near.sum() < 50
+    assert np.all(rel_err(p[keep], op[keep]) <= TOL)
+    assert np.all(rel_err(sc.env_le(d), oe.le(d)) <= 1e-4)
+
+
+def test_quad_light_sample_pdf(ctx, oracle):
+    cb = pyre_amd.Preset.cornell()
+    sc = pyre_amd.Scene.from_preset(ctx, cb)
+    L = cb.arrays()["lights"][0]
+    rng = np.random.default_rng(9)
+    frm = (rng.random((20000, 3)) * 1.8 - 0.9).astype(np.float32)
+    u = rng.random((20000, 2)).astype(np.float32)
+    pos, wi, dist, pdf, ok = sc.light_sample(0, frm, u)
+    opos, owi, odist, opdf, ook = oracle.quad_sample(L, frm, u)
+    assert np.array_equal(ok, ook)
+    assert np.all(rel_err(pdf[ok], opdf[ok]) <= TOL)
+    assert np.all(np.abs(pos[ok] - opos[ok]) <= 1e-6)
+    p2 = sc.light_pdf(0, frm, wi)
+    op2 = oracle.quad_pdf(L, frm, wi)
+    assert np.all(rel_err(p2, op2) <= TOL)

@@ -1,0 +1,141 @@
+"""GPU parity: device BVH build and traversal vs the oracle.
+
+The oracle restates geometry/bvh.rs, geometry/mesh.rs and scene.rs; the
+device path goes through the C ABI (pyre_bvh_build / pyre_bvh_export /
+pyre_intersect_closest / pyre_intersect_any).
+Bar: identical BVH topology and bounds; hit ids and t bit-exact, with rays
+whose hit lies within 1e-6 of a triangle edge excluded and counted.
+"""
+import numpy as np
+import pytest
+
+import pyre_amd
+from tests.helpers import canonical_gpu_tree, canonical_oracle_tree, edge_distance, tree_stats
+
+pytestmark = pytest.mark.gpu
+
+
+def _soup_scene(ctx, tris):
+    n = len(tris)
+    pos = tris.reshape(-1, 3)
+    idx = np.arange(3 * n, dtype=np.uint32).reshape(-1, 3)
+    mats = [pyre_amd.Material(pyre_amd.MAT_LAMBERTIAN, (0.5, 0.5, 0.5))]
+    return pyre_amd.Scene.from_arrays(ctx, [dict(positions=pos, indices=idx, material_id=0)], mats), pos, idx
+
+
+@pytest.mark.parametrize("n,bins,leaf,seed", [(1, 12, 8, 5), (7, 12, 8, 6), (3000, 12, 8, 7), (20000, 32, 4, 8),
+                                              (70000, 12, 8, 9)])
+def test_bvh_topology_matches_reference_build(ctx, oracle, n, bins, leaf, seed):
+    print(f"[bvh topology] n={n} bins={bins} leaf={leaf} seed={seed}", flush=True)
+    tris = pyre_amd.random_triangles(seed, n, 0.01)
+    sc, pos, idx = _soup_scene(ctx, tris)
+    info = sc.build_bvh(bins=bins, max_leaf=leaf)
+    g = canonical_gpu_tree(sc.bvh_export(n))
+    o_sc = oracle.Scene()
+    o_sc.add_material(oracle.material(0, (0.5, 0.5, 0.5)))
+    o_sc.add_mesh(pos, idx, None, 0, bins, leaf)
+    o = canonical_oracle_tree(o_sc.bvh(0))
+    assert g == o
+    nodes, leaves, depth = tree_stats(o)
+    assert info["n_leaves"] == leaves
+    assert info["depth"] == depth
+
+
+def test_bvh_topology_uv_sphere(ctx, oracle):
+    """The Cornell preset's gold sphere (main.rs:455), reference parameters."""
+    o_cb = oracle.Scene.cornell()
+    m = o_cb.mesh(5)
+    sc = pyre_amd.Scene.from_arrays(ctx, [dict(positions=m["positions"], indices=m["indices"],
+                                               normals=m["normals"], material_id=0)],
+                                    [pyre_amd.Material(pyre_amd.MAT_LAMBERTIAN, (0.5, 0.5, 0.5))])
+    sc.build_bvh(bins=12, max_leaf=8)
+    assert canonical_gpu_tree(sc.bvh_export(len(m["indices"]))) == canonical_oracle_tree(o_cb.bvh(5))
+
+
+def _closest_parity(ctx, oracle, n_tris, n_rays, bins, leaf, seed):
+    tris = pyre_amd.random_triangles(seed, n_tris, 0.01)
+    rays = pyre_amd.random_rays(seed + 1000, n_rays, n_rays // 2)
+    sc, pos, idx = _soup_scene(ctx, tris)
+    sc.build_bvh(bins=bins, max_leaf=leaf)
+    g = sc.intersect(rays)
+    o_sc = oracle.Scene()
+    o_sc.add_material(oracle.material(0, (0.5, 0.5, 0.5)))
+    o_sc.add_mesh(pos, idx, None, 0, bins, leaf)
+    o = o_sc.intersect(rays)
+    o_hit = o["kind"] == 1
+    # rays whose oracle hit is within 1e-6 of an edge are excluded (and counted)
+    t3 = tris.reshape(-1, 3, 3)
+    hi = np.where(o_hit)[0]
+    ed = edge_distance(t3[o["tri"][hi], 0], t3[o["tri"][hi], 1], t3[o["tri"][hi], 2], o["u"][hi], o["v"][hi])
+    excl = np.zeros(len(rays), bool)
+    excl[hi[ed < 1e-6]] = True
+    keep = ~excl
+    g_hit = g["inst"] != pyre_amd.MISS
+    assert np.array_equal(g_hit[keep], o_hit[keep])
+    both = keep & o_hit
+    assert np.array_equal(g["tri"][both], o["tri"][both])
+    assert np.array_equal(g["t"][both].view(np.uint32), o["t"][both].view(np.uint32))
+    assert np.array_equal(g["u"][both].view(np.uint32), o["u"][both].view(np.uint32))
+    assert np.array_equal(g["v"][both].view(np.uint32), o["v"][both].view(np.uint32))
+    assert np.all(g["inst"][both] == 0)
+    return int(excl.sum()), int(o_hit.sum())
+
+
+@pytest.mark.parametrize("n_tris,n_rays,bins,leaf,seed", [(50000, 40000, 32, 4, 11), (20000, 20000, 12, 8, 12)])
+def test_closest_hit_bit_exact(ctx, oracle, n_tris, n_rays, bins, leaf, seed):
+    print(f"[closest] tris={n_tris} rays={n_rays} bins={bins} leaf={leaf}", flush=True)
+    excluded, hits = _closest_parity(ctx, oracle, n_tris, n_rays, bins, leaf, seed)
+    print(f"  hits={hits} excluded_near_edge={excluded}")
+    assert hits > n_rays // 4
+    assert excluded < n_rays // 1000 + 5
+
+
+def test_any_hit_matches_occluded(ctx, oracle):
+    print("[any-hit] tris=50000 rays=40000", flush=True)
+    tris = pyre_amd.random_triangles(21, 50000, 0.01)
+    rays = pyre_amd.random_rays(22, 40000, 0)  # all rays have t_max ~ U[0.1, 3]
+    sc, pos, idx = _soup_scene(ctx, tris)
+    sc.build_bvh(bins=32, max_leaf=4)
+    g = sc.occluded(rays)
+    o_sc = oracle.Scene()
+    o_sc.add_material(oracle.material(0, (0.5, 0.5, 0.5)))
+    o_sc.add_mesh(pos, idx, None, 0, 32, 4)
+    o = o_sc.any_hit(rays)
+    # any-hit must agree with closest-hit existence on the same ray (scene.rs:102-140)
+    c = sc.intersect(rays)
+    assert np.array_equal(g.astype(bool), c["inst"] != pyre_amd.MISS)
+    mism = np.nonzero(g.astype(bool) != o.astype(bool))[0]
+    assert len(mism) <= 2, f"{len(mism)} any-hit mismatches"
+    assert 0.05 < g.mean() < 0.95
+
+
+def test_empty_and_degenerate_inputs(ctx):
+    tris = pyre_amd.random_triangles(31, 100, 0.01)
+    sc, _, _ = _soup_scene(ctx, tris)
+    sc.build_bvh(bins=12, max_leaf=8)
+    out = sc.intersect(np.zeros((0, 9), np.float32))
+    assert len(out["t"]) == 0
+    # zero-length range and t_max < t_min never hit; NaN direction never hits
+    rays = np.array([[0, 0, -5, 0, 0, 1, 1e-4, 1e-4, 0],
+                     [0, 0, -5, 0, 0, 1, 1.0, 0.5, 0],
+                     [0, 0, -5, np.nan, 0, 1, 1e-4, np.inf, 0]], np.float32)
+    h = sc.intersect(rays)
+    assert np.all(h["inst"] == pyre_amd.MISS)
+    assert np.all(sc.occluded(rays) == 0)
+
+
+def test_degenerate_coincident_triangles(ctx, oracle):
+    """Coincident centroids force the extent < 1e-9 leaf (bvh.rs:205-208)."""
+    tri = np.array([0, 0, 0, 1, 0, 0, 0, 1, 0], np.float32)
+    tris = np.tile(tri, (40, 1))
+    sc, pos, idx = _soup_scene(ctx, tris)
+    sc.build_bvh(bins=12, max_leaf=8)
+    g = canonical_gpu_tree(sc.bvh_export(40))
+    o_sc = oracle.Scene()
+    o_sc.add_material(oracle.material(0, (0.5, 0.5, 0.5)))
+    o_sc.add_mesh(pos, idx, None, 0, 12, 8)
+    assert g == canonical_oracle_tree(o_sc.bvh(0))
+    r = np.array([[0.2, 0.2, 1, 0, 0, -1, 1e-4, np.inf, 0]], np.float32)
+    h = sc.intersect(r)
+    assert h["inst"][0] == 0 and h["tri"][0] == 0  # exact tie -> lowest triangle id
+    assert h["t"][0] == np.float32(1.0)
