@@ -43,6 +43,7 @@ def lib():
             "orc_rng_state": (None, [C.c_uint64, _u64p]),
             "orc_rng_f32": (None, [C.c_uint64, _u32, _f32p]),
             "orc_rng_u64": (None, [C.c_uint64, _u32, _u64p]),
+            "orc_rng_u64_from_state": (None, [_u64p, _u32, _u64p]),
             "orc_concentric_disk": (None, [_f32p, _u32, _f32p]),
             "orc_dist1d_new": (_vp, [_f32p, _u32]),
             "orc_dist1d_free": (None, [_vp]),
@@ -132,6 +133,12 @@ def rng_f32(seed: int, n: int) -> np.ndarray:
 def rng_u64(seed: int, n: int) -> np.ndarray:
     out = np.empty(n, np.uint64)
     lib().orc_rng_u64(seed, n, out)
+    return out
+
+
+def rng_u64_from_state(state4, n: int) -> np.ndarray:
+    out = np.empty(n, np.uint64)
+    lib().orc_rng_u64_from_state(np.ascontiguousarray(state4, np.uint64), n, out)
     return out
 
 

@@ -1066,6 +1066,13 @@ void orc_rng_u64(uint64_t seed, uint32_t n, uint64_t* out) {
   for (uint32_t i = 0; i < n; ++i) out[i] = s.next_u64();
 }
 
+// xoshiro256++ outputs from an explicit state (for published test vectors).
+void orc_rng_u64_from_state(const uint64_t* state4, uint32_t n, uint64_t* out) {
+  Sampler s(0);
+  std::memcpy(s.s, state4, 32);
+  for (uint32_t i = 0; i < n; ++i) out[i] = s.next_u64();
+}
+
 void orc_concentric_disk(const float* u, uint32_t n, float* out) {
   for (uint32_t i = 0; i < n; ++i) {
     V2 d = concentric_disk(V2{u[2 * i], u[2 * i + 1]});

@@ -309,13 +309,33 @@ int pyre_preset_production(uint64_t seed, uint32_t n_instances, uint32_t level, 
     float v = ((float)yy + 0.5f) / (float)env_h;
     float cos_theta = 1.0f - 2.0f * v;
     V3 color = cos_theta >= 0.0f ? lerp(horizon, zenith, cos_theta) : horizon * rmax(1.0f + cos_theta * 0.5f, 0.0f);
-    double theta = ((double)yy + 0.5) / env_h * M_PI;
+    const double dth = M_PI / env_h, dph = 2.0 * M_PI / env_w;
     for (uint32_t xx = 0; xx < env_w; ++xx) {
-      double phi = ((double)xx + 0.5) / env_w * 2.0 * M_PI - M_PI;  // light.rs:282-287 convention
-      double d[3] = {std::sin(theta) * std::cos(phi), std::cos(theta), std::sin(theta) * std::sin(phi)};
-      double c = d[0] * sd[0] + d[1] * sd[1] + d[2] * sd[2];
-      double ang = std::acos(std::fmax(-1.0, std::fmin(1.0, c)));
-      double sun = sun_peak * std::exp(-0.5 * (ang / sigma) * (ang / sigma));
+      // sun lobe averaged over the texel (8x8 subsamples near the sun, so the
+      // sun's energy does not depend on the map resolution)
+      auto lobe = [&](double th, double ph) {
+        double d[3] = {std::sin(th) * std::cos(ph), std::cos(th), std::sin(th) * std::sin(ph)};
+        double c = d[0] * sd[0] + d[1] * sd[1] + d[2] * sd[2];
+        double ang = std::acos(std::fmax(-1.0, std::fmin(1.0, c)));
+        return sun_peak * std::exp(-0.5 * (ang / sigma) * (ang / sigma));
+      };
+      double th0 = (double)yy * dth, ph0 = (double)xx * dph - M_PI;  // light.rs:282-287 convention
+      double centre_d[3] = {std::sin(th0 + 0.5 * dth) * std::cos(ph0 + 0.5 * dph), std::cos(th0 + 0.5 * dth),
+                            std::sin(th0 + 0.5 * dth) * std::sin(ph0 + 0.5 * dph)};
+      double cang = std::acos(std::fmax(-1.0, std::fmin(1.0, centre_d[0] * sd[0] + centre_d[1] * sd[1] +
+                                                                 centre_d[2] * sd[2])));
+      double sun = 0.0;
+      if (cang < 6.0 * sigma + dth + dph) {
+        double wsum = 0.0;
+        for (int a = 0; a < 8; ++a)
+          for (int b = 0; b < 8; ++b) {
+            double th = th0 + (a + 0.5) / 8.0 * dth, ph = ph0 + (b + 0.5) / 8.0 * dph;
+            double w = std::sin(th);
+            sun += w * lobe(th, ph);
+            wsum += w;
+          }
+        sun /= wsum;
+      }
       float* o = &p->env_rgb[3 * ((size_t)yy * env_w + xx)];
       o[0] = (float)(color.x + sun);
       o[1] = (float)(color.y + sun * 0.95);
