@@ -68,17 +68,67 @@ def cpu_baseline(tris, rays36, sample):
                                 f"(reference build, {BINS} bins, leaf {MAX_LEAF}; build {build_s:.1f} s untimed)")
 
 
+def bench_cornell(args):
+    """Cornell preset (main.rs:365-496), 512x512, 64 spp, depth 8, pinhole:
+    the reference's own CPU-runnable case. Step = one full render."""
+    import pyre_amd
+    ctx = pyre_amd.Context(0)
+    dev_name, _ = ctx.device_info()
+    W = H = 512
+    SPP = 64
+    pre = pyre_amd.Preset.cornell()
+    scene = pyre_amd.Scene.from_preset(ctx, pre)
+    scene.build_bvh()  # reference parameters: 12 bins, leaf 8
+    cam = pre.camera(1.0)
+    ip = pyre_amd.PathIntegrator(8, 3)
+    film = pyre_amd.Film(W, H)
+    for _ in range(args.warmup):
+        film.render(scene, cam, ip, SPP)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        film.render(scene, cam, ip, SPP)
+    wall = time.perf_counter() - t0
+    ext, sh = scene.render_stats()
+    per = wall / args.steps
+    line = {
+        "metric": "samples/s (Cornell preset, NEE+MIS path tracer)", "value": round(W * H * SPP / per / 1e6, 2),
+        "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(per * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (built-in Cornell preset)",
+        "config": {"workload": "Cornell preset 512x512, 64 spp, max depth 8, pinhole", "width": W, "height": H,
+                   "spp": SPP, "max_depth": 8},
+        "extension_mrays_s": round(ext / per / 1e6, 1), "shadow_mrays_s": round(sh / per / 1e6, 1),
+        "rays_per_sample": round((ext + sh) / (W * H * SPP), 3), "device": dev_name,
+    }
+    if not args.no_cpu:
+        from oracle import oracle as O
+        threads = O.nthreads()
+        osc = O.Scene.cornell()
+        ocam = O.camera(cam.origin, cam.target, cam.up, cam.vfov_deg, cam.aspect)
+        cspp = 8
+        t0 = time.perf_counter()
+        osc.render(ocam, W, H, cspp, threads=threads)
+        dt = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": round(W * H * cspp / dt / 1e6, 3), "unit": "Msamples/s", "cores": threads,
+                                "kind": "port", "sample": f"full 512x512 frame at {cspp} spp (of 64)"}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity spot check")
+    ap.add_argument("--config", default="traversal", choices=["traversal", "cornell"],
+                    help="traversal: the headline microbench; cornell: the Cornell render")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(rays and pixels shard with no exchange; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.config == "cornell":
+        return bench_cornell(args)
 
     import torch
     import pyre_amd
@@ -178,5 +228,20 @@ def main():
     print(json.dumps(line), flush=True)
 
 
+def copy_probe():
+    """A plain device copy of known size (1 GiB read + 1 GiB write), for
+    scaling the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+    x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": x.numel() * 4}))
+
+
 if __name__ == "__main__":
+    if len(sys.argv) > 1 and sys.argv[1] == "--copy-probe":
+        copy_probe()
+        sys.exit(0)
     main()

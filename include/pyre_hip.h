@@ -246,6 +246,43 @@ int pyre_render(pyre_ctx* ctx, pyre_scene* scene, const pyre_camera_desc* camera
                 const pyre_integrator_params* integrator, uint32_t width, uint32_t height, uint32_t spp,
                 uint32_t spp_offset, pyre_pixel_sample* out, double* accum64);
 
+/* Rays traced by the last pyre_render on this scene: closest-hit extension
+ * rays (camera rays included) and any-hit shadow rays. */
+int pyre_render_stats(pyre_ctx* ctx, pyre_scene* scene, uint64_t* extension_rays, uint64_t* shadow_rays);
+
+/* ------------------------------------------------ neural radiance cache */
+/* No counterpart in the reference. A 5x64 ReLU MLP on BF16 MFMA (FP32
+ * accumulation, FP32 master weights, Adam) mapping a query (13 floats:
+ * position xyz, outgoing direction xyz, normal xyz, roughness, albedo rgb)
+ * to outgoing radiance (RGB). Positions are normalised by the given box. */
+typedef struct pyre_nrc pyre_nrc;
+
+typedef struct pyre_nrc_params {
+  uint64_t seed;
+  float learning_rate; /* Adam, e.g. 1e-2 */
+  float beta1;         /* 0.9 */
+  float beta2;         /* 0.99 */
+  float epsilon;       /* 1e-15 */
+  float bounds_min[3];
+  float bounds_max[3];
+} pyre_nrc_params;
+
+int pyre_nrc_create(pyre_ctx* ctx, const pyre_nrc_params* params, pyre_nrc** out);
+void pyre_nrc_destroy(pyre_nrc* nrc);
+/* Number of FP32 parameters (layer by layer, [in][out] row-major). */
+size_t pyre_nrc_param_count(void);
+int pyre_nrc_get_weights(pyre_ctx* ctx, pyre_nrc* nrc, float* out);
+/* Replaces the weights and resets the Adam moments. */
+int pyre_nrc_set_weights(pyre_ctx* ctx, pyre_nrc* nrc, const float* weights);
+/* rgb = 3n floats for feats = 13n floats. */
+int pyre_nrc_infer(pyre_ctx* ctx, pyre_nrc* nrc, const float* feats, size_t n, float* rgb);
+/* Loss and FP32 gradient (param_count floats) of one batch, no update. */
+int pyre_nrc_gradients(pyre_ctx* ctx, pyre_nrc* nrc, const float* feats, const float* targets, size_t n,
+                       float* grad, float* loss);
+/* One Adam step on the batch; loss receives the pre-step loss. */
+int pyre_nrc_train_step(pyre_ctx* ctx, pyre_nrc* nrc, const float* feats, const float* targets, size_t n,
+                        float* loss);
+
 #ifdef __cplusplus
 }
 #endif

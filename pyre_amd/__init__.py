@@ -18,7 +18,7 @@ from ._lib import PyreError
 
 __all__ = [
     "Context", "Scene", "Preset", "PathIntegrator", "Film", "Camera", "Material", "PyreError",
-    "random_triangles", "random_rays", "auto_frame", "MAT_LAMBERTIAN", "MAT_DISNEY", "MISS",
+    "random_triangles", "random_rays", "auto_frame", "MAT_LAMBERTIAN", "MAT_DISNEY", "MISS", "RadianceCache",
 ]
 
 MAT_LAMBERTIAN = L.MAT_LAMBERTIAN
@@ -322,6 +322,11 @@ class Scene:
                                                      _ptr(pdf)), "pyre_light_pdf_batch")
         return pdf
 
+    def render_stats(self):
+        e, sh = C.c_uint64(), C.c_uint64()
+        self.ctx.check(self.lib.pyre_render_stats(self.ctx.h, self.h, C.byref(e), C.byref(sh)), "pyre_render_stats")
+        return e.value, sh.value
+
     def close(self):
         if getattr(self, "h", None):
             self.lib.pyre_scene_destroy(self.h)
@@ -405,3 +410,63 @@ def auto_frame(bmin, bmax, vfov_deg=45.0, aspect=1.0):
     o, t = (C.c_float * 3)(), (C.c_float * 3)()
     L.load().pyre_auto_frame(lo.ctypes.data_as(L._fp), hi.ctypes.data_as(L._fp), vfov_deg, aspect, o, t)
     return np.array(o, np.float32), np.array(t, np.float32)
+
+
+class RadianceCache:
+    """Neural radiance cache (pyre_nrc_*): 13 query floats -> RGB."""
+
+    def __init__(self, ctx: Context, bounds_min, bounds_max, seed=5, learning_rate=1e-2, beta1=0.9, beta2=0.99,
+                 epsilon=1e-15):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        p = L.NrcParams(seed, learning_rate, beta1, beta2, epsilon, (C.c_float * 3)(*bounds_min),
+                        (C.c_float * 3)(*bounds_max))
+        h = C.c_void_p()
+        ctx.check(self.lib.pyre_nrc_create(ctx.h, C.byref(p), C.byref(h)), "pyre_nrc_create")
+        self.h = h
+        self.n_params = int(self.lib.pyre_nrc_param_count())
+
+    def weights(self):
+        w = np.empty(self.n_params, np.float32)
+        self.ctx.check(self.lib.pyre_nrc_get_weights(self.ctx.h, self.h, _ptr(w)), "pyre_nrc_get_weights")
+        return w
+
+    def set_weights(self, w):
+        w = _f32(w)
+        assert w.size == self.n_params
+        self.ctx.check(self.lib.pyre_nrc_set_weights(self.ctx.h, self.h, _ptr(w)), "pyre_nrc_set_weights")
+
+    def infer(self, feats):
+        feats = _f32(feats).reshape(-1, 13)
+        out = np.empty((len(feats), 3), np.float32)
+        self.ctx.check(self.lib.pyre_nrc_infer(self.ctx.h, self.h, _ptr(feats), len(feats), _ptr(out)),
+                       "pyre_nrc_infer")
+        return out
+
+    def gradients(self, feats, targets):
+        feats = _f32(feats).reshape(-1, 13)
+        targets = _f32(targets).reshape(-1, 3)
+        g = np.empty(self.n_params, np.float32)
+        loss = C.c_float()
+        self.ctx.check(self.lib.pyre_nrc_gradients(self.ctx.h, self.h, _ptr(feats), _ptr(targets), len(feats),
+                                                   _ptr(g), C.byref(loss)), "pyre_nrc_gradients")
+        return float(loss.value), g
+
+    def train_step(self, feats, targets):
+        feats = _f32(feats).reshape(-1, 13)
+        targets = _f32(targets).reshape(-1, 3)
+        loss = C.c_float()
+        self.ctx.check(self.lib.pyre_nrc_train_step(self.ctx.h, self.h, _ptr(feats), _ptr(targets), len(feats),
+                                                    C.byref(loss)), "pyre_nrc_train_step")
+        return float(loss.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.pyre_nrc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -142,3 +142,21 @@ class PyreError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"pyre status {code}: {msg}")
         self.code = code
+
+
+class NrcParams(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("learning_rate", _f), ("beta1", _f), ("beta2", _f), ("epsilon", _f),
+                ("bounds_min", _f * 3), ("bounds_max", _f * 3)]
+
+
+SIGNATURES.update({
+    "pyre_render_stats": (C.c_int, [_vp, _vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "pyre_nrc_create": (C.c_int, [_vp, C.POINTER(NrcParams), C.POINTER(_vp)]),
+    "pyre_nrc_destroy": (None, [_vp]),
+    "pyre_nrc_param_count": (C.c_size_t, []),
+    "pyre_nrc_get_weights": (C.c_int, [_vp, _vp, _vp]),
+    "pyre_nrc_set_weights": (C.c_int, [_vp, _vp, _vp]),
+    "pyre_nrc_infer": (C.c_int, [_vp, _vp, _vp, C.c_size_t, _vp]),
+    "pyre_nrc_gradients": (C.c_int, [_vp, _vp, _vp, _vp, C.c_size_t, _vp, C.POINTER(_f)]),
+    "pyre_nrc_train_step": (C.c_int, [_vp, _vp, _vp, _vp, C.c_size_t, C.POINTER(_f)]),
+})

@@ -64,6 +64,7 @@ struct Paths {
   uint32_t* q_next;
   uint32_t* q_shadow;
   uint32_t* counters;  // [0] current queue, [1] next queue, [2] shadow queue
+  unsigned long long* stats;  // [0] extension rays, [1] shadow rays (whole render)
   uint32_t slots;      // shadow slots per path (lights + env)
 };
 
@@ -102,7 +103,8 @@ __global__ void k_counters(uint32_t* c, uint32_t a, uint32_t b, uint32_t d) {
 }
 
 // next queue becomes current; next and shadow queues are emptied
-__global__ void k_rotate(uint32_t* c) {
+__global__ void k_rotate(uint32_t* c, unsigned long long* stats) {
+  stats[0] += c[0];
   c[0] = c[1];
   c[1] = 0;
   c[2] = 0;
@@ -208,9 +210,10 @@ __device__ __forceinline__ uint32_t mesh_of(const SceneDev& S, uint32_t gid) {
 // One bounce of integrator.rs:76-222 for path p. Returns whether the path
 // continues (next ray written) and whether it queued shadow rays.
 __device__ void shade_path(const Paths& P, const SceneDev& S, const PassParams& pp, uint32_t depth, uint32_t p,
-                           bool* cont, bool* shadow) {
+                           bool* cont, bool* shadow, uint32_t* n_shadow) {
   *cont = false;
   *shadow = false;
+  *n_shadow = 0;
   float4 ra = P.ray[2 * (size_t)p], rb = P.ray[2 * (size_t)p + 1];
   V3 ro = xyz(ra), rd = xyz(rb);
   float r_tmin = ra.w;
@@ -349,6 +352,7 @@ __device__ void shade_path(const Paths& P, const SceneDev& S, const PassParams& 
   }
   P.scount[p] = ns_count;
   *shadow = ns_count > 0;
+  *n_shadow = ns_count;
   // BSDF sample (integrator.rs:190-201)
   bool alive = true;
   V3 bwi, bf;
@@ -387,11 +391,13 @@ __global__ __launch_bounds__(kThreads) void k_shade(Paths P, SceneDev S, PassPar
   for (uint32_t i0 = base; i0 < n; i0 += stride) {
     uint32_t i = i0 + (threadIdx.x & 63u);
     bool cont = false, shadow = false;
-    uint32_t p = 0;
+    uint32_t p = 0, nsh = 0;
     if (i < n) {
       p = P.q_cur[i];
-      shade_path(P, S, pp, depth, p, &cont, &shadow);
+      shade_path(P, S, pp, depth, p, &cont, &shadow, &nsh);
     }
+    for (int o = 32; o > 0; o >>= 1) nsh += __shfl_xor(nsh, o);
+    if ((threadIdx.x & 63u) == 0 && nsh) atomicAdd(&P.stats[1], (unsigned long long)nsh);
     uint32_t qn = wave_append(&P.counters[1], cont);
     if (cont) P.q_next[qn] = p;
     uint32_t qs = wave_append(&P.counters[2], shadow);
@@ -480,6 +486,13 @@ void trace_stage(uint32_t depth, bool shadow, unsigned grid, hipStream_t st, con
 
 }  // namespace
 
+extern "C" int pyre_render_stats(pyre_ctx* ctx, pyre_scene* s, uint64_t* extension_rays, uint64_t* shadow_rays) {
+  if (!ctx || !s) return PYRE_EINVAL;
+  if (extension_rays) *extension_rays = s->last_extension_rays;
+  if (shadow_rays) *shadow_rays = s->last_shadow_rays;
+  return PYRE_OK;
+}
+
 extern "C" int pyre_render(pyre_ctx* ctx, pyre_scene* s, const pyre_camera_desc* camera,
                            const pyre_integrator_params* integrator, uint32_t width, uint32_t height, uint32_t spp,
                            uint32_t spp_offset, pyre_pixel_sample* out, double* accum64) {
@@ -503,6 +516,8 @@ extern "C" int pyre_render(pyre_ctx* ctx, pyre_scene* s, const pyre_camera_desc*
         scon(cap * slots);
     Buf<ulonglong2> rng(2 * cap);
     Buf<uint32_t> scount(cap), q0(cap), q1(cap), qs(cap), ctr(4);
+    Buf<unsigned long long> stats(2);
+    PYRE_HIP_CHECK(hipMemsetAsync(stats.p, 0, 16, st));
     Buf<float> acc(10 * (size_t)npix), outd(10 * (size_t)npix);
     Buf<double> acc64(accum64 ? 3 * (size_t)npix : 1);
     hipLaunchKernelGGL(k_zero_f32, dim3((unsigned)((10 * (size_t)npix + 255) / 256)), dim3(256), 0, st, acc.p,
@@ -511,7 +526,7 @@ extern "C" int pyre_render(pyre_ctx* ctx, pyre_scene* s, const pyre_camera_desc*
       PYRE_HIP_CHECK(hipMemcpyAsync(acc64.p, accum64, 3 * sizeof(double) * npix, hipMemcpyHostToDevice, st));
     PYRE_HIP_CHECK(hipMemsetAsync(s->d_flag, 0, 4, st));
     Paths P{ray.p, rng.p, bp.p, lum.p, aov0.p, aov1.p, hit.p, sray.p, scon.p, scount.p, q0.p, q1.p, qs.p, ctr.p,
-            slots};
+            stats.p, slots};
     const unsigned grid = (unsigned)std::max(1, ctx->cus) * 8;
     for (uint32_t s0 = 0; s0 < spp; s0 += per_pass) {
       uint32_t ns = std::min(per_pass, spp - s0);
@@ -525,7 +540,7 @@ extern "C" int pyre_render(pyre_ctx* ctx, pyre_scene* s, const pyre_camera_desc*
         trace_stage(s->depth, false, grid, st, P, S, s->d_flag);
         hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kThreads), 0, st, P, S, pp, d);
         if (s->n_lights + (s->has_env ? 1u : 0u) > 0) trace_stage(s->depth, true, grid, st, P, S, s->d_flag);
-        hipLaunchKernelGGL(k_rotate, dim3(1), dim3(1), 0, st, ctr.p);
+        hipLaunchKernelGGL(k_rotate, dim3(1), dim3(1), 0, st, ctr.p, stats.p);
         std::swap(P.q_cur, P.q_next);
       }
       hipLaunchKernelGGL(k_accumulate, dim3(grid), dim3(kThreads), 0, st, P, pp, acc.p,
@@ -541,9 +556,13 @@ extern "C" int pyre_render(pyre_ctx* ctx, pyre_scene* s, const pyre_camera_desc*
     if (accum64)
       PYRE_HIP_CHECK(hipMemcpyAsync(accum64, acc64.p, 3 * sizeof(double) * npix, hipMemcpyDeviceToHost, st));
     uint32_t ovf = 0;
+    unsigned long long hs[2] = {0, 0};
     PYRE_HIP_CHECK(hipMemcpyAsync(&ovf, s->d_flag, 4, hipMemcpyDeviceToHost, st));
+    PYRE_HIP_CHECK(hipMemcpyAsync(hs, stats.p, 16, hipMemcpyDeviceToHost, st));
     PYRE_HIP_CHECK(hipStreamSynchronize(st));
     if (ovf) throw Error(PYRE_EOVERFLOW, "traversal stack overflow");
+    s->last_extension_rays = hs[0];
+    s->last_shadow_rays = hs[1];
     ctx->err.clear();
     return PYRE_OK;
   } catch (const Error& e) {

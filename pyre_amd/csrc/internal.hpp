@@ -141,6 +141,7 @@ struct pyre_scene {
   uint32_t n_leaves = 0, n_interior = 0, depth = 0;
   double build_ms = 0.0;
   uint32_t* d_flag = nullptr;  // traversal overflow flag (sticky)
+  uint64_t last_extension_rays = 0, last_shadow_rays = 0;  // of the last pyre_render
 };
 
 namespace pyre {

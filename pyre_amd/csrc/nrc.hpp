@@ -1,0 +1,23 @@
+// Neural radiance cache: internal interface shared with the integrator.
+#pragma once
+
+#include "internal.hpp"
+
+namespace pyre {
+
+// Scene box used to normalise query positions to [0, 1]^3.
+struct NrcBounds {
+  float lo[3];
+  float inv_ext[3];
+};
+
+int nrc_param_count();
+// Forward + relative-L2 loss + backward into the cache's gradient buffer;
+// feats (13 floats per query) and targets (RGB) are device arrays. Returns
+// the mean loss.
+float nrc_gradients_dev(pyre_nrc* nrc, const float* feats, const float* targets, uint32_t n, hipStream_t st);
+void nrc_adam_step(pyre_nrc* nrc, hipStream_t st);
+// Fused BF16-MFMA inference: rgb[3n] from feats[13n] (device arrays).
+void nrc_infer_dev(pyre_nrc* nrc, const float* feats, uint32_t n, float* rgb, hipStream_t st);
+
+}  // namespace pyre

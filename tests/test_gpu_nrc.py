@@ -1,0 +1,102 @@
+"""GPU radiance cache (BF16 MFMA forward/backward, Adam) vs the float64 model.
+
+The model (oracle/nrc_oracle.py) is evaluated on the device's own weights
+with bf16 rounding at the points the kernels round (weights, encoding,
+hidden activations), so the remaining differences are FP32 accumulation
+order and encoding features that land on the other side of a bf16 rounding
+boundary: tolerances are stated per check below.
+"""
+import numpy as np
+import pytest
+
+import pyre_amd
+from oracle import nrc_oracle as N
+
+pytestmark = pytest.mark.gpu
+
+BMIN, BMAX = (-2.0, -1.0, -3.0), (2.0, 3.0, 1.0)
+
+
+def _queries(rng, n):
+    p = rng.uniform(BMIN, BMAX, (n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    nn = rng.normal(size=(n, 3))
+    nn /= np.linalg.norm(nn, axis=1, keepdims=True)
+    return np.concatenate([p, d, nn, rng.random((n, 1)), rng.random((n, 3))], 1).astype(np.float32)
+
+
+def _target(q):
+    # a smooth radiance-like field of position and direction
+    p, d = q[:, 0:3], q[:, 3:6]
+    base = 0.5 + 0.4 * np.sin(p[:, 0:1] + 0.5 * p[:, 1:2]) * np.cos(p[:, 2:3])
+    return (base * (1.2 + d[:, 1:2]) * q[:, 10:13] * 2).astype(np.float32)
+
+
+@pytest.fixture(scope="module")
+def cache(ctx):
+    c = pyre_amd.RadianceCache(ctx, BMIN, BMAX, seed=5)
+    yield c
+    c.close()
+
+
+def test_inference_matches_model(cache):
+    rng = np.random.default_rng(1)
+    q = _queries(rng, 5000)  # not a multiple of 16: exercises the tail tile
+    w = N.unflatten(cache.weights().astype(np.float64))
+    ref, _ = N.forward(w, N.encode(q, BMIN, BMAX), quant=True)
+    ref = ref[:, :3]
+    got = cache.infer(q)
+    scale = np.abs(ref).mean()
+    err = np.abs(got - ref)
+    print(f"  infer: mean|ref| {scale:.3f} max err {err.max():.2e} mean err {err.mean():.2e}")
+    assert err.mean() < 2e-3 * scale + 1e-4
+    assert np.all(err < 2e-2 * (np.abs(ref) + scale))
+
+
+def test_gradients_match_model(cache):
+    rng = np.random.default_rng(2)
+    q = _queries(rng, 4096)
+    t = _target(q)
+    loss, g = cache.gradients(q, t)
+    w = N.unflatten(cache.weights().astype(np.float64))
+    pred, acts = N.forward(w, N.encode(q, BMIN, BMAX), quant=True)
+    rloss, gout = N.loss_and_grad_out(pred, t)
+    rg = N.backward([N.bf16(x) for x in w], acts, gout)
+    assert abs(loss - rloss) < 1e-3 * rloss + 1e-6
+    gl = N.unflatten(g.astype(np.float64))
+    for li, (a, b) in enumerate(zip(gl, rg)):
+        cos = float((a * b).sum() / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-30))
+        rel = float(np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-30))
+        print(f"  layer {li}: cos {cos:.5f} rel {rel:.3e}")
+        assert cos > 0.999 and rel < 0.03
+    # output columns past RGB and padding rows of layer 0 get no gradient
+    assert not gl[-1][:, 3:].any()
+    assert not gl[0][109:, :].any()
+
+
+def test_adam_step_and_training_reduces_loss(ctx):
+    c = pyre_amd.RadianceCache(ctx, BMIN, BMAX, seed=7, learning_rate=1e-2)
+    rng = np.random.default_rng(3)
+    q = _queries(rng, 1 << 14)
+    t = _target(q)
+    w0 = c.weights()
+    _, g = c.gradients(q, t)
+    l0 = c.train_step(q, t)
+    w1 = c.weights()
+    # first Adam step: w1 = w0 - lr * g / (|g| + eps)  (Kingma & Ba, bias-corrected)
+    expect, _, _ = N.adam(w0.astype(np.float64), g.astype(np.float64), np.zeros_like(w0, np.float64),
+                          np.zeros_like(w0, np.float64), 1, lr=1e-2, b1=0.9, b2=0.99, eps=1e-15)
+    assert np.allclose(w1, expect, rtol=1e-5, atol=1e-7)
+    losses = [l0]
+    for _ in range(300):
+        losses.append(c.train_step(q, t))
+    print(f"  loss {losses[0]:.4f} -> {losses[-1]:.4f}")
+    assert losses[-1] < 0.1 * losses[0]
+    # generalises to unseen queries of the same field
+    q2 = _queries(np.random.default_rng(4), 4096)
+    pred = c.infer(q2)
+    t2 = _target(q2)
+    rel = np.mean((pred - t2) ** 2 / (pred ** 2 + 0.01))
+    assert rel < 0.05
+    c.close()
