@@ -42,6 +42,22 @@ BYTES_CLOSEST = 32 + 8 + 32 * N_NODE_CLOSEST + 36 * N_TRI_CLOSEST
 BYTES_ANY = 32 + 4 + 32 * N_NODE_ANY + 36 * N_TRI_ANY
 HBM_PEAK_GBS = 8000.0
 CPU_SAMPLE = 1 << 22
+TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                               "k_trace4_traversal_2e20tris.md")
+
+
+def measured_traffic(key):
+    """HBM bytes per ray of the traversal kernel from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (counters scaled by a plain copy of known
+    size); None when the summary is absent."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith(key + ":"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def log(msg):
@@ -188,6 +204,7 @@ def main():
 
     value = N_CLOSEST * args.steps / wall_c / 1e6
     achieved = N_CLOSEST * BYTES_CLOSEST / (kern_c * 1e-3) / 1e9
+    tpr = measured_traffic("traffic_bytes_per_ray_closest")
     line = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -205,7 +222,8 @@ def main():
                                "leaf<=4), 2^24 rays: 2^23 closest-hit + 2^23 any-hit",
                    "n_triangles": N_TRIS, "n_rays": N_RAYS, "bins": BINS, "max_leaf": MAX_LEAF},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None if tpr is None else round(tpr * N_CLOSEST),
                      "bytes_per_ray": round(BYTES_CLOSEST, 1), "kernel_ms": round(kern_c, 4)},
         "anyhit_mrays_s": round((N_RAYS - N_CLOSEST) * args.steps / wall_a / 1e6, 2),
         "anyhit_kernel_ms": round(kern_a, 4),

@@ -1011,9 +1011,144 @@ __global__ void k_count_leaves(const BNode* nodes, uint32_t n, uint32_t* cnt) {
 
 inline unsigned blocks(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
+// ------------------------------------------------- 4-wide traversal tree
+// The binary tree (the reference's topology) is collapsed into 4-wide nodes
+// for traversal: each 4-wide node takes a binary interior node's two
+// children and repeatedly opens the interior child with the largest surface
+// area until it has four children (the standard greedy collapse). The set of
+// children depends only on the subtree, so every binary node's candidate is
+// planned in parallel; reachability from the root marks the ones used.
+__device__ __forceinline__ float node_area(const BNode& b) {
+  V3 lo = v3(key2f(b.bkey[0]), key2f(b.bkey[1]), key2f(b.bkey[2]));
+  V3 hi = v3(key2f(b.bkey[3]), key2f(b.bkey[4]), key2f(b.bkey[5]));
+  return surface_area(lo, hi);
+}
+
+__device__ __forceinline__ bool is_interior(const BNode& b) { return (b.flags & FLAG_USED) && !(b.flags & FLAG_LEAF); }
+
+__global__ void k_collapse_plan(const BNode* nodes, uint32_t n, int4* cand) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !is_interior(nodes[i])) return;
+  int kids[4] = {nodes[i].left, nodes[i].right, -1, -1};
+  int nk = 2;
+  while (nk < 4) {
+    int best = -1;
+    float best_a = -1.0f;
+    for (int k = 0; k < nk; ++k) {
+      const BNode& c = nodes[kids[k]];
+      if (is_interior(c)) {
+        float a = node_area(c);
+        if (a > best_a) {
+          best_a = a;
+          best = k;
+        }
+      }
+    }
+    if (best < 0) break;
+    int open = kids[best];
+    kids[best] = nodes[open].left;
+    kids[nk++] = nodes[open].right;
+  }
+  cand[i] = make_int4(kids[0], kids[1], kids[2], kids[3]);
+}
+
+__global__ void k_collapse_mark(const BNode* nodes, uint32_t n, const int4* cand, uint32_t* used) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !used[i]) return;
+  int4 c = cand[i];
+  int kk[4] = {c.x, c.y, c.z, c.w};
+  for (int k = 0; k < 4; ++k)
+    if (kk[k] >= 0 && is_interior(nodes[kk[k]])) used[kk[k]] = 1u;
+}
+
+__global__ void k_collapse_root(uint32_t* used) { used[0] = 1u; }
+
+__global__ void k_collapse_write(const BNode* nodes, uint32_t n, const int4* cand, const uint32_t* used,
+                                 const uint32_t* map, Node4* out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !used[i]) return;
+  int4 c = cand[i];
+  int kk[4] = {c.x, c.y, c.z, c.w};
+  float lo[3][4], hi[3][4];
+  int ref[4], cnt[4];
+  for (int k = 0; k < 4; ++k) {
+    if (kk[k] < 0) {
+      for (int a = 0; a < 3; ++a) lo[a][k] = hi[a][k] = 0.0f;
+      ref[k] = -1;
+      cnt[k] = -1;
+      continue;
+    }
+    const BNode& b = nodes[kk[k]];
+    for (int a = 0; a < 3; ++a) {
+      lo[a][k] = key2f(b.bkey[a]);
+      hi[a][k] = key2f(b.bkey[3 + a]);
+    }
+    bool leaf = (b.flags & FLAG_LEAF) != 0;
+    ref[k] = leaf ? (int)b.start : (int)map[kk[k]];
+    cnt[k] = leaf ? (int)b.count : 0;
+  }
+  Node4 o;
+  o.lox = make_float4(lo[0][0], lo[0][1], lo[0][2], lo[0][3]);
+  o.hix = make_float4(hi[0][0], hi[0][1], hi[0][2], hi[0][3]);
+  o.loy = make_float4(lo[1][0], lo[1][1], lo[1][2], lo[1][3]);
+  o.hiy = make_float4(hi[1][0], hi[1][1], hi[1][2], hi[1][3]);
+  o.loz = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
+  o.hiz = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
+  o.ref = make_int4(ref[0], ref[1], ref[2], ref[3]);
+  o.cnt = make_int4(cnt[0], cnt[1], cnt[2], cnt[3]);
+  out[map[i]] = o;
+}
+
+__global__ void k_collapse_leaf_root(const BNode* nodes, Node4* out) {
+  const BNode& r = nodes[0];
+  Node4 o;
+  o.lox = make_float4(key2f(r.bkey[0]), 0, 0, 0);
+  o.hix = make_float4(key2f(r.bkey[3]), 0, 0, 0);
+  o.loy = make_float4(key2f(r.bkey[1]), 0, 0, 0);
+  o.hiy = make_float4(key2f(r.bkey[4]), 0, 0, 0);
+  o.loz = make_float4(key2f(r.bkey[2]), 0, 0, 0);
+  o.hiz = make_float4(key2f(r.bkey[5]), 0, 0, 0);
+  o.ref = make_int4((int)r.start, -1, -1, -1);
+  o.cnt = make_int4((int)r.count, -1, -1, -1);
+  out[0] = o;
+}
+
 }  // namespace
 
+void bvh_collapse4(pyre_scene* s, Scratch& S, hipStream_t st) {
+  const uint32_t nb = s->n_bnodes;
+  dev_free(s->d_nodes4);
+  if (s->n_interior == 0) {
+    s->n_nodes4 = 1;
+    s->d_nodes4 = dev_alloc<Node4>(1);
+    hipLaunchKernelGGL(k_collapse_leaf_root, dim3(1), dim3(1), 0, st, s->d_bnodes, s->d_nodes4);
+    s->depth4 = 1;
+    return;
+  }
+  int4* cand = S.get<int4>(nb);
+  uint32_t* used = S.get<uint32_t>(nb + 1);
+  uint32_t* map = S.get<uint32_t>(nb + 1);
+  uint32_t* total = S.get<uint32_t>(1);
+  PYRE_HIP_CHECK(hipMemsetAsync(used, 0, 4 * (size_t)(nb + 1), st));
+  hipLaunchKernelGGL(k_collapse_plan, dim3(blocks(nb, 256)), dim3(256), 0, st, s->d_bnodes, nb, cand);
+  hipLaunchKernelGGL(k_collapse_root, dim3(1), dim3(1), 0, st, used);
+  // every 4-wide level consumes at least one binary level
+  for (uint32_t it = 0; it <= s->depth; ++it)
+    hipLaunchKernelGGL(k_collapse_mark, dim3(blocks(nb, 256)), dim3(256), 0, st, s->d_bnodes, nb, cand, used);
+  exclusive_scan(S, used, map, nb + 1, total, st);
+  uint32_t h_total = 0;
+  PYRE_HIP_CHECK(hipMemcpyAsync(&h_total, total, 4, hipMemcpyDeviceToHost, st));
+  PYRE_HIP_CHECK(hipStreamSynchronize(st));
+  s->n_nodes4 = h_total;
+  s->d_nodes4 = dev_alloc<Node4>(h_total);
+  hipLaunchKernelGGL(k_collapse_write, dim3(blocks(nb, 256)), dim3(256), 0, st, s->d_bnodes, nb, cand, used, map,
+                     s->d_nodes4);
+  // each 4-wide level spans at most two binary levels and at least one
+  s->depth4 = s->depth;
+}
+
 void bvh_release(pyre_scene* s) {
+  dev_free(s->d_nodes4);
   dev_free(s->d_nodes);
   dev_free(s->d_tris);
   dev_free(s->d_prim_order);
@@ -1135,6 +1270,7 @@ void bvh_build(pyre_scene* s, const pyre_bvh_params& prm, hipStream_t st) {
     hipLaunchKernelGGL(k_flatten, dim3(blocks(nb, 256)), dim3(256), 0, st, nodes, nb, flag, map, s->d_nodes);
   s->d_tris = dev_alloc<float4>(3 * (size_t)n);
   hipLaunchKernelGGL(k_reorder_tris, dim3(blocks(n, 256)), dim3(256), 0, st, s->d_verts, ref, n, s->d_tris);
+  bvh_collapse4(s, S, st);
   PYRE_HIP_CHECK(hipGetLastError());
   PYRE_HIP_CHECK(hipStreamSynchronize(st));
   s->built = true;

@@ -37,7 +37,7 @@ constexpr uint32_t FL_SPECULAR = 1u, FL_AOV = 2u;
 constexpr float kRayEps = 1e-3f;  // integrator.rs:34
 
 struct SceneDev {
-  const Node2* nodes;
+  const Node4* nodes4;
   const float4* tris;
   const float* verts;
   const float* normals;
@@ -136,27 +136,31 @@ __global__ __launch_bounds__(kThreads) void k_generate(Paths P, PassParams pp, C
   }
 }
 
-template <int STACK>
-__global__ __launch_bounds__(kThreads) void k_extend(Paths P, SceneDev S, uint32_t* overflow) {
-  __shared__ int32_t lds_stack[STACK * kThreads];
+constexpr int kRing = 8;
+
+__global__ __launch_bounds__(kThreads) void k_extend(Paths P, SceneDev S, int32_t* gstack) {
+  __shared__ int32_t ring[kRing * kThreads];
   const uint32_t n = P.counters[0];
+  const uint32_t tid = blockIdx.x * kThreads + threadIdx.x;
   const uint32_t stride = gridDim.x * kThreads;
-  for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) {
+  HybridStack<kRing> st{ring + threadIdx.x, kThreads, gstack + tid, stride, 0, 0};
+  for (uint32_t i = tid; i < n; i += stride) {
     uint32_t p = P.q_cur[i];
     float4 a = P.ray[2 * (size_t)p], b = P.ray[2 * (size_t)p + 1];
     TRay r = make_tray(xyz(a), xyz(b), a.w, b.w);
     THit h;
-    if (!traverse<false>(S.nodes, S.tris, r, lds_stack + threadIdx.x, kThreads, STACK, &h)) atomicOr(overflow, 1u);
+    traverse4<false, kRing>(S.nodes4, S.tris, r, st, &h);
     P.hit[p] = make_float4(__builtin_bit_cast(float, h.gid), h.t, h.u, h.v);
   }
 }
 
-template <int STACK>
-__global__ __launch_bounds__(kThreads) void k_shadow(Paths P, SceneDev S, uint32_t* overflow) {
-  __shared__ int32_t lds_stack[STACK * kThreads];
+__global__ __launch_bounds__(kThreads) void k_shadow(Paths P, SceneDev S, int32_t* gstack) {
+  __shared__ int32_t ring[kRing * kThreads];
   const uint32_t n = P.counters[2];
+  const uint32_t tid = blockIdx.x * kThreads + threadIdx.x;
   const uint32_t stride = gridDim.x * kThreads;
-  for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) {
+  HybridStack<kRing> st{ring + threadIdx.x, kThreads, gstack + tid, stride, 0, 0};
+  for (uint32_t i = tid; i < n; i += stride) {
     uint32_t p = P.q_shadow[i];
     uint32_t cnt = P.scount[p];
     float4 L = P.lum[p];
@@ -167,8 +171,7 @@ __global__ __launch_bounds__(kThreads) void k_shadow(Paths P, SceneDev S, uint32
       if (b.w > a.w) {  // empty range (distance <= 0): nothing can occlude (scene.rs:123-125)
         TRay r = make_tray(xyz(a), xyz(b), a.w, b.w);
         THit h;
-        if (!traverse<true>(S.nodes, S.tris, r, lds_stack + threadIdx.x, kThreads, STACK, &h))
-          atomicOr(overflow, 1u);
+        traverse4<true, kRing>(S.nodes4, S.tris, r, st, &h);
         occ = h.gid != PYRE_MISS;
       }
       if (!occ) {
@@ -460,28 +463,12 @@ struct Buf {
   ~Buf() { dev_free(p); }
 };
 
-template <int STACK>
-void launch_trace_stages(bool shadow, unsigned grid, hipStream_t st, const Paths& P, const SceneDev& S,
-                         uint32_t* ovf) {
+void trace_stage(pyre_scene* s, bool shadow, hipStream_t st, const Paths& P, const SceneDev& S) {
+  const unsigned grid = trace_grid(s);
   if (!shadow)
-    hipLaunchKernelGGL((k_extend<STACK>), dim3(grid), dim3(kThreads), 0, st, P, S, ovf);
+    hipLaunchKernelGGL(k_extend, dim3(grid), dim3(kThreads), 0, st, P, S, trace_gstack(s));
   else
-    hipLaunchKernelGGL((k_shadow<STACK>), dim3(grid), dim3(kThreads), 0, st, P, S, ovf);
-}
-
-void trace_stage(uint32_t depth, bool shadow, unsigned grid, hipStream_t st, const Paths& P, const SceneDev& S,
-                 uint32_t* ovf) {
-  const uint32_t need = depth + 1;
-  if (need <= 16)
-    launch_trace_stages<16>(shadow, grid, st, P, S, ovf);
-  else if (need <= 32)
-    launch_trace_stages<32>(shadow, grid, st, P, S, ovf);
-  else if (need <= 48)
-    launch_trace_stages<48>(shadow, grid, st, P, S, ovf);
-  else if (need <= 64)
-    launch_trace_stages<64>(shadow, grid, st, P, S, ovf);
-  else
-    throw Error(PYRE_EOVERFLOW, "BVH deeper than 63 levels (the reference's traversal stack is 64 entries)");
+    hipLaunchKernelGGL(k_shadow, dim3(grid), dim3(kThreads), 0, st, P, S, trace_gstack(s));
 }
 
 }  // namespace
@@ -505,7 +492,7 @@ extern "C" int pyre_render(pyre_ctx* ctx, pyre_scene* s, const pyre_camera_desc*
     hipStream_t st = ctx->stream;
     const uint32_t npix = width * height;
     const CamDev cam = make_camera(*camera);
-    SceneDev S{s->d_nodes, s->d_tris, s->d_verts, s->d_normals, s->d_tri_offset, s->d_mesh_info, s->d_mats,
+    SceneDev S{s->d_nodes4, s->d_tris, s->d_verts, s->d_normals, s->d_tri_offset, s->d_mesh_info, s->d_mats,
                s->d_lights, s->n_meshes, s->n_lights, s->has_env ? 1u : 0u, s->env};
     const uint32_t slots = std::max(1u, s->n_lights + (s->has_env ? 1u : 0u));
     // paths per pass: whole samples of the image, about 2M paths
@@ -537,9 +524,9 @@ extern "C" int pyre_render(pyre_ctx* ctx, pyre_scene* s, const pyre_camera_desc*
       hipLaunchKernelGGL(k_counters, dim3(1), dim3(1), 0, st, ctr.p, pp.paths, 0u, 0u);
       hipLaunchKernelGGL(k_generate, dim3(grid), dim3(kThreads), 0, st, P, pp, cam);
       for (uint32_t d = 0; d < integrator->max_depth; ++d) {
-        trace_stage(s->depth, false, grid, st, P, S, s->d_flag);
+        trace_stage(s, false, st, P, S);
         hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kThreads), 0, st, P, S, pp, d);
-        if (s->n_lights + (s->has_env ? 1u : 0u) > 0) trace_stage(s->depth, true, grid, st, P, S, s->d_flag);
+        if (s->n_lights + (s->has_env ? 1u : 0u) > 0) trace_stage(s, true, st, P, S);
         hipLaunchKernelGGL(k_rotate, dim3(1), dim3(1), 0, st, ctr.p, stats.p);
         std::swap(P.q_cur, P.q_next);
       }

@@ -63,6 +63,15 @@ struct alignas(16) Node2 {
   int4 meta;
 };
 
+// 4-wide traversal node, 128 bytes (one L2 line): child boxes as
+// per-axis lanes, child refs and counts as in Node2 (count -1 = no child).
+struct alignas(16) Node4 {
+  float4 lox, hix, loy, hiy, loz, hiz;
+  int4 ref;
+  int4 cnt;
+};
+static_assert(sizeof(Node4) == 128, "Node4 layout");
+
 // Build node (binned-SAH builder). Bounds are kept as order-preserving int
 // keys so they can be accumulated with integer atomics.
 struct BNode {
@@ -134,6 +143,10 @@ struct pyre_scene {
   bool built = false;
   pyre::Node2* d_nodes = nullptr;
   uint32_t n_nodes2 = 0;
+  pyre::Node4* d_nodes4 = nullptr;
+  uint32_t n_nodes4 = 0, depth4 = 0;
+  int32_t* d_gstack = nullptr;  // spill columns of the persistent traversal kernels
+  size_t gstack_cap = 0;
   float4* d_tris = nullptr;        // 3 float4 per slot: (v0, gid bits), (e1, 0), (e2, 0)
   uint32_t* d_prim_order = nullptr;  // scene-global triangle id per slot
   pyre::BNode* d_bnodes = nullptr;
@@ -153,4 +166,6 @@ void bvh_release(pyre_scene* s);
 void trace_closest(pyre_scene* s, const float4* rays, size_t n, pyre_hit8* hits, float2* uv, hipStream_t st,
                    bool check);
 void trace_any(pyre_scene* s, const float4* rays, size_t n, uint32_t* occluded, hipStream_t st, bool check);
+unsigned trace_grid(pyre_scene* s);
+int32_t* trace_gstack(pyre_scene* s);
 }  // namespace pyre

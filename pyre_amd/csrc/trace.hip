@@ -1,11 +1,20 @@
 // Batched closest-hit / any-hit kernels (Scene::intersect / occluded over a
-// ray array) built on the traversal core in traverse.hpp.
+// ray array) built on the traversal cores in traverse.hpp.
+//
+// Default: the 4-wide tree, persistent grid-stride threads, an 8-entry LDS
+// stack ring per thread spilling to a per-thread global column. The binary
+// tree kernels (LDS stack as deep as the tree) remain selectable with
+// PYRE_TRAVERSAL=binary for comparison.
+#include <cstdlib>
+#include <cstring>
+
 #include "traverse.hpp"
 
 namespace pyre {
 namespace {
 
 constexpr int kTraceThreads = 256;
+constexpr int kRing = 8;
 
 template <bool ANY, int STACK>
 __global__ __launch_bounds__(kTraceThreads) void k_trace(const Node2* __restrict__ nodes,
@@ -46,13 +55,54 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(const Node2* __restrict
 }
 
 template <bool ANY>
+__global__ __launch_bounds__(kTraceThreads) void k_trace4(const Node4* __restrict__ nodes,
+                                                          const float4* __restrict__ tris,
+                                                          const float4* __restrict__ rays, uint32_t n,
+                                                          pyre_hit8* __restrict__ out_hit,
+                                                          float2* __restrict__ out_uv,
+                                                          uint32_t* __restrict__ out_occ, int32_t* gstack) {
+  __shared__ int32_t ring[kRing * kTraceThreads];
+  const uint32_t tid = blockIdx.x * kTraceThreads + threadIdx.x;
+  const uint32_t nthreads = gridDim.x * kTraceThreads;
+  HybridStack<kRing> st{ring + threadIdx.x, kTraceThreads, gstack + tid, nthreads, 0, 0};
+  for (uint32_t i = tid; i < n; i += nthreads) {
+    const float4 ra = rays[2 * (size_t)i], rb = rays[2 * (size_t)i + 1];
+    const TRay r = make_tray(v3(ra.x, ra.y, ra.z), v3(rb.x, rb.y, rb.z), ra.w, rb.w);
+    THit h;
+    traverse4<ANY, kRing>(nodes, tris, r, st, &h);
+    if (ANY) {
+      out_occ[i] = h.gid != PYRE_MISS ? 1u : 0u;
+    } else {
+      pyre_hit8 o;
+      o.prim = h.gid;
+      o.t = h.t;
+      out_hit[i] = o;
+      if (out_uv) out_uv[i] = make_float2(h.u, h.v);
+    }
+  }
+}
+
+bool use_binary() {
+  const char* e = std::getenv("PYRE_TRAVERSAL");
+  return e && std::strcmp(e, "binary") == 0;
+}
+
+template <bool ANY>
 void launch(pyre_scene* s, const float4* rays, size_t n, pyre_hit8* hit, float2* uv, uint32_t* occ, hipStream_t st,
             bool check) {
   require(s->built, "trace before pyre_bvh_build");
   require(n <= 0xFFFFFFFFull, "too many rays in one call");
   if (n == 0) return;
-  dim3 grid((unsigned)((n + kTraceThreads - 1) / kTraceThreads)), block(kTraceThreads);
   uint32_t* ovf = s->d_flag;
+  if (!use_binary()) {
+    unsigned want = (unsigned)((n + kTraceThreads - 1) / kTraceThreads);
+    unsigned grid = std::min(want, trace_grid(s));
+    hipLaunchKernelGGL((k_trace4<ANY>), dim3(grid), dim3(kTraceThreads), 0, st, s->d_nodes4, s->d_tris, rays,
+                       (uint32_t)n, hit, uv, occ, trace_gstack(s));
+    PYRE_HIP_CHECK(hipGetLastError());
+    return;
+  }
+  dim3 grid((unsigned)((n + kTraceThreads - 1) / kTraceThreads)), block(kTraceThreads);
   int32_t* gst = nullptr;
   const uint32_t need = s->depth + 1;
   if (need <= 16)
@@ -83,6 +133,28 @@ void launch(pyre_scene* s, const float4* rays, size_t n, pyre_hit8* hit, float2*
 }
 
 }  // namespace
+
+// Persistent grid of the 4-wide kernels: blocks per CU (PYRE_TRACE_BLOCKS_PER_CU,
+// default 8) x CUs.
+unsigned trace_grid(pyre_scene* s) {
+  int per_cu = 8;
+  if (const char* e = std::getenv("PYRE_TRACE_BLOCKS_PER_CU")) per_cu = std::max(1, std::atoi(e));
+  return (unsigned)std::max(1, s->ctx->cus) * (unsigned)per_cu;
+}
+
+// Spill columns for the persistent kernels: 3 entries per binary level bound
+// the pending children of every 4-wide node on the current path.
+int32_t* trace_gstack(pyre_scene* s) {
+  const size_t threads = (size_t)trace_grid(s) * kTraceThreads;
+  const size_t need = threads * (3 * (size_t)s->depth + 4);
+  if (need > s->gstack_cap) {
+    PYRE_HIP_CHECK(hipDeviceSynchronize());
+    dev_free(s->d_gstack);
+    s->d_gstack = dev_alloc<int32_t>(need);
+    s->gstack_cap = need;
+  }
+  return s->d_gstack;
+}
 
 void trace_closest(pyre_scene* s, const float4* rays, size_t n, pyre_hit8* hits, float2* uv, hipStream_t st,
                    bool check) {

@@ -133,4 +133,153 @@ __device__ __forceinline__ bool traverse(const Node2* __restrict__ nodes, const 
   return ok;
 }
 
+// Per-thread traversal stack: a ring of SL entries in LDS (a column of a
+// [SL][threads] array, so a wave's accesses at equal depth hit distinct
+// banks) whose oldest entries spill to a per-thread column of a global array
+// when the ring is full. Positions below `lo` live in global memory.
+template <int SL>
+struct HybridStack {
+  int32_t* lds;
+  int lstride;
+  int32_t* glb;
+  size_t gstride;
+  int sp, lo;
+  __device__ __forceinline__ void push(int32_t v) {
+    if (sp - lo == SL) {
+      glb[(size_t)lo * gstride] = lds[(lo & (SL - 1)) * lstride];
+      lo++;
+    }
+    lds[(sp & (SL - 1)) * lstride] = v;
+    sp++;
+  }
+  __device__ __forceinline__ int32_t pop() {
+    sp--;
+    if (sp < lo) {
+      lo = sp;
+      return glb[(size_t)sp * gstride];
+    }
+    return lds[(sp & (SL - 1)) * lstride];
+  }
+};
+
+// Leaf triangles [s, s + cnt): Moller-Trumbore against the current closest.
+template <bool ANY>
+__device__ __forceinline__ bool leaf_hits(const float4* __restrict__ tris, const TRay& r, int32_t s, int cnt,
+                                          float& best_t, uint32_t& best_gid, float& bu, float& bv, bool& found) {
+  for (int k = 0; k < cnt; ++k) {
+    const float4 a = tris[3 * (size_t)(s + k)];
+    const float4 b = tris[3 * (size_t)(s + k) + 1];
+    const float4 c = tris[3 * (size_t)(s + k) + 2];
+    float t, u, v;
+    if (ANY) {
+      if (mt_intersect(r.o, r.d, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), r.t_min, r.t_max, false, &t,
+                       &u, &v)) {
+        found = true;
+        return true;
+      }
+    } else if (mt_intersect(r.o, r.d, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), r.t_min, best_t, true,
+                            &t, &u, &v)) {
+      uint32_t gid = __builtin_bit_cast(uint32_t, a.w);
+      if (t < best_t || (found && gid < best_gid)) {
+        best_t = t;
+        best_gid = gid;
+        bu = u;
+        bv = v;
+        found = true;
+      }
+    }
+  }
+  return false;
+}
+
+__device__ __forceinline__ void cswap(float& ka, int32_t& ra, int& ca, float& kb, int32_t& rb, int& cb) {
+  if (kb < ka) {
+    float tk = ka;
+    ka = kb;
+    kb = tk;
+    int32_t tr = ra;
+    ra = rb;
+    rb = tr;
+    int tc = ca;
+    ca = cb;
+    cb = tc;
+  }
+}
+
+// Traversal of the 4-wide tree. Same hit semantics as traverse(): the
+// children are visited nearest-first by slab entry distance, leaves as soon
+// as their box is hit.
+template <bool ANY, int SL>
+__device__ __forceinline__ void traverse4(const Node4* __restrict__ nodes, const float4* __restrict__ tris,
+                                          const TRay& r, HybridStack<SL>& st, THit* out) {
+  float best_t = r.t_max;
+  uint32_t best_gid = PYRE_MISS;
+  float bu = 0.0f, bv = 0.0f;
+  bool found = false;
+  st.sp = 0;
+  st.lo = 0;
+  int32_t cur = 0;
+  for (;;) {
+    const Node4 nd = nodes[cur];
+    // missed children: key +inf and count -1 (a hit child can have key +inf
+    // only if its box is at infinity, so the count marks misses)
+    float k0 = kInf, k1 = kInf, k2 = kInf, k3 = kInf;
+    int c0 = -1, c1 = -1, c2 = -1, c3 = -1;
+    float e;
+    if (nd.cnt.x >= 0 && slab_test(r, nd.lox.x, nd.hix.x, nd.loy.x, nd.hiy.x, nd.loz.x, nd.hiz.x, best_t, &e)) {
+      k0 = e;
+      c0 = nd.cnt.x;
+    }
+    if (nd.cnt.y >= 0 && slab_test(r, nd.lox.y, nd.hix.y, nd.loy.y, nd.hiy.y, nd.loz.y, nd.hiz.y, best_t, &e)) {
+      k1 = e;
+      c1 = nd.cnt.y;
+    }
+    if (nd.cnt.z >= 0 && slab_test(r, nd.lox.z, nd.hix.z, nd.loy.z, nd.hiy.z, nd.loz.z, nd.hiz.z, best_t, &e)) {
+      k2 = e;
+      c2 = nd.cnt.z;
+    }
+    if (nd.cnt.w >= 0 && slab_test(r, nd.lox.w, nd.hix.w, nd.loy.w, nd.hiy.w, nd.loz.w, nd.hiz.w, best_t, &e)) {
+      k3 = e;
+      c3 = nd.cnt.w;
+    }
+    int32_t r0 = nd.ref.x, r1 = nd.ref.y, r2 = nd.ref.z, r3 = nd.ref.w;
+    // sort the four children by entry distance (misses sort last as +inf)
+    cswap(k0, r0, c0, k1, r1, c1);
+    cswap(k2, r2, c2, k3, r3, c3);
+    cswap(k0, r0, c0, k2, r2, c2);
+    cswap(k1, r1, c1, k3, r3, c3);
+    cswap(k1, r1, c1, k2, r2, c2);
+    float ks[4] = {k0, k1, k2, k3};
+    int32_t rs[4] = {r0, r1, r2, r3};
+    int cs[4] = {c0, c1, c2, c3};
+    int32_t inner[4];
+    int ni = 0;
+    bool done = false;
+    for (int q = 0; q < 4; ++q) {
+      if (cs[q] < 0) continue;       // missed
+      if (ks[q] > best_t) continue;  // entered beyond the closest hit found meanwhile
+      if (cs[q] > 0) {
+        if (leaf_hits<ANY>(tris, r, rs[q], cs[q], best_t, best_gid, bu, bv, found)) {
+          done = true;
+          break;
+        }
+      } else {
+        inner[ni++] = rs[q];
+      }
+    }
+    if (done) break;
+    if (ni > 0) {
+      for (int q = ni - 1; q >= 1; --q) st.push(inner[q]);
+      cur = inner[0];
+    } else {
+      if (st.sp == 0) break;
+      cur = st.pop();
+    }
+  }
+  out->gid = ANY ? (found ? 0u : PYRE_MISS) : (found ? best_gid : PYRE_MISS);
+  out->t = found ? best_t : 0.0f;
+  out->u = bu;
+  out->v = bv;
+}
+
 }  // namespace pyre

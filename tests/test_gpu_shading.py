@@ -120,11 +120,15 @@ def test_env_sample_pdf_le(ctx, oracle):
     near = (np.abs(fu - np.round(fu)) < 1e-6 * env["width"]) | (np.abs(fv - np.round(fv)) < 1e-6 * env["height"])
     p = sc.env_pdf(d)
     op = oe.pdf(d)
-    # acos(y) is ill-conditioned in f32 next to the poles (light.rs:275), so
-    # sin(theta) there differs by more than an ulp between libm implementations
+    # pdf divides by sin(theta), theta = acos(y) (light.rs:275, :340-349). The
+    # device acosf and glibc's may differ by an ulp or two of theta; near
+    # theta = pi an ulp is 2.4e-7, i.e. a relative change of 2.4e-7 / sin(theta)
+    # in the pdf (1.7e-5 at |y| = 0.9999).
     keep = ~near & (np.abs(d[:, 1]) < 0.9999)
     assert near.sum() < 50
-    assert np.all(rel_err(p[keep], op[keep]) <= TOL)
+    sin_t = np.sqrt(np.maximum(1.0 - d[:, 1].astype(np.float64) ** 2, 1e-30))
+    tol = TOL + 3.0 * float(np.spacing(np.float32(np.pi))) / sin_t
+    assert np.all(rel_err(p[keep], op[keep]) <= tol[keep])
     assert np.all(rel_err(sc.env_le(d), oe.le(d)) <= 1e-4)
 
 
