@@ -282,6 +282,55 @@ int pyre_nrc_gradients(pyre_ctx* ctx, pyre_nrc* nrc, const float* feats, const f
 /* One Adam step on the batch; loss receives the pre-step loss. */
 int pyre_nrc_train_step(pyre_ctx* ctx, pyre_nrc* nrc, const float* feats, const float* targets, size_t n,
                         float* loss);
+/* Adam steps taken so far and the loss of the last training batch. */
+int pyre_nrc_stats(pyre_ctx* ctx, pyre_nrc* nrc, uint64_t* steps, float* last_loss);
+
+/* -------------------------------------------------- progressive session */
+/* The render loop of main.rs:253-287 (and the viewer's progressive passes,
+ * viewer.rs:114-184) with device-resident state: path buffers, the f32
+ * PixelSample sums and FP64 radiance sums stay on the GPU between calls, and
+ * each pass is a captured hipGraph replayed with new sample indices.
+ * Samples are taken in order s = 0, 1, 2, ... per pixel with the streams
+ * pixel_seed(x, y, s), so N spp from a session equal pyre_render(spp = N). */
+typedef struct pyre_session pyre_session;
+
+/* nrc_train_paths > 0 reserves the radiance-cache buffers (training paths
+ * per frame); 0 for plain rendering only. */
+int pyre_session_create(pyre_ctx* ctx, pyre_scene* scene, const pyre_camera_desc* camera,
+                        const pyre_integrator_params* integrator, uint32_t width, uint32_t height,
+                        uint32_t nrc_train_paths, pyre_session** out);
+void pyre_session_destroy(pyre_session* session);
+/* Adds spp samples per pixel (plain path tracer). */
+int pyre_session_render(pyre_ctx* ctx, pyre_session* session, uint32_t spp);
+
+/* Radiance-cache rendering (no counterpart in the reference). Per frame:
+ * train_paths full unbiased paths from scattered pixels are traced, every
+ * surface vertex yields a training sample (the query there and the radiance
+ * the rest of the path carried back to it), train_steps Adam steps run on
+ * batches of train_batch samples, then spp_per_frame samples per pixel are
+ * rendered with paths ending in the cache at the first surface reached after
+ * a bounce off a diffuse material (Lambertian, or Disney roughness >=
+ * diffuse_roughness). Frames before warmup_frames train but do not add to
+ * the image. */
+typedef struct pyre_nrc_render_params {
+  uint32_t frames;
+  uint32_t spp_per_frame;
+  uint32_t train_paths;
+  uint32_t train_batch;
+  uint32_t train_steps;
+  float diffuse_roughness;
+  uint32_t warmup_frames;
+} pyre_nrc_render_params;
+
+/* losses: frames floats (mean training loss per frame) or NULL. */
+int pyre_session_render_nrc(pyre_ctx* ctx, pyre_session* session, pyre_nrc* nrc, const pyre_nrc_render_params* p,
+                            float* losses);
+/* Per-pixel mean over the samples so far (PixelSample, width*height) and/or
+ * the FP64 radiance sums (3 doubles per pixel), and the samples per pixel. */
+int pyre_session_read(pyre_ctx* ctx, pyre_session* session, pyre_pixel_sample* mean, double* radiance_sum64,
+                      uint32_t* spp_done);
+/* Clears the sums; the sample index keeps advancing. */
+int pyre_session_reset(pyre_ctx* ctx, pyre_session* session);
 
 #ifdef __cplusplus
 }

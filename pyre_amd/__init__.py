@@ -9,6 +9,7 @@ call raises.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -48,6 +49,12 @@ class Context:
         if rc != 0:
             raise PyreError(rc, f"pyre_ctx_create({device}) failed")
         self.h = h
+        # scenes, caches and sessions hold device memory of this context and
+        # must be destroyed before it
+        self._children = weakref.WeakSet()
+
+    def _adopt(self, obj):
+        self._children.add(obj)
 
     def check(self, rc: int, what: str):
         if rc != 0:
@@ -62,6 +69,11 @@ class Context:
 
     def close(self):
         if getattr(self, "h", None):
+            kids = list(getattr(self, "_children", ()))
+            for kind in (Session, RadianceCache, Scene):
+                for k in kids:
+                    if isinstance(k, kind):
+                        k.close()
             self.lib.pyre_ctx_destroy(self.h)
             self.h = None
 
@@ -187,6 +199,7 @@ class Scene:
         ctx.check(self.lib.pyre_scene_upload(ctx.h, C.byref(desc), C.byref(h)), "pyre_scene_upload")
         self.h = h
         self._keep = keep
+        ctx._adopt(self)
 
     @classmethod
     def from_preset(cls, ctx: Context, preset: Preset) -> "Scene":
@@ -424,6 +437,7 @@ class RadianceCache:
         h = C.c_void_p()
         ctx.check(self.lib.pyre_nrc_create(ctx.h, C.byref(p), C.byref(h)), "pyre_nrc_create")
         self.h = h
+        ctx._adopt(self)
         self.n_params = int(self.lib.pyre_nrc_param_count())
 
     def weights(self):
@@ -460,9 +474,73 @@ class RadianceCache:
                                                     C.byref(loss)), "pyre_nrc_train_step")
         return float(loss.value)
 
+    def stats(self):
+        """(Adam steps taken, loss of the last training batch)."""
+        steps, loss = C.c_uint64(), C.c_float()
+        self.ctx.check(self.lib.pyre_nrc_stats(self.ctx.h, self.h, C.byref(steps), C.byref(loss)), "pyre_nrc_stats")
+        return int(steps.value), float(loss.value)
+
     def close(self):
         if getattr(self, "h", None):
             self.lib.pyre_nrc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Session:
+    """Progressive render session (pyre_session_*): the main.rs:253-287 loop
+    with device-resident sums; samples s = 0, 1, 2, ... per pixel, so N spp
+    equal Film.render(spp=N). nrc_train_paths > 0 enables render_nrc."""
+
+    def __init__(self, scene: Scene, camera: Camera, integrator: PathIntegrator, width: int, height: int,
+                 nrc_train_paths: int = 0):
+        self.scene = scene
+        self.ctx = scene.ctx
+        self.lib = scene.lib
+        self.width, self.height = width, height
+        cam = camera.to_c()
+        ip = L.IntegratorParams(integrator.max_depth, integrator.min_rr_depth)
+        h = C.c_void_p()
+        self.ctx.check(self.lib.pyre_session_create(self.ctx.h, scene.h, C.byref(cam), C.byref(ip), width, height,
+                                                    nrc_train_paths, C.byref(h)), "pyre_session_create")
+        self.h = h
+        self.ctx._adopt(self)
+
+    def render(self, spp: int):
+        self.ctx.check(self.lib.pyre_session_render(self.ctx.h, self.h, spp), "pyre_session_render")
+
+    def render_nrc(self, cache: RadianceCache, frames: int, spp_per_frame: int = 1, train_paths: int = 1 << 14,
+                   train_batch: int = 1 << 14, train_steps: int = 4, diffuse_roughness: float = 0.25,
+                   warmup_frames: int = 0):
+        p = L.NrcRenderParams(frames, spp_per_frame, train_paths, train_batch, train_steps, diffuse_roughness,
+                              warmup_frames)
+        losses = np.zeros(max(frames, 1), np.float32)
+        self.ctx.check(self.lib.pyre_session_render_nrc(self.ctx.h, self.h, cache.h, C.byref(p), _ptr(losses)),
+                       "pyre_session_render_nrc")
+        return losses[:frames]
+
+    def read(self, mean: bool = True, radiance64: bool = False):
+        """(PixelSample means (npix, 10) or None, FP64 radiance sums (npix, 3) or None, spp so far)."""
+        n = self.width * self.height
+        m = np.empty((n, 10), np.float32) if mean else None
+        r = np.empty((n, 3), np.float64) if radiance64 else None
+        spp = C.c_uint32()
+        self.ctx.check(self.lib.pyre_session_read(self.ctx.h, self.h, _ptr(m) if mean else None,
+                                                  _ptr(r) if radiance64 else None, C.byref(spp)),
+                       "pyre_session_read")
+        return m, r, int(spp.value)
+
+    def reset(self):
+        self.ctx.check(self.lib.pyre_session_reset(self.ctx.h, self.h), "pyre_session_reset")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.pyre_session_destroy(self.h)
             self.h = None
 
     def __del__(self):

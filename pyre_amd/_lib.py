@@ -160,3 +160,20 @@ SIGNATURES.update({
     "pyre_nrc_gradients": (C.c_int, [_vp, _vp, _vp, _vp, C.c_size_t, _vp, C.POINTER(_f)]),
     "pyre_nrc_train_step": (C.c_int, [_vp, _vp, _vp, _vp, C.c_size_t, C.POINTER(_f)]),
 })
+
+
+class NrcRenderParams(C.Structure):
+    _fields_ = [("frames", _u32), ("spp_per_frame", _u32), ("train_paths", _u32), ("train_batch", _u32),
+                ("train_steps", _u32), ("diffuse_roughness", _f), ("warmup_frames", _u32)]
+
+
+SIGNATURES.update({
+    "pyre_nrc_stats": (C.c_int, [_vp, _vp, C.POINTER(C.c_uint64), C.POINTER(_f)]),
+    "pyre_session_create": (C.c_int, [_vp, _vp, C.POINTER(CameraDesc), C.POINTER(IntegratorParams), _u32, _u32,
+                                      _u32, C.POINTER(_vp)]),
+    "pyre_session_destroy": (None, [_vp]),
+    "pyre_session_render": (C.c_int, [_vp, _vp, _u32]),
+    "pyre_session_render_nrc": (C.c_int, [_vp, _vp, _vp, C.POINTER(NrcRenderParams), _vp]),
+    "pyre_session_read": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(_u32)]),
+    "pyre_session_reset": (C.c_int, [_vp, _vp]),
+})

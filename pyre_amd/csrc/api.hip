@@ -131,6 +131,7 @@ void scene_free(pyre_scene* s) {
   dev_free(s->d_env_marg);
   dev_free(s->d_flag);
   dev_free(s->d_gstack);
+  dev_free(s->d_fetch);
 }
 
 void upload_env(pyre_scene* s, const pyre_env_desc& e, hipStream_t st) {

@@ -14,17 +14,29 @@
 //               scene.rs:102-140); unoccluded contributions are added to the
 //               path radiance in the reference's order (lights, then env)
 //   accumulate  per-pixel f32 sums over samples in sample order
-//               (main.rs:275-285), optional FP64 radiance sums
+//               (main.rs:275-285) and FP64 radiance sums
 //
 // Queues are compacted with wave64 ballots: one atomic per wave. Every
-// stage is a persistent grid-stride kernel whose item count lives in device
-// memory, so a whole bounce needs no host synchronisation. RNG draws happen
-// in exactly the reference's order per path (jitter x/y, lens x/y, time; per
+// stage is a persistent grid-stride kernel whose item count and pass
+// parameters live in device memory, so a pass needs no host synchronisation
+// and is captured once into a hipGraph and replayed. RNG draws happen in
+// exactly the reference's order per path (jitter x/y, lens x/y, time; per
 // surface bounce: 2 per light, 2 for the environment, 2 for the BSDF, 1 for
 // Russian roulette), so a GPU path consumes the same stream as the CPU one.
+//
+// Radiance-cache modes (no counterpart in the reference): TRAIN records, at
+// every surface vertex of a full unbiased path, the cache query and the
+// radiance/throughput before the vertex, so that after the path ends the
+// vertex's outgoing radiance estimate is (L_end - L_before) / beta; CACHE ends
+// a path at the first surface vertex reached by a bounce off a diffuse
+// material and adds beta x cache(query) there.
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <new>
 #include <vector>
 
+#include "nrc.hpp"
 #include "shade.hpp"
 #include "traverse.hpp"
 
@@ -33,8 +45,13 @@ using namespace pyre;
 namespace {
 
 constexpr int kThreads = 256;
-constexpr uint32_t FL_SPECULAR = 1u, FL_AOV = 2u;
+constexpr int kRing = 8;
+constexpr uint32_t FL_SPECULAR = 1u, FL_AOV = 2u, FL_DIFFUSE = 4u;
+constexpr uint32_t MODE_PLAIN = 0u, MODE_TRAIN = 1u, MODE_CACHE = 2u;
 constexpr float kRayEps = 1e-3f;  // integrator.rs:34
+constexpr uint32_t kVrec = 5;     // float4 per training vertex record
+constexpr uint32_t kTrainStreamBase = 0x80000000u;  // sample indices of training paths
+constexpr uint32_t kCounters = 32;
 
 struct SceneDev {
   const Node4* nodes4;
@@ -47,6 +64,13 @@ struct SceneDev {
   const QuadLightDev* lights;
   uint32_t n_meshes, n_lights, has_env;
   EnvDev env;
+};
+
+// Per-pass parameters, read by every stage from device memory.
+struct PassParams {
+  uint32_t width, height, npix, s_base, n_samples, paths, max_depth, min_rr;
+  uint32_t mode, pix_mul, pix_add, vmax;
+  float diffuse_rough;
 };
 
 struct Paths {
@@ -63,13 +87,15 @@ struct Paths {
   uint32_t* q_cur;
   uint32_t* q_next;
   uint32_t* q_shadow;
-  uint32_t* counters;  // [0] current queue, [1] next queue, [2] shadow queue
-  unsigned long long* stats;  // [0] extension rays, [1] shadow rays (whole render)
-  uint32_t slots;      // shadow slots per path (lights + env)
-};
-
-struct PassParams {
-  uint32_t width, height, npix, s_base, n_samples, paths, max_depth, min_rr;
+  uint32_t* counters;  // [0] current, [1] next, [2] shadow, [3] cache queries, [4] training samples,
+                       // [8..15] extend work, [16..23] shadow work
+  unsigned long long* stats;  // [0] extension rays, [1] shadow rays
+  float4* vrec;      // TRAIN: kVrec float4 per (path, vertex)
+  uint32_t* vcount;  // TRAIN: vertices recorded per path
+  float* qfeat;      // CACHE: 13 floats per path (its query, if it ended in the cache)
+  uint32_t* qpath;   // CACHE: path of each query
+  float* qrgb;       // CACHE: cache output per query
+  uint32_t slots;    // shadow slots per path (lights + env)
 };
 
 __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << __lane_id()) - 1ull; }
@@ -96,25 +122,35 @@ __device__ __forceinline__ void store_rng(const Paths& P, uint32_t p, const Rng&
 __device__ __forceinline__ V3 xyz(float4 a) { return v3(a.x, a.y, a.z); }
 __device__ __forceinline__ float4 f4(V3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
 
-__global__ void k_counters(uint32_t* c, uint32_t a, uint32_t b, uint32_t d) {
-  c[0] = a;
-  c[1] = b;
-  c[2] = d;
+__global__ void k_set_pass(PassParams* pp, PassParams v) { *pp = v; }
+
+// counters[8..15] / [16..23]: work counters of the extend / shadow slices
+__global__ void k_counters(uint32_t* c, const PassParams* pp) {
+  const uint32_t i = threadIdx.x;
+  if (i < kCounters) c[i] = i == 0 ? pp->paths : 0u;
 }
 
-// next queue becomes current; next and shadow queues are emptied
+// next queue becomes current; next and shadow queues and the work counters
+// are emptied
 __global__ void k_rotate(uint32_t* c, unsigned long long* stats) {
-  stats[0] += c[0];
-  c[0] = c[1];
-  c[1] = 0;
-  c[2] = 0;
+  const uint32_t i = threadIdx.x;
+  const uint32_t cur = c[0], next = c[1];
+  __syncthreads();
+  if (i == 0) {
+    stats[0] += cur;
+    c[0] = next;
+  } else if (i == 1 || i == 2 || (i >= 8 && i < kCounters)) {
+    c[i] = 0u;
+  }
 }
 
 // main.rs:261-273 + camera.rs + integrator.rs:63-74 state
-__global__ __launch_bounds__(kThreads) void k_generate(Paths P, PassParams pp, CamDev cam) {
+__global__ __launch_bounds__(kThreads) void k_generate(Paths P, const PassParams* ppd, CamDev cam) {
+  const PassParams pp = *ppd;
   const uint32_t stride = gridDim.x * kThreads;
   for (uint32_t p = blockIdx.x * kThreads + threadIdx.x; p < pp.paths; p += stride) {
-    uint32_t pixel = p % pp.npix, sl = p / pp.npix;
+    uint32_t pixel = (uint32_t)(((uint64_t)p * pp.pix_mul + pp.pix_add) % pp.npix);
+    uint32_t sl = p / pp.npix;
     uint32_t x = pixel % pp.width, y = pixel / pp.width;
     Rng rng = rng_seed(pixel_seed(x, y, pp.s_base + sl));
     float jx = rng_f32(rng);
@@ -132,56 +168,102 @@ __global__ __launch_bounds__(kThreads) void k_generate(Paths P, PassParams pp, C
     P.lum[p] = make_float4(0.0f, 0.0f, 0.0f, __builtin_bit_cast(float, FL_SPECULAR));
     P.aov0[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     P.aov1[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (pp.mode == MODE_TRAIN) P.vcount[p] = 0;
     P.q_cur[p] = p;
   }
 }
 
-constexpr int kRing = 8;
-
-__global__ __launch_bounds__(kThreads) void k_extend(Paths P, SceneDev S, int32_t* gstack) {
-  __shared__ int32_t ring[kRing * kThreads];
-  const uint32_t n = P.counters[0];
+// Closest hit for every queued path, lanes refilled from the queue as their
+// rays finish.
+__global__ __launch_bounds__(kThreads) void k_extend(Paths P, SceneDev S, int2* gstack) {
+  __shared__ int2 ring[kRing * kThreads];
   const uint32_t tid = blockIdx.x * kThreads + threadIdx.x;
-  const uint32_t stride = gridDim.x * kThreads;
-  HybridStack<kRing> st{ring + threadIdx.x, kThreads, gstack + tid, stride, 0, 0};
-  for (uint32_t i = tid; i < n; i += stride) {
-    uint32_t p = P.q_cur[i];
-    float4 a = P.ray[2 * (size_t)p], b = P.ray[2 * (size_t)p + 1];
-    TRay r = make_tray(xyz(a), xyz(b), a.w, b.w);
-    THit h;
-    traverse4<false, kRing>(S.nodes4, S.tris, r, st, &h);
-    P.hit[p] = make_float4(__builtin_bit_cast(float, h.gid), h.t, h.u, h.v);
+  HybridStack<kRing> st{ring + threadIdx.x, kThreads, gstack + tid, gridDim.x * kThreads, 0, 0};
+  WaveFeed feed(P.counters[0], P.counters + 8);
+  Trav4<false, kRing> tv;
+  uint32_t p = 0, item = 0;
+  bool active = false;
+  for (;;) {
+    const uint64_t idle = __ballot(!active);
+    if (__popcll(idle) >= kRefillLanes) {
+      if (feed.take(idle, !active, item)) {
+        p = P.q_cur[item];
+        const float4 a = P.ray[2 * (size_t)p], b = P.ray[2 * (size_t)p + 1];
+        tv.start(make_tray(xyz(a), xyz(b), a.w, b.w), st);
+        active = true;
+      }
+      if (!__any(active)) break;
+    }
+    if (active && tv.step(S.nodes4, S.tris, st)) {
+      THit h;
+      tv.result(&h);
+      P.hit[p] = make_float4(__builtin_bit_cast(float, h.gid), h.t, h.u, h.v);
+      active = false;
+    }
   }
 }
 
-__global__ __launch_bounds__(kThreads) void k_shadow(Paths P, SceneDev S, int32_t* gstack) {
-  __shared__ int32_t ring[kRing * kThreads];
-  const uint32_t n = P.counters[2];
+// Any-hit for every shadow slot of the queued paths; a lane keeps a path
+// until all its slots are traced so unoccluded contributions are added to the
+// path radiance in slot order (lights, then environment).
+__global__ __launch_bounds__(kThreads) void k_shadow(Paths P, SceneDev S, int2* gstack) {
+  __shared__ int2 ring[kRing * kThreads];
   const uint32_t tid = blockIdx.x * kThreads + threadIdx.x;
-  const uint32_t stride = gridDim.x * kThreads;
-  HybridStack<kRing> st{ring + threadIdx.x, kThreads, gstack + tid, stride, 0, 0};
-  for (uint32_t i = tid; i < n; i += stride) {
-    uint32_t p = P.q_shadow[i];
-    uint32_t cnt = P.scount[p];
-    float4 L = P.lum[p];
-    for (uint32_t k = 0; k < cnt; ++k) {
-      size_t slot = (size_t)p * P.slots + k;
-      float4 a = P.sray[2 * slot], b = P.sray[2 * slot + 1];
-      bool occ = false;
-      if (b.w > a.w) {  // empty range (distance <= 0): nothing can occlude (scene.rs:123-125)
-        TRay r = make_tray(xyz(a), xyz(b), a.w, b.w);
-        THit h;
-        traverse4<true, kRing>(S.nodes4, S.tris, r, st, &h);
-        occ = h.gid != PYRE_MISS;
+  HybridStack<kRing> st{ring + threadIdx.x, kThreads, gstack + tid, gridDim.x * kThreads, 0, 0};
+  WaveFeed feed(P.counters[2], P.counters + 16);
+  Trav4<true, kRing> tv;
+  tv.cur = -1;  // -1: no traversal in progress for the current slot
+  uint32_t p = 0, k = 0, cnt = 0, item = 0;
+  float4 L = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  bool active = false;
+  for (;;) {
+    const uint64_t idle = __ballot(!active);
+    if (__popcll(idle) >= kRefillLanes) {
+      if (feed.take(idle, !active, item)) {
+        p = P.q_shadow[item];
+        cnt = P.scount[p];
+        k = 0;
+        L = P.lum[p];
+        tv.cur = -1;
+        active = true;
       }
-      if (!occ) {
-        float4 c = P.scontrib[slot];
-        L.x = L.x + c.x;
-        L.y = L.y + c.y;
-        L.z = L.z + c.z;
+      if (!__any(active)) break;
+    }
+    if (!active) continue;
+    bool occ = false, traced = false;
+    if (k < cnt) {
+      // a slot whose ray has an empty range (distance <= 0) cannot be
+      // occluded (scene.rs:123-125); others start a traversal on first visit
+      const size_t slot = (size_t)p * P.slots + k;
+      if (tv.cur < 0) {
+        const float4 a = P.sray[2 * slot], b = P.sray[2 * slot + 1];
+        if (b.w > a.w) tv.start(make_tray(xyz(a), xyz(b), a.w, b.w), st);
+      }
+      if (tv.cur >= 0) {
+        if (tv.step(S.nodes4, S.tris, st)) {
+          THit h;
+          tv.result(&h);
+          occ = h.gid != PYRE_MISS;
+          traced = true;
+          tv.cur = -1;
+        }
+      } else {
+        traced = true;
+      }
+      if (traced) {
+        if (!occ) {
+          const float4 cc = P.scontrib[slot];
+          L.x = L.x + cc.x;
+          L.y = L.y + cc.y;
+          L.z = L.z + cc.z;
+        }
+        ++k;
       }
     }
-    P.lum[p] = L;
+    if (k >= cnt) {
+      P.lum[p] = L;
+      active = false;
+    }
   }
 }
 
@@ -210,13 +292,14 @@ __device__ __forceinline__ uint32_t mesh_of(const SceneDev& S, uint32_t gid) {
   return lo;
 }
 
-// One bounce of integrator.rs:76-222 for path p. Returns whether the path
-// continues (next ray written) and whether it queued shadow rays.
+struct ShadeOut {
+  bool cont, shadow, query;
+  uint32_t n_shadow;
+};
+
+// One bounce of integrator.rs:76-222 for path p.
 __device__ void shade_path(const Paths& P, const SceneDev& S, const PassParams& pp, uint32_t depth, uint32_t p,
-                           bool* cont, bool* shadow, uint32_t* n_shadow) {
-  *cont = false;
-  *shadow = false;
-  *n_shadow = 0;
+                           ShadeOut& o) {
   float4 ra = P.ray[2 * (size_t)p], rb = P.ray[2 * (size_t)p + 1];
   V3 ro = xyz(ra), rd = xyz(rb);
   float r_tmin = ra.w;
@@ -300,6 +383,29 @@ __device__ void shade_path(const Paths& P, const SceneDev& S, const PassParams& 
     P.lum[p] = f4(l, __builtin_bit_cast(float, flags));
     return;
   }
+  if (pp.mode != MODE_PLAIN) {
+    const float rough = mat.kind == PYRE_MAT_LAMBERTIAN ? 1.0f : mat.roughness;
+    const float qf[13] = {pos.x, pos.y, pos.z, -rd.x, -rd.y, -rd.z, ns.x, ns.y, ns.z, rough,
+                          mat.base.x, mat.base.y, mat.base.z};
+    if (pp.mode == MODE_CACHE && depth >= 1 && (flags & FL_DIFFUSE)) {
+      for (int k = 0; k < 13; ++k) P.qfeat[13 * (size_t)p + k] = qf[k];
+      o.query = true;
+      P.lum[p] = f4(l, __builtin_bit_cast(float, flags));
+      return;
+    }
+    if (pp.mode == MODE_TRAIN) {
+      uint32_t k = P.vcount[p];
+      if (k < pp.vmax) {
+        float4* v = P.vrec + ((size_t)p * pp.vmax + k) * kVrec;
+        v[0] = make_float4(qf[0], qf[1], qf[2], qf[3]);
+        v[1] = make_float4(qf[4], qf[5], qf[6], qf[7]);
+        v[2] = make_float4(qf[8], qf[9], qf[10], qf[11]);
+        v[3] = make_float4(qf[12], l.x, l.y, l.z);
+        v[4] = make_float4(beta.x, beta.y, beta.z, 0.0f);
+        P.vcount[p] = k + 1;
+      }
+    }
+  }
   Rng rng = load_rng(P, p);
   uint32_t ns_count = 0;
   const size_t slot0 = (size_t)p * P.slots;
@@ -354,8 +460,8 @@ __device__ void shade_path(const Paths& P, const SceneDev& S, const PassParams& 
     }
   }
   P.scount[p] = ns_count;
-  *shadow = ns_count > 0;
-  *n_shadow = ns_count;
+  o.shadow = ns_count > 0;
+  o.n_shadow = ns_count;
   // BSDF sample (integrator.rs:190-201)
   bool alive = true;
   V3 bwi, bf;
@@ -375,51 +481,58 @@ __device__ void shade_path(const Paths& P, const SceneDev& S, const PassParams& 
     }
   }
   if (alive) {
-    flags &= ~FL_SPECULAR;
+    flags &= ~(FL_SPECULAR | FL_DIFFUSE);
+    if (mat.kind == PYRE_MAT_LAMBERTIAN || mat.roughness >= pp.diffuse_rough) flags |= FL_DIFFUSE;
     V3 wi_world = to_world(frame, bwi);
     P.ray[2 * (size_t)p] = f4(pos + kRayEps * ns, 1e-4f);
     P.ray[2 * (size_t)p + 1] = f4(wi_world, kInf);
     P.beta_pdf[p] = f4(beta, bpdf);
-    *cont = depth + 1 < pp.max_depth;
+    o.cont = depth + 1 < pp.max_depth;
   }
   store_rng(P, p, rng);
   P.lum[p] = f4(l, __builtin_bit_cast(float, flags));
 }
 
-__global__ __launch_bounds__(kThreads) void k_shade(Paths P, SceneDev S, PassParams pp, uint32_t depth) {
+__global__ __launch_bounds__(kThreads) void k_shade(Paths P, SceneDev S, const PassParams* ppd, uint32_t depth) {
+  const PassParams pp = *ppd;
   const uint32_t n = P.counters[0];
   const uint32_t stride = gridDim.x * kThreads;
   // uniform trip count per wave so the ballots below see converged lanes
   const uint32_t base = blockIdx.x * kThreads + (threadIdx.x & ~63u);
   for (uint32_t i0 = base; i0 < n; i0 += stride) {
     uint32_t i = i0 + (threadIdx.x & 63u);
-    bool cont = false, shadow = false;
-    uint32_t p = 0, nsh = 0;
+    ShadeOut o;
+    o.cont = o.shadow = o.query = false;
+    o.n_shadow = 0;
+    uint32_t p = 0;
     if (i < n) {
       p = P.q_cur[i];
-      shade_path(P, S, pp, depth, p, &cont, &shadow, &nsh);
+      shade_path(P, S, pp, depth, p, o);
     }
-    for (int o = 32; o > 0; o >>= 1) nsh += __shfl_xor(nsh, o);
+    uint32_t nsh = o.n_shadow;
+    for (int k = 32; k > 0; k >>= 1) nsh += __shfl_xor(nsh, k);
     if ((threadIdx.x & 63u) == 0 && nsh) atomicAdd(&P.stats[1], (unsigned long long)nsh);
-    uint32_t qn = wave_append(&P.counters[1], cont);
-    if (cont) P.q_next[qn] = p;
-    uint32_t qs = wave_append(&P.counters[2], shadow);
-    if (shadow) P.q_shadow[qs] = p;
+    uint32_t qn = wave_append(&P.counters[1], o.cont);
+    if (o.cont) P.q_next[qn] = p;
+    uint32_t qs = wave_append(&P.counters[2], o.shadow);
+    if (o.shadow) P.q_shadow[qs] = p;
+    if (pp.mode == MODE_CACHE) {
+      uint32_t qq = wave_append(&P.counters[3], o.query);
+      if (o.query) {
+        P.qpath[qq] = p;
+      }
+    }
   }
 }
 
-// main.rs:275-285 in sample order; optional FP64 radiance sums.
-__global__ __launch_bounds__(kThreads) void k_accumulate(Paths P, PassParams pp, float* acc, double* acc64) {
+// main.rs:275-285 in sample order; FP64 radiance sums beside the f32 ones.
+__global__ __launch_bounds__(kThreads) void k_accumulate(Paths P, const PassParams* ppd, float* acc, double* acc64) {
+  const PassParams pp = *ppd;
   const uint32_t stride = gridDim.x * kThreads;
   for (uint32_t px = blockIdx.x * kThreads + threadIdx.x; px < pp.npix; px += stride) {
     float a[10];
     for (int k = 0; k < 10; ++k) a[k] = acc[10 * (size_t)px + k];
-    double d0 = 0, d1 = 0, d2 = 0;
-    if (acc64) {
-      d0 = acc64[3 * (size_t)px];
-      d1 = acc64[3 * (size_t)px + 1];
-      d2 = acc64[3 * (size_t)px + 2];
-    }
+    double d0 = acc64[3 * (size_t)px], d1 = acc64[3 * (size_t)px + 1], d2 = acc64[3 * (size_t)px + 2];
     for (uint32_t sl = 0; sl < pp.n_samples; ++sl) {
       uint32_t p = sl * pp.npix + px;
       float4 L = P.lum[p], A = P.aov0[p], N = P.aov1[p];
@@ -438,11 +551,61 @@ __global__ __launch_bounds__(kThreads) void k_accumulate(Paths P, PassParams pp,
       d2 += (double)L.z;
     }
     for (int k = 0; k < 10; ++k) acc[10 * (size_t)px + k] = a[k];
-    if (acc64) {
-      acc64[3 * (size_t)px] = d0;
-      acc64[3 * (size_t)px + 1] = d1;
-      acc64[3 * (size_t)px + 2] = d2;
+    acc64[3 * (size_t)px] = d0;
+    acc64[3 * (size_t)px + 1] = d1;
+    acc64[3 * (size_t)px + 2] = d2;
+  }
+}
+
+// TRAIN: per recorded vertex, target = (L_end - L_before) / beta, compacted
+// into (feats, targets) up to cap samples.
+__global__ __launch_bounds__(kThreads) void k_train_targets(Paths P, const PassParams* ppd, float* feats,
+                                                            float* targets, uint32_t cap) {
+  const PassParams pp = *ppd;
+  const uint32_t total = pp.paths * pp.vmax;
+  const uint32_t stride = gridDim.x * kThreads;
+  const uint32_t base = blockIdx.x * kThreads + (threadIdx.x & ~63u);
+  for (uint32_t i0 = base; i0 < total; i0 += stride) {
+    uint32_t i = i0 + (threadIdx.x & 63u);
+    bool ok = false;
+    float f[13], t[3];
+    if (i < total) {
+      uint32_t p = i / pp.vmax, k = i % pp.vmax;
+      if (k < P.vcount[p]) {
+        const float4* v = P.vrec + (size_t)i * kVrec;
+        float4 v0 = v[0], v1 = v[1], v2 = v[2], v3r = v[3], v4 = v[4];
+        const float tmp[13] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w, v2.x, v2.y, v2.z, v2.w, v3r.x};
+        for (int q = 0; q < 13; ++q) f[q] = tmp[q];
+        float4 L = P.lum[p];
+        const float lb[3] = {v3r.y, v3r.z, v3r.w}, le[3] = {L.x, L.y, L.z}, bb[3] = {v4.x, v4.y, v4.z};
+        ok = true;
+        for (int c = 0; c < 3; ++c) {
+          if (!(bb[c] > 1e-6f)) ok = false;
+          float tv = (le[c] - lb[c]) / bb[c];
+          if (!isfinite(tv)) ok = false;
+          t[c] = isfinite(tv) ? fmaxf(tv, 0.0f) : 0.0f;
+        }
+      }
     }
+    uint32_t slot = wave_append(&P.counters[4], ok);
+    if (ok && slot < cap) {
+      for (int q = 0; q < 13; ++q) feats[13 * (size_t)slot + q] = f[q];
+      for (int c = 0; c < 3; ++c) targets[3 * (size_t)slot + c] = t[c];
+    }
+  }
+}
+
+// CACHE: L += beta x max(cache, 0) for every path that ended in the cache.
+__global__ __launch_bounds__(kThreads) void k_cache_apply(Paths P) {
+  const uint32_t n = P.counters[3];
+  const uint32_t stride = gridDim.x * kThreads;
+  for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) {
+    uint32_t p = P.qpath[i];
+    float4 L = P.lum[p], b = P.beta_pdf[p];
+    L.x = L.x + b.x * fmaxf(P.qrgb[3 * (size_t)i], 0.0f);
+    L.y = L.y + b.y * fmaxf(P.qrgb[3 * (size_t)i + 1], 0.0f);
+    L.z = L.z + b.z * fmaxf(P.qrgb[3 * (size_t)i + 2], 0.0f);
+    P.lum[p] = L;
   }
 }
 
@@ -451,105 +614,257 @@ __global__ void k_scale(const float* acc, size_t n, float inv_spp, float* out) {
   if (i < n) out[i] = acc[i] * inv_spp;
 }
 
-__global__ void k_zero_f32(float* a, size_t n) {
+__global__ void k_fill_f32(float* a, size_t n, float v) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) a[i] = 0.0f;
+  if (i < n) a[i] = v;
+}
+__global__ void k_fill_f64(double* a, size_t n, double v) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+__global__ void k_fill_u64(unsigned long long* a, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0ull;
 }
 
-template <class T>
-struct Buf {
-  T* p = nullptr;
-  explicit Buf(size_t n) { p = dev_alloc<T>(n); }
-  ~Buf() { dev_free(p); }
+inline unsigned nblocks(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
+
+// Device path state for up to `cap` paths.
+struct Work {
+  size_t cap = 0;
+  uint32_t slots = 1, vmax = 0, tcap = 0;
+  Paths P{};
+  PassParams* ppd = nullptr;
+  std::vector<void*> bufs;
+  float* tfeat = nullptr;
+  float* ttarget = nullptr;
+  template <class T>
+  T* get(size_t n) {
+    T* p = dev_alloc<T>(n);
+    bufs.push_back(p);
+    return p;
+  }
+  void alloc(size_t cap_, uint32_t slots_, uint32_t vmax_, uint32_t tcap_, bool cache) {
+    cap = cap_;
+    slots = slots_;
+    vmax = vmax_;
+    tcap = tcap_;
+    P.ray = get<float4>(2 * cap);
+    P.rng = get<ulonglong2>(2 * cap);
+    P.beta_pdf = get<float4>(cap);
+    P.lum = get<float4>(cap);
+    P.aov0 = get<float4>(cap);
+    P.aov1 = get<float4>(cap);
+    P.hit = get<float4>(cap);
+    P.sray = get<float4>(2 * cap * slots);
+    P.scontrib = get<float4>(cap * slots);
+    P.scount = get<uint32_t>(cap);
+    P.q_cur = get<uint32_t>(cap);
+    P.q_next = get<uint32_t>(cap);
+    P.q_shadow = get<uint32_t>(cap);
+    P.counters = get<uint32_t>(kCounters);
+    P.stats = get<unsigned long long>(2);
+    P.slots = slots;
+    if (vmax) {
+      P.vrec = get<float4>(cap * vmax * kVrec);
+      P.vcount = get<uint32_t>(cap);
+      tfeat = get<float>(13 * (size_t)tcap);
+      ttarget = get<float>(3 * (size_t)tcap);
+    }
+    if (cache) {
+      P.qfeat = get<float>(13 * cap);
+      P.qpath = get<uint32_t>(cap);
+      P.qrgb = get<float>(3 * cap);
+    }
+    ppd = get<PassParams>(1);
+  }
+  ~Work() {
+    for (void* p : bufs) (void)hipFree(p);
+  }
 };
 
-void trace_stage(pyre_scene* s, bool shadow, hipStream_t st, const Paths& P, const SceneDev& S) {
-  const unsigned grid = trace_grid(s);
-  if (!shadow)
-    hipLaunchKernelGGL(k_extend, dim3(grid), dim3(kThreads), 0, st, P, S, trace_gstack(s));
-  else
-    hipLaunchKernelGGL(k_shadow, dim3(grid), dim3(kThreads), 0, st, P, S, trace_gstack(s));
+SceneDev scene_dev(pyre_scene* s) {
+  return SceneDev{s->d_nodes4, s->d_tris, s->d_verts, s->d_normals, s->d_tri_offset, s->d_mesh_info, s->d_mats,
+                  s->d_lights, s->n_meshes, s->n_lights, s->has_env ? 1u : 0u, s->env};
+}
+
+unsigned stage_grid(const pyre_ctx* ctx) { return (unsigned)std::max(1, ctx->cus) * 8; }
+
+// All stages of one pass on the stream; the pass parameters are already in
+// W.ppd. acc == nullptr skips the accumulation.
+void launch_pass(pyre_ctx* ctx, pyre_scene* s, const Work& W, const CamDev& cam, uint32_t max_depth, float* acc,
+                 double* acc64, hipStream_t st) {
+  const unsigned grid = stage_grid(ctx);
+  const unsigned tgrid = trace_grid(s);
+  int2* gst = trace_gstack(s);
+  const SceneDev S = scene_dev(s);
+  const PassParams* ppd = W.ppd;
+  Paths P = W.P;
+  hipLaunchKernelGGL(k_counters, dim3(1), dim3(kCounters), 0, st, P.counters, ppd);
+  hipLaunchKernelGGL(k_generate, dim3(grid), dim3(kThreads), 0, st, P, ppd, cam);
+  const bool nee = s->n_lights + (s->has_env ? 1u : 0u) > 0;
+  for (uint32_t d = 0; d < max_depth; ++d) {
+    hipLaunchKernelGGL(k_extend, dim3(tgrid), dim3(kThreads), 0, st, P, S, gst);
+    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kThreads), 0, st, P, S, ppd, d);
+    if (nee) hipLaunchKernelGGL(k_shadow, dim3(tgrid), dim3(kThreads), 0, st, P, S, gst);
+    hipLaunchKernelGGL(k_rotate, dim3(1), dim3(kCounters), 0, st, P.counters, P.stats);
+    std::swap(P.q_cur, P.q_next);
+  }
+  if (acc) hipLaunchKernelGGL(k_accumulate, dim3(grid), dim3(kThreads), 0, st, P, ppd, acc, acc64);
+}
+
+bool graphs_enabled() {
+  const char* e = std::getenv("PYRE_GRAPH");
+  return !(e && std::strcmp(e, "0") == 0);
 }
 
 }  // namespace
 
-extern "C" int pyre_render_stats(pyre_ctx* ctx, pyre_scene* s, uint64_t* extension_rays, uint64_t* shadow_rays) {
-  if (!ctx || !s) return PYRE_EINVAL;
-  if (extension_rays) *extension_rays = s->last_extension_rays;
-  if (shadow_rays) *shadow_rays = s->last_shadow_rays;
-  return PYRE_OK;
+struct pyre_session {
+  pyre_ctx* ctx = nullptr;
+  pyre_scene* s = nullptr;
+  CamDev cam{};
+  pyre_integrator_params ip{};
+  uint32_t width = 0, height = 0, npix = 0, per_pass = 1;
+  Work W;
+  float* acc = nullptr;
+  double* acc64 = nullptr;
+  uint32_t spp_done = 0, next_sample = 0, train_frames = 0;
+  bool use_graph = true;
+  hipGraphExec_t exec = nullptr;
+  ~pyre_session() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    dev_free(acc);
+    dev_free(acc64);
+  }
+};
+
+namespace {
+
+PassParams pass_params(const pyre_session* S, uint32_t s_base, uint32_t n_samples, uint32_t mode, float diffuse_rough) {
+  PassParams pp{};
+  pp.width = S->width;
+  pp.height = S->height;
+  pp.npix = S->npix;
+  pp.s_base = s_base;
+  pp.n_samples = n_samples;
+  pp.paths = n_samples * S->npix;
+  pp.max_depth = S->ip.max_depth;
+  pp.min_rr = S->ip.min_rr_depth;
+  pp.mode = mode;
+  pp.pix_mul = 1;
+  pp.pix_add = 0;
+  pp.vmax = S->W.vmax;
+  pp.diffuse_rough = diffuse_rough;
+  return pp;
 }
 
-extern "C" int pyre_render(pyre_ctx* ctx, pyre_scene* s, const pyre_camera_desc* camera,
-                           const pyre_integrator_params* integrator, uint32_t width, uint32_t height, uint32_t spp,
-                           uint32_t spp_offset, pyre_pixel_sample* out, double* accum64) {
-  if (!ctx || !s || !camera || !integrator) return PYRE_EINVAL;
+// Samples [s_base, s_base + n_samples) of every pixel. PLAIN passes of the
+// full pass size replay one captured graph; the rest launch eagerly.
+void session_pass(pyre_session* S, uint32_t s_base, uint32_t n_samples, uint32_t mode, pyre_nrc* nrc,
+                  float diffuse_rough, bool accumulate) {
+  hipStream_t st = S->ctx->stream;
+  hipLaunchKernelGGL(k_set_pass, dim3(1), dim3(1), 0, st, S->W.ppd,
+                     pass_params(S, s_base, n_samples, mode, diffuse_rough));
+  if (mode == MODE_PLAIN && accumulate && n_samples == S->per_pass && S->use_graph) {
+    if (!S->exec) {
+      trace_gstack(S->s);  // any allocation happens before the capture
+      hipGraph_t g = nullptr;
+      PYRE_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      launch_pass(S->ctx, S->s, S->W, S->cam, S->ip.max_depth, S->acc, S->acc64, st);
+      PYRE_HIP_CHECK(hipStreamEndCapture(st, &g));
+      hipError_t e = hipGraphInstantiate(&S->exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      PYRE_HIP_CHECK(e);
+    }
+    PYRE_HIP_CHECK(hipGraphLaunch(S->exec, st));
+    return;
+  }
+  if (mode == MODE_CACHE) {
+    require(nrc != nullptr, "cache pass without a radiance cache");
+    launch_pass(S->ctx, S->s, S->W, S->cam, S->ip.max_depth, nullptr, nullptr, st);
+    nrc_infer_dev_count(nrc, S->W.P.qfeat, S->W.P.qpath, S->W.P.counters + 3, (uint32_t)S->W.cap, S->W.P.qrgb, st);
+    hipLaunchKernelGGL(k_cache_apply, dim3(stage_grid(S->ctx)), dim3(kThreads), 0, st, S->W.P);
+    if (accumulate)
+      hipLaunchKernelGGL(k_accumulate, dim3(stage_grid(S->ctx)), dim3(kThreads), 0, st, S->W.P,
+                         (const PassParams*)S->W.ppd, S->acc, S->acc64);
+    return;
+  }
+  launch_pass(S->ctx, S->s, S->W, S->cam, S->ip.max_depth, accumulate ? S->acc : nullptr, S->acc64, st);
+}
+
+void clear_sums(pyre_session* S) {
+  hipStream_t st = S->ctx->stream;
+  const size_t n10 = 10 * (size_t)S->npix, n3 = 3 * (size_t)S->npix;
+  hipLaunchKernelGGL(k_fill_f32, dim3(nblocks(n10, 256)), dim3(256), 0, st, S->acc, n10, 0.0f);
+  hipLaunchKernelGGL(k_fill_f64, dim3(nblocks(n3, 256)), dim3(256), 0, st, S->acc64, n3, 0.0);
+  S->spp_done = 0;
+}
+
+void session_init(pyre_session* S, pyre_ctx* ctx, pyre_scene* s, const pyre_camera_desc& cam,
+                  const pyre_integrator_params& ip, uint32_t w, uint32_t h, uint32_t train_paths) {
+  require(w > 0 && h > 0, "width and height must be positive");
+  require(s->built, "render before pyre_bvh_build");
+  require((uint64_t)w * h < (1ull << 28), "image too large");
+  require(train_paths < (1u << 24), "too many training paths");
+  S->ctx = ctx;
+  S->s = s;
+  S->cam = make_camera(cam);
+  S->ip = ip;
+  S->width = w;
+  S->height = h;
+  S->npix = w * h;
+  // paths per pass: whole samples of the image, about 2M paths (PYRE_PASS_PATHS overrides)
+  uint32_t target = 1u << 21;
+  if (const char* e = std::getenv("PYRE_PASS_PATHS")) target = (uint32_t)std::max(1, std::atoi(e));
+  S->per_pass = std::max(1u, target / S->npix);
+  const size_t cap = std::max((size_t)S->per_pass * S->npix, (size_t)train_paths);
+  require(cap < (1ull << 31), "path pool too large");
+  const uint32_t slots = std::max(1u, s->n_lights + (s->has_env ? 1u : 0u));
+  const uint32_t vmax = train_paths ? std::max(1u, ip.max_depth) : 0;
+  S->W.alloc(cap, slots, vmax, train_paths ? std::min<uint32_t>(train_paths * vmax, 1u << 20) : 0,
+             train_paths > 0);
+  S->acc = dev_alloc<float>(10 * (size_t)S->npix);
+  S->acc64 = dev_alloc<double>(3 * (size_t)S->npix);
+  clear_sums(S);
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, S->W.P.stats, (size_t)2);
+  S->use_graph = graphs_enabled();
+}
+
+void session_render(pyre_session* S, uint32_t spp) {
+  for (uint32_t s0 = 0; s0 < spp; s0 += S->per_pass) {
+    uint32_t ns = std::min(S->per_pass, spp - s0);
+    session_pass(S, S->next_sample, ns, MODE_PLAIN, nullptr, 0.25f, true);
+    S->next_sample += ns;
+    S->spp_done += ns;
+  }
+  PYRE_HIP_CHECK(hipGetLastError());
+}
+
+void finish_stats(pyre_session* S) {
+  unsigned long long hs[2] = {0, 0};
+  PYRE_HIP_CHECK(hipMemcpyAsync(hs, S->W.P.stats, 16, hipMemcpyDeviceToHost, S->ctx->stream));
+  PYRE_HIP_CHECK(hipStreamSynchronize(S->ctx->stream));
+  S->s->last_extension_rays = hs[0];
+  S->s->last_shadow_rays = hs[1];
+}
+
+void read_mean(pyre_session* S, pyre_pixel_sample* out, float inv_spp) {
+  hipStream_t st = S->ctx->stream;
+  const size_t n10 = 10 * (size_t)S->npix;
+  float* o = dev_alloc<float>(n10);
+  hipLaunchKernelGGL(k_scale, dim3(nblocks(n10, 256)), dim3(256), 0, st, S->acc, n10, inv_spp, o);
+  hipError_t e = hipMemcpyAsync(out, o, n10 * sizeof(float), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  dev_free(o);
+  PYRE_HIP_CHECK(e);
+}
+
+template <class F>
+int sguard(pyre_ctx* ctx, F&& f) {
   try {
     PYRE_HIP_CHECK(hipSetDevice(ctx->device));
-    require(width > 0 && height > 0, "width and height must be positive");
-    require(s->built, "pyre_render before pyre_bvh_build");
-    require((uint64_t)width * height < (1ull << 31), "image too large");
-    hipStream_t st = ctx->stream;
-    const uint32_t npix = width * height;
-    const CamDev cam = make_camera(*camera);
-    SceneDev S{s->d_nodes4, s->d_tris, s->d_verts, s->d_normals, s->d_tri_offset, s->d_mesh_info, s->d_mats,
-               s->d_lights, s->n_meshes, s->n_lights, s->has_env ? 1u : 0u, s->env};
-    const uint32_t slots = std::max(1u, s->n_lights + (s->has_env ? 1u : 0u));
-    // paths per pass: whole samples of the image, about 2M paths
-    const uint32_t target = 1u << 21;
-    uint32_t per_pass = std::max(1u, std::min(spp ? spp : 1u, (target + npix - 1) / npix));
-    const size_t cap = (size_t)per_pass * npix;
-    Buf<float4> ray(2 * cap), bp(cap), lum(cap), aov0(cap), aov1(cap), hit(cap), sray(2 * cap * slots),
-        scon(cap * slots);
-    Buf<ulonglong2> rng(2 * cap);
-    Buf<uint32_t> scount(cap), q0(cap), q1(cap), qs(cap), ctr(4);
-    Buf<unsigned long long> stats(2);
-    PYRE_HIP_CHECK(hipMemsetAsync(stats.p, 0, 16, st));
-    Buf<float> acc(10 * (size_t)npix), outd(10 * (size_t)npix);
-    Buf<double> acc64(accum64 ? 3 * (size_t)npix : 1);
-    hipLaunchKernelGGL(k_zero_f32, dim3((unsigned)((10 * (size_t)npix + 255) / 256)), dim3(256), 0, st, acc.p,
-                       10 * (size_t)npix);
-    if (accum64)
-      PYRE_HIP_CHECK(hipMemcpyAsync(acc64.p, accum64, 3 * sizeof(double) * npix, hipMemcpyHostToDevice, st));
-    PYRE_HIP_CHECK(hipMemsetAsync(s->d_flag, 0, 4, st));
-    Paths P{ray.p, rng.p, bp.p, lum.p, aov0.p, aov1.p, hit.p, sray.p, scon.p, scount.p, q0.p, q1.p, qs.p, ctr.p,
-            stats.p, slots};
-    const unsigned grid = (unsigned)std::max(1, ctx->cus) * 8;
-    for (uint32_t s0 = 0; s0 < spp; s0 += per_pass) {
-      uint32_t ns = std::min(per_pass, spp - s0);
-      PassParams pp{width, height, npix, spp_offset + s0, ns, ns * npix, integrator->max_depth,
-                    integrator->min_rr_depth};
-      P.q_cur = q0.p;
-      P.q_next = q1.p;
-      hipLaunchKernelGGL(k_counters, dim3(1), dim3(1), 0, st, ctr.p, pp.paths, 0u, 0u);
-      hipLaunchKernelGGL(k_generate, dim3(grid), dim3(kThreads), 0, st, P, pp, cam);
-      for (uint32_t d = 0; d < integrator->max_depth; ++d) {
-        trace_stage(s, false, st, P, S);
-        hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kThreads), 0, st, P, S, pp, d);
-        if (s->n_lights + (s->has_env ? 1u : 0u) > 0) trace_stage(s, true, st, P, S);
-        hipLaunchKernelGGL(k_rotate, dim3(1), dim3(1), 0, st, ctr.p, stats.p);
-        std::swap(P.q_cur, P.q_next);
-      }
-      hipLaunchKernelGGL(k_accumulate, dim3(grid), dim3(kThreads), 0, st, P, pp, acc.p,
-                         accum64 ? acc64.p : (double*)nullptr);
-      PYRE_HIP_CHECK(hipGetLastError());
-    }
-    if (out) {
-      const float inv_spp = 1.0f / (float)spp;  // main.rs:255
-      hipLaunchKernelGGL(k_scale, dim3((unsigned)((10 * (size_t)npix + 255) / 256)), dim3(256), 0, st, acc.p,
-                         10 * (size_t)npix, inv_spp, outd.p);
-      PYRE_HIP_CHECK(hipMemcpyAsync(out, outd.p, 10 * sizeof(float) * npix, hipMemcpyDeviceToHost, st));
-    }
-    if (accum64)
-      PYRE_HIP_CHECK(hipMemcpyAsync(accum64, acc64.p, 3 * sizeof(double) * npix, hipMemcpyDeviceToHost, st));
-    uint32_t ovf = 0;
-    unsigned long long hs[2] = {0, 0};
-    PYRE_HIP_CHECK(hipMemcpyAsync(&ovf, s->d_flag, 4, hipMemcpyDeviceToHost, st));
-    PYRE_HIP_CHECK(hipMemcpyAsync(hs, stats.p, 16, hipMemcpyDeviceToHost, st));
-    PYRE_HIP_CHECK(hipStreamSynchronize(st));
-    if (ovf) throw Error(PYRE_EOVERFLOW, "traversal stack overflow");
-    s->last_extension_rays = hs[0];
-    s->last_shadow_rays = hs[1];
+    f();
     ctx->err.clear();
     return PYRE_OK;
   } catch (const Error& e) {
@@ -560,3 +875,136 @@ extern "C" int pyre_render(pyre_ctx* ctx, pyre_scene* s, const pyre_camera_desc*
     return PYRE_EINVAL;
   }
 }
+
+}  // namespace
+
+extern "C" {
+
+int pyre_render_stats(pyre_ctx* ctx, pyre_scene* s, uint64_t* extension_rays, uint64_t* shadow_rays) {
+  if (!ctx || !s) return PYRE_EINVAL;
+  if (extension_rays) *extension_rays = s->last_extension_rays;
+  if (shadow_rays) *shadow_rays = s->last_shadow_rays;
+  return PYRE_OK;
+}
+
+int pyre_render(pyre_ctx* ctx, pyre_scene* s, const pyre_camera_desc* camera, const pyre_integrator_params* integrator,
+                uint32_t width, uint32_t height, uint32_t spp, uint32_t spp_offset, pyre_pixel_sample* out,
+                double* accum64) {
+  if (!ctx || !s || !camera || !integrator) return PYRE_EINVAL;
+  return sguard(ctx, [&] {
+    pyre_session S;
+    session_init(&S, ctx, s, *camera, *integrator, width, height, 0);
+    hipStream_t st = ctx->stream;
+    const size_t n3 = 3 * (size_t)S.npix;
+    if (accum64) PYRE_HIP_CHECK(hipMemcpyAsync(S.acc64, accum64, n3 * sizeof(double), hipMemcpyHostToDevice, st));
+    S.next_sample = spp_offset;
+    session_render(&S, spp);
+    if (out) read_mean(&S, out, 1.0f / (float)spp);  // main.rs:255
+    if (accum64) PYRE_HIP_CHECK(hipMemcpyAsync(accum64, S.acc64, n3 * sizeof(double), hipMemcpyDeviceToHost, st));
+    finish_stats(&S);
+  });
+}
+
+int pyre_session_create(pyre_ctx* ctx, pyre_scene* s, const pyre_camera_desc* camera,
+                        const pyre_integrator_params* integrator, uint32_t width, uint32_t height,
+                        uint32_t nrc_train_paths, pyre_session** out) {
+  if (!ctx || !s || !camera || !integrator || !out) return PYRE_EINVAL;
+  *out = nullptr;
+  pyre_session* S = new (std::nothrow) pyre_session();
+  if (!S) return PYRE_ENOMEM;
+  int rc = sguard(ctx, [&] {
+    session_init(S, ctx, s, *camera, *integrator, width, height, nrc_train_paths);
+    PYRE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  });
+  if (rc != PYRE_OK) {
+    delete S;
+    return rc;
+  }
+  *out = S;
+  return PYRE_OK;
+}
+
+void pyre_session_destroy(pyre_session* S) {
+  if (!S) return;
+  (void)hipSetDevice(S->ctx->device);
+  (void)hipStreamSynchronize(S->ctx->stream);
+  delete S;
+}
+
+int pyre_session_render(pyre_ctx* ctx, pyre_session* S, uint32_t spp) {
+  if (!ctx || !S) return PYRE_EINVAL;
+  return sguard(ctx, [&] {
+    session_render(S, spp);
+    finish_stats(S);
+  });
+}
+
+int pyre_session_render_nrc(pyre_ctx* ctx, pyre_session* S, pyre_nrc* nrc, const pyre_nrc_render_params* p,
+                            float* losses) {
+  if (!ctx || !S || !nrc || !p) return PYRE_EINVAL;
+  return sguard(ctx, [&] {
+    require(S->W.vmax > 0, "session created without radiance-cache buffers (nrc_train_paths = 0)");
+    require(p->train_paths > 0 && p->train_paths <= S->W.cap, "train_paths exceeds the session's capacity");
+    require(p->train_batch >= 16, "train_batch below 16");
+    hipStream_t st = ctx->stream;
+    for (uint32_t f = 0; f < p->frames; ++f) {
+      // training paths: full unbiased paths from scattered pixels, own streams
+      PassParams pp = pass_params(S, kTrainStreamBase + S->train_frames, 1, MODE_TRAIN, p->diffuse_roughness);
+      pp.paths = p->train_paths;
+      pp.pix_mul = (uint32_t)(2654435761ull % S->npix) | 1u;
+      pp.pix_add = (uint32_t)(pixel_seed(S->train_frames, 7, 11) % S->npix);
+      hipLaunchKernelGGL(k_set_pass, dim3(1), dim3(1), 0, st, S->W.ppd, pp);
+      launch_pass(ctx, S->s, S->W, S->cam, S->ip.max_depth, nullptr, nullptr, st);
+      hipLaunchKernelGGL(k_train_targets, dim3(stage_grid(ctx)), dim3(kThreads), 0, st, S->W.P,
+                         (const PassParams*)S->W.ppd, S->W.tfeat, S->W.ttarget, S->W.tcap);
+      uint32_t nsamp = 0;
+      PYRE_HIP_CHECK(hipMemcpyAsync(&nsamp, S->W.P.counters + 4, 4, hipMemcpyDeviceToHost, st));
+      PYRE_HIP_CHECK(hipStreamSynchronize(st));
+      nsamp = std::min(nsamp, S->W.tcap);
+      double loss = 0.0;
+      uint32_t steps = 0;
+      for (uint32_t k = 0; k < p->train_steps; ++k) {
+        const uint32_t off = k * p->train_batch;
+        if (off + 16 > nsamp) break;
+        const uint32_t nb = std::min(p->train_batch, nsamp - off);
+        loss += nrc_gradients_dev(nrc, S->W.tfeat + 13 * (size_t)off, S->W.ttarget + 3 * (size_t)off, nb, st);
+        nrc_adam_step(nrc, st);
+        ++steps;
+      }
+      if (losses) losses[f] = steps ? (float)(loss / steps) : 0.0f;
+      S->train_frames++;
+      const bool acc = f >= p->warmup_frames;
+      for (uint32_t s0 = 0; s0 < p->spp_per_frame; s0 += S->per_pass) {
+        uint32_t ns = std::min(S->per_pass, p->spp_per_frame - s0);
+        session_pass(S, S->next_sample, ns, MODE_CACHE, nrc, p->diffuse_roughness, acc);
+        S->next_sample += ns;
+        if (acc) S->spp_done += ns;
+      }
+    }
+    finish_stats(S);
+  });
+}
+
+int pyre_session_read(pyre_ctx* ctx, pyre_session* S, pyre_pixel_sample* mean, double* radiance_sum64,
+                      uint32_t* spp_done) {
+  if (!ctx || !S) return PYRE_EINVAL;
+  return sguard(ctx, [&] {
+    hipStream_t st = ctx->stream;
+    if (mean) read_mean(S, mean, S->spp_done ? 1.0f / (float)S->spp_done : 0.0f);
+    if (radiance_sum64)
+      PYRE_HIP_CHECK(hipMemcpyAsync(radiance_sum64, S->acc64, 3 * sizeof(double) * (size_t)S->npix,
+                                    hipMemcpyDeviceToHost, st));
+    PYRE_HIP_CHECK(hipStreamSynchronize(st));
+    if (spp_done) *spp_done = S->spp_done;
+  });
+}
+
+int pyre_session_reset(pyre_ctx* ctx, pyre_session* S) {
+  if (!ctx || !S) return PYRE_EINVAL;
+  return sguard(ctx, [&] {
+    clear_sums(S);
+    PYRE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+}  // extern "C"

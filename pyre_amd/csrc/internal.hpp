@@ -145,8 +145,9 @@ struct pyre_scene {
   uint32_t n_nodes2 = 0;
   pyre::Node4* d_nodes4 = nullptr;
   uint32_t n_nodes4 = 0, depth4 = 0;
-  int32_t* d_gstack = nullptr;  // spill columns of the persistent traversal kernels
+  int2* d_gstack = nullptr;  // spill columns of the persistent traversal kernels
   size_t gstack_cap = 0;
+  uint32_t* d_fetch = nullptr;  // work counters of the batched trace kernels
   float4* d_tris = nullptr;        // 3 float4 per slot: (v0, gid bits), (e1, 0), (e2, 0)
   uint32_t* d_prim_order = nullptr;  // scene-global triangle id per slot
   pyre::BNode* d_bnodes = nullptr;
@@ -167,5 +168,7 @@ void trace_closest(pyre_scene* s, const float4* rays, size_t n, pyre_hit8* hits,
                    bool check);
 void trace_any(pyre_scene* s, const float4* rays, size_t n, uint32_t* occluded, hipStream_t st, bool check);
 unsigned trace_grid(pyre_scene* s);
-int32_t* trace_gstack(pyre_scene* s);
+// Spill columns for trace_grid(s) x 256 threads (allocated on first use and
+// whenever the tree gets deeper; never inside a stream capture).
+int2* trace_gstack(pyre_scene* s);
 }  // namespace pyre

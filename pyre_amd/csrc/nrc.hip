@@ -175,9 +175,13 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 __global__ __launch_bounds__(64 * kInferWaves) void k_infer(const bf16x8* __restrict__ wfrag,
-                                                            const float* __restrict__ feats, uint32_t n,
-                                                            NrcBounds b, float* __restrict__ out_rgb) {
+                                                            const float* __restrict__ feats,
+                                                            const uint32_t* __restrict__ rows, uint32_t n_cap,
+                                                            const uint32_t* __restrict__ n_dev, NrcBounds b,
+                                                            float* __restrict__ out_rgb) {
   __shared__ bf16x8 sw[kFwdFrags * 64];
+  // query count: n_cap, or the device-side count (clamped to n_cap) when given
+  const uint32_t n = n_dev ? min(*n_dev, n_cap) : n_cap;
   __shared__ __attribute__((aligned(16))) __bf16 sact[kInferWaves][16][64 + 8];
   for (int i = threadIdx.x; i < kFwdFrags * 64; i += blockDim.x) sw[i] = wfrag[i];
   __syncthreads();
@@ -187,7 +191,8 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_infer(const bf16x8* __rest
   for (uint32_t tile = blockIdx.x * kInferWaves + wave; tile < ntiles; tile += gridDim.x * kInferWaves) {
     const uint32_t qi = tile * 16 + r;
     const bool valid = qi < n;
-    Query q = load_query(feats + 13 * (size_t)(valid ? qi : 0), b);
+    const uint32_t row = valid ? (rows ? rows[qi] : qi) : (rows ? rows[0] : 0u);
+    Query q = load_query(feats + 13 * (size_t)row, b);
     // layer 0: K = 128 from the encoding
     f32x4 acc[4];
     for (int ot = 0; ot < 4; ++ot) acc[ot] = f32x4{0, 0, 0, 0};
@@ -371,6 +376,7 @@ struct pyre_nrc {
   bf16x8* wf = nullptr;  // forward fragments
   bf16x8* wb = nullptr;  // backward fragments
   uint32_t step = 0;
+  float last_loss = 0.0f;
   // training scratch, sized for the largest batch seen
   uint32_t cap = 0;
   __bf16* acts = nullptr;  // layer inputs: 128 + 5 * 64 features per query
@@ -452,7 +458,8 @@ float nrc_gradients_dev(pyre_nrc* N, const float* feats, const float* targets, u
   PYRE_HIP_CHECK(hipStreamSynchronize(st));
   double s = 0;
   for (float x : lp) s += x;
-  return (float)(s / (3.0 * n));
+  N->last_loss = (float)(s / (3.0 * n));
+  return N->last_loss;
 }
 
 void nrc_adam_step(pyre_nrc* N, hipStream_t st) {
@@ -467,7 +474,16 @@ void nrc_adam_step(pyre_nrc* N, hipStream_t st) {
 void nrc_infer_dev(pyre_nrc* N, const float* feats, uint32_t n, float* rgb, hipStream_t st) {
   if (n == 0) return;
   unsigned grid = std::max(1u, std::min(blocks((n + 15) / 16, kInferWaves), (unsigned)N->ctx->cus * 2));
-  hipLaunchKernelGGL(k_infer, dim3(grid), dim3(64 * kInferWaves), 0, st, N->wf, feats, n, N->bounds, rgb);
+  hipLaunchKernelGGL(k_infer, dim3(grid), dim3(64 * kInferWaves), 0, st, N->wf, feats, (const uint32_t*)nullptr, n,
+                     (const uint32_t*)nullptr, N->bounds, rgb);
+}
+
+void nrc_infer_dev_count(pyre_nrc* N, const float* feats, const uint32_t* rows, const uint32_t* n_dev, uint32_t cap,
+                         float* rgb, hipStream_t st) {
+  if (cap == 0) return;
+  unsigned grid = std::max(1u, std::min(blocks((cap + 15) / 16, kInferWaves), (unsigned)N->ctx->cus * 2));
+  hipLaunchKernelGGL(k_infer, dim3(grid), dim3(64 * kInferWaves), 0, st, N->wf, feats, rows, cap, n_dev, N->bounds,
+                     rgb);
 }
 
 }  // namespace pyre
@@ -614,6 +630,13 @@ int pyre_nrc_gradients(pyre_ctx* ctx, pyre_nrc* N, const float* feats, const flo
       PYRE_HIP_CHECK(hipStreamSynchronize(st));
     }
   });
+}
+
+int pyre_nrc_stats(pyre_ctx* ctx, pyre_nrc* N, uint64_t* steps, float* last_loss) {
+  if (!ctx || !N) return PYRE_EINVAL;
+  if (steps) *steps = N->step;
+  if (last_loss) *last_loss = N->last_loss;
+  return PYRE_OK;
 }
 
 int pyre_nrc_train_step(pyre_ctx* ctx, pyre_nrc* N, const float* feats, const float* targets, size_t n, float* loss) {

@@ -19,5 +19,9 @@ float nrc_gradients_dev(pyre_nrc* nrc, const float* feats, const float* targets,
 void nrc_adam_step(pyre_nrc* nrc, hipStream_t st);
 // Fused BF16-MFMA inference: rgb[3n] from feats[13n] (device arrays).
 void nrc_infer_dev(pyre_nrc* nrc, const float* feats, uint32_t n, float* rgb, hipStream_t st);
+// Inference over a device-side query count (*n_dev, at most cap queries):
+// query i reads its 13 features at row rows[i] of feats, writes rgb[3i..].
+void nrc_infer_dev_count(pyre_nrc* nrc, const float* feats, const uint32_t* rows, const uint32_t* n_dev, uint32_t cap,
+                         float* rgb, hipStream_t st);
 
 }  // namespace pyre

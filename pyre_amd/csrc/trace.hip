@@ -1,10 +1,11 @@
 // Batched closest-hit / any-hit kernels (Scene::intersect / occluded over a
 // ray array) built on the traversal cores in traverse.hpp.
 //
-// Default: the 4-wide tree, persistent grid-stride threads, an 8-entry LDS
-// stack ring per thread spilling to a per-thread global column. The binary
+// Default: the 4-wide tree, persistent threads with dynamic ray fetch, an
+// 8-entry LDS stack ring per thread spilling to a per-thread global column. The binary
 // tree kernels (LDS stack as deep as the tree) remain selectable with
 // PYRE_TRAVERSAL=binary for comparison.
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 
@@ -54,22 +55,69 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(const Node2* __restrict
   }
 }
 
+// Persistent 4-wide traversal with dynamic ray fetch: a lane that finishes
+// its ray takes the next one of its wave's chunk as soon as a quarter of the
+// wave is idle, instead of waiting for the wave's slowest ray.
 template <bool ANY>
 __global__ __launch_bounds__(kTraceThreads) void k_trace4(const Node4* __restrict__ nodes,
                                                           const float4* __restrict__ tris,
                                                           const float4* __restrict__ rays, uint32_t n,
                                                           pyre_hit8* __restrict__ out_hit,
                                                           float2* __restrict__ out_uv,
-                                                          uint32_t* __restrict__ out_occ, int32_t* gstack) {
-  __shared__ int32_t ring[kRing * kTraceThreads];
+                                                          uint32_t* __restrict__ out_occ, int2* gstack,
+                                                          uint32_t* fetch) {
+  __shared__ int2 ring[kRing * kTraceThreads];
+  const uint32_t tid = blockIdx.x * kTraceThreads + threadIdx.x;
+  const uint32_t nthreads = gridDim.x * kTraceThreads;
+  HybridStack<kRing> st{ring + threadIdx.x, kTraceThreads, gstack + tid, nthreads, 0, 0};
+  WaveFeed feed(n, fetch);
+  Trav4<ANY, kRing> tv;
+  uint32_t ray = 0;
+  bool active = false;
+  for (;;) {
+    const uint64_t idle = __ballot(!active);
+    if (__popcll(idle) >= kRefillLanes) {
+      if (feed.take(idle, !active, ray)) {
+        const float4 ra = rays[2 * (size_t)ray], rb = rays[2 * (size_t)ray + 1];
+        tv.start(make_tray(v3(ra.x, ra.y, ra.z), v3(rb.x, rb.y, rb.z), ra.w, rb.w), st);
+        active = true;
+      }
+      if (!__any(active)) break;  // slice exhausted and every lane idle
+    }
+    if (active && tv.step(nodes, tris, st)) {
+      THit h;
+      tv.result(&h);
+      if (ANY) {
+        out_occ[ray] = h.gid != PYRE_MISS ? 1u : 0u;
+      } else {
+        pyre_hit8 o;
+        o.prim = h.gid;
+        o.t = h.t;
+        out_hit[ray] = o;
+        if (out_uv) out_uv[ray] = make_float2(h.u, h.v);
+      }
+      active = false;
+    }
+  }
+}
+
+// Grid-stride variant (each lane keeps its ray to the end), selectable with
+// PYRE_TRACE_FETCH=static for comparison.
+template <bool ANY>
+__global__ __launch_bounds__(kTraceThreads) void k_trace4s(const Node4* __restrict__ nodes,
+                                                           const float4* __restrict__ tris,
+                                                           const float4* __restrict__ rays, uint32_t n,
+                                                           pyre_hit8* __restrict__ out_hit,
+                                                           float2* __restrict__ out_uv,
+                                                           uint32_t* __restrict__ out_occ, int2* gstack) {
+  __shared__ int2 ring[kRing * kTraceThreads];
   const uint32_t tid = blockIdx.x * kTraceThreads + threadIdx.x;
   const uint32_t nthreads = gridDim.x * kTraceThreads;
   HybridStack<kRing> st{ring + threadIdx.x, kTraceThreads, gstack + tid, nthreads, 0, 0};
   for (uint32_t i = tid; i < n; i += nthreads) {
     const float4 ra = rays[2 * (size_t)i], rb = rays[2 * (size_t)i + 1];
-    const TRay r = make_tray(v3(ra.x, ra.y, ra.z), v3(rb.x, rb.y, rb.z), ra.w, rb.w);
     THit h;
-    traverse4<ANY, kRing>(nodes, tris, r, st, &h);
+    traverse4<ANY, kRing>(nodes, tris, make_tray(v3(ra.x, ra.y, ra.z), v3(rb.x, rb.y, rb.z), ra.w, rb.w), st, &h);
     if (ANY) {
       out_occ[i] = h.gid != PYRE_MISS ? 1u : 0u;
     } else {
@@ -80,6 +128,11 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace4(const Node4* __restric
       if (out_uv) out_uv[i] = make_float2(h.u, h.v);
     }
   }
+}
+
+__global__ void k_zero_u32(uint32_t* p, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
 }
 
 bool use_binary() {
@@ -95,10 +148,17 @@ void launch(pyre_scene* s, const float4* rays, size_t n, pyre_hit8* hit, float2*
   if (n == 0) return;
   uint32_t* ovf = s->d_flag;
   if (!use_binary()) {
-    unsigned want = (unsigned)((n + kTraceThreads - 1) / kTraceThreads);
-    unsigned grid = std::min(want, trace_grid(s));
-    hipLaunchKernelGGL((k_trace4<ANY>), dim3(grid), dim3(kTraceThreads), 0, st, s->d_nodes4, s->d_tris, rays,
-                       (uint32_t)n, hit, uv, occ, trace_gstack(s));
+    int2* gst = trace_gstack(s);
+    const char* fe = std::getenv("PYRE_TRACE_FETCH");
+    if (fe && std::strcmp(fe, "static") == 0) {
+      hipLaunchKernelGGL((k_trace4s<ANY>), dim3(trace_grid(s)), dim3(kTraceThreads), 0, st, s->d_nodes4, s->d_tris,
+                         rays, (uint32_t)n, hit, uv, occ, gst);
+      PYRE_HIP_CHECK(hipGetLastError());
+      return;
+    }
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, s->d_fetch, (uint32_t)kFetchParts);
+    hipLaunchKernelGGL((k_trace4<ANY>), dim3(trace_grid(s)), dim3(kTraceThreads), 0, st, s->d_nodes4, s->d_tris,
+                       rays, (uint32_t)n, hit, uv, occ, gst, s->d_fetch);
     PYRE_HIP_CHECK(hipGetLastError());
     return;
   }
@@ -144,13 +204,14 @@ unsigned trace_grid(pyre_scene* s) {
 
 // Spill columns for the persistent kernels: 3 entries per binary level bound
 // the pending children of every 4-wide node on the current path.
-int32_t* trace_gstack(pyre_scene* s) {
+int2* trace_gstack(pyre_scene* s) {
   const size_t threads = (size_t)trace_grid(s) * kTraceThreads;
   const size_t need = threads * (3 * (size_t)s->depth + 4);
+  if (!s->d_fetch) s->d_fetch = dev_alloc<uint32_t>(64);
   if (need > s->gstack_cap) {
     PYRE_HIP_CHECK(hipDeviceSynchronize());
     dev_free(s->d_gstack);
-    s->d_gstack = dev_alloc<int32_t>(need);
+    s->d_gstack = dev_alloc<int2>(need);
     s->gstack_cap = need;
   }
   return s->d_gstack;

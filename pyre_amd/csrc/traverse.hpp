@@ -133,18 +133,19 @@ __device__ __forceinline__ bool traverse(const Node2* __restrict__ nodes, const 
   return ok;
 }
 
-// Per-thread traversal stack: a ring of SL entries in LDS (a column of a
-// [SL][threads] array, so a wave's accesses at equal depth hit distinct
-// banks) whose oldest entries spill to a per-thread column of a global array
-// when the ring is full. Positions below `lo` live in global memory.
+// Per-thread traversal stack of (node, entry distance) pairs: a ring of SL
+// entries in LDS (a column of a [SL][threads] array, so a wave's accesses at
+// equal depth hit distinct banks) whose oldest entries spill to a per-thread
+// column of a global array when the ring is full. Positions below `lo` live
+// in global memory.
 template <int SL>
 struct HybridStack {
-  int32_t* lds;
+  int2* lds;
   int lstride;
-  int32_t* glb;
+  int2* glb;
   size_t gstride;
   int sp, lo;
-  __device__ __forceinline__ void push(int32_t v) {
+  __device__ __forceinline__ void push(int2 v) {
     if (sp - lo == SL) {
       glb[(size_t)lo * gstride] = lds[(lo & (SL - 1)) * lstride];
       lo++;
@@ -152,7 +153,7 @@ struct HybridStack {
     lds[(sp & (SL - 1)) * lstride] = v;
     sp++;
   }
-  __device__ __forceinline__ int32_t pop() {
+  __device__ __forceinline__ int2 pop() {
     sp--;
     if (sp < lo) {
       lo = sp;
@@ -206,20 +207,35 @@ __device__ __forceinline__ void cswap(float& ka, int32_t& ra, int& ca, float& kb
   }
 }
 
-// Traversal of the 4-wide tree. Same hit semantics as traverse(): the
-// children are visited nearest-first by slab entry distance, leaves as soon
-// as their box is hit.
+// Traversal of the 4-wide tree as a resumable state machine, one node visit
+// per step() so a persistent kernel can hand a finished lane a new ray at
+// any step. Same hit semantics as traverse(): children are visited
+// nearest-first by slab entry distance, leaves as soon as their box is hit;
+// a stacked child whose entry distance lies beyond the closest hit found
+// since it was pushed is dropped without fetching it (the reference's slab
+// test against the tightened t_max would reject it, bvh.rs:146-153).
 template <bool ANY, int SL>
-__device__ __forceinline__ void traverse4(const Node4* __restrict__ nodes, const float4* __restrict__ tris,
-                                          const TRay& r, HybridStack<SL>& st, THit* out) {
-  float best_t = r.t_max;
-  uint32_t best_gid = PYRE_MISS;
-  float bu = 0.0f, bv = 0.0f;
-  bool found = false;
-  st.sp = 0;
-  st.lo = 0;
-  int32_t cur = 0;
-  for (;;) {
+struct Trav4 {
+  TRay r;
+  float best_t, bu, bv;
+  uint32_t best_gid;
+  bool found;
+  int32_t cur;
+
+  __device__ __forceinline__ void start(const TRay& ray, HybridStack<SL>& st) {
+    r = ray;
+    best_t = ray.t_max;
+    bu = bv = 0.0f;
+    best_gid = PYRE_MISS;
+    found = false;
+    cur = 0;
+    st.sp = 0;
+    st.lo = 0;
+  }
+
+  // Visits node `cur`; true when the ray is finished.
+  __device__ __forceinline__ bool step(const Node4* __restrict__ nodes, const float4* __restrict__ tris,
+                                       HybridStack<SL>& st) {
     const Node4 nd = nodes[cur];
     // missed children: key +inf and count -1 (a hit child can have key +inf
     // only if its box is at infinity, so the count marks misses)
@@ -249,37 +265,109 @@ __device__ __forceinline__ void traverse4(const Node4* __restrict__ nodes, const
     cswap(k0, r0, c0, k2, r2, c2);
     cswap(k1, r1, c1, k3, r3, c3);
     cswap(k1, r1, c1, k2, r2, c2);
-    float ks[4] = {k0, k1, k2, k3};
-    int32_t rs[4] = {r0, r1, r2, r3};
-    int cs[4] = {c0, c1, c2, c3};
-    int32_t inner[4];
-    int ni = 0;
-    bool done = false;
-    for (int q = 0; q < 4; ++q) {
-      if (cs[q] < 0) continue;       // missed
-      if (ks[q] > best_t) continue;  // entered beyond the closest hit found meanwhile
-      if (cs[q] > 0) {
-        if (leaf_hits<ANY>(tris, r, rs[q], cs[q], best_t, best_gid, bu, bv, found)) {
-          done = true;
-          break;
-        }
-      } else {
-        inner[ni++] = rs[q];
+    // leaves first, nearest first; a child entered beyond the closest hit
+    // found meanwhile is skipped
+    if (leaf(tris, k0, r0, c0) || leaf(tris, k1, r1, c1) || leaf(tris, k2, r2, c2) || leaf(tris, k3, r3, c3))
+      return true;
+    // inner children: the nearest becomes the next node, the others are
+    // stacked farthest first
+    int2 next = make_int2(-1, 0);
+    inner(st, next, k3, r3, c3);
+    inner(st, next, k2, r2, c2);
+    inner(st, next, k1, r1, c1);
+    inner(st, next, k0, r0, c0);
+    if (next.x >= 0) {
+      cur = next.x;
+      return false;
+    }
+    while (st.sp > 0) {
+      const int2 en = st.pop();
+      if (ANY || __builtin_bit_cast(float, en.y) <= best_t) {
+        cur = en.x;
+        return false;
       }
     }
-    if (done) break;
-    if (ni > 0) {
-      for (int q = ni - 1; q >= 1; --q) st.push(inner[q]);
-      cur = inner[0];
-    } else {
-      if (st.sp == 0) break;
-      cur = st.pop();
-    }
+    return true;
   }
-  out->gid = ANY ? (found ? 0u : PYRE_MISS) : (found ? best_gid : PYRE_MISS);
-  out->t = found ? best_t : 0.0f;
-  out->u = bu;
-  out->v = bv;
+
+  // A hit leaf child: its triangles; true when an any-hit query is answered.
+  __device__ __forceinline__ bool leaf(const float4* __restrict__ tris, float k, int32_t ref, int cnt) {
+    if (cnt <= 0 || k > best_t) return false;
+    return leaf_hits<ANY>(tris, r, ref, cnt, best_t, best_gid, bu, bv, found);
+  }
+
+  __device__ __forceinline__ void inner(HybridStack<SL>& st, int2& next, float k, int32_t ref, int cnt) const {
+    if (cnt != 0 || k > best_t) return;
+    if (next.x >= 0) st.push(next);
+    next = make_int2(ref, __builtin_bit_cast(int32_t, k));
+  }
+
+  __device__ __forceinline__ void result(THit* out) const {
+    out->gid = ANY ? (found ? 0u : PYRE_MISS) : (found ? best_gid : PYRE_MISS);
+    out->t = found ? best_t : 0.0f;
+    out->u = bu;
+    out->v = bv;
+  }
+};
+
+// Whole traversal of one ray.
+template <bool ANY, int SL>
+__device__ __forceinline__ void traverse4(const Node4* __restrict__ nodes, const float4* __restrict__ tris,
+                                          const TRay& r, HybridStack<SL>& st, THit* out) {
+  Trav4<ANY, SL> tv;
+  tv.start(r, st);
+  while (!tv.step(nodes, tris, st)) {
+  }
+  tv.result(out);
 }
+
+// Lanes of a wave refill once at least this many are idle.
+constexpr int kRefillLanes = 16;
+// Independent work counters (one per XCD-sized slice of the work items).
+constexpr int kFetchParts = 8;
+// Items a wave claims from its slice's counter at a time.
+constexpr uint32_t kWaveChunk = 256;
+
+// Work distribution of the persistent kernels: the n items are cut into
+// kFetchParts slices, taken by the blocks with blockIdx % kFetchParts equal
+// to the slice (blocks are dealt to the XCDs round-robin, so a slice's
+// counter stays within one XCD); a wave claims kWaveChunk items of its slice
+// with one atomic and hands them to its idle lanes in order. All members are
+// wave-uniform.
+struct WaveFeed {
+  uint32_t lo, hi, next, end;
+  uint32_t* ctr;
+  bool dry;  // the slice has no unclaimed items left
+  __device__ __forceinline__ WaveFeed(uint32_t n, uint32_t* counters) {
+    const uint32_t part = blockIdx.x % kFetchParts;
+    const uint32_t per = n / kFetchParts + (n % kFetchParts != 0);
+    lo = part * per < n ? part * per : n;  // part * per < n + kFetchParts, no overflow for n < 2^32 - 8
+    hi = n - lo > per ? lo + per : n;
+    ctr = counters + part;
+    next = end = 0;
+    dry = false;
+  }
+  // Gives the idle lanes (ballot `idle`) the next items; true for an idle
+  // lane that got one (in `item`). Called by the whole wave.
+  __device__ __forceinline__ bool take(uint64_t idle, bool is_idle, uint32_t& item) {
+    if (next >= end) {
+      if (dry) return false;
+      uint32_t base = 0;
+      if (__lane_id() == 0) base = atomicAdd(ctr, kWaveChunk);
+      base = __builtin_amdgcn_readfirstlane(base);
+      next = base < hi - lo ? lo + base : hi;
+      end = next + (hi - next < kWaveChunk ? hi - next : kWaveChunk);
+      dry = end >= hi;
+    }
+    const uint32_t k = next + (uint32_t)__popcll(idle & ((1ull << __lane_id()) - 1ull));
+    const uint32_t cnt = (uint32_t)__popcll(idle);
+    next = end - next > cnt ? next + cnt : end;
+    if (is_idle && k < end) {
+      item = k;
+      return true;
+    }
+    return false;
+  }
+};
 
 }  // namespace pyre

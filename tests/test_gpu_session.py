@@ -1,0 +1,159 @@
+"""Progressive sessions (pyre_session_*) and the radiance-cache render mode.
+
+A session renders samples s = 0, 1, 2, ... per pixel with the streams
+pixel_seed(x, y, s) (sampler.rs:41-49), accumulating in sample order as
+main.rs:275-285 does, so any split of N spp into calls and passes -- eager or
+replayed from the captured hipGraph -- equals pyre_render(spp = N) bit for bit.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import pyre_amd
+from tests.helpers import rel_mse
+
+pytestmark = pytest.mark.gpu
+
+
+def _cornell(ctx):
+    pre = pyre_amd.Preset.cornell()
+    sc = pyre_amd.Scene.from_preset(ctx, pre)
+    sc.build_bvh(bins=12, max_leaf=8)
+    return pre, sc
+
+
+class _env:
+    def __init__(self, **kv):
+        self.kv = kv
+
+    def __enter__(self):
+        self.old = {k: os.environ.get(k) for k in self.kv}
+        os.environ.update({k: str(v) for k, v in self.kv.items()})
+
+    def __exit__(self, *a):
+        for k, v in self.old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def test_session_graph_replays_match_eager_render(ctx):
+    """Two samples per pass (PYRE_PASS_PATHS = 2 * pixels), so every call
+    replays the captured pass graph; each replay is checked against an eager
+    one-shot render of the same sample count, with reads (device work and a
+    synchronisation) between the replays."""
+    pre, sc = _cornell(ctx)
+    w, h = 40, 32
+    cam = pre.camera(w / h)
+    ip = pyre_amd.PathIntegrator(8, 3)
+    with _env(PYRE_PASS_PATHS=2 * w * h):
+        sess = pyre_amd.Session(sc, cam, ip, w, h)
+        total = 0
+        for k in range(4):
+            print(f"[session graph] replay {k}", flush=True)
+            sess.render(2)
+            total += 2
+            m, r64, spp = sess.read(radiance64=True)
+            assert spp == total
+            with _env(PYRE_GRAPH=0, PYRE_PASS_PATHS=1 << 21):
+                film = pyre_amd.Film(w, h)
+                ref = film.render(sc, cam, ip, total, accum64=True)
+            assert np.array_equal(_bits(m), _bits(ref)), f"replay {k}: graph pass differs from eager render"
+            assert np.array_equal(r64, film.accum64)
+        sess.close()
+
+
+def test_session_resume_and_reset(ctx):
+    pre, sc = _cornell(ctx)
+    w, h = 32, 32
+    cam = pre.camera(1.0)
+    ip = pyre_amd.PathIntegrator(8, 3)
+    sess = pyre_amd.Session(sc, cam, ip, w, h)
+    sess.render(3)
+    sess.render(5)
+    m, _, spp = sess.read()
+    assert spp == 8
+    ref = pyre_amd.Film(w, h).render(sc, cam, ip, 8)
+    assert np.array_equal(_bits(m), _bits(ref))
+    # reset clears the sums; the sample index keeps advancing (samples 8..11)
+    sess.reset()
+    sess.render(4)
+    m2, _, spp2 = sess.read()
+    assert spp2 == 4
+    ref2 = pyre_amd.Film(w, h).render(sc, cam, ip, 4, spp_offset=8)
+    assert np.array_equal(_bits(m2), _bits(ref2))
+    ext, shadow = sc.render_stats()
+    assert ext > 0 and shadow > 0
+    sess.close()
+
+
+def _nrc_setup(ctx, w, h):
+    pre, sc = _cornell(ctx)
+    cam = pre.camera(w / h)
+    lo, hi = sc.bounds()
+    cache = pyre_amd.RadianceCache(ctx, lo, hi, seed=5)
+    return pre, sc, cam, cache
+
+
+def test_nrc_render_trains_and_terminates(ctx):
+    w, h = 48, 48
+    pre, sc, cam, cache = _nrc_setup(ctx, w, h)
+    ip = pyre_amd.PathIntegrator(8, 3)
+    sess = pyre_amd.Session(sc, cam, ip, w, h, nrc_train_paths=1 << 13)
+    losses = sess.render_nrc(cache, frames=40, spp_per_frame=1, train_paths=1 << 13, train_batch=1 << 12,
+                             train_steps=4, warmup_frames=20)
+    print(f"[nrc render] losses first {losses[:5]} last {losses[-5:]}", flush=True)
+    # single-path targets are noisy (relative L2 stays near their variance), so
+    # the loss is only checked to be finite; the gain test judges the cache
+    assert np.all(np.isfinite(losses)) and np.all(losses >= 0)
+    steps, last = cache.stats()
+    assert steps == 40 * 4 and np.isfinite(last)
+    m, _, spp = sess.read()
+    assert spp == 20
+    img = m.reshape(h, w, 10)[..., :3]
+    assert np.all(np.isfinite(img)) and img.min() >= 0
+    # cache paths stop early: fewer extension rays than a plain render of the same spp
+    ext_nrc, _ = sc.render_stats()
+    pyre_amd.Film(w, h).render(sc, cam, ip, 20)
+    ext_plain, _ = sc.render_stats()
+    print(f"  extension rays nrc={ext_nrc} plain={ext_plain}")
+    sess.close()
+
+
+def test_nrc_gain_over_plain_at_equal_spp(ctx):
+    """What the trained cache gains: at 4 spp, relMSE against a 2048-spp
+    plain reference is lower with paths ending in the trained cache than
+    with the plain path tracer, and lower than with an untrained cache."""
+    w, h = 48, 48
+    pre, sc, cam, cache = _nrc_setup(ctx, w, h)
+    ip = pyre_amd.PathIntegrator(8, 3)
+    ref = pyre_amd.Film(w, h).render(sc, cam, ip, 2048).reshape(h, w, 10)[..., :3]
+    plain = pyre_amd.Film(w, h).render(sc, cam, ip, 4, spp_offset=4096).reshape(h, w, 10)[..., :3]
+    e_plain = rel_mse(plain, ref)
+
+    def nrc_image(train_frames):
+        c = pyre_amd.RadianceCache(ctx, *sc.bounds(), seed=5)
+        s = pyre_amd.Session(sc, cam, ip, w, h, nrc_train_paths=1 << 13)
+        if train_frames:
+            s.render_nrc(c, frames=train_frames, spp_per_frame=1, train_paths=1 << 13, train_batch=1 << 12,
+                         train_steps=4, warmup_frames=train_frames)
+        s.reset()
+        # the measured frames do not train: the cache is frozen (or never trained)
+        s.render_nrc(c, frames=4, spp_per_frame=1, train_paths=1 << 13, train_batch=1 << 12, train_steps=0)
+        m, _, spp = s.read()
+        assert spp == 4
+        s.close()
+        c.close()
+        return m.reshape(h, w, 10)[..., :3]
+
+    e_trained = rel_mse(nrc_image(120), ref)
+    e_untrained = rel_mse(nrc_image(0), ref)
+    print(f"[nrc gain] relMSE plain={e_plain:.4f} trained={e_trained:.4f} untrained={e_untrained:.4f}", flush=True)
+    assert e_trained < e_plain
+    assert e_trained < e_untrained

@@ -128,8 +128,15 @@ def test_env_sample_pdf_le(ctx, oracle):
     assert near.sum() < 50
     sin_t = np.sqrt(np.maximum(1.0 - d[:, 1].astype(np.float64) ** 2, 1e-30))
     tol = TOL + 3.0 * float(np.spacing(np.float32(np.pi))) / sin_t
-    assert np.all(rel_err(p[keep], op[keep]) <= tol[keep])
-    assert np.all(rel_err(sc.env_le(d), oe.le(d)) <= 1e-4)
+    bad = keep & ~(rel_err(p, op) <= tol)
+    if bad.any():
+        i = np.flatnonzero(bad)[:8]
+        print("env pdf mismatches: dir", d[i], "gpu", p[i], "cpu", op[i], "uv", uv[i], "n", bad.sum())
+    assert not bad.any()
+    # bilinear radiance at a uv an ulp or two apart: between random background
+    # texels (up to 0.75 apart) that moves le by ~1e-5 absolute
+    g, o = sc.env_le(d), oe.le(d)
+    assert np.all(np.abs(g - o) <= 1e-4 * np.maximum(np.abs(o), 1.0))
 
 
 def test_quad_light_sample_pdf(ctx, oracle):
