@@ -130,21 +130,236 @@ def bench_cornell(args):
     print(json.dumps(line), flush=True)
 
 
+# ---------------------------------------------------------------- production
+PROD_W, PROD_H = 1920, 1080
+REL_EPS = 1e-2  # relMSE = mean((I - R)^2 / (R^2 + eps))
+
+
+def production_scene(ctx):
+    """64 level-7 icosphere instances (20,971,520 triangles) with random
+    affine transforms and Disney materials, a ground quad, a procedural
+    2048x1024 sky + sun HDRI (seed 3); camera auto-framed, 16:9, thin lens
+    radius 0.05. BVH with the reference's build parameters (12 bins, leaf 8)."""
+    import pyre_amd
+    t0 = time.perf_counter()
+    pre = pyre_amd.Preset.production(seed=3, n_instances=64, level=7, env_w=2048, env_h=1024)
+    scene = pyre_amd.Scene.from_preset(ctx, pre)
+    gen_s = time.perf_counter() - t0
+    info = scene.build_bvh()
+    cam = pre.camera(PROD_W / PROD_H, aperture=0.05)
+    log(f"production scene: generated+uploaded in {gen_s:.1f}s, bvh {info['build_ms']:.1f} ms, "
+        f"nodes={info['n_nodes']} depth={info['depth']}")
+    return pre, scene, cam, info
+
+
+def rel_mse(img, ref):
+    img = np.asarray(img, np.float64)
+    ref = np.asarray(ref, np.float64)
+    return float(np.mean((img - ref) ** 2 / (ref ** 2 + REL_EPS)))
+
+
+def load_reference(path):
+    if not path or not os.path.exists(path):
+        return None
+    ref = np.load(path)
+    return ref.reshape(PROD_H * PROD_W, 3).astype(np.float64)
+
+
+def bench_production(args):
+    """Production scene at 1920x1080, 64 spp, max depth 12: one step = one
+    full render through a progressive session (a graph-replayed pass per spp)."""
+    import pyre_amd
+    ctx = pyre_amd.Context(0)
+    dev_name, _ = ctx.device_info()
+    SPP = 64
+    pre, scene, cam, info = production_scene(ctx)
+    ip = pyre_amd.PathIntegrator(12, 3)
+    sess = pyre_amd.Session(scene, cam, ip, PROD_W, PROD_H)
+    for _ in range(args.warmup):
+        sess.render(SPP)
+        sess.reset()
+    walls = []
+    for k in range(args.steps):
+        sess.reset()
+        t0 = time.perf_counter()
+        sess.render(SPP)
+        walls.append(time.perf_counter() - t0)
+        log(f"step {k}: {walls[-1]:.3f}s")
+    ext, sh = scene.render_stats()
+    per = float(np.mean(walls))
+    line = {
+        "metric": "samples/s (production scene, 1080p, NEE+MIS path tracer)",
+        "value": round(PROD_W * PROD_H * SPP / per / 1e6, 2), "unit": "Msamples/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(per * 1e3, 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (procedural scene, seed 3)",
+        "config": {"workload": "64 level-7 icosphere instances (20,971,520 tris) + ground, 2048x1024 sky/sun HDRI, "
+                               "1920x1080, 64 spp, max depth 12, thin lens 0.05",
+                   "triangles": 20971522, "width": PROD_W, "height": PROD_H, "spp": SPP, "max_depth": 12},
+        "extension_mrays_s": round(ext / per / 1e6, 1), "shadow_mrays_s": round(sh / per / 1e6, 1),
+        "rays_per_sample": round((ext + sh) / (PROD_W * PROD_H * SPP), 3),
+        "bvh_build_ms": round(info["build_ms"], 1), "device": dev_name,
+    }
+    ref = load_reference(args.reference)
+    if ref is not None:
+        _, r64, spp = sess.read(mean=False, radiance64=True)
+        line["relmse_64spp"] = round(rel_mse(r64 / spp, ref), 5)
+        line["relmse_eps"] = REL_EPS
+    if not args.no_cpu:
+        line["cpu_baseline"] = production_cpu_baseline(pre, cam, cspp=2)
+    print(json.dumps(line), flush=True)
+
+
+def production_cpu_baseline(pre, cam, cspp):
+    from oracle import oracle as O
+    threads = O.nthreads()
+    arr = pre.arrays()
+    osc = O.Scene()
+    for m in arr["materials"]:
+        osc.add_material(O.material(m.kind, m.base_color, m.metallic, m.roughness, m.specular))
+    t0 = time.perf_counter()
+    for m in arr["meshes"]:
+        osc.add_mesh(m["positions"], m["indices"], m["normals"], m["material_id"])
+    e = arr["env"]
+    osc.set_env(O.Hdri(e["width"], e["height"], e["rgb"], e["intensity"]))
+    build_s = time.perf_counter() - t0
+    log(f"oracle scene built in {build_s:.1f}s")
+    ocam = O.camera(cam.origin, cam.target, cam.up, cam.vfov_deg, cam.aspect, cam.aperture_radius,
+                    cam.focus_distance if cam.focus_distance > 0 else None)
+    t0 = time.perf_counter()
+    osc.render(ocam, PROD_W, PROD_H, cspp, max_depth=12, min_rr=3, threads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": round(PROD_W * PROD_H * cspp / dt / 1e6, 4), "unit": "Msamples/s", "cores": threads,
+            "kind": "port", "sample": f"full 1920x1080 frame at {cspp} spp (of 64); per-mesh BVH build "
+                                      f"{build_s:.0f} s untimed"}
+
+
+def make_reference(args):
+    """FP64-accumulated plain reference of the production view: args.ref_spp
+    samples per pixel, saved as float32 radiance (H, W, 3) .npy."""
+    import pyre_amd
+    ctx = pyre_amd.Context(0)
+    pre, scene, cam, info = production_scene(ctx)
+    sess = pyre_amd.Session(scene, cam, pyre_amd.PathIntegrator(12, 3), PROD_W, PROD_H)
+    t0 = time.perf_counter()
+    done = 0
+    while done < args.ref_spp:
+        n = min(256, args.ref_spp - done)
+        sess.render(n)
+        done += n
+        log(f"reference {done}/{args.ref_spp} spp, {time.perf_counter() - t0:.1f}s")
+    _, r64, spp = sess.read(mean=False, radiance64=True)
+    img = (r64 / spp).astype(np.float32).reshape(PROD_H, PROD_W, 3)
+    os.makedirs(os.path.dirname(os.path.abspath(args.make_reference)), exist_ok=True)
+    np.save(args.make_reference, img)
+    print(json.dumps({"reference": args.make_reference, "spp": spp, "seconds": round(time.perf_counter() - t0, 1),
+                      "mean": float(img.mean())}), flush=True)
+
+
+def _crossing(curve, target):
+    """First time at which relMSE falls to target (log-log interpolation
+    between checkpoints); None if it never does."""
+    prev = None
+    for t, e in curve:
+        if e <= target:
+            if prev is None:
+                return t
+            (t0, e0) = prev
+            a = (np.log(target) - np.log(e0)) / (np.log(e) - np.log(e0))
+            return float(np.exp(np.log(t0) + a * (np.log(t) - np.log(t0))))
+        prev = (t, e)
+    return None
+
+
+def bench_nrc(args):
+    """Seconds to relMSE 0.01 at 1080p on the production scene: the plain
+    path tracer vs the radiance cache (trained online from a cold start, paths
+    ending in the cache after the first diffuse bounce). Both render
+    progressive 1-spp passes; relMSE is checked against the FP64 reference at
+    checkpoints outside the timed region."""
+    import pyre_amd
+    ref = load_reference(args.reference)
+    if ref is None:
+        print("bench.py --config nrc needs --reference (make one with --make-reference PATH)", file=sys.stderr)
+        sys.exit(2)
+    ctx = pyre_amd.Context(0)
+    dev_name, _ = ctx.device_info()
+    pre, scene, cam, info = production_scene(ctx)
+    ip = pyre_amd.PathIntegrator(12, 3)
+    target = 0.01
+    checkpoints = [1, 2, 4, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512]
+    checkpoints = [c for c in checkpoints if c <= args.max_spp]
+
+    def run(kind):
+        lo, hi = scene.bounds()
+        cache = pyre_amd.RadianceCache(ctx, lo, hi, seed=5) if kind == "nrc" else None
+        sess = pyre_amd.Session(scene, cam, ip, PROD_W, PROD_H, nrc_train_paths=(1 << 16) if cache else 0)
+        curve, t_acc, done = [], 0.0, 0
+        for c in checkpoints:
+            n = c - done
+            t0 = time.perf_counter()
+            if cache is None:
+                sess.render(n)
+            else:
+                sess.render_nrc(cache, frames=n, spp_per_frame=1, train_paths=1 << 14, train_batch=1 << 14,
+                                train_steps=4)
+            t_acc += time.perf_counter() - t0
+            done = c
+            _, r64, spp = sess.read(mean=False, radiance64=True)
+            e = rel_mse(r64 / spp, ref)
+            curve.append((t_acc, e))
+            log(f"{kind}: {c} spp, {t_acc:.3f}s, relMSE {e:.5f}")
+            if e <= target * 0.7:
+                break
+        sess.close()
+        if cache is not None:
+            cache.close()
+        return curve
+
+    plain = run("plain")
+    nrc = run("nrc")
+    tp, tn = _crossing(plain, target), _crossing(nrc, target)
+    line = {
+        "metric": "seconds to relMSE 0.01 at 1080p, plain vs NRC", "value": round(tn, 4) if tn else None,
+        "unit": "s (NRC)", "n_gpus": 1, "steps": 1, "warmup": 0, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 (cache: bf16 MFMA, f32 accumulate)",
+        "data": "synthetic (procedural scene, seed 3)",
+        "config": {"workload": "production scene 1920x1080, max depth 12, thin lens 0.05; NRC 5x64 ReLU MLP, "
+                               "2^16 training samples/frame (4 Adam steps x 2^14), lr 1e-2",
+                   "relmse_eps": REL_EPS, "reference_spp": args.ref_spp},
+        "plain_seconds": round(tp, 4) if tp else None, "nrc_seconds": round(tn, 4) if tn else None,
+        "nrc_over_plain": round(tn / tp, 4) if (tn and tp) else None,
+        "plain_curve": [(round(t, 4), round(e, 5)) for t, e in plain],
+        "nrc_curve": [(round(t, 4), round(e, 5)) for t, e in nrc], "device": dev_name,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity spot check")
-    ap.add_argument("--config", default="traversal", choices=["traversal", "cornell"],
-                    help="traversal: the headline microbench; cornell: the Cornell render")
+    ap.add_argument("--config", default="traversal", choices=["traversal", "cornell", "production", "nrc"],
+                    help="traversal: the headline microbench; cornell: the Cornell render; production: the "
+                         "1080p production render; nrc: seconds to relMSE 0.01, plain vs radiance cache")
+    ap.add_argument("--reference", default=None, help="production reference .npy (relMSE)")
+    ap.add_argument("--make-reference", default=None, help="render the production reference to this .npy")
+    ap.add_argument("--ref-spp", type=int, default=16384)
+    ap.add_argument("--max-spp", type=int, default=512, help="nrc: largest spp tried")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(rays and pixels shard with no exchange; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.make_reference:
+        return make_reference(args)
     if args.config == "cornell":
         return bench_cornell(args)
+    if args.config == "production":
+        return bench_production(args)
+    if args.config == "nrc":
+        return bench_nrc(args)
 
     import torch
     import pyre_amd

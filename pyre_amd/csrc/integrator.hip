@@ -632,7 +632,7 @@ inline unsigned nblocks(size_t n, unsigned t) { return (unsigned)((n + t - 1) / 
 // Device path state for up to `cap` paths.
 struct Work {
   size_t cap = 0;
-  uint32_t slots = 1, vmax = 0, tcap = 0;
+  uint32_t slots = 1, vmax = 0, tcap = 0, tpaths = 0;  // tpaths: training paths with vertex records
   Paths P{};
   PassParams* ppd = nullptr;
   std::vector<void*> bufs;
@@ -644,10 +644,11 @@ struct Work {
     bufs.push_back(p);
     return p;
   }
-  void alloc(size_t cap_, uint32_t slots_, uint32_t vmax_, uint32_t tcap_, bool cache) {
+  void alloc(size_t cap_, uint32_t slots_, uint32_t vmax_, uint32_t tpaths_, uint32_t tcap_, bool cache) {
     cap = cap_;
     slots = slots_;
     vmax = vmax_;
+    tpaths = tpaths_;
     tcap = tcap_;
     P.ray = get<float4>(2 * cap);
     P.rng = get<ulonglong2>(2 * cap);
@@ -666,7 +667,7 @@ struct Work {
     P.stats = get<unsigned long long>(2);
     P.slots = slots;
     if (vmax) {
-      P.vrec = get<float4>(cap * vmax * kVrec);
+      P.vrec = get<float4>((size_t)tpaths * vmax * kVrec);
       P.vcount = get<uint32_t>(cap);
       tfeat = get<float>(13 * (size_t)tcap);
       ttarget = get<float>(3 * (size_t)tcap);
@@ -822,7 +823,7 @@ void session_init(pyre_session* S, pyre_ctx* ctx, pyre_scene* s, const pyre_came
   require(cap < (1ull << 31), "path pool too large");
   const uint32_t slots = std::max(1u, s->n_lights + (s->has_env ? 1u : 0u));
   const uint32_t vmax = train_paths ? std::max(1u, ip.max_depth) : 0;
-  S->W.alloc(cap, slots, vmax, train_paths ? std::min<uint32_t>(train_paths * vmax, 1u << 20) : 0,
+  S->W.alloc(cap, slots, vmax, train_paths, train_paths ? std::min<uint32_t>(train_paths * vmax, 1u << 20) : 0,
              train_paths > 0);
   S->acc = dev_alloc<float>(10 * (size_t)S->npix);
   S->acc64 = dev_alloc<double>(3 * (size_t)S->npix);
@@ -944,7 +945,8 @@ int pyre_session_render_nrc(pyre_ctx* ctx, pyre_session* S, pyre_nrc* nrc, const
   if (!ctx || !S || !nrc || !p) return PYRE_EINVAL;
   return sguard(ctx, [&] {
     require(S->W.vmax > 0, "session created without radiance-cache buffers (nrc_train_paths = 0)");
-    require(p->train_paths > 0 && p->train_paths <= S->W.cap, "train_paths exceeds the session's capacity");
+    require(p->train_paths > 0 && p->train_paths <= S->W.tpaths,
+            "train_paths exceeds the session's nrc_train_paths");
     require(p->train_batch >= 16, "train_batch below 16");
     hipStream_t st = ctx->stream;
     for (uint32_t f = 0; f < p->frames; ++f) {
