@@ -334,15 +334,113 @@ def bench_nrc(args):
     print(json.dumps(line), flush=True)
 
 
+# ------------------------------------------------------------------- batch
+def batch_jobs(ctx, n_jobs=96, seed=4):
+    """96 jobs: square images with side log-uniform in [128, 1024] (128^2 to
+    1024^2 pixels), spp from {4, 16, 64, 256}, scenes of log-uniform
+    10k..2M triangles built from instanced icospheres (level and count
+    chosen to hit the triangle count), half Lambertian / half Disney
+    materials, procedural 512x256 sky + sun HDRI; pinhole, max depth 8."""
+    import pyre_amd
+    rng = np.random.default_rng(seed)
+    jobs, scenes = [], []
+    t0 = time.perf_counter()
+    for j in range(n_jobs):
+        side = int(round(np.exp(rng.uniform(np.log(128), np.log(1024)))))
+        spp = int(rng.choice([4, 16, 64, 256]))
+        tris = float(np.exp(rng.uniform(np.log(1e4), np.log(2e6))))
+        level = 1
+        while tris / (20 * 4 ** level) > 64:
+            level += 1
+        n_inst = int(min(64, max(1, round(tris / (20 * 4 ** level)))))
+        pre = pyre_amd.Preset.instances(1000 + j, n_inst, level, 512, 256, 0.5)
+        sc = pyre_amd.Scene.from_preset(ctx, pre)
+        sc.build_bvh()
+        scenes.append((pre, sc, n_inst * 20 * 4 ** level + 2))
+        jobs.append(pyre_amd.Job(sc, pre.camera(1.0), pyre_amd.PathIntegrator(8, 3), side, side, spp))
+    log(f"batch: {n_jobs} scenes generated and built in {time.perf_counter() - t0:.1f}s")
+    return jobs, scenes
+
+
+def bench_batch(args):
+    """All 96 jobs in one pyre_render_batch call (shared path pool, graph-
+    replayed passes, autotuned launch shape); step = the whole batch."""
+    import pyre_amd
+    ctx = pyre_amd.Context(0)
+    dev_name, _ = ctx.device_info()
+    jobs, scenes = batch_jobs(ctx)
+    total = sum(j.width * j.height * j.spp for j in jobs)
+    _, _, st0 = pyre_amd.render_batch(ctx, jobs, autotune=True)  # warmup + tuning
+    tune = dict(pool_paths=st0["pool_paths"], trace_blocks_per_cu=st0["trace_blocks_per_cu"],
+                stack_ring=st0["stack_ring"])
+    log(f"autotuned: {tune}")
+    walls, stats = [], None
+    for k in range(args.steps):
+        t0 = time.perf_counter()
+        _, _, stats = pyre_amd.render_batch(ctx, jobs, **tune)
+        walls.append(time.perf_counter() - t0)
+        log(f"batch step {k}: {walls[-1]:.3f}s ({stats['passes']} passes, gpu {stats['gpu_ms']:.1f} ms)")
+    per = float(np.mean(walls))
+    # the same jobs one pyre_render call after another
+    t0 = time.perf_counter()
+    for j in jobs:
+        pyre_amd.Film(j.width, j.height).render(j.scene, j.camera, j.integrator, j.spp)
+    seq = time.perf_counter() - t0
+    line = {
+        "metric": "samples/s (96 batched uneven jobs)", "value": round(total / per / 1e6, 2), "unit": "Msamples/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": round(per * 1e3, 1), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (job list seed 4)",
+        "config": {"workload": "96 jobs: 128^2..1024^2 px (log-uniform), spp {4,16,64,256}, 10k..2M-triangle "
+                               "icosphere-instance scenes (log-uniform), Lambertian/Disney 50/50, 512x256 sky+sun "
+                               "HDRI, pinhole, depth 8", "jobs": len(jobs), "samples": total,
+                   "triangles": int(sum(t for _, _, t in scenes))},
+        "tuning": tune, "passes": stats["passes"], "gpu_ms": round(stats["gpu_ms"], 1),
+        "extension_mrays_s": round(stats["extension_rays"] / per / 1e6, 1),
+        "shadow_mrays_s": round(stats["shadow_rays"] / per / 1e6, 1),
+        "sequential_gpu_s": round(seq, 3), "batched_over_sequential_speedup": round(seq / per, 3), "device": dev_name,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = batch_cpu_baseline(jobs, scenes)
+    print(json.dumps(line), flush=True)
+
+
+def batch_cpu_baseline(jobs, scenes, n=4):
+    """Oracle on the n jobs with the fewest triangles, one after another, at
+    1 spp each (per-mesh BVH builds untimed)."""
+    from oracle import oracle as O
+    threads = O.nthreads()
+    order = sorted(range(len(jobs)), key=lambda k: scenes[k][2])[:n]
+    samples, secs = 0, 0.0
+    for k in order:
+        pre, _, _ = scenes[k]
+        arr = pre.arrays()
+        osc = O.Scene()
+        for m in arr["materials"]:
+            osc.add_material(O.material(m.kind, m.base_color, m.metallic, m.roughness, m.specular))
+        for m in arr["meshes"]:
+            osc.add_mesh(m["positions"], m["indices"], m["normals"], m["material_id"])
+        e = arr["env"]
+        osc.set_env(O.Hdri(e["width"], e["height"], e["rgb"], e["intensity"]))
+        cam = jobs[k].camera
+        ocam = O.camera(cam.origin, cam.target, cam.up, cam.vfov_deg, cam.aspect)
+        t0 = time.perf_counter()
+        osc.render(ocam, jobs[k].width, jobs[k].height, 1, max_depth=8, min_rr=3, threads=threads)
+        secs += time.perf_counter() - t0
+        samples += jobs[k].width * jobs[k].height
+    return {"value": round(samples / secs / 1e6, 4), "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "sample": f"the {n} jobs with the fewest triangles at 1 spp each, sequentially"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity spot check")
-    ap.add_argument("--config", default="traversal", choices=["traversal", "cornell", "production", "nrc"],
+    ap.add_argument("--config", default="traversal", choices=["traversal", "cornell", "production", "nrc", "batch"],
                     help="traversal: the headline microbench; cornell: the Cornell render; production: the "
-                         "1080p production render; nrc: seconds to relMSE 0.01, plain vs radiance cache")
+                         "1080p production render; nrc: seconds to relMSE 0.01, plain vs radiance cache; "
+                         "batch: 96 uneven jobs in one batched call")
     ap.add_argument("--reference", default=None, help="production reference .npy (relMSE)")
     ap.add_argument("--make-reference", default=None, help="render the production reference to this .npy")
     ap.add_argument("--ref-spp", type=int, default=16384)
@@ -360,6 +458,8 @@ def main():
         return bench_production(args)
     if args.config == "nrc":
         return bench_nrc(args)
+    if args.config == "batch":
+        return bench_batch(args)
 
     import torch
     import pyre_amd

@@ -49,6 +49,11 @@ int pyre_preset_cornell(int motion_blur, pyre_preset** out);
  * zenith luminance). */
 int pyre_preset_production(uint64_t seed, uint32_t n_instances, uint32_t level, uint32_t env_w, uint32_t env_h,
                            pyre_preset** out);
+/* The same scene family with a Lambertian material (albedo = the drawn base
+ * colour) on a lambert_fraction share of the instances; production is
+ * lambert_fraction = 0. */
+int pyre_preset_instances(uint64_t seed, uint32_t n_instances, uint32_t level, uint32_t env_w, uint32_t env_h,
+                          float lambert_fraction, pyre_preset** out);
 
 /* Points desc at the preset's arrays (valid until pyre_preset_destroy). */
 int pyre_preset_desc(const pyre_preset* p, pyre_scene_desc* desc);

@@ -250,6 +250,44 @@ int pyre_render(pyre_ctx* ctx, pyre_scene* scene, const pyre_camera_desc* camera
  * rays (camera rays included) and any-hit shadow rays. */
 int pyre_render_stats(pyre_ctx* ctx, pyre_scene* scene, uint64_t* extension_rays, uint64_t* shadow_rays);
 
+/* ------------------------------------------------------------- batches */
+/* Many main.rs:253-287 renders in one go: the jobs (different scenes,
+ * cameras, resolutions, spp) share one path pool; every pass of the wavefront
+ * stages carries whole samples of several jobs (largest jobs first), and one
+ * captured hipGraph per batch replays the pass. Each job's output equals
+ * pyre_render of that job alone (same per-sample streams, same per-pixel
+ * summation order). Scenes must belong to ctx and have a built BVH. */
+typedef struct pyre_job {
+  pyre_scene* scene;
+  pyre_camera_desc camera;
+  pyre_integrator_params integrator;
+  uint32_t width, height, spp, spp_offset;
+  pyre_pixel_sample* out; /* width*height means, or NULL */
+  double* accum64;        /* FP64 radiance sums, added to (as pyre_render), or NULL */
+} pyre_job;
+
+/* Launch configuration; 0 fields keep the defaults (2^21 paths per pass, 8
+ * traversal blocks per CU, 8-entry LDS stack ring). autotune != 0 first times
+ * trial passes of the batch over pool sizes {2^20, 2^21, 2^22}, blocks per CU
+ * {4, 8} and stack rings {4, 8, 16} and keeps the fastest. */
+typedef struct pyre_batch_params {
+  uint32_t pool_paths;
+  uint32_t trace_blocks_per_cu;
+  uint32_t stack_ring;
+  uint32_t autotune;
+} pyre_batch_params;
+
+typedef struct pyre_batch_stats {
+  uint64_t extension_rays, shadow_rays;
+  uint32_t passes;
+  float gpu_ms; /* GPU time of the passes (autotuning trials excluded) */
+  uint32_t pool_paths, trace_blocks_per_cu, stack_ring; /* configuration used */
+} pyre_batch_stats;
+
+/* params and stats may be NULL. */
+int pyre_render_batch(pyre_ctx* ctx, const pyre_job* jobs, size_t n, const pyre_batch_params* params,
+                      pyre_batch_stats* stats);
+
 /* ------------------------------------------------ neural radiance cache */
 /* No counterpart in the reference. A 5x64 ReLU MLP on BF16 MFMA (FP32
  * accumulation, FP32 master weights, Adam) mapping a query (13 floats:
@@ -310,8 +348,8 @@ int pyre_session_render(pyre_ctx* ctx, pyre_session* session, uint32_t spp);
  * batches of train_batch samples, then spp_per_frame samples per pixel are
  * rendered with paths ending in the cache at the first surface reached after
  * a bounce off a diffuse material (Lambertian, or Disney roughness >=
- * diffuse_roughness). Frames before warmup_frames train but do not add to
- * the image. */
+ * diffuse_roughness). Frames before warmup_frames only train (no image
+ * samples). */
 typedef struct pyre_nrc_render_params {
   uint32_t frames;
   uint32_t spp_per_frame;

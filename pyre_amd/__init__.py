@@ -152,6 +152,17 @@ class Preset:
             raise PyreError(rc, "pyre_preset_production")
         return cls(h, lib)
 
+    @classmethod
+    def instances(cls, seed, n_instances, level, env_w=512, env_h=256, lambert_fraction=0.5) -> "Preset":
+        """Instanced icospheres like production with a Lambertian share of the
+        materials (the batched-jobs scene family)."""
+        lib = L.load()
+        h = C.c_void_p()
+        rc = lib.pyre_preset_instances(seed, n_instances, level, env_w, env_h, lambert_fraction, C.byref(h))
+        if rc:
+            raise PyreError(rc, "pyre_preset_instances")
+        return cls(h, lib)
+
     def desc(self) -> L.SceneDesc:
         d = L.SceneDesc()
         self.lib.pyre_preset_desc(self.h, C.byref(d))
@@ -548,3 +559,40 @@ class Session:
             self.close()
         except Exception:
             pass
+
+
+@dataclass
+class Job:
+    """One render of a batch (pyre_job): scene, camera, integrator, image size,
+    spp and first sample index."""
+    scene: Scene
+    camera: Camera
+    integrator: PathIntegrator
+    width: int
+    height: int
+    spp: int
+    spp_offset: int = 0
+
+
+def render_batch(ctx: Context, jobs, pool_paths=0, trace_blocks_per_cu=0, stack_ring=0, autotune=False,
+                 accum64=False):
+    """pyre_render_batch: every job's PixelSample means (npix, 10), its FP64
+    radiance sums (npix, 3) if accum64, and the batch statistics."""
+    n = len(jobs)
+    outs = [np.zeros((j.width * j.height, 10), np.float32) for j in jobs]
+    accs = [np.zeros((j.width * j.height, 3), np.float64) for j in jobs] if accum64 else None
+    arr = (L.Job * max(n, 1))()
+    for i, j in enumerate(jobs):
+        arr[i].scene = j.scene.h
+        arr[i].camera = j.camera.to_c()
+        arr[i].integrator = L.IntegratorParams(j.integrator.max_depth, j.integrator.min_rr_depth)
+        arr[i].width, arr[i].height, arr[i].spp, arr[i].spp_offset = j.width, j.height, j.spp, j.spp_offset
+        arr[i].out = _ptr(outs[i])
+        arr[i].accum64 = _ptr(accs[i]) if accum64 else None
+    p = L.BatchParams(pool_paths, trace_blocks_per_cu, stack_ring, 1 if autotune else 0)
+    st = L.BatchStats()
+    ctx.check(ctx.lib.pyre_render_batch(ctx.h, arr, n, C.byref(p), C.byref(st)), "pyre_render_batch")
+    stats = dict(extension_rays=int(st.extension_rays), shadow_rays=int(st.shadow_rays), passes=int(st.passes),
+                 gpu_ms=float(st.gpu_ms), pool_paths=int(st.pool_paths),
+                 trace_blocks_per_cu=int(st.trace_blocks_per_cu), stack_ring=int(st.stack_ring))
+    return outs, accs, stats

@@ -114,6 +114,7 @@ SIGNATURES = {
     "pyre_gen_random_rays": (C.c_int, [C.c_uint64, _u32, _u32, _vp, _vp]),
     "pyre_preset_cornell": (C.c_int, [C.c_int, C.POINTER(_vp)]),
     "pyre_preset_production": (C.c_int, [C.c_uint64, _u32, _u32, _u32, _u32, C.POINTER(_vp)]),
+    "pyre_preset_instances": (C.c_int, [C.c_uint64, _u32, _u32, _u32, _u32, _f, C.POINTER(_vp)]),
     "pyre_preset_desc": (C.c_int, [_vp, C.POINTER(SceneDesc)]),
     "pyre_preset_camera": (C.c_int, [_vp, _f, _f, C.POINTER(CameraDesc)]),
     "pyre_preset_destroy": (None, [_vp]),
@@ -176,4 +177,24 @@ SIGNATURES.update({
     "pyre_session_render_nrc": (C.c_int, [_vp, _vp, _vp, C.POINTER(NrcRenderParams), _vp]),
     "pyre_session_read": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(_u32)]),
     "pyre_session_reset": (C.c_int, [_vp, _vp]),
+})
+
+
+class Job(C.Structure):
+    _fields_ = [("scene", _vp), ("camera", CameraDesc), ("integrator", IntegratorParams), ("width", _u32),
+                ("height", _u32), ("spp", _u32), ("spp_offset", _u32), ("out", _vp), ("accum64", _vp)]
+
+
+class BatchParams(C.Structure):
+    _fields_ = [("pool_paths", _u32), ("trace_blocks_per_cu", _u32), ("stack_ring", _u32), ("autotune", _u32)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("passes", _u32), ("gpu_ms", _f),
+                ("pool_paths", _u32), ("trace_blocks_per_cu", _u32), ("stack_ring", _u32)]
+
+
+SIGNATURES.update({
+    "pyre_render_batch": (C.c_int, [_vp, C.POINTER(Job), C.c_size_t, C.POINTER(BatchParams),
+                                    C.POINTER(BatchStats)]),
 })

@@ -241,7 +241,13 @@ int pyre_preset_cornell(int motion_blur, pyre_preset** out) {
 
 int pyre_preset_production(uint64_t seed, uint32_t n_instances, uint32_t level, uint32_t env_w, uint32_t env_h,
                            pyre_preset** out) {
+  return pyre_preset_instances(seed, n_instances, level, env_w, env_h, 0.0f, out);
+}
+
+int pyre_preset_instances(uint64_t seed, uint32_t n_instances, uint32_t level, uint32_t env_w, uint32_t env_h,
+                          float lambert_fraction, pyre_preset** out) {
   if (!out || n_instances == 0 || level > 9 || env_w == 0 || env_h == 0) return PYRE_EINVAL;
+  if (!(lambert_fraction >= 0.0f && lambert_fraction <= 1.0f)) return PYRE_EINVAL;
   pyre_preset* p = new (std::nothrow) pyre_preset();
   if (!p) return PYRE_ENOMEM;
   HostRng g(seed);
@@ -263,7 +269,13 @@ int pyre_preset_production(uint64_t seed, uint32_t n_instances, uint32_t level, 
     double base[3] = {g.f64(), g.f64(), g.f64()};
     double rough = g.uniform(0.05, 1.0);
     double metal = g.f64() < 0.3 ? 1.0 : 0.0;
-    p->materials.push_back(disney(v3((float)base[0], (float)base[1], (float)base[2]), (float)metal, (float)rough, 0.5f));
+    const V3 bc = v3((float)base[0], (float)base[1], (float)base[2]);
+    // the Lambertian draw is taken only when asked for, so the Disney-only
+    // scene keeps its random stream
+    if (lambert_fraction > 0.0f && g.f64() < lambert_fraction)
+      p->materials.push_back(lambert(bc));
+    else
+      p->materials.push_back(disney(bc, (float)metal, (float)rough, 0.5f));
     Mesh m;
     m.material = i;
     m.pos.resize(3 * nv);

@@ -33,7 +33,9 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "nrc.hpp"
@@ -45,7 +47,6 @@ using namespace pyre;
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kRing = 8;
 constexpr uint32_t FL_SPECULAR = 1u, FL_AOV = 2u, FL_DIFFUSE = 4u;
 constexpr uint32_t MODE_PLAIN = 0u, MODE_TRAIN = 1u, MODE_CACHE = 2u;
 constexpr float kRayEps = 1e-3f;  // integrator.rs:34
@@ -66,12 +67,53 @@ struct SceneDev {
   EnvDev env;
 };
 
-// Per-pass parameters, read by every stage from device memory.
-struct PassParams {
-  uint32_t width, height, npix, s_base, n_samples, paths, max_depth, min_rr;
+// One render job: scene, camera, image and its sums (film.rs:38-109).
+struct JobDev {
+  SceneDev S;
+  CamDev cam;
+  uint32_t width, height, npix, max_depth, min_rr;
+  float* acc;     // 10 f32 sums per pixel (PixelSample order)
+  double* acc64;  // 3 FP64 radiance sums per pixel
+};
+
+// Samples [s_base, s_base + n_samples) of every pixel of one job, as the
+// pool's paths [path0, path0 + n_samples * npix) (sample-major) and the
+// accumulation items [pix0, pix0 + npix).
+struct SliceDev {
+  uint32_t job, s_base, n_samples, path0, pix0;
+};
+
+// One pass over the path pool: slices [slice0, slice0 + n_slices).
+struct PassDev {
+  uint32_t slice0, n_slices, paths, pixels;
   uint32_t mode, pix_mul, pix_add, vmax;
   float diffuse_rough;
 };
+
+// What every stage reads from device memory: the pass being run is
+// passes[*pass_idx], so captured passes replay with new contents.
+struct Sched {
+  const JobDev* jobs;
+  const SliceDev* slices;
+  const PassDev* passes;
+  const uint32_t* pass_idx;
+};
+
+__device__ __forceinline__ const PassDev& cur_pass(const Sched& sc) { return sc.passes[*sc.pass_idx]; }
+
+// Slice of pool index v: the last slice k of the pass with key(k) <= v.
+template <class Key>
+__device__ __forceinline__ uint32_t find_slice(const Sched& sc, const PassDev& pp, uint32_t v, Key key) {
+  uint32_t lo = pp.slice0, hi = pp.slice0 + pp.n_slices;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) / 2;
+    if (key(sc.slices[mid]) <= v)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
 
 struct Paths {
   float4* ray;       // 2 per path: (o, t_min), (d, t_max)
@@ -90,6 +132,7 @@ struct Paths {
   uint32_t* counters;  // [0] current, [1] next, [2] shadow, [3] cache queries, [4] training samples,
                        // [8..15] extend work, [16..23] shadow work
   unsigned long long* stats;  // [0] extension rays, [1] shadow rays
+  uint32_t* pjob;    // job of each path
   float4* vrec;      // TRAIN: kVrec float4 per (path, vertex)
   uint32_t* vcount;  // TRAIN: vertices recorded per path
   float* qfeat;      // CACHE: 13 floats per path (its query, if it ended in the cache)
@@ -122,12 +165,18 @@ __device__ __forceinline__ void store_rng(const Paths& P, uint32_t p, const Rng&
 __device__ __forceinline__ V3 xyz(float4 a) { return v3(a.x, a.y, a.z); }
 __device__ __forceinline__ float4 f4(V3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
 
-__global__ void k_set_pass(PassParams* pp, PassParams v) { *pp = v; }
+__global__ void k_set_pass(PassDev* pp, SliceDev* sl, PassDev v, SliceDev w) {
+  *pp = v;
+  *sl = w;
+}
+
+__global__ void k_set_u32(uint32_t* a, uint32_t v) { *a = v; }
+__global__ void k_advance(uint32_t* pass_idx) { *pass_idx += 1u; }
 
 // counters[8..15] / [16..23]: work counters of the extend / shadow slices
-__global__ void k_counters(uint32_t* c, const PassParams* pp) {
+__global__ void k_counters(uint32_t* c, Sched sc) {
   const uint32_t i = threadIdx.x;
-  if (i < kCounters) c[i] = i == 0 ? pp->paths : 0u;
+  if (i < kCounters) c[i] = i == 0 ? cur_pass(sc).paths : 0u;
 }
 
 // next queue becomes current; next and shadow queues and the work counters
@@ -145,22 +194,26 @@ __global__ void k_rotate(uint32_t* c, unsigned long long* stats) {
 }
 
 // main.rs:261-273 + camera.rs + integrator.rs:63-74 state
-__global__ __launch_bounds__(kThreads) void k_generate(Paths P, const PassParams* ppd, CamDev cam) {
-  const PassParams pp = *ppd;
+__global__ __launch_bounds__(kThreads) void k_generate(Paths P, Sched sc) {
+  const PassDev pp = cur_pass(sc);
   const uint32_t stride = gridDim.x * kThreads;
   for (uint32_t p = blockIdx.x * kThreads + threadIdx.x; p < pp.paths; p += stride) {
-    uint32_t pixel = (uint32_t)(((uint64_t)p * pp.pix_mul + pp.pix_add) % pp.npix);
-    uint32_t sl = p / pp.npix;
-    uint32_t x = pixel % pp.width, y = pixel / pp.width;
-    Rng rng = rng_seed(pixel_seed(x, y, pp.s_base + sl));
+    const SliceDev slc = sc.slices[find_slice(sc, pp, p, [](const SliceDev& q) { return q.path0; })];
+    const JobDev& J = sc.jobs[slc.job];
+    const uint32_t local = p - slc.path0;
+    const uint32_t npix = J.npix, width = J.width;
+    uint32_t pixel = (uint32_t)(((uint64_t)local * pp.pix_mul + pp.pix_add) % npix);
+    uint32_t sl = local / npix;
+    uint32_t x = pixel % width, y = pixel / width;
+    Rng rng = rng_seed(pixel_seed(x, y, slc.s_base + sl));
     float jx = rng_f32(rng);
     float jy = rng_f32(rng);
-    float ndc_x = 2.0f * ((float)x + jx) / (float)pp.width - 1.0f;
-    float ndc_y = 1.0f - 2.0f * ((float)y + jy) / (float)pp.height;
+    float ndc_x = 2.0f * ((float)x + jx) / (float)width - 1.0f;
+    float ndc_y = 1.0f - 2.0f * ((float)y + jy) / (float)J.height;
     V2 lens = rng_vec2(rng);
     (void)rng_f32(rng);  // shutter time (static scenes)
     V3 o, d;
-    camera_ray(cam, V2{ndc_x, ndc_y}, lens, &o, &d);
+    camera_ray(J.cam, V2{ndc_x, ndc_y}, lens, &o, &d);
     P.ray[2 * (size_t)p] = f4(o, 1e-4f);
     P.ray[2 * (size_t)p + 1] = f4(d, kInf);
     store_rng(P, p, rng);
@@ -168,6 +221,7 @@ __global__ __launch_bounds__(kThreads) void k_generate(Paths P, const PassParams
     P.lum[p] = make_float4(0.0f, 0.0f, 0.0f, __builtin_bit_cast(float, FL_SPECULAR));
     P.aov0[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     P.aov1[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    P.pjob[p] = slc.job;
     if (pp.mode == MODE_TRAIN) P.vcount[p] = 0;
     P.q_cur[p] = p;
   }
@@ -175,26 +229,32 @@ __global__ __launch_bounds__(kThreads) void k_generate(Paths P, const PassParams
 
 // Closest hit for every queued path, lanes refilled from the queue as their
 // rays finish.
-__global__ __launch_bounds__(kThreads) void k_extend(Paths P, SceneDev S, int2* gstack) {
-  __shared__ int2 ring[kRing * kThreads];
+template <int RING>
+__global__ __launch_bounds__(kThreads) void k_extend(Paths P, const JobDev* __restrict__ jobs, int2* gstack) {
+  __shared__ int2 ring[RING * kThreads];
   const uint32_t tid = blockIdx.x * kThreads + threadIdx.x;
-  HybridStack<kRing> st{ring + threadIdx.x, kThreads, gstack + tid, gridDim.x * kThreads, 0, 0};
+  HybridStack<RING> st{ring + threadIdx.x, kThreads, gstack + tid, gridDim.x * kThreads, 0, 0};
   WaveFeed feed(P.counters[0], P.counters + 8);
-  Trav4<false, kRing> tv;
+  Trav4<false, RING> tv;
   uint32_t p = 0, item = 0;
+  const Node4* nodes = nullptr;
+  const float4* tris = nullptr;
   bool active = false;
   for (;;) {
     const uint64_t idle = __ballot(!active);
     if (__popcll(idle) >= kRefillLanes) {
       if (feed.take(idle, !active, item)) {
         p = P.q_cur[item];
+        const JobDev& J = jobs[P.pjob[p]];
+        nodes = J.S.nodes4;
+        tris = J.S.tris;
         const float4 a = P.ray[2 * (size_t)p], b = P.ray[2 * (size_t)p + 1];
         tv.start(make_tray(xyz(a), xyz(b), a.w, b.w), st);
         active = true;
       }
       if (!__any(active)) break;
     }
-    if (active && tv.step(S.nodes4, S.tris, st)) {
+    if (active && tv.step(nodes, tris, st)) {
       THit h;
       tv.result(&h);
       P.hit[p] = make_float4(__builtin_bit_cast(float, h.gid), h.t, h.u, h.v);
@@ -206,14 +266,17 @@ __global__ __launch_bounds__(kThreads) void k_extend(Paths P, SceneDev S, int2* 
 // Any-hit for every shadow slot of the queued paths; a lane keeps a path
 // until all its slots are traced so unoccluded contributions are added to the
 // path radiance in slot order (lights, then environment).
-__global__ __launch_bounds__(kThreads) void k_shadow(Paths P, SceneDev S, int2* gstack) {
-  __shared__ int2 ring[kRing * kThreads];
+template <int RING>
+__global__ __launch_bounds__(kThreads) void k_shadow(Paths P, const JobDev* __restrict__ jobs, int2* gstack) {
+  __shared__ int2 ring[RING * kThreads];
   const uint32_t tid = blockIdx.x * kThreads + threadIdx.x;
-  HybridStack<kRing> st{ring + threadIdx.x, kThreads, gstack + tid, gridDim.x * kThreads, 0, 0};
+  HybridStack<RING> st{ring + threadIdx.x, kThreads, gstack + tid, gridDim.x * kThreads, 0, 0};
   WaveFeed feed(P.counters[2], P.counters + 16);
-  Trav4<true, kRing> tv;
+  Trav4<true, RING> tv;
   tv.cur = -1;  // -1: no traversal in progress for the current slot
   uint32_t p = 0, k = 0, cnt = 0, item = 0;
+  const Node4* nodes = nullptr;
+  const float4* tris = nullptr;
   float4 L = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   bool active = false;
   for (;;) {
@@ -221,6 +284,9 @@ __global__ __launch_bounds__(kThreads) void k_shadow(Paths P, SceneDev S, int2* 
     if (__popcll(idle) >= kRefillLanes) {
       if (feed.take(idle, !active, item)) {
         p = P.q_shadow[item];
+        const JobDev& J = jobs[P.pjob[p]];
+        nodes = J.S.nodes4;
+        tris = J.S.tris;
         cnt = P.scount[p];
         k = 0;
         L = P.lum[p];
@@ -240,7 +306,7 @@ __global__ __launch_bounds__(kThreads) void k_shadow(Paths P, SceneDev S, int2* 
         if (b.w > a.w) tv.start(make_tray(xyz(a), xyz(b), a.w, b.w), st);
       }
       if (tv.cur >= 0) {
-        if (tv.step(S.nodes4, S.tris, st)) {
+        if (tv.step(nodes, tris, st)) {
           THit h;
           tv.result(&h);
           occ = h.gid != PYRE_MISS;
@@ -298,8 +364,9 @@ struct ShadeOut {
 };
 
 // One bounce of integrator.rs:76-222 for path p.
-__device__ void shade_path(const Paths& P, const SceneDev& S, const PassParams& pp, uint32_t depth, uint32_t p,
+__device__ void shade_path(const Paths& P, const JobDev& J, const PassDev& pp, uint32_t depth, uint32_t p,
                            ShadeOut& o) {
+  const SceneDev& S = J.S;
   float4 ra = P.ray[2 * (size_t)p], rb = P.ray[2 * (size_t)p + 1];
   V3 ro = xyz(ra), rd = xyz(rb);
   float r_tmin = ra.w;
@@ -472,7 +539,7 @@ __device__ void shade_path(const Paths& P, const SceneDev& S, const PassParams& 
   if (alive && cos_wi == 0.0f) alive = false;
   if (alive) {
     beta = beta * (bf * cos_wi / bpdf);
-    if (depth >= pp.min_rr) {  // integrator.rs:203-209
+    if (depth >= J.min_rr) {  // integrator.rs:203-209
       float q = rclamp(1.0f - max_element(beta), 0.05f, 0.95f);
       if (rng_f32(rng) < q)
         alive = false;
@@ -487,14 +554,14 @@ __device__ void shade_path(const Paths& P, const SceneDev& S, const PassParams& 
     P.ray[2 * (size_t)p] = f4(pos + kRayEps * ns, 1e-4f);
     P.ray[2 * (size_t)p + 1] = f4(wi_world, kInf);
     P.beta_pdf[p] = f4(beta, bpdf);
-    o.cont = depth + 1 < pp.max_depth;
+    o.cont = depth + 1 < J.max_depth;
   }
   store_rng(P, p, rng);
   P.lum[p] = f4(l, __builtin_bit_cast(float, flags));
 }
 
-__global__ __launch_bounds__(kThreads) void k_shade(Paths P, SceneDev S, const PassParams* ppd, uint32_t depth) {
-  const PassParams pp = *ppd;
+__global__ __launch_bounds__(kThreads) void k_shade(Paths P, Sched sc, uint32_t depth) {
+  const PassDev pp = cur_pass(sc);
   const uint32_t n = P.counters[0];
   const uint32_t stride = gridDim.x * kThreads;
   // uniform trip count per wave so the ballots below see converged lanes
@@ -507,7 +574,7 @@ __global__ __launch_bounds__(kThreads) void k_shade(Paths P, SceneDev S, const P
     uint32_t p = 0;
     if (i < n) {
       p = P.q_cur[i];
-      shade_path(P, S, pp, depth, p, o);
+      shade_path(P, sc.jobs[P.pjob[p]], pp, depth, p, o);
     }
     uint32_t nsh = o.n_shadow;
     for (int k = 32; k > 0; k >>= 1) nsh += __shfl_xor(nsh, k);
@@ -526,15 +593,21 @@ __global__ __launch_bounds__(kThreads) void k_shade(Paths P, SceneDev S, const P
 }
 
 // main.rs:275-285 in sample order; FP64 radiance sums beside the f32 ones.
-__global__ __launch_bounds__(kThreads) void k_accumulate(Paths P, const PassParams* ppd, float* acc, double* acc64) {
-  const PassParams pp = *ppd;
+// One item per (slice, pixel): the slice's samples of that pixel in order.
+__global__ __launch_bounds__(kThreads) void k_accumulate(Paths P, Sched sc) {
+  const PassDev pp = cur_pass(sc);
   const uint32_t stride = gridDim.x * kThreads;
-  for (uint32_t px = blockIdx.x * kThreads + threadIdx.x; px < pp.npix; px += stride) {
+  for (uint32_t it = blockIdx.x * kThreads + threadIdx.x; it < pp.pixels; it += stride) {
+    const SliceDev slc = sc.slices[find_slice(sc, pp, it, [](const SliceDev& q) { return q.pix0; })];
+    const JobDev& J = sc.jobs[slc.job];
+    const uint32_t px = it - slc.pix0;
+    float* acc = J.acc + 10 * (size_t)px;
+    double* acc64 = J.acc64 + 3 * (size_t)px;
     float a[10];
-    for (int k = 0; k < 10; ++k) a[k] = acc[10 * (size_t)px + k];
-    double d0 = acc64[3 * (size_t)px], d1 = acc64[3 * (size_t)px + 1], d2 = acc64[3 * (size_t)px + 2];
-    for (uint32_t sl = 0; sl < pp.n_samples; ++sl) {
-      uint32_t p = sl * pp.npix + px;
+    for (int k = 0; k < 10; ++k) a[k] = acc[k];
+    double d0 = acc64[0], d1 = acc64[1], d2 = acc64[2];
+    for (uint32_t sl = 0; sl < slc.n_samples; ++sl) {
+      uint32_t p = slc.path0 + sl * J.npix + px;
       float4 L = P.lum[p], A = P.aov0[p], N = P.aov1[p];
       a[0] = a[0] + L.x;
       a[1] = a[1] + L.y;
@@ -550,18 +623,18 @@ __global__ __launch_bounds__(kThreads) void k_accumulate(Paths P, const PassPara
       d1 += (double)L.y;
       d2 += (double)L.z;
     }
-    for (int k = 0; k < 10; ++k) acc[10 * (size_t)px + k] = a[k];
-    acc64[3 * (size_t)px] = d0;
-    acc64[3 * (size_t)px + 1] = d1;
-    acc64[3 * (size_t)px + 2] = d2;
+    for (int k = 0; k < 10; ++k) acc[k] = a[k];
+    acc64[0] = d0;
+    acc64[1] = d1;
+    acc64[2] = d2;
   }
 }
 
 // TRAIN: per recorded vertex, target = (L_end - L_before) / beta, compacted
 // into (feats, targets) up to cap samples.
-__global__ __launch_bounds__(kThreads) void k_train_targets(Paths P, const PassParams* ppd, float* feats,
-                                                            float* targets, uint32_t cap) {
-  const PassParams pp = *ppd;
+__global__ __launch_bounds__(kThreads) void k_train_targets(Paths P, Sched sc, float* feats, float* targets,
+                                                            uint32_t cap) {
+  const PassDev pp = cur_pass(sc);
   const uint32_t total = pp.paths * pp.vmax;
   const uint32_t stride = gridDim.x * kThreads;
   const uint32_t base = blockIdx.x * kThreads + (threadIdx.x & ~63u);
@@ -629,12 +702,35 @@ __global__ void k_fill_u64(unsigned long long* a, size_t n) {
 
 inline unsigned nblocks(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
-// Device path state for up to `cap` paths.
+// Launch configuration of the path stages: paths per pass (queue capacity),
+// persistent traversal blocks per CU and the LDS part of the traversal stack
+// (ring entries per thread). Results do not depend on it.
+struct Tuning {
+  uint32_t pool_paths = 1u << 21;
+  uint32_t blocks_per_cu = 8;
+  uint32_t ring = 8;
+};
+
+Tuning env_tuning() {
+  Tuning t;
+  if (const char* e = std::getenv("PYRE_PASS_PATHS")) t.pool_paths = (uint32_t)std::max(1, std::atoi(e));
+  if (const char* e = std::getenv("PYRE_TRACE_BLOCKS_PER_CU")) t.blocks_per_cu = (uint32_t)std::max(1, std::atoi(e));
+  if (const char* e = std::getenv("PYRE_STACK_RING")) t.ring = (uint32_t)std::atoi(e);
+  if (t.ring != 4 && t.ring != 16) t.ring = 8;
+  return t;
+}
+
+// Device path state for up to `cap` paths, plus the job/slice/pass tables.
 struct Work {
   size_t cap = 0;
   uint32_t slots = 1, vmax = 0, tcap = 0, tpaths = 0;  // tpaths: training paths with vertex records
   Paths P{};
-  PassParams* ppd = nullptr;
+  JobDev* jobs = nullptr;
+  SliceDev* slices = nullptr;
+  PassDev* passes = nullptr;
+  uint32_t* pass_idx = nullptr;
+  int2* gstack = nullptr;  // traversal stack spill columns
+  size_t gstack_cap = 0;
   std::vector<void*> bufs;
   float* tfeat = nullptr;
   float* ttarget = nullptr;
@@ -644,7 +740,8 @@ struct Work {
     bufs.push_back(p);
     return p;
   }
-  void alloc(size_t cap_, uint32_t slots_, uint32_t vmax_, uint32_t tpaths_, uint32_t tcap_, bool cache) {
+  void alloc(size_t cap_, uint32_t slots_, uint32_t vmax_, uint32_t tpaths_, uint32_t tcap_, bool cache,
+             size_t n_jobs, size_t n_slices, size_t n_passes) {
     cap = cap_;
     slots = slots_;
     vmax = vmax_;
@@ -663,6 +760,7 @@ struct Work {
     P.q_cur = get<uint32_t>(cap);
     P.q_next = get<uint32_t>(cap);
     P.q_shadow = get<uint32_t>(cap);
+    P.pjob = get<uint32_t>(cap);
     P.counters = get<uint32_t>(kCounters);
     P.stats = get<unsigned long long>(2);
     P.slots = slots;
@@ -677,8 +775,25 @@ struct Work {
       P.qpath = get<uint32_t>(cap);
       P.qrgb = get<float>(3 * cap);
     }
-    ppd = get<PassParams>(1);
+    jobs = get<JobDev>(n_jobs);
+    slices = get<SliceDev>(n_slices);
+    passes = get<PassDev>(n_passes);
+    pass_idx = get<uint32_t>(1);
   }
+  // Spill columns for `threads` traversal threads on trees of binary depth
+  // `depth` (3 pending children per level); allocated before any capture.
+  void ensure_gstack(size_t threads, uint32_t depth) {
+    const size_t need = threads * (3 * (size_t)depth + 4);
+    if (need <= gstack_cap) return;
+    PYRE_HIP_CHECK(hipDeviceSynchronize());
+    if (gstack) {
+      bufs.erase(std::find(bufs.begin(), bufs.end(), (void*)gstack));
+      (void)hipFree(gstack);
+    }
+    gstack = get<int2>(need);
+    gstack_cap = need;
+  }
+  Sched sched() const { return Sched{jobs, slices, passes, pass_idx}; }
   ~Work() {
     for (void* p : bufs) (void)hipFree(p);
   }
@@ -689,29 +804,68 @@ SceneDev scene_dev(pyre_scene* s) {
                   s->d_lights, s->n_meshes, s->n_lights, s->has_env ? 1u : 0u, s->env};
 }
 
-unsigned stage_grid(const pyre_ctx* ctx) { return (unsigned)std::max(1, ctx->cus) * 8; }
+JobDev job_dev(pyre_scene* s, const pyre_camera_desc& cam, const pyre_integrator_params& ip, uint32_t w, uint32_t h,
+               float* acc, double* acc64) {
+  JobDev J{};
+  J.S = scene_dev(s);
+  J.cam = make_camera(cam);
+  J.width = w;
+  J.height = h;
+  J.npix = w * h;
+  J.max_depth = ip.max_depth;
+  J.min_rr = ip.min_rr_depth;
+  J.acc = acc;
+  J.acc64 = acc64;
+  return J;
+}
 
-// All stages of one pass on the stream; the pass parameters are already in
-// W.ppd. acc == nullptr skips the accumulation.
-void launch_pass(pyre_ctx* ctx, pyre_scene* s, const Work& W, const CamDev& cam, uint32_t max_depth, float* acc,
-                 double* acc64, hipStream_t st) {
+__global__ void k_set_job(JobDev* dst, JobDev v) { *dst = v; }
+
+unsigned stage_grid(const pyre_ctx* ctx) { return (unsigned)std::max(1, ctx->cus) * 8; }
+unsigned trace_grid_of(const pyre_ctx* ctx, const Tuning& t) {
+  return (unsigned)std::max(1, ctx->cus) * t.blocks_per_cu;
+}
+
+// What a pass needs to know about its jobs: deepest path, whether any job
+// has lights or an environment (shadow stage), deepest tree.
+struct PassShape {
+  uint32_t max_depth = 0, tree_depth = 0;
+  bool nee = false;
+};
+
+template <int RING>
+void launch_trace_stages(const Work& W, Paths P, unsigned tgrid, bool shadow, hipStream_t st, bool extend) {
+  if (extend) hipLaunchKernelGGL((k_extend<RING>), dim3(tgrid), dim3(kThreads), 0, st, P, (const JobDev*)W.jobs, W.gstack);
+  if (shadow) hipLaunchKernelGGL((k_shadow<RING>), dim3(tgrid), dim3(kThreads), 0, st, P, (const JobDev*)W.jobs, W.gstack);
+}
+
+void launch_trace(const Work& W, const Paths& P, const Tuning& t, unsigned tgrid, bool extend, bool shadow,
+                  hipStream_t st) {
+  if (t.ring == 4)
+    launch_trace_stages<4>(W, P, tgrid, shadow, st, extend);
+  else if (t.ring == 16)
+    launch_trace_stages<16>(W, P, tgrid, shadow, st, extend);
+  else
+    launch_trace_stages<8>(W, P, tgrid, shadow, st, extend);
+}
+
+// All stages of the pass passes[*pass_idx] on the stream; accumulate = false
+// skips the film update.
+void launch_pass(pyre_ctx* ctx, Work& W, const PassShape& shape, const Tuning& t, bool accumulate, hipStream_t st) {
   const unsigned grid = stage_grid(ctx);
-  const unsigned tgrid = trace_grid(s);
-  int2* gst = trace_gstack(s);
-  const SceneDev S = scene_dev(s);
-  const PassParams* ppd = W.ppd;
+  const unsigned tgrid = trace_grid_of(ctx, t);
+  const Sched sc = W.sched();
   Paths P = W.P;
-  hipLaunchKernelGGL(k_counters, dim3(1), dim3(kCounters), 0, st, P.counters, ppd);
-  hipLaunchKernelGGL(k_generate, dim3(grid), dim3(kThreads), 0, st, P, ppd, cam);
-  const bool nee = s->n_lights + (s->has_env ? 1u : 0u) > 0;
-  for (uint32_t d = 0; d < max_depth; ++d) {
-    hipLaunchKernelGGL(k_extend, dim3(tgrid), dim3(kThreads), 0, st, P, S, gst);
-    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kThreads), 0, st, P, S, ppd, d);
-    if (nee) hipLaunchKernelGGL(k_shadow, dim3(tgrid), dim3(kThreads), 0, st, P, S, gst);
+  hipLaunchKernelGGL(k_counters, dim3(1), dim3(kCounters), 0, st, P.counters, sc);
+  hipLaunchKernelGGL(k_generate, dim3(grid), dim3(kThreads), 0, st, P, sc);
+  for (uint32_t d = 0; d < shape.max_depth; ++d) {
+    launch_trace(W, P, t, tgrid, true, false, st);
+    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kThreads), 0, st, P, sc, d);
+    if (shape.nee) launch_trace(W, P, t, tgrid, false, true, st);
     hipLaunchKernelGGL(k_rotate, dim3(1), dim3(kCounters), 0, st, P.counters, P.stats);
     std::swap(P.q_cur, P.q_next);
   }
-  if (acc) hipLaunchKernelGGL(k_accumulate, dim3(grid), dim3(kThreads), 0, st, P, ppd, acc, acc64);
+  if (accumulate) hipLaunchKernelGGL(k_accumulate, dim3(grid), dim3(kThreads), 0, st, P, sc);
 }
 
 bool graphs_enabled() {
@@ -719,146 +873,16 @@ bool graphs_enabled() {
   return !(e && std::strcmp(e, "0") == 0);
 }
 
-}  // namespace
-
-struct pyre_session {
-  pyre_ctx* ctx = nullptr;
-  pyre_scene* s = nullptr;
-  CamDev cam{};
-  pyre_integrator_params ip{};
-  uint32_t width = 0, height = 0, npix = 0, per_pass = 1;
-  Work W;
-  float* acc = nullptr;
-  double* acc64 = nullptr;
-  uint32_t spp_done = 0, next_sample = 0, train_frames = 0;
-  bool use_graph = true;
+hipGraphExec_t capture(hipStream_t st, const std::function<void()>& body) {
+  hipGraph_t g = nullptr;
+  PYRE_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  body();
+  PYRE_HIP_CHECK(hipStreamEndCapture(st, &g));
   hipGraphExec_t exec = nullptr;
-  ~pyre_session() {
-    if (exec) (void)hipGraphExecDestroy(exec);
-    dev_free(acc);
-    dev_free(acc64);
-  }
-};
-
-namespace {
-
-PassParams pass_params(const pyre_session* S, uint32_t s_base, uint32_t n_samples, uint32_t mode, float diffuse_rough) {
-  PassParams pp{};
-  pp.width = S->width;
-  pp.height = S->height;
-  pp.npix = S->npix;
-  pp.s_base = s_base;
-  pp.n_samples = n_samples;
-  pp.paths = n_samples * S->npix;
-  pp.max_depth = S->ip.max_depth;
-  pp.min_rr = S->ip.min_rr_depth;
-  pp.mode = mode;
-  pp.pix_mul = 1;
-  pp.pix_add = 0;
-  pp.vmax = S->W.vmax;
-  pp.diffuse_rough = diffuse_rough;
-  return pp;
-}
-
-// Samples [s_base, s_base + n_samples) of every pixel. PLAIN passes of the
-// full pass size replay one captured graph; the rest launch eagerly.
-void session_pass(pyre_session* S, uint32_t s_base, uint32_t n_samples, uint32_t mode, pyre_nrc* nrc,
-                  float diffuse_rough, bool accumulate) {
-  hipStream_t st = S->ctx->stream;
-  hipLaunchKernelGGL(k_set_pass, dim3(1), dim3(1), 0, st, S->W.ppd,
-                     pass_params(S, s_base, n_samples, mode, diffuse_rough));
-  if (mode == MODE_PLAIN && accumulate && n_samples == S->per_pass && S->use_graph) {
-    if (!S->exec) {
-      trace_gstack(S->s);  // any allocation happens before the capture
-      hipGraph_t g = nullptr;
-      PYRE_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      launch_pass(S->ctx, S->s, S->W, S->cam, S->ip.max_depth, S->acc, S->acc64, st);
-      PYRE_HIP_CHECK(hipStreamEndCapture(st, &g));
-      hipError_t e = hipGraphInstantiate(&S->exec, g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
-      PYRE_HIP_CHECK(e);
-    }
-    PYRE_HIP_CHECK(hipGraphLaunch(S->exec, st));
-    return;
-  }
-  if (mode == MODE_CACHE) {
-    require(nrc != nullptr, "cache pass without a radiance cache");
-    launch_pass(S->ctx, S->s, S->W, S->cam, S->ip.max_depth, nullptr, nullptr, st);
-    nrc_infer_dev_count(nrc, S->W.P.qfeat, S->W.P.qpath, S->W.P.counters + 3, (uint32_t)S->W.cap, S->W.P.qrgb, st);
-    hipLaunchKernelGGL(k_cache_apply, dim3(stage_grid(S->ctx)), dim3(kThreads), 0, st, S->W.P);
-    if (accumulate)
-      hipLaunchKernelGGL(k_accumulate, dim3(stage_grid(S->ctx)), dim3(kThreads), 0, st, S->W.P,
-                         (const PassParams*)S->W.ppd, S->acc, S->acc64);
-    return;
-  }
-  launch_pass(S->ctx, S->s, S->W, S->cam, S->ip.max_depth, accumulate ? S->acc : nullptr, S->acc64, st);
-}
-
-void clear_sums(pyre_session* S) {
-  hipStream_t st = S->ctx->stream;
-  const size_t n10 = 10 * (size_t)S->npix, n3 = 3 * (size_t)S->npix;
-  hipLaunchKernelGGL(k_fill_f32, dim3(nblocks(n10, 256)), dim3(256), 0, st, S->acc, n10, 0.0f);
-  hipLaunchKernelGGL(k_fill_f64, dim3(nblocks(n3, 256)), dim3(256), 0, st, S->acc64, n3, 0.0);
-  S->spp_done = 0;
-}
-
-void session_init(pyre_session* S, pyre_ctx* ctx, pyre_scene* s, const pyre_camera_desc& cam,
-                  const pyre_integrator_params& ip, uint32_t w, uint32_t h, uint32_t train_paths) {
-  require(w > 0 && h > 0, "width and height must be positive");
-  require(s->built, "render before pyre_bvh_build");
-  require((uint64_t)w * h < (1ull << 28), "image too large");
-  require(train_paths < (1u << 24), "too many training paths");
-  S->ctx = ctx;
-  S->s = s;
-  S->cam = make_camera(cam);
-  S->ip = ip;
-  S->width = w;
-  S->height = h;
-  S->npix = w * h;
-  // paths per pass: whole samples of the image, about 2M paths (PYRE_PASS_PATHS overrides)
-  uint32_t target = 1u << 21;
-  if (const char* e = std::getenv("PYRE_PASS_PATHS")) target = (uint32_t)std::max(1, std::atoi(e));
-  S->per_pass = std::max(1u, target / S->npix);
-  const size_t cap = std::max((size_t)S->per_pass * S->npix, (size_t)train_paths);
-  require(cap < (1ull << 31), "path pool too large");
-  const uint32_t slots = std::max(1u, s->n_lights + (s->has_env ? 1u : 0u));
-  const uint32_t vmax = train_paths ? std::max(1u, ip.max_depth) : 0;
-  S->W.alloc(cap, slots, vmax, train_paths, train_paths ? std::min<uint32_t>(train_paths * vmax, 1u << 20) : 0,
-             train_paths > 0);
-  S->acc = dev_alloc<float>(10 * (size_t)S->npix);
-  S->acc64 = dev_alloc<double>(3 * (size_t)S->npix);
-  clear_sums(S);
-  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, S->W.P.stats, (size_t)2);
-  S->use_graph = graphs_enabled();
-}
-
-void session_render(pyre_session* S, uint32_t spp) {
-  for (uint32_t s0 = 0; s0 < spp; s0 += S->per_pass) {
-    uint32_t ns = std::min(S->per_pass, spp - s0);
-    session_pass(S, S->next_sample, ns, MODE_PLAIN, nullptr, 0.25f, true);
-    S->next_sample += ns;
-    S->spp_done += ns;
-  }
-  PYRE_HIP_CHECK(hipGetLastError());
-}
-
-void finish_stats(pyre_session* S) {
-  unsigned long long hs[2] = {0, 0};
-  PYRE_HIP_CHECK(hipMemcpyAsync(hs, S->W.P.stats, 16, hipMemcpyDeviceToHost, S->ctx->stream));
-  PYRE_HIP_CHECK(hipStreamSynchronize(S->ctx->stream));
-  S->s->last_extension_rays = hs[0];
-  S->s->last_shadow_rays = hs[1];
-}
-
-void read_mean(pyre_session* S, pyre_pixel_sample* out, float inv_spp) {
-  hipStream_t st = S->ctx->stream;
-  const size_t n10 = 10 * (size_t)S->npix;
-  float* o = dev_alloc<float>(n10);
-  hipLaunchKernelGGL(k_scale, dim3(nblocks(n10, 256)), dim3(256), 0, st, S->acc, n10, inv_spp, o);
-  hipError_t e = hipMemcpyAsync(out, o, n10 * sizeof(float), hipMemcpyDeviceToHost, st);
-  if (e == hipSuccess) e = hipStreamSynchronize(st);
-  dev_free(o);
+  hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
   PYRE_HIP_CHECK(e);
+  return exec;
 }
 
 template <class F>
@@ -875,6 +899,299 @@ int sguard(pyre_ctx* ctx, F&& f) {
     ctx->err = e.what();
     return PYRE_EINVAL;
   }
+}
+
+}  // namespace
+
+struct pyre_session {
+  pyre_ctx* ctx = nullptr;
+  pyre_scene* s = nullptr;
+  pyre_integrator_params ip{};
+  uint32_t width = 0, height = 0, npix = 0, per_pass = 1;
+  Tuning tune;
+  PassShape shape;
+  Work W;
+  float* acc = nullptr;
+  double* acc64 = nullptr;
+  uint32_t spp_done = 0, next_sample = 0, train_frames = 0;
+  bool use_graph = true;
+  hipGraphExec_t exec = nullptr;
+  ~pyre_session() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    dev_free(acc);
+    dev_free(acc64);
+  }
+};
+
+namespace {
+
+void set_pass(pyre_session* S, uint32_t s_base, uint32_t n_samples, uint32_t mode, float diffuse_rough,
+              uint32_t paths = 0, uint32_t pix_mul = 1, uint32_t pix_add = 0) {
+  PassDev pp{};
+  pp.slice0 = 0;
+  pp.n_slices = 1;
+  pp.paths = paths ? paths : n_samples * S->npix;
+  pp.pixels = S->npix;
+  pp.mode = mode;
+  pp.pix_mul = pix_mul;
+  pp.pix_add = pix_add;
+  pp.vmax = S->W.vmax;
+  pp.diffuse_rough = diffuse_rough;
+  SliceDev sl{0u, s_base, n_samples, 0u, 0u};
+  hipLaunchKernelGGL(k_set_pass, dim3(1), dim3(1), 0, S->ctx->stream, S->W.passes, S->W.slices, pp, sl);
+}
+
+// Samples [s_base, s_base + n_samples) of every pixel. PLAIN passes of the
+// full pass size replay one captured graph; the rest launch eagerly.
+void session_pass(pyre_session* S, uint32_t s_base, uint32_t n_samples, uint32_t mode, pyre_nrc* nrc,
+                  float diffuse_rough, bool accumulate) {
+  hipStream_t st = S->ctx->stream;
+  set_pass(S, s_base, n_samples, mode, diffuse_rough);
+  if (mode == MODE_PLAIN && accumulate && n_samples == S->per_pass && S->use_graph) {
+    if (!S->exec) S->exec = capture(st, [&] { launch_pass(S->ctx, S->W, S->shape, S->tune, true, st); });
+    PYRE_HIP_CHECK(hipGraphLaunch(S->exec, st));
+    return;
+  }
+  if (mode == MODE_CACHE) {
+    require(nrc != nullptr, "cache pass without a radiance cache");
+    launch_pass(S->ctx, S->W, S->shape, S->tune, false, st);
+    nrc_infer_dev_count(nrc, S->W.P.qfeat, S->W.P.qpath, S->W.P.counters + 3, (uint32_t)S->W.cap, S->W.P.qrgb, st);
+    hipLaunchKernelGGL(k_cache_apply, dim3(stage_grid(S->ctx)), dim3(kThreads), 0, st, S->W.P);
+    if (accumulate)
+      hipLaunchKernelGGL(k_accumulate, dim3(stage_grid(S->ctx)), dim3(kThreads), 0, st, S->W.P, S->W.sched());
+    return;
+  }
+  launch_pass(S->ctx, S->W, S->shape, S->tune, accumulate, st);
+}
+
+void clear_sums(pyre_ctx* ctx, float* acc, double* acc64, size_t npix) {
+  hipStream_t st = ctx->stream;
+  hipLaunchKernelGGL(k_fill_f32, dim3(nblocks(10 * npix, 256)), dim3(256), 0, st, acc, 10 * npix, 0.0f);
+  hipLaunchKernelGGL(k_fill_f64, dim3(nblocks(3 * npix, 256)), dim3(256), 0, st, acc64, 3 * npix, 0.0);
+}
+
+void session_init(pyre_session* S, pyre_ctx* ctx, pyre_scene* s, const pyre_camera_desc& cam,
+                  const pyre_integrator_params& ip, uint32_t w, uint32_t h, uint32_t train_paths) {
+  require(w > 0 && h > 0, "width and height must be positive");
+  require(s->built, "render before pyre_bvh_build");
+  require((uint64_t)w * h < (1ull << 28), "image too large");
+  require(train_paths < (1u << 24), "too many training paths");
+  S->ctx = ctx;
+  S->s = s;
+  S->ip = ip;
+  S->width = w;
+  S->height = h;
+  S->npix = w * h;
+  S->tune = env_tuning();
+  // paths per pass: whole samples of the image, about pool_paths paths
+  S->per_pass = std::max(1u, S->tune.pool_paths / S->npix);
+  const size_t cap = std::max((size_t)S->per_pass * S->npix, (size_t)train_paths);
+  require(cap < (1ull << 31), "path pool too large");
+  const uint32_t slots = std::max(1u, s->n_lights + (s->has_env ? 1u : 0u));
+  const uint32_t vmax = train_paths ? std::max(1u, ip.max_depth) : 0;
+  S->W.alloc(cap, slots, vmax, train_paths, train_paths ? std::min<uint32_t>(train_paths * vmax, 1u << 20) : 0,
+             train_paths > 0, 1, 1, 1);
+  S->W.ensure_gstack((size_t)trace_grid_of(ctx, S->tune) * kThreads, s->depth);
+  S->shape.max_depth = ip.max_depth;
+  S->shape.tree_depth = s->depth;
+  S->shape.nee = s->n_lights + (s->has_env ? 1u : 0u) > 0;
+  S->acc = dev_alloc<float>(10 * (size_t)S->npix);
+  S->acc64 = dev_alloc<double>(3 * (size_t)S->npix);
+  clear_sums(ctx, S->acc, S->acc64, S->npix);
+  S->spp_done = 0;
+  hipStream_t st = ctx->stream;
+  hipLaunchKernelGGL(k_set_job, dim3(1), dim3(1), 0, st, S->W.jobs, job_dev(s, cam, ip, w, h, S->acc, S->acc64));
+  hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, st, S->W.pass_idx, 0u);
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, S->W.P.stats, (size_t)2);
+  S->use_graph = graphs_enabled();
+}
+
+void session_render(pyre_session* S, uint32_t spp) {
+  for (uint32_t s0 = 0; s0 < spp; s0 += S->per_pass) {
+    uint32_t ns = std::min(S->per_pass, spp - s0);
+    session_pass(S, S->next_sample, ns, MODE_PLAIN, nullptr, 0.25f, true);
+    S->next_sample += ns;
+    S->spp_done += ns;
+  }
+  PYRE_HIP_CHECK(hipGetLastError());
+}
+
+void read_stats(pyre_ctx* ctx, const Work& W, uint64_t* ext, uint64_t* shadow) {
+  unsigned long long hs[2] = {0, 0};
+  PYRE_HIP_CHECK(hipMemcpyAsync(hs, W.P.stats, 16, hipMemcpyDeviceToHost, ctx->stream));
+  PYRE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  *ext = hs[0];
+  *shadow = hs[1];
+}
+
+void finish_stats(pyre_session* S) { read_stats(S->ctx, S->W, &S->s->last_extension_rays, &S->s->last_shadow_rays); }
+
+void read_mean(pyre_ctx* ctx, const float* acc, size_t npix, pyre_pixel_sample* out, float inv_spp) {
+  hipStream_t st = ctx->stream;
+  const size_t n10 = 10 * npix;
+  float* o = dev_alloc<float>(n10);
+  hipLaunchKernelGGL(k_scale, dim3(nblocks(n10, 256)), dim3(256), 0, st, acc, n10, inv_spp, o);
+  hipError_t e = hipMemcpyAsync(out, o, n10 * sizeof(float), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  dev_free(o);
+  PYRE_HIP_CHECK(e);
+}
+
+// ------------------------------------------------------------ batch
+// Passes over a shared path pool: jobs sorted by work (largest first), each
+// pass filled greedily with whole samples of the jobs that still have
+// samples left (at most one slice per job and pass).
+struct Schedule {
+  std::vector<SliceDev> slices;
+  std::vector<PassDev> passes;
+};
+
+Schedule make_schedule(const pyre_job* jobs, size_t n, uint32_t pool) {
+  std::vector<size_t> order(n);
+  for (size_t i = 0; i < n; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+    return (uint64_t)jobs[a].width * jobs[a].height * jobs[a].spp > (uint64_t)jobs[b].width * jobs[b].height * jobs[b].spp;
+  });
+  std::vector<uint32_t> left(n), next(n);
+  for (size_t i = 0; i < n; ++i) {
+    left[i] = jobs[i].spp;
+    next[i] = jobs[i].spp_offset;
+  }
+  Schedule sc;
+  for (;;) {
+    PassDev pp{};
+    pp.slice0 = (uint32_t)sc.slices.size();
+    pp.pix_mul = 1;
+    uint32_t free = pool;
+    for (size_t oi = 0; oi < n; ++oi) {
+      const size_t j = order[oi];
+      if (!left[j]) continue;
+      const uint32_t npix = jobs[j].width * jobs[j].height;
+      const uint32_t ns = std::min(left[j], free / npix);
+      if (!ns) continue;
+      sc.slices.push_back(SliceDev{(uint32_t)j, next[j], ns, pp.paths, pp.pixels});
+      pp.paths += ns * npix;
+      pp.pixels += npix;
+      free -= ns * npix;
+      left[j] -= ns;
+      next[j] += ns;
+    }
+    pp.n_slices = (uint32_t)sc.slices.size() - pp.slice0;
+    if (!pp.n_slices) break;
+    sc.passes.push_back(pp);
+  }
+  return sc;
+}
+
+struct Batch {
+  pyre_ctx* ctx;
+  const pyre_job* jobs;
+  size_t n;
+  PassShape shape;
+  uint32_t slots = 1, max_npix = 0;
+  std::vector<float*> acc;
+  std::vector<double*> acc64;
+  ~Batch() {
+    for (auto& p : acc) dev_free(p);
+    for (auto& p : acc64) dev_free(p);
+  }
+};
+
+// Runs the whole schedule with tuning t into the jobs' sums (accumulate) or,
+// for a trial, its first `max_passes` passes without touching them; returns
+// the GPU time in ms.
+float run_batch(Batch& B, const Tuning& t, bool accumulate, uint32_t max_passes, uint64_t* ext, uint64_t* shadow,
+                uint32_t* n_passes) {
+  pyre_ctx* ctx = B.ctx;
+  hipStream_t st = ctx->stream;
+  const uint32_t pool = std::max(t.pool_paths, B.max_npix);
+  Schedule sc = make_schedule(B.jobs, B.n, pool);
+  const uint32_t np = std::min<uint32_t>((uint32_t)sc.passes.size(), max_passes);
+  Work W;
+  W.alloc(pool, B.slots, 0, 0, 0, false, B.n, std::max<size_t>(1, sc.slices.size()), std::max<size_t>(1, sc.passes.size()));
+  W.ensure_gstack((size_t)trace_grid_of(ctx, t) * kThreads, B.shape.tree_depth);
+  std::vector<JobDev> jd(B.n);
+  for (size_t j = 0; j < B.n; ++j)
+    jd[j] = job_dev(B.jobs[j].scene, B.jobs[j].camera, B.jobs[j].integrator, B.jobs[j].width, B.jobs[j].height,
+                    B.acc[j], B.acc64[j]);
+  PYRE_HIP_CHECK(hipMemcpyAsync(W.jobs, jd.data(), jd.size() * sizeof(JobDev), hipMemcpyHostToDevice, st));
+  PYRE_HIP_CHECK(hipMemcpyAsync(W.slices, sc.slices.data(), sc.slices.size() * sizeof(SliceDev),
+                                hipMemcpyHostToDevice, st));
+  PYRE_HIP_CHECK(hipMemcpyAsync(W.passes, sc.passes.data(), sc.passes.size() * sizeof(PassDev),
+                                hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, st, W.pass_idx, 0u);
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, W.P.stats, (size_t)2);
+  PYRE_HIP_CHECK(hipStreamSynchronize(st));  // host tables may go out of scope after this
+  auto one_pass = [&] {
+    launch_pass(ctx, W, B.shape, t, accumulate, st);
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, st, W.pass_idx);
+  };
+  hipEvent_t e0, e1;
+  PYRE_HIP_CHECK(hipEventCreate(&e0));
+  PYRE_HIP_CHECK(hipEventCreate(&e1));
+  hipGraphExec_t exec = nullptr;
+  float ms = 0.0f;
+  try {
+    if (graphs_enabled() && np > 1) exec = capture(st, one_pass);
+    PYRE_HIP_CHECK(hipEventRecord(e0, st));
+    for (uint32_t k = 0; k < np; ++k) {
+      if (exec)
+        PYRE_HIP_CHECK(hipGraphLaunch(exec, st));
+      else
+        one_pass();
+    }
+    PYRE_HIP_CHECK(hipEventRecord(e1, st));
+    PYRE_HIP_CHECK(hipEventSynchronize(e1));
+    PYRE_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    PYRE_HIP_CHECK(hipGetLastError());
+  } catch (...) {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    throw;
+  }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (ext) read_stats(ctx, W, ext, shadow);
+  if (n_passes) *n_passes = np;
+  return ms;
+}
+
+// Trial runs of the first passes: pool size and blocks per CU at ring 8,
+// then the ring size; the fastest paths-per-second wins.
+Tuning autotune(Batch& B, Tuning base) {
+  auto rate = [&](const Tuning& t) {
+    uint64_t ext = 0, sh = 0;
+    uint32_t np = 0;
+    float ms = run_batch(B, t, false, 2, &ext, &sh, &np);
+    return ms > 0.0f ? (double)(ext + sh) / ms : 0.0;
+  };
+  Tuning best = base;
+  best.ring = 8;
+  double best_rate = -1.0;
+  for (uint32_t pool : {1u << 20, 1u << 21, 1u << 22})
+    for (uint32_t bpc : {4u, 8u}) {
+      Tuning t = best;
+      t.pool_paths = pool;
+      t.blocks_per_cu = bpc;
+      double r = rate(t);
+      if (r > best_rate) {
+        best_rate = r;
+        best.pool_paths = pool;
+        best.blocks_per_cu = bpc;
+      }
+    }
+  for (uint32_t ring : {4u, 16u}) {
+    Tuning t = best;
+    t.ring = ring;
+    double r = rate(t);
+    if (r > best_rate) {
+      best_rate = r;
+      best.ring = ring;
+    }
+  }
+  return best;
 }
 
 }  // namespace
@@ -900,9 +1217,71 @@ int pyre_render(pyre_ctx* ctx, pyre_scene* s, const pyre_camera_desc* camera, co
     if (accum64) PYRE_HIP_CHECK(hipMemcpyAsync(S.acc64, accum64, n3 * sizeof(double), hipMemcpyHostToDevice, st));
     S.next_sample = spp_offset;
     session_render(&S, spp);
-    if (out) read_mean(&S, out, 1.0f / (float)spp);  // main.rs:255
+    if (out) read_mean(ctx, S.acc, S.npix, out, 1.0f / (float)spp);  // main.rs:255
     if (accum64) PYRE_HIP_CHECK(hipMemcpyAsync(accum64, S.acc64, n3 * sizeof(double), hipMemcpyDeviceToHost, st));
     finish_stats(&S);
+  });
+}
+
+int pyre_render_batch(pyre_ctx* ctx, const pyre_job* jobs, size_t n, const pyre_batch_params* params,
+                      pyre_batch_stats* stats) {
+  if (!ctx || (!jobs && n)) return PYRE_EINVAL;
+  return sguard(ctx, [&] {
+    if (stats) *stats = pyre_batch_stats{};
+    if (n == 0) return;
+    require(n < (1u << 20), "too many jobs");
+    Batch B{ctx, jobs, n, {}, 1, 0, {}, {}};
+    for (size_t j = 0; j < n; ++j) {
+      const pyre_job& J = jobs[j];
+      require(J.scene && J.scene->built, "job " + std::to_string(j) + ": scene missing or BVH not built");
+      require(J.scene->ctx == ctx, "job " + std::to_string(j) + ": scene belongs to another context");
+      require(J.width > 0 && J.height > 0 && (uint64_t)J.width * J.height <= (1u << 24),
+              "job " + std::to_string(j) + ": bad image size");
+      require(J.spp > 0, "job " + std::to_string(j) + ": spp must be positive");
+      B.max_npix = std::max(B.max_npix, J.width * J.height);
+      B.shape.max_depth = std::max(B.shape.max_depth, J.integrator.max_depth);
+      B.shape.tree_depth = std::max(B.shape.tree_depth, J.scene->depth);
+      const uint32_t sl = J.scene->n_lights + (J.scene->has_env ? 1u : 0u);
+      B.shape.nee = B.shape.nee || sl > 0;
+      B.slots = std::max(B.slots, sl);
+    }
+    B.acc.assign(n, nullptr);
+    B.acc64.assign(n, nullptr);
+    hipStream_t st = ctx->stream;
+    for (size_t j = 0; j < n; ++j) {
+      const size_t npix = (size_t)jobs[j].width * jobs[j].height;
+      B.acc[j] = dev_alloc<float>(10 * npix);
+      B.acc64[j] = dev_alloc<double>(3 * npix);
+      clear_sums(ctx, B.acc[j], B.acc64[j], npix);
+      if (jobs[j].accum64)
+        PYRE_HIP_CHECK(hipMemcpyAsync(B.acc64[j], jobs[j].accum64, 3 * npix * sizeof(double), hipMemcpyHostToDevice, st));
+    }
+    Tuning t = env_tuning();
+    if (params) {
+      if (params->pool_paths) t.pool_paths = params->pool_paths;
+      if (params->trace_blocks_per_cu) t.blocks_per_cu = params->trace_blocks_per_cu;
+      if (params->stack_ring == 4 || params->stack_ring == 8 || params->stack_ring == 16) t.ring = params->stack_ring;
+      if (params->autotune) t = autotune(B, t);
+    }
+    uint64_t ext = 0, sh = 0;
+    uint32_t np = 0;
+    float ms = run_batch(B, t, true, 0xFFFFFFFFu, &ext, &sh, &np);
+    for (size_t j = 0; j < n; ++j) {
+      const size_t npix = (size_t)jobs[j].width * jobs[j].height;
+      if (jobs[j].out) read_mean(ctx, B.acc[j], npix, jobs[j].out, 1.0f / (float)jobs[j].spp);
+      if (jobs[j].accum64)
+        PYRE_HIP_CHECK(hipMemcpyAsync(jobs[j].accum64, B.acc64[j], 3 * npix * sizeof(double), hipMemcpyDeviceToHost, st));
+    }
+    PYRE_HIP_CHECK(hipStreamSynchronize(st));
+    if (stats) {
+      stats->extension_rays = ext;
+      stats->shadow_rays = sh;
+      stats->passes = np;
+      stats->gpu_ms = ms;
+      stats->pool_paths = std::max(t.pool_paths, B.max_npix);
+      stats->trace_blocks_per_cu = t.blocks_per_cu;
+      stats->stack_ring = t.ring;
+    }
   });
 }
 
@@ -951,14 +1330,11 @@ int pyre_session_render_nrc(pyre_ctx* ctx, pyre_session* S, pyre_nrc* nrc, const
     hipStream_t st = ctx->stream;
     for (uint32_t f = 0; f < p->frames; ++f) {
       // training paths: full unbiased paths from scattered pixels, own streams
-      PassParams pp = pass_params(S, kTrainStreamBase + S->train_frames, 1, MODE_TRAIN, p->diffuse_roughness);
-      pp.paths = p->train_paths;
-      pp.pix_mul = (uint32_t)(2654435761ull % S->npix) | 1u;
-      pp.pix_add = (uint32_t)(pixel_seed(S->train_frames, 7, 11) % S->npix);
-      hipLaunchKernelGGL(k_set_pass, dim3(1), dim3(1), 0, st, S->W.ppd, pp);
-      launch_pass(ctx, S->s, S->W, S->cam, S->ip.max_depth, nullptr, nullptr, st);
-      hipLaunchKernelGGL(k_train_targets, dim3(stage_grid(ctx)), dim3(kThreads), 0, st, S->W.P,
-                         (const PassParams*)S->W.ppd, S->W.tfeat, S->W.ttarget, S->W.tcap);
+      set_pass(S, kTrainStreamBase + S->train_frames, 1, MODE_TRAIN, p->diffuse_roughness, p->train_paths,
+               (uint32_t)(2654435761ull % S->npix) | 1u, (uint32_t)(pixel_seed(S->train_frames, 7, 11) % S->npix));
+      launch_pass(ctx, S->W, S->shape, S->tune, false, st);
+      hipLaunchKernelGGL(k_train_targets, dim3(stage_grid(ctx)), dim3(kThreads), 0, st, S->W.P, S->W.sched(),
+                         S->W.tfeat, S->W.ttarget, S->W.tcap);
       uint32_t nsamp = 0;
       PYRE_HIP_CHECK(hipMemcpyAsync(&nsamp, S->W.P.counters + 4, 4, hipMemcpyDeviceToHost, st));
       PYRE_HIP_CHECK(hipStreamSynchronize(st));
@@ -975,12 +1351,12 @@ int pyre_session_render_nrc(pyre_ctx* ctx, pyre_session* S, pyre_nrc* nrc, const
       }
       if (losses) losses[f] = steps ? (float)(loss / steps) : 0.0f;
       S->train_frames++;
-      const bool acc = f >= p->warmup_frames;
+      if (f < p->warmup_frames) continue;  // training only
       for (uint32_t s0 = 0; s0 < p->spp_per_frame; s0 += S->per_pass) {
         uint32_t ns = std::min(S->per_pass, p->spp_per_frame - s0);
-        session_pass(S, S->next_sample, ns, MODE_CACHE, nrc, p->diffuse_roughness, acc);
+        session_pass(S, S->next_sample, ns, MODE_CACHE, nrc, p->diffuse_roughness, true);
         S->next_sample += ns;
-        if (acc) S->spp_done += ns;
+        S->spp_done += ns;
       }
     }
     finish_stats(S);
@@ -992,7 +1368,7 @@ int pyre_session_read(pyre_ctx* ctx, pyre_session* S, pyre_pixel_sample* mean, d
   if (!ctx || !S) return PYRE_EINVAL;
   return sguard(ctx, [&] {
     hipStream_t st = ctx->stream;
-    if (mean) read_mean(S, mean, S->spp_done ? 1.0f / (float)S->spp_done : 0.0f);
+    if (mean) read_mean(ctx, S->acc, S->npix, mean, S->spp_done ? 1.0f / (float)S->spp_done : 0.0f);
     if (radiance_sum64)
       PYRE_HIP_CHECK(hipMemcpyAsync(radiance_sum64, S->acc64, 3 * sizeof(double) * (size_t)S->npix,
                                     hipMemcpyDeviceToHost, st));
@@ -1004,7 +1380,8 @@ int pyre_session_read(pyre_ctx* ctx, pyre_session* S, pyre_pixel_sample* mean, d
 int pyre_session_reset(pyre_ctx* ctx, pyre_session* S) {
   if (!ctx || !S) return PYRE_EINVAL;
   return sguard(ctx, [&] {
-    clear_sums(S);
+    clear_sums(ctx, S->acc, S->acc64, S->npix);
+    S->spp_done = 0;
     PYRE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   });
 }
