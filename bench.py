@@ -300,8 +300,10 @@ def bench_nrc(args):
             if cache is None:
                 sess.render(n)
             else:
-                sess.render_nrc(cache, frames=n, spp_per_frame=1, train_paths=1 << 14, train_batch=1 << 14,
-                                train_steps=4)
+                # cold start: the first call trains for nrc_warmup frames before any image sample
+                warm = args.nrc_warmup if done == 0 else 0
+                sess.render_nrc(cache, frames=n + warm, spp_per_frame=1, train_paths=1 << 14,
+                                train_batch=1 << 14, train_steps=4, warmup_frames=warm)
             t_acc += time.perf_counter() - t0
             done = c
             _, r64, spp = sess.read(mean=False, radiance64=True)
@@ -324,7 +326,8 @@ def bench_nrc(args):
         "vs_baseline": None, "dtype": "f32 (cache: bf16 MFMA, f32 accumulate)",
         "data": "synthetic (procedural scene, seed 3)",
         "config": {"workload": "production scene 1920x1080, max depth 12, thin lens 0.05; NRC 5x64 ReLU MLP, "
-                               "2^16 training samples/frame (4 Adam steps x 2^14), lr 1e-2",
+                               "2^16 training samples/frame (4 Adam steps x 2^14), lr 1e-2, "
+                               f"{args.nrc_warmup} training-only frames from a cold start (timed)",
                    "relmse_eps": REL_EPS, "reference_spp": args.ref_spp},
         "plain_seconds": round(tp, 4) if tp else None, "nrc_seconds": round(tn, 4) if tn else None,
         "nrc_over_plain": round(tn / tp, 4) if (tn and tp) else None,
@@ -445,6 +448,7 @@ def main():
     ap.add_argument("--make-reference", default=None, help="render the production reference to this .npy")
     ap.add_argument("--ref-spp", type=int, default=16384)
     ap.add_argument("--max-spp", type=int, default=512, help="nrc: largest spp tried")
+    ap.add_argument("--nrc-warmup", type=int, default=16, help="nrc: training-only frames before the first sample")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "

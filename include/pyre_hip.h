@@ -367,7 +367,11 @@ int pyre_session_render_nrc(pyre_ctx* ctx, pyre_session* session, pyre_nrc* nrc,
  * the FP64 radiance sums (3 doubles per pixel), and the samples per pixel. */
 int pyre_session_read(pyre_ctx* ctx, pyre_session* session, pyre_pixel_sample* mean, double* radiance_sum64,
                       uint32_t* spp_done);
-/* Clears the sums; the sample index keeps advancing. */
+/* The viewer's displayed image (viewer.rs:114-184): the per-pixel running
+ * mean of sample radiance updated sample by sample as d += (s - d) / n, in
+ * f32, 3 floats per pixel. */
+int pyre_session_read_viewer(pyre_ctx* ctx, pyre_session* session, float* rgb);
+/* Clears the sums and the running mean; the sample index keeps advancing. */
 int pyre_session_reset(pyre_ctx* ctx, pyre_session* session);
 
 #ifdef __cplusplus

@@ -546,6 +546,12 @@ class Session:
                        "pyre_session_read")
         return m, r, int(spp.value)
 
+    def read_viewer(self):
+        """The viewer's running-mean radiance (npix, 3) f32 (viewer.rs:165-177)."""
+        out = np.empty((self.width * self.height, 3), np.float32)
+        self.ctx.check(self.lib.pyre_session_read_viewer(self.ctx.h, self.h, _ptr(out)), "pyre_session_read_viewer")
+        return out
+
     def reset(self):
         self.ctx.check(self.lib.pyre_session_reset(self.ctx.h, self.h), "pyre_session_reset")
 

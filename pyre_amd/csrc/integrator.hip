@@ -72,15 +72,17 @@ struct JobDev {
   SceneDev S;
   CamDev cam;
   uint32_t width, height, npix, max_depth, min_rr;
-  float* acc;     // 10 f32 sums per pixel (PixelSample order)
-  double* acc64;  // 3 FP64 radiance sums per pixel
+  float* acc;      // 10 f32 sums per pixel (PixelSample order)
+  double* acc64;   // 3 FP64 radiance sums per pixel
+  float* welford;  // 3 f32 running-mean radiance per pixel (the viewer's accumulator) or null
 };
 
 // Samples [s_base, s_base + n_samples) of every pixel of one job, as the
 // pool's paths [path0, path0 + n_samples * npix) (sample-major) and the
-// accumulation items [pix0, pix0 + npix).
+// accumulation items [pix0, pix0 + npix); n_before = samples already in the
+// job's running mean.
 struct SliceDev {
-  uint32_t job, s_base, n_samples, path0, pix0;
+  uint32_t job, s_base, n_samples, path0, pix0, n_before;
 };
 
 // One pass over the path pool: slices [slice0, slice0 + n_slices).
@@ -627,6 +629,22 @@ __global__ __launch_bounds__(kThreads) void k_accumulate(Paths P, Sched sc) {
     acc64[0] = d0;
     acc64[1] = d1;
     acc64[2] = d2;
+    if (J.welford) {
+      // viewer.rs:165-177: after pass n (1-based) the mean becomes
+      // d + (s - d) / n, component by component
+      float* w = J.welford + 3 * (size_t)px;
+      float w0 = w[0], w1 = w[1], w2 = w[2];
+      for (uint32_t sl = 0; sl < slc.n_samples; ++sl) {
+        const float4 L = P.lum[slc.path0 + sl * J.npix + px];
+        const float n = (float)(slc.n_before + sl + 1u);
+        w0 = w0 + (L.x - w0) / n;
+        w1 = w1 + (L.y - w1) / n;
+        w2 = w2 + (L.z - w2) / n;
+      }
+      w[0] = w0;
+      w[1] = w1;
+      w[2] = w2;
+    }
   }
 }
 
@@ -805,7 +823,7 @@ SceneDev scene_dev(pyre_scene* s) {
 }
 
 JobDev job_dev(pyre_scene* s, const pyre_camera_desc& cam, const pyre_integrator_params& ip, uint32_t w, uint32_t h,
-               float* acc, double* acc64) {
+               float* acc, double* acc64, float* welford = nullptr) {
   JobDev J{};
   J.S = scene_dev(s);
   J.cam = make_camera(cam);
@@ -816,6 +834,7 @@ JobDev job_dev(pyre_scene* s, const pyre_camera_desc& cam, const pyre_integrator
   J.min_rr = ip.min_rr_depth;
   J.acc = acc;
   J.acc64 = acc64;
+  J.welford = welford;
   return J;
 }
 
@@ -913,6 +932,7 @@ struct pyre_session {
   Work W;
   float* acc = nullptr;
   double* acc64 = nullptr;
+  float* welford = nullptr;
   uint32_t spp_done = 0, next_sample = 0, train_frames = 0;
   bool use_graph = true;
   hipGraphExec_t exec = nullptr;
@@ -920,6 +940,7 @@ struct pyre_session {
     if (exec) (void)hipGraphExecDestroy(exec);
     dev_free(acc);
     dev_free(acc64);
+    dev_free(welford);
   }
 };
 
@@ -937,7 +958,7 @@ void set_pass(pyre_session* S, uint32_t s_base, uint32_t n_samples, uint32_t mod
   pp.pix_add = pix_add;
   pp.vmax = S->W.vmax;
   pp.diffuse_rough = diffuse_rough;
-  SliceDev sl{0u, s_base, n_samples, 0u, 0u};
+  SliceDev sl{0u, s_base, n_samples, 0u, 0u, S->spp_done};
   hipLaunchKernelGGL(k_set_pass, dim3(1), dim3(1), 0, S->ctx->stream, S->W.passes, S->W.slices, pp, sl);
 }
 
@@ -997,10 +1018,14 @@ void session_init(pyre_session* S, pyre_ctx* ctx, pyre_scene* s, const pyre_came
   S->shape.nee = s->n_lights + (s->has_env ? 1u : 0u) > 0;
   S->acc = dev_alloc<float>(10 * (size_t)S->npix);
   S->acc64 = dev_alloc<double>(3 * (size_t)S->npix);
+  S->welford = dev_alloc<float>(3 * (size_t)S->npix);
   clear_sums(ctx, S->acc, S->acc64, S->npix);
-  S->spp_done = 0;
   hipStream_t st = ctx->stream;
-  hipLaunchKernelGGL(k_set_job, dim3(1), dim3(1), 0, st, S->W.jobs, job_dev(s, cam, ip, w, h, S->acc, S->acc64));
+  hipLaunchKernelGGL(k_fill_f32, dim3(nblocks(3 * (size_t)S->npix, 256)), dim3(256), 0, st, S->welford,
+                     3 * (size_t)S->npix, 0.0f);
+  S->spp_done = 0;
+  hipLaunchKernelGGL(k_set_job, dim3(1), dim3(1), 0, st, S->W.jobs,
+                     job_dev(s, cam, ip, w, h, S->acc, S->acc64, S->welford));
   hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, st, S->W.pass_idx, 0u);
   hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, S->W.P.stats, (size_t)2);
   S->use_graph = graphs_enabled();
@@ -1069,7 +1094,7 @@ Schedule make_schedule(const pyre_job* jobs, size_t n, uint32_t pool) {
       const uint32_t npix = jobs[j].width * jobs[j].height;
       const uint32_t ns = std::min(left[j], free / npix);
       if (!ns) continue;
-      sc.slices.push_back(SliceDev{(uint32_t)j, next[j], ns, pp.paths, pp.pixels});
+      sc.slices.push_back(SliceDev{(uint32_t)j, next[j], ns, pp.paths, pp.pixels, next[j] - jobs[j].spp_offset});
       pp.paths += ns * npix;
       pp.pixels += npix;
       free -= ns * npix;
@@ -1328,6 +1353,15 @@ int pyre_session_render_nrc(pyre_ctx* ctx, pyre_session* S, pyre_nrc* nrc, const
             "train_paths exceeds the session's nrc_train_paths");
     require(p->train_batch >= 16, "train_batch below 16");
     hipStream_t st = ctx->stream;
+    // per-frame loss sums stay on the device until the end (no sync per step)
+    float* dloss = dev_alloc<float>(std::max(1u, p->frames));
+    std::vector<uint32_t> nsteps(p->frames, 0);
+    hipLaunchKernelGGL(k_fill_f32, dim3(nblocks(std::max(1u, p->frames), 256)), dim3(256), 0, st, dloss,
+                       (size_t)std::max(1u, p->frames), 0.0f);
+    struct Free {
+      float* p;
+      ~Free() { dev_free(p); }
+    } free_loss{dloss};
     for (uint32_t f = 0; f < p->frames; ++f) {
       // training paths: full unbiased paths from scattered pixels, own streams
       set_pass(S, kTrainStreamBase + S->train_frames, 1, MODE_TRAIN, p->diffuse_roughness, p->train_paths,
@@ -1339,17 +1373,14 @@ int pyre_session_render_nrc(pyre_ctx* ctx, pyre_session* S, pyre_nrc* nrc, const
       PYRE_HIP_CHECK(hipMemcpyAsync(&nsamp, S->W.P.counters + 4, 4, hipMemcpyDeviceToHost, st));
       PYRE_HIP_CHECK(hipStreamSynchronize(st));
       nsamp = std::min(nsamp, S->W.tcap);
-      double loss = 0.0;
-      uint32_t steps = 0;
       for (uint32_t k = 0; k < p->train_steps; ++k) {
         const uint32_t off = k * p->train_batch;
         if (off + 16 > nsamp) break;
         const uint32_t nb = std::min(p->train_batch, nsamp - off);
-        loss += nrc_gradients_dev(nrc, S->W.tfeat + 13 * (size_t)off, S->W.ttarget + 3 * (size_t)off, nb, st);
+        nrc_gradients_dev(nrc, S->W.tfeat + 13 * (size_t)off, S->W.ttarget + 3 * (size_t)off, nb, st, dloss + f);
         nrc_adam_step(nrc, st);
-        ++steps;
+        ++nsteps[f];
       }
-      if (losses) losses[f] = steps ? (float)(loss / steps) : 0.0f;
       S->train_frames++;
       if (f < p->warmup_frames) continue;  // training only
       for (uint32_t s0 = 0; s0 < p->spp_per_frame; s0 += S->per_pass) {
@@ -1358,6 +1389,12 @@ int pyre_session_render_nrc(pyre_ctx* ctx, pyre_session* S, pyre_nrc* nrc, const
         S->next_sample += ns;
         S->spp_done += ns;
       }
+    }
+    if (losses && p->frames) {
+      std::vector<float> h(p->frames);
+      PYRE_HIP_CHECK(hipMemcpyAsync(h.data(), dloss, 4 * (size_t)p->frames, hipMemcpyDeviceToHost, st));
+      PYRE_HIP_CHECK(hipStreamSynchronize(st));
+      for (uint32_t f = 0; f < p->frames; ++f) losses[f] = nsteps[f] ? h[f] / (float)nsteps[f] : 0.0f;
     }
     finish_stats(S);
   });
@@ -1377,10 +1414,21 @@ int pyre_session_read(pyre_ctx* ctx, pyre_session* S, pyre_pixel_sample* mean, d
   });
 }
 
+int pyre_session_read_viewer(pyre_ctx* ctx, pyre_session* S, float* rgb) {
+  if (!ctx || !S || !rgb) return PYRE_EINVAL;
+  return sguard(ctx, [&] {
+    PYRE_HIP_CHECK(hipMemcpyAsync(rgb, S->welford, 3 * sizeof(float) * (size_t)S->npix, hipMemcpyDeviceToHost,
+                                  ctx->stream));
+    PYRE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
 int pyre_session_reset(pyre_ctx* ctx, pyre_session* S) {
   if (!ctx || !S) return PYRE_EINVAL;
   return sguard(ctx, [&] {
     clear_sums(ctx, S->acc, S->acc64, S->npix);
+    hipLaunchKernelGGL(k_fill_f32, dim3(nblocks(3 * (size_t)S->npix, 256)), dim3(256), 0, ctx->stream, S->welford,
+                       3 * (size_t)S->npix, 0.0f);
     S->spp_done = 0;
     PYRE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   });

@@ -85,33 +85,44 @@ __device__ __forceinline__ float sh4(int i, float x, float y, float z) {
 }
 
 struct Query {
-  float p[3];  // normalised position
-  float d[3], n[3];
-  float rough, alb[3];
+  float px, py, pz;  // normalised position
+  float dx, dy, dz, nx, ny, nz;
+  float rough, ar, ag, ab;
 };
 
 __device__ __forceinline__ Query load_query(const float* f, const NrcBounds& b) {
   Query q;
-  for (int a = 0; a < 3; ++a) {
-    q.p[a] = (f[a] - b.lo[a]) * b.inv_ext[a];
-    q.d[a] = f[3 + a];
-    q.n[a] = f[6 + a];
-    q.alb[a] = f[10 + a];
-  }
+  q.px = (f[0] - b.lo[0]) * b.inv_ext[0];
+  q.py = (f[1] - b.lo[1]) * b.inv_ext[1];
+  q.pz = (f[2] - b.lo[2]) * b.inv_ext[2];
+  q.dx = f[3];
+  q.dy = f[4];
+  q.dz = f[5];
+  q.nx = f[6];
+  q.ny = f[7];
+  q.nz = f[8];
   q.rough = f[9];
+  q.ar = f[10];
+  q.ag = f[11];
+  q.ab = f[12];
   return q;
 }
 
+// Feature f of the 128-wide encoding (selects instead of indexed arrays, so
+// the query stays in registers when f differs across lanes).
 __device__ __forceinline__ float encode_feature(int f, const Query& q) {
   if (f < 72) {
-    int a = f / 24, k = (f % 24) / 2;
-    float arg = ldexpf(q.p[a], k);  // 2^k * p, exact
+    const int a = f / 24, k = (f % 24) / 2;
+    const float pa = a == 0 ? q.px : (a == 1 ? q.py : q.pz);
+    const float arg = ldexpf(pa, k);  // 2^k * p, exact
     return (f & 1) ? cospif(arg) : sinpif(arg);
   }
-  if (f < 88) return sh4(f - 72, q.d[0], q.d[1], q.d[2]);
-  if (f < 104) return sh4(f - 88, q.n[0], q.n[1], q.n[2]);
+  if (f < 88) return sh4(f - 72, q.dx, q.dy, q.dz);
+  if (f < 104) return sh4(f - 88, q.nx, q.ny, q.nz);
   if (f == 104) return q.rough;
-  if (f < 108) return q.alb[f - 105];
+  if (f == 105) return q.ar;
+  if (f == 106) return q.ag;
+  if (f == 107) return q.ab;
   return f == 108 ? 1.0f : 0.0f;
 }
 
@@ -166,7 +177,7 @@ __global__ void k_make_frags(const float* w, bf16x8* fwd, bf16x8* bwd) {
 // ---------------------------------------------------------- inference
 // Fused forward of whole 16-query tiles: every layer's weight fragments sit
 // in LDS (51 KB), hidden activations go through a 2 KB per-wave LDS tile.
-constexpr int kInferWaves = 4;
+constexpr int kInferWaves = 8;
 
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -195,23 +206,36 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_infer(const bf16x8* __rest
     Query q = load_query(feats + 13 * (size_t)row, b);
     // layer 0: K = 128 from the encoding
     f32x4 acc[4];
+#pragma unroll
     for (int ot = 0; ot < 4; ++ot) acc[ot] = f32x4{0, 0, 0, 0};
+#pragma unroll
     for (int kb = 0; kb < 4; ++kb) {
       bf16x8 bx = encode_frag(q, kb, g);
+#pragma unroll
       for (int ot = 0; ot < 4; ++ot) acc[ot] = mfma(sw[(fwd_frag_offset(0) + ot * 4 + kb) * 64 + lane], bx, acc[ot]);
     }
+#pragma unroll
     for (int l = 1; l < kLayers; ++l) {
       // ReLU(previous) -> LDS tile [query][feature]
+#pragma unroll
       for (int ot = 0; ot < 4; ++ot)
+#pragma unroll
         for (int i = 0; i < 4; ++i) sact[wave][r][16 * ot + 4 * g + i] = to_bf16(fmaxf(acc[ot][i], 0.0f));
       wave_lds_sync();
       bf16x8 bx[2];
+#pragma unroll
       for (int kb = 0; kb < 2; ++kb) bx[kb] = *(const bf16x8*)&sact[wave][r][32 * kb + 8 * g];
       wave_lds_sync();
       const int nout = kOut[l] / 16;
-      for (int ot = 0; ot < 4; ++ot) acc[ot] = f32x4{0, 0, 0, 0};
-      for (int ot = 0; ot < nout; ++ot)
-        for (int kb = 0; kb < 2; ++kb) acc[ot] = mfma(sw[(fwd_frag_offset(l) + ot * 2 + kb) * 64 + lane], bx[kb], acc[ot]);
+#pragma unroll
+      for (int ot = 0; ot < 4; ++ot) {
+        acc[ot] = f32x4{0, 0, 0, 0};
+        if (ot < nout) {
+#pragma unroll
+          for (int kb = 0; kb < 2; ++kb)
+            acc[ot] = mfma(sw[(fwd_frag_offset(l) + ot * 2 + kb) * 64 + lane], bx[kb], acc[ot]);
+        }
+      }
     }
     if (valid && g == 0) {
       out_rgb[3 * (size_t)qi] = acc[0][0];
@@ -424,7 +448,22 @@ static size_t act_offset(int layer, uint32_t cap) {
 }
 
 // Forward + loss + backward into N->grad. feats/targets are device arrays.
-float nrc_gradients_dev(pyre_nrc* N, const float* feats, const float* targets, uint32_t n, hipStream_t st) {
+// *out += scale * sum(part[0..n)) (one block).
+__global__ void k_sum_into(const float* part, uint32_t n, float scale, float* out) {
+  __shared__ float red[256];
+  float s = 0.0f;
+  for (uint32_t i = threadIdx.x; i < n; i += 256) s += part[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out += scale * red[0];
+}
+
+float nrc_gradients_dev(pyre_nrc* N, const float* feats, const float* targets, uint32_t n, hipStream_t st,
+                        float* loss_dev) {
   require(n > 0, "empty training batch");
   ensure_cap(N, n);
   const uint32_t np = (n + 15) / 16 * 16;
@@ -452,6 +491,11 @@ float nrc_gradients_dev(pyre_nrc* N, const float* feats, const float* targets, u
       hipLaunchKernelGGL(k_bwd_dx, dim3(tb), dim3(256), 0, st, N->wb, l, dz, N->acts + act_offset(l, cap), np, dzn);
       std::swap(dz, dzn);
     }
+  }
+  if (loss_dev) {
+    hipLaunchKernelGGL(k_sum_into, dim3(1), dim3(256), 0, st, N->loss_part, blocks(np, 256), 1.0f / (3.0f * n),
+                       loss_dev);
+    return 0.0f;
   }
   std::vector<float> lp(blocks(np, 256));
   PYRE_HIP_CHECK(hipMemcpyAsync(lp.data(), N->loss_part, lp.size() * 4, hipMemcpyDeviceToHost, st));
@@ -623,7 +667,7 @@ int pyre_nrc_gradients(pyre_ctx* ctx, pyre_nrc* N, const float* feats, const flo
     hipStream_t st = ctx->stream;
     PYRE_HIP_CHECK(hipMemcpyAsync(N->feats, feats, 13 * 4 * n, hipMemcpyHostToDevice, st));
     PYRE_HIP_CHECK(hipMemcpyAsync(N->targets, targets, 3 * 4 * n, hipMemcpyHostToDevice, st));
-    float l = nrc_gradients_dev(N, N->feats, N->targets, (uint32_t)n, st);
+    float l = nrc_gradients_dev(N, N->feats, N->targets, (uint32_t)n, st, nullptr);
     if (loss) *loss = l;
     if (grad) {
       PYRE_HIP_CHECK(hipMemcpyAsync(grad, N->grad, kParams * 4, hipMemcpyDeviceToHost, st));
@@ -647,7 +691,7 @@ int pyre_nrc_train_step(pyre_ctx* ctx, pyre_nrc* N, const float* feats, const fl
     hipStream_t st = ctx->stream;
     PYRE_HIP_CHECK(hipMemcpyAsync(N->feats, feats, 13 * 4 * n, hipMemcpyHostToDevice, st));
     PYRE_HIP_CHECK(hipMemcpyAsync(N->targets, targets, 3 * 4 * n, hipMemcpyHostToDevice, st));
-    float l = nrc_gradients_dev(N, N->feats, N->targets, (uint32_t)n, st);
+    float l = nrc_gradients_dev(N, N->feats, N->targets, (uint32_t)n, st, nullptr);
     nrc_adam_step(N, st);
     PYRE_HIP_CHECK(hipStreamSynchronize(st));
     if (loss) *loss = l;

@@ -14,8 +14,10 @@ struct NrcBounds {
 int nrc_param_count();
 // Forward + relative-L2 loss + backward into the cache's gradient buffer;
 // feats (13 floats per query) and targets (RGB) are device arrays. Returns
-// the mean loss.
-float nrc_gradients_dev(pyre_nrc* nrc, const float* feats, const float* targets, uint32_t n, hipStream_t st);
+// the mean loss, or, when loss_dev is given, adds it to *loss_dev on the
+// stream without synchronising (and returns 0).
+float nrc_gradients_dev(pyre_nrc* nrc, const float* feats, const float* targets, uint32_t n, hipStream_t st,
+                        float* loss_dev = nullptr);
 void nrc_adam_step(pyre_nrc* nrc, hipStream_t st);
 // Fused BF16-MFMA inference: rgb[3n] from feats[13n] (device arrays).
 void nrc_infer_dev(pyre_nrc* nrc, const float* feats, uint32_t n, float* rgb, hipStream_t st);

@@ -157,3 +157,26 @@ def test_nrc_gain_over_plain_at_equal_spp(ctx):
     print(f"[nrc gain] relMSE plain={e_plain:.4f} trained={e_trained:.4f} untrained={e_untrained:.4f}", flush=True)
     assert e_trained < e_plain
     assert e_trained < e_untrained
+
+
+def test_viewer_running_mean_equals_welford_of_samples(ctx):
+    """viewer.rs:165-177 keeps d += (s - d) / n per pass in f32. The session's
+    running mean must equal that update applied on the host to the per-sample
+    radiance (one-sample renders return the sample itself, mean x * (1/1)),
+    also when several samples share one pass."""
+    pre, sc = _cornell(ctx)
+    w, h = 24, 20
+    cam = pre.camera(w / h)
+    ip = pyre_amd.PathIntegrator(8, 3)
+    with _env(PYRE_PASS_PATHS=3 * w * h):
+        sess = pyre_amd.Session(sc, cam, ip, w, h)
+        sess.render(2)
+        sess.render(5)
+        got = sess.read_viewer()
+        sess.close()
+    d = np.zeros((w * h, 3), np.float32)
+    for s in range(7):
+        L = pyre_amd.Film(w, h).render(sc, cam, ip, 1, spp_offset=s)[:, :3].astype(np.float32)
+        n = np.float32(s + 1)
+        d = (d + (L - d) / n).astype(np.float32)
+    assert np.array_equal(got.view(np.uint32), d.view(np.uint32))
