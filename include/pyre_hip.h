@@ -246,8 +246,9 @@ int pyre_render(pyre_ctx* ctx, pyre_scene* scene, const pyre_camera_desc* camera
                 const pyre_integrator_params* integrator, uint32_t width, uint32_t height, uint32_t spp,
                 uint32_t spp_offset, pyre_pixel_sample* out, double* accum64);
 
-/* Rays traced by the last pyre_render on this scene: closest-hit extension
- * rays (camera rays included) and any-hit shadow rays. */
+/* Rays traced by the last pyre_render (or session render call) on this
+ * scene: closest-hit extension rays (camera rays included) and any-hit
+ * shadow rays. */
 int pyre_render_stats(pyre_ctx* ctx, pyre_scene* scene, uint64_t* extension_rays, uint64_t* shadow_rays);
 
 /* ------------------------------------------------------------- batches */
@@ -266,9 +267,9 @@ typedef struct pyre_job {
   double* accum64;        /* FP64 radiance sums, added to (as pyre_render), or NULL */
 } pyre_job;
 
-/* Launch configuration; 0 fields keep the defaults (2^21 paths per pass, 8
+/* Launch configuration; 0 fields keep the defaults (2^24 paths per pass, 8
  * traversal blocks per CU, 8-entry LDS stack ring). autotune != 0 first times
- * trial passes of the batch over pool sizes {2^20, 2^21, 2^22}, blocks per CU
+ * trial passes of the batch over pool sizes {2^22, 2^23, 2^24}, blocks per CU
  * {4, 8} and stack rings {4, 8, 16} and keeps the fastest. */
 typedef struct pyre_batch_params {
   uint32_t pool_paths;

@@ -724,7 +724,7 @@ inline unsigned nblocks(size_t n, unsigned t) { return (unsigned)((n + t - 1) / 
 // persistent traversal blocks per CU and the LDS part of the traversal stack
 // (ring entries per thread). Results do not depend on it.
 struct Tuning {
-  uint32_t pool_paths = 1u << 21;
+  uint32_t pool_paths = 1u << 24;
   uint32_t blocks_per_cu = 8;
   uint32_t ring = 8;
 };
@@ -934,10 +934,16 @@ struct pyre_session {
   double* acc64 = nullptr;
   float* welford = nullptr;
   uint32_t spp_done = 0, next_sample = 0, train_frames = 0;
+  uint64_t stats_seen[2] = {0, 0};  // ray counters already reported
   bool use_graph = true;
-  hipGraphExec_t exec = nullptr;
+  hipGraphExec_t exec = nullptr;        // PLAIN pass
+  hipGraphExec_t exec_train = nullptr;  // TRAIN pass + training targets
+  hipGraphExec_t exec_cache = nullptr;  // CACHE pass + inference + accumulate, for cache_of
+  const pyre_nrc* cache_of = nullptr;
   ~pyre_session() {
     if (exec) (void)hipGraphExecDestroy(exec);
+    if (exec_train) (void)hipGraphExecDestroy(exec_train);
+    if (exec_cache) (void)hipGraphExecDestroy(exec_cache);
     dev_free(acc);
     dev_free(acc64);
     dev_free(welford);
@@ -962,24 +968,38 @@ void set_pass(pyre_session* S, uint32_t s_base, uint32_t n_samples, uint32_t mod
   hipLaunchKernelGGL(k_set_pass, dim3(1), dim3(1), 0, S->ctx->stream, S->W.passes, S->W.slices, pp, sl);
 }
 
-// Samples [s_base, s_base + n_samples) of every pixel. PLAIN passes of the
-// full pass size replay one captured graph; the rest launch eagerly.
+// Samples [s_base, s_base + n_samples) of every pixel. Every stage reads
+// its pass from device memory, so one captured graph per kind of pass
+// (plain, cache) serves any sample count; PYRE_GRAPH=0 launches eagerly.
 void session_pass(pyre_session* S, uint32_t s_base, uint32_t n_samples, uint32_t mode, pyre_nrc* nrc,
                   float diffuse_rough, bool accumulate) {
   hipStream_t st = S->ctx->stream;
   set_pass(S, s_base, n_samples, mode, diffuse_rough);
-  if (mode == MODE_PLAIN && accumulate && n_samples == S->per_pass && S->use_graph) {
-    if (!S->exec) S->exec = capture(st, [&] { launch_pass(S->ctx, S->W, S->shape, S->tune, true, st); });
-    PYRE_HIP_CHECK(hipGraphLaunch(S->exec, st));
-    return;
-  }
   if (mode == MODE_CACHE) {
     require(nrc != nullptr, "cache pass without a radiance cache");
-    launch_pass(S->ctx, S->W, S->shape, S->tune, false, st);
-    nrc_infer_dev_count(nrc, S->W.P.qfeat, S->W.P.qpath, S->W.P.counters + 3, (uint32_t)S->W.cap, S->W.P.qrgb, st);
-    hipLaunchKernelGGL(k_cache_apply, dim3(stage_grid(S->ctx)), dim3(kThreads), 0, st, S->W.P);
-    if (accumulate)
+    require(accumulate, "cache passes always accumulate");
+    auto body = [&] {
+      launch_pass(S->ctx, S->W, S->shape, S->tune, false, st);
+      nrc_infer_dev_count(nrc, S->W.P.qfeat, S->W.P.qpath, S->W.P.counters + 3, (uint32_t)S->W.cap, S->W.P.qrgb,
+                          st);
+      hipLaunchKernelGGL(k_cache_apply, dim3(stage_grid(S->ctx)), dim3(kThreads), 0, st, S->W.P);
       hipLaunchKernelGGL(k_accumulate, dim3(stage_grid(S->ctx)), dim3(kThreads), 0, st, S->W.P, S->W.sched());
+    };
+    if (!S->use_graph) return body();
+    if (S->exec_cache && S->cache_of != nrc) {
+      (void)hipGraphExecDestroy(S->exec_cache);
+      S->exec_cache = nullptr;
+    }
+    if (!S->exec_cache) {
+      S->exec_cache = capture(st, body);
+      S->cache_of = nrc;
+    }
+    PYRE_HIP_CHECK(hipGraphLaunch(S->exec_cache, st));
+    return;
+  }
+  if (mode == MODE_PLAIN && accumulate && S->use_graph) {
+    if (!S->exec) S->exec = capture(st, [&] { launch_pass(S->ctx, S->W, S->shape, S->tune, true, st); });
+    PYRE_HIP_CHECK(hipGraphLaunch(S->exec, st));
     return;
   }
   launch_pass(S->ctx, S->W, S->shape, S->tune, accumulate, st);
@@ -991,8 +1011,11 @@ void clear_sums(pyre_ctx* ctx, float* acc, double* acc64, size_t npix) {
   hipLaunchKernelGGL(k_fill_f64, dim3(nblocks(3 * npix, 256)), dim3(256), 0, st, acc64, 3 * npix, 0.0);
 }
 
+// spp_hint > 0: the session renders at most that many samples per pixel
+// (a one-shot render), so the path pool need not exceed spp_hint images.
 void session_init(pyre_session* S, pyre_ctx* ctx, pyre_scene* s, const pyre_camera_desc& cam,
-                  const pyre_integrator_params& ip, uint32_t w, uint32_t h, uint32_t train_paths) {
+                  const pyre_integrator_params& ip, uint32_t w, uint32_t h, uint32_t train_paths,
+                  uint32_t spp_hint = 0) {
   require(w > 0 && h > 0, "width and height must be positive");
   require(s->built, "render before pyre_bvh_build");
   require((uint64_t)w * h < (1ull << 28), "image too large");
@@ -1006,6 +1029,7 @@ void session_init(pyre_session* S, pyre_ctx* ctx, pyre_scene* s, const pyre_came
   S->tune = env_tuning();
   // paths per pass: whole samples of the image, about pool_paths paths
   S->per_pass = std::max(1u, S->tune.pool_paths / S->npix);
+  if (spp_hint) S->per_pass = std::min(S->per_pass, spp_hint);
   const size_t cap = std::max((size_t)S->per_pass * S->npix, (size_t)train_paths);
   require(cap < (1ull << 31), "path pool too large");
   const uint32_t slots = std::max(1u, s->n_lights + (s->has_env ? 1u : 0u));
@@ -1049,7 +1073,16 @@ void read_stats(pyre_ctx* ctx, const Work& W, uint64_t* ext, uint64_t* shadow) {
   *shadow = hs[1];
 }
 
-void finish_stats(pyre_session* S) { read_stats(S->ctx, S->W, &S->s->last_extension_rays, &S->s->last_shadow_rays); }
+// Rays traced since the previous report (the device counters run for the
+// session's lifetime).
+void finish_stats(pyre_session* S) {
+  uint64_t ext = 0, sh = 0;
+  read_stats(S->ctx, S->W, &ext, &sh);
+  S->s->last_extension_rays = ext - S->stats_seen[0];
+  S->s->last_shadow_rays = sh - S->stats_seen[1];
+  S->stats_seen[0] = ext;
+  S->stats_seen[1] = sh;
+}
 
 void read_mean(pyre_ctx* ctx, const float* acc, size_t npix, pyre_pixel_sample* out, float inv_spp) {
   hipStream_t st = ctx->stream;
@@ -1195,7 +1228,7 @@ Tuning autotune(Batch& B, Tuning base) {
   Tuning best = base;
   best.ring = 8;
   double best_rate = -1.0;
-  for (uint32_t pool : {1u << 20, 1u << 21, 1u << 22})
+  for (uint32_t pool : {1u << 22, 1u << 23, 1u << 24})
     for (uint32_t bpc : {4u, 8u}) {
       Tuning t = best;
       t.pool_paths = pool;
@@ -1236,7 +1269,7 @@ int pyre_render(pyre_ctx* ctx, pyre_scene* s, const pyre_camera_desc* camera, co
   if (!ctx || !s || !camera || !integrator) return PYRE_EINVAL;
   return sguard(ctx, [&] {
     pyre_session S;
-    session_init(&S, ctx, s, *camera, *integrator, width, height, 0);
+    session_init(&S, ctx, s, *camera, *integrator, width, height, 0, std::max(1u, spp));
     hipStream_t st = ctx->stream;
     const size_t n3 = 3 * (size_t)S.npix;
     if (accum64) PYRE_HIP_CHECK(hipMemcpyAsync(S.acc64, accum64, n3 * sizeof(double), hipMemcpyHostToDevice, st));
@@ -1366,9 +1399,17 @@ int pyre_session_render_nrc(pyre_ctx* ctx, pyre_session* S, pyre_nrc* nrc, const
       // training paths: full unbiased paths from scattered pixels, own streams
       set_pass(S, kTrainStreamBase + S->train_frames, 1, MODE_TRAIN, p->diffuse_roughness, p->train_paths,
                (uint32_t)(2654435761ull % S->npix) | 1u, (uint32_t)(pixel_seed(S->train_frames, 7, 11) % S->npix));
-      launch_pass(ctx, S->W, S->shape, S->tune, false, st);
-      hipLaunchKernelGGL(k_train_targets, dim3(stage_grid(ctx)), dim3(kThreads), 0, st, S->W.P, S->W.sched(),
-                         S->W.tfeat, S->W.ttarget, S->W.tcap);
+      auto train_pass = [&] {
+        launch_pass(ctx, S->W, S->shape, S->tune, false, st);
+        hipLaunchKernelGGL(k_train_targets, dim3(stage_grid(ctx)), dim3(kThreads), 0, st, S->W.P, S->W.sched(),
+                           S->W.tfeat, S->W.ttarget, S->W.tcap);
+      };
+      if (S->use_graph) {
+        if (!S->exec_train) S->exec_train = capture(st, train_pass);
+        PYRE_HIP_CHECK(hipGraphLaunch(S->exec_train, st));
+      } else {
+        train_pass();
+      }
       uint32_t nsamp = 0;
       PYRE_HIP_CHECK(hipMemcpyAsync(&nsamp, S->W.P.counters + 4, 4, hipMemcpyDeviceToHost, st));
       PYRE_HIP_CHECK(hipStreamSynchronize(st));

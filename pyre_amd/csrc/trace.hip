@@ -58,20 +58,20 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(const Node2* __restrict
 // Persistent 4-wide traversal with dynamic ray fetch: a lane that finishes
 // its ray takes the next one of its wave's chunk as soon as a quarter of the
 // wave is idle, instead of waiting for the wave's slowest ray.
-template <bool ANY>
-__global__ __launch_bounds__(kTraceThreads) void k_trace4(const Node4* __restrict__ nodes,
+template <bool ANY, int RING>
+__global__ __launch_bounds__(kTraceThreads, 8) void k_trace4(const Node4* __restrict__ nodes,
                                                           const float4* __restrict__ tris,
                                                           const float4* __restrict__ rays, uint32_t n,
                                                           pyre_hit8* __restrict__ out_hit,
                                                           float2* __restrict__ out_uv,
                                                           uint32_t* __restrict__ out_occ, int2* gstack,
                                                           uint32_t* fetch) {
-  __shared__ int2 ring[kRing * kTraceThreads];
+  __shared__ int2 ring[RING * kTraceThreads];
   const uint32_t tid = blockIdx.x * kTraceThreads + threadIdx.x;
   const uint32_t nthreads = gridDim.x * kTraceThreads;
-  HybridStack<kRing> st{ring + threadIdx.x, kTraceThreads, gstack + tid, nthreads, 0, 0};
+  HybridStack<RING> st{ring + threadIdx.x, kTraceThreads, gstack + tid, nthreads, 0, 0};
   WaveFeed feed(n, fetch);
-  Trav4<ANY, kRing> tv;
+  Trav4<ANY, RING> tv;
   uint32_t ray = 0;
   bool active = false;
   for (;;) {
@@ -135,6 +135,13 @@ __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
   if (i < n) p[i] = 0u;
 }
 
+// LDS stack ring entries per thread (PYRE_STACK_RING = 4, 8 or 16; default 8).
+int trace_ring() {
+  const char* e = std::getenv("PYRE_STACK_RING");
+  const int r = e ? std::atoi(e) : 8;
+  return (r == 4 || r == 16) ? r : 8;
+}
+
 bool use_binary() {
   const char* e = std::getenv("PYRE_TRAVERSAL");
   return e && std::strcmp(e, "binary") == 0;
@@ -156,9 +163,17 @@ void launch(pyre_scene* s, const float4* rays, size_t n, pyre_hit8* hit, float2*
       PYRE_HIP_CHECK(hipGetLastError());
       return;
     }
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, s->d_fetch, (uint32_t)kFetchParts);
-    hipLaunchKernelGGL((k_trace4<ANY>), dim3(trace_grid(s)), dim3(kTraceThreads), 0, st, s->d_nodes4, s->d_tris,
-                       rays, (uint32_t)n, hit, uv, occ, gst, s->d_fetch);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, s->d_fetch, 1u);
+    const int ring = trace_ring();
+    if (ring == 4)
+      hipLaunchKernelGGL((k_trace4<ANY, 4>), dim3(trace_grid(s)), dim3(kTraceThreads), 0, st, s->d_nodes4, s->d_tris,
+                         rays, (uint32_t)n, hit, uv, occ, gst, s->d_fetch);
+    else if (ring == 16)
+      hipLaunchKernelGGL((k_trace4<ANY, 16>), dim3(trace_grid(s)), dim3(kTraceThreads), 0, st, s->d_nodes4,
+                         s->d_tris, rays, (uint32_t)n, hit, uv, occ, gst, s->d_fetch);
+    else
+      hipLaunchKernelGGL((k_trace4<ANY, 8>), dim3(trace_grid(s)), dim3(kTraceThreads), 0, st, s->d_nodes4, s->d_tris,
+                         rays, (uint32_t)n, hit, uv, occ, gst, s->d_fetch);
     PYRE_HIP_CHECK(hipGetLastError());
     return;
   }

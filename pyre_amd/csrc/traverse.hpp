@@ -323,29 +323,33 @@ __device__ __forceinline__ void traverse4(const Node4* __restrict__ nodes, const
 
 // Lanes of a wave refill once at least this many are idle.
 constexpr int kRefillLanes = 16;
-// Independent work counters (one per XCD-sized slice of the work items).
-constexpr int kFetchParts = 8;
-// Items a wave claims from its slice's counter at a time.
+// Items each wave takes without any atomic (its static share), and items a
+// wave claims from the shared counter at a time once that is used up.
+constexpr uint32_t kStaticChunk = 64;
 constexpr uint32_t kWaveChunk = 256;
 
-// Work distribution of the persistent kernels: the n items are cut into
-// kFetchParts slices, taken by the blocks with blockIdx % kFetchParts equal
-// to the slice (blocks are dealt to the XCDs round-robin, so a slice's
-// counter stays within one XCD); a wave claims kWaveChunk items of its slice
-// with one atomic and hands them to its idle lanes in order. All members are
+// Work distribution of the persistent kernels over n items: wave w first
+// takes [w * kStaticChunk, (w + 1) * kStaticChunk) with no atomic, so a
+// short queue (late bounces, small images) costs no atomics at all; the
+// items past the static shares are claimed kWaveChunk at a time from one
+// counter, which balances expensive and cheap rays across the GPU. A wave
+// that sees the counter reach the end stops claiming. All members are
 // wave-uniform.
 struct WaveFeed {
-  uint32_t lo, hi, next, end;
+  uint32_t n, next, end, dyn0;
   uint32_t* ctr;
-  bool dry;  // the slice has no unclaimed items left
-  __device__ __forceinline__ WaveFeed(uint32_t n, uint32_t* counters) {
-    const uint32_t part = blockIdx.x % kFetchParts;
-    const uint32_t per = n / kFetchParts + (n % kFetchParts != 0);
-    lo = part * per < n ? part * per : n;  // part * per < n + kFetchParts, no overflow for n < 2^32 - 8
-    hi = n - lo > per ? lo + per : n;
-    ctr = counters + part;
-    next = end = 0;
-    dry = false;
+  bool dry;
+  __device__ __forceinline__ WaveFeed(uint32_t n_, uint32_t* counter) {
+    n = n_;
+    ctr = counter;
+    const uint32_t waves_per_block = blockDim.x / 64u;
+    const uint64_t wave = (uint64_t)blockIdx.x * waves_per_block + (threadIdx.x >> 6);
+    const uint64_t waves = (uint64_t)gridDim.x * waves_per_block;
+    const uint64_t s0 = wave * kStaticChunk, d0 = waves * kStaticChunk;
+    next = s0 < n ? (uint32_t)s0 : n;
+    end = n - next > kStaticChunk ? next + kStaticChunk : n;
+    dyn0 = d0 < n ? (uint32_t)d0 : n;
+    dry = dyn0 >= n;
   }
   // Gives the idle lanes (ballot `idle`) the next items; true for an idle
   // lane that got one (in `item`). Called by the whole wave.
@@ -355,9 +359,13 @@ struct WaveFeed {
       uint32_t base = 0;
       if (__lane_id() == 0) base = atomicAdd(ctr, kWaveChunk);
       base = __builtin_amdgcn_readfirstlane(base);
-      next = base < hi - lo ? lo + base : hi;
-      end = next + (hi - next < kWaveChunk ? hi - next : kWaveChunk);
-      dry = end >= hi;
+      if (base >= n - dyn0) {
+        dry = true;
+        return false;
+      }
+      next = dyn0 + base;
+      end = n - next > kWaveChunk ? next + kWaveChunk : n;
+      dry = end >= n;
     }
     const uint32_t k = next + (uint32_t)__popcll(idle & ((1ull << __lane_id()) - 1ull));
     const uint32_t cnt = (uint32_t)__popcll(idle);
