@@ -312,11 +312,14 @@ def bench_nrc(args):
             log(f"{kind}: {c} spp, {t_acc:.3f}s, relMSE {e:.5f}")
             if e <= target * 0.7:
                 break
+        stats[kind] = sess.stats()
+        stats[kind]["samples"] = done * PROD_W * PROD_H
         sess.close()
         if cache is not None:
             cache.close()
         return curve
 
+    stats = {}
     plain = run("plain")
     nrc = run("nrc")
     tp, tn = _crossing(plain, target), _crossing(nrc, target)
@@ -332,7 +335,7 @@ def bench_nrc(args):
         "plain_seconds": round(tp, 4) if tp else None, "nrc_seconds": round(tn, 4) if tn else None,
         "nrc_over_plain": round(tn / tp, 4) if (tn and tp) else None,
         "plain_curve": [(round(t, 4), round(e, 5)) for t, e in plain],
-        "nrc_curve": [(round(t, 4), round(e, 5)) for t, e in nrc], "device": dev_name,
+        "nrc_curve": [(round(t, 4), round(e, 5)) for t, e in nrc], "path_stats": stats, "device": dev_name,
     }
     print(json.dumps(line), flush=True)
 

@@ -372,6 +372,10 @@ int pyre_session_read(pyre_ctx* ctx, pyre_session* session, pyre_pixel_sample* m
  * mean of sample radiance updated sample by sample as d += (s - d) / n, in
  * f32, 3 floats per pixel. */
 int pyre_session_read_viewer(pyre_ctx* ctx, pyre_session* session, float* rgb);
+/* Totals over the session's lifetime: out[0] extension rays, out[1] shadow
+ * rays, out[2] paths ended in the radiance cache, out[3] cache training
+ * samples produced. */
+int pyre_session_stats(pyre_ctx* ctx, pyre_session* session, uint64_t out[4]);
 /* Clears the sums and the running mean; the sample index keeps advancing. */
 int pyre_session_reset(pyre_ctx* ctx, pyre_session* session);
 

@@ -552,6 +552,13 @@ class Session:
         self.ctx.check(self.lib.pyre_session_read_viewer(self.ctx.h, self.h, _ptr(out)), "pyre_session_read_viewer")
         return out
 
+    def stats(self):
+        """Lifetime totals: extension rays, shadow rays, cache-terminated paths, training samples."""
+        out = (C.c_uint64 * 4)()
+        self.ctx.check(self.lib.pyre_session_stats(self.ctx.h, self.h, out), "pyre_session_stats")
+        return dict(extension_rays=int(out[0]), shadow_rays=int(out[1]), cache_queries=int(out[2]),
+                    training_samples=int(out[3]))
+
     def reset(self):
         self.ctx.check(self.lib.pyre_session_reset(self.ctx.h, self.h), "pyre_session_reset")
 

@@ -178,6 +178,7 @@ SIGNATURES.update({
     "pyre_session_read": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(_u32)]),
     "pyre_session_reset": (C.c_int, [_vp, _vp]),
     "pyre_session_read_viewer": (C.c_int, [_vp, _vp, _vp]),
+    "pyre_session_stats": (C.c_int, [_vp, _vp, C.POINTER(C.c_uint64)]),
 })
 
 

@@ -50,14 +50,17 @@ __global__ void k_gather(const float* pos, const uint32_t* idx, const float* nrm
   }
 }
 
-__global__ void k_bounds(const float* verts, size_t nv, int32_t* keys) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Scene bounds over nv vertices: per-thread grid-stride min/max, then per
+// wave and per block, one set of atomics per block.
+__global__ __launch_bounds__(256) void k_bounds(const float* verts, size_t nv, int32_t* keys) {
+  __shared__ int32_t part[4][6];
   int32_t lo[3] = {f2key(kInf), f2key(kInf), f2key(kInf)};
   int32_t hi[3] = {f2key(-kInf), f2key(-kInf), f2key(-kInf)};
-  if (i < nv)
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += (size_t)gridDim.x * blockDim.x)
     for (int c = 0; c < 3; ++c) {
-      lo[c] = f2key(verts[3 * i + c]);
-      hi[c] = lo[c];
+      const int32_t k = f2key(verts[3 * i + c]);
+      lo[c] = min(lo[c], k);
+      hi[c] = max(hi[c], k);
     }
   for (int c = 0; c < 3; ++c) {
     for (int o = 32; o > 0; o >>= 1) {
@@ -65,11 +68,24 @@ __global__ void k_bounds(const float* verts, size_t nv, int32_t* keys) {
       hi[c] = max(hi[c], __shfl_xor(hi[c], o));
     }
   }
-  if (__lane_id() == 0 && i < nv + 63)
+  const int wave = threadIdx.x >> 6;
+  if (__lane_id() == 0)
+    for (int c = 0; c < 3; ++c) {
+      part[wave][c] = lo[c];
+      part[wave][3 + c] = hi[c];
+    }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
+      for (int c = 0; c < 3; ++c) {
+        lo[c] = min(lo[c], part[w][c]);
+        hi[c] = max(hi[c], part[w][3 + c]);
+      }
     for (int c = 0; c < 3; ++c) {
       atomicMin(&keys[c], lo[c]);
       atomicMax(&keys[3 + c], hi[c]);
     }
+  }
 }
 
 __global__ void k_bounds_init(int32_t* keys) {
@@ -437,7 +453,8 @@ int pyre_scene_upload(pyre_ctx* ctx, const pyre_scene_desc* desc, pyre_scene** o
     DevVec<int32_t> keys(6);
     hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(1), 0, st, keys.p);
     if (tri_total)
-      hipLaunchKernelGGL(k_bounds, dim3(blocks(3 * tri_total, 256)), dim3(256), 0, st, s->d_verts, 3 * tri_total,
+      hipLaunchKernelGGL(k_bounds, dim3(std::min<unsigned>(blocks(3 * tri_total, 256), (unsigned)std::max(1, ctx->cus) * 8)),
+                         dim3(256), 0, st, s->d_verts, 3 * tri_total,
                          keys.p);
     int32_t hk[6];
     d2h(hk, keys.p, 6, st);

@@ -339,14 +339,15 @@ void exclusive_scan(Scratch& S, const uint32_t* in, uint32_t* out, size_t n, uin
 }
 
 // ------------------------------------------------------------------ prep
-// Per triangle bounds and centroid (mesh.rs:32-40); root bounds by
-// wave reduction + atomics.
+// Per triangle bounds and centroid (mesh.rs:32-40); root bounds reduced per
+// thread over a grid-stride range, then per wave and per block (LDS), so
+// the root's keys see one set of atomics per block.
 __global__ __launch_bounds__(256) void k_prep(const float* __restrict__ verts, uint32_t n, float4* plo, float4* phi,
                                               float4* pc, uint32_t* ref, BNode* root) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ int32_t part[4][12];
   KeyAcc acc;
   acc.init();
-  if (i < n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const float* v = verts + 9 * (size_t)i;
     V3 a = v3(v[0], v[1], v[2]), b = v3(v[3], v[4], v[5]), c = v3(v[6], v[7], v[8]);
     V3 lo = vmin(vmin(a, b), c);
@@ -361,7 +362,16 @@ __global__ __launch_bounds__(256) void k_prep(const float* __restrict__ verts, u
     acc.add(l4, h4, c4);
   }
   acc.wave_reduce();
-  if (__lane_id() == 0) keys_atomic(root->bkey, root->ckey, acc);
+  const int wave = threadIdx.x >> 6;
+  if (__lane_id() == 0)
+    for (int k = 0; k < 12; ++k) part[wave][k] = acc.k[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
+      for (int k = 0; k < 12; ++k)
+        acc.k[k] = (k % 6) < 3 ? min(acc.k[k], part[w][k]) : max(acc.k[k], part[w][k]);
+    keys_atomic(root->bkey, root->ckey, acc);
+  }
 }
 
 // Root record, counters and the first work list (level list or, for small
@@ -1001,12 +1011,20 @@ __global__ void k_reorder_tris(const float* verts, const uint32_t* ref, uint32_t
   tris[3 * (size_t)k + 2] = make_float4(e2.x, e2.y, e2.z, 0.0f);
 }
 
-__global__ void k_count_leaves(const BNode* nodes, uint32_t n, uint32_t* cnt) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+// Leaf count: grid-stride per thread, then per wave and block, one atomic
+// per block.
+__global__ __launch_bounds__(256) void k_count_leaves(const BNode* nodes, uint32_t n, uint32_t* cnt) {
+  __shared__ uint32_t part[4];
   uint32_t v = 0;
-  if (i < n && (nodes[i].flags & FLAG_USED) && (nodes[i].flags & FLAG_LEAF)) v = 1;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    if ((nodes[i].flags & FLAG_USED) && (nodes[i].flags & FLAG_LEAF)) ++v;
   v = wave_sum_u(v);
-  if (__lane_id() == 0 && v) atomicAdd(cnt, v);
+  if (__lane_id() == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) v += part[w];
+    if (v) atomicAdd(cnt, v);
+  }
 }
 
 inline unsigned blocks(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
@@ -1201,7 +1219,8 @@ void bvh_build(pyre_scene* s, const pyre_bvh_params& prm, hipStream_t st) {
   h.region_cursor = small ? 1 + 2 * n : 0;
   h.node_cursor = 1;
   uint32_t L = small ? 0 : 1;
-  hipLaunchKernelGGL(k_prep, dim3(blocks(n, 256)), dim3(256), 0, st, s->d_verts, n, plo, phi, pc, ref, nodes);
+  hipLaunchKernelGGL(k_prep, dim3(std::min<unsigned>(blocks(n, 256), (unsigned)std::max(1, s->ctx->cus) * 8)),
+                     dim3(256), 0, st, s->d_verts, n, plo, phi, pc, ref, nodes);
   uint32_t level = 0;
   while (L > 0) {
     hipLaunchKernelGGL(k_chunk_counts, dim3(blocks(L, 256)), dim3(256), 0, st, nodes, lvl, L, nch);
@@ -1253,7 +1272,8 @@ void bvh_build(pyre_scene* s, const pyre_bvh_params& prm, hipStream_t st) {
   PYRE_HIP_CHECK(hipMemsetAsync(total, 0, 8, st));
   hipLaunchKernelGGL(k_interior_flags, dim3(blocks(nb, 256)), dim3(256), 0, st, nodes, nb, flag);
   exclusive_scan(S, flag, map, nb, total, st);
-  hipLaunchKernelGGL(k_count_leaves, dim3(blocks(nb, 256)), dim3(256), 0, st, nodes, nb, total + 1);
+  hipLaunchKernelGGL(k_count_leaves, dim3(std::min<unsigned>(blocks(nb, 256), (unsigned)std::max(1, s->ctx->cus) * 8)),
+                     dim3(256), 0, st, nodes, nb, total + 1);
   uint32_t ht[2];
   Counters hc{};
   PYRE_HIP_CHECK(hipMemcpyAsync(ht, total, 8, hipMemcpyDeviceToHost, st));

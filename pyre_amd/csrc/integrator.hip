@@ -133,7 +133,7 @@ struct Paths {
   uint32_t* q_shadow;
   uint32_t* counters;  // [0] current, [1] next, [2] shadow, [3] cache queries, [4] training samples,
                        // [8..15] extend work, [16..23] shadow work
-  unsigned long long* stats;  // [0] extension rays, [1] shadow rays
+  unsigned long long* stats;  // [0] extension rays, [1] shadow rays, [2] cache queries
   uint32_t* pjob;    // job of each path
   float4* vrec;      // TRAIN: kVrec float4 per (path, vertex)
   uint32_t* vcount;  // TRAIN: vertices recorded per path
@@ -562,7 +562,9 @@ __device__ void shade_path(const Paths& P, const JobDev& J, const PassDev& pp, u
   P.lum[p] = f4(l, __builtin_bit_cast(float, flags));
 }
 
-__global__ __launch_bounds__(kThreads) void k_shade(Paths P, Sched sc, uint32_t depth) {
+// MINW: minimum waves per SIMD asked of the register allocator (0 = none).
+template <int MINW>
+__global__ __launch_bounds__(kThreads, MINW) void k_shade(Paths P, Sched sc, uint32_t depth) {
   const PassDev pp = cur_pass(sc);
   const uint32_t n = P.counters[0];
   const uint32_t stride = gridDim.x * kThreads;
@@ -689,6 +691,7 @@ __global__ __launch_bounds__(kThreads) void k_train_targets(Paths P, Sched sc, f
 // CACHE: L += beta x max(cache, 0) for every path that ended in the cache.
 __global__ __launch_bounds__(kThreads) void k_cache_apply(Paths P) {
   const uint32_t n = P.counters[3];
+  if (blockIdx.x == 0 && threadIdx.x == 0) P.stats[2] += n;
   const uint32_t stride = gridDim.x * kThreads;
   for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) {
     uint32_t p = P.qpath[i];
@@ -727,6 +730,7 @@ struct Tuning {
   uint32_t pool_paths = 1u << 24;
   uint32_t blocks_per_cu = 8;
   uint32_t ring = 8;
+  uint32_t shade_waves = 1;  // >= 4: shade kernel compiled for 4 waves per SIMD
 };
 
 Tuning env_tuning() {
@@ -735,6 +739,7 @@ Tuning env_tuning() {
   if (const char* e = std::getenv("PYRE_TRACE_BLOCKS_PER_CU")) t.blocks_per_cu = (uint32_t)std::max(1, std::atoi(e));
   if (const char* e = std::getenv("PYRE_STACK_RING")) t.ring = (uint32_t)std::atoi(e);
   if (t.ring != 4 && t.ring != 16) t.ring = 8;
+  if (const char* e = std::getenv("PYRE_SHADE_WAVES")) t.shade_waves = (uint32_t)std::max(1, std::atoi(e));
   return t;
 }
 
@@ -780,7 +785,7 @@ struct Work {
     P.q_shadow = get<uint32_t>(cap);
     P.pjob = get<uint32_t>(cap);
     P.counters = get<uint32_t>(kCounters);
-    P.stats = get<unsigned long long>(2);
+    P.stats = get<unsigned long long>(4);
     P.slots = slots;
     if (vmax) {
       P.vrec = get<float4>((size_t)tpaths * vmax * kVrec);
@@ -879,7 +884,10 @@ void launch_pass(pyre_ctx* ctx, Work& W, const PassShape& shape, const Tuning& t
   hipLaunchKernelGGL(k_generate, dim3(grid), dim3(kThreads), 0, st, P, sc);
   for (uint32_t d = 0; d < shape.max_depth; ++d) {
     launch_trace(W, P, t, tgrid, true, false, st);
-    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kThreads), 0, st, P, sc, d);
+    if (t.shade_waves >= 4)
+      hipLaunchKernelGGL(k_shade<4>, dim3(grid), dim3(kThreads), 0, st, P, sc, d);
+    else
+      hipLaunchKernelGGL(k_shade<1>, dim3(grid), dim3(kThreads), 0, st, P, sc, d);
     if (shape.nee) launch_trace(W, P, t, tgrid, false, true, st);
     hipLaunchKernelGGL(k_rotate, dim3(1), dim3(kCounters), 0, st, P.counters, P.stats);
     std::swap(P.q_cur, P.q_next);
@@ -935,6 +943,7 @@ struct pyre_session {
   float* welford = nullptr;
   uint32_t spp_done = 0, next_sample = 0, train_frames = 0;
   uint64_t stats_seen[2] = {0, 0};  // ray counters already reported
+  uint64_t train_samples = 0;       // radiance-cache training samples produced
   bool use_graph = true;
   hipGraphExec_t exec = nullptr;        // PLAIN pass
   hipGraphExec_t exec_train = nullptr;  // TRAIN pass + training targets
@@ -1051,7 +1060,7 @@ void session_init(pyre_session* S, pyre_ctx* ctx, pyre_scene* s, const pyre_came
   hipLaunchKernelGGL(k_set_job, dim3(1), dim3(1), 0, st, S->W.jobs,
                      job_dev(s, cam, ip, w, h, S->acc, S->acc64, S->welford));
   hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, st, S->W.pass_idx, 0u);
-  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, S->W.P.stats, (size_t)2);
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, S->W.P.stats, (size_t)4);
   S->use_graph = graphs_enabled();
 }
 
@@ -1178,7 +1187,7 @@ float run_batch(Batch& B, const Tuning& t, bool accumulate, uint32_t max_passes,
   PYRE_HIP_CHECK(hipMemcpyAsync(W.passes, sc.passes.data(), sc.passes.size() * sizeof(PassDev),
                                 hipMemcpyHostToDevice, st));
   hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, st, W.pass_idx, 0u);
-  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, W.P.stats, (size_t)2);
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, W.P.stats, (size_t)4);
   PYRE_HIP_CHECK(hipStreamSynchronize(st));  // host tables may go out of scope after this
   auto one_pass = [&] {
     launch_pass(ctx, W, B.shape, t, accumulate, st);
@@ -1413,6 +1422,7 @@ int pyre_session_render_nrc(pyre_ctx* ctx, pyre_session* S, pyre_nrc* nrc, const
       uint32_t nsamp = 0;
       PYRE_HIP_CHECK(hipMemcpyAsync(&nsamp, S->W.P.counters + 4, 4, hipMemcpyDeviceToHost, st));
       PYRE_HIP_CHECK(hipStreamSynchronize(st));
+      S->train_samples += nsamp;
       nsamp = std::min(nsamp, S->W.tcap);
       for (uint32_t k = 0; k < p->train_steps; ++k) {
         const uint32_t off = k * p->train_batch;
@@ -1452,6 +1462,19 @@ int pyre_session_read(pyre_ctx* ctx, pyre_session* S, pyre_pixel_sample* mean, d
                                     hipMemcpyDeviceToHost, st));
     PYRE_HIP_CHECK(hipStreamSynchronize(st));
     if (spp_done) *spp_done = S->spp_done;
+  });
+}
+
+int pyre_session_stats(pyre_ctx* ctx, pyre_session* S, uint64_t out[4]) {
+  if (!ctx || !S || !out) return PYRE_EINVAL;
+  return sguard(ctx, [&] {
+    unsigned long long hs[4] = {0, 0, 0, 0};
+    PYRE_HIP_CHECK(hipMemcpyAsync(hs, S->W.P.stats, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
+    PYRE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    out[0] = hs[0];
+    out[1] = hs[1];
+    out[2] = hs[2];
+    out[3] = S->train_samples;
   });
 }
 
