@@ -116,6 +116,19 @@ def bench_cornell(args):
         "extension_mrays_s": round(ext / per / 1e6, 1), "shadow_mrays_s": round(sh / per / 1e6, 1),
         "rays_per_sample": round((ext + sh) / (W * H * SPP), 3), "device": dev_name,
     }
+    # FP64 reference: 16384 spp accumulated in double on the GPU (samples
+    # 2^20.. so they are independent of the measured ones)
+    ref = None
+    if args.ref_spp:
+        t0 = time.perf_counter()
+        film_ref = pyre_amd.Film(W, H)
+        film_ref.render(scene, cam, ip, args.ref_spp, spp_offset=1 << 20, accum64=True)
+        ref = film_ref.accum64 / args.ref_spp
+        img = film.render(scene, cam, ip, SPP)
+        line["relmse_64spp"] = round(rel_mse(img[:, :3], ref), 6)
+        line["relmse_eps"] = REL_EPS
+        line["reference_spp"] = args.ref_spp
+        log(f"cornell reference {args.ref_spp} spp in {time.perf_counter() - t0:.1f}s")
     if not args.no_cpu:
         from oracle import oracle as O
         threads = O.nthreads()
@@ -123,10 +136,16 @@ def bench_cornell(args):
         ocam = O.camera(cam.origin, cam.target, cam.up, cam.vfov_deg, cam.aspect)
         cspp = 8
         t0 = time.perf_counter()
-        osc.render(ocam, W, H, cspp, threads=threads)
+        cimg = osc.render(ocam, W, H, cspp, threads=threads)
         dt = time.perf_counter() - t0
         line["cpu_baseline"] = {"value": round(W * H * cspp / dt / 1e6, 3), "unit": "Msamples/s", "cores": threads,
                                 "kind": "port", "sample": f"full 512x512 frame at {cspp} spp (of 64)"}
+        # correlated check: the GPU with the same 8 per-sample streams vs the CPU
+        gimg = pyre_amd.Film(W, H).render(scene, cam, ip, cspp)
+        corr = {"relmse_gpu_vs_cpu_8spp": round(rel_mse(gimg[:, :3], cimg.reshape(-1, 10)[:, :3]), 7)}
+        if ref is not None:
+            corr["relmse_cpu_8spp_vs_reference"] = round(rel_mse(cimg.reshape(-1, 10)[:, :3], ref), 6)
+        line["correlated_check"] = corr
     print(json.dumps(line), flush=True)
 
 
@@ -405,9 +424,27 @@ def bench_batch(args):
         "shadow_mrays_s": round(stats["shadow_rays"] / per / 1e6, 1),
         "sequential_gpu_s": round(seq, 3), "batched_over_sequential_speedup": round(seq / per, 3), "device": dev_name,
     }
+    if args.ref_spp:
+        line["relmse_subset"] = batch_relmse(ctx, jobs, args.ref_spp)
     if not args.no_cpu:
         line["cpu_baseline"] = batch_cpu_baseline(jobs, scenes)
     print(json.dumps(line), flush=True)
+
+
+def batch_relmse(ctx, jobs, ref_spp, n=8):
+    """relMSE of the n jobs with the fewest pixels against references of
+    ref_spp samples per pixel (FP64 sums, samples 2^20.., one batch call)."""
+    import pyre_amd
+    order = sorted(range(len(jobs)), key=lambda k: jobs[k].width * jobs[k].height)[:n]
+    outs, _, _ = pyre_amd.render_batch(ctx, [jobs[k] for k in order])
+    refs = [pyre_amd.Job(jobs[k].scene, jobs[k].camera, jobs[k].integrator, jobs[k].width, jobs[k].height, ref_spp,
+                         spp_offset=1 << 20) for k in order]
+    _, acc, _ = pyre_amd.render_batch(ctx, refs, accum64=True)
+    res = []
+    for k, o, a in zip(order, outs, acc):
+        res.append({"job": k, "pixels": jobs[k].width * jobs[k].height, "spp": jobs[k].spp,
+                    "relmse": round(rel_mse(o[:, :3], a / ref_spp), 6)})
+    return {"reference_spp": ref_spp, "relmse_eps": REL_EPS, "jobs": res}
 
 
 def batch_cpu_baseline(jobs, scenes, n=4):
@@ -449,7 +486,8 @@ def main():
                          "batch: 96 uneven jobs in one batched call")
     ap.add_argument("--reference", default=None, help="production reference .npy (relMSE)")
     ap.add_argument("--make-reference", default=None, help="render the production reference to this .npy")
-    ap.add_argument("--ref-spp", type=int, default=16384)
+    ap.add_argument("--ref-spp", type=int, default=16384,
+                    help="spp of the FP64 references (0 = skip the relMSE figures of cornell / batch)")
     ap.add_argument("--max-spp", type=int, default=512, help="nrc: largest spp tried")
     ap.add_argument("--nrc-warmup", type=int, default=16, help="nrc: training-only frames before the first sample")
     args = ap.parse_args()

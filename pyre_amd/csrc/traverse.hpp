@@ -322,7 +322,7 @@ __device__ __forceinline__ void traverse4(const Node4* __restrict__ nodes, const
 }
 
 // Lanes of a wave refill once at least this many are idle.
-constexpr int kRefillLanes = 16;
+constexpr int kRefillLanes = 32;
 // Items each wave takes without any atomic (its static share), and items a
 // wave claims from the shared counter at a time once that is used up.
 constexpr uint32_t kStaticChunk = 64;
