@@ -132,7 +132,7 @@ struct Paths {
   uint32_t* q_next;
   uint32_t* q_shadow;
   uint32_t* counters;  // [0] current, [1] next, [2] shadow, [3] cache queries, [4] training samples,
-                       // [8..15] extend work, [16..23] shadow work
+                       // [8] extend work, [16] shadow work
   unsigned long long* stats;  // [0] extension rays, [1] shadow rays, [2] cache queries
   uint32_t* pjob;    // job of each path
   float4* vrec;      // TRAIN: kVrec float4 per (path, vertex)
@@ -154,6 +154,44 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool push) {
   if ((int)__lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
   base = __shfl(base, leader);
   return base + (uint32_t)__popcll(m & lanemask_lt());
+}
+
+// Block-wide append to Q queues: one global atomic per queue per call for
+// the whole block (contended atomics on a handful of counters are the cost
+// that matters at 2^24 paths per pass, not the LDS traffic). Every thread of
+// the block must call it; slot[q] is meaningful where flag[q] is set.
+constexpr int kWaves = kThreads / 64;
+template <int Q>
+struct BlockQueues {
+  uint32_t cnt[Q][kWaves];
+  uint32_t base[Q][kWaves];
+};
+
+template <int Q>
+__device__ __forceinline__ void block_append(BlockQueues<Q>& sh, uint32_t* const (&ctr)[Q], const bool (&flag)[Q],
+                                             uint32_t (&slot)[Q]) {
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  uint64_t m[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) m[q] = __ballot(flag[q]);
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) sh.cnt[q][wave] = (uint32_t)__popcll(m[q]);
+  }
+  __syncthreads();
+  if (threadIdx.x < (uint32_t)Q && ctr[threadIdx.x]) {
+    const int q = threadIdx.x;
+    uint32_t tot = 0;
+    for (int w = 0; w < kWaves; ++w) tot += sh.cnt[q][w];
+    uint32_t b = tot ? atomicAdd(ctr[q], tot) : 0u;
+    for (int w = 0; w < kWaves; ++w) {
+      sh.base[q][w] = b;
+      b += sh.cnt[q][w];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < Q; ++q) slot[q] = sh.base[q][wave] + (uint32_t)__popcll(m[q] & lanemask_lt());
 }
 
 __device__ __forceinline__ Rng load_rng(const Paths& P, uint32_t p) {
@@ -565,13 +603,16 @@ __device__ void shade_path(const Paths& P, const JobDev& J, const PassDev& pp, u
 // MINW: minimum waves per SIMD asked of the register allocator (0 = none).
 template <int MINW>
 __global__ __launch_bounds__(kThreads, MINW) void k_shade(Paths P, Sched sc, uint32_t depth) {
+  __shared__ BlockQueues<3> qs;
+  __shared__ unsigned long long s_nsh[kWaves];
   const PassDev pp = cur_pass(sc);
   const uint32_t n = P.counters[0];
   const uint32_t stride = gridDim.x * kThreads;
-  // uniform trip count per wave so the ballots below see converged lanes
-  const uint32_t base = blockIdx.x * kThreads + (threadIdx.x & ~63u);
-  for (uint32_t i0 = base; i0 < n; i0 += stride) {
-    uint32_t i = i0 + (threadIdx.x & 63u);
+  uint32_t* const ctr[3] = {&P.counters[1], &P.counters[2], pp.mode == MODE_CACHE ? &P.counters[3] : nullptr};
+  unsigned long long nsh = 0;
+  // block-uniform trip count: the appends below synchronise the block
+  for (uint32_t b0 = blockIdx.x * kThreads; b0 < n; b0 += stride) {
+    const uint32_t i = b0 + threadIdx.x;
     ShadeOut o;
     o.cont = o.shadow = o.query = false;
     o.n_shadow = 0;
@@ -580,19 +621,22 @@ __global__ __launch_bounds__(kThreads, MINW) void k_shade(Paths P, Sched sc, uin
       p = P.q_cur[i];
       shade_path(P, sc.jobs[P.pjob[p]], pp, depth, p, o);
     }
-    uint32_t nsh = o.n_shadow;
-    for (int k = 32; k > 0; k >>= 1) nsh += __shfl_xor(nsh, k);
-    if ((threadIdx.x & 63u) == 0 && nsh) atomicAdd(&P.stats[1], (unsigned long long)nsh);
-    uint32_t qn = wave_append(&P.counters[1], o.cont);
-    if (o.cont) P.q_next[qn] = p;
-    uint32_t qs = wave_append(&P.counters[2], o.shadow);
-    if (o.shadow) P.q_shadow[qs] = p;
-    if (pp.mode == MODE_CACHE) {
-      uint32_t qq = wave_append(&P.counters[3], o.query);
-      if (o.query) {
-        P.qpath[qq] = p;
-      }
-    }
+    nsh += o.n_shadow;
+    const bool flag[3] = {o.cont, o.shadow, o.query};
+    uint32_t slot[3];
+    block_append<3>(qs, ctr, flag, slot);
+    if (o.cont) P.q_next[slot[0]] = p;
+    if (o.shadow) P.q_shadow[slot[1]] = p;
+    if (o.query) P.qpath[slot[2]] = p;
+  }
+  // shadow-ray statistics: one atomic per block
+  for (int k = 32; k > 0; k >>= 1) nsh += __shfl_xor(nsh, k);
+  if ((threadIdx.x & 63u) == 0) s_nsh[threadIdx.x >> 6] = nsh;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < kWaves; ++w) t += s_nsh[w];
+    if (t) atomicAdd(&P.stats[1], t);
   }
 }
 

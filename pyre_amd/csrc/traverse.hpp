@@ -329,12 +329,12 @@ constexpr uint32_t kStaticChunk = 64;
 constexpr uint32_t kWaveChunk = 256;
 
 // Work distribution of the persistent kernels over n items: wave w first
-// takes [w * kStaticChunk, (w + 1) * kStaticChunk) with no atomic, so a
-// short queue (late bounces, small images) costs no atomics at all; the
-// items past the static shares are claimed kWaveChunk at a time from one
-// counter, which balances expensive and cheap rays across the GPU. A wave
-// that sees the counter reach the end stops claiming. All members are
-// wave-uniform.
+// takes a static share [w * share, (w + 1) * share) with no atomic (share =
+// half of an even split, at least kStaticChunk), so a short queue costs no
+// atomics at all and a long one few; the items past the static shares are
+// claimed kWaveChunk at a time from one counter, which balances expensive
+// and cheap rays across the GPU. A wave that sees the counter reach the end
+// stops claiming. All members are wave-uniform.
 struct WaveFeed {
   uint32_t n, next, end, dyn0;
   uint32_t* ctr;
@@ -345,9 +345,13 @@ struct WaveFeed {
     const uint32_t waves_per_block = blockDim.x / 64u;
     const uint64_t wave = (uint64_t)blockIdx.x * waves_per_block + (threadIdx.x >> 6);
     const uint64_t waves = (uint64_t)gridDim.x * waves_per_block;
-    const uint64_t s0 = wave * kStaticChunk, d0 = waves * kStaticChunk;
+    // static share: half of an even split (at least kStaticChunk), the rest
+    // is claimed dynamically
+    const uint64_t even = (uint64_t)n / waves / 2;
+    const uint64_t share = even > kStaticChunk ? (even & ~63ull) : kStaticChunk;
+    const uint64_t s0 = wave * share, d0 = waves * share;
     next = s0 < n ? (uint32_t)s0 : n;
-    end = n - next > kStaticChunk ? next + kStaticChunk : n;
+    end = n - next > share ? (uint32_t)(next + share) : n;
     dyn0 = d0 < n ? (uint32_t)d0 : n;
     dry = dyn0 >= n;
   }
