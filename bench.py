@@ -289,6 +289,11 @@ def _crossing(curve, target):
     return None
 
 
+# Training paths per frame: most camera rays of the production view see the
+# sky, so 2^18 paths give about 2^16 surface-vertex training samples.
+NRC_TRAIN_PATHS = 1 << 18
+
+
 def bench_nrc(args):
     """Seconds to relMSE 0.01 at 1080p on the production scene: the plain
     path tracer vs the radiance cache (trained online from a cold start, paths
@@ -311,7 +316,7 @@ def bench_nrc(args):
     def run(kind):
         lo, hi = scene.bounds()
         cache = pyre_amd.RadianceCache(ctx, lo, hi, seed=5) if kind == "nrc" else None
-        sess = pyre_amd.Session(scene, cam, ip, PROD_W, PROD_H, nrc_train_paths=(1 << 16) if cache else 0)
+        sess = pyre_amd.Session(scene, cam, ip, PROD_W, PROD_H, nrc_train_paths=NRC_TRAIN_PATHS if cache else 0)
         curve, t_acc, done = [], 0.0, 0
         for c in checkpoints:
             n = c - done
@@ -321,7 +326,7 @@ def bench_nrc(args):
             else:
                 # cold start: the first call trains for nrc_warmup frames before any image sample
                 warm = args.nrc_warmup if done == 0 else 0
-                sess.render_nrc(cache, frames=n + warm, spp_per_frame=1, train_paths=1 << 14,
+                sess.render_nrc(cache, frames=n + warm, spp_per_frame=1, train_paths=NRC_TRAIN_PATHS,
                                 train_batch=1 << 14, train_steps=4, warmup_frames=warm)
             t_acc += time.perf_counter() - t0
             done = c
@@ -348,7 +353,7 @@ def bench_nrc(args):
         "vs_baseline": None, "dtype": "f32 (cache: bf16 MFMA, f32 accumulate)",
         "data": "synthetic (procedural scene, seed 3)",
         "config": {"workload": "production scene 1920x1080, max depth 12, thin lens 0.05; NRC 5x64 ReLU MLP, "
-                               "2^16 training samples/frame (4 Adam steps x 2^14), lr 1e-2, "
+                               "2^18 training paths/frame (~2^16 samples; 4 Adam steps x 2^14), lr 1e-2, "
                                f"{args.nrc_warmup} training-only frames from a cold start (timed)",
                    "relmse_eps": REL_EPS, "reference_spp": args.ref_spp},
         "plain_seconds": round(tp, 4) if tp else None, "nrc_seconds": round(tn, 4) if tn else None,
@@ -489,7 +494,7 @@ def main():
     ap.add_argument("--ref-spp", type=int, default=16384,
                     help="spp of the FP64 references (0 = skip the relMSE figures of cornell / batch)")
     ap.add_argument("--max-spp", type=int, default=512, help="nrc: largest spp tried")
-    ap.add_argument("--nrc-warmup", type=int, default=16, help="nrc: training-only frames before the first sample")
+    ap.add_argument("--nrc-warmup", type=int, default=4, help="nrc: training-only frames before the first sample")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "

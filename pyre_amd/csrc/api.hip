@@ -366,6 +366,8 @@ int pyre_ctx_create(int device, pyre_ctx** out) {
 void pyre_ctx_destroy(pyre_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  ctx->render_cache.reset();
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }

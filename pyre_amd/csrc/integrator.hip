@@ -145,17 +145,6 @@ struct Paths {
 
 __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << __lane_id()) - 1ull; }
 
-// Wave-aggregated queue append: every lane with `push` gets a distinct slot.
-__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool push) {
-  uint64_t m = __ballot(push);
-  if (m == 0) return 0;
-  int leader = __ffsll((unsigned long long)m) - 1;
-  uint32_t base = 0;
-  if ((int)__lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
-  base = __shfl(base, leader);
-  return base + (uint32_t)__popcll(m & lanemask_lt());
-}
-
 // Block-wide append to Q queues: one global atomic per queue per call for
 // the whole block (contended atomics on a handful of counters are the cost
 // that matters at 2^24 paths per pass, not the LDS traffic). Every thread of
@@ -698,12 +687,13 @@ __global__ __launch_bounds__(kThreads) void k_accumulate(Paths P, Sched sc) {
 // into (feats, targets) up to cap samples.
 __global__ __launch_bounds__(kThreads) void k_train_targets(Paths P, Sched sc, float* feats, float* targets,
                                                             uint32_t cap) {
+  __shared__ BlockQueues<1> qs;
   const PassDev pp = cur_pass(sc);
   const uint32_t total = pp.paths * pp.vmax;
   const uint32_t stride = gridDim.x * kThreads;
-  const uint32_t base = blockIdx.x * kThreads + (threadIdx.x & ~63u);
-  for (uint32_t i0 = base; i0 < total; i0 += stride) {
-    uint32_t i = i0 + (threadIdx.x & 63u);
+  uint32_t* const ctr[1] = {&P.counters[4]};
+  for (uint32_t b0 = blockIdx.x * kThreads; b0 < total; b0 += stride) {  // block-uniform
+    const uint32_t i = b0 + threadIdx.x;
     bool ok = false;
     float f[13], t[3];
     if (i < total) {
@@ -724,7 +714,10 @@ __global__ __launch_bounds__(kThreads) void k_train_targets(Paths P, Sched sc, f
         }
       }
     }
-    uint32_t slot = wave_append(&P.counters[4], ok);
+    const bool flag[1] = {ok};
+    uint32_t slots[1];
+    block_append<1>(qs, ctr, flag, slots);
+    const uint32_t slot = slots[0];
     if (ok && slot < cap) {
       for (int q = 0; q < 13; ++q) feats[13 * (size_t)slot + q] = f[q];
       for (int c = 0; c < 3; ++c) targets[3 * (size_t)slot + c] = t[c];
@@ -788,9 +781,12 @@ Tuning env_tuning() {
 }
 
 // Device path state for up to `cap` paths, plus the job/slice/pass tables.
+// Movable, so a context can keep one for the next one-shot render.
 struct Work {
   size_t cap = 0;
   uint32_t slots = 1, vmax = 0, tcap = 0, tpaths = 0;  // tpaths: training paths with vertex records
+  bool cache = false;
+  size_t n_jobs = 0, n_slices = 0, n_passes = 0;
   Paths P{};
   JobDev* jobs = nullptr;
   SliceDev* slices = nullptr;
@@ -807,13 +803,62 @@ struct Work {
     bufs.push_back(p);
     return p;
   }
-  void alloc(size_t cap_, uint32_t slots_, uint32_t vmax_, uint32_t tpaths_, uint32_t tcap_, bool cache,
-             size_t n_jobs, size_t n_slices, size_t n_passes) {
+  Work() = default;
+  Work(const Work&) = delete;
+  Work& operator=(const Work&) = delete;
+  Work& operator=(Work&& o) noexcept {
+    if (this == &o) return *this;
+    release();
+    cap = o.cap;
+    slots = o.slots;
+    vmax = o.vmax;
+    tcap = o.tcap;
+    tpaths = o.tpaths;
+    cache = o.cache;
+    n_jobs = o.n_jobs;
+    n_slices = o.n_slices;
+    n_passes = o.n_passes;
+    P = o.P;
+    jobs = o.jobs;
+    slices = o.slices;
+    passes = o.passes;
+    pass_idx = o.pass_idx;
+    gstack = o.gstack;
+    gstack_cap = o.gstack_cap;
+    tfeat = o.tfeat;
+    ttarget = o.ttarget;
+    bufs = std::move(o.bufs);
+    o.bufs.clear();
+    o.cap = 0;
+    o.gstack = nullptr;
+    o.gstack_cap = 0;
+    return *this;
+  }
+  void release() {
+    for (void* p : bufs) (void)hipFree(p);
+    bufs.clear();
+    cap = 0;
+    gstack = nullptr;
+    gstack_cap = 0;
+  }
+  // Buffers large enough for this request (radiance-cache sizes must match).
+  bool fits(size_t cap_, uint32_t slots_, uint32_t vmax_, uint32_t tpaths_, uint32_t tcap_, bool cache_,
+            size_t nj, size_t ns, size_t np) const {
+    return !bufs.empty() && cap >= cap_ && slots >= slots_ && vmax == vmax_ && tpaths == tpaths_ && tcap == tcap_ &&
+           cache == cache_ && n_jobs >= nj && n_slices >= ns && n_passes >= np;
+  }
+  void alloc(size_t cap_, uint32_t slots_, uint32_t vmax_, uint32_t tpaths_, uint32_t tcap_, bool cache_,
+             size_t n_jobs_, size_t n_slices_, size_t n_passes_) {
+    release();
     cap = cap_;
     slots = slots_;
     vmax = vmax_;
     tpaths = tpaths_;
     tcap = tcap_;
+    cache = cache_;
+    n_jobs = n_jobs_;
+    n_slices = n_slices_;
+    n_passes = n_passes_;
     P.ray = get<float4>(2 * cap);
     P.rng = get<ulonglong2>(2 * cap);
     P.beta_pdf = get<float4>(cap);
@@ -837,14 +882,14 @@ struct Work {
       tfeat = get<float>(13 * (size_t)tcap);
       ttarget = get<float>(3 * (size_t)tcap);
     }
-    if (cache) {
+    if (cache_) {
       P.qfeat = get<float>(13 * cap);
       P.qpath = get<uint32_t>(cap);
       P.qrgb = get<float>(3 * cap);
     }
-    jobs = get<JobDev>(n_jobs);
-    slices = get<SliceDev>(n_slices);
-    passes = get<PassDev>(n_passes);
+    jobs = get<JobDev>(n_jobs_);
+    slices = get<SliceDev>(n_slices_);
+    passes = get<PassDev>(n_passes_);
     pass_idx = get<uint32_t>(1);
   }
   // Spill columns for `threads` traversal threads on trees of binary depth
@@ -861,9 +906,7 @@ struct Work {
     gstack_cap = need;
   }
   Sched sched() const { return Sched{jobs, slices, passes, pass_idx}; }
-  ~Work() {
-    for (void* p : bufs) (void)hipFree(p);
-  }
+  ~Work() { release(); }
 };
 
 SceneDev scene_dev(pyre_scene* s) {
@@ -1087,8 +1130,9 @@ void session_init(pyre_session* S, pyre_ctx* ctx, pyre_scene* s, const pyre_came
   require(cap < (1ull << 31), "path pool too large");
   const uint32_t slots = std::max(1u, s->n_lights + (s->has_env ? 1u : 0u));
   const uint32_t vmax = train_paths ? std::max(1u, ip.max_depth) : 0;
-  S->W.alloc(cap, slots, vmax, train_paths, train_paths ? std::min<uint32_t>(train_paths * vmax, 1u << 20) : 0,
-             train_paths > 0, 1, 1, 1);
+  const uint32_t tcap = train_paths ? std::min<uint32_t>(train_paths * vmax, 1u << 20) : 0;
+  if (!S->W.fits(cap, slots, vmax, train_paths, tcap, train_paths > 0, 1, 1, 1))
+    S->W.alloc(cap, slots, vmax, train_paths, tcap, train_paths > 0, 1, 1, 1);
   S->W.ensure_gstack((size_t)trace_grid_of(ctx, S->tune) * kThreads, s->depth);
   S->shape.max_depth = ip.max_depth;
   S->shape.tree_depth = s->depth;
@@ -1322,6 +1366,11 @@ int pyre_render(pyre_ctx* ctx, pyre_scene* s, const pyre_camera_desc* camera, co
   if (!ctx || !s || !camera || !integrator) return PYRE_EINVAL;
   return sguard(ctx, [&] {
     pyre_session S;
+    // path buffers of the previous one-shot render on this context, if any
+    if (ctx->render_cache) {
+      S.W = std::move(*static_cast<Work*>(ctx->render_cache.get()));
+      ctx->render_cache.reset();
+    }
     session_init(&S, ctx, s, *camera, *integrator, width, height, 0, std::max(1u, spp));
     hipStream_t st = ctx->stream;
     const size_t n3 = 3 * (size_t)S.npix;
@@ -1331,6 +1380,10 @@ int pyre_render(pyre_ctx* ctx, pyre_scene* s, const pyre_camera_desc* camera, co
     if (out) read_mean(ctx, S.acc, S.npix, out, 1.0f / (float)spp);  // main.rs:255
     if (accum64) PYRE_HIP_CHECK(hipMemcpyAsync(accum64, S.acc64, n3 * sizeof(double), hipMemcpyDeviceToHost, st));
     finish_stats(&S);
+    // keep the path buffers for the next one-shot render on this context
+    auto keep = std::make_shared<Work>();
+    *keep = std::move(S.W);
+    ctx->render_cache = std::static_pointer_cast<void>(keep);
   });
 }
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -17,6 +18,9 @@ struct pyre_ctx {
   std::string err;
   std::string name;
   int cus = 0;
+  // path buffers kept from the last one-shot render (owned; freed with the
+  // context, on its device)
+  std::shared_ptr<void> render_cache;
 };
 
 namespace pyre {
