@@ -1223,7 +1223,8 @@ void bvh_build(pyre_scene* s, const pyre_bvh_params& prm, hipStream_t st) {
                      dim3(256), 0, st, s->d_verts, n, plo, phi, pc, ref, nodes);
   uint32_t level = 0;
   while (L > 0) {
-    hipLaunchKernelGGL(k_chunk_counts, dim3(blocks(L, 256)), dim3(256), 0, st, nodes, lvl, L, nch);
+    // L + 1 threads: the last writes the scan sentinel nch[L] = 0
+    hipLaunchKernelGGL(k_chunk_counts, dim3(blocks((size_t)L + 1, 256)), dim3(256), 0, st, nodes, lvl, L, nch);
     exclusive_scan(S, nch, nch_off, L + 1, &ctr->total_chunks, st);
     hipLaunchKernelGGL(k_chunk_fill, dim3(blocks(L, 256)), dim3(256), 0, st, nodes, lvl, L, nch_off, chunk_node,
                        chunk_first);

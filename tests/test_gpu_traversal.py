@@ -41,6 +41,28 @@ def test_bvh_topology_matches_reference_build(ctx, oracle, n, bins, leaf, seed):
     assert info["depth"] == depth
 
 
+def test_bvh_builds_in_sequence(ctx, oracle):
+    """Scenes of growing size built one after another in one process, so the
+    builder's scratch memory is recycled between builds (a scan sentinel once
+    read a stale value there): topology equal to the reference build for the
+    smaller ones, node/leaf/depth counts for the larger ones."""
+    for lg in (12, 14, 16, 18, 20):
+        n = 1 << lg
+        print(f"[bvh sequence] n=2^{lg}", flush=True)
+        tris = pyre_amd.random_triangles(30 + lg, n, 0.01 * 2 ** ((20 - lg) / 3))
+        sc, pos, idx = _soup_scene(ctx, tris)
+        info = sc.build_bvh(bins=32, max_leaf=4)
+        o_sc = oracle.Scene()
+        o_sc.add_material(oracle.material(0, (0.5, 0.5, 0.5)))
+        o_sc.add_mesh(pos, idx, None, 0, 32, 4)
+        o = canonical_oracle_tree(o_sc.bvh(0))
+        nodes, leaves, depth = tree_stats(o)
+        assert (info["n_leaves"], info["depth"]) == (leaves, depth)
+        if lg <= 16:
+            assert canonical_gpu_tree(sc.bvh_export(n)) == o
+        sc.close()
+
+
 def test_bvh_topology_uv_sphere(ctx, oracle):
     """The Cornell preset's gold sphere (main.rs:455), reference parameters."""
     o_cb = oracle.Scene.cornell()
