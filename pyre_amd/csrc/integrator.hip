@@ -895,7 +895,7 @@ struct Work {
   // Spill columns for `threads` traversal threads on trees of binary depth
   // `depth` (3 pending children per level); allocated before any capture.
   void ensure_gstack(size_t threads, uint32_t depth) {
-    const size_t need = threads * (3 * (size_t)depth + 4);
+    const size_t need = threads * (3 * (size_t)depth + 8);
     if (need <= gstack_cap) return;
     PYRE_HIP_CHECK(hipDeviceSynchronize());
     if (gstack) {

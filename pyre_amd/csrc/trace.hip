@@ -217,11 +217,12 @@ unsigned trace_grid(pyre_scene* s) {
   return (unsigned)std::max(1, s->ctx->cus) * (unsigned)per_cu;
 }
 
-// Spill columns for the persistent kernels: 3 entries per binary level bound
-// the pending children of every 4-wide node on the current path.
+// Spill columns for the persistent kernels: a 4-wide node visit leaves at
+// most 3 pending siblings per level of the current path, and the 4-wide
+// tree is no deeper than the binary one: 3 * depth entries (+ 8 spare).
 int2* trace_gstack(pyre_scene* s) {
   const size_t threads = (size_t)trace_grid(s) * kTraceThreads;
-  const size_t need = threads * (3 * (size_t)s->depth + 4);
+  const size_t need = threads * (3 * (size_t)s->depth + 8);
   if (!s->d_fetch) s->d_fetch = dev_alloc<uint32_t>(64);
   if (need > s->gstack_cap) {
     PYRE_HIP_CHECK(hipDeviceSynchronize());
