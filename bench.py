@@ -290,16 +290,35 @@ def _crossing(curve, target):
 
 
 # Training paths per frame: most camera rays of the production view see the
-# sky, so 2^18 paths give about 2^16 surface-vertex training samples.
-NRC_TRAIN_PATHS = 1 << 18
+# sky, so 2^20 paths give about 2^18 surface-vertex training samples (2^15
+# per spp of an 8-spp frame).
+NRC_TRAIN_PATHS = 1 << 20
+# Path pool of the plain-vs-NRC comparison: 16 spp per pass at 1080p for both.
+NRC_POOL_PATHS = 1 << 25
+
+
+def _at_time(curve, t):
+    """relMSE at time t on a (time, relMSE) curve, log-log interpolated; None
+    outside the measured span."""
+    for (t0, e0), (t1, e1) in zip(curve, curve[1:]):
+        if t0 <= t <= t1:
+            a = (np.log(t) - np.log(t0)) / (np.log(t1) - np.log(t0))
+            return float(np.exp(np.log(e0) + a * (np.log(e1) - np.log(e0))))
+    return None
 
 
 def bench_nrc(args):
     """Seconds to relMSE 0.01 at 1080p on the production scene: the plain
-    path tracer vs the radiance cache (trained online from a cold start, paths
-    ending in the cache after the first diffuse bounce). Both render
-    progressive 1-spp passes; relMSE is checked against the FP64 reference at
-    checkpoints outside the timed region."""
+    path tracer vs the radiance cache (paths ending in the cache after the
+    first diffuse bounce). Every checkpoint is its own render from a cold
+    start, timed whole: plain = one session render of N spp; NRC = a fresh
+    cache, nrc_warmup training-only frames, then frames of min(N, 16) spp.
+    Each NRC frame traces its samples and 2^20 training paths in one pass,
+    trains on the training paths' ~2^18 samples (Adam, batches of 2^14, every
+    sample once), then adds the updated cache's radiance to the paths that
+    ended in it. Both use a pool of 2^25 paths (passes of up to 16 spp).
+    relMSE against the FP64 reference is taken outside the timed region."""
+    os.environ["PYRE_PASS_PATHS"] = str(NRC_POOL_PATHS)
     import pyre_amd
     ref = load_reference(args.reference)
     if ref is None:
@@ -309,57 +328,63 @@ def bench_nrc(args):
     dev_name, _ = ctx.device_info()
     pre, scene, cam, info = production_scene(ctx)
     ip = pyre_amd.PathIntegrator(12, 3)
+    lo, hi = scene.bounds()
     target = 0.01
-    checkpoints = [1, 2, 4, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512]
-    checkpoints = [c for c in checkpoints if c <= args.max_spp]
+    checkpoints = [c for c in (4, 6, 8, 10, 12, 16, 24, 32, 48, 64, 96, 128) if c <= args.max_spp]
+    stats = {}
 
-    def run(kind):
-        lo, hi = scene.bounds()
+    def render(kind, n):
         cache = pyre_amd.RadianceCache(ctx, lo, hi, seed=5) if kind == "nrc" else None
         sess = pyre_amd.Session(scene, cam, ip, PROD_W, PROD_H, nrc_train_paths=NRC_TRAIN_PATHS if cache else 0)
-        curve, t_acc, done = [], 0.0, 0
-        for c in checkpoints:
-            n = c - done
-            t0 = time.perf_counter()
-            if cache is None:
-                sess.render(n)
-            else:
-                # cold start: the first call trains for nrc_warmup frames before any image sample
-                warm = args.nrc_warmup if done == 0 else 0
-                sess.render_nrc(cache, frames=n + warm, spp_per_frame=1, train_paths=NRC_TRAIN_PATHS,
-                                train_batch=1 << 14, train_steps=4, warmup_frames=warm)
-            t_acc += time.perf_counter() - t0
-            done = c
-            _, r64, spp = sess.read(mean=False, radiance64=True)
-            e = rel_mse(r64 / spp, ref)
-            curve.append((t_acc, e))
-            log(f"{kind}: {c} spp, {t_acc:.3f}s, relMSE {e:.5f}")
-            if e <= target * 0.7:
-                break
-        stats[kind] = sess.stats()
-        stats[kind]["samples"] = done * PROD_W * PROD_H
+        t0 = time.perf_counter()  # session set-up has synchronised the stream
+        if cache is None:
+            sess.render(n)
+        else:
+            spf = min(n, 16)
+            sess.render_nrc(cache, frames=args.nrc_warmup + n // spf, spp_per_frame=spf,
+                            train_paths=NRC_TRAIN_PATHS, train_batch=1 << 14, train_steps=64,
+                            warmup_frames=args.nrc_warmup)
+        dt = time.perf_counter() - t0  # render calls return after the stream is idle
+        _, r64, spp = sess.read(mean=False, radiance64=True)
+        assert spp == n
+        e = rel_mse(r64 / spp, ref)
+        stats[kind] = dict(sess.stats(), samples=n * PROD_W * PROD_H, spp=n)
         sess.close()
         if cache is not None:
             cache.close()
+        log(f"{kind}: {n} spp from a cold start, {dt:.4f}s, relMSE {e:.5f}")
+        return dt, e
+
+    def run(kind):
+        render(kind, checkpoints[0])  # untimed: first-use costs (graph capture, code load)
+        curve = []
+        for n in checkpoints:
+            if kind == "nrc" and n > 16 and n % 16:
+                continue
+            curve.append(render(kind, n))
+            if curve[-1][1] <= target * 0.7:
+                break
         return curve
 
-    stats = {}
     plain = run("plain")
     nrc = run("nrc")
     tp, tn = _crossing(plain, target), _crossing(nrc, target)
+    eq = _at_time(nrc, tp) if tp else None
     line = {
         "metric": "seconds to relMSE 0.01 at 1080p, plain vs NRC", "value": round(tn, 4) if tn else None,
         "unit": "s (NRC)", "n_gpus": 1, "steps": 1, "warmup": 0, "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32 (cache: bf16 MFMA, f32 accumulate)",
         "data": "synthetic (procedural scene, seed 3)",
         "config": {"workload": "production scene 1920x1080, max depth 12, thin lens 0.05; NRC 5x64 ReLU MLP, "
-                               "2^18 training paths/frame (~2^16 samples; 4 Adam steps x 2^14), lr 1e-2, "
-                               f"{args.nrc_warmup} training-only frames from a cold start (timed)",
+                               "2^20 training paths/frame (~2^18 samples, Adam batches of 2^14), lr 1e-2, "
+                               f"{args.nrc_warmup} training-only frames, frames of min(N, 16) spp; "
+                               "pool 2^25 paths for both; every checkpoint rendered from a cold start",
                    "relmse_eps": REL_EPS, "reference_spp": args.ref_spp},
         "plain_seconds": round(tp, 4) if tp else None, "nrc_seconds": round(tn, 4) if tn else None,
         "nrc_over_plain": round(tn / tp, 4) if (tn and tp) else None,
+        "relmse_at_plain_seconds": {"plain": target, "nrc": round(eq, 5) if eq else None},
         "plain_curve": [(round(t, 4), round(e, 5)) for t, e in plain],
-        "nrc_curve": [(round(t, 4), round(e, 5)) for t, e in nrc], "path_stats": stats, "device": dev_name,
+        "nrc_curve": [(round(t, 4), round(e, 5)) for t, e in nrc], "path_stats_last": stats, "device": dev_name,
     }
     print(json.dumps(line), flush=True)
 
@@ -493,8 +518,8 @@ def main():
     ap.add_argument("--make-reference", default=None, help="render the production reference to this .npy")
     ap.add_argument("--ref-spp", type=int, default=16384,
                     help="spp of the FP64 references (0 = skip the relMSE figures of cornell / batch)")
-    ap.add_argument("--max-spp", type=int, default=512, help="nrc: largest spp tried")
-    ap.add_argument("--nrc-warmup", type=int, default=4, help="nrc: training-only frames before the first sample")
+    ap.add_argument("--max-spp", type=int, default=128, help="nrc: largest spp tried")
+    ap.add_argument("--nrc-warmup", type=int, default=0, help="nrc: training-only frames before the first sample")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "

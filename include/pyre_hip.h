@@ -342,15 +342,16 @@ void pyre_session_destroy(pyre_session* session);
 /* Adds spp samples per pixel (plain path tracer). */
 int pyre_session_render(pyre_ctx* ctx, pyre_session* session, uint32_t spp);
 
-/* Radiance-cache rendering (no counterpart in the reference). Per frame:
- * train_paths full unbiased paths from scattered pixels are traced, every
- * surface vertex yields a training sample (the query there and the radiance
- * the rest of the path carried back to it), train_steps Adam steps run on
- * batches of train_batch samples, then spp_per_frame samples per pixel are
- * rendered with paths ending in the cache at the first surface reached after
- * a bounce off a diffuse material (Lambertian, or Disney roughness >=
- * diffuse_roughness). Frames before warmup_frames only train (no image
- * samples). */
+/* Radiance-cache rendering (no counterpart in the reference). Per frame, in
+ * one pass over the path pool: spp_per_frame samples per pixel are traced,
+ * paths stopping at the first surface reached after a bounce off a diffuse
+ * material (Lambertian, or Disney roughness >= diffuse_roughness); beside
+ * them train_paths full unbiased paths from scattered pixels are traced, every
+ * surface vertex of which yields a training sample (the query there and the
+ * radiance the rest of the path carried back to it). Then up to train_steps
+ * Adam steps run on consecutive batches of train_batch of the frame's
+ * samples, and the stopped paths add the updated cache's radiance there.
+ * Frames before warmup_frames only train (no image samples). */
 typedef struct pyre_nrc_render_params {
   uint32_t frames;
   uint32_t spp_per_frame;

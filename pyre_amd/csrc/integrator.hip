@@ -24,12 +24,13 @@
 // surface bounce: 2 per light, 2 for the environment, 2 for the BSDF, 1 for
 // Russian roulette), so a GPU path consumes the same stream as the CPU one.
 //
-// Radiance-cache modes (no counterpart in the reference): TRAIN records, at
-// every surface vertex of a full unbiased path, the cache query and the
-// radiance/throughput before the vertex, so that after the path ends the
-// vertex's outgoing radiance estimate is (L_end - L_before) / beta; CACHE ends
-// a path at the first surface vertex reached by a bounce off a diffuse
-// material and adds beta x cache(query) there.
+// Radiance cache (no counterpart in the reference): a CACHE pass ends a
+// render path at the first surface vertex reached by a bounce off a diffuse
+// material and adds beta x cache(query) there. The same pass may carry
+// training paths after its render paths: full unbiased paths that record, at
+// every surface vertex, the cache query and the radiance/throughput before
+// the vertex, so that after the path ends the vertex's outgoing radiance
+// estimate is (L_end - L_before) / beta.
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
@@ -47,8 +48,8 @@ using namespace pyre;
 namespace {
 
 constexpr int kThreads = 256;
-constexpr uint32_t FL_SPECULAR = 1u, FL_AOV = 2u, FL_DIFFUSE = 4u;
-constexpr uint32_t MODE_PLAIN = 0u, MODE_TRAIN = 1u, MODE_CACHE = 2u;
+constexpr uint32_t FL_SPECULAR = 1u, FL_AOV = 2u, FL_DIFFUSE = 4u, FL_TRAIN = 8u;
+constexpr uint32_t MODE_PLAIN = 0u, MODE_CACHE = 2u;
 constexpr float kRayEps = 1e-3f;  // integrator.rs:34
 constexpr uint32_t kVrec = 5;     // float4 per training vertex record
 constexpr uint32_t kTrainStreamBase = 0x80000000u;  // sample indices of training paths
@@ -85,11 +86,16 @@ struct SliceDev {
   uint32_t job, s_base, n_samples, path0, pix0, n_before;
 };
 
-// One pass over the path pool: slices [slice0, slice0 + n_slices).
+// One pass over the path pool: slices [slice0, slice0 + n_slices) hold the
+// render paths [0, train0) and accumulation items [0, pixels). Paths
+// [train0, paths) are radiance-cache training paths of job slices[slice0].job:
+// training path k is sample t_sbase + k / npix of pixel (k * pix_mul +
+// pix_add) % npix.
 struct PassDev {
   uint32_t slice0, n_slices, paths, pixels;
   uint32_t mode, pix_mul, pix_add, vmax;
   float diffuse_rough;
+  uint32_t train0, t_sbase;
 };
 
 // What every stage reads from device memory: the pass being run is
@@ -135,8 +141,8 @@ struct Paths {
                        // [8] extend work, [16] shadow work
   unsigned long long* stats;  // [0] extension rays, [1] shadow rays, [2] cache queries
   uint32_t* pjob;    // job of each path
-  float4* vrec;      // TRAIN: kVrec float4 per (path, vertex)
-  uint32_t* vcount;  // TRAIN: vertices recorded per path
+  float4* vrec;      // training paths: kVrec float4 per (training path, vertex)
+  uint32_t* vcount;  // training paths: vertices recorded
   float* qfeat;      // CACHE: 13 floats per path (its query, if it ended in the cache)
   uint32_t* qpath;   // CACHE: path of each query
   float* qrgb;       // CACHE: cache output per query
@@ -227,14 +233,24 @@ __global__ __launch_bounds__(kThreads) void k_generate(Paths P, Sched sc) {
   const PassDev pp = cur_pass(sc);
   const uint32_t stride = gridDim.x * kThreads;
   for (uint32_t p = blockIdx.x * kThreads + threadIdx.x; p < pp.paths; p += stride) {
-    const SliceDev slc = sc.slices[find_slice(sc, pp, p, [](const SliceDev& q) { return q.path0; })];
-    const JobDev& J = sc.jobs[slc.job];
-    const uint32_t local = p - slc.path0;
+    const bool train = p >= pp.train0;
+    uint32_t job, local, s_base;
+    if (train) {
+      job = sc.slices[pp.slice0].job;
+      local = p - pp.train0;
+      s_base = pp.t_sbase;
+    } else {
+      const SliceDev slc = sc.slices[find_slice(sc, pp, p, [](const SliceDev& q) { return q.path0; })];
+      job = slc.job;
+      local = p - slc.path0;
+      s_base = slc.s_base;
+    }
+    const JobDev& J = sc.jobs[job];
     const uint32_t npix = J.npix, width = J.width;
-    uint32_t pixel = (uint32_t)(((uint64_t)local * pp.pix_mul + pp.pix_add) % npix);
-    uint32_t sl = local / npix;
+    const uint32_t pixel = train ? (uint32_t)(((uint64_t)local * pp.pix_mul + pp.pix_add) % npix) : local % npix;
+    const uint32_t sl = local / npix;
     uint32_t x = pixel % width, y = pixel / width;
-    Rng rng = rng_seed(pixel_seed(x, y, slc.s_base + sl));
+    Rng rng = rng_seed(pixel_seed(x, y, s_base + sl));
     float jx = rng_f32(rng);
     float jy = rng_f32(rng);
     float ndc_x = 2.0f * ((float)x + jx) / (float)width - 1.0f;
@@ -247,11 +263,11 @@ __global__ __launch_bounds__(kThreads) void k_generate(Paths P, Sched sc) {
     P.ray[2 * (size_t)p + 1] = f4(d, kInf);
     store_rng(P, p, rng);
     P.beta_pdf[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    P.lum[p] = make_float4(0.0f, 0.0f, 0.0f, __builtin_bit_cast(float, FL_SPECULAR));
+    P.lum[p] = make_float4(0.0f, 0.0f, 0.0f, __builtin_bit_cast(float, FL_SPECULAR | (train ? FL_TRAIN : 0u)));
     P.aov0[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     P.aov1[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    P.pjob[p] = slc.job;
-    if (pp.mode == MODE_TRAIN) P.vcount[p] = 0;
+    P.pjob[p] = job;
+    if (train) P.vcount[local] = 0;
     P.q_cur[p] = p;
   }
 }
@@ -483,22 +499,23 @@ __device__ void shade_path(const Paths& P, const JobDev& J, const PassDev& pp, u
     const float rough = mat.kind == PYRE_MAT_LAMBERTIAN ? 1.0f : mat.roughness;
     const float qf[13] = {pos.x, pos.y, pos.z, -rd.x, -rd.y, -rd.z, ns.x, ns.y, ns.z, rough,
                           mat.base.x, mat.base.y, mat.base.z};
-    if (pp.mode == MODE_CACHE && depth >= 1 && (flags & FL_DIFFUSE)) {
+    if (!(flags & FL_TRAIN) && depth >= 1 && (flags & FL_DIFFUSE)) {
       for (int k = 0; k < 13; ++k) P.qfeat[13 * (size_t)p + k] = qf[k];
       o.query = true;
       P.lum[p] = f4(l, __builtin_bit_cast(float, flags));
       return;
     }
-    if (pp.mode == MODE_TRAIN) {
-      uint32_t k = P.vcount[p];
+    if (flags & FL_TRAIN) {
+      const uint32_t tp = p - pp.train0;
+      uint32_t k = P.vcount[tp];
       if (k < pp.vmax) {
-        float4* v = P.vrec + ((size_t)p * pp.vmax + k) * kVrec;
+        float4* v = P.vrec + ((size_t)tp * pp.vmax + k) * kVrec;
         v[0] = make_float4(qf[0], qf[1], qf[2], qf[3]);
         v[1] = make_float4(qf[4], qf[5], qf[6], qf[7]);
         v[2] = make_float4(qf[8], qf[9], qf[10], qf[11]);
         v[3] = make_float4(qf[12], l.x, l.y, l.z);
         v[4] = make_float4(beta.x, beta.y, beta.z, 0.0f);
-        P.vcount[p] = k + 1;
+        P.vcount[tp] = k + 1;
       }
     }
   }
@@ -683,44 +700,84 @@ __global__ __launch_bounds__(kThreads) void k_accumulate(Paths P, Sched sc) {
   }
 }
 
-// TRAIN: per recorded vertex, target = (L_end - L_before) / beta, compacted
-// into (feats, targets) up to cap samples.
+// Block-wide append of n items per thread to one queue: one global atomic
+// per block; returns the thread's first slot. Every thread of the block must
+// call it.
+__device__ __forceinline__ uint32_t block_append_n(BlockQueues<1>& sh, uint32_t* ctr, uint32_t n) {
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  uint32_t x = n;  // inclusive scan over the wave
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o);
+    if (lane >= (uint32_t)o) x += y;
+  }
+  if (lane == 63) sh.cnt[0][wave] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tot = 0;
+    for (int w = 0; w < kWaves; ++w) tot += sh.cnt[0][w];
+    uint32_t b = tot ? atomicAdd(ctr, tot) : 0u;
+    for (int w = 0; w < kWaves; ++w) {
+      sh.base[0][w] = b;
+      b += sh.cnt[0][w];
+    }
+  }
+  __syncthreads();
+  return sh.base[0][wave] + x - n;
+}
+
+// Training paths: per recorded vertex, target = (L_end - L_before) / beta,
+// compacted into (feats, targets) up to cap samples. One thread per training
+// path appends its usable vertices (throughput above 1e-6 and a finite target
+// in every channel) in vertex order.
 __global__ __launch_bounds__(kThreads) void k_train_targets(Paths P, Sched sc, float* feats, float* targets,
                                                             uint32_t cap) {
   __shared__ BlockQueues<1> qs;
   const PassDev pp = cur_pass(sc);
-  const uint32_t total = pp.paths * pp.vmax;
+  const uint32_t ntp = pp.paths - pp.train0;
   const uint32_t stride = gridDim.x * kThreads;
-  uint32_t* const ctr[1] = {&P.counters[4]};
-  for (uint32_t b0 = blockIdx.x * kThreads; b0 < total; b0 += stride) {  // block-uniform
-    const uint32_t i = b0 + threadIdx.x;
-    bool ok = false;
-    float f[13], t[3];
-    if (i < total) {
-      uint32_t p = i / pp.vmax, k = i % pp.vmax;
-      if (k < P.vcount[p]) {
-        const float4* v = P.vrec + (size_t)i * kVrec;
-        float4 v0 = v[0], v1 = v[1], v2 = v[2], v3r = v[3], v4 = v[4];
-        const float tmp[13] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w, v2.x, v2.y, v2.z, v2.w, v3r.x};
-        for (int q = 0; q < 13; ++q) f[q] = tmp[q];
-        float4 L = P.lum[p];
-        const float lb[3] = {v3r.y, v3r.z, v3r.w}, le[3] = {L.x, L.y, L.z}, bb[3] = {v4.x, v4.y, v4.z};
-        ok = true;
-        for (int c = 0; c < 3; ++c) {
-          if (!(bb[c] > 1e-6f)) ok = false;
-          float tv = (le[c] - lb[c]) / bb[c];
-          if (!isfinite(tv)) ok = false;
-          t[c] = isfinite(tv) ? fmaxf(tv, 0.0f) : 0.0f;
-        }
+  for (uint32_t b0 = blockIdx.x * kThreads; b0 < ntp; b0 += stride) {  // block-uniform
+    const uint32_t tp = b0 + threadIdx.x;
+    uint32_t nv = 0;
+    uint64_t ok = 0;
+    float4 L = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (tp < ntp) {
+      nv = min(P.vcount[tp], pp.vmax);
+      L = P.lum[pp.train0 + tp];
+      for (uint32_t k = 0; k < nv; ++k) {
+        const float4* v = P.vrec + ((size_t)tp * pp.vmax + k) * kVrec;
+        const float4 lb = v[3], bb = v[4];
+        const float t0 = (L.x - lb.y) / bb.x, t1 = (L.y - lb.z) / bb.y, t2 = (L.z - lb.w) / bb.z;
+        if (bb.x > 1e-6f && bb.y > 1e-6f && bb.z > 1e-6f && isfinite(t0) && isfinite(t1) && isfinite(t2))
+          ok |= 1ull << k;
       }
     }
-    const bool flag[1] = {ok};
-    uint32_t slots[1];
-    block_append<1>(qs, ctr, flag, slots);
-    const uint32_t slot = slots[0];
-    if (ok && slot < cap) {
-      for (int q = 0; q < 13; ++q) feats[13 * (size_t)slot + q] = f[q];
-      for (int c = 0; c < 3; ++c) targets[3 * (size_t)slot + c] = t[c];
+    uint32_t slot = block_append_n(qs, &P.counters[4], (uint32_t)__popcll(ok));
+    for (uint32_t k = 0; k < nv; ++k) {
+      if (!((ok >> k) & 1ull)) continue;
+      if (slot < cap) {
+        const float4* v = P.vrec + ((size_t)tp * pp.vmax + k) * kVrec;
+        const float4 v0 = v[0], v1 = v[1], v2 = v[2], lb = v[3], bb = v[4];
+        float* f = feats + 13 * (size_t)slot;
+        f[0] = v0.x;
+        f[1] = v0.y;
+        f[2] = v0.z;
+        f[3] = v0.w;
+        f[4] = v1.x;
+        f[5] = v1.y;
+        f[6] = v1.z;
+        f[7] = v1.w;
+        f[8] = v2.x;
+        f[9] = v2.y;
+        f[10] = v2.z;
+        f[11] = v2.w;
+        f[12] = lb.x;
+        float* t = targets + 3 * (size_t)slot;
+        t[0] = fmaxf((L.x - lb.y) / bb.x, 0.0f);
+        t[1] = fmaxf((L.y - lb.z) / bb.y, 0.0f);
+        t[2] = fmaxf((L.z - lb.w) / bb.z, 0.0f);
+      }
+      ++slot;
     }
   }
 }
@@ -1033,13 +1090,13 @@ struct pyre_session {
   uint64_t train_samples = 0;       // radiance-cache training samples produced
   bool use_graph = true;
   hipGraphExec_t exec = nullptr;        // PLAIN pass
-  hipGraphExec_t exec_train = nullptr;  // TRAIN pass + training targets
-  hipGraphExec_t exec_cache = nullptr;  // CACHE pass + inference + accumulate, for cache_of
+  hipGraphExec_t exec_cache = nullptr;   // CACHE pass trace + training targets
+  hipGraphExec_t exec_finish = nullptr;  // cache inference + radiance + accumulate, for cache_of
   const pyre_nrc* cache_of = nullptr;
   ~pyre_session() {
     if (exec) (void)hipGraphExecDestroy(exec);
-    if (exec_train) (void)hipGraphExecDestroy(exec_train);
     if (exec_cache) (void)hipGraphExecDestroy(exec_cache);
+    if (exec_finish) (void)hipGraphExecDestroy(exec_finish);
     dev_free(acc);
     dev_free(acc64);
     dev_free(welford);
@@ -1048,49 +1105,77 @@ struct pyre_session {
 
 namespace {
 
+// Training paths of a CACHE pass: how many, their sample index and pixel map.
+struct TrainPaths {
+  uint32_t paths = 0, s_base = 0, pix_mul = 1, pix_add = 0;
+};
+
 void set_pass(pyre_session* S, uint32_t s_base, uint32_t n_samples, uint32_t mode, float diffuse_rough,
-              uint32_t paths = 0, uint32_t pix_mul = 1, uint32_t pix_add = 0) {
+              const TrainPaths& tr = TrainPaths{}) {
   PassDev pp{};
   pp.slice0 = 0;
   pp.n_slices = 1;
-  pp.paths = paths ? paths : n_samples * S->npix;
-  pp.pixels = S->npix;
+  pp.train0 = n_samples * S->npix;
+  pp.paths = pp.train0 + tr.paths;
+  pp.pixels = n_samples ? S->npix : 0u;
   pp.mode = mode;
-  pp.pix_mul = pix_mul;
-  pp.pix_add = pix_add;
+  pp.pix_mul = tr.pix_mul;
+  pp.pix_add = tr.pix_add;
+  pp.t_sbase = tr.s_base;
   pp.vmax = S->W.vmax;
   pp.diffuse_rough = diffuse_rough;
   SliceDev sl{0u, s_base, n_samples, 0u, 0u, S->spp_done};
   hipLaunchKernelGGL(k_set_pass, dim3(1), dim3(1), 0, S->ctx->stream, S->W.passes, S->W.slices, pp, sl);
 }
 
-// Samples [s_base, s_base + n_samples) of every pixel. Every stage reads
-// its pass from device memory, so one captured graph per kind of pass
-// (plain, cache) serves any sample count; PYRE_GRAPH=0 launches eagerly.
+// Samples [s_base, s_base + n_samples) of every pixel (a CACHE pass may
+// also carry training paths, whose samples land in the training buffers).
+// Every stage reads its pass from device memory, so one captured graph per
+// kind of pass (plain, cache) serves any sample count; PYRE_GRAPH=0 launches
+// eagerly.
 void session_pass(pyre_session* S, uint32_t s_base, uint32_t n_samples, uint32_t mode, pyre_nrc* nrc,
-                  float diffuse_rough, bool accumulate) {
+                  float diffuse_rough, bool accumulate, const TrainPaths& tr = TrainPaths{},
+                  const std::function<void()>& between = nullptr) {
   hipStream_t st = S->ctx->stream;
-  set_pass(S, s_base, n_samples, mode, diffuse_rough);
+  require(mode == MODE_CACHE || tr.paths == 0, "training paths ride in cache passes only");
+  require((size_t)n_samples * S->npix + tr.paths <= S->W.cap, "pass exceeds the path pool");
+  set_pass(S, s_base, n_samples, mode, diffuse_rough, tr);
   if (mode == MODE_CACHE) {
+    // trace (+ training targets), then `between` (the frame's training
+    // steps), then cache inference for the paths that ended in the cache,
+    // their radiance, and the film update
     require(nrc != nullptr, "cache pass without a radiance cache");
     require(accumulate, "cache passes always accumulate");
-    auto body = [&] {
+    auto trace = [&] {
       launch_pass(S->ctx, S->W, S->shape, S->tune, false, st);
+      if (S->W.vmax)
+        hipLaunchKernelGGL(k_train_targets, dim3(stage_grid(S->ctx)), dim3(kThreads), 0, st, S->W.P, S->W.sched(),
+                           S->W.tfeat, S->W.ttarget, S->W.tcap);
+    };
+    auto finish = [&] {
       nrc_infer_dev_count(nrc, S->W.P.qfeat, S->W.P.qpath, S->W.P.counters + 3, (uint32_t)S->W.cap, S->W.P.qrgb,
                           st);
       hipLaunchKernelGGL(k_cache_apply, dim3(stage_grid(S->ctx)), dim3(kThreads), 0, st, S->W.P);
       hipLaunchKernelGGL(k_accumulate, dim3(stage_grid(S->ctx)), dim3(kThreads), 0, st, S->W.P, S->W.sched());
     };
-    if (!S->use_graph) return body();
-    if (S->exec_cache && S->cache_of != nrc) {
-      (void)hipGraphExecDestroy(S->exec_cache);
-      S->exec_cache = nullptr;
+    if (!S->use_graph) {
+      trace();
+      if (between) between();
+      finish();
+      return;
     }
-    if (!S->exec_cache) {
-      S->exec_cache = capture(st, body);
+    if (S->exec_finish && S->cache_of != nrc) {
+      (void)hipGraphExecDestroy(S->exec_finish);
+      S->exec_finish = nullptr;
+    }
+    if (!S->exec_cache) S->exec_cache = capture(st, trace);
+    if (!S->exec_finish) {
+      S->exec_finish = capture(st, finish);
       S->cache_of = nrc;
     }
     PYRE_HIP_CHECK(hipGraphLaunch(S->exec_cache, st));
+    if (between) between();
+    PYRE_HIP_CHECK(hipGraphLaunch(S->exec_finish, st));
     return;
   }
   if (mode == MODE_PLAIN && accumulate && S->use_graph) {
@@ -1126,10 +1211,12 @@ void session_init(pyre_session* S, pyre_ctx* ctx, pyre_scene* s, const pyre_came
   // paths per pass: whole samples of the image, about pool_paths paths
   S->per_pass = std::max(1u, S->tune.pool_paths / S->npix);
   if (spp_hint) S->per_pass = std::min(S->per_pass, spp_hint);
-  const size_t cap = std::max((size_t)S->per_pass * S->npix, (size_t)train_paths);
+  // a radiance-cache frame carries its training paths after a full pass of samples
+  const size_t cap = (size_t)S->per_pass * S->npix + train_paths;
   require(cap < (1ull << 31), "path pool too large");
   const uint32_t slots = std::max(1u, s->n_lights + (s->has_env ? 1u : 0u));
-  const uint32_t vmax = train_paths ? std::max(1u, ip.max_depth) : 0;
+  // vertices recorded per training path (at most 64: one bit each in the target pass)
+  const uint32_t vmax = train_paths ? std::min(64u, std::max(1u, ip.max_depth)) : 0;
   const uint32_t tcap = train_paths ? std::min<uint32_t>(train_paths * vmax, 1u << 20) : 0;
   if (!S->W.fits(cap, slots, vmax, train_paths, tcap, train_paths > 0, 1, 1, 1))
     S->W.alloc(cap, slots, vmax, train_paths, tcap, train_paths > 0, 1, 1, 1);
@@ -1232,6 +1319,7 @@ Schedule make_schedule(const pyre_job* jobs, size_t n, uint32_t pool) {
       next[j] += ns;
     }
     pp.n_slices = (uint32_t)sc.slices.size() - pp.slice0;
+    pp.train0 = pp.paths;
     if (!pp.n_slices) break;
     sc.passes.push_back(pp);
   }
@@ -1502,41 +1590,43 @@ int pyre_session_render_nrc(pyre_ctx* ctx, pyre_session* S, pyre_nrc* nrc, const
       ~Free() { dev_free(p); }
     } free_loss{dloss};
     for (uint32_t f = 0; f < p->frames; ++f) {
-      // training paths: full unbiased paths from scattered pixels, own streams
-      set_pass(S, kTrainStreamBase + S->train_frames, 1, MODE_TRAIN, p->diffuse_roughness, p->train_paths,
-               (uint32_t)(2654435761ull % S->npix) | 1u, (uint32_t)(pixel_seed(S->train_frames, 7, 11) % S->npix));
-      auto train_pass = [&] {
-        launch_pass(ctx, S->W, S->shape, S->tune, false, st);
-        hipLaunchKernelGGL(k_train_targets, dim3(stage_grid(ctx)), dim3(kThreads), 0, st, S->W.P, S->W.sched(),
-                           S->W.tfeat, S->W.ttarget, S->W.tcap);
+      // One pass per frame (or several, past the pool size) over the frame's
+      // spp_per_frame samples (none in a warmup frame) and, in its last pass,
+      // the frame's training paths (full unbiased paths from scattered
+      // pixels, their own sample streams). After tracing, the frame trains on
+      // its own samples, then the paths that ended in the cache are given
+      // the updated cache's radiance.
+      const uint32_t spp = f < p->warmup_frames ? 0u : p->spp_per_frame;
+      TrainPaths tr;
+      tr.paths = p->train_paths;
+      tr.s_base = kTrainStreamBase + S->train_frames;
+      tr.pix_mul = (uint32_t)(2654435761ull % S->npix) | 1u;
+      tr.pix_add = (uint32_t)(pixel_seed(S->train_frames, 7, 11) % S->npix);
+      auto train = [&] {
+        uint32_t nsamp = 0;
+        PYRE_HIP_CHECK(hipMemcpyAsync(&nsamp, S->W.P.counters + 4, 4, hipMemcpyDeviceToHost, st));
+        PYRE_HIP_CHECK(hipStreamSynchronize(st));
+        S->train_samples += nsamp;
+        nsamp = std::min(nsamp, S->W.tcap);
+        for (uint32_t k = 0; k < p->train_steps; ++k) {
+          const uint32_t off = k * p->train_batch;
+          if (off + 16 > nsamp) break;
+          const uint32_t nb = std::min(p->train_batch, nsamp - off);
+          nrc_train_step_dev(nrc, S->W.tfeat + 13 * (size_t)off, S->W.ttarget + 3 * (size_t)off, nb, st, dloss + f);
+          ++nsteps[f];
+        }
       };
-      if (S->use_graph) {
-        if (!S->exec_train) S->exec_train = capture(st, train_pass);
-        PYRE_HIP_CHECK(hipGraphLaunch(S->exec_train, st));
-      } else {
-        train_pass();
-      }
-      uint32_t nsamp = 0;
-      PYRE_HIP_CHECK(hipMemcpyAsync(&nsamp, S->W.P.counters + 4, 4, hipMemcpyDeviceToHost, st));
-      PYRE_HIP_CHECK(hipStreamSynchronize(st));
-      S->train_samples += nsamp;
-      nsamp = std::min(nsamp, S->W.tcap);
-      for (uint32_t k = 0; k < p->train_steps; ++k) {
-        const uint32_t off = k * p->train_batch;
-        if (off + 16 > nsamp) break;
-        const uint32_t nb = std::min(p->train_batch, nsamp - off);
-        nrc_gradients_dev(nrc, S->W.tfeat + 13 * (size_t)off, S->W.ttarget + 3 * (size_t)off, nb, st, dloss + f);
-        nrc_adam_step(nrc, st);
-        ++nsteps[f];
-      }
-      S->train_frames++;
-      if (f < p->warmup_frames) continue;  // training only
-      for (uint32_t s0 = 0; s0 < p->spp_per_frame; s0 += S->per_pass) {
-        uint32_t ns = std::min(S->per_pass, p->spp_per_frame - s0);
-        session_pass(S, S->next_sample, ns, MODE_CACHE, nrc, p->diffuse_roughness, true);
+      const uint32_t n_pass = spp ? (spp + S->per_pass - 1) / S->per_pass : 1u;
+      for (uint32_t i = 0, s0 = 0; i < n_pass; ++i) {
+        const uint32_t ns = std::min(S->per_pass, spp - s0);
+        const bool last = i + 1 == n_pass;
+        session_pass(S, S->next_sample, ns, MODE_CACHE, nrc, p->diffuse_roughness, true,
+                     last ? tr : TrainPaths{}, last ? std::function<void()>(train) : nullptr);
         S->next_sample += ns;
         S->spp_done += ns;
+        s0 += ns;
       }
+      S->train_frames++;
     }
     if (losses && p->frames) {
       std::vector<float> h(p->frames);

@@ -133,13 +133,6 @@ __device__ __forceinline__ bf16x8 encode_frag(const Query& q, int kb, int g) {
   return v;
 }
 
-__global__ void k_encode(const float* feats, uint32_t n, NrcBounds b, __bf16* x) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  Query q = load_query(feats + 13 * (size_t)i, b);
-  for (int f = 0; f < 128; ++f) x[128 * (size_t)i + f] = to_bf16(encode_feature(f, q));
-}
-
 // ------------------------------------------------- weights -> fragments
 // Forward fragment (l, ot, kb), lane (g, r): W_l[32kb + 8g + j][16ot + r].
 // Backward fragment (l, it, kb), lane (g, r): W_l[16it + r][32kb + 8g + j]
@@ -246,146 +239,332 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_infer(const bf16x8* __rest
 }
 
 // ---------------------------------------------------------- training
-// Forward layer: Y[n][out] = act(X[n][in] W). One wave per 16-query tile.
-template <bool RELU_BF16_OUT>
-__global__ __launch_bounds__(256) void k_fwd_layer(const bf16x8* __restrict__ wfrag, int layer,
-                                                   const __bf16* __restrict__ x, uint32_t n, __bf16* y_bf16,
-                                                   float* y_f32) {
-  const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
-  const uint32_t tile = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (tile * 16 >= n) return;
-  const int in = kIn[layer], out = kOut[layer];
-  const int nkb = in / 32, nout = out / 16;
-  const __bf16* xr = x + (size_t)(tile * 16 + r) * in;
-  bf16x8 bx[4];
-  for (int kb = 0; kb < nkb; ++kb) bx[kb] = *(const bf16x8*)(xr + 32 * kb + 8 * g);
-  for (int ot = 0; ot < nout; ++ot) {
-    f32x4 acc = {0, 0, 0, 0};
-    for (int kb = 0; kb < nkb; ++kb) acc = mfma(wfrag[(size_t)(fwd_frag_offset(layer) + ot * nkb + kb) * 64 + lane], bx[kb], acc);
-    size_t o = (size_t)(tile * 16 + r) * out + 16 * ot + 4 * g;
-    if (RELU_BF16_OUT) {
-      for (int i = 0; i < 4; ++i) y_bf16[o + i] = to_bf16(fmaxf(acc[i], 0.0f));
-    } else {
-      for (int i = 0; i < 4; ++i) y_f32[o + i] = acc[i];
-    }
-  }
-}
+// Weight gradients are summed over chunks of kDwChunk samples per block.
+constexpr int kDwChunk = 128;
 
-// Relative L2 loss: dP (bf16, 16 wide, zero past RGB) and per-block loss sums.
-__global__ __launch_bounds__(256) void k_loss(const float* pred, const float* target, uint32_t n, uint32_t n_real,
-                                              __bf16* dp, float* loss_part) {
-  __shared__ float red[256];
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  float s = 0.0f;
-  if (i < n) {
-    for (int c = 0; c < 16; ++c) {
+// Four launches per step: fused encode + forward + loss gradient, fused
+// backward dX through all layers, weight gradients of all layers, and their
+// deterministic reduction.
+constexpr int kTrainWaves = 4;
+
+// Encode + forward of every layer + relative-L2 loss gradient for tiles of 16
+// samples. Writes each layer's input (bf16, sample-major), the loss gradient
+// dP (16 wide, zero past RGB and for padding rows q >= n_real) and per-block
+// loss sums.
+__global__ __launch_bounds__(64 * kTrainWaves) void k_train_fwd(const bf16x8* __restrict__ wfrag,
+                                                                const float* __restrict__ feats,
+                                                                const float* __restrict__ targets, uint32_t np,
+                                                                uint32_t n_real, NrcBounds b, __bf16* acts,
+                                                                uint32_t cap, __bf16* dp, float* loss_part) {
+  __shared__ bf16x8 sw[kFwdFrags * 64];
+  __shared__ __attribute__((aligned(16))) __bf16 sact[kTrainWaves][16][64 + 8];
+  __shared__ float red[kTrainWaves];
+  for (int i = threadIdx.x; i < kFwdFrags * 64; i += blockDim.x) sw[i] = wfrag[i];
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int g = lane >> 4, r = lane & 15;
+  const uint32_t ntiles = np / 16;
+  float loss = 0.0f;
+  for (uint32_t tile = blockIdx.x * kTrainWaves + wave; tile < ntiles; tile += gridDim.x * kTrainWaves) {
+    const uint32_t q = tile * 16 + r;
+    const uint32_t row = q < n_real ? q : n_real - 1;  // padding rows repeat a sample; their gradient is zero
+    Query qq = load_query(feats + 13 * (size_t)row, b);
+    f32x4 acc[4];
+#pragma unroll
+    for (int ot = 0; ot < 4; ++ot) acc[ot] = f32x4{0, 0, 0, 0};
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) {
+      bf16x8 bx = encode_frag(qq, kb, g);
+      *(bf16x8*)(acts + (size_t)q * 128 + 32 * kb + 8 * g) = bx;
+#pragma unroll
+      for (int ot = 0; ot < 4; ++ot) acc[ot] = mfma(sw[(fwd_frag_offset(0) + ot * 4 + kb) * 64 + lane], bx, acc[ot]);
+    }
+#pragma unroll
+    for (int l = 1; l < kLayers; ++l) {
+      __bf16* xl = acts + (size_t)cap * (128 + (l - 1) * 64) + (size_t)q * 64;
+#pragma unroll
+      for (int ot = 0; ot < 4; ++ot)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) sact[wave][r][16 * ot + 4 * g + i] = to_bf16(fmaxf(acc[ot][i], 0.0f));
+      wave_lds_sync();
+      // B-operand layout: lane (g, r) holds features 32 kb + 8 g .. + 7 of
+      // sample q, which is also one 16-byte piece of its activation row
+      bf16x8 bx[2];
+#pragma unroll
+      for (int kb = 0; kb < 2; ++kb) {
+        bx[kb] = *(const bf16x8*)&sact[wave][r][32 * kb + 8 * g];
+        *(bf16x8*)(xl + 32 * kb + 8 * g) = bx[kb];
+      }
+      wave_lds_sync();
+      const int nout = kOut[l] / 16;
+#pragma unroll
+      for (int ot = 0; ot < 4; ++ot) {
+        acc[ot] = f32x4{0, 0, 0, 0};
+        if (ot < nout) {
+#pragma unroll
+          for (int kb = 0; kb < 2; ++kb)
+            acc[ot] = mfma(sw[(fwd_frag_offset(l) + ot * 2 + kb) * 64 + lane], bx[kb], acc[ot]);
+        }
+      }
+    }
+    // outputs 4g + i of sample q; relative L2 on RGB: L = (p - t)^2 / (p^2 + 0.01)
+    // with p^2 held constant in the gradient (the stop-gradient form)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int c = 4 * g + i;
       float gval = 0.0f;
-      if (c < 3 && i < n_real) {
-        float p = pred[16 * (size_t)i + c], t = target[3 * (size_t)i + c];
-        float denom = p * p + 0.01f;
-        float d = p - t;
-        s += d * d / denom;
+      if (c < 3 && q < n_real) {
+        const float p = acc[0][i], t = targets[3 * (size_t)q + c];
+        const float denom = p * p + 0.01f;
+        const float d = p - t;
+        loss += d * d / denom;
         gval = 2.0f * d / denom / (3.0f * (float)n_real);
       }
-      dp[16 * (size_t)i + c] = to_bf16(gval);
+      dp[(size_t)q * 16 + c] = to_bf16(gval);
     }
   }
-  red[threadIdx.x] = s;
+  for (int o = 32; o > 0; o >>= 1) loss += __shfl_xor(loss, o);
+  if (lane == 0) red[wave] = loss;
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) loss_part[blockIdx.x] = red[0];
-}
-
-// dZ_{l-1} = (dZ_l W_l^T) * (X_l > 0): dX^T[in x n] = W_l[in x out] . dZ_l^T.
-__global__ __launch_bounds__(256) void k_bwd_dx(const bf16x8* __restrict__ bfrag, int layer,
-                                                const __bf16* __restrict__ dz, const __bf16* __restrict__ xl,
-                                                uint32_t n, __bf16* dz_prev) {
-  const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
-  const uint32_t tile = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (tile * 16 >= n) return;
-  const int in = kIn[layer], out = kOut[layer];
-  const int nkb = (out + 31) / 32, nit = in / 16;
-  const uint32_t q = tile * 16 + r;
-  bf16x8 bz[2];
-  for (int kb = 0; kb < nkb; ++kb) {
-    if (32 * kb + 8 * g < out) {
-      bz[kb] = *(const bf16x8*)(dz + (size_t)q * out + 32 * kb + 8 * g);
-    } else {
-      for (int j = 0; j < 8; ++j) bz[kb][j] = to_bf16(0.0f);
-    }
-  }
-  for (int it = 0; it < nit; ++it) {
-    f32x4 acc = {0, 0, 0, 0};
-    for (int kb = 0; kb < nkb; ++kb) acc = mfma(bfrag[(size_t)(bwd_frag_offset(layer) + it * nkb + kb) * 64 + lane], bz[kb], acc);
-    size_t o = (size_t)q * in + 16 * it + 4 * g;
-    for (int i = 0; i < 4; ++i) dz_prev[o + i] = to_bf16(from_bf16(xl[o + i]) > 0.0f ? acc[i] : 0.0f);
+  if (threadIdx.x == 0) {
+    float t = 0.0f;
+    for (int w = 0; w < kTrainWaves; ++w) t += red[w];
+    loss_part[blockIdx.x] = t;
   }
 }
 
-// dW_l = X_l^T dZ_l over a chunk of kDwChunk queries per block (partial sums,
-// reduced deterministically by k_dw_reduce). A = X^T tile gathered from LDS,
-// B = dZ tile gathered from LDS.
-constexpr int kDwChunk = 128;
-__global__ __launch_bounds__(256) void k_bwd_dw(int layer, const __bf16* __restrict__ xl,
-                                                const __bf16* __restrict__ dz, uint32_t n, float* partial) {
-  __shared__ __bf16 sx[kDwChunk][128 + 2];
-  __shared__ __bf16 sz[kDwChunk][64 + 2];
-  const int in = kIn[layer], out = kOut[layer];
-  const uint32_t q0 = blockIdx.x * kDwChunk;
-  for (int i = threadIdx.x; i < kDwChunk * in; i += blockDim.x) {
-    int s = i / in, f = i % in;
-    sx[s][f] = (q0 + s < n) ? xl[(size_t)(q0 + s) * in + f] : to_bf16(0.0f);
-  }
-  for (int i = threadIdx.x; i < kDwChunk * out; i += blockDim.x) {
-    int s = i / out, f = i % out;
-    sz[s][f] = (q0 + s < n) ? dz[(size_t)(q0 + s) * out + f] : to_bf16(0.0f);
-  }
+// Backward dX through layers 5..1 for tiles of 16 samples:
+// dZ_{l-1} = (dZ_l W_l^T) * (X_l > 0), every dZ_l kept for the weight
+// gradients (dz + dz_offset(l)).
+__host__ __device__ constexpr size_t dz_offset(int l, uint32_t cap) { return (size_t)cap * 64 * (size_t)l; }
+
+__global__ __launch_bounds__(64 * kTrainWaves) void k_train_bwd(const bf16x8* __restrict__ bfrag,
+                                                                const __bf16* __restrict__ acts, uint32_t np,
+                                                                uint32_t cap, __bf16* dz) {
+  constexpr int kUsed = kBwdFrags - bwd_frags(0);  // layers 1..5
+  __shared__ bf16x8 sw[kUsed * 64];
+  __shared__ __attribute__((aligned(16))) __bf16 sdz[kTrainWaves][16][64 + 8];
+  for (int i = threadIdx.x; i < kUsed * 64; i += blockDim.x) sw[i] = bfrag[bwd_frag_offset(1) * 64 + i];
   __syncthreads();
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
-  const int nit = in / 16, not_ = out / 16;
-  float* dst = partial + (size_t)blockIdx.x * in * out;
-  for (int t = wave; t < nit * not_; t += 4) {
-    int it = t / not_, ot = t % not_;
-    f32x4 acc = {0, 0, 0, 0};
-    for (int kb = 0; kb < kDwChunk / 32; ++kb) {
-      bf16x8 a, bb;
-      for (int j = 0; j < 8; ++j) {
-        a[j] = sx[32 * kb + 8 * g + j][16 * it + r];   // A[in row][query k]
-        bb[j] = sz[32 * kb + 8 * g + j][16 * ot + r];  // B[query k][out col]
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int g = lane >> 4, r = lane & 15;
+  const uint32_t ntiles = np / 16;
+  for (uint32_t tile = blockIdx.x * kTrainWaves + wave; tile < ntiles; tile += gridDim.x * kTrainWaves) {
+    const uint32_t q = tile * 16 + r;
+    // dZ_5 = dP (16 wide, zero-padded to the K block of 32)
+    bf16x8 bz[2];
+    {
+      const __bf16* d5 = dz + dz_offset(5, cap) + (size_t)q * 16;
+      if (g < 2) {
+        bz[0] = *(const bf16x8*)(d5 + 8 * g);
+      } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) bz[0][j] = to_bf16(0.0f);
       }
-      acc = mfma(a, bb, acc);
     }
-    for (int i = 0; i < 4; ++i) dst[(size_t)(16 * it + 4 * g + i) * out + 16 * ot + r] = acc[i];
+#pragma unroll
+    for (int l = kLayers - 1; l >= 1; --l) {
+      const int nkb = (kOut[l] + 31) / 32;
+      const __bf16* xl = acts + (size_t)cap * (128 + (l - 1) * 64) + (size_t)q * 64;
+      __bf16* dzo = dz + dz_offset(l - 1, cap) + (size_t)q * 64;
+#pragma unroll
+      for (int it = 0; it < 4; ++it) {  // in = 64 for layers 1..5
+        f32x4 acc = {0, 0, 0, 0};
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb)
+          if (kb < nkb) acc = mfma(sw[(bwd_frag_offset(l) - bwd_frag_offset(1) + it * nkb + kb) * 64 + lane], bz[kb], acc);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) sdz[wave][r][16 * it + 4 * g + i] = to_bf16(acc[i]);
+      }
+      wave_lds_sync();
+      // ReLU mask in the B-operand layout (features 32 kb + 8 g .. + 7),
+      // 16-byte loads of X_l and stores of dZ_{l-1}
+#pragma unroll
+      for (int kb = 0; kb < 2; ++kb) {
+        const bf16x8 raw = *(const bf16x8*)&sdz[wave][r][32 * kb + 8 * g];
+        const bf16x8 xv = *(const bf16x8*)(xl + 32 * kb + 8 * g);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) bz[kb][j] = from_bf16(xv[j]) > 0.0f ? raw[j] : to_bf16(0.0f);
+        *(bf16x8*)(dzo + 32 * kb + 8 * g) = bz[kb];
+      }
+      wave_lds_sync();
+    }
   }
 }
 
-__global__ void k_dw_reduce(const float* partial, uint32_t nchunks, int size, float* grad) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= size) return;
-  float s = 0.0f;
-  for (uint32_t c = 0; c < nchunks; ++c) s += partial[(size_t)c * size + i];
-  grad[i] = s;
+// Weight gradients of every layer: block (p, layer) sums X_l^T dZ_l over the
+// chunks p, p + P, ... of kDwChunk samples (P = gridDim.x) in registers and
+// writes one partial. Each chunk is staged feature-major in LDS, so both MFMA
+// operands (A = X^T tile, B = dZ tile; K = samples) are single 16-byte reads;
+// wave w owns the 16x16 output tiles w, w + 4, ... of the layer.
+constexpr int kDwTiles = 8;               // tiles per wave: 128 x 64 / 256 / 4
+constexpr int kDwStride = kDwChunk + 8;   // bf16 per LDS row: 16-byte aligned, conflict-free 16-byte reads
+static_assert(param_offset(1) % 64 == 0 && param_offset(2) % 64 == 0 && param_offset(5) % 64 == 0 &&
+                  kParams % 64 == 0,
+              "parameter blocks of 64 stay within one layer");
+
+// dst[f * kDwStride + s] = src[(q0 + s) * width + f] for the chunk's samples
+// (zero past np): work items are (sample pair, 8-feature group), sample pair
+// fastest, so each LDS store is one 32-bit word holding two samples.
+__device__ __forceinline__ void stage_transposed(__bf16* dst, const __bf16* __restrict__ src, uint32_t q0,
+                                                 uint32_t np, int width) {
+  constexpr int kPairs = kDwChunk / 2;
+  uint32_t* d = reinterpret_cast<uint32_t*>(dst);
+  const int n = kPairs * (width / 8);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const int sp = i % kPairs, f8 = i / kPairs;
+    const uint32_t s = 2 * sp;
+    uint4 a = make_uint4(0, 0, 0, 0), c = make_uint4(0, 0, 0, 0);
+    if (q0 + s < np) a = *(const uint4*)(src + (size_t)(q0 + s) * width + 8 * f8);
+    if (q0 + s + 1 < np) c = *(const uint4*)(src + (size_t)(q0 + s + 1) * width + 8 * f8);
+    const uint32_t wa[4] = {a.x, a.y, a.z, a.w}, wc[4] = {c.x, c.y, c.z, c.w};
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const uint32_t lo = (wa[j >> 1] >> (16 * (j & 1))) & 0xffffu;
+      const uint32_t hi = (wc[j >> 1] >> (16 * (j & 1))) & 0xffffu;
+      d[((8 * f8 + j) * kDwStride + s) >> 1] = lo | (hi << 16);
+    }
+  }
 }
 
-// Adam (Kingma & Ba) on the FP32 master weights; the RGB-only output columns
-// and the padding rows of layer 0 have zero gradient and stay fixed.
-__global__ void k_adam(float* w, const float* grad, float* m, float* v, int n, float lr, float b1, float b2, float eps,
-                       float c1, float c2) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  float gr = grad[i];
-  float mm = b1 * m[i] + (1.0f - b1) * gr;
-  float vv = b2 * v[i] + (1.0f - b2) * gr * gr;
-  m[i] = mm;
-  v[i] = vv;
-  float mh = mm / c1, vh = vv / c2;
-  w[i] = w[i] - lr * mh / (sqrtf(vh) + eps);
+__global__ __launch_bounds__(256) void k_train_dw(const __bf16* __restrict__ acts, const __bf16* __restrict__ dz,
+                                                  uint32_t np, uint32_t cap, float* partial) {
+  __shared__ __attribute__((aligned(16))) __bf16 sx[128 * kDwStride];
+  __shared__ __attribute__((aligned(16))) __bf16 sz[64 * kDwStride];
+  const int layer = blockIdx.y;
+  const int in = kIn[layer], out = kOut[layer];
+  const __bf16* xl = acts + (layer == 0 ? 0 : (size_t)cap * (128 + (layer - 1) * 64));
+  const __bf16* dzl = dz + dz_offset(layer, cap);
+  const uint32_t nch = (np + kDwChunk - 1) / kDwChunk;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+  const int not_ = out / 16, ntile = (in / 16) * not_;
+  f32x4 acc[kDwTiles];
+#pragma unroll
+  for (int k = 0; k < kDwTiles; ++k) acc[k] = f32x4{0, 0, 0, 0};
+  for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
+    const uint32_t q0 = c * kDwChunk;
+    __syncthreads();  // the previous chunk's LDS reads are done
+    stage_transposed(sx, xl, q0, np, in);
+    stage_transposed(sz, dzl, q0, np, out);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kDwTiles; ++k) {
+      const int t = wave + 4 * k;
+      if (t < ntile) {
+        const int it = t / not_, ot = t % not_;
+        const __bf16* ra = sx + (16 * it + r) * kDwStride + 8 * g;
+        const __bf16* rb = sz + (16 * ot + r) * kDwStride + 8 * g;
+#pragma unroll
+        for (int kb = 0; kb < kDwChunk / 32; ++kb)
+          acc[k] = mfma(*(const bf16x8*)(ra + 32 * kb), *(const bf16x8*)(rb + 32 * kb), acc[k]);
+      }
+    }
+  }
+  float* dst = partial + (size_t)gridDim.x * param_offset(layer) + (size_t)blockIdx.x * in * out;
+#pragma unroll
+  for (int k = 0; k < kDwTiles; ++k) {
+    const int t = wave + 4 * k;
+    if (t < ntile) {
+      const int it = t / not_, ot = t % not_;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) dst[(size_t)(16 * it + 4 * g + i) * out + 16 * ot + r] = acc[k][i];
+    }
+  }
+}
+
+// grad[i] = sum of the P partials: block of 64 parameters (never straddling a
+// layer: every layer size is a multiple of 64), wave w sums partials w, w + 4,
+// ... and the four wave sums are added in wave order (deterministic).
+__global__ __launch_bounds__(256) void k_train_reduce(const float* partial, uint32_t P, float* grad) {
+  __shared__ float red[4][64];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 64 + lane;
+  int l = 0;
+  while (l + 1 < kLayers && blockIdx.x * 64 >= param_offset(l + 1)) ++l;
+  const int size = kIn[l] * kOut[l], j = i - param_offset(l);
+  const float* p = partial + (size_t)P * param_offset(l) + j;
+  float s = 0.0f;
+  for (uint32_t c = wave; c < P; c += 4) s += p[(size_t)c * size];
+  red[wave][lane] = s;
+  __syncthreads();
+  if (wave == 0) grad[i] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+}
+
+// Adam hyper-parameters with the step's bias corrections c1 = 1 - b1^t,
+// c2 = 1 - b2^t.
+struct AdamArgs {
+  float lr, b1, b2, eps, c1, c2;
+};
+
+__device__ __forceinline__ float adam_update(float& m, float& v, float w, float gr, const AdamArgs& a) {
+  m = a.b1 * m + (1.0f - a.b1) * gr;
+  v = a.b2 * v + (1.0f - a.b2) * gr * gr;
+  const float mh = m / a.c1, vh = v / a.c2;
+  return w - a.lr * mh / (sqrtf(vh) + a.eps);
+}
+
+// The tail of a training step in one launch: grad = sum of the P partials
+// (in the order of k_train_reduce), Adam on the FP32 master weights, and the
+// new weights rounded to bf16 straight into their forward and backward
+// fragment slots (the inverse of k_make_frags' gather; the backward
+// fragments' zero padding is never written). The non-RGB output columns and
+// the padding rows of layer 0 have zero gradient and stay fixed. Block 0 also adds
+// loss_scale x sum(loss_part[0..nlp)) to *loss_dev when it is given.
+__global__ __launch_bounds__(256) void k_train_update(const float* partial, uint32_t P, float* grad, float* w,
+                                                      float* m, float* v, AdamArgs a, bf16x8* wf, bf16x8* wb,
+                                                      const float* loss_part, uint32_t nlp, float loss_scale,
+                                                      float* loss_dev) {
+  __shared__ float red[4][64];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 64 + lane;
+  int l = 0;
+  while (l + 1 < kLayers && blockIdx.x * 64 >= param_offset(l + 1)) ++l;
+  const int in = kIn[l], out = kOut[l], j = i - param_offset(l);
+  const float* p = partial + (size_t)P * param_offset(l) + j;
+  float s = 0.0f;
+  for (uint32_t c = wave; c < P; c += 4) s += p[(size_t)c * (in * out)];
+  red[wave][lane] = s;
+  __syncthreads();
+  if (wave == 0) {
+    const float gr = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+    grad[i] = gr;
+    float mm = m[i], vv = v[i];
+    const float wn = adam_update(mm, vv, w[i], gr, a);
+    m[i] = mm;
+    v[i] = vv;
+    w[i] = wn;
+    const __bf16 hb = to_bf16(wn);
+    const int ji = j / out, o = j % out;  // W_l[ji][o]
+    {
+      const int frag = fwd_frag_offset(l) + (o / 16) * (in / 32) + ji / 32;
+      const int ln = ((ji % 32) / 8) * 16 + o % 16;
+      reinterpret_cast<__bf16*>(wf)[((size_t)frag * 64 + ln) * 8 + ji % 8] = hb;
+    }
+    {
+      const int frag = bwd_frag_offset(l) + (ji / 16) * ((out + 31) / 32) + o / 32;
+      const int ln = ((o % 32) / 8) * 16 + ji % 16;
+      reinterpret_cast<__bf16*>(wb)[((size_t)frag * 64 + ln) * 8 + o % 8] = hb;
+    }
+  }
+  if (loss_dev && blockIdx.x == 0) {
+    __shared__ float lred[256];
+    float t = 0.0f;
+    for (uint32_t k = threadIdx.x; k < nlp; k += 256) t += loss_part[k];
+    lred[threadIdx.x] = t;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) lred[threadIdx.x] += lred[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) *loss_dev += loss_scale * lred[0];
+  }
 }
 
 inline unsigned blocks(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
+
+// Weight-gradient partials per layer: at least four chunks per block, at most 64
+// partials (6.5 MB), so the reduction stays a short pass.
+inline unsigned dw_partials(uint32_t np) { return std::max(1u, std::min(64u, blocks(np, 4 * kDwChunk))); }
 
 }  // namespace
 
@@ -404,9 +583,7 @@ struct pyre_nrc {
   // training scratch, sized for the largest batch seen
   uint32_t cap = 0;
   __bf16* acts = nullptr;  // layer inputs: 128 + 5 * 64 features per query
-  __bf16* dz0 = nullptr;
-  __bf16* dz1 = nullptr;
-  float* pred = nullptr;
+  __bf16* dz = nullptr;  // every layer's output gradient (dz_offset)
   float* part = nullptr;
   float* loss_part = nullptr;
   float* feats = nullptr;
@@ -425,20 +602,16 @@ static void ensure_cap(pyre_nrc* N, uint32_t n) {
   uint32_t need = (n + 15) / 16 * 16;
   if (need <= N->cap) return;
   dev_free(N->acts);
-  dev_free(N->dz0);
-  dev_free(N->dz1);
-  dev_free(N->pred);
+  dev_free(N->dz);
   dev_free(N->part);
   dev_free(N->loss_part);
   dev_free(N->feats);
   dev_free(N->targets);
   N->cap = need;
   N->acts = dev_alloc<__bf16>((size_t)need * (128 + 5 * 64));
-  N->dz0 = dev_alloc<__bf16>((size_t)need * 128);
-  N->dz1 = dev_alloc<__bf16>((size_t)need * 128);
-  N->pred = dev_alloc<float>((size_t)need * 16);
-  N->part = dev_alloc<float>((size_t)blocks(need, kDwChunk) * 128 * 64);
-  N->loss_part = dev_alloc<float>(blocks(need, 256));
+  N->dz = dev_alloc<__bf16>((size_t)need * (5 * 64 + 16));
+  N->part = dev_alloc<float>((size_t)dw_partials(need) * kParams);
+  N->loss_part = dev_alloc<float>(std::max<size_t>(blocks(need, 64), (size_t)std::max(1, N->ctx->cus) * 2));
   N->feats = dev_alloc<float>((size_t)need * 13);
   N->targets = dev_alloc<float>((size_t)need * 3);
 }
@@ -462,57 +635,56 @@ __global__ void k_sum_into(const float* part, uint32_t n, float scale, float* ou
   if (threadIdx.x == 0) *out += scale * red[0];
 }
 
-float nrc_gradients_dev(pyre_nrc* N, const float* feats, const float* targets, uint32_t n, hipStream_t st,
-                        float* loss_dev) {
+// Forward + loss + backward + weight-gradient partials of one batch
+// (feats/targets are device arrays). Returns the number of per-block loss
+// sums in N->loss_part; the partials' count is dw_partials(np).
+static unsigned train_pass(pyre_nrc* N, const float* feats, const float* targets, uint32_t n, hipStream_t st) {
   require(n > 0, "empty training batch");
   ensure_cap(N, n);
-  const uint32_t np = (n + 15) / 16 * 16;
+  const uint32_t np = (n + 15) / 16 * 16;  // padding rows repeat a sample with zero gradient
   const uint32_t cap = N->cap;
-  // pad rows: copy features (rows past n reuse query 0; their gradient is masked)
-  hipLaunchKernelGGL(k_encode, dim3(blocks(n, 256)), dim3(256), 0, st, feats, n, N->bounds, N->acts);
-  if (np > n)
-    PYRE_HIP_CHECK(hipMemsetAsync(N->acts + (size_t)n * 128, 0, (size_t)(np - n) * 128 * sizeof(__bf16), st));
-  const unsigned tb = blocks(np / 16, 4);
-  for (int l = 0; l < kLayers - 1; ++l)
-    hipLaunchKernelGGL((k_fwd_layer<true>), dim3(tb), dim3(256), 0, st, N->wf, l, N->acts + act_offset(l, cap), np,
-                       N->acts + act_offset(l + 1, cap), (float*)nullptr);
-  hipLaunchKernelGGL((k_fwd_layer<false>), dim3(tb), dim3(256), 0, st, N->wf, kLayers - 1,
-                     N->acts + act_offset(kLayers - 1, cap), np, (__bf16*)nullptr, N->pred);
-  hipLaunchKernelGGL(k_loss, dim3(blocks(np, 256)), dim3(256), 0, st, N->pred, targets, np, n, N->dz0, N->loss_part);
-  __bf16* dz = N->dz0;
-  __bf16* dzn = N->dz1;
-  const uint32_t nch = blocks(np, kDwChunk);
-  for (int l = kLayers - 1; l >= 0; --l) {
-    hipLaunchKernelGGL(k_bwd_dw, dim3(nch), dim3(256), 0, st, l, N->acts + act_offset(l, cap), dz, np, N->part);
-    const int size = kIn[l] * kOut[l];
-    hipLaunchKernelGGL(k_dw_reduce, dim3(blocks(size, 256)), dim3(256), 0, st, N->part, nch, size,
-                       N->grad + param_offset(l));
-    if (l > 0) {
-      hipLaunchKernelGGL(k_bwd_dx, dim3(tb), dim3(256), 0, st, N->wb, l, dz, N->acts + act_offset(l, cap), np, dzn);
-      std::swap(dz, dzn);
-    }
-  }
-  if (loss_dev) {
-    hipLaunchKernelGGL(k_sum_into, dim3(1), dim3(256), 0, st, N->loss_part, blocks(np, 256), 1.0f / (3.0f * n),
-                       loss_dev);
-    return 0.0f;
-  }
-  std::vector<float> lp(blocks(np, 256));
+  const unsigned tiles = np / 16;
+  const unsigned grid = std::max(1u, std::min(blocks(tiles, kTrainWaves), (unsigned)std::max(1, N->ctx->cus) * 2));
+  hipLaunchKernelGGL(k_train_fwd, dim3(grid), dim3(64 * kTrainWaves), 0, st, N->wf, feats, targets, np, n, N->bounds,
+                     N->acts, cap, N->dz + dz_offset(5, cap), N->loss_part);
+  hipLaunchKernelGGL(k_train_bwd, dim3(grid), dim3(64 * kTrainWaves), 0, st, N->wb, N->acts, np, cap, N->dz);
+  hipLaunchKernelGGL(k_train_dw, dim3(dw_partials(np), kLayers), dim3(256), 0, st, N->acts, N->dz, np, cap, N->part);
+  return grid;
+}
+
+// Mean loss of the batch from the per-block sums (synchronises the stream).
+static float read_loss(pyre_nrc* N, unsigned nlp, uint32_t n, hipStream_t st) {
+  std::vector<float> lp(nlp);
   PYRE_HIP_CHECK(hipMemcpyAsync(lp.data(), N->loss_part, lp.size() * 4, hipMemcpyDeviceToHost, st));
   PYRE_HIP_CHECK(hipStreamSynchronize(st));
-  double s = 0;
-  for (float x : lp) s += x;
-  N->last_loss = (float)(s / (3.0 * n));
+  double sum = 0;
+  for (float x : lp) sum += x;
+  N->last_loss = (float)(sum / (3.0 * n));
   return N->last_loss;
 }
 
-void nrc_adam_step(pyre_nrc* N, hipStream_t st) {
+float nrc_gradients_dev(pyre_nrc* N, const float* feats, const float* targets, uint32_t n, hipStream_t st,
+                        float* loss_dev) {
+  const unsigned nlp = train_pass(N, feats, targets, n, st);
+  const uint32_t np = (n + 15) / 16 * 16;
+  hipLaunchKernelGGL(k_train_reduce, dim3(kParams / 64), dim3(256), 0, st, N->part, dw_partials(np), N->grad);
+  if (loss_dev) {
+    hipLaunchKernelGGL(k_sum_into, dim3(1), dim3(256), 0, st, N->loss_part, nlp, 1.0f / (3.0f * n), loss_dev);
+    return 0.0f;
+  }
+  return read_loss(N, nlp, n, st);
+}
+
+float nrc_train_step_dev(pyre_nrc* N, const float* feats, const float* targets, uint32_t n, hipStream_t st,
+                         float* loss_dev) {
+  const unsigned nlp = train_pass(N, feats, targets, n, st);
+  const uint32_t np = (n + 15) / 16 * 16;
   N->step++;
-  const float c1 = 1.0f - powf(N->prm.beta1, (float)N->step);
-  const float c2 = 1.0f - powf(N->prm.beta2, (float)N->step);
-  hipLaunchKernelGGL(k_adam, dim3(blocks(kParams, 256)), dim3(256), 0, st, N->w, N->grad, N->m, N->v, kParams,
-                     N->prm.learning_rate, N->prm.beta1, N->prm.beta2, N->prm.epsilon, c1, c2);
-  nrc_refresh_frags(N, st);
+  const AdamArgs a{N->prm.learning_rate, N->prm.beta1, N->prm.beta2, N->prm.epsilon,
+                   1.0f - powf(N->prm.beta1, (float)N->step), 1.0f - powf(N->prm.beta2, (float)N->step)};
+  hipLaunchKernelGGL(k_train_update, dim3(kParams / 64), dim3(256), 0, st, N->part, dw_partials(np), N->grad, N->w,
+                     N->m, N->v, a, N->wf, N->wb, N->loss_part, nlp, 1.0f / (3.0f * n), loss_dev);
+  return loss_dev ? 0.0f : read_loss(N, nlp, n, st);
 }
 
 void nrc_infer_dev(pyre_nrc* N, const float* feats, uint32_t n, float* rgb, hipStream_t st) {
@@ -610,9 +782,7 @@ void pyre_nrc_destroy(pyre_nrc* N) {
   dev_free(N->wf);
   dev_free(N->wb);
   dev_free(N->acts);
-  dev_free(N->dz0);
-  dev_free(N->dz1);
-  dev_free(N->pred);
+  dev_free(N->dz);
   dev_free(N->part);
   dev_free(N->loss_part);
   dev_free(N->feats);
@@ -691,9 +861,7 @@ int pyre_nrc_train_step(pyre_ctx* ctx, pyre_nrc* N, const float* feats, const fl
     hipStream_t st = ctx->stream;
     PYRE_HIP_CHECK(hipMemcpyAsync(N->feats, feats, 13 * 4 * n, hipMemcpyHostToDevice, st));
     PYRE_HIP_CHECK(hipMemcpyAsync(N->targets, targets, 3 * 4 * n, hipMemcpyHostToDevice, st));
-    float l = nrc_gradients_dev(N, N->feats, N->targets, (uint32_t)n, st, nullptr);
-    nrc_adam_step(N, st);
-    PYRE_HIP_CHECK(hipStreamSynchronize(st));
+    float l = nrc_train_step_dev(N, N->feats, N->targets, (uint32_t)n, st, nullptr);
     if (loss) *loss = l;
   });
 }

@@ -18,7 +18,11 @@ int nrc_param_count();
 // stream without synchronising (and returns 0).
 float nrc_gradients_dev(pyre_nrc* nrc, const float* feats, const float* targets, uint32_t n, hipStream_t st,
                         float* loss_dev = nullptr);
-void nrc_adam_step(pyre_nrc* nrc, hipStream_t st);
+// One Adam step (Kingma & Ba) on a batch: the gradients as above, then the
+// FP32 master weights, moments and bf16 fragments updated in one launch; the
+// loss is returned or added to *loss_dev as above.
+float nrc_train_step_dev(pyre_nrc* nrc, const float* feats, const float* targets, uint32_t n, hipStream_t st,
+                         float* loss_dev = nullptr);
 // Fused BF16-MFMA inference: rgb[3n] from feats[13n] (device arrays).
 void nrc_infer_dev(pyre_nrc* nrc, const float* feats, uint32_t n, float* rgb, hipStream_t st);
 // Inference over a device-side query count (*n_dev, at most cap queries):
