@@ -293,6 +293,8 @@ def _crossing(curve, target):
 # sky, so 2^20 paths give about 2^18 surface-vertex training samples (2^15
 # per spp of an 8-spp frame).
 NRC_TRAIN_PATHS = 1 << 20
+# Adam steps per frame: batches of 2^14 cover the ~2^18 samples once.
+NRC_STEPS = 20
 # Path pool of the plain-vs-NRC comparison: 16 spp per pass at 1080p for both.
 NRC_POOL_PATHS = 1 << 25
 
@@ -314,8 +316,8 @@ def bench_nrc(args):
     start, timed whole: plain = one session render of N spp; NRC = a fresh
     cache, nrc_warmup training-only frames, then frames of min(N, 16) spp.
     Each NRC frame traces its samples and 2^20 training paths in one pass,
-    trains on the training paths' ~2^18 samples (Adam, batches of 2^14, every
-    sample once), then adds the updated cache's radiance to the paths that
+    trains on the training paths' ~2^18 samples (20 Adam steps on batches of
+    2^14, every sample once), then adds the updated cache's radiance to the paths that
     ended in it. Both use a pool of 2^25 paths (passes of up to 16 spp).
     relMSE against the FP64 reference is taken outside the timed region."""
     os.environ["PYRE_PASS_PATHS"] = str(NRC_POOL_PATHS)
@@ -342,7 +344,7 @@ def bench_nrc(args):
         else:
             spf = min(n, 16)
             sess.render_nrc(cache, frames=args.nrc_warmup + n // spf, spp_per_frame=spf,
-                            train_paths=NRC_TRAIN_PATHS, train_batch=1 << 14, train_steps=64,
+                            train_paths=NRC_TRAIN_PATHS, train_batch=1 << 14, train_steps=NRC_STEPS,
                             warmup_frames=args.nrc_warmup)
         dt = time.perf_counter() - t0  # render calls return after the stream is idle
         _, r64, spp = sess.read(mean=False, radiance64=True)

@@ -134,6 +134,25 @@ __global__ void k_env_marginal(const float* row_integral, uint32_t h, float* mar
   }
 }
 
+// Guide table of each CDF (rows of n + 1 entries, `stride` apart): one
+// thread per (row, k) finds the first i in [1, n) with cdf[i] > k / kEnvGuide.
+__global__ void k_env_guide(const float* cdf, uint32_t n, size_t stride, uint32_t rows, uint32_t* guide) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= rows * (kEnvGuide + 1)) return;
+  const uint32_t row = t / (kEnvGuide + 1), k = t % (kEnvGuide + 1);
+  const float* c = cdf + (size_t)row * stride;
+  const float x = (float)k / (float)kEnvGuide;
+  uint32_t lo = 1, hi = n;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (c[mid] <= x)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  guide[t] = lo;
+}
+
 void scene_free(pyre_scene* s) {
   bvh_release(s);
   dev_free(s->d_verts);
@@ -145,6 +164,7 @@ void scene_free(pyre_scene* s) {
   dev_free(s->d_env_texels);
   dev_free(s->d_env_cond);
   dev_free(s->d_env_marg);
+  dev_free(s->d_env_guide);
   dev_free(s->d_flag);
   dev_free(s->d_gstack);
   dev_free(s->d_fetch);
@@ -172,6 +192,12 @@ void upload_env(pyre_scene* s, const pyre_env_desc& e, hipStream_t st) {
   hipLaunchKernelGGL(k_env_rows, dim3(blocks(e.height, 64)), dim3(64), 0, st, s->d_env_texels, d_sin, e.width,
                      e.height, s->d_env_cond, d_rows);
   hipLaunchKernelGGL(k_env_marginal, dim3(1), dim3(1), 0, st, d_rows, e.height, s->d_env_marg);
+  s->d_env_guide = dev_alloc<uint32_t>((size_t)(e.height + 1) * (kEnvGuide + 1));
+  uint32_t* marg_guide = s->d_env_guide + (size_t)e.height * (kEnvGuide + 1);
+  hipLaunchKernelGGL(k_env_guide, dim3(blocks((size_t)e.height * (kEnvGuide + 1), 256)), dim3(256), 0, st,
+                     s->d_env_cond, e.width, (size_t)e.width + 1, e.height, s->d_env_guide);
+  hipLaunchKernelGGL(k_env_guide, dim3(blocks(kEnvGuide + 1, 256)), dim3(256), 0, st, s->d_env_marg, e.height,
+                     (size_t)0, 1u, marg_guide);
   PYRE_HIP_CHECK(hipStreamSynchronize(st));
   dev_free(d_sin);
   dev_free(d_rows);
@@ -182,6 +208,8 @@ void upload_env(pyre_scene* s, const pyre_env_desc& e, hipStream_t st) {
   s->env.texels = s->d_env_texels;
   s->env.cond_cdf = s->d_env_cond;
   s->env.marg_cdf = s->d_env_marg;
+  s->env.cond_guide = s->d_env_guide;
+  s->env.marg_guide = marg_guide;
 }
 
 // ---------------------------------------------------------- probe kernels

@@ -139,7 +139,7 @@ struct Paths {
   uint32_t* q_shadow;
   uint32_t* counters;  // [0] current, [1] next, [2] shadow, [3] cache queries, [4] training samples,
                        // [8] extend work, [16] shadow work
-  unsigned long long* stats;  // [0] extension rays, [1] shadow rays, [2] cache queries
+  unsigned long long* stats;  // [0] extension rays, [1] shadow rays, [2] cache queries, [3] training samples
   uint32_t* pjob;    // job of each path
   float4* vrec;      // training paths: kVrec float4 per (training path, vertex)
   uint32_t* vcount;  // training paths: vertices recorded
@@ -782,6 +782,9 @@ __global__ __launch_bounds__(kThreads) void k_train_targets(Paths P, Sched sc, f
   }
 }
 
+// stats[3]: training samples produced (counters[4] of the pass just traced)
+__global__ void k_count_train(Paths P) { P.stats[3] += P.counters[4]; }
+
 // CACHE: L += beta x max(cache, 0) for every path that ended in the cache.
 __global__ __launch_bounds__(kThreads) void k_cache_apply(Paths P) {
   const uint32_t n = P.counters[3];
@@ -1087,7 +1090,6 @@ struct pyre_session {
   float* welford = nullptr;
   uint32_t spp_done = 0, next_sample = 0, train_frames = 0;
   uint64_t stats_seen[2] = {0, 0};  // ray counters already reported
-  uint64_t train_samples = 0;       // radiance-cache training samples produced
   bool use_graph = true;
   hipGraphExec_t exec = nullptr;        // PLAIN pass
   hipGraphExec_t exec_cache = nullptr;   // CACHE pass trace + training targets
@@ -1148,9 +1150,11 @@ void session_pass(pyre_session* S, uint32_t s_base, uint32_t n_samples, uint32_t
     require(accumulate, "cache passes always accumulate");
     auto trace = [&] {
       launch_pass(S->ctx, S->W, S->shape, S->tune, false, st);
-      if (S->W.vmax)
+      if (S->W.vmax) {
         hipLaunchKernelGGL(k_train_targets, dim3(stage_grid(S->ctx)), dim3(kThreads), 0, st, S->W.P, S->W.sched(),
                            S->W.tfeat, S->W.ttarget, S->W.tcap);
+        hipLaunchKernelGGL(k_count_train, dim3(1), dim3(1), 0, st, S->W.P);
+      }
     };
     auto finish = [&] {
       nrc_infer_dev_count(nrc, S->W.P.qfeat, S->W.P.qpath, S->W.P.counters + 3, (uint32_t)S->W.cap, S->W.P.qrgb,
@@ -1580,11 +1584,14 @@ int pyre_session_render_nrc(pyre_ctx* ctx, pyre_session* S, pyre_nrc* nrc, const
             "train_paths exceeds the session's nrc_train_paths");
     require(p->train_batch >= 16, "train_batch below 16");
     hipStream_t st = ctx->stream;
-    // per-frame loss sums stay on the device until the end (no sync per step)
-    float* dloss = dev_alloc<float>(std::max(1u, p->frames));
-    std::vector<uint32_t> nsteps(p->frames, 0);
-    hipLaunchKernelGGL(k_fill_f32, dim3(nblocks(std::max(1u, p->frames), 256)), dim3(256), 0, st, dloss,
-                       (size_t)std::max(1u, p->frames), 0.0f);
+    // Per-frame loss sums and step counts stay on the device until the end:
+    // the training steps are queued without knowing how many samples the
+    // frame produced (each step reads the count and skips itself when its
+    // batch would hold fewer than 16), so nothing waits on the host.
+    const size_t nf = std::max(1u, p->frames);
+    float* dloss = dev_alloc<float>(2 * nf);
+    uint32_t* dsteps = reinterpret_cast<uint32_t*>(dloss + nf);
+    hipLaunchKernelGGL(k_fill_f32, dim3(nblocks(2 * nf, 256)), dim3(256), 0, st, dloss, 2 * nf, 0.0f);
     struct Free {
       float* p;
       ~Free() { dev_free(p); }
@@ -1603,17 +1610,11 @@ int pyre_session_render_nrc(pyre_ctx* ctx, pyre_session* S, pyre_nrc* nrc, const
       tr.pix_mul = (uint32_t)(2654435761ull % S->npix) | 1u;
       tr.pix_add = (uint32_t)(pixel_seed(S->train_frames, 7, 11) % S->npix);
       auto train = [&] {
-        uint32_t nsamp = 0;
-        PYRE_HIP_CHECK(hipMemcpyAsync(&nsamp, S->W.P.counters + 4, 4, hipMemcpyDeviceToHost, st));
-        PYRE_HIP_CHECK(hipStreamSynchronize(st));
-        S->train_samples += nsamp;
-        nsamp = std::min(nsamp, S->W.tcap);
         for (uint32_t k = 0; k < p->train_steps; ++k) {
-          const uint32_t off = k * p->train_batch;
-          if (off + 16 > nsamp) break;
-          const uint32_t nb = std::min(p->train_batch, nsamp - off);
-          nrc_train_step_dev(nrc, S->W.tfeat + 13 * (size_t)off, S->W.ttarget + 3 * (size_t)off, nb, st, dloss + f);
-          ++nsteps[f];
+          const uint64_t off = (uint64_t)k * p->train_batch;
+          if (off + 16 > S->W.tcap) break;  // past the sample buffer: never filled
+          nrc_train_step_counted(nrc, S->W.tfeat, S->W.ttarget, S->W.P.counters + 4, S->W.tcap, (uint32_t)off,
+                                 p->train_batch, st, dloss + f, dsteps + f);
         }
       };
       const uint32_t n_pass = spp ? (spp + S->per_pass - 1) / S->per_pass : 1u;
@@ -1628,11 +1629,19 @@ int pyre_session_render_nrc(pyre_ctx* ctx, pyre_session* S, pyre_nrc* nrc, const
       }
       S->train_frames++;
     }
-    if (losses && p->frames) {
-      std::vector<float> h(p->frames);
-      PYRE_HIP_CHECK(hipMemcpyAsync(h.data(), dloss, 4 * (size_t)p->frames, hipMemcpyDeviceToHost, st));
+    if (p->frames) {
+      std::vector<float> h(2 * nf);
+      PYRE_HIP_CHECK(hipMemcpyAsync(h.data(), dloss, 8 * nf, hipMemcpyDeviceToHost, st));
       PYRE_HIP_CHECK(hipStreamSynchronize(st));
-      for (uint32_t f = 0; f < p->frames; ++f) losses[f] = nsteps[f] ? h[f] / (float)nsteps[f] : 0.0f;
+      float last = 0.0f;
+      for (uint32_t f = 0; f < p->frames; ++f) {
+        uint32_t n;
+        std::memcpy(&n, &h[nf + f], 4);
+        const float l = n ? h[f] / (float)n : 0.0f;
+        if (losses) losses[f] = l;
+        if (n) last = l;
+      }
+      nrc_set_last_loss(nrc, last);
     }
     finish_stats(S);
   });
@@ -1661,7 +1670,7 @@ int pyre_session_stats(pyre_ctx* ctx, pyre_session* S, uint64_t out[4]) {
     out[0] = hs[0];
     out[1] = hs[1];
     out[2] = hs[2];
-    out[3] = S->train_samples;
+    out[3] = hs[3];
   });
 }
 

@@ -113,12 +113,20 @@ struct MatDev {
 
 // HdriEnvironmentLight on the device (light.rs:205-211): texels as float4,
 // conditional CDFs (height rows of width + 1), marginal CDF (height + 1).
+// Guide tables over a CDF of n + 1 entries (cutpoint method, Chen & Asau
+// 1974): guide[k], k in [0, kEnvGuide], is the first index i in [1, n) with
+// cdf[i] > k / kEnvGuide, or n. A sample u in [k, k + 1) / kEnvGuide has its
+// bin search confined to [guide[k], guide[k + 1]].
+constexpr uint32_t kEnvGuide = 256;
+
 struct EnvDev {
   uint32_t width = 0, height = 0;
   float intensity = 1.0f;
   const float4* texels = nullptr;
   const float* cond_cdf = nullptr;
   const float* marg_cdf = nullptr;
+  const uint32_t* cond_guide = nullptr;  // kEnvGuide + 1 per row
+  const uint32_t* marg_guide = nullptr;  // kEnvGuide + 1
 };
 
 }  // namespace pyre
@@ -141,6 +149,7 @@ struct pyre_scene {
   float4* d_env_texels = nullptr;
   float* d_env_cond = nullptr;
   float* d_env_marg = nullptr;
+  uint32_t* d_env_guide = nullptr;  // conditional guides (rows x (kEnvGuide + 1)), then the marginal's
   float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
 
   // BVH

@@ -244,18 +244,34 @@ constexpr int kDwChunk = 128;
 
 // Four launches per step: fused encode + forward + loss gradient, fused
 // backward dX through all layers, weight gradients of all layers, and their
-// deterministic reduction.
+// deterministic reduction fused with the Adam update.
 constexpr int kTrainWaves = 4;
+
+// Size of a training batch: n given by the host, or, with count, the samples
+// [off, off + n) of the *count (at most cap) a device stage produced; none
+// when fewer than 16 remain, so a step can be queued before its size is known.
+struct BatchN {
+  const uint32_t* count;
+  uint32_t cap, off, n;
+};
+__device__ __forceinline__ uint32_t batch_size(const BatchN& b) {
+  if (!b.count) return b.n;
+  const uint32_t avail = min(*b.count, b.cap);
+  return avail >= b.off + 16u ? min(b.n, avail - b.off) : 0u;
+}
 
 // Encode + forward of every layer + relative-L2 loss gradient for tiles of 16
 // samples. Writes each layer's input (bf16, sample-major), the loss gradient
 // dP (16 wide, zero past RGB and for padding rows q >= n_real) and per-block
-// loss sums.
+// loss sums; counts the Adam step in *step when given and the batch is not
+// empty.
 __global__ __launch_bounds__(64 * kTrainWaves) void k_train_fwd(const bf16x8* __restrict__ wfrag,
                                                                 const float* __restrict__ feats,
-                                                                const float* __restrict__ targets, uint32_t np,
-                                                                uint32_t n_real, NrcBounds b, __bf16* acts,
-                                                                uint32_t cap, __bf16* dp, float* loss_part) {
+                                                                const float* __restrict__ targets, BatchN bn,
+                                                                NrcBounds b, __bf16* acts, uint32_t cap, __bf16* dp,
+                                                                float* loss_part, uint32_t* step) {
+  const uint32_t n_real = batch_size(bn), np = (n_real + 15u) / 16u * 16u;
+  if (step && n_real && blockIdx.x == 0 && threadIdx.x == 0) *step += 1u;
   __shared__ bf16x8 sw[kFwdFrags * 64];
   __shared__ __attribute__((aligned(16))) __bf16 sact[kTrainWaves][16][64 + 8];
   __shared__ float red[kTrainWaves];
@@ -339,8 +355,9 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_train_fwd(const bf16x8* __
 __host__ __device__ constexpr size_t dz_offset(int l, uint32_t cap) { return (size_t)cap * 64 * (size_t)l; }
 
 __global__ __launch_bounds__(64 * kTrainWaves) void k_train_bwd(const bf16x8* __restrict__ bfrag,
-                                                                const __bf16* __restrict__ acts, uint32_t np,
+                                                                const __bf16* __restrict__ acts, BatchN bn,
                                                                 uint32_t cap, __bf16* dz) {
+  const uint32_t np = (batch_size(bn) + 15u) / 16u * 16u;
   constexpr int kUsed = kBwdFrags - bwd_frags(0);  // layers 1..5
   __shared__ bf16x8 sw[kUsed * 64];
   __shared__ __attribute__((aligned(16))) __bf16 sdz[kTrainWaves][16][64 + 8];
@@ -428,7 +445,8 @@ __device__ __forceinline__ void stage_transposed(__bf16* dst, const __bf16* __re
 }
 
 __global__ __launch_bounds__(256) void k_train_dw(const __bf16* __restrict__ acts, const __bf16* __restrict__ dz,
-                                                  uint32_t np, uint32_t cap, float* partial) {
+                                                  BatchN bn, uint32_t cap, float* partial) {
+  const uint32_t np = (batch_size(bn) + 15u) / 16u * 16u;
   __shared__ __attribute__((aligned(16))) __bf16 sx[128 * kDwStride];
   __shared__ __attribute__((aligned(16))) __bf16 sz[64 * kDwStride];
   const int layer = blockIdx.y;
@@ -490,7 +508,7 @@ __global__ __launch_bounds__(256) void k_train_reduce(const float* partial, uint
   if (wave == 0) grad[i] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
 }
 
-// Adam hyper-parameters with the step's bias corrections c1 = 1 - b1^t,
+// Adam hyper-parameters with a step's bias corrections c1 = 1 - b1^t,
 // c2 = 1 - b2^t.
 struct AdamArgs {
   float lr, b1, b2, eps, c1, c2;
@@ -508,13 +526,20 @@ __device__ __forceinline__ float adam_update(float& m, float& v, float w, float 
 // new weights rounded to bf16 straight into their forward and backward
 // fragment slots (the inverse of k_make_frags' gather; the backward
 // fragments' zero padding is never written). The non-RGB output columns and
-// the padding rows of layer 0 have zero gradient and stay fixed. Block 0 also adds
-// loss_scale x sum(loss_part[0..nlp)) to *loss_dev when it is given.
+// the padding rows of layer 0 have zero gradient and stay fixed. The step
+// number t is *step (counted by k_train_fwd); an empty batch changes nothing.
+// Block 0 also adds the batch's mean loss to *loss_dev and counts the step
+// in *steps_dev when they are given.
 __global__ __launch_bounds__(256) void k_train_update(const float* partial, uint32_t P, float* grad, float* w,
-                                                      float* m, float* v, AdamArgs a, bf16x8* wf, bf16x8* wb,
-                                                      const float* loss_part, uint32_t nlp, float loss_scale,
-                                                      float* loss_dev) {
+                                                      float* m, float* v, AdamArgs a, const uint32_t* step,
+                                                      BatchN bn, bf16x8* wf, bf16x8* wb, const float* loss_part,
+                                                      uint32_t nlp, float* loss_dev, uint32_t* steps_dev) {
   __shared__ float red[4][64];
+  const uint32_t n = batch_size(bn);
+  if (!n) return;
+  const float t = (float)*step;
+  a.c1 = 1.0f - powf(a.b1, t);
+  a.c2 = 1.0f - powf(a.b2, t);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i = blockIdx.x * 64 + lane;
   int l = 0;
@@ -556,8 +581,9 @@ __global__ __launch_bounds__(256) void k_train_update(const float* partial, uint
       if ((int)threadIdx.x < o) lred[threadIdx.x] += lred[threadIdx.x + o];
       __syncthreads();
     }
-    if (threadIdx.x == 0) *loss_dev += loss_scale * lred[0];
+    if (threadIdx.x == 0) *loss_dev += lred[0] / (3.0f * (float)n);
   }
+  if (steps_dev && blockIdx.x == 0 && threadIdx.x == 0) *steps_dev += 1u;
 }
 
 inline unsigned blocks(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
@@ -578,7 +604,7 @@ struct pyre_nrc {
   float* grad = nullptr;
   bf16x8* wf = nullptr;  // forward fragments
   bf16x8* wb = nullptr;  // backward fragments
-  uint32_t step = 0;
+  uint32_t* d_step = nullptr;  // Adam steps taken (device counter)
   float last_loss = 0.0f;
   // training scratch, sized for the largest batch seen
   uint32_t cap = 0;
@@ -635,20 +661,22 @@ __global__ void k_sum_into(const float* part, uint32_t n, float scale, float* ou
   if (threadIdx.x == 0) *out += scale * red[0];
 }
 
-// Forward + loss + backward + weight-gradient partials of one batch
-// (feats/targets are device arrays). Returns the number of per-block loss
-// sums in N->loss_part; the partials' count is dw_partials(np).
-static unsigned train_pass(pyre_nrc* N, const float* feats, const float* targets, uint32_t n, hipStream_t st) {
-  require(n > 0, "empty training batch");
-  ensure_cap(N, n);
-  const uint32_t np = (n + 15) / 16 * 16;  // padding rows repeat a sample with zero gradient
+// Forward + loss + backward + weight-gradient partials of one batch of at
+// most bn.n samples (feats/targets are device arrays); step counts the Adam
+// step on the device. Returns the number of per-block loss sums in
+// N->loss_part; the partials' count is dw_partials of the padded bn.n.
+static unsigned train_pass(pyre_nrc* N, const float* feats, const float* targets, const BatchN& bn, uint32_t* step,
+                           hipStream_t st) {
+  require(bn.n > 0, "empty training batch");
+  ensure_cap(N, bn.n);
+  const uint32_t np = (bn.n + 15) / 16 * 16;
   const uint32_t cap = N->cap;
   const unsigned tiles = np / 16;
   const unsigned grid = std::max(1u, std::min(blocks(tiles, kTrainWaves), (unsigned)std::max(1, N->ctx->cus) * 2));
-  hipLaunchKernelGGL(k_train_fwd, dim3(grid), dim3(64 * kTrainWaves), 0, st, N->wf, feats, targets, np, n, N->bounds,
-                     N->acts, cap, N->dz + dz_offset(5, cap), N->loss_part);
-  hipLaunchKernelGGL(k_train_bwd, dim3(grid), dim3(64 * kTrainWaves), 0, st, N->wb, N->acts, np, cap, N->dz);
-  hipLaunchKernelGGL(k_train_dw, dim3(dw_partials(np), kLayers), dim3(256), 0, st, N->acts, N->dz, np, cap, N->part);
+  hipLaunchKernelGGL(k_train_fwd, dim3(grid), dim3(64 * kTrainWaves), 0, st, N->wf, feats, targets, bn, N->bounds,
+                     N->acts, cap, N->dz + dz_offset(5, cap), N->loss_part, step);
+  hipLaunchKernelGGL(k_train_bwd, dim3(grid), dim3(64 * kTrainWaves), 0, st, N->wb, N->acts, bn, cap, N->dz);
+  hipLaunchKernelGGL(k_train_dw, dim3(dw_partials(np), kLayers), dim3(256), 0, st, N->acts, N->dz, bn, cap, N->part);
   return grid;
 }
 
@@ -663,9 +691,14 @@ static float read_loss(pyre_nrc* N, unsigned nlp, uint32_t n, hipStream_t st) {
   return N->last_loss;
 }
 
+static AdamArgs adam_args(const pyre_nrc* N) {
+  return AdamArgs{N->prm.learning_rate, N->prm.beta1, N->prm.beta2, N->prm.epsilon, 0.0f, 0.0f};
+}
+
 float nrc_gradients_dev(pyre_nrc* N, const float* feats, const float* targets, uint32_t n, hipStream_t st,
                         float* loss_dev) {
-  const unsigned nlp = train_pass(N, feats, targets, n, st);
+  const BatchN bn{nullptr, 0, 0, n};
+  const unsigned nlp = train_pass(N, feats, targets, bn, nullptr, st);
   const uint32_t np = (n + 15) / 16 * 16;
   hipLaunchKernelGGL(k_train_reduce, dim3(kParams / 64), dim3(256), 0, st, N->part, dw_partials(np), N->grad);
   if (loss_dev) {
@@ -677,15 +710,27 @@ float nrc_gradients_dev(pyre_nrc* N, const float* feats, const float* targets, u
 
 float nrc_train_step_dev(pyre_nrc* N, const float* feats, const float* targets, uint32_t n, hipStream_t st,
                          float* loss_dev) {
-  const unsigned nlp = train_pass(N, feats, targets, n, st);
+  const BatchN bn{nullptr, 0, 0, n};
+  const unsigned nlp = train_pass(N, feats, targets, bn, N->d_step, st);
   const uint32_t np = (n + 15) / 16 * 16;
-  N->step++;
-  const AdamArgs a{N->prm.learning_rate, N->prm.beta1, N->prm.beta2, N->prm.epsilon,
-                   1.0f - powf(N->prm.beta1, (float)N->step), 1.0f - powf(N->prm.beta2, (float)N->step)};
   hipLaunchKernelGGL(k_train_update, dim3(kParams / 64), dim3(256), 0, st, N->part, dw_partials(np), N->grad, N->w,
-                     N->m, N->v, a, N->wf, N->wb, N->loss_part, nlp, 1.0f / (3.0f * n), loss_dev);
+                     N->m, N->v, adam_args(N), (const uint32_t*)N->d_step, bn, N->wf, N->wb, N->loss_part, nlp,
+                     loss_dev, (uint32_t*)nullptr);
   return loss_dev ? 0.0f : read_loss(N, nlp, n, st);
 }
+
+void nrc_train_step_counted(pyre_nrc* N, const float* feats, const float* targets, const uint32_t* count,
+                            uint32_t cap, uint32_t off, uint32_t batch, hipStream_t st, float* loss_dev,
+                            uint32_t* steps_dev) {
+  const BatchN bn{count, cap, off, batch};
+  const unsigned nlp = train_pass(N, feats + 13 * (size_t)off, targets + 3 * (size_t)off, bn, N->d_step, st);
+  const uint32_t np = (batch + 15) / 16 * 16;
+  hipLaunchKernelGGL(k_train_update, dim3(kParams / 64), dim3(256), 0, st, N->part, dw_partials(np), N->grad, N->w,
+                     N->m, N->v, adam_args(N), (const uint32_t*)N->d_step, bn, N->wf, N->wb, N->loss_part, nlp,
+                     loss_dev, steps_dev);
+}
+
+void nrc_set_last_loss(pyre_nrc* N, float loss) { N->last_loss = loss; }
 
 void nrc_infer_dev(pyre_nrc* N, const float* feats, uint32_t n, float* rgb, hipStream_t st) {
   if (n == 0) return;
@@ -758,9 +803,11 @@ int pyre_nrc_create(pyre_ctx* ctx, const pyre_nrc_params* p, pyre_nrc** out) {
     N->grad = dev_alloc<float>(kParams);
     N->wf = dev_alloc<bf16x8>((size_t)kFwdFrags * 64);
     N->wb = dev_alloc<bf16x8>((size_t)kBwdFrags * 64);
+    N->d_step = dev_alloc<uint32_t>(1);
     PYRE_HIP_CHECK(hipMemcpyAsync(N->w, w.data(), kParams * 4, hipMemcpyHostToDevice, st));
     PYRE_HIP_CHECK(hipMemsetAsync(N->m, 0, kParams * 4, st));
     PYRE_HIP_CHECK(hipMemsetAsync(N->v, 0, kParams * 4, st));
+    PYRE_HIP_CHECK(hipMemsetAsync(N->d_step, 0, 4, st));
     nrc_refresh_frags(N, st);
     PYRE_HIP_CHECK(hipStreamSynchronize(st));
   });
@@ -781,6 +828,7 @@ void pyre_nrc_destroy(pyre_nrc* N) {
   dev_free(N->grad);
   dev_free(N->wf);
   dev_free(N->wb);
+  dev_free(N->d_step);
   dev_free(N->acts);
   dev_free(N->dz);
   dev_free(N->part);
@@ -806,7 +854,7 @@ int pyre_nrc_set_weights(pyre_ctx* ctx, pyre_nrc* N, const float* w) {
     PYRE_HIP_CHECK(hipMemcpyAsync(N->w, w, kParams * 4, hipMemcpyHostToDevice, ctx->stream));
     PYRE_HIP_CHECK(hipMemsetAsync(N->m, 0, kParams * 4, ctx->stream));
     PYRE_HIP_CHECK(hipMemsetAsync(N->v, 0, kParams * 4, ctx->stream));
-    N->step = 0;
+    PYRE_HIP_CHECK(hipMemsetAsync(N->d_step, 0, 4, ctx->stream));
     nrc_refresh_frags(N, ctx->stream);
     PYRE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   });
@@ -848,9 +896,13 @@ int pyre_nrc_gradients(pyre_ctx* ctx, pyre_nrc* N, const float* feats, const flo
 
 int pyre_nrc_stats(pyre_ctx* ctx, pyre_nrc* N, uint64_t* steps, float* last_loss) {
   if (!ctx || !N) return PYRE_EINVAL;
-  if (steps) *steps = N->step;
-  if (last_loss) *last_loss = N->last_loss;
-  return PYRE_OK;
+  return nrc_guard(ctx, [&] {
+    uint32_t t = 0;
+    PYRE_HIP_CHECK(hipMemcpyAsync(&t, N->d_step, 4, hipMemcpyDeviceToHost, ctx->stream));
+    PYRE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    if (steps) *steps = t;
+    if (last_loss) *last_loss = N->last_loss;
+  });
 }
 
 int pyre_nrc_train_step(pyre_ctx* ctx, pyre_nrc* N, const float* feats, const float* targets, size_t n, float* loss) {

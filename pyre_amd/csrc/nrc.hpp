@@ -23,6 +23,15 @@ float nrc_gradients_dev(pyre_nrc* nrc, const float* feats, const float* targets,
 // loss is returned or added to *loss_dev as above.
 float nrc_train_step_dev(pyre_nrc* nrc, const float* feats, const float* targets, uint32_t n, hipStream_t st,
                          float* loss_dev = nullptr);
+// One Adam step queued before its batch size is known: the samples
+// [off, off + batch) of the min(*count, cap) in feats/targets, skipped on the
+// device when fewer than 16 remain. Adds the batch's mean loss to *loss_dev
+// and counts the step in *steps_dev (either may be null). No host sync.
+void nrc_train_step_counted(pyre_nrc* nrc, const float* feats, const float* targets, const uint32_t* count,
+                            uint32_t cap, uint32_t off, uint32_t batch, hipStream_t st, float* loss_dev,
+                            uint32_t* steps_dev);
+// Records the loss reported by pyre_nrc_stats for training run elsewhere.
+void nrc_set_last_loss(pyre_nrc* nrc, float loss);
 // Fused BF16-MFMA inference: rgb[3n] from feats[13n] (device arrays).
 void nrc_infer_dev(pyre_nrc* nrc, const float* feats, uint32_t n, float* rgb, hipStream_t st);
 // Inference over a device-side query count (*n_dev, at most cap queries):

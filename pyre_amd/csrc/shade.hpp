@@ -216,8 +216,19 @@ PH float quad_pdf(const QuadLightDev& L, V3 from, V3 wi) {  // :135-144
 
 // ----------------------------------------------------------- distribution.rs
 // Distribution1D::sample_continuous over a cdf of n + 1 entries (:78-101).
-PH void dist1d_sample(const float* cdf, uint32_t n, float u, float* t, float* pdf, uint32_t* bin_out) {
+// The reference's binary search finds the first i in [1, n) with cdf[i] > u
+// (n if none); with a guide table the same search runs over the bracket
+// [guide[k], guide[k + 1]] that holds that index (k = floor(u kEnvGuide),
+// exact for a power-of-two table), so the bin is the same.
+PH void dist1d_sample(const float* cdf, uint32_t n, float u, float* t, float* pdf, uint32_t* bin_out,
+                      const uint32_t* guide = nullptr) {
   uint32_t lo = 1, hi = n;
+  if (guide) {
+    uint32_t k = (uint32_t)(u * (float)kEnvGuide);
+    k = k < kEnvGuide ? k : kEnvGuide - 1u;
+    lo = guide[k];
+    hi = guide[k + 1];
+  }
   while (lo < hi) {
     uint32_t mid = (lo + hi) / 2;
     if (cdf[mid] <= u)
@@ -309,8 +320,9 @@ __device__ __forceinline__ V3 env_le(const EnvDev& e, V3 wi) {  // :318-320
 __device__ __forceinline__ void env_sample(const EnvDev& e, V2 u, V3* wi, V3* li, float* pdf) {  // :322-338
   float v, pdf_v, uu, pdf_u;
   uint32_t row, col;
-  dist1d_sample(e.marg_cdf, e.height, u.y, &v, &pdf_v, &row);  // distribution.rs:151-155
-  dist1d_sample(e.cond_cdf + (size_t)row * (e.width + 1), e.width, u.x, &uu, &pdf_u, &col);
+  dist1d_sample(e.marg_cdf, e.height, u.y, &v, &pdf_v, &row, e.marg_guide);  // distribution.rs:151-155
+  dist1d_sample(e.cond_cdf + (size_t)row * (e.width + 1), e.width, u.x, &uu, &pdf_u, &col,
+                e.cond_guide ? e.cond_guide + (size_t)row * (kEnvGuide + 1) : nullptr);
   V2 uv{uu, v};
   float pdf_uv = pdf_u * pdf_v;
   *wi = env_uv_to_dir(uv);

@@ -139,6 +139,26 @@ def test_env_sample_pdf_le(ctx, oracle):
     assert np.all(np.abs(g - o) <= 1e-4 * np.maximum(np.abs(o), 1.0))
 
 
+def test_env_sample_at_guide_boundaries(ctx, oracle):
+    """The device search starts from a guide table of 256 buckets per CDF;
+    u exactly on a bucket boundary k/256 and one float either side must land
+    in the same bins as the reference's plain binary search."""
+    env = _env(oracle)
+    sc = _env_scene(ctx, env)
+    oe = oracle.Hdri(env["width"], env["height"], env["rgb"], env["intensity"])
+    k = np.arange(257, dtype=np.float64) / 256.0
+    edge = np.concatenate([k, np.nextafter(k, -1.0), np.nextafter(k, 2.0)]).astype(np.float32)
+    edge = edge[(edge >= 0.0) & (edge < 1.0)]
+    rng = np.random.default_rng(11)
+    n = len(edge)
+    u = np.stack([np.concatenate([edge, rng.random(n), edge]),
+                  np.concatenate([rng.random(n), edge, edge[::-1]])], 1).astype(np.float32)
+    wi, li, pdf = sc.env_sample(u)
+    owi, oli, opdf = oe.sample(u)
+    assert np.all(np.abs(wi - owi) <= 2e-6)
+    assert np.all(rel_err(pdf, opdf) <= TOL)
+
+
 def test_quad_light_sample_pdf(ctx, oracle):
     cb = pyre_amd.Preset.cornell()
     sc = pyre_amd.Scene.from_preset(ctx, cb)
