@@ -127,9 +127,25 @@ __device__ __forceinline__ float encode_feature(int f, const Query& q) {
 }
 
 // B fragment of the encoding for K block kb: features 32kb + 8g + j.
+// Features 2i and 2i + 1 below 72 are sin and cos of one argument: one
+// sincospi per pair.
 __device__ __forceinline__ bf16x8 encode_frag(const Query& q, int kb, int g) {
   bf16x8 v;
-  for (int j = 0; j < 8; ++j) v[j] = to_bf16(encode_feature(32 * kb + 8 * g + j, q));
+#pragma unroll
+  for (int j = 0; j < 8; j += 2) {
+    const int f = 32 * kb + 8 * g + j;
+    if (f < 72) {
+      const int a = f / 24, k = (f % 24) / 2;
+      const float pa = a == 0 ? q.px : (a == 1 ? q.py : q.pz);
+      float sv, cv;
+      sincospif(ldexpf(pa, k), &sv, &cv);
+      v[j] = to_bf16(sv);
+      v[j + 1] = to_bf16(cv);
+    } else {
+      v[j] = to_bf16(encode_feature(f, q));
+      v[j + 1] = to_bf16(encode_feature(f + 1, q));
+    }
+  }
   return v;
 }
 
@@ -167,6 +183,25 @@ __global__ void k_make_frags(const float* w, bf16x8* fwd, bf16x8* bwd) {
   }
 }
 
+// dst[0..N) = src[0..N) by a block of THREADS threads: every load is issued
+// before the first LDS store, so the copy costs one memory latency, not one
+// per element a thread moves.
+template <int N, int THREADS>
+__device__ __forceinline__ void stage_frags(bf16x8* dst, const bf16x8* __restrict__ src) {
+  constexpr int kIt = (N + THREADS - 1) / THREADS;
+  bf16x8 v[kIt];
+#pragma unroll
+  for (int it = 0; it < kIt; ++it) {
+    const int i = (int)threadIdx.x + it * THREADS;
+    if (N % THREADS == 0 || i < N) v[it] = src[i];
+  }
+#pragma unroll
+  for (int it = 0; it < kIt; ++it) {
+    const int i = (int)threadIdx.x + it * THREADS;
+    if (N % THREADS == 0 || i < N) dst[i] = v[it];
+  }
+}
+
 // ---------------------------------------------------------- inference
 // Fused forward of whole 16-query tiles: every layer's weight fragments sit
 // in LDS (51 KB), hidden activations go through a 2 KB per-wave LDS tile.
@@ -187,7 +222,7 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_infer(const bf16x8* __rest
   // query count: n_cap, or the device-side count (clamped to n_cap) when given
   const uint32_t n = n_dev ? min(*n_dev, n_cap) : n_cap;
   __shared__ __attribute__((aligned(16))) __bf16 sact[kInferWaves][16][64 + 8];
-  for (int i = threadIdx.x; i < kFwdFrags * 64; i += blockDim.x) sw[i] = wfrag[i];
+  stage_frags<kFwdFrags * 64, 64 * kInferWaves>(sw, wfrag);
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int g = lane >> 4, r = lane & 15;
@@ -275,7 +310,7 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_train_fwd(const bf16x8* __
   __shared__ bf16x8 sw[kFwdFrags * 64];
   __shared__ __attribute__((aligned(16))) __bf16 sact[kTrainWaves][16][64 + 8];
   __shared__ float red[kTrainWaves];
-  for (int i = threadIdx.x; i < kFwdFrags * 64; i += blockDim.x) sw[i] = wfrag[i];
+  stage_frags<kFwdFrags * 64, 64 * kTrainWaves>(sw, wfrag);
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int g = lane >> 4, r = lane & 15;
@@ -361,7 +396,7 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_train_bwd(const bf16x8* __
   constexpr int kUsed = kBwdFrags - bwd_frags(0);  // layers 1..5
   __shared__ bf16x8 sw[kUsed * 64];
   __shared__ __attribute__((aligned(16))) __bf16 sdz[kTrainWaves][16][64 + 8];
-  for (int i = threadIdx.x; i < kUsed * 64; i += blockDim.x) sw[i] = bfrag[bwd_frag_offset(1) * 64 + i];
+  stage_frags<kUsed * 64, 64 * kTrainWaves>(sw, bfrag + bwd_frag_offset(1) * 64);
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int g = lane >> 4, r = lane & 15;
@@ -420,26 +455,81 @@ static_assert(param_offset(1) % 64 == 0 && param_offset(2) % 64 == 0 && param_of
                   kParams % 64 == 0,
               "parameter blocks of 64 stay within one layer");
 
-// dst[f * kDwStride + s] = src[(q0 + s) * width + f] for the chunk's samples
-// (zero past np): work items are (sample pair, 8-feature group), sample pair
+// One chunk's samples held in registers between its global loads and its
+// feature-major LDS copy (dst[f * kDwStride + s] = src[(q0 + s) * width + f],
+// zero past np). Work items are (sample pair, 8-feature group), sample pair
 // fastest, so each LDS store is one 32-bit word holding two samples.
-__device__ __forceinline__ void stage_transposed(__bf16* dst, const __bf16* __restrict__ src, uint32_t q0,
-                                                 uint32_t np, int width) {
-  constexpr int kPairs = kDwChunk / 2;
-  uint32_t* d = reinterpret_cast<uint32_t*>(dst);
-  const int n = kPairs * (width / 8);
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    const int sp = i % kPairs, f8 = i / kPairs;
-    const uint32_t s = 2 * sp;
-    uint4 a = make_uint4(0, 0, 0, 0), c = make_uint4(0, 0, 0, 0);
-    if (q0 + s < np) a = *(const uint4*)(src + (size_t)(q0 + s) * width + 8 * f8);
-    if (q0 + s + 1 < np) c = *(const uint4*)(src + (size_t)(q0 + s + 1) * width + 8 * f8);
-    const uint32_t wa[4] = {a.x, a.y, a.z, a.w}, wc[4] = {c.x, c.y, c.z, c.w};
+constexpr int kPairs = kDwChunk / 2;
+template <int W>
+struct ChunkRegs {
+  static constexpr int kTotal = kPairs * (W / 8);
+  static constexpr int kItems = (kTotal + 255) / 256;  // per thread (blocks of 256)
+  uint4 a[kItems], c[kItems];
+  __device__ __forceinline__ void load(const __bf16* __restrict__ src, uint32_t q0, uint32_t np) {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const uint32_t lo = (wa[j >> 1] >> (16 * (j & 1))) & 0xffffu;
-      const uint32_t hi = (wc[j >> 1] >> (16 * (j & 1))) & 0xffffu;
-      d[((8 * f8 + j) * kDwStride + s) >> 1] = lo | (hi << 16);
+    for (int it = 0; it < kItems; ++it) {
+      const int i = threadIdx.x + 256 * it;
+      const int sp = i % kPairs, f8 = i / kPairs;
+      const uint32_t s = 2 * sp;
+      if (kTotal % 256 != 0 && i >= kTotal) break;
+      a[it] = q0 + s < np ? *(const uint4*)(src + (size_t)(q0 + s) * W + 8 * f8) : make_uint4(0, 0, 0, 0);
+      c[it] = q0 + s + 1 < np ? *(const uint4*)(src + (size_t)(q0 + s + 1) * W + 8 * f8) : make_uint4(0, 0, 0, 0);
+    }
+  }
+  __device__ __forceinline__ void store(__bf16* dst) const {
+    uint32_t* d = reinterpret_cast<uint32_t*>(dst);
+#pragma unroll
+    for (int it = 0; it < kItems; ++it) {
+      const int i = threadIdx.x + 256 * it;
+      const int sp = i % kPairs, f8 = i / kPairs;
+      const uint32_t s = 2 * sp;
+      if (kTotal % 256 != 0 && i >= kTotal) break;
+      const uint32_t wa[4] = {a[it].x, a[it].y, a[it].z, a[it].w}, wc[4] = {c[it].x, c[it].y, c[it].z, c[it].w};
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const uint32_t lo = (wa[j >> 1] >> (16 * (j & 1))) & 0xffffu;
+        const uint32_t hi = (wc[j >> 1] >> (16 * (j & 1))) & 0xffffu;
+        d[((8 * f8 + j) * kDwStride + s) >> 1] = lo | (hi << 16);
+      }
+    }
+  }
+};
+
+// Block (p, layer) of k_train_dw for a layer of IN inputs and OUT outputs:
+// chunk c + P's loads are in flight while chunk c's MFMAs run.
+template <int IN, int OUT>
+__device__ __forceinline__ void dw_layer(const __bf16* __restrict__ xl, const __bf16* __restrict__ dzl, uint32_t np,
+                                         __bf16* sx, __bf16* sz, f32x4 (&acc)[kDwTiles]) {
+  const uint32_t nch = (np + kDwChunk - 1) / kDwChunk;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+  constexpr int kNot = OUT / 16, kTiles = (IN / 16) * kNot;
+  ChunkRegs<IN> rx;
+  ChunkRegs<OUT> rz;
+  uint32_t c = blockIdx.x;
+  if (c < nch) {
+    rx.load(xl, c * kDwChunk, np);
+    rz.load(dzl, c * kDwChunk, np);
+  }
+  for (; c < nch; c += gridDim.x) {
+    __syncthreads();  // the previous chunk's LDS reads are done
+    rx.store(sx);
+    rz.store(sz);
+    __syncthreads();
+    if (c + gridDim.x < nch) {
+      rx.load(xl, (c + gridDim.x) * kDwChunk, np);
+      rz.load(dzl, (c + gridDim.x) * kDwChunk, np);
+    }
+#pragma unroll
+    for (int k = 0; k < kDwTiles; ++k) {
+      const int t = wave + 4 * k;
+      if (t < kTiles) {
+        const int it = t / kNot, ot = t % kNot;
+        const __bf16* ra = sx + (16 * it + r) * kDwStride + 8 * g;
+        const __bf16* rb = sz + (16 * ot + r) * kDwStride + 8 * g;
+#pragma unroll
+        for (int kb = 0; kb < kDwChunk / 32; ++kb)
+          acc[k] = mfma(*(const bf16x8*)(ra + 32 * kb), *(const bf16x8*)(rb + 32 * kb), acc[k]);
+      }
     }
   }
 }
@@ -453,31 +543,17 @@ __global__ __launch_bounds__(256) void k_train_dw(const __bf16* __restrict__ act
   const int in = kIn[layer], out = kOut[layer];
   const __bf16* xl = acts + (layer == 0 ? 0 : (size_t)cap * (128 + (layer - 1) * 64));
   const __bf16* dzl = dz + dz_offset(layer, cap);
-  const uint32_t nch = (np + kDwChunk - 1) / kDwChunk;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
   const int not_ = out / 16, ntile = (in / 16) * not_;
   f32x4 acc[kDwTiles];
 #pragma unroll
   for (int k = 0; k < kDwTiles; ++k) acc[k] = f32x4{0, 0, 0, 0};
-  for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
-    const uint32_t q0 = c * kDwChunk;
-    __syncthreads();  // the previous chunk's LDS reads are done
-    stage_transposed(sx, xl, q0, np, in);
-    stage_transposed(sz, dzl, q0, np, out);
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < kDwTiles; ++k) {
-      const int t = wave + 4 * k;
-      if (t < ntile) {
-        const int it = t / not_, ot = t % not_;
-        const __bf16* ra = sx + (16 * it + r) * kDwStride + 8 * g;
-        const __bf16* rb = sz + (16 * ot + r) * kDwStride + 8 * g;
-#pragma unroll
-        for (int kb = 0; kb < kDwChunk / 32; ++kb)
-          acc[k] = mfma(*(const bf16x8*)(ra + 32 * kb), *(const bf16x8*)(rb + 32 * kb), acc[k]);
-      }
-    }
-  }
+  if (layer == 0)
+    dw_layer<128, 64>(xl, dzl, np, sx, sz, acc);
+  else if (layer == kLayers - 1)
+    dw_layer<64, 16>(xl, dzl, np, sx, sz, acc);
+  else
+    dw_layer<64, 64>(xl, dzl, np, sx, sz, acc);
   float* dst = partial + (size_t)gridDim.x * param_offset(layer) + (size_t)blockIdx.x * in * out;
 #pragma unroll
   for (int k = 0; k < kDwTiles; ++k) {
