@@ -309,6 +309,67 @@ def _at_time(curve, t):
     return None
 
 
+# Algorithmic FLOPs of the cache network (110 used inputs, 4 hidden 64x64
+# layers, 3 used outputs): forward per query; per training sample the forward,
+# dX back to the first hidden layer and dW of every layer.
+NRC_FLOP_QUERY = 2 * (110 * 64 + 4 * 64 * 64 + 64 * 3)  # 47,232
+NRC_FLOP_SAMPLE = NRC_FLOP_QUERY + 2 * (4 * 64 * 64 + 64 * 3) + NRC_FLOP_QUERY  # 127,616
+MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16
+
+
+def cache_mfma(ctx, lo, hi, args):
+    """Inference and training throughput of the cache kernels on device
+    arrays (synthetic queries in the scene box), timed with HIP events on
+    the stream the kernels run on; TFLOP/s from the algorithmic FLOPs."""
+    import torch
+    import pyre_amd
+    rng = np.random.default_rng(8)
+
+    def queries(n):
+        p = rng.uniform(lo, hi, (n, 3))
+        d = rng.normal(size=(n, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        m = rng.normal(size=(n, 3))
+        m /= np.linalg.norm(m, axis=1, keepdims=True)
+        return np.concatenate([p, d, m, rng.random((n, 4))], 1).astype(np.float32)
+
+    cache = pyre_amd.RadianceCache(ctx, lo, hi, seed=5)
+    stream = torch.cuda.Stream()
+    out = {"unit": "TFLOP/s", "peak": MFMA_BF16_PEAK_TFLOPS, "flop_per_query": NRC_FLOP_QUERY,
+           "flop_per_training_sample": NRC_FLOP_SAMPLE}
+
+    def timed(fn, k=20, warm=3):
+        for _ in range(warm):
+            fn()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(k)]
+        torch.cuda.synchronize()
+        for a, b in evs:
+            a.record(stream)
+            fn()
+            b.record(stream)
+        torch.cuda.synchronize()
+        return float(np.mean([a.elapsed_time(b) for a, b in evs]))
+
+    n = 1 << 20
+    with torch.cuda.stream(stream):
+        dq = torch.from_numpy(queries(n)).to("cuda")
+        dt = torch.from_numpy(rng.random((n, 3)).astype(np.float32)).to("cuda")
+        rgb = torch.empty((n, 3), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    ms = timed(lambda: cache.infer_dev(dq.data_ptr(), n, rgb.data_ptr(), stream.cuda_stream))
+    tf = n * NRC_FLOP_QUERY / (ms * 1e-3) / 1e12
+    out["infer"] = {"queries": n, "ms": round(ms, 4), "achieved": round(tf, 1), "frac": round(tf / MFMA_BF16_PEAK_TFLOPS, 4)}
+    for lg in (14, 18):
+        b = 1 << lg
+        ms = timed(lambda: cache.train_step_dev(dq.data_ptr(), dt.data_ptr(), b, 0, stream.cuda_stream))
+        tf = b * NRC_FLOP_SAMPLE / (ms * 1e-3) / 1e12
+        out[f"train_step_2e{lg}"] = {"samples": b, "ms": round(ms, 4), "achieved": round(tf, 1),
+                                     "frac": round(tf / MFMA_BF16_PEAK_TFLOPS, 4)}
+    cache.close()
+    log(f"cache MFMA: {out}")
+    return out
+
+
 def bench_nrc(args):
     """Seconds to relMSE 0.01 at 1080p on the production scene: the plain
     path tracer vs the radiance cache (paths ending in the cache after the
@@ -321,6 +382,9 @@ def bench_nrc(args):
     ended in it. Both use a pool of 2^25 paths (passes of up to 16 spp).
     relMSE against the FP64 reference is taken outside the timed region."""
     os.environ["PYRE_PASS_PATHS"] = str(NRC_POOL_PATHS)
+    # torch first: it and the library load a HIP runtime of the same soname,
+    # and the first one loaded serves both
+    import torch  # noqa: F401
     import pyre_amd
     ref = load_reference(args.reference)
     if ref is None:
@@ -370,6 +434,7 @@ def bench_nrc(args):
 
     plain = run("plain")
     nrc = run("nrc")
+    mfma = cache_mfma(ctx, lo, hi, args)
     tp, tn = _crossing(plain, target), _crossing(nrc, target)
     eq = _at_time(nrc, tp) if tp else None
     line = {
@@ -385,6 +450,7 @@ def bench_nrc(args):
         "plain_seconds": round(tp, 4) if tp else None, "nrc_seconds": round(tn, 4) if tn else None,
         "nrc_over_plain": round(tn / tp, 4) if (tn and tp) else None,
         "relmse_at_plain_seconds": {"plain": target, "nrc": round(eq, 5) if eq else None},
+        "cache_mfma": mfma,
         "plain_curve": [(round(t, 4), round(e, 5)) for t, e in plain],
         "nrc_curve": [(round(t, 4), round(e, 5)) for t, e in nrc], "path_stats_last": stats, "device": dev_name,
     }

@@ -321,6 +321,14 @@ int pyre_nrc_gradients(pyre_ctx* ctx, pyre_nrc* nrc, const float* feats, const f
 /* One Adam step on the batch; loss receives the pre-step loss. */
 int pyre_nrc_train_step(pyre_ctx* ctx, pyre_nrc* nrc, const float* feats, const float* targets, size_t n,
                         float* loss);
+/* Device-resident forms (no counterpart in the reference): feats, rgb,
+ * targets and loss are device arrays; the work is queued on stream
+ * (hipStream_t, or NULL for the context's stream) without synchronising.
+ * loss, when not NULL, has the batch's mean loss added to it. A batch larger
+ * than any before grows the training buffers (an allocation). */
+int pyre_nrc_infer_dev(pyre_ctx* ctx, pyre_nrc* nrc, const float* feats, size_t n, float* rgb, void* stream);
+int pyre_nrc_train_step_dev(pyre_ctx* ctx, pyre_nrc* nrc, const float* feats, const float* targets, size_t n,
+                            float* loss, void* stream);
 /* Adam steps taken so far and the loss of the last training batch. */
 int pyre_nrc_stats(pyre_ctx* ctx, pyre_nrc* nrc, uint64_t* steps, float* last_loss);
 

@@ -485,6 +485,17 @@ class RadianceCache:
                                                     C.byref(loss)), "pyre_nrc_train_step")
         return float(loss.value)
 
+    def infer_dev(self, feats_ptr: int, n: int, rgb_ptr: int, stream: int = 0):
+        """Inference on device arrays (13 floats per query in, RGB out), queued on stream."""
+        self.ctx.check(self.lib.pyre_nrc_infer_dev(self.ctx.h, self.h, C.c_void_p(feats_ptr), n,
+                                                   C.c_void_p(rgb_ptr), C.c_void_p(stream)), "pyre_nrc_infer_dev")
+
+    def train_step_dev(self, feats_ptr: int, targets_ptr: int, n: int, loss_ptr: int = 0, stream: int = 0):
+        """One Adam step on device arrays, queued on stream; adds the batch loss to *loss_ptr if given."""
+        self.ctx.check(self.lib.pyre_nrc_train_step_dev(self.ctx.h, self.h, C.c_void_p(feats_ptr),
+                                                        C.c_void_p(targets_ptr), n, C.c_void_p(loss_ptr or None),
+                                                        C.c_void_p(stream)), "pyre_nrc_train_step_dev")
+
     def stats(self):
         """(Adam steps taken, loss of the last training batch)."""
         steps, loss = C.c_uint64(), C.c_float()

@@ -160,6 +160,8 @@ SIGNATURES.update({
     "pyre_nrc_infer": (C.c_int, [_vp, _vp, _vp, C.c_size_t, _vp]),
     "pyre_nrc_gradients": (C.c_int, [_vp, _vp, _vp, _vp, C.c_size_t, _vp, C.POINTER(_f)]),
     "pyre_nrc_train_step": (C.c_int, [_vp, _vp, _vp, _vp, C.c_size_t, C.POINTER(_f)]),
+    "pyre_nrc_infer_dev": (C.c_int, [_vp, _vp, _vp, C.c_size_t, _vp, _vp]),
+    "pyre_nrc_train_step_dev": (C.c_int, [_vp, _vp, _vp, _vp, C.c_size_t, _vp, _vp]),
 })
 
 

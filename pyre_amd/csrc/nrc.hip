@@ -785,14 +785,14 @@ float nrc_gradients_dev(pyre_nrc* N, const float* feats, const float* targets, u
 }
 
 float nrc_train_step_dev(pyre_nrc* N, const float* feats, const float* targets, uint32_t n, hipStream_t st,
-                         float* loss_dev) {
+                         float* loss_dev, bool read_back) {
   const BatchN bn{nullptr, 0, 0, n};
   const unsigned nlp = train_pass(N, feats, targets, bn, N->d_step, st);
   const uint32_t np = (n + 15) / 16 * 16;
   hipLaunchKernelGGL(k_train_update, dim3(kParams / 64), dim3(256), 0, st, N->part, dw_partials(np), N->grad, N->w,
                      N->m, N->v, adam_args(N), (const uint32_t*)N->d_step, bn, N->wf, N->wb, N->loss_part, nlp,
                      loss_dev, (uint32_t*)nullptr);
-  return loss_dev ? 0.0f : read_loss(N, nlp, n, st);
+  return loss_dev || !read_back ? 0.0f : read_loss(N, nlp, n, st);
 }
 
 void nrc_train_step_counted(pyre_nrc* N, const float* feats, const float* targets, const uint32_t* count,
@@ -978,6 +978,23 @@ int pyre_nrc_stats(pyre_ctx* ctx, pyre_nrc* N, uint64_t* steps, float* last_loss
     PYRE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     if (steps) *steps = t;
     if (last_loss) *last_loss = N->last_loss;
+  });
+}
+
+int pyre_nrc_infer_dev(pyre_ctx* ctx, pyre_nrc* N, const float* feats, size_t n, float* rgb, void* stream) {
+  if (!ctx || !N || (n && (!feats || !rgb))) return PYRE_EINVAL;
+  return nrc_guard(ctx, [&] {
+    require(n < (1ull << 31), "too many queries");
+    if (n) nrc_infer_dev(N, feats, (uint32_t)n, rgb, stream ? (hipStream_t)stream : ctx->stream);
+  });
+}
+
+int pyre_nrc_train_step_dev(pyre_ctx* ctx, pyre_nrc* N, const float* feats, const float* targets, size_t n,
+                            float* loss, void* stream) {
+  if (!ctx || !N || !feats || !targets || !n) return PYRE_EINVAL;
+  return nrc_guard(ctx, [&] {
+    require(n < (1u << 26), "training batch too large");
+    nrc_train_step_dev(N, feats, targets, (uint32_t)n, stream ? (hipStream_t)stream : ctx->stream, loss, false);
   });
 }
 

@@ -20,9 +20,10 @@ float nrc_gradients_dev(pyre_nrc* nrc, const float* feats, const float* targets,
                         float* loss_dev = nullptr);
 // One Adam step (Kingma & Ba) on a batch: the gradients as above, then the
 // FP32 master weights, moments and bf16 fragments updated in one launch; the
-// loss is returned or added to *loss_dev as above.
+// loss is added to *loss_dev when given, else returned (read_back, which
+// synchronises) or not computed on the host.
 float nrc_train_step_dev(pyre_nrc* nrc, const float* feats, const float* targets, uint32_t n, hipStream_t st,
-                         float* loss_dev = nullptr);
+                         float* loss_dev = nullptr, bool read_back = true);
 // One Adam step queued before its batch size is known: the samples
 // [off, off + batch) of the min(*count, cap) in feats/targets, skipped on the
 // device when fewer than 16 remain. Adds the batch's mean loss to *loss_dev

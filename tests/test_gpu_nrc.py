@@ -100,3 +100,36 @@ def test_adam_step_and_training_reduces_loss(ctx):
     rel = np.mean((pred - t2) ** 2 / (pred ** 2 + 0.01))
     assert rel < 0.05
     c.close()
+
+
+def test_device_entry_points_match_host_ones(ctx):
+    """pyre_nrc_infer_dev / pyre_nrc_train_step_dev on device arrays and a
+    caller's stream give the host entry points' results bit for bit."""
+    import torch
+    a = pyre_amd.RadianceCache(ctx, BMIN, BMAX, seed=9)
+    b = pyre_amd.RadianceCache(ctx, BMIN, BMAX, seed=9)
+    rng = np.random.default_rng(6)
+    n = 3000
+    q = _queries(rng, n)
+    t = _target(q)
+    ref = a.infer(q)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        dq = torch.from_numpy(q).to("cuda")
+        dt = torch.from_numpy(t).to("cuda")
+        out = torch.empty((n, 3), dtype=torch.float32, device="cuda")
+        loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    a.infer_dev(dq.data_ptr(), n, out.data_ptr(), s.cuda_stream)
+    s.synchronize()
+    assert np.array_equal(out.cpu().numpy().view(np.uint32), ref.view(np.uint32))
+    for _ in range(3):
+        lb = b.train_step(q, t)
+        a.train_step_dev(dq.data_ptr(), dt.data_ptr(), n, loss.data_ptr(), s.cuda_stream)
+    s.synchronize()
+    assert np.array_equal(a.weights().view(np.uint32), b.weights().view(np.uint32))
+    assert a.stats()[0] == b.stats()[0] == 3
+    # loss: the sum of the three batch losses, accumulated on the device
+    assert np.isfinite(loss.item()) and loss.item() > 0
+    a.close()
+    b.close()
