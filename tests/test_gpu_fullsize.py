@@ -1,0 +1,104 @@
+"""The traversal benchmark workload at its full size: 2^20 random triangles
+(binned SAH, 32 bins, leaf <= 4) and 2^24 rays, 2^23 closest-hit and 2^23
+any-hit, on device arrays through pyre_trace_closest_dev / pyre_trace_any_dev.
+
+The oracle cannot trace 2^24 rays in seconds, so the full set is checked by
+properties that do not depend on size, and a slice of it against the oracle:
+  * any-hit on the closest-hit rays is set exactly where a closest hit exists
+    (scene.rs:102-140: occluded is intersect's existence question);
+  * no triangle lies strictly before the closest hit: any-hit on the same
+    rays with t_max lowered to the hit's t finds nothing;
+  * every hit t lies in (t_min, t_max);
+  * the first 2^13 closest-hit rays match the oracle bit for bit (id, t), rays
+    whose oracle hit lies within 1e-6 of a triangle edge excluded and counted,
+    and the first 2^13 any-hit rays agree with the oracle's occluded.
+"""
+import numpy as np
+import pytest
+
+import pyre_amd
+from tests.helpers import edge_distance
+
+pytestmark = pytest.mark.gpu
+
+N_TRIS, N_RAYS, N_CLOSEST = 1 << 20, 1 << 24, 1 << 23
+N_SPOT = 1 << 13
+
+
+def _ray36(r32):
+    """(o, t_min, d, t_max) device records -> (o, d, t_min, t_max, time) host rays."""
+    out = np.zeros((len(r32), 9), np.float32)
+    out[:, 0:3] = r32[:, 0:3]
+    out[:, 3:6] = r32[:, 4:7]
+    out[:, 6] = r32[:, 3]
+    out[:, 7] = r32[:, 7]
+    return out
+
+
+def test_benchmark_workload_full_size(ctx, oracle):
+    import torch
+    print(f"[full size] tris={N_TRIS} rays={N_RAYS}", flush=True)
+    tris = pyre_amd.random_triangles(1, N_TRIS, 0.01)
+    rays = pyre_amd.random_rays(2, N_RAYS, N_CLOSEST, "ray32")
+    pos = tris.reshape(-1, 3)
+    idx = np.arange(3 * N_TRIS, dtype=np.uint32).reshape(-1, 3)
+    sc = pyre_amd.Scene.from_arrays(ctx, [dict(positions=pos, indices=idx, material_id=0)],
+                                    [pyre_amd.Material(pyre_amd.MAT_LAMBERTIAN, (0.5, 0.5, 0.5))])
+    sc.build_bvh(bins=32, max_leaf=4)
+
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        d = torch.from_numpy(rays).to("cuda")
+        hits = torch.empty((N_CLOSEST, 2), dtype=torch.int32, device="cuda")
+        occ_c = torch.empty(N_CLOSEST, dtype=torch.int32, device="cuda")
+        occ_a = torch.empty(N_RAYS - N_CLOSEST, dtype=torch.int32, device="cuda")
+        occ_t = torch.empty(N_CLOSEST, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    closest, shadow = d[:N_CLOSEST], d[N_CLOSEST:]
+    sc.trace_closest_dev(closest.data_ptr(), N_CLOSEST, hits.data_ptr(), s.cuda_stream)
+    sc.trace_any_dev(closest.data_ptr(), N_CLOSEST, occ_c.data_ptr(), s.cuda_stream)
+    sc.trace_any_dev(shadow.data_ptr(), N_RAYS - N_CLOSEST, occ_a.data_ptr(), s.cuda_stream)
+    s.synchronize()
+
+    h = hits.cpu().numpy()
+    prim = h[:, 0].view(np.uint32)
+    t = h[:, 1].view(np.float32)
+    hit = prim != pyre_amd.MISS
+    print(f"  closest hits {hit.mean():.3f}, any-hit rate {occ_a.float().mean().item():.3f}", flush=True)
+    assert 0.05 < hit.mean() < 0.95
+    assert np.array_equal(occ_c.cpu().numpy().astype(bool), hit)
+    r = rays[:N_CLOSEST]
+    assert np.all(prim[hit] < N_TRIS)
+    assert np.all((t[hit] > r[hit, 3]) & (t[hit] < r[hit, 7]))
+
+    # nothing strictly before the closest hit
+    with torch.cuda.stream(s):
+        lowered = closest.clone()
+        th = torch.from_numpy(np.where(hit, t, r[:, 7]).astype(np.float32)).to("cuda")
+        lowered[:, 7] = th
+    s.synchronize()
+    sc.trace_any_dev(lowered.data_ptr(), N_CLOSEST, occ_t.data_ptr(), s.cuda_stream)
+    s.synchronize()
+    assert not occ_t.cpu().numpy().any()
+
+    # a slice against the oracle
+    o_sc = oracle.Scene()
+    o_sc.add_material(oracle.material(0, (0.5, 0.5, 0.5)))
+    o_sc.add_mesh(pos, idx, None, 0, 32, 4)
+    o = o_sc.intersect(_ray36(rays[:N_SPOT]))
+    o_hit = o["kind"] == 1
+    t3 = tris.reshape(-1, 3, 3)
+    hi = np.where(o_hit)[0]
+    ed = edge_distance(t3[o["tri"][hi], 0], t3[o["tri"][hi], 1], t3[o["tri"][hi], 2], o["u"][hi], o["v"][hi])
+    keep = np.ones(N_SPOT, bool)
+    keep[hi[ed < 1e-6]] = False
+    print(f"  spot check: {N_SPOT} rays, {o_hit.sum()} hits, {(~keep).sum()} excluded near an edge", flush=True)
+    assert (~keep).sum() < N_SPOT // 1000 + 5
+    assert np.array_equal(hit[:N_SPOT][keep], o_hit[keep])
+    both = keep & o_hit
+    assert np.array_equal(prim[:N_SPOT][both], o["tri"][both].astype(np.uint32))
+    assert np.array_equal(t[:N_SPOT][both].view(np.uint32), o["t"][both].view(np.uint32))
+    oa = o_sc.any_hit(_ray36(rays[N_CLOSEST:N_CLOSEST + N_SPOT]))
+    ga = occ_a[:N_SPOT].cpu().numpy().astype(bool)
+    assert (ga != oa.astype(bool)).sum() <= 2
+    sc.close()
