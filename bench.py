@@ -224,8 +224,27 @@ def bench_production(args):
         line["relmse_64spp"] = round(rel_mse(r64 / spp, ref), 5)
         line["relmse_eps"] = REL_EPS
     if not args.no_cpu:
-        line["cpu_baseline"] = production_cpu_baseline(pre, cam, cspp=2)
+        line["cpu_baseline"], osc, ocam = production_cpu_baseline(pre, cam, cspp=2)
+        line["traversal_roofline"] = production_traversal_roofline(osc, ocam, ext, sh, per)
     print(json.dumps(line), flush=True)
+
+
+def production_traversal_roofline(osc, ocam, ext, sh, per):
+    """C1's byte model on the production frame: per query, 32 B of ray in, 8 B
+    (closest) or 4 B (shadow) out, 32 B per node fetch and 36 B per triangle
+    test of the reference traversal (Scene::intersect / occluded over every
+    primitive's BVH, bvh.rs:86-143, scene.rs:57-140), counted by the oracle on
+    every 8th pixel in x and y at 1 spp. The frame time bounds the traversal
+    kernels' time from above, so the fraction is a lower bound."""
+    st = osc.render_stats(ocam, PROD_W, PROD_H, spp=1, stride=8, max_depth=12, min_rr=3)
+    b_ext = 32 + 8 + 32 * st["n_node_extension"] + 36 * st["n_tri_extension"]
+    b_sh = 32 + 4 + 32 * st["n_node_shadow"] + 36 * st["n_tri_shadow"]
+    gbs = (ext * b_ext + sh * b_sh) / per / 1e9
+    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
+            "frac_lower_bound": round(gbs / 8000.0, 4), "time": "whole frame",
+            "bytes_per_extension_ray": round(b_ext, 1), "bytes_per_shadow_ray": round(b_sh, 1),
+            "sample": {k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()},
+            "sample_pixels": "every 8th pixel in x and y (240 x 135), 1 spp"}
 
 
 def production_cpu_baseline(pre, cam, cspp):
@@ -249,7 +268,7 @@ def production_cpu_baseline(pre, cam, cspp):
     dt = time.perf_counter() - t0
     return {"value": round(PROD_W * PROD_H * cspp / dt / 1e6, 4), "unit": "Msamples/s", "cores": threads,
             "kind": "port", "sample": f"full 1920x1080 frame at {cspp} spp (of 64); per-mesh BVH build "
-                                      f"{build_s:.0f} s untimed"}
+                                      f"{build_s:.0f} s untimed"}, osc, ocam
 
 
 def make_reference(args):

@@ -93,6 +93,8 @@ def lib():
             "orc_camera_rays": (None, [_f32p, _f32p, _u32, _f32p]),
             "orc_auto_frame": (None, [_f32p, C.c_float, C.c_float, _f32p, _f32p]),
             "orc_render": (None, [_vp, _f32p, _u32, _u32, _u32, _u32, _u32, _u32, _u32, _u32, _u32, _f32p, _vp]),
+            "orc_render_stats": (None, [_vp, _f32p, _u32, _u32, _u32, _u32, _u32, _u32, _u32, _u32,
+                                        C.POINTER(C.c_uint64)]),
             "orc_encode_png": (None, [_f32p, _u32, _u32, _u32, _u8p]),
             "orc_frame": (None, [_f32p, _u32, _f32p]),
         }
@@ -463,6 +465,19 @@ class Scene:
         lib().orc_render(self.h, f32(cam13), width, height, spp, spp_offset, max_depth, min_rr, y0, y1,
                          threads or nthreads(), out, ps.ctypes.data if per_sample else None)
         return (out, ps) if per_sample else out
+
+
+    def render_stats(self, cam13, width, height, spp=1, stride=8, max_depth=8, min_rr=3, threads=None):
+        """Reference traversal work per scene query in a render of every
+        stride-th pixel in x and y: per extension and per shadow query, the
+        mean node fetches and triangle tests (bvh.rs:86-143 over every
+        primitive, scene.rs:57-140)."""
+        out = (C.c_uint64 * 6)()
+        lib().orc_render_stats(self.h, f32(cam13), width, height, spp, 0, max_depth, min_rr, stride,
+                               threads or nthreads(), out)
+        ne, nn_e, nt_e, ns, nn_s, nt_s = (int(v) for v in out)
+        return {"extension_queries": ne, "n_node_extension": nn_e / max(ne, 1), "n_tri_extension": nt_e / max(ne, 1),
+                "shadow_queries": ns, "n_node_shadow": nn_s / max(ns, 1), "n_tri_shadow": nt_s / max(ns, 1)}
 
 
 def camera(origin, target, up=(0, 1, 0), vfov=45.0, aspect=1.0, aperture=0.0, focus=None):

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <mutex>
 #include <cstdio>
 #include <cstring>
 #include <thread>
@@ -754,14 +755,18 @@ Ray Camera::generate_ray(V2 ndc, V2 lens, float time) const {
 Scene::~Scene() { delete env; }
 
 // scene.rs:57-97. Returns 0 miss, 1 surface (id = primitive), 2 light (id = light).
+thread_local RenderStats* g_render_stats = nullptr;
+
 int Scene::intersect(const Ray& ray, SurfaceInteraction* it_out, uint32_t* id_out) const {
   int kind = 0;
   float t_max = ray.t_max;
+  TraversalStats* st = g_render_stats ? &g_render_stats->ext_st : nullptr;
+  if (g_render_stats) g_render_stats->ext++;
   for (size_t id = 0; id < prims.size(); ++id) {
     Ray r = ray;
     r.t_max = t_max;
     SurfaceInteraction it;
-    if (prims[id].intersect(r, &it, nullptr)) {
+    if (prims[id].intersect(r, &it, st)) {
       if (it.t < t_max) {
         t_max = it.t;
         *it_out = it;
@@ -792,8 +797,10 @@ int Scene::intersect(const Ray& ray, SurfaceInteraction* it_out, uint32_t* id_ou
 bool Scene::occluded_dir(V3 from, V3 dir, float time) const {  // scene.rs:102-116
   Ray ray{from, dir, 1e-3f, INFINITY, time};
   SurfaceInteraction it;
+  TraversalStats* st = g_render_stats ? &g_render_stats->shadow_st : nullptr;
+  if (g_render_stats) g_render_stats->shadow++;
   for (const auto& p : prims)
-    if (p.intersect(ray, &it, nullptr)) return true;
+    if (p.intersect(ray, &it, st)) return true;
   return false;
 }
 
@@ -804,8 +811,10 @@ bool Scene::occluded(V3 from, V3 to, float time) const {  // scene.rs:120-140
   V3 dir = to_from / dist;
   Ray ray{from, dir, 1e-3f, dist - 1e-3f, time};
   SurfaceInteraction it;
+  TraversalStats* st = g_render_stats ? &g_render_stats->shadow_st : nullptr;
+  if (g_render_stats) g_render_stats->shadow++;
   for (const auto& p : prims)
-    if (p.intersect(ray, &it, nullptr)) return true;
+    if (p.intersect(ray, &it, st)) return true;
   return false;
 }
 
@@ -1498,6 +1507,43 @@ void orc_render(void* s, const float* cam, uint32_t width, uint32_t height, uint
       }
     }
   });
+}
+
+// The reference traversal's work in a render of the pixels (x, y) with
+// x % stride == 0 and y % stride == 0, spp samples each: out = {extension
+// queries, their node fetches, their triangle tests, shadow queries, their
+// node fetches, their triangle tests}.
+void orc_render_stats(void* s, const float* cam, uint32_t width, uint32_t height, uint32_t spp, uint32_t spp_offset,
+                      uint32_t max_depth, uint32_t min_rr, uint32_t stride, uint32_t threads, uint64_t* out) {
+  const Scene& sc = *(Scene*)s;
+  Camera c = cam_from(cam);
+  Render R{width, height, spp, spp_offset, max_depth, min_rr};
+  const uint32_t rows = (height + stride - 1) / stride;
+  std::mutex mu;
+  RenderStats total;
+  parallel_for(rows, threads, 1, [&](size_t a, size_t b) {
+    RenderStats local;
+    g_render_stats = &local;
+    for (size_t ry = a; ry < b; ++ry) {
+      const uint32_t y = (uint32_t)ry * stride;
+      for (uint32_t x = 0; x < width; x += stride)
+        for (uint32_t si = 0; si < spp; ++si) (void)render_one(sc, c, R, x, y, spp_offset + si);
+    }
+    g_render_stats = nullptr;
+    std::lock_guard<std::mutex> lk(mu);
+    total.ext += local.ext;
+    total.shadow += local.shadow;
+    total.ext_st.nodes += local.ext_st.nodes;
+    total.ext_st.tris += local.ext_st.tris;
+    total.shadow_st.nodes += local.shadow_st.nodes;
+    total.shadow_st.tris += local.shadow_st.tris;
+  });
+  out[0] = total.ext;
+  out[1] = total.ext_st.nodes;
+  out[2] = total.ext_st.tris;
+  out[3] = total.shadow;
+  out[4] = total.shadow_st.nodes;
+  out[5] = total.shadow_st.tris;
 }
 
 // film.rs:181-198 (beauty), :125-134 (albedo), :136-147 (normal), :149-173 (depth)

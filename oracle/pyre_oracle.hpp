@@ -196,6 +196,15 @@ struct TraversalStats {
   uint64_t tris = 0;   // Moller-Trumbore tests
 };
 
+// Work of the reference traversal counted per scene query while a render
+// runs with statistics (orc_render_stats): extension queries
+// (Scene::intersect) and shadow queries (Scene::occluded / occluded_dir).
+struct RenderStats {
+  uint64_t ext = 0, shadow = 0;
+  TraversalStats ext_st, shadow_st;
+};
+extern thread_local RenderStats* g_render_stats;
+
 struct Bvh {
   std::vector<BvhNode> nodes;
   std::vector<uint32_t> prim;

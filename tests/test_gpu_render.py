@@ -88,3 +88,21 @@ def test_production_scene_small(ctx, oracle):
     print(f"  albedo equal={same_alb:.4f} relMSE(gpu,cpu)={e2:.3e}")
     assert same_alb > 0.95
     assert e2 < 5e-3
+
+
+def test_ray_counts_match_reference_queries(ctx, oracle):
+    """The device's extension and shadow ray counts are the reference's
+    Scene::intersect and Scene::occluded calls for the same samples (paths of
+    the same streams take the same bounces; a few may part on a device
+    sin/cos ulp, so the counts agree to 0.5 %). These counts are the units of
+    the traversal byte model."""
+    w, h, spp = 48, 48, 8
+    pre, sc = _cornell(ctx)
+    cam = pre.camera(1.0)
+    pyre_amd.Film(w, h).render(sc, cam, pyre_amd.PathIntegrator(8, 3), spp)
+    ext, sh = sc.render_stats()
+    st = oracle.Scene.cornell().render_stats(_ocam(oracle, cam), w, h, spp=spp, stride=1, max_depth=8, min_rr=3)
+    print(f"  gpu ext={ext} shadow={sh}; oracle {st}", flush=True)
+    assert abs(ext - st["extension_queries"]) <= 0.005 * st["extension_queries"]
+    assert abs(sh - st["shadow_queries"]) <= 0.005 * st["shadow_queries"]
+    assert st["n_node_extension"] > 1 and st["n_tri_extension"] > 0
