@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import pyre_amd
-from tests.helpers import rel_mse
+from tests.helpers import env_override, rel_mse
 
 pytestmark = pytest.mark.gpu
 
@@ -106,3 +106,18 @@ def test_ray_counts_match_reference_queries(ctx, oracle):
     assert abs(ext - st["extension_queries"]) <= 0.005 * st["extension_queries"]
     assert abs(sh - st["shadow_queries"]) <= 0.005 * st["shadow_queries"]
     assert st["n_node_extension"] > 1 and st["n_tri_extension"] > 0
+
+
+@pytest.mark.parametrize("switch", [dict(PYRE_SHADE_WAVES=4), dict(PYRE_STACK_RING=4, PYRE_TRACE_BLOCKS_PER_CU=4),
+                                    dict(PYRE_STACK_RING=16), dict(PYRE_PASS_PATHS=5000)])
+def test_render_tuning_switches_same_bits(ctx, switch):
+    """Tuning switches (the shade kernel built for 4 waves per SIMD, stack
+    rings, traversal grid, pool size) change speed only: the film is
+    bit-identical to the default render."""
+    pre, sc = _cornell(ctx)
+    cam = pre.camera(1.0)
+    ip = pyre_amd.PathIntegrator(8, 3)
+    ref = pyre_amd.Film(32, 32).render(sc, cam, ip, 6).copy()
+    with env_override(**switch):
+        got = pyre_amd.Film(32, 32).render(sc, cam, ip, 6)
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))

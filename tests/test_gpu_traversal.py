@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import pyre_amd
-from tests.helpers import canonical_gpu_tree, canonical_oracle_tree, edge_distance, tree_stats
+from tests.helpers import canonical_gpu_tree, canonical_oracle_tree, edge_distance, env_override, tree_stats
 
 pytestmark = pytest.mark.gpu
 
@@ -161,3 +161,24 @@ def test_degenerate_coincident_triangles(ctx, oracle):
     h = sc.intersect(r)
     assert h["inst"][0] == 0 and h["tri"][0] == 0  # exact tie -> lowest triangle id
     assert h["t"][0] == np.float32(1.0)
+
+
+@pytest.mark.parametrize("switch", [dict(PYRE_TRAVERSAL="binary"), dict(PYRE_TRACE_FETCH="static"),
+                                    dict(PYRE_STACK_RING=4), dict(PYRE_STACK_RING=16),
+                                    dict(PYRE_TRACE_BLOCKS_PER_CU=2)])
+def test_traversal_variants_bit_exact(ctx, oracle, switch):
+    """Every selectable traversal form (binary tree with an LDS stack;
+    grid-stride persistent; LDS stack rings of 4 and 16; a small persistent
+    grid) gives the reference's hits bit for bit, closest and any."""
+    print(f"[traversal variant] {switch}", flush=True)
+    with env_override(**switch):
+        excluded, hits = _closest_parity(ctx, oracle, 30000, 20000, 32, 4, 41)
+        assert hits > 1000 and excluded < 25
+        tris = pyre_amd.random_triangles(43, 30000, 0.01)
+        rays = pyre_amd.random_rays(44, 20000, 0)
+        sc, pos, idx = _soup_scene(ctx, tris)
+        sc.build_bvh(bins=32, max_leaf=4)
+        g = sc.occluded(rays).astype(bool)
+        c = sc.intersect(rays)
+        assert np.array_equal(g, c["inst"] != pyre_amd.MISS)
+        sc.close()
