@@ -239,25 +239,15 @@ struct Trav4 {
     const Node4 nd = nodes[cur];
     // missed children: key +inf and count -1 (a hit child can have key +inf
     // only if its box is at infinity, so the count marks misses)
-    float k0 = kInf, k1 = kInf, k2 = kInf, k3 = kInf;
-    int c0 = -1, c1 = -1, c2 = -1, c3 = -1;
-    float e;
-    if (nd.cnt.x >= 0 && slab_test(r, nd.lox.x, nd.hix.x, nd.loy.x, nd.hiy.x, nd.loz.x, nd.hiz.x, best_t, &e)) {
-      k0 = e;
-      c0 = nd.cnt.x;
-    }
-    if (nd.cnt.y >= 0 && slab_test(r, nd.lox.y, nd.hix.y, nd.loy.y, nd.hiy.y, nd.loz.y, nd.hiz.y, best_t, &e)) {
-      k1 = e;
-      c1 = nd.cnt.y;
-    }
-    if (nd.cnt.z >= 0 && slab_test(r, nd.lox.z, nd.hix.z, nd.loy.z, nd.hiy.z, nd.loz.z, nd.hiz.z, best_t, &e)) {
-      k2 = e;
-      c2 = nd.cnt.z;
-    }
-    if (nd.cnt.w >= 0 && slab_test(r, nd.lox.w, nd.hix.w, nd.loy.w, nd.hiy.w, nd.loz.w, nd.hiz.w, best_t, &e)) {
-      k3 = e;
-      c3 = nd.cnt.w;
-    }
+    // all four slab tests without branches (a missing child's box is
+    // tested too and masked out), so the lanes of a wave stay converged
+    float e0, e1, e2, e3;
+    const bool h0 = (nd.cnt.x >= 0) & slab_test(r, nd.lox.x, nd.hix.x, nd.loy.x, nd.hiy.x, nd.loz.x, nd.hiz.x, best_t, &e0);
+    const bool h1 = (nd.cnt.y >= 0) & slab_test(r, nd.lox.y, nd.hix.y, nd.loy.y, nd.hiy.y, nd.loz.y, nd.hiz.y, best_t, &e1);
+    const bool h2 = (nd.cnt.z >= 0) & slab_test(r, nd.lox.z, nd.hix.z, nd.loy.z, nd.hiy.z, nd.loz.z, nd.hiz.z, best_t, &e2);
+    const bool h3 = (nd.cnt.w >= 0) & slab_test(r, nd.lox.w, nd.hix.w, nd.loy.w, nd.hiy.w, nd.loz.w, nd.hiz.w, best_t, &e3);
+    float k0 = h0 ? e0 : kInf, k1 = h1 ? e1 : kInf, k2 = h2 ? e2 : kInf, k3 = h3 ? e3 : kInf;
+    int c0 = h0 ? nd.cnt.x : -1, c1 = h1 ? nd.cnt.y : -1, c2 = h2 ? nd.cnt.z : -1, c3 = h3 ? nd.cnt.w : -1;
     int32_t r0 = nd.ref.x, r1 = nd.ref.y, r2 = nd.ref.z, r3 = nd.ref.w;
     // sort the four children by entry distance (misses sort last as +inf)
     cswap(k0, r0, c0, k1, r1, c1);
