@@ -156,20 +156,21 @@ PH V2 rng_vec2(Rng& r) {
 // subtractions are the same f32 operations as mesh.rs:154-155, so the result
 // is bit-identical to moller_trumbore (mesh.rs:153-178). Accepts
 // t in (t_min, t_hi] when inclusive_hi, else (t_min, t_hi).
+// Every quantity is computed and the reference's rejection tests are
+// combined at the end (no early returns): a wave's lanes stay converged, and
+// an accepted hit's t/u/v come from the same operations in the same order.
 PH bool mt_intersect(V3 o, V3 d, V3 v0, V3 e1, V3 e2, float t_min, float t_hi, bool inclusive_hi, float* t_out,
                      float* u_out, float* v_out) {
   V3 h = cross(d, e2);
   float a = dot(e1, h);
-  if (fabsf(a) < 1e-8f) return false;
   float f = 1.0f / a;
   V3 s = o - v0;
   float u = f * dot(s, h);
-  if (!(u >= 0.0f && u <= 1.0f)) return false;
   V3 q = cross(s, e1);
   float v = f * dot(d, q);
-  if (v < 0.0f || u + v > 1.0f) return false;
   float t = f * dot(e2, q);
-  bool ok = t > t_min && (inclusive_hi ? t <= t_hi : t < t_hi);
+  const bool ok = !(fabsf(a) < 1e-8f) & (u >= 0.0f) & (u <= 1.0f) & !(v < 0.0f) & !(u + v > 1.0f) & (t > t_min) &
+                  (inclusive_hi ? t <= t_hi : t < t_hi);
   if (!ok) return false;
   *t_out = t;
   *u_out = u;
