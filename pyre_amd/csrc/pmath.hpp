@@ -171,11 +171,10 @@ PH bool mt_intersect(V3 o, V3 d, V3 v0, V3 e1, V3 e2, float t_min, float t_hi, b
   float t = f * dot(e2, q);
   const bool ok = !(fabsf(a) < 1e-8f) & (u >= 0.0f) & (u <= 1.0f) & !(v < 0.0f) & !(u + v > 1.0f) & (t > t_min) &
                   (inclusive_hi ? t <= t_hi : t < t_hi);
-  if (!ok) return false;
-  *t_out = t;
+  *t_out = t;  // meaningful only when the test passes
   *u_out = u;
   *v_out = v;
-  return true;
+  return ok;
 }
 
 }  // namespace pyre

@@ -178,16 +178,17 @@ __device__ __forceinline__ bool leaf_hits(const float4* __restrict__ tris, const
         found = true;
         return true;
       }
-    } else if (mt_intersect(r.o, r.d, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), r.t_min, best_t, true,
-                            &t, &u, &v)) {
-      uint32_t gid = __builtin_bit_cast(uint32_t, a.w);
-      if (t < best_t || (found && gid < best_gid)) {
-        best_t = t;
-        best_gid = gid;
-        bu = u;
-        bv = v;
-        found = true;
-      }
+    } else {
+      // the closest-hit update as selects (no branch on the test's outcome)
+      const bool ok = mt_intersect(r.o, r.d, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), r.t_min, best_t,
+                                   true, &t, &u, &v);
+      const uint32_t gid = __builtin_bit_cast(uint32_t, a.w);
+      const bool take = ok && (t < best_t || (found && gid < best_gid));
+      best_t = take ? t : best_t;
+      best_gid = take ? gid : best_gid;
+      bu = take ? u : bu;
+      bv = take ? v : bv;
+      found = found || take;
     }
   }
   return false;
