@@ -1336,11 +1336,13 @@ struct Batch {
   size_t n;
   PassShape shape;
   uint32_t slots = 1, max_npix = 0;
-  std::vector<float*> acc;
+  float* acc_all = nullptr;     // every job's film sums, job after job
+  double* acc64_all = nullptr;  // every job's FP64 radiance sums
+  std::vector<float*> acc;      // job j's part of acc_all
   std::vector<double*> acc64;
   ~Batch() {
-    for (auto& p : acc) dev_free(p);
-    for (auto& p : acc64) dev_free(p);
+    dev_free(acc_all);
+    dev_free(acc64_all);
   }
 };
 
@@ -1354,8 +1356,15 @@ float run_batch(Batch& B, const Tuning& t, bool accumulate, uint32_t max_passes,
   const uint32_t pool = std::max(t.pool_paths, B.max_npix);
   Schedule sc = make_schedule(B.jobs, B.n, pool);
   const uint32_t np = std::min<uint32_t>((uint32_t)sc.passes.size(), max_passes);
+  // the context's cached path buffers, when they are large enough (one-shot
+  // renders and batches leave theirs there)
   Work W;
-  W.alloc(pool, B.slots, 0, 0, 0, false, B.n, std::max<size_t>(1, sc.slices.size()), std::max<size_t>(1, sc.passes.size()));
+  if (ctx->render_cache) {
+    W = std::move(*static_cast<Work*>(ctx->render_cache.get()));
+    ctx->render_cache.reset();
+  }
+  const size_t n_sl = std::max<size_t>(1, sc.slices.size()), n_pa = std::max<size_t>(1, sc.passes.size());
+  if (!W.fits(pool, B.slots, 0, 0, 0, false, B.n, n_sl, n_pa)) W.alloc(pool, B.slots, 0, 0, 0, false, B.n, n_sl, n_pa);
   W.ensure_gstack((size_t)trace_grid_of(ctx, t) * kThreads, B.shape.tree_depth);
   std::vector<JobDev> jd(B.n);
   for (size_t j = 0; j < B.n; ++j)
@@ -1402,6 +1411,9 @@ float run_batch(Batch& B, const Tuning& t, bool accumulate, uint32_t max_passes,
   (void)hipEventDestroy(e1);
   if (ext) read_stats(ctx, W, ext, shadow);
   if (n_passes) *n_passes = np;
+  auto keep = std::make_shared<Work>();
+  *keep = std::move(W);
+  ctx->render_cache = std::static_pointer_cast<void>(keep);
   return ms;
 }
 
@@ -1501,14 +1513,19 @@ int pyre_render_batch(pyre_ctx* ctx, const pyre_job* jobs, size_t n, const pyre_
       B.shape.nee = B.shape.nee || sl > 0;
       B.slots = std::max(B.slots, sl);
     }
+    // one allocation for every job's sums
+    std::vector<size_t> off(n + 1, 0);
+    for (size_t j = 0; j < n; ++j) off[j + 1] = off[j] + (size_t)jobs[j].width * jobs[j].height;
+    B.acc_all = dev_alloc<float>(10 * off[n]);
+    B.acc64_all = dev_alloc<double>(3 * off[n]);
+    clear_sums(ctx, B.acc_all, B.acc64_all, off[n]);
     B.acc.assign(n, nullptr);
     B.acc64.assign(n, nullptr);
     hipStream_t st = ctx->stream;
     for (size_t j = 0; j < n; ++j) {
-      const size_t npix = (size_t)jobs[j].width * jobs[j].height;
-      B.acc[j] = dev_alloc<float>(10 * npix);
-      B.acc64[j] = dev_alloc<double>(3 * npix);
-      clear_sums(ctx, B.acc[j], B.acc64[j], npix);
+      const size_t npix = off[j + 1] - off[j];
+      B.acc[j] = B.acc_all + 10 * off[j];
+      B.acc64[j] = B.acc64_all + 3 * off[j];
       if (jobs[j].accum64)
         PYRE_HIP_CHECK(hipMemcpyAsync(B.acc64[j], jobs[j].accum64, 3 * npix * sizeof(double), hipMemcpyHostToDevice, st));
     }
@@ -1522,9 +1539,15 @@ int pyre_render_batch(pyre_ctx* ctx, const pyre_job* jobs, size_t n, const pyre_
     uint64_t ext = 0, sh = 0;
     uint32_t np = 0;
     float ms = run_batch(B, t, true, 0xFFFFFFFFu, &ext, &sh, &np);
+    // means (the sums scaled in place: the batch's sums are not kept) and
+    // FP64 sums, copied out with one synchronisation at the end
     for (size_t j = 0; j < n; ++j) {
-      const size_t npix = (size_t)jobs[j].width * jobs[j].height;
-      if (jobs[j].out) read_mean(ctx, B.acc[j], npix, jobs[j].out, 1.0f / (float)jobs[j].spp);
+      const size_t npix = off[j + 1] - off[j];
+      if (jobs[j].out) {
+        hipLaunchKernelGGL(k_scale, dim3(nblocks(10 * npix, 256)), dim3(256), 0, st, B.acc[j], 10 * npix,
+                           1.0f / (float)jobs[j].spp, B.acc[j]);
+        PYRE_HIP_CHECK(hipMemcpyAsync(jobs[j].out, B.acc[j], 10 * npix * sizeof(float), hipMemcpyDeviceToHost, st));
+      }
       if (jobs[j].accum64)
         PYRE_HIP_CHECK(hipMemcpyAsync(jobs[j].accum64, B.acc64[j], 3 * npix * sizeof(double), hipMemcpyDeviceToHost, st));
     }
