@@ -275,24 +275,27 @@ __global__ __launch_bounds__(kThreads) void k_generate(Paths P, Sched sc) {
 // Closest hit for every queued path, lanes refilled from the queue as their
 // rays finish.
 template <int RING>
-__global__ __launch_bounds__(kThreads) void k_extend(Paths P, const JobDev* __restrict__ jobs, int2* gstack) {
+__global__ __launch_bounds__(kThreads) void k_extend(Paths P, const JobDev* __restrict__ jobs, int2* gstack,
+                                                   bool one_job) {
   __shared__ int2 ring[RING * kThreads];
   const uint32_t tid = blockIdx.x * kThreads + threadIdx.x;
   HybridStack<RING> st{ring + threadIdx.x, kThreads, gstack + tid, gridDim.x * kThreads, 0, 0};
   WaveFeed feed(P.counters[0], P.counters + 8);
   Trav4<false, RING> tv;
   uint32_t p = 0, item = 0;
-  const Node4* nodes = nullptr;
-  const float4* tris = nullptr;
+  const Node4* nodes = one_job ? jobs[0].S.nodes4 : nullptr;
+  const float4* tris = one_job ? jobs[0].S.tris : nullptr;
   bool active = false;
   for (;;) {
     const uint64_t idle = __ballot(!active);
     if (__popcll(idle) >= kRefillLanes) {
       if (feed.take(idle, !active, item)) {
         p = P.q_cur[item];
-        const JobDev& J = jobs[P.pjob[p]];
-        nodes = J.S.nodes4;
-        tris = J.S.tris;
+        if (!one_job) {
+          const JobDev& J = jobs[P.pjob[p]];
+          nodes = J.S.nodes4;
+          tris = J.S.tris;
+        }
         const float4 a = P.ray[2 * (size_t)p], b = P.ray[2 * (size_t)p + 1];
         tv.start(make_tray(xyz(a), xyz(b), a.w, b.w), st);
         active = true;
@@ -312,7 +315,8 @@ __global__ __launch_bounds__(kThreads) void k_extend(Paths P, const JobDev* __re
 // until all its slots are traced so unoccluded contributions are added to the
 // path radiance in slot order (lights, then environment).
 template <int RING>
-__global__ __launch_bounds__(kThreads) void k_shadow(Paths P, const JobDev* __restrict__ jobs, int2* gstack) {
+__global__ __launch_bounds__(kThreads) void k_shadow(Paths P, const JobDev* __restrict__ jobs, int2* gstack,
+                                                   bool one_job) {
   __shared__ int2 ring[RING * kThreads];
   const uint32_t tid = blockIdx.x * kThreads + threadIdx.x;
   HybridStack<RING> st{ring + threadIdx.x, kThreads, gstack + tid, gridDim.x * kThreads, 0, 0};
@@ -320,8 +324,8 @@ __global__ __launch_bounds__(kThreads) void k_shadow(Paths P, const JobDev* __re
   Trav4<true, RING> tv;
   tv.cur = -1;  // -1: no traversal in progress for the current slot
   uint32_t p = 0, k = 0, cnt = 0, item = 0;
-  const Node4* nodes = nullptr;
-  const float4* tris = nullptr;
+  const Node4* nodes = one_job ? jobs[0].S.nodes4 : nullptr;
+  const float4* tris = one_job ? jobs[0].S.tris : nullptr;
   float4 L = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   bool active = false;
   for (;;) {
@@ -329,13 +333,18 @@ __global__ __launch_bounds__(kThreads) void k_shadow(Paths P, const JobDev* __re
     if (__popcll(idle) >= kRefillLanes) {
       if (feed.take(idle, !active, item)) {
         p = P.q_shadow[item];
-        const JobDev& J = jobs[P.pjob[p]];
-        nodes = J.S.nodes4;
-        tris = J.S.tris;
+        if (!one_job) {
+          const JobDev& J = jobs[P.pjob[p]];
+          nodes = J.S.nodes4;
+          tris = J.S.tris;
+        }
         cnt = P.scount[p];
         k = 0;
         L = P.lum[p];
         tv.cur = -1;
+        // the first slot's ray with the path's data (one memory round trip)
+        const float4 a = P.sray[2 * (size_t)p * P.slots], b = P.sray[2 * (size_t)p * P.slots + 1];
+        if (cnt > 0 && b.w > a.w) tv.start(make_tray(xyz(a), xyz(b), a.w, b.w), st);
         active = true;
       }
       if (!__any(active)) break;
@@ -1002,22 +1011,28 @@ unsigned trace_grid_of(const pyre_ctx* ctx, const Tuning& t) {
 struct PassShape {
   uint32_t max_depth = 0, tree_depth = 0;
   bool nee = false;
+  bool one_job = false;  // every path belongs to job 0 (sessions): its tree is read once per kernel
 };
 
 template <int RING>
-void launch_trace_stages(const Work& W, Paths P, unsigned tgrid, bool shadow, hipStream_t st, bool extend) {
-  if (extend) hipLaunchKernelGGL((k_extend<RING>), dim3(tgrid), dim3(kThreads), 0, st, P, (const JobDev*)W.jobs, W.gstack);
-  if (shadow) hipLaunchKernelGGL((k_shadow<RING>), dim3(tgrid), dim3(kThreads), 0, st, P, (const JobDev*)W.jobs, W.gstack);
+void launch_trace_stages(const Work& W, Paths P, unsigned tgrid, bool shadow, hipStream_t st, bool extend,
+                         bool one_job) {
+  if (extend)
+    hipLaunchKernelGGL((k_extend<RING>), dim3(tgrid), dim3(kThreads), 0, st, P, (const JobDev*)W.jobs, W.gstack,
+                       one_job);
+  if (shadow)
+    hipLaunchKernelGGL((k_shadow<RING>), dim3(tgrid), dim3(kThreads), 0, st, P, (const JobDev*)W.jobs, W.gstack,
+                       one_job);
 }
 
 void launch_trace(const Work& W, const Paths& P, const Tuning& t, unsigned tgrid, bool extend, bool shadow,
-                  hipStream_t st) {
+                  hipStream_t st, bool one_job) {
   if (t.ring == 4)
-    launch_trace_stages<4>(W, P, tgrid, shadow, st, extend);
+    launch_trace_stages<4>(W, P, tgrid, shadow, st, extend, one_job);
   else if (t.ring == 16)
-    launch_trace_stages<16>(W, P, tgrid, shadow, st, extend);
+    launch_trace_stages<16>(W, P, tgrid, shadow, st, extend, one_job);
   else
-    launch_trace_stages<8>(W, P, tgrid, shadow, st, extend);
+    launch_trace_stages<8>(W, P, tgrid, shadow, st, extend, one_job);
 }
 
 // All stages of the pass passes[*pass_idx] on the stream; accumulate = false
@@ -1030,12 +1045,12 @@ void launch_pass(pyre_ctx* ctx, Work& W, const PassShape& shape, const Tuning& t
   hipLaunchKernelGGL(k_counters, dim3(1), dim3(kCounters), 0, st, P.counters, sc);
   hipLaunchKernelGGL(k_generate, dim3(grid), dim3(kThreads), 0, st, P, sc);
   for (uint32_t d = 0; d < shape.max_depth; ++d) {
-    launch_trace(W, P, t, tgrid, true, false, st);
+    launch_trace(W, P, t, tgrid, true, false, st, shape.one_job);
     if (t.shade_waves >= 4)
       hipLaunchKernelGGL(k_shade<4>, dim3(grid), dim3(kThreads), 0, st, P, sc, d);
     else
       hipLaunchKernelGGL(k_shade<1>, dim3(grid), dim3(kThreads), 0, st, P, sc, d);
-    if (shape.nee) launch_trace(W, P, t, tgrid, false, true, st);
+    if (shape.nee) launch_trace(W, P, t, tgrid, false, true, st, shape.one_job);
     hipLaunchKernelGGL(k_rotate, dim3(1), dim3(kCounters), 0, st, P.counters, P.stats);
     std::swap(P.q_cur, P.q_next);
   }
@@ -1228,6 +1243,7 @@ void session_init(pyre_session* S, pyre_ctx* ctx, pyre_scene* s, const pyre_came
   S->shape.max_depth = ip.max_depth;
   S->shape.tree_depth = s->depth;
   S->shape.nee = s->n_lights + (s->has_env ? 1u : 0u) > 0;
+  S->shape.one_job = true;
   S->acc = dev_alloc<float>(10 * (size_t)S->npix);
   S->acc64 = dev_alloc<double>(3 * (size_t)S->npix);
   S->welford = dev_alloc<float>(3 * (size_t)S->npix);
@@ -1511,6 +1527,7 @@ int pyre_render_batch(pyre_ctx* ctx, const pyre_job* jobs, size_t n, const pyre_
       B.shape.tree_depth = std::max(B.shape.tree_depth, J.scene->depth);
       const uint32_t sl = J.scene->n_lights + (J.scene->has_env ? 1u : 0u);
       B.shape.nee = B.shape.nee || sl > 0;
+      B.shape.one_job = n == 1;
       B.slots = std::max(B.slots, sl);
     }
     // one allocation for every job's sums
