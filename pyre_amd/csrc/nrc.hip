@@ -664,9 +664,10 @@ __global__ __launch_bounds__(256) void k_train_update(const float* partial, uint
 
 inline unsigned blocks(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
-// Weight-gradient partials per layer: at least four chunks per block, at most 64
-// partials (6.5 MB), so the reduction stays a short pass.
-inline unsigned dw_partials(uint32_t np) { return std::max(1u, std::min(64u, blocks(np, 4 * kDwChunk))); }
+// Weight-gradient partials per layer: at least four chunks per block, at most
+// 256 partials (26 MB): large batches get more blocks for the (memory-bound)
+// weight gradients at the price of a longer reduction.
+inline unsigned dw_partials(uint32_t np) { return std::max(1u, std::min(256u, blocks(np, 4 * kDwChunk))); }
 
 }  // namespace
 
