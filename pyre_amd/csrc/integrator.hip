@@ -272,6 +272,11 @@ __global__ __launch_bounds__(kThreads) void k_generate(Paths P, Sched sc) {
   }
 }
 
+// Paths a wave claims at a time from a stage's queue counter once its
+// static share is used up (the stage queues shrink bounce by bounce, so one
+// chunk size; more, smaller claims contend on the counter).
+constexpr uint32_t kPathChunk = 256;
+
 // Closest hit for every queued path, lanes refilled from the queue as their
 // rays finish.
 template <int RING>
@@ -280,7 +285,7 @@ __global__ __launch_bounds__(kThreads) void k_extend(Paths P, const JobDev* __re
   __shared__ int2 ring[RING * kThreads];
   const uint32_t tid = blockIdx.x * kThreads + threadIdx.x;
   HybridStack<RING> st{ring + threadIdx.x, kThreads, gstack + tid, gridDim.x * kThreads, 0, 0};
-  WaveFeed feed(P.counters[0], P.counters + 8);
+  WaveFeed<kPathChunk, kPathChunk> feed(P.counters[0], P.counters + 8);
   Trav4<false, RING> tv;
   uint32_t p = 0, item = 0;
   const Node4* nodes = one_job ? jobs[0].S.nodes4 : nullptr;
@@ -320,7 +325,7 @@ __global__ __launch_bounds__(kThreads) void k_shadow(Paths P, const JobDev* __re
   __shared__ int2 ring[RING * kThreads];
   const uint32_t tid = blockIdx.x * kThreads + threadIdx.x;
   HybridStack<RING> st{ring + threadIdx.x, kThreads, gstack + tid, gridDim.x * kThreads, 0, 0};
-  WaveFeed feed(P.counters[2], P.counters + 16);
+  WaveFeed<kPathChunk, kPathChunk> feed(P.counters[2], P.counters + 16);
   Trav4<true, RING> tv;
   tv.cur = -1;  // -1: no traversal in progress for the current slot
   uint32_t p = 0, k = 0, cnt = 0, item = 0;

@@ -70,7 +70,7 @@ __global__ __launch_bounds__(kTraceThreads, 8) void k_trace4(const Node4* __rest
   const uint32_t tid = blockIdx.x * kTraceThreads + threadIdx.x;
   const uint32_t nthreads = gridDim.x * kTraceThreads;
   HybridStack<RING> st{ring + threadIdx.x, kTraceThreads, gstack + tid, nthreads, 0, 0};
-  WaveFeed feed(n, fetch);
+  WaveFeed<256, 64> feed(n, fetch);
   Trav4<ANY, RING> tv;
   uint32_t ray = 0;
   bool active = false;

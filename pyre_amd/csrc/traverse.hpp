@@ -315,19 +315,24 @@ __device__ __forceinline__ void traverse4(const Node4* __restrict__ nodes, const
 // Lanes of a wave refill once at least this many are idle.
 constexpr int kRefillLanes = 32;
 // Items each wave takes without any atomic (its static share), and items a
-// wave claims from the shared counter at a time once that is used up.
+// wave claims from the shared counter at a time once that is used up: large
+// chunks first, small ones over the last stretch so the waves run dry
+// together.
 constexpr uint32_t kStaticChunk = 64;
-constexpr uint32_t kWaveChunk = 256;
 
 // Work distribution of the persistent kernels over n items: wave w first
 // takes a static share [w * share, (w + 1) * share) with no atomic (share =
 // half of an even split, at least kStaticChunk), so a short queue costs no
-// atomics at all and a long one few; the items past the static shares are
-// claimed kWaveChunk at a time from one counter, which balances expensive
-// and cheap rays across the GPU. A wave that sees the counter reach the end
-// stops claiming. All members are wave-uniform.
+// atomics at all and a long one few. The items past the static shares are
+// claimed a chunk at a time from one counter of chunk numbers, which
+// balances expensive and cheap rays across the GPU: CHUNK items per claim up
+// to the last 2 * waves * TAIL items, TAIL items per claim after that
+// (guided self-scheduling in two sizes; one counter keeps the claims few).
+// A wave that sees the counter pass the end stops claiming. All members are
+// wave-uniform.
+template <uint32_t CHUNK, uint32_t TAIL>
 struct WaveFeed {
-  uint32_t n, next, end, dyn0;
+  uint32_t n, next, end, dyn0, tail0, n_head;
   uint32_t* ctr;
   bool dry;
   __device__ __forceinline__ WaveFeed(uint32_t n_, uint32_t* counter) {
@@ -345,21 +350,35 @@ struct WaveFeed {
     end = n - next > share ? (uint32_t)(next + share) : n;
     dyn0 = d0 < n ? (uint32_t)d0 : n;
     dry = dyn0 >= n;
+    // dynamic range [dyn0, n): large chunks over [dyn0, tail0), small ones
+    // over [tail0, n)
+    const uint64_t tail = 2 * waves * TAIL;
+    const uint64_t head = n - dyn0 > tail ? n - dyn0 - tail : 0;
+    tail0 = dyn0 + (uint32_t)head;
+    n_head = (uint32_t)((head + CHUNK - 1) / CHUNK);
   }
   // Gives the idle lanes (ballot `idle`) the next items; true for an idle
   // lane that got one (in `item`). Called by the whole wave.
   __device__ __forceinline__ bool take(uint64_t idle, bool is_idle, uint32_t& item) {
     if (next >= end) {
       if (dry) return false;
-      uint32_t base = 0;
-      if (__lane_id() == 0) base = atomicAdd(ctr, kWaveChunk);
-      base = __builtin_amdgcn_readfirstlane(base);
-      if (base >= n - dyn0) {
+      uint32_t c = 0;
+      if (__lane_id() == 0) c = atomicAdd(ctr, 1u);
+      c = __builtin_amdgcn_readfirstlane(c);
+      uint64_t s, e;
+      if (c < n_head) {
+        s = dyn0 + (uint64_t)c * CHUNK;
+        e = s + CHUNK < tail0 ? s + CHUNK : tail0;
+      } else {
+        s = tail0 + (uint64_t)(c - n_head) * TAIL;
+        e = s + TAIL < n ? s + TAIL : n;
+      }
+      if (s >= n) {
         dry = true;
         return false;
       }
-      next = dyn0 + base;
-      end = n - next > kWaveChunk ? next + kWaveChunk : n;
+      next = (uint32_t)s;
+      end = (uint32_t)e;
       dry = end >= n;
     }
     const uint32_t k = next + (uint32_t)__popcll(idle & ((1ull << __lane_id()) - 1ull));
