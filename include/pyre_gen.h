@@ -8,6 +8,10 @@
  *                          quad_mesh()         main.rs:353-360
  *                          make_uv_sphere()    main.rs:565-599
  *   pyre_auto_frame        auto_frame_params() main.rs:337-351
+ * Scene files, as the reference CLI reads them (main.rs:114-166):
+ *   pyre_preset_load_pbrt  load_pbrt()         crates/pyre/src/io/pbrt/{mod,lexer,parser,builder}.rs
+ *   pyre_preset_load_gltf  load_gltf()         crates/pyre/src/io/gltf.rs:20-92, main.rs:128-141
+ *   pyre_load_hdr          load_hdri()         crates/pyre/src/io/hdr.rs:24-50
  * Synthetic workloads with no reference counterpart (the reference has no
  * benchmark inputs); all randomness is xoshiro256++ seeded with SplitMix64,
  * transcendental/Gaussian draws are evaluated in double and rounded to f32:
@@ -62,6 +66,60 @@ int pyre_preset_desc(const pyre_preset* p, pyre_scene_desc* desc);
  * auto_frame_params over the instance bounds, vfov 45 (main.rs:186-194). */
 int pyre_preset_camera(const pyre_preset* p, float aspect, float aperture, pyre_camera_desc* cam);
 void pyre_preset_destroy(pyre_preset* p);
+
+/* What a scene file says about the render (io/pbrt/builder.rs:27-43):
+ * LookAt + Camera "perspective" fov (both needed for a camera) and Film
+ * xresolution / yresolution (defaults 800 x 600). Presets have neither. */
+typedef struct pyre_view {
+  int has_camera;
+  float origin[3];
+  float target[3];
+  float up[3];
+  float vfov_deg;
+  int has_film;
+  uint32_t width;
+  uint32_t height;
+} pyre_view;
+
+/* load_pbrt (io/pbrt/mod.rs:56-76): the reference's PBRT subset -- LookAt,
+ * Camera, Film; Identity / Translate / Scale / Rotate / Transform /
+ * ConcatTransform with Attribute and Transform blocks; Material,
+ * MakeNamedMaterial, NamedMaterial (matte -> Lambertian, plastic / metal /
+ * conductor -> Disney, others -> grey Lambertian); AreaLightSource "diffuse"
+ * on a four-vertex two-triangle trianglemesh -> quad light; LightSource
+ * "infinite" (constant L or a mapname .hdr relative to the file);
+ * Shape "sphere" (48 x 24 uv sphere) and "trianglemesh", baked to world
+ * space. Other directives are skipped, like the reference's. Errors: PYRE_EIO
+ * when a file cannot be read; PYRE_EINVAL for lex / parse / build / decode
+ * errors, with the reference's message ("lex: ...", "parse: ...",
+ * "build: ...") in err (err may be NULL). The camera of pyre_preset_camera
+ * is the file's LookAt/fov when it has both, else auto_frame_params over the
+ * scene bounds (main.rs:184-194). */
+int pyre_preset_load_pbrt(const char* path, pyre_preset** out, char* err, size_t err_len);
+
+/* main.rs:128-141 over load_gltf (io/gltf.rs:20-92): every triangle
+ * primitive of the default scene (else the first scene), node transforms
+ * accumulated parent * local and baked into the positions, normals by the
+ * inverse transpose of the upper 3x3 and normalised (zero stays zero), one
+ * Lambertian 0.73 material. .gltf with .bin or base64 data-URI buffers and
+ * binary .glb. Non-triangle modes and primitives without positions are
+ * PYRE_EINVAL, like the reference's GltfError. */
+int pyre_preset_load_gltf(const char* path, pyre_preset** out, char* err, size_t err_len);
+
+/* load_hdri (io/hdr.rs:24-50) for .hdr / .pic: Radiance RGBE, "-Y H +X W"
+ * scanlines (flat, old run-length or new run-length), top row first, each
+ * pixel R * 2^(E - 136) (E = 0: black), as the decoder the reference calls
+ * (crate image 0.25.9, hdr::Rgbe8Pixel::to_hdr). Call with rgb = NULL for
+ * the size; then rgb must hold 3 * width * height floats (rgb_len). */
+int pyre_load_hdr(const char* path, uint32_t* width, uint32_t* height, float* rgb, size_t rgb_len, char* err,
+                  size_t err_len);
+
+/* The CLI's --env (main.rs:162-166): replaces the preset's environment by
+ * the HDRI file at this intensity. */
+int pyre_preset_set_env_hdr(pyre_preset* p, const char* path, float intensity, char* err, size_t err_len);
+
+/* The file's render settings (zeroed for presets). */
+int pyre_preset_view(const pyre_preset* p, pyre_view* view);
 
 /* auto_frame_params (main.rs:337-351). */
 void pyre_auto_frame(const float bmin[3], const float bmax[3], float vfov_deg, float aspect, float origin[3],

@@ -42,6 +42,7 @@ extern "C" {
 #define PYRE_ENOMEM (-3)    /* device allocation failed */
 #define PYRE_EOVERFLOW (-4) /* traversal stack or queue could not be grown */
 #define PYRE_ESTATE (-5)    /* call out of order (e.g. trace before build) */
+#define PYRE_EIO (-6)       /* a file could not be opened or read */
 
 #define PYRE_MISS 0xFFFFFFFFu
 

@@ -9,6 +9,7 @@ call raises.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import weakref
 from dataclasses import dataclass
 
@@ -20,6 +21,7 @@ from ._lib import PyreError
 __all__ = [
     "Context", "Scene", "Preset", "PathIntegrator", "Film", "Camera", "Material", "PyreError",
     "random_triangles", "random_rays", "auto_frame", "MAT_LAMBERTIAN", "MAT_DISNEY", "MISS", "RadianceCache",
+    "load_hdr",
 ]
 
 MAT_LAMBERTIAN = L.MAT_LAMBERTIAN
@@ -126,9 +128,25 @@ class PathIntegrator:
     min_rr_depth: int = 3
 
 
+def load_hdr(path) -> np.ndarray:
+    """load_hdri's decode (io/hdr.rs:24-50): a Radiance .hdr/.pic as an
+    (height, width, 3) float32 array, top row first."""
+    lib = L.load()
+    w, h = L._u32(), L._u32()
+    err = C.create_string_buffer(512)
+    rc = lib.pyre_load_hdr(os.fsencode(path), C.byref(w), C.byref(h), None, 0, err, 512)
+    if rc:
+        raise PyreError(rc, f"pyre_load_hdr({path}): {err.value.decode(errors='replace')}")
+    out = np.empty((h.value, w.value, 3), np.float32)
+    rc = lib.pyre_load_hdr(os.fsencode(path), C.byref(w), C.byref(h), out.ctypes.data, out.size, err, 512)
+    if rc:
+        raise PyreError(rc, f"pyre_load_hdr({path}): {err.value.decode(errors='replace')}")
+    return out
+
+
 class Preset:
-    """Host-built scenes (pyre_gen.h): the CLI's Cornell box and the
-    synthetic production scene."""
+    """Host-built scenes (pyre_gen.h): the CLI's Cornell box, the synthetic
+    production scene, and scene files (.pbrt, .gltf/.glb, .hdr environments)."""
 
     def __init__(self, handle, lib):
         self.h = handle
@@ -162,6 +180,46 @@ class Preset:
         if rc:
             raise PyreError(rc, "pyre_preset_instances")
         return cls(h, lib)
+
+    @classmethod
+    def _load(cls, fn, path) -> "Preset":
+        lib = L.load()
+        h = C.c_void_p()
+        err = C.create_string_buffer(512)
+        rc = getattr(lib, fn)(os.fsencode(path), C.byref(h), err, 512)
+        if rc:
+            raise PyreError(rc, f"{fn}({path}): {err.value.decode(errors='replace')}")
+        return cls(h, lib)
+
+    @classmethod
+    def from_pbrt(cls, path) -> "Preset":
+        """load_pbrt (io/pbrt/mod.rs:56-76): the reference's PBRT subset; the
+        file's LookAt/fov and Film resolution are in .view."""
+        return cls._load("pyre_preset_load_pbrt", path)
+
+    @classmethod
+    def from_gltf(cls, path) -> "Preset":
+        """The CLI's glTF scene (main.rs:128-141 over io/gltf.rs:20-92): every
+        triangle primitive of the default scene, Lambertian 0.73."""
+        return cls._load("pyre_preset_load_gltf", path)
+
+    def set_env_hdr(self, path, intensity: float = 1.0) -> None:
+        """The CLI's --env (main.rs:162-166): a Radiance .hdr environment."""
+        err = C.create_string_buffer(512)
+        rc = self.lib.pyre_preset_set_env_hdr(self.h, os.fsencode(path), intensity, err, 512)
+        if rc:
+            raise PyreError(rc, f"pyre_preset_set_env_hdr({path}): {err.value.decode(errors='replace')}")
+
+    @property
+    def view(self) -> dict:
+        """What the scene file says about the render: camera (origin, target,
+        up, vfov_deg) and film (width, height), each None when absent."""
+        v = L.View()
+        self.lib.pyre_preset_view(self.h, C.byref(v))
+        cam = dict(origin=tuple(v.origin), target=tuple(v.target), up=tuple(v.up),
+                   vfov_deg=v.vfov_deg) if v.has_camera else None
+        film = (v.width, v.height) if v.has_film else None
+        return dict(camera=cam, film=film)
 
     def desc(self) -> L.SceneDesc:
         d = L.SceneDesc()

@@ -17,6 +17,7 @@ PYRE_EHIP = -2
 PYRE_ENOMEM = -3
 PYRE_EOVERFLOW = -4
 PYRE_ESTATE = -5
+PYRE_EIO = -6
 PYRE_MISS = 0xFFFFFFFF
 MAT_LAMBERTIAN = 0
 MAT_DISNEY = 1
@@ -76,6 +77,11 @@ class CameraDesc(C.Structure):
                 ("aperture_radius", _f), ("focus_distance", _f)]
 
 
+class View(C.Structure):
+    _fields_ = [("has_camera", C.c_int), ("origin", _f * 3), ("target", _f * 3), ("up", _f * 3), ("vfov_deg", _f),
+                ("has_film", C.c_int), ("width", _u32), ("height", _u32)]
+
+
 class IntegratorParams(C.Structure):
     _fields_ = [("max_depth", _u32), ("min_rr_depth", _u32)]
 
@@ -119,6 +125,12 @@ SIGNATURES = {
     "pyre_preset_camera": (C.c_int, [_vp, _f, _f, C.POINTER(CameraDesc)]),
     "pyre_preset_destroy": (None, [_vp]),
     "pyre_auto_frame": (None, [_fp, _fp, _f, _f, _fp, _fp]),
+    "pyre_preset_load_pbrt": (C.c_int, [C.c_char_p, C.POINTER(_vp), C.c_char_p, C.c_size_t]),
+    "pyre_preset_load_gltf": (C.c_int, [C.c_char_p, C.POINTER(_vp), C.c_char_p, C.c_size_t]),
+    "pyre_load_hdr": (C.c_int, [C.c_char_p, C.POINTER(_u32), C.POINTER(_u32), _vp, C.c_size_t, C.c_char_p,
+                                C.c_size_t]),
+    "pyre_preset_set_env_hdr": (C.c_int, [_vp, C.c_char_p, _f, C.c_char_p, C.c_size_t]),
+    "pyre_preset_view": (C.c_int, [_vp, C.POINTER(View)]),
 }
 
 _lib = None

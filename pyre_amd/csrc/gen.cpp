@@ -7,6 +7,7 @@
 
 #include "../../include/pyre_gen.h"
 #include "pmath.hpp"
+#include "preset.hpp"
 
 using namespace pyre;
 
@@ -26,26 +27,9 @@ struct HostRng {
   }
 };
 
-struct Mesh {
-  std::vector<float> pos, nrm;
-  std::vector<uint32_t> idx;
-  uint32_t material;
-};
+using Mesh = PresetMesh;
 
 }  // namespace
-
-struct pyre_preset {
-  std::vector<Mesh> meshes;
-  std::vector<pyre_material> materials;
-  std::vector<pyre_quad_light> lights;
-  std::vector<float> env_rgb;
-  uint32_t env_w = 0, env_h = 0;
-  float env_intensity = 1.0f;
-  std::vector<pyre_mesh_desc> mdesc;
-  pyre_env_desc edesc{};
-  bool cornell = false;
-  float frame_min[3] = {0, 0, 0}, frame_max[3] = {0, 0, 0};  // bounds the auto-frame camera uses
-};
 
 namespace {
 
@@ -139,7 +123,9 @@ void icosphere(uint32_t level, std::vector<double>& P, std::vector<uint32_t>& F)
   }
 }
 
-void finish(pyre_preset* p) {
+}  // namespace
+
+void preset_finish(pyre_preset* p) {
   p->mdesc.clear();
   for (auto& m : p->meshes) {
     pyre_mesh_desc d{};
@@ -157,7 +143,20 @@ void finish(pyre_preset* p) {
   p->edesc.intensity = p->env_intensity;
 }
 
-}  // namespace
+void preset_scene_bounds(pyre_preset* p) {
+  // Bounds3::EMPTY grown by every triangle's vertices (math.rs:37-77)
+  float lo[3] = {kInf, kInf, kInf}, hi[3] = {-kInf, -kInf, -kInf};
+  for (const auto& m : p->meshes)
+    for (uint32_t v : m.idx)
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = rmin(lo[a], m.pos[3 * (size_t)v + a]);
+        hi[a] = rmax(hi[a], m.pos[3 * (size_t)v + a]);
+      }
+  for (int a = 0; a < 3; ++a) {
+    p->frame_min[a] = lo[a];
+    p->frame_max[a] = hi[a];
+  }
+}
 
 extern "C" {
 
@@ -214,7 +213,7 @@ int pyre_preset_cornell(int motion_blur, pyre_preset** out) {
   if (!out || motion_blur) return PYRE_EINVAL;
   pyre_preset* p = new (std::nothrow) pyre_preset();
   if (!p) return PYRE_ENOMEM;
-  p->cornell = true;
+  p->kind = PRESET_CORNELL;
   p->materials.push_back(lambert(splat(0.73f)));
   p->materials.push_back(lambert(v3(0.65f, 0.05f, 0.05f)));
   p->materials.push_back(lambert(v3(0.12f, 0.45f, 0.15f)));
@@ -234,7 +233,7 @@ int pyre_preset_cornell(int motion_blur, pyre_preset** out) {
   p->materials.push_back(disney(splat(0.72f), 0.0f, 0.18f, 0.5f));
   p->meshes.push_back(make_uv_sphere(v3(-0.35f, -0.65f, -0.3f), 0.35f, 48, 24, 4));
   p->lights.push_back(pyre_quad_light{{-0.3f, 0.999f, -0.3f}, {0.6f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.6f}, {15, 15, 15}});
-  finish(p);
+  preset_finish(p);
   *out = p;
   return PYRE_OK;
 }
@@ -354,7 +353,7 @@ int pyre_preset_instances(uint64_t seed, uint32_t n_instances, uint32_t level, u
       o[2] = (float)(color.z + sun * 0.85);
     }
   }
-  finish(p);
+  preset_finish(p);
   *out = p;
   return PYRE_OK;
 }
@@ -401,11 +400,32 @@ int pyre_preset_camera(const pyre_preset* p, float aspect, float aperture, pyre_
   cam->aspect = aspect;
   cam->aperture_radius = aperture;
   cam->focus_distance = 0.0f;  // |target - origin|
-  if (p->cornell) {
+  if (p->kind == PRESET_CORNELL) {
     cam->origin[2] = 3.0f;
-  } else {
+  } else if (p->kind == PRESET_INSTANCES) {
     pyre_auto_frame(p->frame_min, p->frame_max, 45.0f, aspect, cam->origin, cam->target);
+  } else if (p->view.has_camera) {  // main.rs:184-185
+    for (int a = 0; a < 3; ++a) {
+      cam->origin[a] = p->view.origin[a];
+      cam->target[a] = p->view.target[a];
+      cam->up[a] = p->view.up[a];
+    }
+    cam->vfov_deg = p->view.vfov_deg;
+  } else {  // main.rs:186-194: framed over the scene bounds, or (0,0,3) -> 0
+    const V3 d = v3(p->frame_max[0], p->frame_max[1], p->frame_max[2]) -
+                 v3(p->frame_min[0], p->frame_min[1], p->frame_min[2]);
+    const float diag = length(d);
+    if (std::isfinite(diag) && diag > 0.0f)
+      pyre_auto_frame(p->frame_min, p->frame_max, 45.0f, aspect, cam->origin, cam->target);
+    else
+      cam->origin[2] = 3.0f;
   }
+  return PYRE_OK;
+}
+
+int pyre_preset_view(const pyre_preset* p, pyre_view* view) {
+  if (!p || !view) return PYRE_EINVAL;
+  *view = p->view;
   return PYRE_OK;
 }
 
