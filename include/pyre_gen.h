@@ -40,8 +40,8 @@ int pyre_gen_random_rays(uint64_t seed, uint32_t n, uint32_t n_closest, pyre_ray
 
 typedef struct pyre_preset pyre_preset;
 
-/* The built-in Cornell box (motion_blur must be 0: the device path has no
- * moving instances). */
+/* The built-in Cornell box; motion_blur != 0 animates the gold sphere from
+ * offset (0, 0, 0) to (0, 0.25, 0) over the shutter (main.rs:455-465). */
 int pyre_preset_cornell(int motion_blur, pyre_preset** out);
 
 /* Production scene: n_instances level-`level` icospheres with random

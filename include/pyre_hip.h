@@ -107,7 +107,11 @@ typedef struct pyre_quad_light {
 
 /* One scene primitive (scene.rs:19-22): an indexed triangle mesh in world
  * space (TriangleMesh, geometry/mesh.rs:5-11) and its material id.
- * normals may be NULL (face normals are used, mesh.rs:131-136). */
+ * normals may be NULL (face normals are used, mesh.rs:131-136). motion may
+ * be NULL (a static mesh) or point at InstanceMotion's six floats
+ * {offset_t0[3], offset_t1[3]} (mesh.rs:46-60): the mesh is hit by rays
+ * shifted by -lerp(offset_t0, offset_t1, clamp(ray.time, 0, 1)), and the hit
+ * position is shifted back (mesh.rs:109-148). */
 typedef struct pyre_mesh_desc {
   const float* positions;  /* 3 * n_vertices */
   const uint32_t* indices; /* 3 * n_triangles */
@@ -115,6 +119,7 @@ typedef struct pyre_mesh_desc {
   uint32_t n_vertices;
   uint32_t n_triangles;
   uint32_t material_id;
+  const float* motion; /* 6 floats or NULL */
 } pyre_mesh_desc;
 
 /* HdriEnvironmentLight::new arguments (light.rs:214-242): row-major RGB32F. */
@@ -212,7 +217,9 @@ int pyre_intersect_closest(pyre_ctx* ctx, pyre_scene* scene, const pyre_ray* ray
  * a hit with t in (t_min, t_max). */
 int pyre_intersect_any(pyre_ctx* ctx, pyre_scene* scene, const pyre_ray* rays, size_t n, uint32_t* occluded);
 /* Device-resident forms: rays = pyre_ray32[n], hits = pyre_hit8[n] /
- * occluded = uint32_t[n]; stream = hipStream_t or NULL. */
+ * occluded = uint32_t[n]; stream = hipStream_t or NULL. pyre_ray32 has no
+ * time field: moving meshes are seen at shutter time 0 (Ray::new's time,
+ * math.rs:16-24); pyre_intersect_* honour pyre_ray.time. */
 int pyre_trace_closest_dev(pyre_ctx* ctx, pyre_scene* scene, const void* rays, size_t n, void* hits, void* stream);
 int pyre_trace_any_dev(pyre_ctx* ctx, pyre_scene* scene, const void* rays, size_t n, void* occluded, void* stream);
 

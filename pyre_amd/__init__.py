@@ -153,10 +153,12 @@ class Preset:
         self.lib = lib
 
     @classmethod
-    def cornell(cls) -> "Preset":
+    def cornell(cls, motion_blur: bool = False) -> "Preset":
+        """cornell_box (main.rs:365-496); motion_blur moves the gold sphere
+        by (0, 0.25, 0) over the shutter."""
         lib = L.load()
         h = C.c_void_p()
-        rc = lib.pyre_preset_cornell(0, C.byref(h))
+        rc = lib.pyre_preset_cornell(int(motion_blur), C.byref(h))
         if rc:
             raise PyreError(rc, "pyre_preset_cornell")
         return cls(h, lib)
@@ -240,7 +242,8 @@ class Preset:
             pos = np.ctypeslib.as_array(m.positions, (m.n_vertices, 3)).copy()
             idx = np.ctypeslib.as_array(m.indices, (m.n_triangles, 3)).copy()
             nrm = np.ctypeslib.as_array(m.normals, (m.n_vertices, 3)).copy() if m.normals else None
-            meshes.append(dict(positions=pos, indices=idx, normals=nrm, material_id=m.material_id))
+            mot = np.ctypeslib.as_array(m.motion, (2, 3)).copy() if m.motion else None
+            meshes.append(dict(positions=pos, indices=idx, normals=nrm, material_id=m.material_id, motion=mot))
         mats = [Material(d.materials[i].kind, tuple(d.materials[i].base_color), d.materials[i].metallic,
                          d.materials[i].roughness, d.materials[i].specular) for i in range(d.n_materials)]
         lights = [np.array([*d.lights[i].p0, *d.lights[i].edge_u, *d.lights[i].edge_v, *d.lights[i].emission],
@@ -276,17 +279,20 @@ class Scene:
 
     @classmethod
     def from_arrays(cls, ctx: Context, meshes, materials, lights=(), env=None) -> "Scene":
-        """meshes: dicts with positions (n,3), indices (m,3), optional normals, material_id."""
+        """meshes: dicts with positions (n,3), indices (m,3), optional normals,
+        material_id and motion ((offset_t0, offset_t1), InstanceMotion)."""
         keep = []
         md = (L.MeshDesc * max(1, len(meshes)))()
         for i, m in enumerate(meshes):
             pos = _f32(m["positions"]).reshape(-1)
             idx = _u32(m["indices"]).reshape(-1)
             nrm = _f32(m["normals"]).reshape(-1) if m.get("normals") is not None else None
-            keep += [pos, idx, nrm]
+            mot = _f32(m["motion"]).reshape(6) if m.get("motion") is not None else None
+            keep += [pos, idx, nrm, mot]
             md[i] = L.MeshDesc(pos.ctypes.data_as(L._fp), idx.ctypes.data_as(L._u32p),
                                nrm.ctypes.data_as(L._fp) if nrm is not None else None,
-                               pos.size // 3, idx.size // 3, int(m.get("material_id", 0)))
+                               pos.size // 3, idx.size // 3, int(m.get("material_id", 0)),
+                               mot.ctypes.data_as(L._fp) if mot is not None else None)
         mats = (L.Material * max(1, len(materials)))(*[m.to_c() for m in materials])
         lts = (L.QuadLight * max(1, len(lights)))()
         for i, l in enumerate(lights):

@@ -55,7 +55,7 @@ class QuadLight(C.Structure):
 
 class MeshDesc(C.Structure):
     _fields_ = [("positions", _fp), ("indices", _u32p), ("normals", _fp), ("n_vertices", _u32),
-                ("n_triangles", _u32), ("material_id", _u32)]
+                ("n_triangles", _u32), ("material_id", _u32), ("motion", _fp)]
 
 
 class EnvDesc(C.Structure):

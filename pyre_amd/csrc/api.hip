@@ -157,6 +157,7 @@ void scene_free(pyre_scene* s) {
   bvh_release(s);
   dev_free(s->d_verts);
   dev_free(s->d_normals);
+  dev_free(s->d_motion);
   dev_free(s->d_tri_offset);
   dev_free(s->d_mesh_info);
   dev_free(s->d_mats);
@@ -297,12 +298,13 @@ __global__ void k_light_pdf(QuadLightDev L, const float* from, const float* wi, 
 }
 
 // Converts host rays (36 B) to the device layout (32 B).
-__global__ void k_pack_rays(const float* in, uint32_t n, float4* out) {
+__global__ void k_pack_rays(const float* in, uint32_t n, float4* out, float* times) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float* r = in + 9 * (size_t)i;
   out[2 * (size_t)i] = make_float4(r[0], r[1], r[2], r[6]);
   out[2 * (size_t)i + 1] = make_float4(r[3], r[4], r[5], r[7]);
+  if (times) times[i] = r[8];
 }
 
 // Scene-global triangle id -> (primitive, local triangle).
@@ -443,6 +445,13 @@ int pyre_scene_upload(pyre_ctx* ctx, const pyre_scene_desc* desc, pyre_scene** o
       max_t = std::max(max_t, (size_t)md.n_triangles);
       any_normals |= md.normals != nullptr;
       s->h_mesh_info.push_back(md.material_id | (md.normals ? 0x80000000u : 0u));
+      const float* mo = md.motion;
+      if (mo)
+        for (int k = 0; k < 6; ++k) require(std::isfinite(mo[k]), "mesh motion offsets must be finite");
+      s->h_moving.push_back(mo ? 1 : 0);
+      s->h_motion.push_back(mo ? make_float4(mo[0], mo[1], mo[2], 0.0f) : make_float4(0, 0, 0, 0));
+      s->h_motion.push_back(mo ? make_float4(mo[3], mo[4], mo[5], 0.0f) : make_float4(0, 0, 0, 0));
+      s->has_motion |= mo != nullptr;
     }
     s->h_tri_offset[d.n_meshes] = (uint32_t)tri_total;
     s->n_tris = (uint32_t)tri_total;
@@ -469,6 +478,10 @@ int pyre_scene_upload(pyre_ctx* ctx, const pyre_scene_desc* desc, pyre_scene** o
     h2d(s->d_tri_offset, s->h_tri_offset.data(), d.n_meshes + 1, st);
     s->d_mesh_info = dev_alloc<uint32_t>(std::max<size_t>(d.n_meshes, 1));
     h2d(s->d_mesh_info, s->h_mesh_info.data(), d.n_meshes, st);
+    if (s->has_motion) {
+      s->d_motion = dev_alloc<float4>(s->h_motion.size());
+      h2d(s->d_motion, s->h_motion.data(), s->h_motion.size(), st);
+    }
     for (uint32_t i = 0; i < d.n_materials; ++i) s->h_mats.push_back(to_dev(d.materials[i]));
     s->d_mats = dev_alloc<MatDev>(std::max<size_t>(d.n_materials, 1));
     h2d(s->d_mats, s->h_mats.data(), d.n_materials, st);
@@ -492,6 +505,34 @@ int pyre_scene_upload(pyre_ctx* ctx, const pyre_scene_desc* desc, pyre_scene** o
     for (int c = 0; c < 3; ++c) {
       s->bmin[c] = key2f(hk[c]);
       s->bmax[c] = key2f(hk[3 + c]);
+    }
+    if (s->has_motion) {
+      // Scene::bounds over MeshInstance::world_bounds (mesh.rs:97-107): a
+      // moving mesh contributes its box at both ends of the shutter
+      V3 lo = splat(kInf), hi = splat(-kInf);
+      for (uint32_t m = 0; m < d.n_meshes; ++m) {
+        const pyre_mesh_desc& md = d.meshes[m];
+        V3 blo = splat(kInf), bhi = splat(-kInf);
+        for (size_t k = 0; k < 3 * (size_t)md.n_triangles; ++k) {
+          const float* v = md.positions + 3 * (size_t)md.indices[k];
+          blo = vmin(blo, v3(v[0], v[1], v[2]));
+          bhi = vmax(bhi, v3(v[0], v[1], v[2]));
+        }
+        if (md.n_triangles == 0) continue;
+        if (md.motion) {
+          const V3 o0 = v3(md.motion[0], md.motion[1], md.motion[2]), o1 = v3(md.motion[3], md.motion[4], md.motion[5]);
+          lo = vmin(lo, vmin(blo + o0, blo + o1));
+          hi = vmax(hi, vmax(bhi + o0, bhi + o1));
+        } else {
+          lo = vmin(lo, blo);
+          hi = vmax(hi, bhi);
+        }
+      }
+      const float l3[3] = {lo.x, lo.y, lo.z}, h3[3] = {hi.x, hi.y, hi.z};
+      for (int c = 0; c < 3; ++c) {
+        s->bmin[c] = l3[c];
+        s->bmax[c] = h3[c];
+      }
     }
   });
   if (rc != PYRE_OK) {
@@ -574,9 +615,11 @@ int pyre_intersect_closest(pyre_ctx* ctx, pyre_scene* s, const pyre_ray* rays, s
     DevVec<pyre_hit8> h8(n);
     DevVec<float2> uv(n);
     DevVec<pyre_hit> out(n);
+    DevVec<float> times(s->has_motion ? n : 1);
     h2d(raw.p, (const float*)rays, 9 * n, st);
-    hipLaunchKernelGGL(k_pack_rays, dim3(blocks(n, 256)), dim3(256), 0, st, raw.p, (uint32_t)n, r32.p);
-    trace_closest(s, r32.p, n, h8.p, uv.p, st, true);
+    hipLaunchKernelGGL(k_pack_rays, dim3(blocks(n, 256)), dim3(256), 0, st, raw.p, (uint32_t)n, r32.p,
+                       s->has_motion ? times.p : nullptr);
+    trace_closest(s, r32.p, n, h8.p, uv.p, st, true, s->has_motion ? times.p : nullptr);
     hipLaunchKernelGGL(k_unpack_hits, dim3(blocks(n, 256)), dim3(256), 0, st, h8.p, uv.p, (uint32_t)n,
                        s->d_tri_offset, s->n_meshes, out.p);
     d2h(hits, out.p, n, st);
@@ -592,9 +635,11 @@ int pyre_intersect_any(pyre_ctx* ctx, pyre_scene* s, const pyre_ray* rays, size_
     DevVec<float> raw(9 * n);
     DevVec<float4> r32(2 * n);
     DevVec<uint32_t> occ(n);
+    DevVec<float> times(s->has_motion ? n : 1);
     h2d(raw.p, (const float*)rays, 9 * n, st);
-    hipLaunchKernelGGL(k_pack_rays, dim3(blocks(n, 256)), dim3(256), 0, st, raw.p, (uint32_t)n, r32.p);
-    trace_any(s, r32.p, n, occ.p, st, true);
+    hipLaunchKernelGGL(k_pack_rays, dim3(blocks(n, 256)), dim3(256), 0, st, raw.p, (uint32_t)n, r32.p,
+                       s->has_motion ? times.p : nullptr);
+    trace_any(s, r32.p, n, occ.p, st, true, s->has_motion ? times.p : nullptr);
     d2h(occluded, occ.p, n, st);
     PYRE_HIP_CHECK(hipStreamSynchronize(st));
   });

@@ -342,8 +342,25 @@ void exclusive_scan(Scratch& S, const uint32_t* in, uint32_t* out, size_t n, uin
 // Per triangle bounds and centroid (mesh.rs:32-40); root bounds reduced per
 // thread over a grid-stride range, then per wave and per block (LDS), so
 // the root's keys see one set of atomics per block.
+// Mesh of scene-global triangle gid: the largest m with tri_offset[m] <= gid.
+__device__ __forceinline__ uint32_t mesh_of_tri(const uint32_t* tri_offset, uint32_t n_meshes, uint32_t gid) {
+  uint32_t lo = 0, hi = n_meshes;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (tri_offset[mid] <= gid)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// motion (scenes with moving meshes, else null): 2 float4 per mesh; a moving
+// mesh's triangle gets the box swept over the shutter (its box at offset_t0
+// and at offset_t1) and the centroid at the mid-shutter offset.
 __global__ __launch_bounds__(256) void k_prep(const float* __restrict__ verts, uint32_t n, float4* plo, float4* phi,
-                                              float4* pc, uint32_t* ref, BNode* root) {
+                                              float4* pc, uint32_t* ref, BNode* root, const float4* motion,
+                                              const uint32_t* tri_offset, uint32_t n_meshes) {
   __shared__ int32_t part[4][12];
   KeyAcc acc;
   acc.init();
@@ -353,6 +370,17 @@ __global__ __launch_bounds__(256) void k_prep(const float* __restrict__ verts, u
     V3 lo = vmin(vmin(a, b), c);
     V3 hi = vmax(vmax(a, b), c);
     V3 ce = (a + b + c) / 3.0f;
+    if (motion) {
+      const uint32_t m = mesh_of_tri(tri_offset, n_meshes, i);
+      const float4 m0 = motion[2 * m], m1 = motion[2 * m + 1];
+      const V3 o0 = v3(m0.x, m0.y, m0.z), o1 = v3(m1.x, m1.y, m1.z);
+      if (o0.x != 0.0f || o0.y != 0.0f || o0.z != 0.0f || o1.x != 0.0f || o1.y != 0.0f || o1.z != 0.0f) {
+        const V3 l0 = lo + o0, l1 = lo + o1, h0 = hi + o0, h1 = hi + o1;
+        lo = vmin(l0, l1);
+        hi = vmax(h0, h1);
+        ce = ce + 0.5f * (o0 + o1);
+      }
+    }
     float4 l4 = make_float4(lo.x, lo.y, lo.z, 0.0f), h4 = make_float4(hi.x, hi.y, hi.z, 0.0f),
            c4 = make_float4(ce.x, ce.y, ce.z, 0.0f);
     plo[i] = l4;
@@ -999,15 +1027,23 @@ __global__ void k_flatten_leaf_root(const BNode* nodes, Node2* out) {
   out[0] = o;
 }
 
-__global__ void k_reorder_tris(const float* verts, const uint32_t* ref, uint32_t n, float4* tris) {
+// moving (per mesh, or null for static scenes): e1.w of a moving mesh's
+// triangle holds the mesh index + 1 (its motion slot for the leaf tests).
+__global__ void k_reorder_tris(const float* verts, const uint32_t* ref, uint32_t n, float4* tris,
+                               const uint32_t* moving, const uint32_t* tri_offset, uint32_t n_meshes) {
   uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   uint32_t gid = ref[k];
   const float* v = verts + 9 * (size_t)gid;
   V3 v0 = v3(v[0], v[1], v[2]), v1 = v3(v[3], v[4], v[5]), v2 = v3(v[6], v[7], v[8]);
   V3 e1 = v1 - v0, e2 = v2 - v0;  // mesh.rs:154-155
+  uint32_t slot = 0;
+  if (moving) {
+    const uint32_t m = mesh_of_tri(tri_offset, n_meshes, gid);
+    slot = moving[m] ? m + 1 : 0u;
+  }
   tris[3 * (size_t)k] = make_float4(v0.x, v0.y, v0.z, __builtin_bit_cast(float, gid));
-  tris[3 * (size_t)k + 1] = make_float4(e1.x, e1.y, e1.z, 0.0f);
+  tris[3 * (size_t)k + 1] = make_float4(e1.x, e1.y, e1.z, __builtin_bit_cast(float, slot));
   tris[3 * (size_t)k + 2] = make_float4(e2.x, e2.y, e2.z, 0.0f);
 }
 
@@ -1220,7 +1256,8 @@ void bvh_build(pyre_scene* s, const pyre_bvh_params& prm, hipStream_t st) {
   h.node_cursor = 1;
   uint32_t L = small ? 0 : 1;
   hipLaunchKernelGGL(k_prep, dim3(std::min<unsigned>(blocks(n, 256), (unsigned)std::max(1, s->ctx->cus) * 8)),
-                     dim3(256), 0, st, s->d_verts, n, plo, phi, pc, ref, nodes);
+                     dim3(256), 0, st, s->d_verts, n, plo, phi, pc, ref, nodes, s->d_motion, s->d_tri_offset,
+                     s->n_meshes);
   uint32_t level = 0;
   while (L > 0) {
     // L + 1 threads: the last writes the scan sentinel nch[L] = 0
@@ -1290,7 +1327,13 @@ void bvh_build(pyre_scene* s, const pyre_bvh_params& prm, hipStream_t st) {
   else
     hipLaunchKernelGGL(k_flatten, dim3(blocks(nb, 256)), dim3(256), 0, st, nodes, nb, flag, map, s->d_nodes);
   s->d_tris = dev_alloc<float4>(3 * (size_t)n);
-  hipLaunchKernelGGL(k_reorder_tris, dim3(blocks(n, 256)), dim3(256), 0, st, s->d_verts, ref, n, s->d_tris);
+  uint32_t* moving = nullptr;
+  if (s->has_motion) {
+    moving = S.get<uint32_t>(s->n_meshes);
+    PYRE_HIP_CHECK(hipMemcpyAsync(moving, s->h_moving.data(), s->n_meshes * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+  }
+  hipLaunchKernelGGL(k_reorder_tris, dim3(blocks(n, 256)), dim3(256), 0, st, s->d_verts, ref, n, s->d_tris, moving,
+                     s->d_tri_offset, s->n_meshes);
   bvh_collapse4(s, S, st);
   PYRE_HIP_CHECK(hipGetLastError());
   PYRE_HIP_CHECK(hipStreamSynchronize(st));

@@ -135,6 +135,7 @@ void preset_finish(pyre_preset* p) {
     d.n_vertices = (uint32_t)(m.pos.size() / 3);
     d.n_triangles = (uint32_t)(m.idx.size() / 3);
     d.material_id = m.material;
+    d.motion = m.motion.size() == 6 ? m.motion.data() : nullptr;
     p->mdesc.push_back(d);
   }
   p->edesc.width = p->env_w;
@@ -210,7 +211,7 @@ int pyre_gen_random_rays(uint64_t seed, uint32_t n, uint32_t n_closest, pyre_ray
 
 // main.rs:365-496
 int pyre_preset_cornell(int motion_blur, pyre_preset** out) {
-  if (!out || motion_blur) return PYRE_EINVAL;
+  if (!out) return PYRE_EINVAL;
   pyre_preset* p = new (std::nothrow) pyre_preset();
   if (!p) return PYRE_ENOMEM;
   p->kind = PRESET_CORNELL;
@@ -230,6 +231,7 @@ int pyre_preset_cornell(int motion_blur, pyre_preset** out) {
   p->meshes.push_back(quad_mesh(right_c, -v3(1, 0, 0), 2));
   p->materials.push_back(disney(v3(1.0f, 0.766f, 0.336f), 1.0f, 0.1f, 0.5f));
   p->meshes.push_back(make_uv_sphere(v3(0.35f, -0.65f, 0.25f), 0.35f, 48, 24, 3));
+  if (motion_blur) p->meshes.back().motion = {0.0f, 0.0f, 0.0f, 0.0f, 0.25f, 0.0f};  // main.rs:455-465
   p->materials.push_back(disney(splat(0.72f), 0.0f, 0.18f, 0.5f));
   p->meshes.push_back(make_uv_sphere(v3(-0.35f, -0.65f, -0.3f), 0.35f, 48, 24, 4));
   p->lights.push_back(pyre_quad_light{{-0.3f, 0.999f, -0.3f}, {0.6f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.6f}, {15, 15, 15}});

@@ -66,6 +66,7 @@ struct SceneDev {
   const QuadLightDev* lights;
   uint32_t n_meshes, n_lights, has_env;
   EnvDev env;
+  const float4* motion;  // 2 per mesh (offset_t0, offset_t1) when some mesh moves, else null
 };
 
 // One render job: scene, camera, image and its sums (film.rs:38-109).
@@ -131,6 +132,7 @@ struct Paths {
   float4* aov0;      // albedo, depth
   float4* aov1;      // normal
   float4* hit;       // gid bits, t, u, v
+  float* time;       // shutter time (written for jobs whose scene has moving meshes)
   float4* sray;      // 2 per slot
   float4* scontrib;  // contribution per slot
   uint32_t* scount;
@@ -256,7 +258,8 @@ __global__ __launch_bounds__(kThreads) void k_generate(Paths P, Sched sc) {
     float ndc_x = 2.0f * ((float)x + jx) / (float)width - 1.0f;
     float ndc_y = 1.0f - 2.0f * ((float)y + jy) / (float)J.height;
     V2 lens = rng_vec2(rng);
-    (void)rng_f32(rng);  // shutter time (static scenes)
+    const float shutter = rng_f32(rng);  // CameraSample::time (main.rs:268)
+    if (J.S.motion) P.time[p] = shutter;
     V3 o, d;
     camera_ray(J.cam, V2{ndc_x, ndc_y}, lens, &o, &d);
     P.ray[2 * (size_t)p] = f4(o, 1e-4f);
@@ -278,18 +281,20 @@ __global__ __launch_bounds__(kThreads) void k_generate(Paths P, Sched sc) {
 constexpr uint32_t kPathChunk = 256;
 
 // Closest hit for every queued path, lanes refilled from the queue as their
-// rays finish.
-template <int RING>
+// rays finish. MOTION: some job's scene has moving meshes (the path's shutter
+// time goes with its ray).
+template <int RING, bool MOTION = false>
 __global__ __launch_bounds__(kThreads) void k_extend(Paths P, const JobDev* __restrict__ jobs, int2* gstack,
                                                    bool one_job) {
   __shared__ int2 ring[RING * kThreads];
   const uint32_t tid = blockIdx.x * kThreads + threadIdx.x;
   HybridStack<RING> st{ring + threadIdx.x, kThreads, gstack + tid, gridDim.x * kThreads, 0, 0};
   WaveFeed<kPathChunk, kPathChunk> feed(P.counters[0], P.counters + 8);
-  Trav4<false, RING> tv;
+  Trav4<false, RING, MOTION> tv;
   uint32_t p = 0, item = 0;
   const Node4* nodes = one_job ? jobs[0].S.nodes4 : nullptr;
   const float4* tris = one_job ? jobs[0].S.tris : nullptr;
+  const float4* motion = MOTION && one_job ? jobs[0].S.motion : nullptr;
   bool active = false;
   for (;;) {
     const uint64_t idle = __ballot(!active);
@@ -300,14 +305,15 @@ __global__ __launch_bounds__(kThreads) void k_extend(Paths P, const JobDev* __re
           const JobDev& J = jobs[P.pjob[p]];
           nodes = J.S.nodes4;
           tris = J.S.tris;
+          if (MOTION) motion = J.S.motion;
         }
         const float4 a = P.ray[2 * (size_t)p], b = P.ray[2 * (size_t)p + 1];
-        tv.start(make_tray(xyz(a), xyz(b), a.w, b.w), st);
+        tv.start(make_tray(xyz(a), xyz(b), a.w, b.w, MOTION && motion ? P.time[p] : 0.0f), st);
         active = true;
       }
       if (!__any(active)) break;
     }
-    if (active && tv.step(nodes, tris, st)) {
+    if (active && tv.step(nodes, tris, st, motion)) {
       THit h;
       tv.result(&h);
       P.hit[p] = make_float4(__builtin_bit_cast(float, h.gid), h.t, h.u, h.v);
@@ -319,18 +325,20 @@ __global__ __launch_bounds__(kThreads) void k_extend(Paths P, const JobDev* __re
 // Any-hit for every shadow slot of the queued paths; a lane keeps a path
 // until all its slots are traced so unoccluded contributions are added to the
 // path radiance in slot order (lights, then environment).
-template <int RING>
+template <int RING, bool MOTION = false>
 __global__ __launch_bounds__(kThreads) void k_shadow(Paths P, const JobDev* __restrict__ jobs, int2* gstack,
                                                    bool one_job) {
   __shared__ int2 ring[RING * kThreads];
   const uint32_t tid = blockIdx.x * kThreads + threadIdx.x;
   HybridStack<RING> st{ring + threadIdx.x, kThreads, gstack + tid, gridDim.x * kThreads, 0, 0};
   WaveFeed<kPathChunk, kPathChunk> feed(P.counters[2], P.counters + 16);
-  Trav4<true, RING> tv;
+  Trav4<true, RING, MOTION> tv;
   tv.cur = -1;  // -1: no traversal in progress for the current slot
   uint32_t p = 0, k = 0, cnt = 0, item = 0;
   const Node4* nodes = one_job ? jobs[0].S.nodes4 : nullptr;
   const float4* tris = one_job ? jobs[0].S.tris : nullptr;
+  const float4* motion = MOTION && one_job ? jobs[0].S.motion : nullptr;
+  float tm = 0.0f;  // the path's shutter time (scene.rs:102-140 take ray.time)
   float4 L = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   bool active = false;
   for (;;) {
@@ -342,14 +350,16 @@ __global__ __launch_bounds__(kThreads) void k_shadow(Paths P, const JobDev* __re
           const JobDev& J = jobs[P.pjob[p]];
           nodes = J.S.nodes4;
           tris = J.S.tris;
+          if (MOTION) motion = J.S.motion;
         }
+        if (MOTION) tm = motion ? P.time[p] : 0.0f;
         cnt = P.scount[p];
         k = 0;
         L = P.lum[p];
         tv.cur = -1;
         // the first slot's ray with the path's data (one memory round trip)
         const float4 a = P.sray[2 * (size_t)p * P.slots], b = P.sray[2 * (size_t)p * P.slots + 1];
-        if (cnt > 0 && b.w > a.w) tv.start(make_tray(xyz(a), xyz(b), a.w, b.w), st);
+        if (cnt > 0 && b.w > a.w) tv.start(make_tray(xyz(a), xyz(b), a.w, b.w, tm), st);
         active = true;
       }
       if (!__any(active)) break;
@@ -362,10 +372,10 @@ __global__ __launch_bounds__(kThreads) void k_shadow(Paths P, const JobDev* __re
       const size_t slot = (size_t)p * P.slots + k;
       if (tv.cur < 0) {
         const float4 a = P.sray[2 * slot], b = P.sray[2 * slot + 1];
-        if (b.w > a.w) tv.start(make_tray(xyz(a), xyz(b), a.w, b.w), st);
+        if (b.w > a.w) tv.start(make_tray(xyz(a), xyz(b), a.w, b.w, tm), st);
       }
       if (tv.cur >= 0) {
-        if (tv.step(nodes, tris, st)) {
+        if (tv.step(nodes, tris, st, motion)) {
           THit h;
           tv.result(&h);
           occ = h.gid != PYRE_MISS;
@@ -477,6 +487,13 @@ __device__ void shade_path(const Paths& P, const JobDev& J, const PassDev& pp, u
   if (kind == 1) {
     mesh = mesh_of(S, gid);
     pos = ro + rd * hv.y;  // Ray::at (math.rs:26-28)
+    if (S.motion) {
+      // a moving mesh's hit: Ray::at on the shifted ray, shifted back
+      // (mesh.rs:118-146)
+      const float4 m0 = S.motion[2 * mesh], m1 = S.motion[2 * mesh + 1];
+      const V3 off = lerp(xyz(m0), xyz(m1), rclamp(P.time[p], 0.0f, 1.0f));
+      if (!(off.x == 0.0f && off.y == 0.0f && off.z == 0.0f)) pos = ((ro - off) + rd * hv.y) + off;
+    }
     ns = surface_normal(S, gid, mesh, hv.z, hv.w);
   } else {
     pos = lpos;
@@ -940,6 +957,7 @@ struct Work {
     P.aov0 = get<float4>(cap);
     P.aov1 = get<float4>(cap);
     P.hit = get<float4>(cap);
+    P.time = get<float>(cap);
     P.sray = get<float4>(2 * cap * slots);
     P.scontrib = get<float4>(cap * slots);
     P.scount = get<uint32_t>(cap);
@@ -985,7 +1003,7 @@ struct Work {
 
 SceneDev scene_dev(pyre_scene* s) {
   return SceneDev{s->d_nodes4, s->d_tris, s->d_verts, s->d_normals, s->d_tri_offset, s->d_mesh_info, s->d_mats,
-                  s->d_lights, s->n_meshes, s->n_lights, s->has_env ? 1u : 0u, s->env};
+                  s->d_lights, s->n_meshes, s->n_lights, s->has_env ? 1u : 0u, s->env, s->d_motion};
 }
 
 JobDev job_dev(pyre_scene* s, const pyre_camera_desc& cam, const pyre_integrator_params& ip, uint32_t w, uint32_t h,
@@ -1017,27 +1035,30 @@ struct PassShape {
   uint32_t max_depth = 0, tree_depth = 0;
   bool nee = false;
   bool one_job = false;  // every path belongs to job 0 (sessions): its tree is read once per kernel
+  bool motion = false;   // some job's scene has moving meshes
 };
 
-template <int RING>
+template <int RING, bool MOTION>
 void launch_trace_stages(const Work& W, Paths P, unsigned tgrid, bool shadow, hipStream_t st, bool extend,
                          bool one_job) {
   if (extend)
-    hipLaunchKernelGGL((k_extend<RING>), dim3(tgrid), dim3(kThreads), 0, st, P, (const JobDev*)W.jobs, W.gstack,
-                       one_job);
+    hipLaunchKernelGGL((k_extend<RING, MOTION>), dim3(tgrid), dim3(kThreads), 0, st, P, (const JobDev*)W.jobs,
+                       W.gstack, one_job);
   if (shadow)
-    hipLaunchKernelGGL((k_shadow<RING>), dim3(tgrid), dim3(kThreads), 0, st, P, (const JobDev*)W.jobs, W.gstack,
-                       one_job);
+    hipLaunchKernelGGL((k_shadow<RING, MOTION>), dim3(tgrid), dim3(kThreads), 0, st, P, (const JobDev*)W.jobs,
+                       W.gstack, one_job);
 }
 
 void launch_trace(const Work& W, const Paths& P, const Tuning& t, unsigned tgrid, bool extend, bool shadow,
-                  hipStream_t st, bool one_job) {
-  if (t.ring == 4)
-    launch_trace_stages<4>(W, P, tgrid, shadow, st, extend, one_job);
+                  hipStream_t st, bool one_job, bool motion) {
+  if (motion)  // scenes with moving meshes: one ring size
+    launch_trace_stages<8, true>(W, P, tgrid, shadow, st, extend, one_job);
+  else if (t.ring == 4)
+    launch_trace_stages<4, false>(W, P, tgrid, shadow, st, extend, one_job);
   else if (t.ring == 16)
-    launch_trace_stages<16>(W, P, tgrid, shadow, st, extend, one_job);
+    launch_trace_stages<16, false>(W, P, tgrid, shadow, st, extend, one_job);
   else
-    launch_trace_stages<8>(W, P, tgrid, shadow, st, extend, one_job);
+    launch_trace_stages<8, false>(W, P, tgrid, shadow, st, extend, one_job);
 }
 
 // All stages of the pass passes[*pass_idx] on the stream; accumulate = false
@@ -1050,12 +1071,12 @@ void launch_pass(pyre_ctx* ctx, Work& W, const PassShape& shape, const Tuning& t
   hipLaunchKernelGGL(k_counters, dim3(1), dim3(kCounters), 0, st, P.counters, sc);
   hipLaunchKernelGGL(k_generate, dim3(grid), dim3(kThreads), 0, st, P, sc);
   for (uint32_t d = 0; d < shape.max_depth; ++d) {
-    launch_trace(W, P, t, tgrid, true, false, st, shape.one_job);
+    launch_trace(W, P, t, tgrid, true, false, st, shape.one_job, shape.motion);
     if (t.shade_waves >= 4)
       hipLaunchKernelGGL(k_shade<4>, dim3(grid), dim3(kThreads), 0, st, P, sc, d);
     else
       hipLaunchKernelGGL(k_shade<1>, dim3(grid), dim3(kThreads), 0, st, P, sc, d);
-    if (shape.nee) launch_trace(W, P, t, tgrid, false, true, st, shape.one_job);
+    if (shape.nee) launch_trace(W, P, t, tgrid, false, true, st, shape.one_job, shape.motion);
     hipLaunchKernelGGL(k_rotate, dim3(1), dim3(kCounters), 0, st, P.counters, P.stats);
     std::swap(P.q_cur, P.q_next);
   }
@@ -1249,6 +1270,7 @@ void session_init(pyre_session* S, pyre_ctx* ctx, pyre_scene* s, const pyre_came
   S->shape.tree_depth = s->depth;
   S->shape.nee = s->n_lights + (s->has_env ? 1u : 0u) > 0;
   S->shape.one_job = true;
+  S->shape.motion = s->has_motion;
   S->acc = dev_alloc<float>(10 * (size_t)S->npix);
   S->acc64 = dev_alloc<double>(3 * (size_t)S->npix);
   S->welford = dev_alloc<float>(3 * (size_t)S->npix);
@@ -1533,6 +1555,7 @@ int pyre_render_batch(pyre_ctx* ctx, const pyre_job* jobs, size_t n, const pyre_
       const uint32_t sl = J.scene->n_lights + (J.scene->has_env ? 1u : 0u);
       B.shape.nee = B.shape.nee || sl > 0;
       B.shape.one_job = n == 1;
+      B.shape.motion = B.shape.motion || J.scene->has_motion;
       B.slots = std::max(B.slots, sl);
     }
     // one allocation for every job's sums

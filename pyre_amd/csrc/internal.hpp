@@ -140,6 +140,10 @@ struct pyre_scene {
   std::vector<pyre::QuadLightDev> h_lights;
   float* d_verts = nullptr;    // 9 floats per scene triangle (v0, v1, v2)
   float* d_normals = nullptr;  // 9 floats per triangle (vertex normals) or null
+  bool has_motion = false;     // some mesh moves over the shutter
+  std::vector<float4> h_motion;  // 2 per mesh: (offset_t0, 0), (offset_t1, 0)
+  std::vector<uint32_t> h_moving;  // per mesh: 1 = moving
+  float4* d_motion = nullptr;    // h_motion on the device (null for static scenes)
   uint32_t* d_tri_offset = nullptr;
   uint32_t* d_mesh_info = nullptr;
   pyre::MatDev* d_mats = nullptr;
@@ -176,10 +180,12 @@ namespace pyre {
 void bvh_build(pyre_scene* s, const pyre_bvh_params& p, hipStream_t st);
 void bvh_release(pyre_scene* s);
 // trace.hip
-// check = synchronise and turn a stack overflow into PYRE_EOVERFLOW.
+// check = synchronise and turn a stack overflow into PYRE_EOVERFLOW. times:
+// per-ray shutter times for scenes with moving meshes (null: time 0).
 void trace_closest(pyre_scene* s, const float4* rays, size_t n, pyre_hit8* hits, float2* uv, hipStream_t st,
-                   bool check);
-void trace_any(pyre_scene* s, const float4* rays, size_t n, uint32_t* occluded, hipStream_t st, bool check);
+                   bool check, const float* times = nullptr);
+void trace_any(pyre_scene* s, const float4* rays, size_t n, uint32_t* occluded, hipStream_t st, bool check,
+               const float* times = nullptr);
 unsigned trace_grid(pyre_scene* s);
 // Spill columns for trace_grid(s) x 256 threads (allocated on first use and
 // whenever the tree gets deeper; never inside a stream capture).

@@ -8,11 +8,13 @@
 #include "../../include/pyre_gen.h"
 
 // One primitive: world-space positions, optional per-vertex normals (empty =
-// face normals), indices and the material id.
+// face normals), indices, the material id and an optional translation over
+// the shutter.
 struct PresetMesh {
   std::vector<float> pos, nrm;
   std::vector<uint32_t> idx;
   uint32_t material = 0;
+  std::vector<float> motion;  // empty, or InstanceMotion {offset_t0, offset_t1}
 };
 
 enum PresetKind : int {

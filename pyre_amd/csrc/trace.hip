@@ -58,20 +58,22 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(const Node2* __restrict
 // Persistent 4-wide traversal with dynamic ray fetch: a lane that finishes
 // its ray takes the next one of its wave's chunk as soon as a quarter of the
 // wave is idle, instead of waiting for the wave's slowest ray.
-template <bool ANY, int RING>
+// MOTION: scenes with moving meshes; times = per-ray shutter times (null: 0).
+template <bool ANY, int RING, bool MOTION = false>
 __global__ __launch_bounds__(kTraceThreads, 8) void k_trace4(const Node4* __restrict__ nodes,
                                                           const float4* __restrict__ tris,
                                                           const float4* __restrict__ rays, uint32_t n,
                                                           pyre_hit8* __restrict__ out_hit,
                                                           float2* __restrict__ out_uv,
                                                           uint32_t* __restrict__ out_occ, int2* gstack,
-                                                          uint32_t* fetch) {
+                                                          uint32_t* fetch, const float* __restrict__ times,
+                                                          const float4* __restrict__ motion) {
   __shared__ int2 ring[RING * kTraceThreads];
   const uint32_t tid = blockIdx.x * kTraceThreads + threadIdx.x;
   const uint32_t nthreads = gridDim.x * kTraceThreads;
   HybridStack<RING> st{ring + threadIdx.x, kTraceThreads, gstack + tid, nthreads, 0, 0};
   WaveFeed<256, 64> feed(n, fetch);
-  Trav4<ANY, RING> tv;
+  Trav4<ANY, RING, MOTION> tv;
   uint32_t ray = 0;
   bool active = false;
   for (;;) {
@@ -79,12 +81,13 @@ __global__ __launch_bounds__(kTraceThreads, 8) void k_trace4(const Node4* __rest
     if (__popcll(idle) >= kRefillLanes) {
       if (feed.take(idle, !active, ray)) {
         const float4 ra = rays[2 * (size_t)ray], rb = rays[2 * (size_t)ray + 1];
-        tv.start(make_tray(v3(ra.x, ra.y, ra.z), v3(rb.x, rb.y, rb.z), ra.w, rb.w), st);
+        const float tm = MOTION && times ? times[ray] : 0.0f;
+        tv.start(make_tray(v3(ra.x, ra.y, ra.z), v3(rb.x, rb.y, rb.z), ra.w, rb.w, tm), st);
         active = true;
       }
       if (!__any(active)) break;  // slice exhausted and every lane idle
     }
-    if (active && tv.step(nodes, tris, st)) {
+    if (active && tv.step(nodes, tris, st, motion)) {
       THit h;
       tv.result(&h);
       if (ANY) {
@@ -149,11 +152,20 @@ bool use_binary() {
 
 template <bool ANY>
 void launch(pyre_scene* s, const float4* rays, size_t n, pyre_hit8* hit, float2* uv, uint32_t* occ, hipStream_t st,
-            bool check) {
+            bool check, const float* times) {
   require(s->built, "trace before pyre_bvh_build");
   require(n <= 0xFFFFFFFFull, "too many rays in one call");
   if (n == 0) return;
   uint32_t* ovf = s->d_flag;
+  if (s->has_motion) {
+    // moving meshes: the persistent 4-wide kernel with shifted leaf tests
+    int2* gst = trace_gstack(s);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, s->d_fetch, 1u);
+    hipLaunchKernelGGL((k_trace4<ANY, 8, true>), dim3(trace_grid(s)), dim3(kTraceThreads), 0, st, s->d_nodes4,
+                       s->d_tris, rays, (uint32_t)n, hit, uv, occ, gst, s->d_fetch, times, s->d_motion);
+    PYRE_HIP_CHECK(hipGetLastError());
+    return;
+  }
   if (!use_binary()) {
     int2* gst = trace_gstack(s);
     const char* fe = std::getenv("PYRE_TRACE_FETCH");
@@ -167,13 +179,13 @@ void launch(pyre_scene* s, const float4* rays, size_t n, pyre_hit8* hit, float2*
     const int ring = trace_ring();
     if (ring == 4)
       hipLaunchKernelGGL((k_trace4<ANY, 4>), dim3(trace_grid(s)), dim3(kTraceThreads), 0, st, s->d_nodes4, s->d_tris,
-                         rays, (uint32_t)n, hit, uv, occ, gst, s->d_fetch);
+                         rays, (uint32_t)n, hit, uv, occ, gst, s->d_fetch, nullptr, nullptr);
     else if (ring == 16)
       hipLaunchKernelGGL((k_trace4<ANY, 16>), dim3(trace_grid(s)), dim3(kTraceThreads), 0, st, s->d_nodes4,
-                         s->d_tris, rays, (uint32_t)n, hit, uv, occ, gst, s->d_fetch);
+                         s->d_tris, rays, (uint32_t)n, hit, uv, occ, gst, s->d_fetch, nullptr, nullptr);
     else
       hipLaunchKernelGGL((k_trace4<ANY, 8>), dim3(trace_grid(s)), dim3(kTraceThreads), 0, st, s->d_nodes4, s->d_tris,
-                         rays, (uint32_t)n, hit, uv, occ, gst, s->d_fetch);
+                         rays, (uint32_t)n, hit, uv, occ, gst, s->d_fetch, nullptr, nullptr);
     PYRE_HIP_CHECK(hipGetLastError());
     return;
   }
@@ -234,12 +246,13 @@ int2* trace_gstack(pyre_scene* s) {
 }
 
 void trace_closest(pyre_scene* s, const float4* rays, size_t n, pyre_hit8* hits, float2* uv, hipStream_t st,
-                   bool check) {
-  launch<false>(s, rays, n, hits, uv, nullptr, st, check);
+                   bool check, const float* times) {
+  launch<false>(s, rays, n, hits, uv, nullptr, st, check, times);
 }
 
-void trace_any(pyre_scene* s, const float4* rays, size_t n, uint32_t* occluded, hipStream_t st, bool check) {
-  launch<true>(s, rays, n, nullptr, nullptr, occluded, st, check);
+void trace_any(pyre_scene* s, const float4* rays, size_t n, uint32_t* occluded, hipStream_t st, bool check,
+               const float* times) {
+  launch<true>(s, rays, n, nullptr, nullptr, occluded, st, check, times);
 }
 
 }  // namespace pyre

@@ -24,14 +24,16 @@ namespace pyre {
 struct TRay {
   V3 o, d, inv;
   float t_min, t_max;
+  float time;  // shutter time; read only by the traversals of scenes with moving meshes
 };
 
-PH TRay make_tray(V3 o, V3 d, float t_min, float t_max) {
+PH TRay make_tray(V3 o, V3 d, float t_min, float t_max, float time = 0.0f) {
   TRay r;
   r.o = o;
   r.d = d;
   r.t_min = t_min;
   r.t_max = t_max;
+  r.time = time;
   r.inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);  // bvh.rs:93
   return r;
 }
@@ -163,24 +165,40 @@ struct HybridStack {
   }
 };
 
+// A moving mesh's ray origin (mesh.rs:109-124): the ray shifted by
+// -lerp(offset_t0, offset_t1, clamp(time, 0, 1)); a zero offset leaves the
+// ray as it is. slot = the mesh's motion slot + 1 (0: static).
+__device__ __forceinline__ V3 motion_origin(const float4* __restrict__ motion, uint32_t slot, const TRay& r) {
+  if (slot == 0) return r.o;
+  const float4 m0 = motion[2 * (slot - 1)], m1 = motion[2 * (slot - 1) + 1];
+  const V3 off = lerp(v3(m0.x, m0.y, m0.z), v3(m1.x, m1.y, m1.z), rclamp(r.time, 0.0f, 1.0f));
+  return (off.x == 0.0f && off.y == 0.0f && off.z == 0.0f) ? r.o : r.o - off;
+}
+
 // Leaf triangles [s, s + cnt): Moller-Trumbore against the current closest.
-template <bool ANY>
+// MOTION: the triangle records carry their mesh's motion slot in e1.w and a
+// moving mesh's triangles are tested with the shifted origin (the boxes
+// above them are swept, so the slab tests use the unshifted ray).
+template <bool ANY, bool MOTION = false>
 __device__ __forceinline__ bool leaf_hits(const float4* __restrict__ tris, const TRay& r, int32_t s, int cnt,
-                                          float& best_t, uint32_t& best_gid, float& bu, float& bv, bool& found) {
+                                          float& best_t, uint32_t& best_gid, float& bu, float& bv, bool& found,
+                                          const float4* __restrict__ motion = nullptr) {
   for (int k = 0; k < cnt; ++k) {
     const float4 a = tris[3 * (size_t)(s + k)];
     const float4 b = tris[3 * (size_t)(s + k) + 1];
     const float4 c = tris[3 * (size_t)(s + k) + 2];
+    V3 o = r.o;
+    if constexpr (MOTION) o = motion_origin(motion, __builtin_bit_cast(uint32_t, b.w), r);
     float t, u, v;
     if (ANY) {
-      if (mt_intersect(r.o, r.d, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), r.t_min, r.t_max, false, &t,
+      if (mt_intersect(o, r.d, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), r.t_min, r.t_max, false, &t,
                        &u, &v)) {
         found = true;
         return true;
       }
     } else {
       // the closest-hit update as selects (no branch on the test's outcome)
-      const bool ok = mt_intersect(r.o, r.d, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), r.t_min, best_t,
+      const bool ok = mt_intersect(o, r.d, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), r.t_min, best_t,
                                    true, &t, &u, &v);
       const uint32_t gid = __builtin_bit_cast(uint32_t, a.w);
       const bool take = ok && (t < best_t || (found && gid < best_gid));
@@ -215,7 +233,7 @@ __device__ __forceinline__ void cswap(float& ka, int32_t& ra, int& ca, float& kb
 // a stacked child whose entry distance lies beyond the closest hit found
 // since it was pushed is dropped without fetching it (the reference's slab
 // test against the tightened t_max would reject it, bvh.rs:146-153).
-template <bool ANY, int SL>
+template <bool ANY, int SL, bool MOTION = false>
 struct Trav4 {
   TRay r;
   float best_t, bu, bv;
@@ -234,9 +252,10 @@ struct Trav4 {
     st.lo = 0;
   }
 
-  // Visits node `cur`; true when the ray is finished.
+  // Visits node `cur`; true when the ray is finished. motion: the scene's
+  // motion table (MOTION only).
   __device__ __forceinline__ bool step(const Node4* __restrict__ nodes, const float4* __restrict__ tris,
-                                       HybridStack<SL>& st) {
+                                       HybridStack<SL>& st, const float4* __restrict__ motion = nullptr) {
     const Node4 nd = nodes[cur];
     // missed children: key +inf and count -1 (a hit child can have key +inf
     // only if its box is at infinity, so the count marks misses)
@@ -258,7 +277,8 @@ struct Trav4 {
     cswap(k1, r1, c1, k2, r2, c2);
     // leaves first, nearest first; a child entered beyond the closest hit
     // found meanwhile is skipped
-    if (leaf(tris, k0, r0, c0) || leaf(tris, k1, r1, c1) || leaf(tris, k2, r2, c2) || leaf(tris, k3, r3, c3))
+    if (leaf(tris, k0, r0, c0, motion) || leaf(tris, k1, r1, c1, motion) || leaf(tris, k2, r2, c2, motion) ||
+        leaf(tris, k3, r3, c3, motion))
       return true;
     // inner children: the nearest becomes the next node, the others are
     // stacked farthest first
@@ -282,9 +302,10 @@ struct Trav4 {
   }
 
   // A hit leaf child: its triangles; true when an any-hit query is answered.
-  __device__ __forceinline__ bool leaf(const float4* __restrict__ tris, float k, int32_t ref, int cnt) {
+  __device__ __forceinline__ bool leaf(const float4* __restrict__ tris, float k, int32_t ref, int cnt,
+                                       const float4* __restrict__ motion) {
     if (cnt <= 0 || k > best_t) return false;
-    return leaf_hits<ANY>(tris, r, ref, cnt, best_t, best_gid, bu, bv, found);
+    return leaf_hits<ANY, MOTION>(tris, r, ref, cnt, best_t, best_gid, bu, bv, found, motion);
   }
 
   __device__ __forceinline__ void inner(HybridStack<SL>& st, int2& next, float k, int32_t ref, int cnt) const {
