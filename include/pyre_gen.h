@@ -5,6 +5,7 @@
  *
  * Restated from the reference CLI:
  *   pyre_preset_cornell    cornell_box()       crates/pyre-cli/src/main.rs:365-496
+ *   pyre_preset_studio     studio_scene()      main.rs:502-562 (+ the default sky, main.rs:167-174)
  *                          quad_mesh()         main.rs:353-360
  *                          make_uv_sphere()    main.rs:565-599
  *   pyre_auto_frame        auto_frame_params() main.rs:337-351
@@ -43,6 +44,15 @@ typedef struct pyre_preset pyre_preset;
 /* The built-in Cornell box; motion_blur != 0 animates the gold sphere from
  * offset (0, 0, 0) to (0, 0.25, 0) over the shutter (main.rs:455-465). */
 int pyre_preset_cornell(int motion_blur, pyre_preset** out);
+
+/* The CLI's studio preset (studio_scene, main.rs:502-562): a 100 x 100
+ * Lambertian 0.55 ground quad and the gold / plastic spheres (64 x 32);
+ * motion_blur != 0 moves the gold sphere from x - 0.15 to x + 0.15. Its sky
+ * is the CLI's default for the studio without an HDRI (main.rs:167-174):
+ * HdriEnvironmentLight::gradient (light.rs:252-272, 128 x 64) at
+ * env_intensity; pyre_preset_set_env_hdr replaces it like --env. Camera
+ * (1.6, 0.5, 2.6) -> (0, 0.05, 0), vfov 38 (main.rs:195-198). */
+int pyre_preset_studio(int motion_blur, float env_intensity, pyre_preset** out);
 
 /* Production scene: n_instances level-`level` icospheres with random
  * similarity transforms (scale U[0.2,1], uniform rotation, translation

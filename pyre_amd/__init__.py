@@ -164,6 +164,16 @@ class Preset:
         return cls(h, lib)
 
     @classmethod
+    def studio(cls, motion_blur: bool = False, env_intensity: float = 1.0) -> "Preset":
+        """studio_scene (main.rs:502-562) with the CLI's default gradient sky."""
+        lib = L.load()
+        h = C.c_void_p()
+        rc = lib.pyre_preset_studio(int(motion_blur), env_intensity, C.byref(h))
+        if rc:
+            raise PyreError(rc, "pyre_preset_studio")
+        return cls(h, lib)
+
+    @classmethod
     def production(cls, seed=3, n_instances=64, level=7, env_w=2048, env_h=1024) -> "Preset":
         lib = L.load()
         h = C.c_void_p()

@@ -119,6 +119,7 @@ SIGNATURES = {
     "pyre_gen_random_triangles": (C.c_int, [C.c_uint64, _u32, _f, _vp]),
     "pyre_gen_random_rays": (C.c_int, [C.c_uint64, _u32, _u32, _vp, _vp]),
     "pyre_preset_cornell": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+    "pyre_preset_studio": (C.c_int, [C.c_int, _f, C.POINTER(_vp)]),
     "pyre_preset_production": (C.c_int, [C.c_uint64, _u32, _u32, _u32, _u32, C.POINTER(_vp)]),
     "pyre_preset_instances": (C.c_int, [C.c_uint64, _u32, _u32, _u32, _u32, _f, C.POINTER(_vp)]),
     "pyre_preset_desc": (C.c_int, [_vp, C.POINTER(SceneDesc)]),

@@ -240,6 +240,44 @@ int pyre_preset_cornell(int motion_blur, pyre_preset** out) {
   return PYRE_OK;
 }
 
+int pyre_preset_studio(int motion_blur, float env_intensity, pyre_preset** out) {
+  if (!out) return PYRE_EINVAL;
+  pyre_preset* p = new (std::nothrow) pyre_preset();
+  if (!p) return PYRE_ENOMEM;
+  p->kind = PRESET_STUDIO;
+  // studio_scene (main.rs:502-562)
+  p->materials.push_back(lambert(splat(0.55f)));
+  const float s = 50.0f;
+  V3 ground[4] = {v3(-s, 0.0f, -s), v3(s, 0.0f, -s), v3(s, 0.0f, s), v3(-s, 0.0f, s)};
+  p->meshes.push_back(quad_mesh(ground, v3(0, 1, 0), 0));
+  p->materials.push_back(disney(v3(1.0f, 0.766f, 0.336f), 1.0f, 0.1f, 0.5f));
+  p->meshes.push_back(make_uv_sphere(v3(0.4f, 0.35f, 0.0f), 0.35f, 64, 32, 1));
+  if (motion_blur) p->meshes.back().motion = {-0.15f, 0.0f, 0.0f, 0.15f, 0.0f, 0.0f};
+  p->materials.push_back(disney(splat(0.72f), 0.0f, 0.2f, 0.5f));
+  p->meshes.push_back(make_uv_sphere(v3(-0.4f, 0.35f, 0.0f), 0.35f, 64, 32, 2));
+  // the CLI's sky for the studio without an HDRI (main.rs:167-174):
+  // HdriEnvironmentLight::gradient (light.rs:252-272), 128 x 64
+  const V3 zenith = v3(0.20f, 0.35f, 0.65f), horizon = v3(0.95f, 0.85f, 0.70f);
+  p->env_w = 128;
+  p->env_h = 64;
+  p->env_intensity = env_intensity;
+  p->env_rgb.resize(3 * 128 * 64);
+  for (uint32_t y = 0; y < 64; ++y) {
+    const float v = ((float)y + 0.5f) / 64.0f;
+    const float cos_theta = 1.0f - 2.0f * v;
+    const V3 c = cos_theta >= 0.0f ? lerp(horizon, zenith, cos_theta) : horizon * rmax(1.0f + cos_theta * 0.5f, 0.0f);
+    for (uint32_t x = 0; x < 128; ++x) {
+      float* q = p->env_rgb.data() + 3 * ((size_t)y * 128 + x);
+      q[0] = c.x;
+      q[1] = c.y;
+      q[2] = c.z;
+    }
+  }
+  preset_finish(p);
+  *out = p;
+  return PYRE_OK;
+}
+
 int pyre_preset_production(uint64_t seed, uint32_t n_instances, uint32_t level, uint32_t env_w, uint32_t env_h,
                            pyre_preset** out) {
   return pyre_preset_instances(seed, n_instances, level, env_w, env_h, 0.0f, out);
@@ -404,6 +442,13 @@ int pyre_preset_camera(const pyre_preset* p, float aspect, float aperture, pyre_
   cam->focus_distance = 0.0f;  // |target - origin|
   if (p->kind == PRESET_CORNELL) {
     cam->origin[2] = 3.0f;
+  } else if (p->kind == PRESET_STUDIO) {  // main.rs:195-198
+    const float o[3] = {1.6f, 0.5f, 2.6f}, t[3] = {0.0f, 0.05f, 0.0f};
+    for (int a = 0; a < 3; ++a) {
+      cam->origin[a] = o[a];
+      cam->target[a] = t[a];
+    }
+    cam->vfov_deg = 38.0f;
   } else if (p->kind == PRESET_INSTANCES) {
     pyre_auto_frame(p->frame_min, p->frame_max, 45.0f, aspect, cam->origin, cam->target);
   } else if (p->view.has_camera) {  // main.rs:184-185

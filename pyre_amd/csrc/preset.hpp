@@ -21,6 +21,7 @@ enum PresetKind : int {
   PRESET_CORNELL = 0,    // the CLI's fixed Cornell camera
   PRESET_INSTANCES = 1,  // auto-framed over the instance bounds
   PRESET_LOADED = 2,     // a scene file: its own camera, else auto-framed over the scene bounds
+  PRESET_STUDIO = 3,     // the CLI's fixed studio camera
 };
 
 struct pyre_preset {

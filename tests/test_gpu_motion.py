@@ -8,7 +8,8 @@ reference); the device keeps one scene-global tree whose boxes for moving
 triangles are swept over the shutter and shifts the origin at the triangle
 test. Hits must match the oracle bit for bit (ids, t, u, v), at times inside
 and outside [0, 1]; renders of the motion-blurred Cornell box are correlated
-with the oracle's like the static ones; batches with moving and static
+with the oracle's like the static ones, and so is the CLI's studio preset
+with its moving sphere and gradient sky; batches with moving and static
 scenes equal single renders.
 """
 import numpy as np
@@ -18,6 +19,7 @@ import pyre_amd
 from tests.helpers import edge_distance, rel_mse
 
 pytestmark = pytest.mark.gpu
+F32 = np.float32
 
 MOTIONS = [None, ((0.0, 0.0, 0.0), (0.3, -0.2, 0.1)), ((-0.2, 0.1, 0.05), (0.1, 0.1, -0.3))]
 
@@ -162,3 +164,36 @@ def test_batch_with_moving_and_static_scenes(ctx):
         ref = f.render(j.scene, j.camera, j.integrator, j.spp, spp_offset=j.spp_offset, accum64=True)
         assert np.array_equal(outs[k].view(np.uint32), ref.view(np.uint32)), k
         assert np.array_equal(accs[k], f.accum64), k
+
+
+def test_studio_preset_with_motion_against_oracle(ctx, oracle):
+    """The CLI's studio preset (main.rs:502-562) with its default gradient sky
+    (light.rs:252-272) and the moving gold sphere, against the oracle built
+    from the same meshes with the oracle's own gradient environment."""
+    print("[motion studio] 48x32 16 spp", flush=True)
+    pre = pyre_amd.Preset.studio(motion_blur=True, env_intensity=1.0)
+    arr = pre.arrays()
+    assert [m["motion"] is not None for m in arr["meshes"]] == [False, True, False]
+    assert (arr["env"]["width"], arr["env"]["height"]) == (128, 64)
+    w, h = 48, 32
+    cam = pre.camera(w / h)
+    assert cam.origin == (F32(1.6), F32(0.5), F32(2.6)) and cam.vfov_deg == 38.0
+    sc = pyre_amd.Scene.from_preset(ctx, pre)
+    sc.build_bvh(bins=12, max_leaf=8)
+    g = pyre_amd.Film(w, h).render(sc, cam, pyre_amd.PathIntegrator(8, 3), 16).reshape(h, w, 10)
+    sc.close()
+    o_sc = oracle.Scene()
+    for m in arr["materials"]:
+        o_sc.add_material(oracle.material(m.kind, m.base_color, m.metallic, m.roughness, m.specular))
+    for m in arr["meshes"]:
+        prim = o_sc.add_mesh(m["positions"], m["indices"], m["normals"], m["material_id"])
+        if m["motion"] is not None:
+            o_sc.set_motion(prim, m["motion"][0], m["motion"][1])
+    o_sc.set_env(oracle.Hdri(gradient=((0.20, 0.35, 0.65), (0.95, 0.85, 0.70)), intensity=1.0))
+    o = o_sc.render(_ocam(oracle, cam), w, h, 16).reshape(h, w, 10)
+    same_alb = np.all(np.abs(g[..., 3:6] - o[..., 3:6]) <= 1e-6, axis=-1).mean()
+    e = rel_mse(g[..., :3], o[..., :3])
+    print(f"  albedo equal {same_alb:.4f} relMSE(gpu, cpu) {e:.2e}", flush=True)
+    assert same_alb > 0.97
+    assert e < 2e-3
+    assert o[..., :3].mean() > 0.05
