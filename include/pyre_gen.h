@@ -13,6 +13,8 @@
  *   pyre_preset_load_pbrt  load_pbrt()         crates/pyre/src/io/pbrt/{mod,lexer,parser,builder}.rs
  *   pyre_preset_load_gltf  load_gltf()         crates/pyre/src/io/gltf.rs:20-92, main.rs:128-141
  *   pyre_load_hdr          load_hdri()         crates/pyre/src/io/hdr.rs:24-50
+ * Film sinks: pyre_film_pack, pyre_write_png, pyre_write_exr -- Film::save_png,
+ * save_aov_pngs, save_exr (crates/pyre/src/film.rs:117-264).
  * Synthetic workloads with no reference counterpart (the reference has no
  * benchmark inputs); all randomness is xoshiro256++ seeded with SplitMix64,
  * transcendental/Gaussian draws are evaluated in double and rounded to f32:
@@ -130,6 +132,18 @@ int pyre_preset_set_env_hdr(pyre_preset* p, const char* path, float intensity, c
 
 /* The file's render settings (zeroed for presets). */
 int pyre_preset_view(const pyre_preset* p, pyre_view* view);
+
+/* Packs a host film (PixelSample means, row-major from the top-left) for
+ * the sinks, on the host. */
+int pyre_film_pack(const pyre_pixel_sample* px, uint32_t width, uint32_t height, uint32_t what, void* out);
+/* An 8-bit grey (channels 1) or RGB (channels 3) PNG of packed bytes. */
+int pyre_write_png(const char* path, uint32_t width, uint32_t height, uint32_t channels, const uint8_t* px,
+                   char* err, size_t err_len);
+/* A single-part scanline OpenEXR of PYRE_PACK_EXR planes: R, G, B and the AOV
+ * groups of aov_mask, FLOAT channels in save_exr's sorted order, stored
+ * uncompressed (the reference writes the same channels RLE-compressed). */
+int pyre_write_exr(const char* path, uint32_t width, uint32_t height, uint32_t aov_mask, const float* planes,
+                   char* err, size_t err_len);
 
 /* auto_frame_params (main.rs:337-351). */
 void pyre_auto_frame(const float bmin[3], const float bmax[3], float vfov_deg, float aspect, float origin[3],

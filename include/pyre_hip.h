@@ -385,6 +385,26 @@ int pyre_session_render_nrc(pyre_ctx* ctx, pyre_session* session, pyre_nrc* nrc,
  * the FP64 radiance sums (3 doubles per pixel), and the samples per pixel. */
 int pyre_session_read(pyre_ctx* ctx, pyre_session* session, pyre_pixel_sample* mean, double* radiance_sum64,
                       uint32_t* spp_done);
+
+/* Film sinks (film.rs:117-264). What pyre_film_pack / pyre_session_pack
+ * produce: */
+#define PYRE_PACK_BEAUTY_PNG 0 /* w*h*3 bytes: clamp(radiance, 0, 1)^(1/2.2) * 255, truncated (save_png) */
+#define PYRE_PACK_ALBEDO_PNG 1 /* w*h*3 bytes: the same of the albedo (<stem>.albedo.png) */
+#define PYRE_PACK_NORMAL_PNG 2 /* w*h*3 bytes: clamp(0.5 * n + 0.5, 0, 1) * 255 (<stem>.normal.png) */
+#define PYRE_PACK_DEPTH_PNG 3  /* w*h bytes: near = bright over the depth range of the hits (<stem>.depth.png) */
+#define PYRE_PACK_EXR 4        /* 10 * w*h floats: the planes B, G, N.X, N.Y, N.Z, R, Z, albedo.B, albedo.G,
+                                  albedo.R (save_exr's channels in its sorted order) */
+/* AOV groups of an EXR (film.rs:229-241; beauty R/G/B is always written). */
+#define PYRE_AOV_ALBEDO 1u
+#define PYRE_AOV_NORMAL 2u
+#define PYRE_AOV_DEPTH 4u
+
+/* The session's film (its means so far) packed on the device for the sinks
+ * of pyre_gen.h: what = PYRE_PACK_*, out = host memory of that size. The
+ * conversions of pyre_film_pack on pyre_session_read's means; the device's
+ * powf may round a gamma-encoded channel one unit differently from the
+ * host's. */
+int pyre_session_pack(pyre_ctx* ctx, pyre_session* session, uint32_t what, void* out);
 /* The viewer's displayed image (viewer.rs:114-184): the per-pixel running
  * mean of sample radiance updated sample by sample as d += (s - d) / n, in
  * f32, 3 floats per pixel. */

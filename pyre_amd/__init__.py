@@ -21,7 +21,7 @@ from ._lib import PyreError
 __all__ = [
     "Context", "Scene", "Preset", "PathIntegrator", "Film", "Camera", "Material", "PyreError",
     "random_triangles", "random_rays", "auto_frame", "MAT_LAMBERTIAN", "MAT_DISNEY", "MISS", "RadianceCache",
-    "load_hdr",
+    "load_hdr", "write_png", "write_exr",
 ]
 
 MAT_LAMBERTIAN = L.MAT_LAMBERTIAN
@@ -460,7 +460,54 @@ def bsdf_sample(ctx: Context, mat: Material, wo, u):
     return wi, f, pdf, ok.astype(bool)
 
 
-class Film:
+def _pack_shape(what, w, h):
+    if what == L.PACK_EXR:
+        return (10, h, w), np.float32
+    return ((h, w) if what == L.PACK_DEPTH_PNG else (h, w, 3)), np.uint8
+
+
+def write_png(path, pixels) -> None:
+    """An 8-bit PNG of (h, w) grey or (h, w, 3) RGB bytes (pyre_write_png)."""
+    px = np.ascontiguousarray(pixels, np.uint8)
+    err = C.create_string_buffer(512)
+    ch = 1 if px.ndim == 2 else 3
+    rc = L.load().pyre_write_png(os.fsencode(path), px.shape[1], px.shape[0], ch, _ptr(px), err, 512)
+    if rc:
+        raise PyreError(rc, f"pyre_write_png({path}): {err.value.decode(errors='replace')}")
+
+
+def write_exr(path, planes, albedo=True, normal=True, depth=True) -> None:
+    """An OpenEXR of PACK_EXR planes (10, h, w): R, G, B and the chosen AOV
+    channels in save_exr's sorted order (pyre_write_exr)."""
+    pl = np.ascontiguousarray(planes, np.float32)
+    mask = (L.AOV_ALBEDO if albedo else 0) | (L.AOV_NORMAL if normal else 0) | (L.AOV_DEPTH if depth else 0)
+    err = C.create_string_buffer(512)
+    rc = L.load().pyre_write_exr(os.fsencode(path), pl.shape[2], pl.shape[1], mask, _ptr(pl), err, 512)
+    if rc:
+        raise PyreError(rc, f"pyre_write_exr({path}): {err.value.decode(errors='replace')}")
+
+
+class _Sinks:
+    """Film::save_png / save_aov_pngs / save_exr (film.rs:117-264) over
+    self.pack(what)."""
+
+    def save_png(self, path) -> None:
+        write_png(path, self.pack(L.PACK_BEAUTY_PNG))
+
+    def save_aov_pngs(self, base_path, albedo=True, normal=True, depth=True) -> None:
+        """<stem>.albedo.png, <stem>.normal.png, <stem>.depth.png beside base_path."""
+        d = os.path.dirname(os.fspath(base_path)) or "."
+        stem = os.path.splitext(os.path.basename(os.fspath(base_path)))[0] or "render"
+        for on, what, name in ((albedo, L.PACK_ALBEDO_PNG, "albedo"), (normal, L.PACK_NORMAL_PNG, "normal"),
+                               (depth, L.PACK_DEPTH_PNG, "depth")):
+            if on:
+                write_png(os.path.join(d, f"{stem}.{name}.png"), self.pack(what))
+
+    def save_exr(self, path, albedo=True, normal=True, depth=True) -> None:
+        write_exr(path, self.pack(L.PACK_EXR), albedo, normal, depth)
+
+
+class Film(_Sinks):
     """Film (film.rs:38-109): per-pixel PixelSample means, row-major, (0,0) top-left."""
 
     def __init__(self, width: int, height: int):
@@ -480,6 +527,16 @@ class Film:
                                               self.height, spp, spp_offset, _ptr(self.samples),
                                               _ptr(self.accum64) if accum64 else None), "pyre_render")
         return self.samples
+
+    def pack(self, what: int) -> np.ndarray:
+        """The film packed for a sink on the host (pyre_film_pack)."""
+        shape, dt = _pack_shape(what, self.width, self.height)
+        out = np.empty(shape, dt)
+        px = np.ascontiguousarray(self.samples, np.float32)
+        rc = L.load().pyre_film_pack(_ptr(px), self.width, self.height, what, _ptr(out))
+        if rc:
+            raise PyreError(rc, "pyre_film_pack")
+        return out
 
 
 def random_triangles(seed: int, n: int, sigma: float = 0.01) -> np.ndarray:
@@ -588,7 +645,7 @@ class RadianceCache:
             pass
 
 
-class Session:
+class Session(_Sinks):
     """Progressive render session (pyre_session_*): the main.rs:253-287 loop
     with device-resident sums; samples s = 0, 1, 2, ... per pixel, so N spp
     equal Film.render(spp=N). nrc_train_paths > 0 enables render_nrc."""
@@ -630,6 +687,13 @@ class Session:
                                                   _ptr(r) if radiance64 else None, C.byref(spp)),
                        "pyre_session_read")
         return m, r, int(spp.value)
+
+    def pack(self, what: int) -> np.ndarray:
+        """The session's means packed for a sink on the device (pyre_session_pack)."""
+        shape, dt = _pack_shape(what, self.width, self.height)
+        out = np.empty(shape, dt)
+        self.ctx.check(self.lib.pyre_session_pack(self.ctx.h, self.h, what, _ptr(out)), "pyre_session_pack")
+        return out
 
     def read_viewer(self):
         """The viewer's running-mean radiance (npix, 3) f32 (viewer.rs:165-177)."""

@@ -18,6 +18,8 @@ PYRE_ENOMEM = -3
 PYRE_EOVERFLOW = -4
 PYRE_ESTATE = -5
 PYRE_EIO = -6
+PACK_BEAUTY_PNG, PACK_ALBEDO_PNG, PACK_NORMAL_PNG, PACK_DEPTH_PNG, PACK_EXR = 0, 1, 2, 3, 4
+AOV_ALBEDO, AOV_NORMAL, AOV_DEPTH = 1, 2, 4
 PYRE_MISS = 0xFFFFFFFF
 MAT_LAMBERTIAN = 0
 MAT_DISNEY = 1
@@ -132,6 +134,10 @@ SIGNATURES = {
                                 C.c_size_t]),
     "pyre_preset_set_env_hdr": (C.c_int, [_vp, C.c_char_p, _f, C.c_char_p, C.c_size_t]),
     "pyre_preset_view": (C.c_int, [_vp, C.POINTER(View)]),
+    "pyre_film_pack": (C.c_int, [_vp, _u32, _u32, _u32, _vp]),
+    "pyre_write_png": (C.c_int, [C.c_char_p, _u32, _u32, _u32, _vp, C.c_char_p, C.c_size_t]),
+    "pyre_write_exr": (C.c_int, [C.c_char_p, _u32, _u32, _u32, _vp, C.c_char_p, C.c_size_t]),
+    "pyre_session_pack": (C.c_int, [_vp, _vp, _u32, _vp]),
 }
 
 _lib = None

@@ -19,7 +19,7 @@ LIB = os.path.join(HERE, "libpyre_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PYRE_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["api.hip", "bvh_build.hip", "trace.hip", "integrator.hip", "nrc.hip", "gen.cpp", "scene_io.cpp"]
+SOURCES = ["api.hip", "bvh_build.hip", "trace.hip", "integrator.hip", "nrc.hip", "gen.cpp", "scene_io.cpp", "film_io.cpp"]
 HEADERS = ["internal.hpp", "pmath.hpp", "shade.hpp", "traverse.hpp", "nrc.hpp", "preset.hpp"]
 INCLUDE_HEADERS = ["pyre_hip.h", "pyre_gen.h"]
 

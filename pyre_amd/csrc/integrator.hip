@@ -836,6 +836,63 @@ __global__ void k_scale(const float* acc, size_t n, float inv_spp, float* out) {
   if (i < n) out[i] = acc[i] * inv_spp;
 }
 
+// Film sinks on the device (film.rs:117-264) from a film's sums: the mean is
+// sum * inv_spp as for pyre_session_read, then save_png's / save_aov_pngs' /
+// save_exr's conversions.
+__device__ __forceinline__ uint8_t u8_sat(float x) {  // Rust's `f32 as u8`
+  if (!(x > 0.0f)) return 0;
+  if (x >= 255.0f) return 255;
+  return (uint8_t)x;
+}
+
+__device__ __forceinline__ uint8_t gamma8(float x) {
+  return u8_sat(powf(rmin(rmax(x, 0.0f), 1.0f), 1.0f / 2.2f) * 255.0f);
+}
+
+// depth range over the hits (depth > 0): positive floats order like their bits
+__global__ void k_depth_range(const float* acc, uint32_t npix, float inv_spp, uint32_t* range) {
+  uint32_t lo = 0x7F800000u, hi = 0u;  // +inf, 0
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += gridDim.x * blockDim.x) {
+    const float d = acc[10 * (size_t)i + 9] * inv_spp;
+    if (d > 0.0f) {
+      const uint32_t b = __builtin_bit_cast(uint32_t, d);
+      lo = min(lo, b);
+      hi = max(hi, b);
+    }
+  }
+  for (int k = 32; k > 0; k >>= 1) {
+    lo = min(lo, (uint32_t)__shfl_xor((int)lo, k));
+    hi = max(hi, (uint32_t)__shfl_xor((int)hi, k));
+  }
+  if (__lane_id() == 0) {
+    atomicMin(&range[0], lo);
+    atomicMax(&range[1], hi);
+  }
+}
+
+__global__ void k_pack(const float* acc, uint32_t npix, float inv_spp, uint32_t what, const uint32_t* range,
+                       uint8_t* o8, float* of) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npix) return;
+  float m[10];
+  for (int k = 0; k < 10; ++k) m[k] = acc[10 * (size_t)i + k] * inv_spp;
+  if (what == PYRE_PACK_BEAUTY_PNG || what == PYRE_PACK_ALBEDO_PNG) {
+    const int b = what == PYRE_PACK_BEAUTY_PNG ? 0 : 3;
+    for (int c = 0; c < 3; ++c) o8[3 * (size_t)i + c] = gamma8(m[b + c]);
+  } else if (what == PYRE_PACK_NORMAL_PNG) {
+    for (int c = 0; c < 3; ++c) o8[3 * (size_t)i + c] = u8_sat(rmin(rmax(m[6 + c] * 0.5f + 0.5f, 0.0f), 1.0f) * 255.0f);
+  } else if (what == PYRE_PACK_DEPTH_PNG) {
+    const float lo = range[0] == 0x7F800000u ? kInf : __builtin_bit_cast(float, range[0]);
+    const float hi = __builtin_bit_cast(float, range[1]);
+    const float r = rmax(hi - lo, 1e-6f);
+    const float d = m[9];
+    o8[i] = u8_sat((d > 0.0f ? rmin(rmax(1.0f - (d - lo) / r, 0.0f), 1.0f) : 0.0f) * 255.0f);
+  } else {
+    const float v[10] = {m[2], m[1], m[6], m[7], m[8], m[0], m[9], m[5], m[4], m[3]};
+    for (int k = 0; k < 10; ++k) of[(size_t)k * npix + i] = v[k];
+  }
+}
+
 __global__ void k_fill_f32(float* a, size_t n, float v) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = v;
@@ -1726,6 +1783,33 @@ int pyre_session_read(pyre_ctx* ctx, pyre_session* S, pyre_pixel_sample* mean, d
                                     hipMemcpyDeviceToHost, st));
     PYRE_HIP_CHECK(hipStreamSynchronize(st));
     if (spp_done) *spp_done = S->spp_done;
+  });
+}
+
+int pyre_session_pack(pyre_ctx* ctx, pyre_session* S, uint32_t what, void* out) {
+  if (!ctx || !S || !out || what > PYRE_PACK_EXR) return PYRE_EINVAL;
+  return sguard(ctx, [&] {
+    hipStream_t st = ctx->stream;
+    const float inv = S->spp_done ? 1.0f / (float)S->spp_done : 0.0f;
+    const size_t bytes = what == PYRE_PACK_EXR ? 40 * (size_t)S->npix
+                         : what == PYRE_PACK_DEPTH_PNG ? (size_t)S->npix : 3 * (size_t)S->npix;
+    uint8_t* buf = dev_alloc<uint8_t>(bytes);
+    uint32_t* range = dev_alloc<uint32_t>(2);
+    const uint32_t init[2] = {0x7F800000u, 0u};
+    hipError_t e = hipMemcpyAsync(range, init, sizeof(init), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && what == PYRE_PACK_DEPTH_PNG)
+      hipLaunchKernelGGL(k_depth_range, dim3(std::min<size_t>(nblocks(S->npix, 256), 1024)), dim3(256), 0, st, S->acc,
+                         S->npix, inv, range);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_pack, dim3(nblocks(S->npix, 256)), dim3(256), 0, st, S->acc, S->npix, inv, what, range,
+                         buf, (float*)buf);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out, buf, bytes, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    dev_free(buf);
+    dev_free(range);
+    PYRE_HIP_CHECK(e);
   });
 }
 
