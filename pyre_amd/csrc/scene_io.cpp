@@ -540,7 +540,7 @@ HdrImage decode_radiance(const std::vector<uint8_t>& b) {
   if (std::sscanf(s.c_str(), "%7s %lld %7s %lld %c", ya, &hh, xa, &ww, &tail) != 4 || std::strcmp(ya, "-Y") ||
       std::strcmp(xa, "+X"))
     hdr_err("unsupported dimensions line " + dbg(s) + " (only -Y H +X W)");
-  if (hh <= 0 || ww <= 0 || hh > 65535 || ww > 65535) hdr_err("image dimensions out of range");
+  if (hh <= 0 || ww <= 0 || hh > 65535 || ww > 65535 || hh * ww > (1ll << 28)) hdr_err("image dimensions out of range");
   HdrImage img;
   img.w = (uint32_t)ww;
   img.h = (uint32_t)hh;
@@ -1256,7 +1256,10 @@ struct Gltf {
     const Json* nz = a.get("normalized");
     r.normalized = nz && nz->kind == Json::BOOL && nz->b;
     const Json* bvj = a.get("bufferView");
-    if (!bvj) return r;
+    if (!bvj) {  // all zeros
+      if (r.count > (1u << 26)) gltf_err("accessor without bufferView is too large");
+      return r;
+    }
     const std::vector<Json>& views = arr(doc, "bufferViews");
     const Json& bv = views[index_of(bvj, views.size(), "bufferView")];
     const std::vector<uint8_t>& buf = buffers[index_of(bv.get("buffer"), buffers.size(), "buffer")];
@@ -1331,8 +1334,12 @@ void normal_matrix(const M4& x, V3 col[3]) {
   col[2] = t2 * inv_det;
 }
 
-void walk_node(const Gltf& g, size_t ni, const M4& parent, pyre_preset* p, int depth) {
-  if (depth > 64) gltf_err("node hierarchy too deep (cycle?)");
+void walk_node(const Gltf& g, size_t ni, const M4& parent, pyre_preset* p, std::vector<uint8_t>& seen, int depth) {
+  // glTF nodes form a forest: a node reached twice (two parents, a cycle)
+  // is an invalid document
+  if (seen[ni]) gltf_err("node " + std::to_string(ni) + " has more than one parent");
+  if (depth > 4096) gltf_err("node hierarchy deeper than 4096");
+  seen[ni] = 1;
   const std::vector<Json>& nodes = arr(g.doc, "nodes");
   const Json& n = nodes[ni];
   const M4 xf = m4_mul(parent, node_matrix(n));
@@ -1380,7 +1387,7 @@ void walk_node(const Gltf& g, size_t ni, const M4& parent, pyre_preset* p, int d
     }
   }
   if (const Json* ch = n.get("children"))
-    for (const Json& c : ch->a) walk_node(g, index_of(&c, nodes.size(), "node"), xf, p, depth + 1);
+    for (const Json& c : ch->a) walk_node(g, index_of(&c, nodes.size(), "node"), xf, p, seen, depth + 1);
 }
 
 void load_gltf(const std::string& path, pyre_preset* p) {
@@ -1439,7 +1446,9 @@ void load_gltf(const std::string& path, pyre_preset* p) {
   size_t si = 0;
   if (const Json* s = g.doc.get("scene")) si = index_of(s, scenes.size(), "scene");
   const std::vector<Json>& nodes = arr(g.doc, "nodes");
-  for (const Json& n : arr(scenes[si], "nodes")) walk_node(g, index_of(&n, nodes.size(), "node"), m4_identity(), p, 0);
+  std::vector<uint8_t> seen(nodes.size(), 0);
+  for (const Json& n : arr(scenes[si], "nodes"))
+    walk_node(g, index_of(&n, nodes.size(), "node"), m4_identity(), p, seen, 0);
 }
 
 void put_err(const LoadError& e, char* err, size_t err_len) {

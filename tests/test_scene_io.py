@@ -456,3 +456,8 @@ def test_gltf_errors(tmp_path):
         pyre_amd.Preset.from_gltf(_write(tmp_path, "b.gltf", json.dumps(doc)))
     with pytest.raises(PyreError, match="invalid JSON"):
         pyre_amd.Preset.from_gltf(_write(tmp_path, "c.gltf", "{\"asset\": "))
+    # a node with two parents (or a cycle) is not a tree
+    doc, blob = _gltf_doc(pos, pos, np.array([0, 1, 2], np.uint16), [{"children": [1, 1]}, {"mesh": 0}], [0])
+    doc["buffers"][0]["uri"] = "b.bin"
+    with pytest.raises(PyreError, match="more than one parent"):
+        pyre_amd.Preset.from_gltf(_write(tmp_path, "d.gltf", json.dumps(doc)))
