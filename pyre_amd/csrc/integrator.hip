@@ -1095,27 +1095,44 @@ struct PassShape {
   bool motion = false;   // some job's scene has moving meshes
 };
 
-template <int RING, bool MOTION>
-void launch_trace_stages(const Work& W, Paths P, unsigned tgrid, bool shadow, hipStream_t st, bool extend,
-                         bool one_job) {
-  if (extend)
-    hipLaunchKernelGGL((k_extend<RING, MOTION>), dim3(tgrid), dim3(kThreads), 0, st, P, (const JobDev*)W.jobs,
-                       W.gstack, one_job);
-  if (shadow)
-    hipLaunchKernelGGL((k_shadow<RING, MOTION>), dim3(tgrid), dim3(kThreads), 0, st, P, (const JobDev*)W.jobs,
-                       W.gstack, one_job);
+// Blocks of a persistent stage kernel that fit on one CU at once (its
+// registers and LDS), at most `want`: a block that is not resident from the
+// start would run its static share of the queue after everyone else.
+template <class K>
+unsigned resident_blocks(K kernel, unsigned want) {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, kThreads, 0) != hipSuccess || n <= 0) return want;
+  return std::min<unsigned>(want, (unsigned)n);
 }
 
+template <int RING, bool MOTION>
+void launch_trace_stages(const Work& W, Paths P, unsigned cus, unsigned bpc, bool shadow, hipStream_t st,
+                         bool extend, bool one_job) {
+  if (extend) {
+    static const unsigned occ = resident_blocks(k_extend<RING, MOTION>, 64);
+    hipLaunchKernelGGL((k_extend<RING, MOTION>), dim3(cus * std::min(bpc, occ)), dim3(kThreads), 0, st, P,
+                       (const JobDev*)W.jobs, W.gstack, one_job);
+  }
+  if (shadow) {
+    static const unsigned occ = resident_blocks(k_shadow<RING, MOTION>, 64);
+    hipLaunchKernelGGL((k_shadow<RING, MOTION>), dim3(cus * std::min(bpc, occ)), dim3(kThreads), 0, st, P,
+                       (const JobDev*)W.jobs, W.gstack, one_job);
+  }
+}
+
+// tgrid = CUs x t.blocks_per_cu (the spill columns are sized for it); each
+// stage kernel runs on as many of those blocks per CU as are resident at once.
 void launch_trace(const Work& W, const Paths& P, const Tuning& t, unsigned tgrid, bool extend, bool shadow,
                   hipStream_t st, bool one_job, bool motion) {
+  const unsigned bpc = std::max(1u, t.blocks_per_cu), cus = std::max(1u, tgrid / bpc);
   if (motion)  // scenes with moving meshes: one ring size
-    launch_trace_stages<8, true>(W, P, tgrid, shadow, st, extend, one_job);
+    launch_trace_stages<8, true>(W, P, cus, bpc, shadow, st, extend, one_job);
   else if (t.ring == 4)
-    launch_trace_stages<4, false>(W, P, tgrid, shadow, st, extend, one_job);
+    launch_trace_stages<4, false>(W, P, cus, bpc, shadow, st, extend, one_job);
   else if (t.ring == 16)
-    launch_trace_stages<16, false>(W, P, tgrid, shadow, st, extend, one_job);
+    launch_trace_stages<16, false>(W, P, cus, bpc, shadow, st, extend, one_job);
   else
-    launch_trace_stages<8, false>(W, P, tgrid, shadow, st, extend, one_job);
+    launch_trace_stages<8, false>(W, P, cus, bpc, shadow, st, extend, one_job);
 }
 
 // All stages of the pass passes[*pass_idx] on the stream; accumulate = false
