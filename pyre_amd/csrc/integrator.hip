@@ -1138,23 +1138,27 @@ void launch_trace(const Work& W, const Paths& P, const Tuning& t, unsigned tgrid
 // All stages of the pass passes[*pass_idx] on the stream; accumulate = false
 // skips the film update.
 void launch_pass(pyre_ctx* ctx, Work& W, const PassShape& shape, const Tuning& t, bool accumulate, hipStream_t st) {
-  const unsigned grid = stage_grid(ctx);
+  // grid-stride stages: as many blocks per CU as are resident at once (a
+  // second, partial round of blocks would leave most of the GPU idle)
+  const unsigned cus = (unsigned)std::max(1, ctx->cus);
+  static const unsigned gen_bpc = resident_blocks(k_generate, 8), acc_bpc = resident_blocks(k_accumulate, 8),
+                        shade1_bpc = resident_blocks(k_shade<1>, 8), shade4_bpc = resident_blocks(k_shade<4>, 8);
   const unsigned tgrid = trace_grid_of(ctx, t);
   const Sched sc = W.sched();
   Paths P = W.P;
   hipLaunchKernelGGL(k_counters, dim3(1), dim3(kCounters), 0, st, P.counters, sc);
-  hipLaunchKernelGGL(k_generate, dim3(grid), dim3(kThreads), 0, st, P, sc);
+  hipLaunchKernelGGL(k_generate, dim3(cus * gen_bpc), dim3(kThreads), 0, st, P, sc);
   for (uint32_t d = 0; d < shape.max_depth; ++d) {
     launch_trace(W, P, t, tgrid, true, false, st, shape.one_job, shape.motion);
     if (t.shade_waves >= 4)
-      hipLaunchKernelGGL(k_shade<4>, dim3(grid), dim3(kThreads), 0, st, P, sc, d);
+      hipLaunchKernelGGL(k_shade<4>, dim3(cus * shade4_bpc), dim3(kThreads), 0, st, P, sc, d);
     else
-      hipLaunchKernelGGL(k_shade<1>, dim3(grid), dim3(kThreads), 0, st, P, sc, d);
+      hipLaunchKernelGGL(k_shade<1>, dim3(cus * shade1_bpc), dim3(kThreads), 0, st, P, sc, d);
     if (shape.nee) launch_trace(W, P, t, tgrid, false, true, st, shape.one_job, shape.motion);
     hipLaunchKernelGGL(k_rotate, dim3(1), dim3(kCounters), 0, st, P.counters, P.stats);
     std::swap(P.q_cur, P.q_next);
   }
-  if (accumulate) hipLaunchKernelGGL(k_accumulate, dim3(grid), dim3(kThreads), 0, st, P, sc);
+  if (accumulate) hipLaunchKernelGGL(k_accumulate, dim3(cus * acc_bpc), dim3(kThreads), 0, st, P, sc);
 }
 
 bool graphs_enabled() {
@@ -1266,7 +1270,8 @@ void session_pass(pyre_session* S, uint32_t s_base, uint32_t n_samples, uint32_t
     auto trace = [&] {
       launch_pass(S->ctx, S->W, S->shape, S->tune, false, st);
       if (S->W.vmax) {
-        hipLaunchKernelGGL(k_train_targets, dim3(stage_grid(S->ctx)), dim3(kThreads), 0, st, S->W.P, S->W.sched(),
+        static const unsigned tt_bpc = resident_blocks(k_train_targets, 8);
+        hipLaunchKernelGGL(k_train_targets, dim3((unsigned)std::max(1, S->ctx->cus) * tt_bpc), dim3(kThreads), 0, st, S->W.P, S->W.sched(),
                            S->W.tfeat, S->W.ttarget, S->W.tcap);
         hipLaunchKernelGGL(k_count_train, dim3(1), dim3(1), 0, st, S->W.P);
       }
@@ -1274,8 +1279,9 @@ void session_pass(pyre_session* S, uint32_t s_base, uint32_t n_samples, uint32_t
     auto finish = [&] {
       nrc_infer_dev_count(nrc, S->W.P.qfeat, S->W.P.qpath, S->W.P.counters + 3, (uint32_t)S->W.cap, S->W.P.qrgb,
                           st);
-      hipLaunchKernelGGL(k_cache_apply, dim3(stage_grid(S->ctx)), dim3(kThreads), 0, st, S->W.P);
-      hipLaunchKernelGGL(k_accumulate, dim3(stage_grid(S->ctx)), dim3(kThreads), 0, st, S->W.P, S->W.sched());
+      static const unsigned ap_bpc = resident_blocks(k_cache_apply, 8), acc_bpc = resident_blocks(k_accumulate, 8);
+      hipLaunchKernelGGL(k_cache_apply, dim3((unsigned)std::max(1, S->ctx->cus) * ap_bpc), dim3(kThreads), 0, st, S->W.P);
+      hipLaunchKernelGGL(k_accumulate, dim3((unsigned)std::max(1, S->ctx->cus) * acc_bpc), dim3(kThreads), 0, st, S->W.P, S->W.sched());
     };
     if (!S->use_graph) {
       trace();
