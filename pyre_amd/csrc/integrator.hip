@@ -915,7 +915,7 @@ struct Tuning {
   uint32_t pool_paths = 1u << 24;
   uint32_t blocks_per_cu = 8;
   uint32_t ring = 8;
-  uint32_t shade_waves = 1;  // >= 4: shade kernel compiled for 4 waves per SIMD
+  uint32_t shade_waves = 4;  // >= 4: shade kernel compiled for 4 waves per SIMD (1: no occupancy target)
 };
 
 Tuning env_tuning() {
