@@ -108,11 +108,11 @@ def test_ray_counts_match_reference_queries(ctx, oracle):
     assert st["n_node_extension"] > 1 and st["n_tri_extension"] > 0
 
 
-@pytest.mark.parametrize("switch", [dict(PYRE_SHADE_WAVES=4), dict(PYRE_STACK_RING=4, PYRE_TRACE_BLOCKS_PER_CU=4),
+@pytest.mark.parametrize("switch", [dict(PYRE_SHADE_WAVES=1), dict(PYRE_STACK_RING=4, PYRE_TRACE_BLOCKS_PER_CU=4),
                                     dict(PYRE_STACK_RING=16), dict(PYRE_PASS_PATHS=5000)])
 def test_render_tuning_switches_same_bits(ctx, switch):
-    """Tuning switches (the shade kernel built for 4 waves per SIMD, stack
-    rings, traversal grid, pool size) change speed only: the film is
+    """Tuning switches (the shade kernel built without an occupancy target,
+    stack rings, traversal grid, pool size) change speed only: the film is
     bit-identical to the default render."""
     pre, sc = _cornell(ctx)
     cam = pre.camera(1.0)
