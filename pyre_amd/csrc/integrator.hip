@@ -339,7 +339,6 @@ __global__ __launch_bounds__(kThreads) void k_shadow(Paths P, const JobDev* __re
   const float4* tris = one_job ? jobs[0].S.tris : nullptr;
   const float4* motion = MOTION && one_job ? jobs[0].S.motion : nullptr;
   float tm = 0.0f;  // the path's shutter time (scene.rs:102-140 take ray.time)
-  float4 L = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   bool active = false;
   for (;;) {
     const uint64_t idle = __ballot(!active);
@@ -355,7 +354,6 @@ __global__ __launch_bounds__(kThreads) void k_shadow(Paths P, const JobDev* __re
         if (MOTION) tm = motion ? P.time[p] : 0.0f;
         cnt = P.scount[p];
         k = 0;
-        L = P.lum[p];
         tv.cur = -1;
         // the first slot's ray with the path's data (one memory round trip)
         const float4 a = P.sray[2 * (size_t)p * P.slots], b = P.sray[2 * (size_t)p * P.slots + 1];
@@ -387,18 +385,19 @@ __global__ __launch_bounds__(kThreads) void k_shadow(Paths P, const JobDev* __re
       }
       if (traced) {
         if (!occ) {
+          // the lane owns path p: its radiance is updated in memory, slot by
+          // slot in order, rather than held in registers for the whole path
           const float4 cc = P.scontrib[slot];
+          float4 L = P.lum[p];
           L.x = L.x + cc.x;
           L.y = L.y + cc.y;
           L.z = L.z + cc.z;
+          P.lum[p] = L;
         }
         ++k;
       }
     }
-    if (k >= cnt) {
-      P.lum[p] = L;
-      active = false;
-    }
+    if (k >= cnt) active = false;
   }
 }
 
