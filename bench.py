@@ -171,6 +171,21 @@ def production_scene(ctx):
     return pre, scene, cam, info
 
 
+def nrc_view(ctx, name):
+    """The scene of the cache comparison and of its reference: the production
+    view (depth 12), or the Cornell preset at the same 1920x1080 (depth 8;
+    a closed, indirect-lit box, where a cache that takes over after the
+    first diffuse bounce has most to gain)."""
+    import pyre_amd
+    if name == "cornell":
+        pre = pyre_amd.Preset.cornell()
+        scene = pyre_amd.Scene.from_preset(ctx, pre)
+        scene.build_bvh()
+        return scene, pre.camera(PROD_W / PROD_H), pyre_amd.PathIntegrator(8, 3)
+    pre, scene, cam, info = production_scene(ctx)
+    return scene, cam, pyre_amd.PathIntegrator(12, 3)
+
+
 def rel_mse(img, ref):
     img = np.asarray(img, np.float64)
     ref = np.asarray(ref, np.float64)
@@ -276,8 +291,8 @@ def make_reference(args):
     samples per pixel, saved as float32 radiance (H, W, 3) .npy."""
     import pyre_amd
     ctx = pyre_amd.Context(0)
-    pre, scene, cam, info = production_scene(ctx)
-    sess = pyre_amd.Session(scene, cam, pyre_amd.PathIntegrator(12, 3), PROD_W, PROD_H)
+    scene, cam, ip = nrc_view(ctx, args.nrc_scene)
+    sess = pyre_amd.Session(scene, cam, ip, PROD_W, PROD_H)
     t0 = time.perf_counter()
     done = 0
     while done < args.ref_spp:
@@ -411,8 +426,7 @@ def bench_nrc(args):
         sys.exit(2)
     ctx = pyre_amd.Context(0)
     dev_name, _ = ctx.device_info()
-    pre, scene, cam, info = production_scene(ctx)
-    ip = pyre_amd.PathIntegrator(12, 3)
+    scene, cam, ip = nrc_view(ctx, args.nrc_scene)
     lo, hi = scene.bounds()
     target = 0.01
     checkpoints = [c for c in (4, 6, 8, 10, 12, 16, 24, 32, 48, 64, 96, 128) if c <= args.max_spp]
@@ -461,7 +475,8 @@ def bench_nrc(args):
         "unit": "s (NRC)", "n_gpus": 1, "steps": 1, "warmup": 0, "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32 (cache: bf16 MFMA, f32 accumulate)",
         "data": "synthetic (procedural scene, seed 3)",
-        "config": {"workload": "production scene 1920x1080, max depth 12, thin lens 0.05; NRC 5x64 ReLU MLP, "
+        "config": {"workload": ("production scene 1920x1080, max depth 12, thin lens 0.05" if args.nrc_scene == "production"
+                                else "Cornell preset 1920x1080, max depth 8, pinhole") + "; NRC 5x64 ReLU MLP, "
                                "2^20 training paths/frame (~2^18 samples, Adam batches of 2^14), lr 1e-2, "
                                f"{args.nrc_warmup} training-only frames, frames of min(N, 16) spp; "
                                "pool 2^25 paths for both; every checkpoint rendered from a cold start",
@@ -607,6 +622,8 @@ def main():
                     help="spp of the FP64 references (0 = skip the relMSE figures of cornell / batch)")
     ap.add_argument("--max-spp", type=int, default=128, help="nrc: largest spp tried")
     ap.add_argument("--nrc-warmup", type=int, default=0, help="nrc: training-only frames before the first sample")
+    ap.add_argument("--nrc-scene", default="production", choices=["production", "cornell"],
+                    help="nrc / --make-reference: the view of the cache comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
