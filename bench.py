@@ -626,8 +626,9 @@ def main():
                     help="nrc / --make-reference: the view of the cache comparison")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(rays and pixels shard with no exchange; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path here "
+              "(pixel samples are independent, so rays and pixels would shard across GPUs with no exchange)",
+              file=sys.stderr)
         sys.exit(2)
     if args.make_reference:
         return make_reference(args)

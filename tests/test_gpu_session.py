@@ -113,7 +113,7 @@ def test_nrc_render_trains_and_terminates(ctx):
                              train_steps=4, warmup_frames=20)
     print(f"[nrc render] losses first {losses[:5]} last {losses[-5:]}", flush=True)
     # single-path targets are noisy (relative L2 stays near their variance), so
-    # the loss is only checked to be finite; the gain test judges the cache
+    # the loss is only checked to be finite; the gain test checks the cache
     assert np.all(np.isfinite(losses)) and np.all(losses >= 0)
     steps, last = cache.stats()
     assert steps == 40 * 4 and np.isfinite(last)
