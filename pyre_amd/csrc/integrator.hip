@@ -267,8 +267,6 @@ __global__ __launch_bounds__(kThreads) void k_generate(Paths P, Sched sc) {
     store_rng(P, p, rng);
     P.beta_pdf[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     P.lum[p] = make_float4(0.0f, 0.0f, 0.0f, __builtin_bit_cast(float, FL_SPECULAR | (train ? FL_TRAIN : 0u)));
-    P.aov0[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    P.aov1[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     P.pjob[p] = job;
     if (train) P.vcount[local] = 0;
     P.q_cur[p] = p;
@@ -692,7 +690,12 @@ __global__ __launch_bounds__(kThreads) void k_accumulate(Paths P, Sched sc) {
     double d0 = acc64[0], d1 = acc64[1], d2 = acc64[2];
     for (uint32_t sl = 0; sl < slc.n_samples; ++sl) {
       uint32_t p = slc.path0 + sl * J.npix + px;
-      float4 L = P.lum[p], A = P.aov0[p], N = P.aov1[p];
+      // a path's AOVs exist once its first vertex was shaded (FL_AOV);
+      // before that they are zero (integrator.rs:63-74), and are not stored
+      const float4 L = P.lum[p];
+      const bool has_aov = (__builtin_bit_cast(uint32_t, L.w) & FL_AOV) != 0u;
+      const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      const float4 A = has_aov ? P.aov0[p] : zero, N = has_aov ? P.aov1[p] : zero;
       a[0] = a[0] + L.x;
       a[1] = a[1] + L.y;
       a[2] = a[2] + L.z;
