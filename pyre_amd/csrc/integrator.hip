@@ -265,7 +265,6 @@ __global__ __launch_bounds__(kThreads) void k_generate(Paths P, Sched sc) {
     P.ray[2 * (size_t)p] = f4(o, 1e-4f);
     P.ray[2 * (size_t)p + 1] = f4(d, kInf);
     store_rng(P, p, rng);
-    P.beta_pdf[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     P.lum[p] = make_float4(0.0f, 0.0f, 0.0f, __builtin_bit_cast(float, FL_SPECULAR | (train ? FL_TRAIN : 0u)));
     P.pjob[p] = job;
     if (train) P.vcount[local] = 0;
@@ -438,7 +437,9 @@ __device__ void shade_path(const Paths& P, const JobDev& J, const PassDev& pp, u
   float r_tmin = ra.w;
   float4 hv = P.hit[p];
   uint32_t gid = __builtin_bit_cast(uint32_t, hv.x);
-  float4 bp = P.beta_pdf[p];
+  // a camera path starts with beta = 1 and no BSDF pdf (integrator.rs:63-74):
+  // the first bounce needs no stored state for them
+  const float4 bp = depth == 0 ? make_float4(1.0f, 1.0f, 1.0f, 0.0f) : P.beta_pdf[p];
   V3 beta = xyz(bp);
   float last_pdf = bp.w;
   float4 lf = P.lum[p];
