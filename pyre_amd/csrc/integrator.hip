@@ -230,33 +230,52 @@ __global__ void k_rotate(uint32_t* c, unsigned long long* stats) {
   }
 }
 
-// main.rs:261-273 + camera.rs + integrator.rs:63-74 state
+// Which sample path p of the current pass is: its job, its index within the
+// job's slice (or among the training paths), its pixel and its sample
+// number (main.rs:261: one stream per (x, y, sample)).
+struct PathSample {
+  uint32_t job, local, x, y, sample;
+  bool train;
+};
+
+__device__ __forceinline__ PathSample path_sample(const Sched& sc, const PassDev& pp, uint32_t p) {
+  PathSample r;
+  r.train = p >= pp.train0;
+  uint32_t s_base;
+  if (r.train) {
+    r.job = sc.slices[pp.slice0].job;
+    r.local = p - pp.train0;
+    s_base = pp.t_sbase;
+  } else {
+    const SliceDev slc = sc.slices[find_slice(sc, pp, p, [](const SliceDev& q) { return q.path0; })];
+    r.job = slc.job;
+    r.local = p - slc.path0;
+    s_base = slc.s_base;
+  }
+  const JobDev& J = sc.jobs[r.job];
+  const uint32_t npix = J.npix;
+  const uint32_t pixel =
+      r.train ? (uint32_t)(((uint64_t)r.local * pp.pix_mul + pp.pix_add) % npix) : r.local % npix;
+  r.sample = s_base + r.local / npix;
+  r.x = pixel % J.width;
+  r.y = pixel / J.width;
+  return r;
+}
+
+// main.rs:261-273 + camera.rs + integrator.rs:63-74 state. The stream's
+// state after the camera's draws is not stored: the first bounce reseeds it
+// (camera_stream) instead of reading 32 bytes per path back.
 __global__ __launch_bounds__(kThreads) void k_generate(Paths P, Sched sc) {
   const PassDev pp = cur_pass(sc);
   const uint32_t stride = gridDim.x * kThreads;
   for (uint32_t p = blockIdx.x * kThreads + threadIdx.x; p < pp.paths; p += stride) {
-    const bool train = p >= pp.train0;
-    uint32_t job, local, s_base;
-    if (train) {
-      job = sc.slices[pp.slice0].job;
-      local = p - pp.train0;
-      s_base = pp.t_sbase;
-    } else {
-      const SliceDev slc = sc.slices[find_slice(sc, pp, p, [](const SliceDev& q) { return q.path0; })];
-      job = slc.job;
-      local = p - slc.path0;
-      s_base = slc.s_base;
-    }
-    const JobDev& J = sc.jobs[job];
-    const uint32_t npix = J.npix, width = J.width;
-    const uint32_t pixel = train ? (uint32_t)(((uint64_t)local * pp.pix_mul + pp.pix_add) % npix) : local % npix;
-    const uint32_t sl = local / npix;
-    uint32_t x = pixel % width, y = pixel / width;
-    Rng rng = rng_seed(pixel_seed(x, y, s_base + sl));
+    const PathSample ps = path_sample(sc, pp, p);
+    const JobDev& J = sc.jobs[ps.job];
+    Rng rng = rng_seed(pixel_seed(ps.x, ps.y, ps.sample));
     float jx = rng_f32(rng);
     float jy = rng_f32(rng);
-    float ndc_x = 2.0f * ((float)x + jx) / (float)width - 1.0f;
-    float ndc_y = 1.0f - 2.0f * ((float)y + jy) / (float)J.height;
+    float ndc_x = 2.0f * ((float)ps.x + jx) / (float)J.width - 1.0f;
+    float ndc_y = 1.0f - 2.0f * ((float)ps.y + jy) / (float)J.height;
     V2 lens = rng_vec2(rng);
     const float shutter = rng_f32(rng);  // CameraSample::time (main.rs:268)
     if (J.S.motion) P.time[p] = shutter;
@@ -264,12 +283,20 @@ __global__ __launch_bounds__(kThreads) void k_generate(Paths P, Sched sc) {
     camera_ray(J.cam, V2{ndc_x, ndc_y}, lens, &o, &d);
     P.ray[2 * (size_t)p] = f4(o, 1e-4f);
     P.ray[2 * (size_t)p + 1] = f4(d, kInf);
-    store_rng(P, p, rng);
-    P.lum[p] = make_float4(0.0f, 0.0f, 0.0f, __builtin_bit_cast(float, FL_SPECULAR | (train ? FL_TRAIN : 0u)));
-    P.pjob[p] = job;
-    if (train) P.vcount[local] = 0;
+    P.lum[p] = make_float4(0.0f, 0.0f, 0.0f, __builtin_bit_cast(float, FL_SPECULAR | (ps.train ? FL_TRAIN : 0u)));
+    P.pjob[p] = ps.job;
+    if (ps.train) P.vcount[ps.local] = 0;
     P.q_cur[p] = p;
   }
+}
+
+// The stream of path p as the first bounce finds it: seeded by pixel_seed
+// and past the camera's five draws (jitter x / y, lens x / y, time).
+__device__ __forceinline__ Rng camera_stream(const Sched& sc, const PassDev& pp, uint32_t p) {
+  const PathSample ps = path_sample(sc, pp, p);
+  Rng rng = rng_seed(pixel_seed(ps.x, ps.y, ps.sample));
+  for (int k = 0; k < 5; ++k) (void)rng_next_u64(rng);
+  return rng;
 }
 
 // Paths a wave claims at a time from a stage's queue counter once its
@@ -429,8 +456,8 @@ struct ShadeOut {
 };
 
 // One bounce of integrator.rs:76-222 for path p.
-__device__ void shade_path(const Paths& P, const JobDev& J, const PassDev& pp, uint32_t depth, uint32_t p,
-                           ShadeOut& o) {
+__device__ void shade_path(const Paths& P, const Sched& sc, const JobDev& J, const PassDev& pp, uint32_t depth,
+                           uint32_t p, ShadeOut& o) {
   const SceneDev& S = J.S;
   float4 ra = P.ray[2 * (size_t)p], rb = P.ray[2 * (size_t)p + 1];
   V3 ro = xyz(ra), rd = xyz(rb);
@@ -548,7 +575,7 @@ __device__ void shade_path(const Paths& P, const JobDev& J, const PassDev& pp, u
       }
     }
   }
-  Rng rng = load_rng(P, p);
+  Rng rng = depth == 0 ? camera_stream(sc, pp, p) : load_rng(P, p);
   uint32_t ns_count = 0;
   const size_t slot0 = (size_t)p * P.slots;
   // NEE toward each quad light (integrator.rs:142-167)
@@ -654,7 +681,7 @@ __global__ __launch_bounds__(kThreads, MINW) void k_shade(Paths P, Sched sc, uin
     uint32_t p = 0;
     if (i < n) {
       p = P.q_cur[i];
-      shade_path(P, sc.jobs[P.pjob[p]], pp, depth, p, o);
+      shade_path(P, sc, sc.jobs[P.pjob[p]], pp, depth, p, o);
     }
     nsh += o.n_shadow;
     const bool flag[3] = {o.cont, o.shadow, o.query};
