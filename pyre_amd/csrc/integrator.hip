@@ -286,7 +286,6 @@ __global__ __launch_bounds__(kThreads) void k_generate(Paths P, Sched sc) {
     P.lum[p] = make_float4(0.0f, 0.0f, 0.0f, __builtin_bit_cast(float, FL_SPECULAR | (ps.train ? FL_TRAIN : 0u)));
     P.pjob[p] = ps.job;
     if (ps.train) P.vcount[ps.local] = 0;
-    P.q_cur[p] = p;
   }
 }
 
@@ -306,10 +305,11 @@ constexpr uint32_t kPathChunk = 256;
 
 // Closest hit for every queued path, lanes refilled from the queue as their
 // rays finish. MOTION: some job's scene has moving meshes (the path's shutter
-// time goes with its ray).
+// time goes with its ray). first: the first bounce, whose queue is every
+// path of the pass in order (not stored).
 template <int RING, bool MOTION = false>
 __global__ __launch_bounds__(kThreads) void k_extend(Paths P, const JobDev* __restrict__ jobs, int2* gstack,
-                                                   bool one_job) {
+                                                   bool one_job, bool first) {
   __shared__ int2 ring[RING * kThreads];
   const uint32_t tid = blockIdx.x * kThreads + threadIdx.x;
   HybridStack<RING> st{ring + threadIdx.x, kThreads, gstack + tid, gridDim.x * kThreads, 0, 0};
@@ -324,7 +324,7 @@ __global__ __launch_bounds__(kThreads) void k_extend(Paths P, const JobDev* __re
     const uint64_t idle = __ballot(!active);
     if (__popcll(idle) >= kRefillLanes) {
       if (feed.take(idle, !active, item)) {
-        p = P.q_cur[item];
+        p = first ? item : P.q_cur[item];  // the first bounce's queue is every path in order
         if (!one_job) {
           const JobDev& J = jobs[P.pjob[p]];
           nodes = J.S.nodes4;
@@ -680,7 +680,7 @@ __global__ __launch_bounds__(kThreads, MINW) void k_shade(Paths P, Sched sc, uin
     o.n_shadow = 0;
     uint32_t p = 0;
     if (i < n) {
-      p = P.q_cur[i];
+      p = depth == 0 ? i : P.q_cur[i];
       shade_path(P, sc, sc.jobs[P.pjob[p]], pp, depth, p, o);
     }
     nsh += o.n_shadow;
@@ -1137,11 +1137,11 @@ unsigned resident_blocks(K kernel, unsigned want) {
 
 template <int RING, bool MOTION>
 void launch_trace_stages(const Work& W, Paths P, unsigned cus, unsigned bpc, bool shadow, hipStream_t st,
-                         bool extend, bool one_job) {
+                         bool extend, bool one_job, bool first) {
   if (extend) {
     static const unsigned occ = resident_blocks(k_extend<RING, MOTION>, 64);
     hipLaunchKernelGGL((k_extend<RING, MOTION>), dim3(cus * std::min(bpc, occ)), dim3(kThreads), 0, st, P,
-                       (const JobDev*)W.jobs, W.gstack, one_job);
+                       (const JobDev*)W.jobs, W.gstack, one_job, first);
   }
   if (shadow) {
     static const unsigned occ = resident_blocks(k_shadow<RING, MOTION>, 64);
@@ -1153,16 +1153,16 @@ void launch_trace_stages(const Work& W, Paths P, unsigned cus, unsigned bpc, boo
 // tgrid = CUs x t.blocks_per_cu (the spill columns are sized for it); each
 // stage kernel runs on as many of those blocks per CU as are resident at once.
 void launch_trace(const Work& W, const Paths& P, const Tuning& t, unsigned tgrid, bool extend, bool shadow,
-                  hipStream_t st, bool one_job, bool motion) {
+                  hipStream_t st, bool one_job, bool motion, bool first) {
   const unsigned bpc = std::max(1u, t.blocks_per_cu), cus = std::max(1u, tgrid / bpc);
   if (motion)  // scenes with moving meshes: one ring size
-    launch_trace_stages<8, true>(W, P, cus, bpc, shadow, st, extend, one_job);
+    launch_trace_stages<8, true>(W, P, cus, bpc, shadow, st, extend, one_job, first);
   else if (t.ring == 4)
-    launch_trace_stages<4, false>(W, P, cus, bpc, shadow, st, extend, one_job);
+    launch_trace_stages<4, false>(W, P, cus, bpc, shadow, st, extend, one_job, first);
   else if (t.ring == 16)
-    launch_trace_stages<16, false>(W, P, cus, bpc, shadow, st, extend, one_job);
+    launch_trace_stages<16, false>(W, P, cus, bpc, shadow, st, extend, one_job, first);
   else
-    launch_trace_stages<8, false>(W, P, cus, bpc, shadow, st, extend, one_job);
+    launch_trace_stages<8, false>(W, P, cus, bpc, shadow, st, extend, one_job, first);
 }
 
 // All stages of the pass passes[*pass_idx] on the stream; accumulate = false
@@ -1179,12 +1179,12 @@ void launch_pass(pyre_ctx* ctx, Work& W, const PassShape& shape, const Tuning& t
   hipLaunchKernelGGL(k_counters, dim3(1), dim3(kCounters), 0, st, P.counters, sc);
   hipLaunchKernelGGL(k_generate, dim3(cus * gen_bpc), dim3(kThreads), 0, st, P, sc);
   for (uint32_t d = 0; d < shape.max_depth; ++d) {
-    launch_trace(W, P, t, tgrid, true, false, st, shape.one_job, shape.motion);
+    launch_trace(W, P, t, tgrid, true, false, st, shape.one_job, shape.motion, d == 0);
     if (t.shade_waves >= 4)
       hipLaunchKernelGGL(k_shade<4>, dim3(cus * shade4_bpc), dim3(kThreads), 0, st, P, sc, d);
     else
       hipLaunchKernelGGL(k_shade<1>, dim3(cus * shade1_bpc), dim3(kThreads), 0, st, P, sc, d);
-    if (shape.nee) launch_trace(W, P, t, tgrid, false, true, st, shape.one_job, shape.motion);
+    if (shape.nee) launch_trace(W, P, t, tgrid, false, true, st, shape.one_job, shape.motion, false);
     hipLaunchKernelGGL(k_rotate, dim3(1), dim3(kCounters), 0, st, P.counters, P.stats);
     std::swap(P.q_cur, P.q_next);
   }
