@@ -275,10 +275,12 @@ typedef struct pyre_job {
   double* accum64;        /* FP64 radiance sums, added to (as pyre_render), or NULL */
 } pyre_job;
 
-/* Launch configuration; 0 fields keep the defaults (2^24 paths per pass, 8
- * traversal blocks per CU, 8-entry LDS stack ring). autotune != 0 first times
- * trial passes of the batch over pool sizes {2^22, 2^23, 2^24}, blocks per CU
- * {4, 8} and stack rings {4, 8, 16} and keeps the fastest. */
+/* Launch configuration; 0 fields keep the defaults (2^27 paths per pass, 8
+ * traversal blocks per CU, 8-entry LDS stack ring). A batch never uses more
+ * paths per pass than the sum of its jobs' width * height * spp (nor fewer
+ * than one image of its largest job). autotune != 0 first times trial passes
+ * of the batch over pool sizes {2^22, 2^24, 2^26}, blocks per CU {4, 8} and
+ * stack rings {4, 8, 16} and keeps the fastest. */
 typedef struct pyre_batch_params {
   uint32_t pool_paths;
   uint32_t trace_blocks_per_cu;
