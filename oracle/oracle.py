@@ -62,6 +62,7 @@ def lib():
             "orc_hdri_new": (_vp, [_u32, _u32, _f32p, C.c_float]),
             "orc_hdri_gradient": (_vp, [_f32p, _f32p, C.c_float]),
             "orc_hdri_free": (None, [_vp]),
+            "orc_hdri_texels": (None, [_vp, _u32p, _vp]),
             "orc_hdri_sample": (None, [_vp, _f32p, _u32, _f32p, _f32p, _f32p]),
             "orc_hdri_pdf": (None, [_vp, _f32p, _u32, _f32p]),
             "orc_hdri_le": (None, [_vp, _f32p, _u32, _f32p]),
@@ -299,6 +300,14 @@ class Hdri:
         wi = f32(wi).reshape(-1, 3)
         out = np.empty_like(wi)
         lib().orc_hdri_le(self.h, wi, len(wi), out)
+        return out
+
+    def texels(self):
+        """The map's pixels, (height, width, 3), before the intensity."""
+        wh = np.empty(2, np.uint32)
+        lib().orc_hdri_texels(self.h, wh, None)
+        out = np.empty((int(wh[1]), int(wh[0]), 3), np.float32)
+        lib().orc_hdri_texels(self.h, wh, out.ctypes.data)
         return out
 
 

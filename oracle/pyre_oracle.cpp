@@ -1178,6 +1178,14 @@ void* orc_hdri_gradient(const float* zenith, const float* horizon, float intensi
   return new Hdri(w, h, std::move(px), intensity);
 }
 void orc_hdri_free(void* e) { delete (Hdri*)e; }
+// The map's size (w, h) and, when out != NULL, its texels (3 floats each, row-major).
+void orc_hdri_texels(void* e, uint32_t* wh, float* out) {
+  const Hdri* H = (const Hdri*)e;
+  wh[0] = H->width;
+  wh[1] = H->height;
+  if (out)
+    for (size_t i = 0; i < H->pixels.size(); ++i) st3(out + 3 * i, H->pixels[i]);
+}
 void orc_hdri_sample(void* e, const float* u, uint32_t n, float* wi, float* li, float* pdf) {
   for (uint32_t i = 0; i < n; ++i) {
     V3 w, l;

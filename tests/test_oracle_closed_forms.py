@@ -213,3 +213,152 @@ def test_frame_is_orthonormal(oracle):
     assert np.allclose(np.einsum("ij,ij->i", t, bt), 0, atol=1e-6)
     assert np.allclose(np.linalg.norm(t, axis=1), 1, atol=1e-6)
     assert np.allclose(np.cross(t, bt), nn, atol=1e-5)
+
+
+# ------------------------------------------------------------ InstanceMotion
+# mesh.rs:43-60 (InstanceMotion::at = lerp(offset_t0, offset_t1,
+# clamp(time, 0, 1))), mesh.rs:97-107 (swept world bounds) and
+# mesh.rs:109-148 (the ray shifted by -offset, the hit position shifted back).
+# Vertices, origins and offsets lie on a 1/8 grid, so o - offset is exact and
+# a moving mesh at time tau must give the bits of the static mesh hit by the
+# ray whose origin is pre-shifted by the expected offset.
+
+_OFF0 = np.array([0.25, -0.5, 0.125], np.float32)
+_OFF1 = np.array([-0.75, 0.5, 0.375], np.float32)
+
+
+def _grid_mesh_and_rays(n_tri=300, n_ray=6000, seed=21):
+    rng = np.random.default_rng(seed)
+    pos = _f(rng.integers(-8, 9, (3 * n_tri, 3)) / 8.0)
+    idx = np.arange(3 * n_tri, dtype=np.uint32).reshape(-1, 3)
+    o = _f(rng.integers(-16, 17, (n_ray, 3)) / 8.0)
+    d = rng.normal(size=(n_ray, 3))
+    d = _f(d / np.linalg.norm(d, axis=1, keepdims=True))
+    return pos, idx, o, d
+
+
+def _rays9(o, d, time):
+    r = np.zeros((len(o), 9), np.float32)
+    r[:, :3] = o
+    r[:, 3:6] = d
+    r[:, 6] = np.float32(1e-4)
+    r[:, 7] = np.inf
+    r[:, 8] = time
+    return r
+
+
+def _expected_offset(tau):
+    s = min(max(tau, 0.0), 1.0)  # the shutter clamp
+    # at the taus used below both lerp forms (a(1-s) + bs, a + (b-a)s) are exact
+    return _f(_OFF0.astype(np.float64) * (1 - s) + _OFF1.astype(np.float64) * s)
+
+
+@pytest.mark.parametrize("tau", [0.0, 1.0, 0.25, 0.5, 0.75, -1.0, -0.25, 1.5, 3.0])
+def test_motion_equals_preshifted_static_mesh(oracle, tau):
+    pos, idx, o, d = _grid_mesh_and_rays()
+    mov = oracle.Scene()
+    mov.add_material(oracle.material(0, (0.5, 0.5, 0.5)))
+    mov.add_mesh(pos, idx, None, 0)
+    mov.set_motion(0, _OFF0, _OFF1)
+    sta = oracle.Scene()
+    sta.add_material(oracle.material(0, (0.5, 0.5, 0.5)))
+    sta.add_mesh(pos, idx, None, 0)
+    off = _expected_offset(tau)
+    a = mov.intersect(_rays9(o, d, tau), with_pn=True)
+    b = sta.intersect(_rays9(o - off, d, 0.0), with_pn=True)
+    assert (a["kind"] == 1).sum() > 500
+    assert np.array_equal(a["kind"], b["kind"])
+    hit = a["kind"] == 1
+    assert np.array_equal(a["tri"][hit], b["tri"][hit])
+    for k in ("t", "u", "v"):
+        assert np.array_equal(a[k][hit].view(np.uint32), b[k][hit].view(np.uint32)), k
+    # the hit position is shifted back to world space (mesh.rs:144-146)
+    assert np.array_equal(a["position"][hit], _f(b["position"][hit] + off))
+    assert np.array_equal(a["normal"][hit], b["normal"][hit])
+    # occlusion sees the mesh where it is at that time
+    to = _f(o + 4.0 * d)
+    assert np.array_equal(mov.occluded(o, to, tau), sta.occluded(o - off, to - off, 0.0))
+
+
+def test_motion_zero_offset_is_static_and_bounds_are_swept(oracle):
+    pos, idx, o, d = _grid_mesh_and_rays(seed=22)
+    sta = oracle.Scene()
+    sta.add_material(oracle.material(0, (0.5, 0.5, 0.5)))
+    sta.add_mesh(pos, idx, None, 0)
+    zero = oracle.Scene()
+    zero.add_material(oracle.material(0, (0.5, 0.5, 0.5)))
+    zero.add_mesh(pos, idx, None, 0)
+    zero.set_motion(0, np.zeros(3), np.zeros(3))
+    for tau in (0.0, 0.6, 2.0):
+        a = zero.intersect(_rays9(o, d, tau), with_pn=True)
+        b = sta.intersect(_rays9(o, d, tau), with_pn=True)
+        for k in ("kind", "tri", "t", "u", "v", "position"):
+            assert np.array_equal(a[k], b[k]), k
+    # Scene::bounds of a moving mesh: the union of its box at both shutter ends
+    mov = oracle.Scene()
+    mov.add_material(oracle.material(0, (0.5, 0.5, 0.5)))
+    mov.add_mesh(pos, idx, None, 0)
+    mov.set_motion(0, _OFF0, _OFF1)
+    b0 = sta.bounds()
+    want = np.concatenate([np.minimum(b0[:3] + _OFF0, b0[:3] + _OFF1), np.maximum(b0[3:] + _OFF0, b0[3:] + _OFF1)])
+    assert np.array_equal(mov.bounds(), _f(want))
+    assert not np.array_equal(mov.bounds(), b0)
+
+
+# ------------------------------------------------------ gradient environment
+# HdriEnvironmentLight::gradient (light.rs:252-272): a 128 x 64 map, zenith
+# colour straight up, horizon colour at the equator, dimmed below it without
+# changing hue.
+
+_ZEN = np.array([0.1, 0.3, 0.9], np.float32)
+_HOR = np.array([0.9, 0.6, 0.3], np.float32)
+
+
+def test_gradient_map_rows(oracle):
+    env = oracle.Hdri(gradient=(_ZEN, _HOR), intensity=2.0)
+    tex = env.texels()
+    assert tex.shape == (64, 128, 3)
+    assert np.all(tex == tex[:, :1, :])  # rows are constant in x
+    span = np.abs(_ZEN - _HOR).max()
+    assert np.abs(tex[0, 0] - _ZEN).max() < 0.02 * span  # top row ~ zenith
+    assert np.abs(tex[31, 0] - _HOR).max() < 0.02 * span  # just above the equator ~ horizon
+    assert np.abs(tex[32, 0] - _HOR).max() < 0.02 * span  # just below it ~ horizon
+    # the bottom row is about half the horizon colour, hue unchanged
+    ratio = tex[63, 0] / _HOR
+    assert np.ptp(ratio) < 1e-6 and abs(ratio[0] - 0.5) < 0.01
+    # upper half: convex combination of horizon and zenith. The reference's
+    # weight ("cos_theta", light.rs:259-261) is linear in the row's v, 1 - 2v,
+    # not the y of the row centre's direction under dir_to_uv (cos(pi v)):
+    # from 1 - 1/64 at the top row down by 1/32 per row
+    ys = np.arange(32)
+    dirs = oracle.uv_to_dir(_f(np.stack([np.full(32, 0.5), (ys + 0.5) / 64], 1)))
+    w = ((tex[ys, 0] - _HOR) @ (_ZEN - _HOR)) / float((_ZEN - _HOR) @ (_ZEN - _HOR))
+    assert np.allclose(w, 1.0 - (2 * ys + 1) / 64.0, atol=2e-6)
+    assert np.all(w[1:-1] < dirs[1:-1, 1])  # below cos(pi v) away from the ends
+    resid = tex[ys, 0] - (_HOR + w[:, None] * (_ZEN - _HOR))
+    assert np.abs(resid).max() < 1e-6
+    assert np.all(np.diff(w) < 0)  # brighter toward the zenith's colour going up
+    # lower half: the horizon colour scaled by a factor falling from 1 toward 1/2
+    ys = np.arange(32, 64)
+    f = tex[ys, 0] / _HOR
+    assert np.all(np.ptp(f, axis=1) < 1e-6)
+    assert np.all(np.diff(f[:, 0]) < 0) and f[0, 0] < 1.0 and f[-1, 0] > 0.5
+
+
+def test_gradient_light_le_and_pdf(oracle):
+    env = oracle.Hdri(gradient=(_ZEN, _HOR), intensity=2.0)
+    le = env.le(_f([[0, 1, 0], [1, 0, 0], [0, 0, -1], [0, -1, 0]]))
+    span = np.abs(_ZEN - _HOR).max()
+    assert np.abs(le[0] / 2.0 - _ZEN).max() < 0.02 * span  # straight up
+    assert np.abs(le[1] / 2.0 - _HOR).max() < 0.02 * span  # equator
+    assert np.abs(le[2] / 2.0 - _HOR).max() < 0.02 * span
+    assert np.abs(le[3] / 2.0 - 0.5 * _HOR).max() < 0.01  # straight down: half the horizon
+    # the sampling pdf integrates to one over the sphere
+    rng = np.random.default_rng(23)
+    d = rng.normal(size=(1_000_000, 3))
+    d = _f(d / np.linalg.norm(d, axis=1, keepdims=True))
+    assert abs(4 * np.pi * env.pdf(d).astype(np.float64).mean() - 1.0) < 0.01
+    # and puts more density where the map is brighter (lum: zenith 0.30, horizon 0.64)
+    up = env.pdf(_f([[0.0, 0.2, 0.98]]))[0]
+    eq = env.pdf(_f([[0.98, 0.0, 0.2]]))[0]
+    assert eq > up
