@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <thread>
 
 #include "internal.hpp"
 #include "shade.hpp"
@@ -225,14 +226,14 @@ __global__ void k_bsdf_eval(MatDev m, const float* wo, const float* wi, uint32_t
   pdf[i] = bsdf_pdf(m, o, w);
 }
 
-__global__ void k_bsdf_sample(MatDev m, const float* wo, const float* u, uint32_t n, float* wi, float* f, float* pdf,
-                              uint32_t* ok) {
+__global__ void k_bsdf_sample(MatDev m, const float* wo, const float* u, uint32_t n, PhiTable T, float* wi, float* f,
+                              float* pdf, uint32_t* ok) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   V3 o = v3(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]);
   V3 w = splat(0.0f), fv = splat(0.0f);
   float p = 0.0f;
-  bool good = bsdf_sample(m, o, V2{u[2 * i], u[2 * i + 1]}, &w, &fv, &p);
+  bool good = bsdf_sample(m, o, V2{u[2 * i], u[2 * i + 1]}, T, &w, &fv, &p);
   wi[3 * i] = w.x;
   wi[3 * i + 1] = w.y;
   wi[3 * i + 2] = w.z;
@@ -366,6 +367,77 @@ MatDev to_dev(const pyre_material& m) {
 
 }  // namespace
 
+// The exception table of PhiTable (shade.hpp), built once per process from
+// the host's libm: for every grid point k < 2^24, phi = (2 pi) * (k 2^-24) in
+// f32 as the samplers form it; an exception where libm's sinf or cosf differs
+// from the double evaluation rounded to f32, or where that double lies within
+// 2^-40 (relative) of a float rounding boundary (so a device double that is
+// off by a few ulps could round the other way).
+struct HostPhiTable {
+  std::vector<uint4> rank;
+  std::vector<float2> val;
+};
+
+static bool near_boundary(double d, float f) {
+  if (d == 0.0) return false;
+  const double up = 0.5 * ((double)f + (double)std::nextafter(f, INFINITY));
+  const double dn = 0.5 * ((double)f + (double)std::nextafter(f, -INFINITY));
+  return std::min(std::fabs(d - up), std::fabs(d - dn)) < std::fabs(d) * 0x1p-40;
+}
+
+static bool phi_exception(uint32_t k, float* s_out, float* c_out) {
+  const float phi = 2.0f * kPi * ((float)k * (1.0f / 16777216.0f));
+  const float s = ::sinf(phi), c = ::cosf(phi);
+  const double ds = ::sin((double)phi), dc = ::cos((double)phi);
+  *s_out = s;
+  *c_out = c;
+  return __builtin_bit_cast(uint32_t, s) != __builtin_bit_cast(uint32_t, (float)ds) ||
+         __builtin_bit_cast(uint32_t, c) != __builtin_bit_cast(uint32_t, (float)dc) ||
+         near_boundary(ds, (float)ds) || near_boundary(dc, (float)dc);
+}
+
+static const HostPhiTable& host_phi_table() {
+  static const HostPhiTable table = [] {
+    constexpr uint32_t words = kPhiGrid / 64;
+    std::vector<uint64_t> bits(words, 0);
+    const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    auto par = [&](auto&& body) {
+      std::vector<std::thread> th;
+      for (unsigned t = 0; t < nt; ++t)
+        th.emplace_back([&, t] {
+          for (uint32_t w = t; w < words; w += nt) body(w);
+        });
+      for (auto& x : th) x.join();
+    };
+    par([&](uint32_t w) {
+      uint64_t b = 0;
+      float s, c;
+      for (uint32_t j = 0; j < 64; ++j)
+        if (phi_exception(w * 64 + j, &s, &c)) b |= 1ull << j;
+      bits[w] = b;
+    });
+    HostPhiTable T;
+    T.rank.resize(words);
+    uint32_t before = 0;
+    for (uint32_t w = 0; w < words; ++w) {
+      T.rank[w] = make_uint4((uint32_t)bits[w], (uint32_t)(bits[w] >> 32), before, 0u);
+      before += (uint32_t)__builtin_popcountll(bits[w]);
+    }
+    T.val.resize(std::max(1u, before));
+    par([&](uint32_t w) {
+      uint32_t i = T.rank[w].z;
+      for (uint32_t j = 0; j < 64; ++j)
+        if ((bits[w] >> j) & 1ull) {
+          float s, c;
+          phi_exception(w * 64 + j, &s, &c);
+          T.val[i++] = make_float2(s, c);
+        }
+    });
+    return T;
+  }();
+  return table;
+}
+
 extern "C" {
 
 int pyre_ctx_create(int device, pyre_ctx** out) {
@@ -384,9 +456,14 @@ int pyre_ctx_create(int device, pyre_ctx** out) {
     c->name = prop.gcnArchName;
     c->cus = prop.multiProcessorCount;
     PYRE_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    const HostPhiTable& T = host_phi_table();
+    c->d_phi_rank = dev_alloc<uint4>(T.rank.size());
+    c->d_phi_val = dev_alloc<float2>(T.val.size());
+    PYRE_HIP_CHECK(hipMemcpy(c->d_phi_rank, T.rank.data(), T.rank.size() * sizeof(uint4), hipMemcpyHostToDevice));
+    PYRE_HIP_CHECK(hipMemcpy(c->d_phi_val, T.val.data(), T.val.size() * sizeof(float2), hipMemcpyHostToDevice));
   });
   if (rc != PYRE_OK) {
-    delete c;
+    pyre_ctx_destroy(c);
     return rc;
   }
   *out = c;
@@ -398,6 +475,8 @@ void pyre_ctx_destroy(pyre_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   ctx->render_cache.reset();
+  dev_free(ctx->d_phi_rank);
+  dev_free(ctx->d_phi_val);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -689,8 +768,11 @@ int pyre_bsdf_sample_batch(pyre_ctx* ctx, const pyre_material* m, const float* w
     DevVec<uint32_t> okd(n);
     h2d(a.p, wo, 3 * n, st);
     h2d(uu.p, u, 2 * n, st);
-    hipLaunchKernelGGL(k_bsdf_sample, dim3(blocks(n, 256)), dim3(256), 0, st, md, a.p, uu.p, (uint32_t)n, wo_.p, fo.p,
-                       po.p, okd.p);
+    PhiTable T;
+    T.rank = ctx->d_phi_rank;
+    T.val = ctx->d_phi_val;
+    hipLaunchKernelGGL(k_bsdf_sample, dim3(blocks(n, 256)), dim3(256), 0, st, md, a.p, uu.p, (uint32_t)n, T, wo_.p,
+                       fo.p, po.p, okd.p);
     d2h(wi, wo_.p, 3 * n, st);
     d2h(f, fo.p, 3 * n, st);
     d2h(pdf, po.p, n, st);

@@ -67,6 +67,7 @@ struct SceneDev {
   uint32_t n_meshes, n_lights, has_env;
   EnvDev env;
   const float4* motion;  // 2 per mesh (offset_t0, offset_t1) when some mesh moves, else null
+  PhiTable phi;          // the host libm's sinf/cosf on the BSDF samplers' grid
 };
 
 // One render job: scene, camera, image and its sums (film.rs:38-109).
@@ -635,7 +636,7 @@ __device__ void shade_path(const Paths& P, const Sched& sc, const JobDev& J, con
   bool alive = true;
   V3 bwi, bf;
   float bpdf;
-  if (!bsdf_sample(mat, wo, rng_vec2(rng), &bwi, &bf, &bpdf)) alive = false;
+  if (!bsdf_sample(mat, wo, rng_vec2(rng), S.phi, &bwi, &bf, &bpdf)) alive = false;
   if (alive && bpdf <= 0.0f) alive = false;
   float cos_wi = alive ? fabsf(bwi.z) : 0.0f;
   if (alive && cos_wi == 0.0f) alive = false;
@@ -875,8 +876,52 @@ __device__ __forceinline__ uint8_t u8_sat(float x) {  // Rust's `f32 as u8`
   return (uint8_t)x;
 }
 
-__device__ __forceinline__ uint8_t gamma8(float x) {
-  return u8_sat(powf(rmin(rmax(x, 0.0f), 1.0f), 1.0f / 2.2f) * 255.0f);
+// save_png's byte `(clamp(x, 0, 1).powf(1 / 2.2) * 255) as u8` (film.rs:181-198)
+// as the host's libm computes it: t[k - 1] is the smallest f32 in [0, 1] whose
+// byte is at least k (the byte is monotone in x; thresholds found on the host
+// by bisection over the f32 bit patterns), so the byte of x is the number of
+// thresholds <= clamp(x, 0, 1) and device packing equals host packing.
+struct Gamma8 {
+  float t[255];
+};
+
+__device__ __forceinline__ uint8_t gamma8(float x, const Gamma8& G) {
+  const float c = rmin(rmax(x, 0.0f), 1.0f);  // NaN -> 0 (f32::max ignores NaN)
+  uint32_t lo = 0, hi = 255;                   // count of thresholds <= c lies in [lo, hi]
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (G.t[mid - 1] <= c)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return (uint8_t)lo;
+}
+
+static uint8_t host_gamma8(float x) {
+  const float v = ::powf(rmin(rmax(x, 0.0f), 1.0f), 1.0f / 2.2f) * 255.0f;
+  if (!(v > 0.0f)) return 0;
+  if (v >= 255.0f) return 255;
+  return (uint8_t)v;
+}
+
+static const Gamma8& gamma8_thresholds() {
+  static const Gamma8 G = [] {
+    Gamma8 g{};
+    for (uint32_t k = 1; k <= 255; ++k) {
+      uint32_t lo = 0, hi = 0x3F800000u;  // bit patterns of 0 and 1.0f; host_gamma8(1) = 255
+      while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (host_gamma8(__builtin_bit_cast(float, mid)) >= k)
+          hi = mid;
+        else
+          lo = mid + 1;
+      }
+      g.t[k - 1] = __builtin_bit_cast(float, lo);
+    }
+    return g;
+  }();
+  return G;
 }
 
 // depth range over the hits (depth > 0): positive floats order like their bits
@@ -901,14 +946,14 @@ __global__ void k_depth_range(const float* acc, uint32_t npix, float inv_spp, ui
 }
 
 __global__ void k_pack(const float* acc, uint32_t npix, float inv_spp, uint32_t what, const uint32_t* range,
-                       uint8_t* o8, float* of) {
+                       Gamma8 G, uint8_t* o8, float* of) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= npix) return;
   float m[10];
   for (int k = 0; k < 10; ++k) m[k] = acc[10 * (size_t)i + k] * inv_spp;
   if (what == PYRE_PACK_BEAUTY_PNG || what == PYRE_PACK_ALBEDO_PNG) {
     const int b = what == PYRE_PACK_BEAUTY_PNG ? 0 : 3;
-    for (int c = 0; c < 3; ++c) o8[3 * (size_t)i + c] = gamma8(m[b + c]);
+    for (int c = 0; c < 3; ++c) o8[3 * (size_t)i + c] = gamma8(m[b + c], G);
   } else if (what == PYRE_PACK_NORMAL_PNG) {
     for (int c = 0; c < 3; ++c) o8[3 * (size_t)i + c] = u8_sat(rmin(rmax(m[6 + c] * 0.5f + 0.5f, 0.0f), 1.0f) * 255.0f);
   } else if (what == PYRE_PACK_DEPTH_PNG) {
@@ -1092,8 +1137,11 @@ struct Work {
 };
 
 SceneDev scene_dev(pyre_scene* s) {
+  PhiTable phi;
+  phi.rank = s->ctx->d_phi_rank;
+  phi.val = s->ctx->d_phi_val;
   return SceneDev{s->d_nodes4, s->d_tris, s->d_verts, s->d_normals, s->d_tri_offset, s->d_mesh_info, s->d_mats,
-                  s->d_lights, s->n_meshes, s->n_lights, s->has_env ? 1u : 0u, s->env, s->d_motion};
+                  s->d_lights, s->n_meshes, s->n_lights, s->has_env ? 1u : 0u, s->env, s->d_motion, phi};
 }
 
 JobDev job_dev(pyre_scene* s, const pyre_camera_desc& cam, const pyre_integrator_params& ip, uint32_t w, uint32_t h,
@@ -1867,7 +1915,7 @@ int pyre_session_pack(pyre_ctx* ctx, pyre_session* S, uint32_t what, void* out) 
                          S->npix, inv, range);
     if (e == hipSuccess) {
       hipLaunchKernelGGL(k_pack, dim3(nblocks(S->npix, 256)), dim3(256), 0, st, S->acc, S->npix, inv, what, range,
-                         buf, (float*)buf);
+                         gamma8_thresholds(), buf, (float*)buf);
       e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(out, buf, bytes, hipMemcpyDeviceToHost, st);

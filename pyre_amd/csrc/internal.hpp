@@ -21,6 +21,10 @@ struct pyre_ctx {
   // path buffers kept from the last one-shot render (owned; freed with the
   // context, on its device)
   std::shared_ptr<void> render_cache;
+  // the host libm's sinf/cosf on the BSDF samplers' angle grid (PhiTable in
+  // shade.hpp), uploaded at context creation
+  uint4* d_phi_rank = nullptr;
+  float2* d_phi_val = nullptr;
 };
 
 namespace pyre {

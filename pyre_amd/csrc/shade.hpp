@@ -1,23 +1,101 @@
 // Shade-stage functions on the device: BSDFs (crates/pyre/src/material.rs),
 // quad and environment lights (light.rs), inverse-CDF sampling
 // (distribution.rs) and camera rays (camera.rs). Same f32 operation order as
-// the reference; sin/cos/acos/atan2/tan are the device math library's, so
-// values agree with the reference to ~1 ulp where those functions appear.
+// the reference. The BSDF samplers' sin/cos reproduce the host libm's sinf /
+// cosf bit for bit (PhiTable below); elsewhere (thin-lens disk, environment
+// directions) sin/cos/acos/atan2 are the device math library's and agree with
+// the reference to ~1 ulp.
 #pragma once
 
 #include "internal.hpp"
 
 namespace pyre {
 
+// sinf / cosf of phi = 2 pi u on the sampler's grid u = k 2^-24 (k < 2^24:
+// every u a sample stream yields, sampler.rs:24-33), as the host's libm
+// computes them: the angle of cosine_sample_hemisphere (material.rs:239-246)
+// and sample_ggx_vndf (:291-317). The reference calls libm's sinf / cosf,
+// which are not correctly rounded; the device evaluates sin / cos in double
+// and rounds once (the correctly rounded float wherever the double lies far
+// from a float rounding boundary), and the grid points where libm's float
+// differs from that, or where the double lies within 2^-40 of a boundary,
+// are exceptions listed with libm's values. rank[k / 64] = (exception bits of
+// grid points 64(k/64) .. +63 (low, high word), exceptions before them, 0);
+// val = (sinf, cosf) of every exception in grid order. Off the grid (the
+// shade-stage probes' arbitrary u) the double evaluation alone is used.
+struct PhiTable {
+  const uint4* rank = nullptr;
+  const float2* val = nullptr;
+};
+constexpr uint32_t kPhiGrid = 1u << 24;
+
+// sin and cos in double for moderate |x| (the samplers' [0, 2 pi)): x reduced
+// by the nearest multiple of pi/2 (pi/2 as a double plus its remainder, each
+// product exact inside the fma), then the Taylor series of sin to x^15 and of
+// cos to x^16 on |r| <= pi/4 (truncation below 2^-53 relative). Relative
+// error a few double ulps: far inside the 2^-40 margin the exception table
+// keeps from every float rounding boundary.
+PH void sincos_f64(double x, double* s_out, double* c_out) {
+  const double kPio2Hi = 1.5707963267948966, kPio2Lo = 6.123233995736766e-17;
+  const double k = rint(x * 0.63661977236758134);
+  double r = fma(-k, kPio2Hi, x);
+  r = fma(-k, kPio2Lo, r);
+  const double r2 = r * r;
+  double ps = 1.0 / 1307674368000.0;           // 1/15!
+  ps = fma(ps, -r2, 1.0 / 6227020800.0);       // 1/13!
+  ps = fma(ps, -r2, 1.0 / 39916800.0);         // 1/11!
+  ps = fma(ps, -r2, 1.0 / 362880.0);           // 1/9!
+  ps = fma(ps, -r2, 1.0 / 5040.0);             // 1/7!
+  ps = fma(ps, -r2, 1.0 / 120.0);              // 1/5!
+  ps = fma(ps, -r2, 1.0 / 6.0);                // 1/3!
+  const double sr = fma(-r * r2, ps, r);
+  double pc = 1.0 / 20922789888000.0;          // 1/16!
+  pc = fma(pc, -r2, 1.0 / 87178291200.0);      // 1/14!
+  pc = fma(pc, -r2, 1.0 / 479001600.0);        // 1/12!
+  pc = fma(pc, -r2, 1.0 / 3628800.0);          // 1/10!
+  pc = fma(pc, -r2, 1.0 / 40320.0);            // 1/8!
+  pc = fma(pc, -r2, 1.0 / 720.0);              // 1/6!
+  pc = fma(pc, -r2, 1.0 / 24.0);               // 1/4!
+  pc = fma(pc, -r2, 0.5);                      // 1/2!
+  const double cr = fma(-r2, pc, 1.0);
+  const int q = (int)((int64_t)k & 3);
+  *s_out = q == 0 ? sr : q == 1 ? cr : q == 2 ? -sr : -cr;
+  *c_out = q == 0 ? cr : q == 1 ? -sr : q == 2 ? -cr : sr;
+}
+
+PH void sincos_phi(float uy, float phi, const PhiTable& T, float* s, float* c) {
+  const float ks = uy * 16777216.0f;  // exact: a power-of-two scale
+  if (T.rank && ks >= 0.0f && ks < 16777216.0f) {
+    const uint32_t k = (uint32_t)ks;
+    if ((float)k == ks) {
+      const uint4 r = T.rank[k >> 6];
+      const uint64_t bits = (uint64_t)r.x | ((uint64_t)r.y << 32);
+      const uint32_t b = k & 63u;
+      if ((bits >> b) & 1ull) {
+        const float2 v = T.val[r.z + (uint32_t)__builtin_popcountll(bits & ((1ull << b) - 1ull))];
+        *s = v.x;
+        *c = v.y;
+        return;
+      }
+    }
+  }
+  double ds, dc;
+  sincos_f64((double)phi, &ds, &dc);
+  *s = (float)ds;
+  *c = (float)dc;
+}
+
 // ---------------------------------------------------------------- material.rs
 PH float luminance(V3 c) { return dot(c, v3(0.2126f, 0.7152f, 0.0722f)); }  // :232-235
 PH float powi5(float x) { return x * ((x * x) * (x * x)); }  // llvm.powi(x, 5) as emitted for f32::powi(5)
 
-PH V3 cosine_sample_hemisphere(V2 u) {  // :239-246
+PH V3 cosine_sample_hemisphere(V2 u, const PhiTable& T) {  // :239-246
   float r = sqrtf(u.x);
   float phi = 2.0f * kPi * u.y;
-  float x = r * cosf(phi);
-  float y = r * sinf(phi);
+  float sp, cp;
+  sincos_phi(u.y, phi, T, &sp, &cp);
+  float x = r * cp;
+  float y = r * sp;
   float z = sqrtf(rmax(1.0f - u.x, 0.0f));
   return v3(x, y, z);
 }
@@ -47,15 +125,17 @@ PH float ggx_g2(V3 wo, V3 wi, float alpha) {                                    
 }
 PH V3 reflect(V3 v, V3 n) { return 2.0f * dot(v, n) * n - v; }  // :285-287
 
-PH V3 sample_ggx_vndf(V3 wo, float alpha, V2 u) {  // :291-317
+PH V3 sample_ggx_vndf(V3 wo, float alpha, V2 u, const PhiTable& T) {  // :291-317
   V3 vh = normalize(v3(alpha * wo.x, alpha * wo.y, wo.z));
   float lensq = vh.x * vh.x + vh.y * vh.y;
   V3 t1 = lensq > 0.0f ? v3(-vh.y, vh.x, 0.0f) / sqrtf(lensq) : v3(1.0f, 0.0f, 0.0f);
   V3 t2 = cross(vh, t1);
   float r = sqrtf(u.x);
   float phi = 2.0f * kPi * u.y;
-  float t1c = r * cosf(phi);
-  float t2c = r * sinf(phi);
+  float sp, cp;
+  sincos_phi(u.y, phi, T, &sp, &cp);
+  float t1c = r * cp;
+  float t2c = r * sp;
   float s = 0.5f * (1.0f + vh.z);
   t2c = (1.0f - s) * sqrtf(rmax(1.0f - t1c * t1c, 0.0f)) + s * t2c;
   V3 nh = t1c * t1 + t2c * t2 + sqrtf(rmax(1.0f - t1c * t1c - t2c * t2c, 0.0f)) * vh;
@@ -126,10 +206,10 @@ PH float bsdf_pdf(const MatDev& m, V3 wo, V3 wi) {
 }
 
 // Bsdf::sample; returns false where the reference returns None.
-PH bool bsdf_sample(const MatDev& m, V3 wo, V2 u, V3* wi_out, V3* f_out, float* pdf_out) {
+PH bool bsdf_sample(const MatDev& m, V3 wo, V2 u, const PhiTable& T, V3* wi_out, V3* f_out, float* pdf_out) {
   if (m.kind == PYRE_MAT_LAMBERTIAN) {  // :68-76
     if (wo.z <= 0.0f) return false;
-    V3 wi = cosine_sample_hemisphere(u);
+    V3 wi = cosine_sample_hemisphere(u, T);
     *wi_out = wi;
     *pdf_out = wi.z / kPi;
     *f_out = m.base / kPi;
@@ -142,10 +222,10 @@ PH bool bsdf_sample(const MatDev& m, V3 wo, V2 u, V3* wi_out, V3* f_out, float* 
   V3 wi;
   if (u.x < pd) {
     V2 ur{rmin(u.x / pd, 1.0f - 1e-7f), u.y};
-    wi = cosine_sample_hemisphere(ur);
+    wi = cosine_sample_hemisphere(ur, T);
   } else {
     V2 ur{rmin((u.x - pd) / ps, 1.0f - 1e-7f), u.y};
-    V3 h = sample_ggx_vndf(wo, disney_alpha(m), ur);
+    V3 h = sample_ggx_vndf(wo, disney_alpha(m), ur, T);
     wi = reflect(wo, h);
   }
   if (wi.z <= 0.0f) return false;

@@ -47,6 +47,22 @@ def edge_distance(v0, v1, v2, u, v):
     return np.nan_to_num(d, nan=0.0)
 
 
+def near_edge_hits(o_scene, rays, tri_arrays):
+    """Rays whose oracle closest hit in (t_min, t_max) lies within 1e-6 of a
+    triangle edge (tri_arrays[p] = (n, 3, 3) vertices of primitive p), as a
+    bool mask: excluded from any-hit parity and counted."""
+    o = o_scene.intersect(rays)
+    hit = np.where(o["kind"] == 1)[0]
+    ed = np.empty(len(hit))
+    for p, t3 in enumerate(tri_arrays):
+        sel = o["id"][hit] == p
+        tri = o["tri"][hit][sel]
+        ed[sel] = edge_distance(t3[tri, 0], t3[tri, 1], t3[tri, 2], o["u"][hit][sel], o["v"][hit][sel])
+    mask = np.zeros(len(rays), bool)
+    mask[hit[ed < 1e-6]] = True
+    return mask
+
+
 def rel_err(a, b, floor=1e-30):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)

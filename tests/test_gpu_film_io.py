@@ -1,8 +1,7 @@
 """Film sinks packed on the device (pyre_session_pack) against the host
 packing (pyre_film_pack) of the same session's means (pyre_session_read):
-the linear conversions and the EXR planes bit for bit; the gamma-encoded
-PNG bytes within one unit where the device's and the host's powf round the
-last ulp differently next to a truncation boundary."""
+every sink byte for byte (the device's gamma bytes come from thresholds the
+host's libm powf defines)."""
 import numpy as np
 import pytest
 
@@ -30,7 +29,7 @@ def test_session_pack_matches_host_pack(ctx, tmp_path):
     for what in (L.PACK_BEAUTY_PNG, L.PACK_ALBEDO_PNG):
         d = np.abs(sess.pack(what).astype(int) - host.pack(what).astype(int))
         print(f"  what={what}: bytes differing {(d > 0).sum()} of {d.size}", flush=True)
-        assert d.max() <= 1 and (d > 0).mean() < 1e-3
+        assert d.max() == 0
     assert sess.pack(L.PACK_BEAUTY_PNG).mean() > 10
     sess.save_png(tmp_path / "c.png")
     sess.save_aov_pngs(tmp_path / "c.png")
@@ -46,8 +45,9 @@ def test_milestone8_end_to_end(ctx, tmp_path):
     default camera, 256 spp, written by Film::save_png / save_aov_pngs).
     The GPU renders the whole frame, packs it on the device and writes it
     with pyre_write_png: the AOV images must equal the reference's byte for
-    byte, and the beauty image nearly so (device sin/cos/acos ulps send a
-    few paths another way than the reference's)."""
+    byte, and the beauty image too except the one byte the oracle's own
+    render of this frame differs in (milestone8_meta.json beauty_diff: a sum
+    whose last ulp the reference's build rounded the other way)."""
     import json
     import os
 
@@ -72,8 +72,9 @@ def test_milestone8_end_to_end(ctx, tmp_path):
         assert eq == 1.0, n
     d = np.abs(got("").astype(int) - ref("").astype(int))
     print(f"  beauty: equal bytes {(d == 0).mean():.6f}, max difference {d.max()}", flush=True)
-    # measured: 40 of 1,440,000 bytes differ, by at most 14 (a path of 256
-    # that took another turn moves its pixel's mean)
-    assert (d > 0).sum() < 150 and d.max() <= 64
+    diff = [list(map(int, x)) for x in zip(*np.nonzero(d))]
+    known = [[y, x, c] for y, x, c, _, _ in meta["beauty_diff"]]
+    assert all(p in known for p in diff), diff
+    assert d.max() <= 1
     sess.close()
     sc.close()

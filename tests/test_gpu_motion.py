@@ -16,7 +16,7 @@ import numpy as np
 import pytest
 
 import pyre_amd
-from tests.helpers import edge_distance, rel_mse
+from tests.helpers import edge_distance, near_edge_hits, rel_mse
 
 pytestmark = pytest.mark.gpu
 F32 = np.float32
@@ -99,7 +99,10 @@ def test_moving_meshes_any_hit_and_bounds(ctx, oracle):
     o = o_sc.any_hit(rays).astype(bool)
     c = sc.intersect(rays)
     assert np.array_equal(g, c["inst"] != pyre_amd.MISS)
-    assert (g != o).sum() <= 2
+    excl = near_edge_hits(o_sc, rays, [sp[0].reshape(-1, 3, 3) for sp in soups])
+    print(f"  excluded near an edge {excl.sum()}", flush=True)
+    assert ((g != o) & ~excl).sum() == 0
+    assert excl.sum() < len(rays) // 1000 + 5
     assert 0.05 < g.mean() < 0.95
     # Scene::bounds with each moving mesh's box at both ends of the shutter
     lo = np.full(3, np.inf, np.float32)
@@ -130,17 +133,17 @@ def test_cornell_motion_blur_correlated_with_oracle(ctx, oracle):
     ip = pyre_amd.PathIntegrator(8, 3)
     g = pyre_amd.Film(w, h).render(sc, cam, ip, 32).reshape(h, w, 10)
     o = oracle.Scene.cornell(motion_blur=True).render(_ocam(oracle, cam), w, h, 32).reshape(h, w, 10)
-    same_alb = np.all(np.abs(g[..., 3:6] - o[..., 3:6]) <= 1e-6, axis=-1).mean()
+    same = np.all(g.view(np.uint32) == o.view(np.uint32), axis=-1).mean()
     e = rel_mse(g[..., :3], o[..., :3])
     # the blur is there: the static box renders differently around the sphere
     st = pyre_amd.Scene.from_preset(ctx, pyre_amd.Preset.cornell())
     st.build_bvh(bins=12, max_leaf=8)
     s = pyre_amd.Film(w, h).render(st, cam, ip, 32).reshape(h, w, 10)
     moved = np.any(s[..., 3:6] != g[..., 3:6], axis=-1).mean()
-    print(f"  albedo equal {same_alb:.4f} relMSE(gpu, cpu) {e:.2e}; pixels changed by the motion {moved:.3f}",
+    print(f"  bit-equal pixels {same:.4f} relMSE(gpu, cpu) {e:.2e}; pixels changed by the motion {moved:.3f}",
           flush=True)
-    assert same_alb > 0.97
-    assert e < 2e-3
+    assert same >= 0.99
+    assert e < 1e-9
     assert moved > 0.01
     sc.close()
     st.close()
@@ -191,9 +194,14 @@ def test_studio_preset_with_motion_against_oracle(ctx, oracle):
             o_sc.set_motion(prim, m["motion"][0], m["motion"][1])
     o_sc.set_env(oracle.Hdri(gradient=((0.20, 0.35, 0.65), (0.95, 0.85, 0.70)), intensity=1.0))
     o = o_sc.render(_ocam(oracle, cam), w, h, 16).reshape(h, w, 10)
-    same_alb = np.all(np.abs(g[..., 3:6] - o[..., 3:6]) <= 1e-6, axis=-1).mean()
+    same_aov = np.all(g[..., 3:].view(np.uint32) == o[..., 3:].view(np.uint32), axis=-1).mean()
+    rel = (np.abs(g[..., :3] - o[..., :3]) / np.maximum(np.abs(o[..., :3]), 1e-6)).max(-1)
     e = rel_mse(g[..., :3], o[..., :3])
-    print(f"  albedo equal {same_alb:.4f} relMSE(gpu, cpu) {e:.2e}", flush=True)
-    assert same_alb > 0.97
-    assert e < 2e-3
+    print(f"  AOV bit-equal {same_aov:.4f} radiance within 1e-5 {np.mean(rel <= 1e-5):.4f} relMSE(gpu, cpu) {e:.2e}",
+          flush=True)
+    # the gradient sky's direction mapping (acos/atan2) is the device math
+    # library's: radiance to 1e-5, first-hit AOVs (pinhole camera) bit-exact
+    assert same_aov >= 0.99
+    assert np.mean(rel <= 1e-5) >= 0.99
+    assert e < 1e-8
     assert o[..., :3].mean() > 0.05

@@ -1,10 +1,13 @@
 """GPU wavefront integrator vs the oracle's PathIntegrator (integrator.rs).
 
 Each GPU path consumes the same per-sample stream (pixel_seed, xoshiro256++)
-as the reference; only device sin/cos/acos/atan2 ulps separate the two, so
-most paths are identical and images agree far more closely than two
-independent renders would (correlated-sampling check). First-hit AOVs come
-from bit-exact camera rays and traversal and must match except on edges.
+as the reference, with the same f32 operations, and the BSDF samplers' sin /
+cos reproduce the host libm's. Scenes without an environment or thin lens
+(the Cornell box) therefore render to the oracle's bits. The environment's
+direction mapping (acos, atan2, sin, cos of arbitrary angles) and the
+thin-lens disk use the device math library, which can differ from libm by an
+ulp: there the images agree to 1e-5 relative, far closer than two independent
+renders would (correlated-sampling check).
 """
 import numpy as np
 import pytest
@@ -35,16 +38,12 @@ def test_cornell_correlated_with_oracle(ctx, oracle, w, h, spp):
     film = pyre_amd.Film(w, h)
     g = film.render(sc, cam, pyre_amd.PathIntegrator(8, 3), spp).reshape(h, w, 10)
     o = oracle.Scene.cornell().render(_ocam(oracle, cam), w, h, spp).reshape(h, w, 10)
-    # AOVs: identical except on silhouette / wall-edge pixels
-    for sl, name in ((slice(3, 6), "albedo"), (slice(6, 9), "normal"), (slice(9, 10), "depth")):
-        same = np.all(np.abs(g[..., sl] - o[..., sl]) <= 1e-6, axis=-1)
-        assert same.mean() > 0.97, f"{name} equal fraction {same.mean()}"
-    # beauty: correlated samples -> tiny relMSE; most pixels equal to 1e-4
+    # every PixelSample field of every pixel bit-equal to the oracle's
+    same = np.all(g.view(np.uint32) == o.view(np.uint32), axis=-1)
     e = rel_mse(g[..., :3], o[..., :3])
-    close = np.all(np.abs(g[..., :3] - o[..., :3]) <= 1e-4 * np.maximum(1, np.abs(o[..., :3])), axis=-1)
-    print(f"  relMSE(gpu, cpu)={e:.3e} close={close.mean():.4f}")
-    assert e < 2e-3
-    assert close.mean() > 0.6
+    print(f"  bit-equal pixels {same.mean():.4f}, relMSE(gpu, cpu)={e:.3e}")
+    assert same.mean() >= 0.99
+    assert e < 1e-9
     # and the image is not trivially dark
     assert o[..., :3].mean() > 0.05
 
@@ -83,19 +82,23 @@ def test_production_scene_small(ctx, oracle):
     e = arr["env"]
     os_.set_env(oracle.Hdri(e["width"], e["height"], e["rgb"], e["intensity"]))
     o = os_.render(_ocam(oracle, cam), w, h, spp).reshape(h, w, 10)
-    same_alb = np.all(np.abs(g[..., 3:6] - o[..., 3:6]) <= 1e-6, axis=-1).mean()
+    same_aov = np.all(g[..., 3:].view(np.uint32) == o[..., 3:].view(np.uint32), axis=-1).mean()
+    rel = (np.abs(g[..., :3] - o[..., :3]) / np.maximum(np.abs(o[..., :3]), 1e-6)).max(-1)
     e2 = rel_mse(g[..., :3], o[..., :3])
-    print(f"  albedo equal={same_alb:.4f} relMSE(gpu,cpu)={e2:.3e}")
-    assert same_alb > 0.95
-    assert e2 < 5e-3
+    print(f"  AOV bit-equal={same_aov:.4f} radiance within 1e-5: {np.mean(rel <= 1e-5):.4f} "
+          f"relMSE(gpu,cpu)={e2:.3e}")
+    # first-hit AOVs: thin-lens camera rays are within an ulp of the oracle's,
+    # so a few pixels see another triangle or a depth an ulp away
+    assert same_aov > 0.95
+    assert np.mean(rel <= 1e-5) >= 0.99
+    assert e2 < 1e-8
 
 
 def test_ray_counts_match_reference_queries(ctx, oracle):
     """The device's extension and shadow ray counts are the reference's
     Scene::intersect and Scene::occluded calls for the same samples (paths of
-    the same streams take the same bounces; a few may part on a device
-    sin/cos ulp, so the counts agree to 0.5 %). These counts are the units of
-    the traversal byte model."""
+    the same streams take the same bounces, so the counts are equal). These
+    counts are the units of the traversal byte model."""
     w, h, spp = 48, 48, 8
     pre, sc = _cornell(ctx)
     cam = pre.camera(1.0)
@@ -103,8 +106,8 @@ def test_ray_counts_match_reference_queries(ctx, oracle):
     ext, sh = sc.render_stats()
     st = oracle.Scene.cornell().render_stats(_ocam(oracle, cam), w, h, spp=spp, stride=1, max_depth=8, min_rr=3)
     print(f"  gpu ext={ext} shadow={sh}; oracle {st}", flush=True)
-    assert abs(ext - st["extension_queries"]) <= 0.005 * st["extension_queries"]
-    assert abs(sh - st["shadow_queries"]) <= 0.005 * st["shadow_queries"]
+    assert ext == st["extension_queries"]
+    assert sh == st["shadow_queries"]
     assert st["n_node_extension"] > 1 and st["n_tri_extension"] > 0
 
 

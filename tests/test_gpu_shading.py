@@ -56,10 +56,32 @@ def test_bsdf_eval_pdf(ctx, oracle, mi):
 
 
 @pytest.mark.parametrize("mi", range(1 + len(DISNEYS)))
+def test_bsdf_sample_on_sampler_grid_bit_exact(ctx, oracle, mi):
+    """u on the sample streams' grid (k 2^-24, sampler.rs:24-33, as every
+    render draws it): the device reproduces the host libm's sinf/cosf of the
+    sampling angle, so sampled directions, values and pdfs are the
+    reference's bits."""
+    m = _mats()[mi]
+    rng = np.random.default_rng(300 + mi)
+    wo = _dirs(rng, 40000)
+    u = (rng.integers(0, 1 << 24, (40000, 2)) * 2.0 ** -24).astype(np.float32)
+    u[:64, 1] = np.arange(64) * np.float32(2.0 ** -24)  # the first grid points
+    u[64:128, 1] = 1.0 - np.arange(1, 65) * np.float32(2.0 ** -24)  # and the last
+    wi, f, pdf, ok = pyre_amd.bsdf_sample(ctx, m, wo, u)
+    owi, of, opdf, ook = oracle.bsdf_sample(_omat(oracle, m), wo, u)
+    assert np.array_equal(ok, ook)
+    assert ok.mean() > 0.5
+    for a, b in ((wi, owi), (f, of), (pdf, opdf)):
+        assert np.array_equal(a[ok].view(np.uint32), b[ok].view(np.uint32))
+
+
+@pytest.mark.parametrize("mi", range(1 + len(DISNEYS)))
 def test_bsdf_sample(ctx, oracle, mi):
-    """Sampled directions agree to a few ulps of the device sin/cos; the value
-    and pdf returned with them equal the reference eval/pdf at that direction
-    (eval and pdf use only IEEE +-*/ and sqrt, so they are exact)."""
+    """u anywhere in [0, 1) (off the sampler's grid the device's sin/cos is
+    the correctly rounded one, which may differ from libm's by an ulp):
+    sampled directions agree to a few ulps; the value and pdf returned with
+    them equal the reference eval/pdf at that direction (eval and pdf use only
+    IEEE +-*/ and sqrt, so they are exact)."""
     m = _mats()[mi]
     rng = np.random.default_rng(200 + mi)
     wo = _dirs(rng, 20000)

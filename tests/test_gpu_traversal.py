@@ -10,7 +10,8 @@ import numpy as np
 import pytest
 
 import pyre_amd
-from tests.helpers import canonical_gpu_tree, canonical_oracle_tree, edge_distance, env_override, tree_stats
+from tests.helpers import (canonical_gpu_tree, canonical_oracle_tree, edge_distance, env_override, near_edge_hits,
+                           tree_stats)
 
 pytestmark = pytest.mark.gpu
 
@@ -126,8 +127,12 @@ def test_any_hit_matches_occluded(ctx, oracle):
     # any-hit must agree with closest-hit existence on the same ray (scene.rs:102-140)
     c = sc.intersect(rays)
     assert np.array_equal(g.astype(bool), c["inst"] != pyre_amd.MISS)
-    mism = np.nonzero(g.astype(bool) != o.astype(bool))[0]
-    assert len(mism) <= 2, f"{len(mism)} any-hit mismatches"
+    # no mismatch outside rays whose closest hit lies within 1e-6 of an edge
+    excl = near_edge_hits(o_sc, rays, [tris.reshape(-1, 3, 3)])
+    mism = np.nonzero((g.astype(bool) != o.astype(bool)) & ~excl)[0]
+    print(f"  any-hit: excluded near an edge {excl.sum()}, mismatches {len(mism)}")
+    assert len(mism) == 0, f"{len(mism)} any-hit mismatches"
+    assert excl.sum() < len(rays) // 1000 + 5
     assert 0.05 < g.mean() < 0.95
 
 
