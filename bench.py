@@ -12,6 +12,13 @@ A step is one closest-hit pass over the 2^23 closest-hit rays, inputs
 resident in HBM; value = closest-hit Mrays/s over K timed steps. The any-hit
 pass is timed the same way in its own block and reported beside it.
 
+The metric's second half follows on the same line: seconds to relMSE 0.01
+at 1920x1080 on the production scene (64 level-7 icospheres, sky/sun HDRI,
+depth 12, thin lens 0.05), the plain path tracer vs the radiance cache
+trained online on 2^16 samples per frame, against a 16384-spp FP64
+reference rendered on the GPU in the same run (nrc_over_plain; --no-nrc
+skips it).
+
 Roofline: algorithmic bytes per closest-hit ray =
 32 (ray) + 8 (hit) + 32 * N_node + 36 * N_tri, with N_node / N_tri the node
 fetches and triangle tests of the reference traversal (geometry/bvh.rs:86-143)
@@ -64,6 +71,22 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def host_info():
+    """The host the CPU baseline runs on: CPUs visible to this process
+    (nproc) and the CPU model (lscpu). The baseline's thread pool is capped
+    at 16, the host-CPU share a one-GPU box grants its jobs."""
+    info = {"nproc": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()}
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                info["cpu_model"] = line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
 def cpu_baseline(tris, rays36, sample):
     """Oracle (C++ restatement of the reference) on the host cores."""
     from oracle import oracle as O
@@ -81,7 +104,8 @@ def cpu_baseline(tris, rays36, sample):
     dt = time.perf_counter() - t0
     return sc, out, dict(value=round(sample / dt / 1e6, 3), unit="Mrays/s", cores=threads, kind="port",
                          sample=f"first {sample} closest-hit rays of the workload, oracle BVH "
-                                f"(reference build, {BINS} bins, leaf {MAX_LEAF}; build {build_s:.1f} s untimed)")
+                                f"(reference build, {BINS} bins, leaf {MAX_LEAF}; build {build_s:.1f} s untimed)",
+                         host=host_info())
 
 
 def bench_cornell(args):
@@ -323,14 +347,15 @@ def _crossing(curve, target):
     return None
 
 
-# Training paths per frame: most camera rays of the production view see the
-# sky, so 2^20 paths give about 2^18 surface-vertex training samples (2^15
-# per spp of an 8-spp frame).
-NRC_TRAIN_PATHS = 1 << 20
-# Adam steps per frame: batches of 2^14 cover the ~2^18 samples once.
-NRC_STEPS = 20
-# Path pool of the plain-vs-NRC comparison: 16 spp per pass at 1080p for both.
-NRC_POOL_PATHS = 1 << 25
+# Radiance-cache training per frame (the configuration's 2^16 samples): about
+# a quarter of the production view's training paths reach a surface vertex
+# per bounce, so 2^18 paths give at least 2^16 samples; 4 Adam steps on
+# batches of 2^14 use 2^16 of them.
+NRC_TRAIN_PATHS = 1 << 18
+NRC_BATCH = 1 << 14
+NRC_STEPS = 4
+# Frames of at most this many spp (a 1080p frame of 16 spp is one pass).
+NRC_MAX_SPF = 16
 
 
 def _at_time(curve, t):
@@ -393,7 +418,7 @@ def cache_mfma(ctx, lo, hi, args):
     ms = timed(lambda: cache.infer_dev(dq.data_ptr(), n, rgb.data_ptr(), stream.cuda_stream))
     tf = n * NRC_FLOP_QUERY / (ms * 1e-3) / 1e12
     out["infer"] = {"queries": n, "ms": round(ms, 4), "achieved": round(tf, 1), "frac": round(tf / MFMA_BF16_PEAK_TFLOPS, 4)}
-    for lg in (14, 18):
+    for lg in (14, 16, 18):
         b = 1 << lg
         ms = timed(lambda: cache.train_step_dev(dq.data_ptr(), dt.data_ptr(), b, 0, stream.cuda_stream))
         tf = b * NRC_FLOP_SAMPLE / (ms * 1e-3) / 1e12
@@ -404,45 +429,50 @@ def cache_mfma(ctx, lo, hi, args):
     return out
 
 
-def bench_nrc(args):
-    """Seconds to relMSE 0.01 at 1080p on the production scene: the plain
-    path tracer vs the radiance cache (paths ending in the cache after the
-    first diffuse bounce). Every checkpoint is its own render from a cold
-    start, timed whole: plain = one session render of N spp; NRC = a fresh
-    cache, nrc_warmup training-only frames, then frames of min(N, 16) spp.
-    Each NRC frame traces its samples and 2^20 training paths in one pass,
-    trains on the training paths' ~2^18 samples (20 Adam steps on batches of
-    2^14, every sample once), then adds the updated cache's radiance to the paths that
-    ended in it. Both use a pool of 2^25 paths (passes of up to 16 spp).
-    relMSE against the FP64 reference is taken outside the timed region."""
-    os.environ["PYRE_PASS_PATHS"] = str(NRC_POOL_PATHS)
-    # torch first: it and the library load a HIP runtime of the same soname,
-    # and the first one loaded serves both
-    import torch  # noqa: F401
+def render_reference(ctx, scene, cam, ip, ref_spp):
+    """FP64-accumulated plain render of ref_spp samples per pixel (samples
+    2^20.., independent of the measured ones) on the GPU: radiance (H*W, 3)."""
     import pyre_amd
-    ref = load_reference(args.reference)
-    if ref is None:
-        print("bench.py --config nrc needs --reference (make one with --make-reference PATH)", file=sys.stderr)
-        sys.exit(2)
-    ctx = pyre_amd.Context(0)
-    dev_name, _ = ctx.device_info()
-    scene, cam, ip = nrc_view(ctx, args.nrc_scene)
+    t0 = time.perf_counter()
+    film = pyre_amd.Film(PROD_W, PROD_H)
+    done = 0
+    while done < ref_spp:
+        n = min(256, ref_spp - done)
+        film.render(scene, cam, ip, n, spp_offset=(1 << 20) + done, accum64=True)
+        done += n
+    log(f"reference {ref_spp} spp in {time.perf_counter() - t0:.1f}s")
+    return film.accum64 / ref_spp
+
+
+def nrc_comparison(ctx, scene, cam, ip, ref, max_spp=128, warmup=0):
+    """Seconds to relMSE 0.01 against `ref`, plain path tracer vs radiance
+    cache (paths ending in the cache after the first diffuse bounce). Every
+    checkpoint N spp is its own render from a cold start, timed whole from
+    the first launch to the last result (pools are reserved before the
+    clock starts): plain = one session render of N spp; cache = a fresh
+    cache and frames of min(N, 16) spp, each tracing its samples and 2^18
+    training paths in one pass, taking 4 Adam steps of 2^14 of the frame's
+    training samples, then adding the updated cache's radiance to the paths
+    that ended in it. relMSE is taken outside the timed region."""
+    import pyre_amd
     lo, hi = scene.bounds()
     target = 0.01
-    checkpoints = [c for c in (4, 6, 8, 10, 12, 16, 24, 32, 48, 64, 96, 128) if c <= args.max_spp]
+    checkpoints = [c for c in (4, 6, 8, 10, 12, 16, 24, 32, 48, 64, 96, 128) if c <= max_spp]
     stats = {}
 
     def render(kind, n):
         cache = pyre_amd.RadianceCache(ctx, lo, hi, seed=5) if kind == "nrc" else None
         sess = pyre_amd.Session(scene, cam, ip, PROD_W, PROD_H, nrc_train_paths=NRC_TRAIN_PATHS if cache else 0)
-        t0 = time.perf_counter()  # session set-up has synchronised the stream
+        spf = min(n, NRC_MAX_SPF)
+        sess.reserve(n if cache is None else spf)
+        if cache is not None:
+            cache.reserve(NRC_BATCH)
+        t0 = time.perf_counter()
         if cache is None:
             sess.render(n)
         else:
-            spf = min(n, 16)
-            sess.render_nrc(cache, frames=args.nrc_warmup + n // spf, spp_per_frame=spf,
-                            train_paths=NRC_TRAIN_PATHS, train_batch=1 << 14, train_steps=NRC_STEPS,
-                            warmup_frames=args.nrc_warmup)
+            sess.render_nrc(cache, frames=warmup + n // spf, spp_per_frame=spf, train_paths=NRC_TRAIN_PATHS,
+                            train_batch=NRC_BATCH, train_steps=NRC_STEPS, warmup_frames=warmup)
         dt = time.perf_counter() - t0  # render calls return after the stream is idle
         _, r64, spp = sess.read(mean=False, radiance64=True)
         assert spp == n
@@ -458,7 +488,7 @@ def bench_nrc(args):
         render(kind, checkpoints[0])  # untimed: first-use costs (graph capture, code load)
         curve = []
         for n in checkpoints:
-            if kind == "nrc" and n > 16 and n % 16:
+            if kind == "nrc" and n > NRC_MAX_SPF and n % NRC_MAX_SPF:
                 continue
             curve.append(render(kind, n))
             if curve[-1][1] <= target * 0.7:
@@ -467,26 +497,65 @@ def bench_nrc(args):
 
     plain = run("plain")
     nrc = run("nrc")
-    mfma = cache_mfma(ctx, lo, hi, args)
     tp, tn = _crossing(plain, target), _crossing(nrc, target)
     eq = _at_time(nrc, tp) if tp else None
-    line = {
-        "metric": "seconds to relMSE 0.01 at 1080p, plain vs NRC", "value": round(tn, 4) if tn else None,
-        "unit": "s (NRC)", "n_gpus": 1, "steps": 1, "warmup": 0, "higher_is_better": False, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32 (cache: bf16 MFMA, f32 accumulate)",
-        "data": "synthetic (procedural scene, seed 3)",
-        "config": {"workload": ("production scene 1920x1080, max depth 12, thin lens 0.05" if args.nrc_scene == "production"
-                                else "Cornell preset 1920x1080, max depth 8, pinhole") + "; NRC 5x64 ReLU MLP, "
-                               "2^20 training paths/frame (~2^18 samples, Adam batches of 2^14), lr 1e-2, "
-                               f"{args.nrc_warmup} training-only frames, frames of min(N, 16) spp; "
-                               "pool 2^25 paths for both; every checkpoint rendered from a cold start",
-                   "relmse_eps": REL_EPS, "reference_spp": args.ref_spp},
+    return {
         "plain_seconds": round(tp, 4) if tp else None, "nrc_seconds": round(tn, 4) if tn else None,
         "nrc_over_plain": round(tn / tp, 4) if (tn and tp) else None,
         "relmse_at_plain_seconds": {"plain": target, "nrc": round(eq, 5) if eq else None},
-        "cache_mfma": mfma,
+        "training": {"train_paths_per_frame": NRC_TRAIN_PATHS, "adam_steps_per_frame": NRC_STEPS,
+                     "batch": NRC_BATCH, "samples_per_frame": NRC_STEPS * NRC_BATCH, "lr": 1e-2,
+                     "max_spp_per_frame": NRC_MAX_SPF, "warmup_frames": warmup},
         "plain_curve": [(round(t, 4), round(e, 5)) for t, e in plain],
-        "nrc_curve": [(round(t, 4), round(e, 5)) for t, e in nrc], "path_stats_last": stats, "device": dev_name,
+        "nrc_curve": [(round(t, 4), round(e, 5)) for t, e in nrc], "path_stats_last": stats,
+    }
+
+
+def nrc_block(ctx, args):
+    """The second half of the metric on the production view at 1920x1080:
+    the FP64 reference rendered here, plain vs cache seconds to relMSE 0.01,
+    and the cache kernels' MFMA throughput."""
+    import pyre_amd
+    t0 = time.perf_counter()
+    pre, scene, cam, info = production_scene(ctx)
+    ip = pyre_amd.PathIntegrator(12, 3)
+    ref = render_reference(ctx, scene, cam, ip, args.ref_spp)
+    out = nrc_comparison(ctx, scene, cam, ip, ref, max_spp=args.max_spp, warmup=args.nrc_warmup)
+    lo, hi = scene.bounds()
+    out["cache_mfma"] = cache_mfma(ctx, lo, hi, args)
+    out["config"] = {"workload": "production scene 1920x1080 (64 level-7 icospheres, 2048x1024 sky/sun HDRI), "
+                                 "max depth 12, thin lens 0.05; cache: 5x64 ReLU MLP, bf16 MFMA",
+                     "reference_spp": args.ref_spp, "relmse_eps": REL_EPS}
+    out["seconds"] = round(time.perf_counter() - t0, 1)
+    scene.close()
+    return out
+
+
+def bench_nrc(args):
+    """Seconds to relMSE 0.01 at 1080p, plain vs radiance cache, on its own
+    (--reference: a saved reference .npy; otherwise one is rendered here)."""
+    # torch first: it and the library load a HIP runtime of the same soname,
+    # and the first one loaded serves both
+    import torch  # noqa: F401
+    import pyre_amd
+    ctx = pyre_amd.Context(0)
+    dev_name, _ = ctx.device_info()
+    scene, cam, ip = nrc_view(ctx, args.nrc_scene)
+    ref = load_reference(args.reference)
+    if ref is None:
+        ref = render_reference(ctx, scene, cam, ip, args.ref_spp)
+    out = nrc_comparison(ctx, scene, cam, ip, ref, max_spp=args.max_spp, warmup=args.nrc_warmup)
+    lo, hi = scene.bounds()
+    out["cache_mfma"] = cache_mfma(ctx, lo, hi, args)
+    tn = out["nrc_seconds"]
+    line = {
+        "metric": "seconds to relMSE 0.01 at 1080p, plain vs NRC", "value": tn, "unit": "s (NRC)", "n_gpus": 1,
+        "steps": 1, "warmup": 0, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (cache: bf16 MFMA, f32 accumulate)", "data": "synthetic (procedural scene, seed 3)",
+        "config": {"workload": ("production scene 1920x1080, max depth 12, thin lens 0.05" if args.nrc_scene ==
+                                "production" else "Cornell preset 1920x1080, max depth 8, pinhole"),
+                   "relmse_eps": REL_EPS, "reference_spp": args.ref_spp},
+        "device": dev_name, **out,
     }
     print(json.dumps(line), flush=True)
 
@@ -612,6 +681,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity spot check")
+    ap.add_argument("--no-nrc", action="store_true",
+                    help="traversal: skip the plain-vs-radiance-cache comparison at 1080p")
     ap.add_argument("--config", default="traversal", choices=["traversal", "cornell", "production", "nrc", "batch"],
                     help="traversal: the headline microbench; cornell: the Cornell render; production: the "
                          "1080p production render; nrc: seconds to relMSE 0.01, plain vs radiance cache; "
@@ -738,6 +809,20 @@ def main():
         o_t = np.where(o["kind"][: 1 << 16] == 1, o["t"][: 1 << 16], 0).astype(np.float32).view(np.uint32)
         line["parity_spot_check"] = {"rays": 1 << 16, "prim_mismatch": int((g_prim != o_prim).sum()),
                                      "t_mismatch": int(((g_t != o_t) & (g_prim == o_prim)).sum())}
+    del d_rays, closest, shadow, hits, occ
+    scene.close()
+    if not args.no_nrc:
+        # the metric's second half: seconds to relMSE 0.01 at 1080p, plain vs
+        # the radiance cache, on the production view
+        try:
+            nrc = nrc_block(ctx, args)
+            line["plain_seconds_to_relmse_0.01"] = nrc["plain_seconds"]
+            line["nrc_seconds_to_relmse_0.01"] = nrc["nrc_seconds"]
+            line["nrc_over_plain"] = nrc["nrc_over_plain"]
+            line["nrc"] = nrc
+        except Exception as e:  # the traversal line is still reported
+            line["nrc_error"] = f"{type(e).__name__}: {e}"
+            log(f"nrc block failed: {line['nrc_error']}")
     print(json.dumps(line), flush=True)
 
 

@@ -20,9 +20,14 @@
  *   - Host pointers are owned by the caller and copied in/out. Entry points
  *     whose name ends in _dev take device pointers (memory on the context's
  *     device) and enqueue on the given HIP stream (NULL = the context stream)
- *     without synchronising.
+ *     without synchronising and without allocating.
  *   - A context is used by one host thread at a time. A scene is read-only
- *     after upload + build.
+ *     after upload + build. The trace entry points of a context share one
+ *     traversal scratch (work counter, stack spill columns), sized when a BVH
+ *     is built; calls enqueued on different streams are ordered on the device
+ *     in the order they were made (each waits for the previous one).
+ *   - Library behaviour depends on arguments only: launch settings live in
+ *     the context's pyre_tuning, never in environment variables.
  *   - All arithmetic on the hot path is IEEE FP32 without contraction
  *     (-ffp-contract=off), matching the reference's x86-64 build.
  */
@@ -183,6 +188,25 @@ const char* pyre_last_error(const pyre_ctx* ctx);
 /* Device name and CU count of the context's GPU (for reports). */
 int pyre_ctx_device_info(pyre_ctx* ctx, char* name, size_t name_len, int* cu_count);
 
+/* Launch configuration of a context's kernels (no counterpart in the
+ * reference). Results never depend on it: every setting gives the same bits.
+ * A 0 field selects the default; settings apply to later calls. */
+typedef struct pyre_tuning {
+  uint32_t pool_paths;          /* path-pool paths per render pass; 0 = 2^27. Never more than
+                                   memory_fraction of the device's free memory holds */
+  uint32_t trace_blocks_per_cu; /* persistent traversal blocks per CU, 1..8; 0 = 8 */
+  uint32_t stack_ring;          /* LDS traversal stack entries per thread: 4, 8 or 16; 0 = 8 */
+  uint32_t shade_waves;         /* 0 = shade kernel built for 4 waves per SIMD; 1 = no occupancy target */
+  uint32_t traversal;           /* trace entry points: 0 = 4-wide tree, persistent threads; 1 = binary
+                                   tree, LDS stack (trees of depth <= 47); 2 = 4-wide tree, grid-stride */
+  uint32_t eager;               /* 1 = render passes launched kernel by kernel instead of replaying a
+                                   captured hipGraph */
+  float memory_fraction;        /* share of the device's free memory a path pool may take, (0, 1];
+                                   0 = 0.5 */
+} pyre_tuning;
+int pyre_ctx_set_tuning(pyre_ctx* ctx, const pyre_tuning* tuning);
+int pyre_ctx_get_tuning(const pyre_ctx* ctx, pyre_tuning* tuning);
+
 /* ------------------------------------------------------------------ scene */
 /* Copies a flattened Scene (scene.rs:24-34) to the device. The triangle
  * arrays become one scene-global triangle list (primitive by primitive, in
@@ -275,12 +299,12 @@ typedef struct pyre_job {
   double* accum64;        /* FP64 radiance sums, added to (as pyre_render), or NULL */
 } pyre_job;
 
-/* Launch configuration; 0 fields keep the defaults (2^27 paths per pass, 8
- * traversal blocks per CU, 8-entry LDS stack ring). A batch never uses more
- * paths per pass than the sum of its jobs' width * height * spp (nor fewer
- * than one image of its largest job). autotune != 0 first times trial passes
- * of the batch over pool sizes {2^22, 2^24, 2^26}, blocks per CU {4, 8} and
- * stack rings {4, 8, 16} and keeps the fastest. */
+/* Launch configuration; 0 fields keep the context's pyre_tuning. A batch
+ * never uses more paths per pass than the sum of its jobs' width * height *
+ * spp (nor fewer than one image of its largest job), nor more than the
+ * tuning's share of free device memory holds. autotune != 0 first times
+ * trial passes of the batch over pool sizes {2^22, 2^24, 2^26}, blocks per
+ * CU {4, 8} and stack rings {4, 8, 16} and keeps the fastest. */
 typedef struct pyre_batch_params {
   uint32_t pool_paths;
   uint32_t trace_blocks_per_cu;
@@ -339,6 +363,9 @@ int pyre_nrc_train_step(pyre_ctx* ctx, pyre_nrc* nrc, const float* feats, const 
 int pyre_nrc_infer_dev(pyre_ctx* ctx, pyre_nrc* nrc, const float* feats, size_t n, float* rgb, void* stream);
 int pyre_nrc_train_step_dev(pyre_ctx* ctx, pyre_nrc* nrc, const float* feats, const float* targets, size_t n,
                             float* loss, void* stream);
+/* Sizes the training buffers for batches of up to max_batch samples now, so
+ * that no later training step of at most that many allocates. */
+int pyre_nrc_reserve(pyre_ctx* ctx, pyre_nrc* nrc, size_t max_batch);
 /* Adam steps taken so far and the loss of the last training batch. */
 int pyre_nrc_stats(pyre_ctx* ctx, pyre_nrc* nrc, uint64_t* steps, float* last_loss);
 
@@ -357,8 +384,16 @@ int pyre_session_create(pyre_ctx* ctx, pyre_scene* scene, const pyre_camera_desc
                         const pyre_integrator_params* integrator, uint32_t width, uint32_t height,
                         uint32_t nrc_train_paths, pyre_session** out);
 void pyre_session_destroy(pyre_session* session);
-/* Adds spp samples per pixel (plain path tracer). */
+/* Adds spp samples per pixel (plain path tracer). A call renders its
+ * samples in passes over the session's path pool, which is sized on the
+ * first call (and grown by later calls that ask for more samples) to hold
+ * the call's samples, within the context's pyre_tuning pool size and memory
+ * share, and at least one sample per pixel. */
 int pyre_session_render(pyre_ctx* ctx, pyre_session* session, uint32_t spp);
+/* Sizes the session's path pool for passes of up to spp_per_pass samples per
+ * pixel now, so that no later render call of at most that many samples
+ * allocates (for timing, or for callers that must not allocate later). */
+int pyre_session_reserve(pyre_ctx* ctx, pyre_session* session, uint32_t spp_per_pass);
 
 /* Radiance-cache rendering (no counterpart in the reference). Per frame, in
  * one pass over the path pool: spp_per_frame samples per pixel are traced,

@@ -8,6 +8,7 @@ call raises.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import os
 import weakref
@@ -68,6 +69,34 @@ class Context:
         cu = C.c_int()
         self.check(self.lib.pyre_ctx_device_info(self.h, buf, 128, C.byref(cu)), "device_info")
         return buf.value.decode(), cu.value
+
+    def get_tuning(self) -> dict:
+        """pyre_ctx_get_tuning: the context's launch settings (0 = default)."""
+        t = L.Tuning()
+        self.check(self.lib.pyre_ctx_get_tuning(self.h, C.byref(t)), "pyre_ctx_get_tuning")
+        return {name: getattr(t, name) for name, _ in L.Tuning._fields_}
+
+    def set_tuning(self, **kw) -> None:
+        """pyre_ctx_set_tuning: pool_paths, trace_blocks_per_cu, stack_ring,
+        shade_waves, traversal, eager, memory_fraction (unnamed fields keep
+        their current values)."""
+        cur = self.get_tuning()
+        unknown = set(kw) - set(cur)
+        if unknown:
+            raise ValueError(f"unknown tuning fields {sorted(unknown)}")
+        cur.update(kw)
+        t = L.Tuning(**cur)
+        self.check(self.lib.pyre_ctx_set_tuning(self.h, C.byref(t)), "pyre_ctx_set_tuning")
+
+    @contextlib.contextmanager
+    def tuning(self, **kw):
+        """Launch settings for the duration of a with-block."""
+        old = self.get_tuning()
+        self.set_tuning(**kw)
+        try:
+            yield self
+        finally:
+            self.set_tuning(**old)
 
     def close(self):
         if getattr(self, "h", None):
@@ -582,6 +611,10 @@ class RadianceCache:
         ctx._adopt(self)
         self.n_params = int(self.lib.pyre_nrc_param_count())
 
+    def reserve(self, max_batch: int):
+        """Sizes the training buffers for batches of up to max_batch now."""
+        self.ctx.check(self.lib.pyre_nrc_reserve(self.ctx.h, self.h, max_batch), "pyre_nrc_reserve")
+
     def weights(self):
         w = np.empty(self.n_params, np.float32)
         self.ctx.check(self.lib.pyre_nrc_get_weights(self.ctx.h, self.h, _ptr(w)), "pyre_nrc_get_weights")
@@ -666,6 +699,10 @@ class Session(_Sinks):
 
     def render(self, spp: int):
         self.ctx.check(self.lib.pyre_session_render(self.ctx.h, self.h, spp), "pyre_session_render")
+
+    def reserve(self, spp_per_pass: int):
+        """Sizes the path pool now for passes of up to spp_per_pass samples."""
+        self.ctx.check(self.lib.pyre_session_reserve(self.ctx.h, self.h, spp_per_pass), "pyre_session_reserve")
 
     def render_nrc(self, cache: RadianceCache, frames: int, spp_per_frame: int = 1, train_paths: int = 1 << 14,
                    train_batch: int = 1 << 14, train_steps: int = 4, diffuse_roughness: float = 0.25,

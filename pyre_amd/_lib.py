@@ -84,6 +84,11 @@ class View(C.Structure):
                 ("has_film", C.c_int), ("width", _u32), ("height", _u32)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("pool_paths", _u32), ("trace_blocks_per_cu", _u32), ("stack_ring", _u32), ("shade_waves", _u32),
+                ("traversal", _u32), ("eager", _u32), ("memory_fraction", _f)]
+
+
 class IntegratorParams(C.Structure):
     _fields_ = [("max_depth", _u32), ("min_rr_depth", _u32)]
 
@@ -98,6 +103,8 @@ SIGNATURES = {
     "pyre_ctx_destroy": (None, [_vp]),
     "pyre_last_error": (C.c_char_p, [_vp]),
     "pyre_ctx_device_info": (C.c_int, [_vp, C.c_char_p, C.c_size_t, C.POINTER(C.c_int)]),
+    "pyre_ctx_set_tuning": (C.c_int, [_vp, C.POINTER(Tuning)]),
+    "pyre_ctx_get_tuning": (C.c_int, [_vp, C.POINTER(Tuning)]),
     "pyre_scene_upload": (C.c_int, [_vp, C.POINTER(SceneDesc), C.POINTER(_vp)]),
     "pyre_scene_destroy": (None, [_vp]),
     "pyre_scene_bounds": (C.c_int, [_vp, _vp, _fp, _fp]),
@@ -191,10 +198,12 @@ class NrcRenderParams(C.Structure):
 
 SIGNATURES.update({
     "pyre_nrc_stats": (C.c_int, [_vp, _vp, C.POINTER(C.c_uint64), C.POINTER(_f)]),
+    "pyre_nrc_reserve": (C.c_int, [_vp, _vp, C.c_size_t]),
     "pyre_session_create": (C.c_int, [_vp, _vp, C.POINTER(CameraDesc), C.POINTER(IntegratorParams), _u32, _u32,
                                       _u32, C.POINTER(_vp)]),
     "pyre_session_destroy": (None, [_vp]),
     "pyre_session_render": (C.c_int, [_vp, _vp, _u32]),
+    "pyre_session_reserve": (C.c_int, [_vp, _vp, _u32]),
     "pyre_session_render_nrc": (C.c_int, [_vp, _vp, _vp, C.POINTER(NrcRenderParams), _vp]),
     "pyre_session_read": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(_u32)]),
     "pyre_session_reset": (C.c_int, [_vp, _vp]),

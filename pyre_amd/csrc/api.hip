@@ -167,9 +167,6 @@ void scene_free(pyre_scene* s) {
   dev_free(s->d_env_cond);
   dev_free(s->d_env_marg);
   dev_free(s->d_env_guide);
-  dev_free(s->d_flag);
-  dev_free(s->d_gstack);
-  dev_free(s->d_fetch);
 }
 
 void upload_env(pyre_scene* s, const pyre_env_desc& e, hipStream_t st) {
@@ -475,6 +472,7 @@ void pyre_ctx_destroy(pyre_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   ctx->render_cache.reset();
+  trace_scratch_release(ctx);
   dev_free(ctx->d_phi_rank);
   dev_free(ctx->d_phi_val);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -482,6 +480,27 @@ void pyre_ctx_destroy(pyre_ctx* ctx) {
 }
 
 const char* pyre_last_error(const pyre_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int pyre_ctx_set_tuning(pyre_ctx* ctx, const pyre_tuning* t) {
+  if (!ctx || !t) return PYRE_EINVAL;
+  const bool ok = t->trace_blocks_per_cu <= 8 && (t->stack_ring == 0 || t->stack_ring == 4 || t->stack_ring == 8 ||
+                                                  t->stack_ring == 16) &&
+                  t->shade_waves <= 4 && t->traversal <= 2 && t->eager <= 1 && t->memory_fraction >= 0.0f &&
+                  t->memory_fraction <= 1.0f;
+  if (!ok) {
+    ctx->err = "pyre_ctx_set_tuning: a field is out of range";
+    return PYRE_EINVAL;
+  }
+  ctx->tune = *t;
+  ctx->err.clear();
+  return PYRE_OK;
+}
+
+int pyre_ctx_get_tuning(const pyre_ctx* ctx, pyre_tuning* t) {
+  if (!ctx || !t) return PYRE_EINVAL;
+  *t = ctx->tune;
+  return PYRE_OK;
+}
 
 int pyre_ctx_device_info(pyre_ctx* ctx, char* name, size_t name_len, int* cu_count) {
   if (!ctx) return PYRE_EINVAL;
@@ -534,8 +553,6 @@ int pyre_scene_upload(pyre_ctx* ctx, const pyre_scene_desc* desc, pyre_scene** o
     }
     s->h_tri_offset[d.n_meshes] = (uint32_t)tri_total;
     s->n_tris = (uint32_t)tri_total;
-    s->d_flag = dev_alloc<uint32_t>(1);
-    PYRE_HIP_CHECK(hipMemsetAsync(s->d_flag, 0, 4, st));
     s->d_verts = dev_alloc<float>(9 * tri_total);
     if (any_normals) s->d_normals = dev_alloc<float>(9 * tri_total);
     {
@@ -642,7 +659,10 @@ int pyre_scene_bounds(pyre_ctx* ctx, pyre_scene* s, float out_min[3], float out_
 int pyre_bvh_build(pyre_ctx* ctx, pyre_scene* s, const pyre_bvh_params* p) {
   if (!ctx || !s) return PYRE_EINVAL;
   pyre_bvh_params def{12, 8, 1.0f, 2.0f};  // bvh.rs:13-16
-  return guarded(ctx, [&] { bvh_build(s, p ? *p : def, ctx->stream); });
+  return guarded(ctx, [&] {
+    bvh_build(s, p ? *p : def, ctx->stream);
+    trace_scratch_reserve(ctx, s->depth);
+  });
 }
 
 int pyre_bvh_info(pyre_ctx* ctx, pyre_scene* s, uint32_t* n_nodes, uint32_t* n_leaves, uint32_t* depth,

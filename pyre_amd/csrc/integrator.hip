@@ -993,17 +993,36 @@ struct Tuning {
   uint32_t pool_paths = 1u << 27;
   uint32_t blocks_per_cu = 8;
   uint32_t ring = 8;
-  uint32_t shade_waves = 4;  // >= 4: shade kernel compiled for 4 waves per SIMD (1: no occupancy target)
+  uint32_t shade_waves = 4;  // 4: shade kernel compiled for 4 waves per SIMD (1: no occupancy target)
 };
 
-Tuning env_tuning() {
+// The context's pyre_tuning resolved to values.
+Tuning ctx_tuning(const pyre_ctx* ctx) {
   Tuning t;
-  if (const char* e = std::getenv("PYRE_PASS_PATHS")) t.pool_paths = (uint32_t)std::max(1, std::atoi(e));
-  if (const char* e = std::getenv("PYRE_TRACE_BLOCKS_PER_CU")) t.blocks_per_cu = (uint32_t)std::max(1, std::atoi(e));
-  if (const char* e = std::getenv("PYRE_STACK_RING")) t.ring = (uint32_t)std::atoi(e);
-  if (t.ring != 4 && t.ring != 16) t.ring = 8;
-  if (const char* e = std::getenv("PYRE_SHADE_WAVES")) t.shade_waves = (uint32_t)std::max(1, std::atoi(e));
+  if (ctx->tune.pool_paths) t.pool_paths = ctx->tune.pool_paths;
+  t.blocks_per_cu = tune_blocks_per_cu(ctx);
+  t.ring = tune_ring(ctx);
+  t.shade_waves = ctx->tune.shade_waves == 1 ? 1u : 4u;
   return t;
+}
+
+// Device bytes per pool path (Work::alloc): ray 32, stream 32, throughput and
+// pdf 16, radiance and flags 16, AOVs 32, hit 16, shutter time 4, the three
+// queues, the job id and the shadow-slot count 20, and per shadow slot a ray
+// (32) and its contribution (16); a radiance-cache pool adds a query (52),
+// its path (4), the cache output (12) and a vertex count (4).
+size_t path_bytes(uint32_t slots, bool cache) { return 168 + 48 * (size_t)slots + (cache ? 72 : 0); }
+
+// Paths a pool may hold: the tuning's pool size, bounded by the tuning's
+// share of the device's free memory. `reclaimable` = bytes of a pool that is
+// released before the new one is allocated; `fixed` = bytes the new pool
+// needs whatever its size.
+uint64_t pool_limit(const pyre_ctx* ctx, const Tuning& t, size_t per_path, size_t fixed, size_t reclaimable) {
+  size_t free_b = 0, total_b = 0;
+  PYRE_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+  const double budget = (double)tune_memory_fraction(ctx) * (double)(free_b + reclaimable) - (double)fixed;
+  const double by_mem = budget > 0.0 ? budget / (double)per_path : 0.0;
+  return std::min<uint64_t>(t.pool_paths, (uint64_t)by_mem);
 }
 
 // Device path state for up to `cap` paths, plus the job/slice/pass tables.
@@ -1021,12 +1040,14 @@ struct Work {
   int2* gstack = nullptr;  // traversal stack spill columns
   size_t gstack_cap = 0;
   std::vector<void*> bufs;
+  size_t bytes = 0;  // device memory held
   float* tfeat = nullptr;
   float* ttarget = nullptr;
   template <class T>
   T* get(size_t n) {
     T* p = dev_alloc<T>(n);
     bufs.push_back(p);
+    bytes += std::max<size_t>(n, 1) * sizeof(T);
     return p;
   }
   Work() = default;
@@ -1054,7 +1075,9 @@ struct Work {
     tfeat = o.tfeat;
     ttarget = o.ttarget;
     bufs = std::move(o.bufs);
+    bytes = o.bytes;
     o.bufs.clear();
+    o.bytes = 0;
     o.cap = 0;
     o.gstack = nullptr;
     o.gstack_cap = 0;
@@ -1063,6 +1086,7 @@ struct Work {
   void release() {
     for (void* p : bufs) (void)hipFree(p);
     bufs.clear();
+    bytes = 0;
     cap = 0;
     gstack = nullptr;
     gstack_cap = 0;
@@ -1128,6 +1152,7 @@ struct Work {
     if (gstack) {
       bufs.erase(std::find(bufs.begin(), bufs.end(), (void*)gstack));
       (void)hipFree(gstack);
+      bytes -= gstack_cap * sizeof(int2);
     }
     gstack = get<int2>(need);
     gstack_cap = need;
@@ -1186,19 +1211,48 @@ unsigned resident_blocks(K kernel, unsigned want) {
   return std::min<unsigned>(want, (unsigned)n);
 }
 
+// Resident blocks per CU of every stage kernel, queried once per process
+// before the first launch (stage_occ() is called outside any stream capture,
+// so no runtime query is made while a pass is being captured).
+struct StageOcc {
+  unsigned extend[3][2], shadow[3][2];  // [ring 4 / 8 / 16][static / moving]
+  unsigned generate, accumulate, shade1, shade4, train_targets, cache_apply;
+};
+
+const StageOcc& stage_occ() {
+  static const StageOcc o = [] {
+    StageOcc r{};
+    r.extend[0][0] = resident_blocks(k_extend<4, false>, 64);
+    r.extend[1][0] = resident_blocks(k_extend<8, false>, 64);
+    r.extend[2][0] = resident_blocks(k_extend<16, false>, 64);
+    r.extend[1][1] = resident_blocks(k_extend<8, true>, 64);
+    r.shadow[0][0] = resident_blocks(k_shadow<4, false>, 64);
+    r.shadow[1][0] = resident_blocks(k_shadow<8, false>, 64);
+    r.shadow[2][0] = resident_blocks(k_shadow<16, false>, 64);
+    r.shadow[1][1] = resident_blocks(k_shadow<8, true>, 64);
+    r.generate = resident_blocks(k_generate, 8);
+    r.accumulate = resident_blocks(k_accumulate, 8);
+    r.shade1 = resident_blocks(k_shade<1>, 8);
+    r.shade4 = resident_blocks(k_shade<4>, 8);
+    r.train_targets = resident_blocks(k_train_targets, 8);
+    r.cache_apply = resident_blocks(k_cache_apply, 8);
+    return r;
+  }();
+  return o;
+}
+
+constexpr int ring_index(int ring) { return ring == 4 ? 0 : ring == 16 ? 2 : 1; }
+
 template <int RING, bool MOTION>
 void launch_trace_stages(const Work& W, Paths P, unsigned cus, unsigned bpc, bool shadow, hipStream_t st,
                          bool extend, bool one_job, bool first) {
-  if (extend) {
-    static const unsigned occ = resident_blocks(k_extend<RING, MOTION>, 64);
-    hipLaunchKernelGGL((k_extend<RING, MOTION>), dim3(cus * std::min(bpc, occ)), dim3(kThreads), 0, st, P,
-                       (const JobDev*)W.jobs, W.gstack, one_job, first);
-  }
-  if (shadow) {
-    static const unsigned occ = resident_blocks(k_shadow<RING, MOTION>, 64);
-    hipLaunchKernelGGL((k_shadow<RING, MOTION>), dim3(cus * std::min(bpc, occ)), dim3(kThreads), 0, st, P,
-                       (const JobDev*)W.jobs, W.gstack, one_job);
-  }
+  const StageOcc& occ = stage_occ();
+  if (extend)
+    hipLaunchKernelGGL((k_extend<RING, MOTION>), dim3(cus * std::min(bpc, occ.extend[ring_index(RING)][MOTION])),
+                       dim3(kThreads), 0, st, P, (const JobDev*)W.jobs, W.gstack, one_job, first);
+  if (shadow)
+    hipLaunchKernelGGL((k_shadow<RING, MOTION>), dim3(cus * std::min(bpc, occ.shadow[ring_index(RING)][MOTION])),
+                       dim3(kThreads), 0, st, P, (const JobDev*)W.jobs, W.gstack, one_job);
 }
 
 // tgrid = CUs x t.blocks_per_cu (the spill columns are sized for it); each
@@ -1222,8 +1276,8 @@ void launch_pass(pyre_ctx* ctx, Work& W, const PassShape& shape, const Tuning& t
   // grid-stride stages: as many blocks per CU as are resident at once (a
   // second, partial round of blocks would leave most of the GPU idle)
   const unsigned cus = (unsigned)std::max(1, ctx->cus);
-  static const unsigned gen_bpc = resident_blocks(k_generate, 8), acc_bpc = resident_blocks(k_accumulate, 8),
-                        shade1_bpc = resident_blocks(k_shade<1>, 8), shade4_bpc = resident_blocks(k_shade<4>, 8);
+  const StageOcc& occ = stage_occ();
+  const unsigned gen_bpc = occ.generate, acc_bpc = occ.accumulate, shade1_bpc = occ.shade1, shade4_bpc = occ.shade4;
   const unsigned tgrid = trace_grid_of(ctx, t);
   const Sched sc = W.sched();
   Paths P = W.P;
@@ -1242,10 +1296,7 @@ void launch_pass(pyre_ctx* ctx, Work& W, const PassShape& shape, const Tuning& t
   if (accumulate) hipLaunchKernelGGL(k_accumulate, dim3(cus * acc_bpc), dim3(kThreads), 0, st, P, sc);
 }
 
-bool graphs_enabled() {
-  const char* e = std::getenv("PYRE_GRAPH");
-  return !(e && std::strcmp(e, "0") == 0);
-}
+bool graphs_enabled(const pyre_ctx* ctx) { return ctx->tune.eager == 0; }
 
 hipGraphExec_t capture(hipStream_t st, const std::function<void()>& body) {
   hipGraph_t g = nullptr;
@@ -1280,14 +1331,19 @@ int sguard(pyre_ctx* ctx, F&& f) {
 struct pyre_session {
   pyre_ctx* ctx = nullptr;
   pyre_scene* s = nullptr;
+  pyre_camera_desc cam{};
   pyre_integrator_params ip{};
-  uint32_t width = 0, height = 0, npix = 0, per_pass = 1;
+  uint32_t width = 0, height = 0, npix = 0;
+  uint32_t per_pass = 0;  // samples per pixel a pass of the current pool holds (0: no pool yet)
+  uint32_t train_paths = 0, slots = 1, vmax = 0, tcap = 0;
   Tuning tune;
   PassShape shape;
   Work W;
   float* acc = nullptr;
   double* acc64 = nullptr;
   float* welford = nullptr;
+  unsigned long long* stats = nullptr;  // ray / cache counters over the session's lifetime
+  bool bound = false;                   // W's job table and counters are this session's
   uint32_t spp_done = 0, next_sample = 0, train_frames = 0;
   uint64_t stats_seen[2] = {0, 0};  // ray counters already reported
   bool use_graph = true;
@@ -1302,6 +1358,7 @@ struct pyre_session {
     dev_free(acc);
     dev_free(acc64);
     dev_free(welford);
+    dev_free(stats);
   }
 };
 
@@ -1333,14 +1390,15 @@ void set_pass(pyre_session* S, uint32_t s_base, uint32_t n_samples, uint32_t mod
 // Samples [s_base, s_base + n_samples) of every pixel (a CACHE pass may
 // also carry training paths, whose samples land in the training buffers).
 // Every stage reads its pass from device memory, so one captured graph per
-// kind of pass (plain, cache) serves any sample count; PYRE_GRAPH=0 launches
-// eagerly.
+// kind of pass (plain, cache) serves any sample count; a context tuned
+// `eager` launches the stages directly.
 void session_pass(pyre_session* S, uint32_t s_base, uint32_t n_samples, uint32_t mode, pyre_nrc* nrc,
                   float diffuse_rough, bool accumulate, const TrainPaths& tr = TrainPaths{},
                   const std::function<void()>& between = nullptr) {
   hipStream_t st = S->ctx->stream;
   require(mode == MODE_CACHE || tr.paths == 0, "training paths ride in cache passes only");
-  require((size_t)n_samples * S->npix + tr.paths <= S->W.cap, "pass exceeds the path pool");
+  require(S->bound && (size_t)n_samples * S->npix + tr.paths <= S->W.cap, "pass exceeds the path pool");
+  (void)stage_occ();  // occupancy queries before any capture
   set_pass(S, s_base, n_samples, mode, diffuse_rough, tr);
   if (mode == MODE_CACHE) {
     // trace (+ training targets), then `between` (the frame's training
@@ -1351,7 +1409,7 @@ void session_pass(pyre_session* S, uint32_t s_base, uint32_t n_samples, uint32_t
     auto trace = [&] {
       launch_pass(S->ctx, S->W, S->shape, S->tune, false, st);
       if (S->W.vmax) {
-        static const unsigned tt_bpc = resident_blocks(k_train_targets, 8);
+        const unsigned tt_bpc = stage_occ().train_targets;
         hipLaunchKernelGGL(k_train_targets, dim3((unsigned)std::max(1, S->ctx->cus) * tt_bpc), dim3(kThreads), 0, st, S->W.P, S->W.sched(),
                            S->W.tfeat, S->W.ttarget, S->W.tcap);
         hipLaunchKernelGGL(k_count_train, dim3(1), dim3(1), 0, st, S->W.P);
@@ -1360,7 +1418,7 @@ void session_pass(pyre_session* S, uint32_t s_base, uint32_t n_samples, uint32_t
     auto finish = [&] {
       nrc_infer_dev_count(nrc, S->W.P.qfeat, S->W.P.qpath, S->W.P.counters + 3, (uint32_t)S->W.cap, S->W.P.qrgb,
                           st);
-      static const unsigned ap_bpc = resident_blocks(k_cache_apply, 8), acc_bpc = resident_blocks(k_accumulate, 8);
+      const unsigned ap_bpc = stage_occ().cache_apply, acc_bpc = stage_occ().accumulate;
       hipLaunchKernelGGL(k_cache_apply, dim3((unsigned)std::max(1, S->ctx->cus) * ap_bpc), dim3(kThreads), 0, st, S->W.P);
       hipLaunchKernelGGL(k_accumulate, dim3((unsigned)std::max(1, S->ctx->cus) * acc_bpc), dim3(kThreads), 0, st, S->W.P, S->W.sched());
     };
@@ -1398,35 +1456,25 @@ void clear_sums(pyre_ctx* ctx, float* acc, double* acc64, size_t npix) {
   hipLaunchKernelGGL(k_fill_f64, dim3(nblocks(3 * npix, 256)), dim3(256), 0, st, acc64, 3 * npix, 0.0);
 }
 
-// spp_hint > 0: the session renders at most that many samples per pixel
-// (a one-shot render), so the path pool need not exceed spp_hint images.
 void session_init(pyre_session* S, pyre_ctx* ctx, pyre_scene* s, const pyre_camera_desc& cam,
-                  const pyre_integrator_params& ip, uint32_t w, uint32_t h, uint32_t train_paths,
-                  uint32_t spp_hint = 0) {
+                  const pyre_integrator_params& ip, uint32_t w, uint32_t h, uint32_t train_paths) {
   require(w > 0 && h > 0, "width and height must be positive");
   require(s->built, "render before pyre_bvh_build");
   require((uint64_t)w * h < (1ull << 28), "image too large");
   require(train_paths < (1u << 24), "too many training paths");
   S->ctx = ctx;
   S->s = s;
+  S->cam = cam;
   S->ip = ip;
   S->width = w;
   S->height = h;
   S->npix = w * h;
-  S->tune = env_tuning();
-  // paths per pass: whole samples of the image, about pool_paths paths
-  S->per_pass = std::max(1u, S->tune.pool_paths / S->npix);
-  if (spp_hint) S->per_pass = std::min(S->per_pass, spp_hint);
-  // a radiance-cache frame carries its training paths after a full pass of samples
-  const size_t cap = (size_t)S->per_pass * S->npix + train_paths;
-  require(cap < (1ull << 31), "path pool too large");
-  const uint32_t slots = std::max(1u, s->n_lights + (s->has_env ? 1u : 0u));
+  S->tune = ctx_tuning(ctx);
+  S->train_paths = train_paths;
+  S->slots = std::max(1u, s->n_lights + (s->has_env ? 1u : 0u));
   // vertices recorded per training path (at most 64: one bit each in the target pass)
-  const uint32_t vmax = train_paths ? std::min(64u, std::max(1u, ip.max_depth)) : 0;
-  const uint32_t tcap = train_paths ? std::min<uint32_t>(train_paths * vmax, 1u << 20) : 0;
-  if (!S->W.fits(cap, slots, vmax, train_paths, tcap, train_paths > 0, 1, 1, 1))
-    S->W.alloc(cap, slots, vmax, train_paths, tcap, train_paths > 0, 1, 1, 1);
-  S->W.ensure_gstack((size_t)trace_grid_of(ctx, S->tune) * kThreads, s->depth);
+  S->vmax = train_paths ? std::min(64u, std::max(1u, ip.max_depth)) : 0;
+  S->tcap = train_paths ? std::min<uint32_t>(train_paths * S->vmax, 1u << 20) : 0;
   S->shape.max_depth = ip.max_depth;
   S->shape.tree_depth = s->depth;
   S->shape.nee = s->n_lights + (s->has_env ? 1u : 0u) > 0;
@@ -1435,19 +1483,62 @@ void session_init(pyre_session* S, pyre_ctx* ctx, pyre_scene* s, const pyre_came
   S->acc = dev_alloc<float>(10 * (size_t)S->npix);
   S->acc64 = dev_alloc<double>(3 * (size_t)S->npix);
   S->welford = dev_alloc<float>(3 * (size_t)S->npix);
+  S->stats = dev_alloc<unsigned long long>(4);
   clear_sums(ctx, S->acc, S->acc64, S->npix);
   hipStream_t st = ctx->stream;
   hipLaunchKernelGGL(k_fill_f32, dim3(nblocks(3 * (size_t)S->npix, 256)), dim3(256), 0, st, S->welford,
                      3 * (size_t)S->npix, 0.0f);
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, S->stats, (size_t)4);
   S->spp_done = 0;
+  S->use_graph = graphs_enabled(ctx);
+}
+
+// A path pool for passes of up to `want` samples per pixel (plus the
+// session's training paths): at most the tuning's pool and its share of free
+// device memory, at least one sample per pixel. A pool that already holds
+// that many is kept; growing it drops the captured pass graphs, which hold
+// the old buffers' addresses.
+void session_reserve(pyre_session* S, uint32_t want) {
+  want = std::max(1u, want);
+  const size_t tp = S->train_paths;
+  const size_t fixed = tp * S->vmax * kVrec * sizeof(float4) + (size_t)S->tcap * 16 * sizeof(float);
+  const bool cache = tp > 0;
+  const size_t held = (S->W.cap > tp ? S->W.cap - tp : 0) / S->npix;
+  uint64_t per = want;
+  if (!(S->W.fits(S->npix + tp, S->slots, S->vmax, S->train_paths, S->tcap, cache, 1, 1, 1) && held >= want)) {
+    const uint64_t limit = pool_limit(S->ctx, S->tune, path_bytes(S->slots, cache), fixed, S->W.bytes);
+    per = limit > tp ? (limit - tp) / S->npix : 0;
+    per = std::min<uint64_t>(per, ((1ull << 31) - 1 - tp) / S->npix);
+    per = std::max<uint64_t>(1, std::min<uint64_t>(per, want));
+    if (!S->W.fits(per * S->npix + tp, S->slots, S->vmax, S->train_paths, S->tcap, cache, 1, 1, 1)) {
+      for (hipGraphExec_t* e : {&S->exec, &S->exec_cache, &S->exec_finish})
+        if (*e) {
+          (void)hipGraphExecDestroy(*e);
+          *e = nullptr;
+        }
+      S->cache_of = nullptr;
+      S->W.release();
+      S->W.alloc(per * S->npix + tp, S->slots, S->vmax, S->train_paths, S->tcap, cache, 1, 1, 1);
+      S->bound = false;
+    }
+  }
+  S->per_pass = (uint32_t)per;
+  if (S->bound) return;
+  // a new pool, or one another render left in the context: this session's
+  // job table, counters and spill columns
+  S->W.ensure_gstack((size_t)trace_grid_of(S->ctx, S->tune) * kThreads, S->s->depth);
+  S->W.P.stats = S->stats;  // the session's counters outlive its pools
+  S->bound = true;
+  hipStream_t st = S->ctx->stream;
   hipLaunchKernelGGL(k_set_job, dim3(1), dim3(1), 0, st, S->W.jobs,
-                     job_dev(s, cam, ip, w, h, S->acc, S->acc64, S->welford));
+                     job_dev(S->s, S->cam, S->ip, S->width, S->height, S->acc, S->acc64, S->welford));
   hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, st, S->W.pass_idx, 0u);
-  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, S->W.P.stats, (size_t)4);
-  S->use_graph = graphs_enabled();
+  PYRE_HIP_CHECK(hipGetLastError());
 }
 
 void session_render(pyre_session* S, uint32_t spp) {
+  if (!spp) return;
+  session_reserve(S, spp);
   for (uint32_t s0 = 0; s0 < spp; s0 += S->per_pass) {
     uint32_t ns = std::min(S->per_pass, spp - s0);
     session_pass(S, S->next_sample, ns, MODE_PLAIN, nullptr, 0.25f, true);
@@ -1469,7 +1560,11 @@ void read_stats(pyre_ctx* ctx, const Work& W, uint64_t* ext, uint64_t* shadow) {
 // session's lifetime).
 void finish_stats(pyre_session* S) {
   uint64_t ext = 0, sh = 0;
-  read_stats(S->ctx, S->W, &ext, &sh);
+  unsigned long long hs[2] = {0, 0};
+  PYRE_HIP_CHECK(hipMemcpyAsync(hs, S->stats, 16, hipMemcpyDeviceToHost, S->ctx->stream));
+  PYRE_HIP_CHECK(hipStreamSynchronize(S->ctx->stream));
+  ext = hs[0];
+  sh = hs[1];
   S->s->last_extension_rays = ext - S->stats_seen[0];
   S->s->last_shadow_rays = sh - S->stats_seen[1];
   S->stats_seen[0] = ext;
@@ -1551,23 +1646,23 @@ struct Batch {
   }
 };
 
-// Paths per pass of a batch: the tuning's pool, no more than the whole
-// batch needs (the path buffers stay cached in the context), and at least
-// one image of the largest job.
-uint32_t batch_pool(const Tuning& t, const Batch& B) {
-  return (uint32_t)std::max<uint64_t>(std::min<uint64_t>(t.pool_paths, B.total_paths), B.max_npix);
+// Paths per pass of a batch: the tuning's pool within its share of free
+// device memory (`reclaimable` = the cached pool's bytes), no more than the
+// whole batch needs (the path buffers stay cached in the context), and at
+// least one image of the largest job.
+uint32_t batch_pool(const pyre_ctx* ctx, const Tuning& t, const Batch& B, size_t reclaimable) {
+  const uint64_t lim = pool_limit(ctx, t, path_bytes(B.slots, false), 0, reclaimable);
+  return (uint32_t)std::max<uint64_t>(std::min<uint64_t>(lim, B.total_paths), B.max_npix);
 }
 
 // Runs the whole schedule with tuning t into the jobs' sums (accumulate) or,
 // for a trial, its first `max_passes` passes without touching them; returns
 // the GPU time in ms.
 float run_batch(Batch& B, const Tuning& t, bool accumulate, uint32_t max_passes, uint64_t* ext, uint64_t* shadow,
-                uint32_t* n_passes) {
+                uint32_t* n_passes, uint32_t* pool_used = nullptr) {
   pyre_ctx* ctx = B.ctx;
   hipStream_t st = ctx->stream;
-  const uint32_t pool = batch_pool(t, B);
-  Schedule sc = make_schedule(B.jobs, B.n, pool);
-  const uint32_t np = std::min<uint32_t>((uint32_t)sc.passes.size(), max_passes);
+  (void)stage_occ();  // occupancy queries before any capture
   // the context's cached path buffers, when they are large enough (one-shot
   // renders and batches leave theirs there)
   Work W;
@@ -1575,8 +1670,15 @@ float run_batch(Batch& B, const Tuning& t, bool accumulate, uint32_t max_passes,
     W = std::move(*static_cast<Work*>(ctx->render_cache.get()));
     ctx->render_cache.reset();
   }
+  const uint32_t pool = batch_pool(ctx, t, B, W.bytes);
+  if (pool_used) *pool_used = pool;
+  Schedule sc = make_schedule(B.jobs, B.n, pool);
+  const uint32_t np = std::min<uint32_t>((uint32_t)sc.passes.size(), max_passes);
   const size_t n_sl = std::max<size_t>(1, sc.slices.size()), n_pa = std::max<size_t>(1, sc.passes.size());
-  if (!W.fits(pool, B.slots, 0, 0, 0, false, B.n, n_sl, n_pa)) W.alloc(pool, B.slots, 0, 0, 0, false, B.n, n_sl, n_pa);
+  if (!W.fits(pool, B.slots, 0, 0, 0, false, B.n, n_sl, n_pa)) {
+    W.release();
+    W.alloc(pool, B.slots, 0, 0, 0, false, B.n, n_sl, n_pa);
+  }
   W.ensure_gstack((size_t)trace_grid_of(ctx, t) * kThreads, B.shape.tree_depth);
   std::vector<JobDev> jd(B.n);
   for (size_t j = 0; j < B.n; ++j)
@@ -1600,7 +1702,7 @@ float run_batch(Batch& B, const Tuning& t, bool accumulate, uint32_t max_passes,
   hipGraphExec_t exec = nullptr;
   float ms = 0.0f;
   try {
-    if (graphs_enabled() && np > 1) exec = capture(st, one_pass);
+    if (graphs_enabled(ctx) && np > 1) exec = capture(st, one_pass);
     PYRE_HIP_CHECK(hipEventRecord(e0, st));
     for (uint32_t k = 0; k < np; ++k) {
       if (exec)
@@ -1687,7 +1789,7 @@ int pyre_render(pyre_ctx* ctx, pyre_scene* s, const pyre_camera_desc* camera, co
       S.W = std::move(*static_cast<Work*>(ctx->render_cache.get()));
       ctx->render_cache.reset();
     }
-    session_init(&S, ctx, s, *camera, *integrator, width, height, 0, std::max(1u, spp));
+    session_init(&S, ctx, s, *camera, *integrator, width, height, 0);
     hipStream_t st = ctx->stream;
     const size_t n3 = 3 * (size_t)S.npix;
     if (accum64) PYRE_HIP_CHECK(hipMemcpyAsync(S.acc64, accum64, n3 * sizeof(double), hipMemcpyHostToDevice, st));
@@ -1744,7 +1846,7 @@ int pyre_render_batch(pyre_ctx* ctx, const pyre_job* jobs, size_t n, const pyre_
       if (jobs[j].accum64)
         PYRE_HIP_CHECK(hipMemcpyAsync(B.acc64[j], jobs[j].accum64, 3 * npix * sizeof(double), hipMemcpyHostToDevice, st));
     }
-    Tuning t = env_tuning();
+    Tuning t = ctx_tuning(ctx);
     if (params) {
       if (params->pool_paths) t.pool_paths = params->pool_paths;
       if (params->trace_blocks_per_cu) t.blocks_per_cu = params->trace_blocks_per_cu;
@@ -1753,7 +1855,8 @@ int pyre_render_batch(pyre_ctx* ctx, const pyre_job* jobs, size_t n, const pyre_
     }
     uint64_t ext = 0, sh = 0;
     uint32_t np = 0;
-    float ms = run_batch(B, t, true, 0xFFFFFFFFu, &ext, &sh, &np);
+    uint32_t pool_used = 0;
+    float ms = run_batch(B, t, true, 0xFFFFFFFFu, &ext, &sh, &np, &pool_used);
     // means (the sums scaled in place: the batch's sums are not kept) and
     // FP64 sums, copied out with one synchronisation at the end
     for (size_t j = 0; j < n; ++j) {
@@ -1772,7 +1875,7 @@ int pyre_render_batch(pyre_ctx* ctx, const pyre_job* jobs, size_t n, const pyre_
       stats->shadow_rays = sh;
       stats->passes = np;
       stats->gpu_ms = ms;
-      stats->pool_paths = batch_pool(t, B);
+      stats->pool_paths = pool_used;
       stats->trace_blocks_per_cu = t.blocks_per_cu;
       stats->stack_ring = t.ring;
     }
@@ -1813,15 +1916,24 @@ int pyre_session_render(pyre_ctx* ctx, pyre_session* S, uint32_t spp) {
   });
 }
 
+int pyre_session_reserve(pyre_ctx* ctx, pyre_session* S, uint32_t spp_per_pass) {
+  if (!ctx || !S) return PYRE_EINVAL;
+  return sguard(ctx, [&] {
+    session_reserve(S, spp_per_pass);
+    PYRE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
 int pyre_session_render_nrc(pyre_ctx* ctx, pyre_session* S, pyre_nrc* nrc, const pyre_nrc_render_params* p,
                             float* losses) {
   if (!ctx || !S || !nrc || !p) return PYRE_EINVAL;
   return sguard(ctx, [&] {
-    require(S->W.vmax > 0, "session created without radiance-cache buffers (nrc_train_paths = 0)");
-    require(p->train_paths > 0 && p->train_paths <= S->W.tpaths,
+    require(S->vmax > 0, "session created without radiance-cache buffers (nrc_train_paths = 0)");
+    require(p->train_paths > 0 && p->train_paths <= S->train_paths,
             "train_paths exceeds the session's nrc_train_paths");
     require(p->train_batch >= 16, "train_batch below 16");
     hipStream_t st = ctx->stream;
+    session_reserve(S, p->spp_per_frame);
     // Per-frame loss sums and step counts stay on the device until the end:
     // the training steps are queued without knowing how many samples the
     // frame produced (each step reads the count and skips itself when its
@@ -1930,7 +2042,7 @@ int pyre_session_stats(pyre_ctx* ctx, pyre_session* S, uint64_t out[4]) {
   if (!ctx || !S || !out) return PYRE_EINVAL;
   return sguard(ctx, [&] {
     unsigned long long hs[4] = {0, 0, 0, 0};
-    PYRE_HIP_CHECK(hipMemcpyAsync(hs, S->W.P.stats, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
+    PYRE_HIP_CHECK(hipMemcpyAsync(hs, S->stats, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
     PYRE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     out[0] = hs[0];
     out[1] = hs[1];

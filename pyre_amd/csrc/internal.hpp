@@ -18,6 +18,16 @@ struct pyre_ctx {
   std::string err;
   std::string name;
   int cus = 0;
+  pyre_tuning tune{};  // launch settings (0 fields = defaults)
+  // traversal scratch of the trace entry points: work counter, stack-overflow
+  // flag, spill columns for the largest persistent grid (8 blocks per CU) on
+  // trees up to trace_depth; grown by pyre_bvh_build, never by a trace call.
+  // trace_ev orders calls that share it across streams.
+  uint32_t* trace_fetch = nullptr;
+  uint32_t* trace_flag = nullptr;
+  int2* trace_gstack = nullptr;
+  uint32_t trace_depth = 0;
+  hipEvent_t trace_ev = nullptr;
   // path buffers kept from the last one-shot render (owned; freed with the
   // context, on its device)
   std::shared_ptr<void> render_cache;
@@ -166,16 +176,12 @@ struct pyre_scene {
   uint32_t n_nodes2 = 0;
   pyre::Node4* d_nodes4 = nullptr;
   uint32_t n_nodes4 = 0, depth4 = 0;
-  int2* d_gstack = nullptr;  // spill columns of the persistent traversal kernels
-  size_t gstack_cap = 0;
-  uint32_t* d_fetch = nullptr;  // work counters of the batched trace kernels
   float4* d_tris = nullptr;        // 3 float4 per slot: (v0, gid bits), (e1, 0), (e2, 0)
   uint32_t* d_prim_order = nullptr;  // scene-global triangle id per slot
   pyre::BNode* d_bnodes = nullptr;
   uint32_t n_bnodes = 0;
   uint32_t n_leaves = 0, n_interior = 0, depth = 0;
   double build_ms = 0.0;
-  uint32_t* d_flag = nullptr;  // traversal overflow flag (sticky)
   uint64_t last_extension_rays = 0, last_shadow_rays = 0;  // of the last pyre_render
 };
 
@@ -190,8 +196,22 @@ void trace_closest(pyre_scene* s, const float4* rays, size_t n, pyre_hit8* hits,
                    bool check, const float* times = nullptr);
 void trace_any(pyre_scene* s, const float4* rays, size_t n, uint32_t* occluded, hipStream_t st, bool check,
                const float* times = nullptr);
-unsigned trace_grid(pyre_scene* s);
-// Spill columns for trace_grid(s) x 256 threads (allocated on first use and
-// whenever the tree gets deeper; never inside a stream capture).
-int2* trace_gstack(pyre_scene* s);
+// The context's trace scratch sized for trees of binary depth `depth`
+// (synchronises the device when it has to grow; called by pyre_bvh_build).
+void trace_scratch_reserve(pyre_ctx* ctx, uint32_t depth);
+void trace_scratch_release(pyre_ctx* ctx);
+
+// Resolved tuning values.
+inline uint32_t tune_blocks_per_cu(const pyre_ctx* c) {
+  const uint32_t v = c->tune.trace_blocks_per_cu;
+  return v == 0 || v > 8 ? 8u : v;
+}
+inline uint32_t tune_ring(const pyre_ctx* c) {
+  const uint32_t v = c->tune.stack_ring;
+  return v == 4 || v == 16 ? v : 8u;
+}
+inline float tune_memory_fraction(const pyre_ctx* c) {
+  const float f = c->tune.memory_fraction;
+  return f > 0.0f && f <= 1.0f ? f : 0.5f;
+}
 }  // namespace pyre

@@ -971,6 +971,14 @@ int pyre_nrc_gradients(pyre_ctx* ctx, pyre_nrc* N, const float* feats, const flo
   });
 }
 
+int pyre_nrc_reserve(pyre_ctx* ctx, pyre_nrc* N, size_t max_batch) {
+  if (!ctx || !N || !max_batch) return PYRE_EINVAL;
+  return nrc_guard(ctx, [&] {
+    require(max_batch < (1u << 26), "training batch too large");
+    ensure_cap(N, (uint32_t)max_batch);
+  });
+}
+
 int pyre_nrc_stats(pyre_ctx* ctx, pyre_nrc* N, uint64_t* steps, float* last_loss) {
   if (!ctx || !N) return PYRE_EINVAL;
   return nrc_guard(ctx, [&] {

@@ -61,7 +61,7 @@ PH float length(V3 a) { return sqrtf(dot(a, a)); }
 PH V3 normalize(V3 a) { return a * (1.0f / length(a)); }
 PH V3 normalize_or_zero(V3 a) {
   float rcp = 1.0f / length(a);
-  if (isfinite(rcp) && rcp > 0.0f) return a * rcp;
+  if (__builtin_isfinite(rcp) && rcp > 0.0f) return a * rcp;
   return V3{0.0f, 0.0f, 0.0f};
 }
 PH V3 vmin(V3 a, V3 b) { return V3{rmin(a.x, b.x), rmin(a.y, b.y), rmin(a.z, b.z)}; }

@@ -1160,6 +1160,16 @@ Json parse_json(const char* b, size_t n) {
 
 double num(const Json* j, double dflt) { return j && j->kind == Json::NUM ? j->n : dflt; }
 
+// A JSON number that must be a non-negative integer no larger than `max`
+// (absent: dflt). Rejected before any cast, so no huge, negative or
+// fractional value reaches size arithmetic.
+size_t json_size(const Json* j, size_t dflt, size_t max, const char* what) {
+  if (!j) return dflt;
+  if (j->kind != Json::NUM || !(j->n >= 0.0) || j->n != std::floor(j->n) || j->n > (double)max)
+    gltf_err(std::string("invalid ") + what);
+  return (size_t)j->n;
+}
+
 size_t index_of(const Json* j, size_t count, const char* what) {
   if (!j || j->kind != Json::NUM || j->n < 0 || j->n != std::floor(j->n) || j->n >= (double)count)
     gltf_err(std::string("invalid ") + what + " index");
@@ -1248,11 +1258,11 @@ struct Gltf {
     const Json& a = acc[i];
     Accessor r;
     if (a.get("sparse")) gltf_err("sparse accessors are not supported");
-    r.ctype = (uint32_t)num(a.get("componentType"), 0);
+    r.ctype = (uint32_t)json_size(a.get("componentType"), 0, 0xFFFFFFFFu, "accessor componentType");
     const Json* ty = a.get("type");
     r.comps = type_comps(ty && ty->kind == Json::STR ? ty->s : "");
     r.elem = comp_size(r.ctype) * r.comps;
-    r.count = (size_t)num(a.get("count"), 0);
+    r.count = json_size(a.get("count"), 0, 1ull << 40, "accessor count");
     const Json* nz = a.get("normalized");
     r.normalized = nz && nz->kind == Json::BOOL && nz->b;
     const Json* bvj = a.get("bufferView");
@@ -1263,12 +1273,19 @@ struct Gltf {
     const std::vector<Json>& views = arr(doc, "bufferViews");
     const Json& bv = views[index_of(bvj, views.size(), "bufferView")];
     const std::vector<uint8_t>& buf = buffers[index_of(bv.get("buffer"), buffers.size(), "buffer")];
-    const size_t view_off = (size_t)num(bv.get("byteOffset"), 0), view_len = (size_t)num(bv.get("byteLength"), 0);
-    const size_t off = (size_t)num(a.get("byteOffset"), 0);
-    r.stride = (size_t)num(bv.get("byteStride"), 0);
+    // every size is at most the buffer's, and bounds are checked by
+    // subtraction so no sum or product can wrap
+    const size_t view_off = json_size(bv.get("byteOffset"), 0, buf.size(), "bufferView byteOffset");
+    const size_t view_len = json_size(bv.get("byteLength"), 0, buf.size(), "bufferView byteLength");
+    if (view_len > buf.size() - view_off) gltf_err("bufferView exceeds its buffer");
+    const size_t off = json_size(a.get("byteOffset"), 0, view_len, "accessor byteOffset");
+    r.stride = json_size(bv.get("byteStride"), 0, 252, "bufferView byteStride");
     if (r.stride == 0) r.stride = r.elem;
-    if (view_off + view_len > buf.size()) gltf_err("bufferView exceeds its buffer");
-    if (r.count && (off + (r.count - 1) * r.stride + r.elem > view_len)) gltf_err("accessor exceeds its bufferView");
+    if (r.stride < r.elem) gltf_err("bufferView byteStride below its accessor's element size");
+    if (r.count) {
+      if (r.elem > view_len - off) gltf_err("accessor exceeds its bufferView");
+      if (r.count - 1 > (view_len - off - r.elem) / r.stride) gltf_err("accessor exceeds its bufferView");
+    }
     r.data = buf.data() + view_off + off;
     return r;
   }
@@ -1436,7 +1453,7 @@ void load_gltf(const std::string& path, pyre_preset* p) {
     } else {
       data = read_bytes(join(base, percent_decode(uri->s)));
     }
-    const size_t len = (size_t)num(bufs[i].get("byteLength"), 0);
+    const size_t len = json_size(bufs[i].get("byteLength"), 0, 1ull << 40, "buffer byteLength");
     if (data.size() < len) gltf_err("buffer shorter than its byteLength");
     g.buffers.push_back(std::move(data));
   }

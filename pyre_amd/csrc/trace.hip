@@ -2,12 +2,11 @@
 // ray array) built on the traversal cores in traverse.hpp.
 //
 // Default: the 4-wide tree, persistent threads with dynamic ray fetch, an
-// 8-entry LDS stack ring per thread spilling to a per-thread global column. The binary
-// tree kernels (LDS stack as deep as the tree) remain selectable with
-// PYRE_TRAVERSAL=binary for comparison.
+// 8-entry LDS stack ring per thread spilling to a per-thread global column.
+// The binary-tree kernels (LDS stack as deep as the tree) and a grid-stride
+// form of the 4-wide kernel remain selectable through the context's tuning
+// (pyre_tuning.traversal) for comparison.
 #include <algorithm>
-#include <cstdlib>
-#include <cstring>
 
 #include "traverse.hpp"
 
@@ -105,7 +104,7 @@ __global__ __launch_bounds__(kTraceThreads, 8) void k_trace4(const Node4* __rest
 }
 
 // Grid-stride variant (each lane keeps its ray to the end), selectable with
-// PYRE_TRACE_FETCH=static for comparison.
+// pyre_tuning.traversal = 2 for comparison.
 template <bool ANY>
 __global__ __launch_bounds__(kTraceThreads) void k_trace4s(const Node4* __restrict__ nodes,
                                                            const float4* __restrict__ tris,
@@ -138,17 +137,15 @@ __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
   if (i < n) p[i] = 0u;
 }
 
-// LDS stack ring entries per thread (PYRE_STACK_RING = 4, 8 or 16; default 8).
-int trace_ring() {
-  const char* e = std::getenv("PYRE_STACK_RING");
-  const int r = e ? std::atoi(e) : 8;
-  return (r == 4 || r == 16) ? r : 8;
+// Persistent grid of the 4-wide kernels: the tuning's blocks per CU x CUs.
+unsigned trace_grid(const pyre_ctx* ctx) {
+  return (unsigned)std::max(1, ctx->cus) * tune_blocks_per_cu(ctx);
 }
 
-bool use_binary() {
-  const char* e = std::getenv("PYRE_TRAVERSAL");
-  return e && std::strcmp(e, "binary") == 0;
-}
+// Spill-column entries per thread: a 4-wide node visit leaves at most 3
+// pending siblings per level of the current path, and the 4-wide tree is no
+// deeper than the binary one: 3 * depth entries (+ 8 spare).
+size_t spill_entries(uint32_t depth) { return 3 * (size_t)depth + 8; }
 
 template <bool ANY>
 void launch(pyre_scene* s, const float4* rays, size_t n, pyre_hit8* hit, float2* uv, uint32_t* occ, hipStream_t st,
@@ -156,64 +153,55 @@ void launch(pyre_scene* s, const float4* rays, size_t n, pyre_hit8* hit, float2*
   require(s->built, "trace before pyre_bvh_build");
   require(n <= 0xFFFFFFFFull, "too many rays in one call");
   if (n == 0) return;
-  uint32_t* ovf = s->d_flag;
+  pyre_ctx* ctx = s->ctx;
+  require(ctx->trace_gstack && ctx->trace_depth >= s->depth, "trace scratch smaller than the scene's tree");
+  const uint32_t mode = ctx->tune.traversal;
+  require(mode <= 2, "pyre_tuning.traversal must be 0, 1 or 2");
+  require(mode != 1 || s->depth + 1 <= 48, "binary traversal supports trees of depth <= 47");
+  // calls sharing the context's scratch run in the order they were made,
+  // whatever streams they are on
+  PYRE_HIP_CHECK(hipStreamWaitEvent(st, ctx->trace_ev, 0));
+  const unsigned grid = trace_grid(ctx);
+  // spill columns are laid out for the grid in use (gstride = its threads)
+  int2* gst = ctx->trace_gstack;
   if (s->has_motion) {
     // moving meshes: the persistent 4-wide kernel with shifted leaf tests
-    int2* gst = trace_gstack(s);
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, s->d_fetch, 1u);
-    hipLaunchKernelGGL((k_trace4<ANY, 8, true>), dim3(trace_grid(s)), dim3(kTraceThreads), 0, st, s->d_nodes4,
-                       s->d_tris, rays, (uint32_t)n, hit, uv, occ, gst, s->d_fetch, times, s->d_motion);
-    PYRE_HIP_CHECK(hipGetLastError());
-    return;
-  }
-  if (!use_binary()) {
-    int2* gst = trace_gstack(s);
-    const char* fe = std::getenv("PYRE_TRACE_FETCH");
-    if (fe && std::strcmp(fe, "static") == 0) {
-      hipLaunchKernelGGL((k_trace4s<ANY>), dim3(trace_grid(s)), dim3(kTraceThreads), 0, st, s->d_nodes4, s->d_tris,
-                         rays, (uint32_t)n, hit, uv, occ, gst);
-      PYRE_HIP_CHECK(hipGetLastError());
-      return;
-    }
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, s->d_fetch, 1u);
-    const int ring = trace_ring();
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, ctx->trace_fetch, 1u);
+    hipLaunchKernelGGL((k_trace4<ANY, 8, true>), dim3(grid), dim3(kTraceThreads), 0, st, s->d_nodes4, s->d_tris,
+                       rays, (uint32_t)n, hit, uv, occ, gst, ctx->trace_fetch, times, s->d_motion);
+  } else if (mode == 2) {
+    hipLaunchKernelGGL((k_trace4s<ANY>), dim3(grid), dim3(kTraceThreads), 0, st, s->d_nodes4, s->d_tris, rays,
+                       (uint32_t)n, hit, uv, occ, gst);
+  } else if (mode == 0) {
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, ctx->trace_fetch, 1u);
+    const uint32_t ring = tune_ring(ctx);
     if (ring == 4)
-      hipLaunchKernelGGL((k_trace4<ANY, 4>), dim3(trace_grid(s)), dim3(kTraceThreads), 0, st, s->d_nodes4, s->d_tris,
-                         rays, (uint32_t)n, hit, uv, occ, gst, s->d_fetch, nullptr, nullptr);
+      hipLaunchKernelGGL((k_trace4<ANY, 4>), dim3(grid), dim3(kTraceThreads), 0, st, s->d_nodes4, s->d_tris, rays,
+                         (uint32_t)n, hit, uv, occ, gst, ctx->trace_fetch, nullptr, nullptr);
     else if (ring == 16)
-      hipLaunchKernelGGL((k_trace4<ANY, 16>), dim3(trace_grid(s)), dim3(kTraceThreads), 0, st, s->d_nodes4,
-                         s->d_tris, rays, (uint32_t)n, hit, uv, occ, gst, s->d_fetch, nullptr, nullptr);
+      hipLaunchKernelGGL((k_trace4<ANY, 16>), dim3(grid), dim3(kTraceThreads), 0, st, s->d_nodes4, s->d_tris, rays,
+                         (uint32_t)n, hit, uv, occ, gst, ctx->trace_fetch, nullptr, nullptr);
     else
-      hipLaunchKernelGGL((k_trace4<ANY, 8>), dim3(trace_grid(s)), dim3(kTraceThreads), 0, st, s->d_nodes4, s->d_tris,
-                         rays, (uint32_t)n, hit, uv, occ, gst, s->d_fetch, nullptr, nullptr);
-    PYRE_HIP_CHECK(hipGetLastError());
-    return;
-  }
-  dim3 grid((unsigned)((n + kTraceThreads - 1) / kTraceThreads)), block(kTraceThreads);
-  int32_t* gst = nullptr;
-  const uint32_t need = s->depth + 1;
-  if (need <= 16)
-    hipLaunchKernelGGL((k_trace<ANY, 16>), grid, block, 0, st, s->d_nodes, s->d_tris, rays, (uint32_t)n, hit, uv, occ,
-                       gst, 0, ovf);
-  else if (need <= 32)
-    hipLaunchKernelGGL((k_trace<ANY, 32>), grid, block, 0, st, s->d_nodes, s->d_tris, rays, (uint32_t)n, hit, uv, occ,
-                       gst, 0, ovf);
-  else if (need <= 48)
-    hipLaunchKernelGGL((k_trace<ANY, 48>), grid, block, 0, st, s->d_nodes, s->d_tris, rays, (uint32_t)n, hit, uv, occ,
-                       gst, 0, ovf);
-  else {
-    gst = dev_alloc<int32_t>((size_t)need * n);
-    hipLaunchKernelGGL((k_trace<ANY, 0>), grid, block, 0, st, s->d_nodes, s->d_tris, rays, (uint32_t)n, hit, uv, occ,
-                       gst, (int)need, ovf);
+      hipLaunchKernelGGL((k_trace4<ANY, 8>), dim3(grid), dim3(kTraceThreads), 0, st, s->d_nodes4, s->d_tris, rays,
+                         (uint32_t)n, hit, uv, occ, gst, ctx->trace_fetch, nullptr, nullptr);
+  } else {
+    dim3 g((unsigned)((n + kTraceThreads - 1) / kTraceThreads)), block(kTraceThreads);
+    const uint32_t need = s->depth + 1;
+    if (need <= 16)
+      hipLaunchKernelGGL((k_trace<ANY, 16>), g, block, 0, st, s->d_nodes, s->d_tris, rays, (uint32_t)n, hit, uv, occ,
+                         nullptr, 0, ctx->trace_flag);
+    else if (need <= 32)
+      hipLaunchKernelGGL((k_trace<ANY, 32>), g, block, 0, st, s->d_nodes, s->d_tris, rays, (uint32_t)n, hit, uv, occ,
+                         nullptr, 0, ctx->trace_flag);
+    else
+      hipLaunchKernelGGL((k_trace<ANY, 48>), g, block, 0, st, s->d_nodes, s->d_tris, rays, (uint32_t)n, hit, uv, occ,
+                         nullptr, 0, ctx->trace_flag);
   }
   PYRE_HIP_CHECK(hipGetLastError());
-  if (gst) {
-    PYRE_HIP_CHECK(hipStreamSynchronize(st));
-    dev_free(gst);
-  }
-  if (check) {
+  PYRE_HIP_CHECK(hipEventRecord(ctx->trace_ev, st));
+  if (check && mode == 1) {
     uint32_t h_ovf = 0;
-    PYRE_HIP_CHECK(hipMemcpyAsync(&h_ovf, ovf, 4, hipMemcpyDeviceToHost, st));
+    PYRE_HIP_CHECK(hipMemcpyAsync(&h_ovf, ctx->trace_flag, 4, hipMemcpyDeviceToHost, st));
     PYRE_HIP_CHECK(hipStreamSynchronize(st));
     if (h_ovf) throw Error(PYRE_EOVERFLOW, "traversal stack overflow (tree deeper than recorded depth)");
   }
@@ -221,28 +209,29 @@ void launch(pyre_scene* s, const float4* rays, size_t n, pyre_hit8* hit, float2*
 
 }  // namespace
 
-// Persistent grid of the 4-wide kernels: blocks per CU (PYRE_TRACE_BLOCKS_PER_CU,
-// default 8) x CUs.
-unsigned trace_grid(pyre_scene* s) {
-  int per_cu = 8;
-  if (const char* e = std::getenv("PYRE_TRACE_BLOCKS_PER_CU")) per_cu = std::max(1, std::atoi(e));
-  return (unsigned)std::max(1, s->ctx->cus) * (unsigned)per_cu;
+void trace_scratch_reserve(pyre_ctx* ctx, uint32_t depth) {
+  if (!ctx->trace_ev) PYRE_HIP_CHECK(hipEventCreateWithFlags(&ctx->trace_ev, hipEventDisableTiming));
+  if (!ctx->trace_fetch) ctx->trace_fetch = dev_alloc<uint32_t>(64);
+  if (!ctx->trace_flag) {
+    ctx->trace_flag = dev_alloc<uint32_t>(1);
+    PYRE_HIP_CHECK(hipMemset(ctx->trace_flag, 0, 4));
+  }
+  if (ctx->trace_gstack && depth <= ctx->trace_depth) return;
+  // every grid the tuning allows (at most 8 blocks per CU) fits the columns
+  const size_t threads = (size_t)std::max(1, ctx->cus) * 8 * kTraceThreads;
+  PYRE_HIP_CHECK(hipDeviceSynchronize());  // earlier trace calls may still use the old columns
+  dev_free(ctx->trace_gstack);
+  ctx->trace_gstack = dev_alloc<int2>(threads * spill_entries(depth));
+  ctx->trace_depth = depth;
 }
 
-// Spill columns for the persistent kernels: a 4-wide node visit leaves at
-// most 3 pending siblings per level of the current path, and the 4-wide
-// tree is no deeper than the binary one: 3 * depth entries (+ 8 spare).
-int2* trace_gstack(pyre_scene* s) {
-  const size_t threads = (size_t)trace_grid(s) * kTraceThreads;
-  const size_t need = threads * (3 * (size_t)s->depth + 8);
-  if (!s->d_fetch) s->d_fetch = dev_alloc<uint32_t>(64);
-  if (need > s->gstack_cap) {
-    PYRE_HIP_CHECK(hipDeviceSynchronize());
-    dev_free(s->d_gstack);
-    s->d_gstack = dev_alloc<int2>(need);
-    s->gstack_cap = need;
-  }
-  return s->d_gstack;
+void trace_scratch_release(pyre_ctx* ctx) {
+  dev_free(ctx->trace_gstack);
+  dev_free(ctx->trace_fetch);
+  dev_free(ctx->trace_flag);
+  ctx->trace_depth = 0;
+  if (ctx->trace_ev) (void)hipEventDestroy(ctx->trace_ev);
+  ctx->trace_ev = nullptr;
 }
 
 void trace_closest(pyre_scene* s, const float4* rays, size_t n, pyre_hit8* hits, float2* uv, hipStream_t st,

@@ -73,24 +73,3 @@ def rel_mse(img, ref, eps=1e-2):
     img = np.asarray(img, np.float64)
     ref = np.asarray(ref, np.float64)
     return float(np.mean((img - ref) ** 2 / (ref ** 2 + eps)))
-
-
-class env_override:
-    """Temporarily set environment variables (the library reads its tuning
-    switches at each launch)."""
-
-    def __init__(self, **kv):
-        self.kv = kv
-
-    def __enter__(self):
-        import os
-        self.old = {k: os.environ.get(k) for k in self.kv}
-        os.environ.update({k: str(v) for k, v in self.kv.items()})
-
-    def __exit__(self, *a):
-        import os
-        for k, v in self.old.items():
-            if v is None:
-                os.environ.pop(k, None)
-            else:
-                os.environ[k] = v

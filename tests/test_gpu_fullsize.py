@@ -140,4 +140,20 @@ def test_production_render_full_size(ctx):
     g.render(sc, cam, ip, 4, spp_offset=4, accum64=True)
     assert np.array_equal(g.accum64.reshape(-1).view(np.uint64), one.reshape(-1).view(np.uint64))
     print(f"  mean radiance {img[:, :3].mean():.4f}", flush=True)
+    # the configuration's 64 spp: one pass of the default 2^27-path pool
+    # (64 x 1920 x 1080 = 132,710,400 paths) against eight one-shot renders of
+    # 8 spp, FP64 sums bit for bit, and the same rays traced
+    print("[full size] production 1920x1080, 64 spp in one pass vs 8 x 8 spp", flush=True)
+    f64 = pyre_amd.Film(w, h)
+    m64 = f64.render(sc, cam, ip, 64, accum64=True)
+    ext64, sh64 = sc.render_stats()
+    f8 = pyre_amd.Film(w, h)
+    ext8 = sh8 = 0
+    for k in range(8):
+        f8.render(sc, cam, ip, 8, spp_offset=8 * k, accum64=True)
+        e, s = sc.render_stats()
+        ext8, sh8 = ext8 + e, sh8 + s
+    assert np.array_equal(f64.accum64.reshape(-1).view(np.uint64), f8.accum64.reshape(-1).view(np.uint64))
+    assert (ext64, sh64) == (ext8, sh8) and ext64 >= 64 * w * h
+    assert np.all(np.isfinite(m64)) and m64[:, :3].min() >= 0.0
     sc.close()

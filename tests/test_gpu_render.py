@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 import pyre_amd
-from tests.helpers import env_override, rel_mse
+from tests.helpers import rel_mse
 
 pytestmark = pytest.mark.gpu
 
@@ -111,16 +111,18 @@ def test_ray_counts_match_reference_queries(ctx, oracle):
     assert st["n_node_extension"] > 1 and st["n_tri_extension"] > 0
 
 
-@pytest.mark.parametrize("switch", [dict(PYRE_SHADE_WAVES=1), dict(PYRE_STACK_RING=4, PYRE_TRACE_BLOCKS_PER_CU=4),
-                                    dict(PYRE_STACK_RING=16), dict(PYRE_PASS_PATHS=5000)])
+@pytest.mark.parametrize("switch", [dict(shade_waves=1), dict(stack_ring=4, trace_blocks_per_cu=4),
+                                    dict(stack_ring=16), dict(pool_paths=5000), dict(eager=1),
+                                    dict(memory_fraction=1e-6)])
 def test_render_tuning_switches_same_bits(ctx, switch):
     """Tuning switches (the shade kernel built without an occupancy target,
-    stack rings, traversal grid, pool size) change speed only: the film is
+    stack rings, traversal grid, pool size, eager launches, a pool bounded by
+    a tiny share of device memory) change speed only: the film is
     bit-identical to the default render."""
     pre, sc = _cornell(ctx)
     cam = pre.camera(1.0)
     ip = pyre_amd.PathIntegrator(8, 3)
     ref = pyre_amd.Film(32, 32).render(sc, cam, ip, 6).copy()
-    with env_override(**switch):
+    with ctx.tuning(**switch):
         got = pyre_amd.Film(32, 32).render(sc, cam, ip, 6)
     assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))

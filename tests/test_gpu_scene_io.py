@@ -48,6 +48,21 @@ def _sky_hdr(path, w=64, h=32):
     path.write_bytes(_rgbe_header(w, h) + px.tobytes())
 
 
+def _assert_env_scene_parity(g, o):
+    """Pinhole scenes lit by an environment map: first-hit AOVs are the
+    oracle's bits; radiance agrees to 1e-5 relative (the environment's
+    direction mapping uses the device's acos/atan2/sin/cos, an ulp from
+    libm's)."""
+    same_aov = np.all(g[..., 3:].view(np.uint32) == o[..., 3:].view(np.uint32), axis=-1).mean()
+    rel = (np.abs(g[..., :3] - o[..., :3]) / np.maximum(np.abs(o[..., :3]), 1e-6)).max(-1)
+    e = rel_mse(g[..., :3], o[..., :3])
+    print(f"  AOV bit-equal {same_aov:.4f} radiance within 1e-5 {np.mean(rel <= 1e-5):.4f} relMSE(gpu, cpu) {e:.2e}",
+          flush=True)
+    assert same_aov >= 0.99
+    assert np.mean(rel <= 1e-5) >= 0.99
+    assert e < 1e-8
+
+
 def test_pbrt_cornell_renders_like_the_preset(ctx, tmp_path):
     print("[scene io] pbrt cornell vs preset, 64x64 16 spp", flush=True)
     f = tmp_path / "cornell.pbrt"
@@ -94,11 +109,7 @@ def test_pbrt_scene_with_hdr_environment_against_oracle(ctx, oracle, tmp_path):
     g = pyre_amd.Film(w, h).render(sc, cam, pyre_amd.PathIntegrator(8, 3), 16).reshape(h, w, 10)
     sc.close()
     o = _oracle_scene(oracle, pre).render(_ocam(oracle, cam), w, h, 16).reshape(h, w, 10)
-    same_alb = np.all(np.abs(g[..., 3:6] - o[..., 3:6]) <= 1e-6, axis=-1).mean()
-    e = rel_mse(g[..., :3], o[..., :3])
-    print(f"  albedo equal {same_alb:.4f} relMSE(gpu, cpu) {e:.2e} mean {o[..., :3].mean():.3f}", flush=True)
-    assert same_alb > 0.95
-    assert e < 5e-3
+    _assert_env_scene_parity(g, o)
     assert o[..., :3].mean() > 0.05
 
 
@@ -144,8 +155,4 @@ def test_gltf_scene_against_oracle(ctx, oracle, tmp_path):
     g = pyre_amd.Film(w, h).render(sc, cam, pyre_amd.PathIntegrator(8, 3), 16).reshape(h, w, 10)
     sc.close()
     o = _oracle_scene(oracle, pre).render(_ocam(oracle, cam), w, h, 16).reshape(h, w, 10)
-    same_alb = np.all(np.abs(g[..., 3:6] - o[..., 3:6]) <= 1e-6, axis=-1).mean()
-    e = rel_mse(g[..., :3], o[..., :3])
-    print(f"  albedo equal {same_alb:.4f} relMSE(gpu, cpu) {e:.2e}", flush=True)
-    assert same_alb > 0.95
-    assert e < 5e-3
+    _assert_env_scene_parity(g, o)

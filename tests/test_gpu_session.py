@@ -5,8 +5,6 @@ pixel_seed(x, y, s) (sampler.rs:41-49), accumulating in sample order as
 main.rs:275-285 does, so any split of N spp into calls and passes -- eager or
 replayed from the captured hipGraph -- equals pyre_render(spp = N) bit for bit.
 """
-import os
-
 import numpy as np
 import pytest
 
@@ -23,28 +21,12 @@ def _cornell(ctx):
     return pre, sc
 
 
-class _env:
-    def __init__(self, **kv):
-        self.kv = kv
-
-    def __enter__(self):
-        self.old = {k: os.environ.get(k) for k in self.kv}
-        os.environ.update({k: str(v) for k, v in self.kv.items()})
-
-    def __exit__(self, *a):
-        for k, v in self.old.items():
-            if v is None:
-                os.environ.pop(k, None)
-            else:
-                os.environ[k] = v
-
-
 def _bits(a):
     return np.ascontiguousarray(a, np.float32).view(np.uint32)
 
 
 def test_session_graph_replays_match_eager_render(ctx):
-    """Two samples per pass (PYRE_PASS_PATHS = 2 * pixels), so every call
+    """Two samples per pass (a pool of 2 * pixels paths), so every call
     replays the captured pass graph; each replay is checked against an eager
     one-shot render of the same sample count, with reads (device work and a
     synchronisation) between the replays."""
@@ -52,7 +34,7 @@ def test_session_graph_replays_match_eager_render(ctx):
     w, h = 40, 32
     cam = pre.camera(w / h)
     ip = pyre_amd.PathIntegrator(8, 3)
-    with _env(PYRE_PASS_PATHS=2 * w * h):
+    with ctx.tuning(pool_paths=2 * w * h):
         sess = pyre_amd.Session(sc, cam, ip, w, h)
         total = 0
         for k in range(4):
@@ -61,7 +43,7 @@ def test_session_graph_replays_match_eager_render(ctx):
             total += 2
             m, r64, spp = sess.read(radiance64=True)
             assert spp == total
-            with _env(PYRE_GRAPH=0, PYRE_PASS_PATHS=1 << 21):
+            with ctx.tuning(eager=1, pool_paths=1 << 21):
                 film = pyre_amd.Film(w, h)
                 ref = film.render(sc, cam, ip, total, accum64=True)
             assert np.array_equal(_bits(m), _bits(ref)), f"replay {k}: graph pass differs from eager render"
@@ -75,21 +57,23 @@ def test_session_resume_and_reset(ctx):
     cam = pre.camera(1.0)
     ip = pyre_amd.PathIntegrator(8, 3)
     sess = pyre_amd.Session(sc, cam, ip, w, h)
-    # one graph, captured on the first pass, replays passes of 3 and 5 samples
+    # the pool grows from 3 to 5 samples per pass (the pass graph is captured
+    # again on the new buffers), then the graph replays a pass of 2 samples
     sess.render(3)
     sess.render(5)
+    sess.render(2)
     m, _, spp = sess.read()
-    assert spp == 8
-    with _env(PYRE_GRAPH=0):
-        ref = pyre_amd.Film(w, h).render(sc, cam, ip, 8)
+    assert spp == 10
+    with ctx.tuning(eager=1):
+        ref = pyre_amd.Film(w, h).render(sc, cam, ip, 10)
     assert np.array_equal(_bits(m), _bits(ref))
-    # reset clears the sums; the sample index keeps advancing (samples 8..11)
+    # reset clears the sums; the sample index keeps advancing (samples 10..13)
     sess.reset()
     sess.render(4)
     m2, _, spp2 = sess.read()
     assert spp2 == 4
-    with _env(PYRE_GRAPH=0):
-        ref2 = pyre_amd.Film(w, h).render(sc, cam, ip, 4, spp_offset=8)
+    with ctx.tuning(eager=1):
+        ref2 = pyre_amd.Film(w, h).render(sc, cam, ip, 4, spp_offset=10)
     assert np.array_equal(_bits(m2), _bits(ref2))
     ext, shadow = sc.render_stats()
     assert ext > 0 and shadow > 0
@@ -171,7 +155,7 @@ def test_viewer_running_mean_equals_welford_of_samples(ctx):
     w, h = 24, 20
     cam = pre.camera(w / h)
     ip = pyre_amd.PathIntegrator(8, 3)
-    with _env(PYRE_PASS_PATHS=3 * w * h):
+    with ctx.tuning(pool_paths=3 * w * h):
         sess = pyre_amd.Session(sc, cam, ip, w, h)
         sess.render(2)
         sess.render(5)

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import pyre_amd
-from tests.helpers import (canonical_gpu_tree, canonical_oracle_tree, edge_distance, env_override, near_edge_hits,
+from tests.helpers import (canonical_gpu_tree, canonical_oracle_tree, edge_distance, near_edge_hits,
                            tree_stats)
 
 pytestmark = pytest.mark.gpu
@@ -168,15 +168,14 @@ def test_degenerate_coincident_triangles(ctx, oracle):
     assert h["t"][0] == np.float32(1.0)
 
 
-@pytest.mark.parametrize("switch", [dict(PYRE_TRAVERSAL="binary"), dict(PYRE_TRACE_FETCH="static"),
-                                    dict(PYRE_STACK_RING=4), dict(PYRE_STACK_RING=16),
-                                    dict(PYRE_TRACE_BLOCKS_PER_CU=2)])
+@pytest.mark.parametrize("switch", [dict(traversal=1), dict(traversal=2), dict(stack_ring=4), dict(stack_ring=16),
+                                    dict(trace_blocks_per_cu=2)])
 def test_traversal_variants_bit_exact(ctx, oracle, switch):
     """Every selectable traversal form (binary tree with an LDS stack;
     grid-stride persistent; LDS stack rings of 4 and 16; a small persistent
     grid) gives the reference's hits bit for bit, closest and any."""
     print(f"[traversal variant] {switch}", flush=True)
-    with env_override(**switch):
+    with ctx.tuning(**switch):
         excluded, hits = _closest_parity(ctx, oracle, 30000, 20000, 32, 4, 41)
         assert hits > 1000 and excluded < 25
         tris = pyre_amd.random_triangles(43, 30000, 0.01)

@@ -461,3 +461,48 @@ def test_gltf_errors(tmp_path):
     doc["buffers"][0]["uri"] = "b.bin"
     with pytest.raises(PyreError, match="more than one parent"):
         pyre_amd.Preset.from_gltf(_write(tmp_path, "d.gltf", json.dumps(doc)))
+
+
+def _crafted(tmp_path, name, edit):
+    """A valid one-triangle glTF whose document `edit` then corrupts."""
+    pos = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float64)
+    doc, blob = _gltf_doc(pos, pos, np.array([0, 1, 2], np.uint16), [{"mesh": 0}], [0])
+    doc["meshes"][0]["primitives"] = doc["meshes"][0]["primitives"][:1]
+    doc["buffers"][0]["uri"] = "data:application/octet-stream;base64," + base64.b64encode(blob).decode()
+    edit(doc)
+    return _write(tmp_path, name, json.dumps(doc))
+
+
+@pytest.mark.parametrize("edit,msg", [
+    # byteOffset near 2^64: the old sum view_off + view_len wrapped below the buffer size
+    (lambda d: d["bufferViews"][0].update(byteOffset=2.0 ** 64 - 2048, byteLength=4096), "bufferView byteOffset"),
+    (lambda d: d["bufferViews"][0].update(byteOffset=-8), "bufferView byteOffset"),
+    (lambda d: d["bufferViews"][0].update(byteLength=1e300), "bufferView byteLength"),
+    (lambda d: d["bufferViews"][0].update(byteLength=3.5), "bufferView byteLength"),
+    (lambda d: d["bufferViews"][1].update(byteOffset=50, byteLength=40), "bufferView exceeds its buffer"),
+    # count * stride chosen to wrap: 2^62 elements of 12 bytes
+    (lambda d: d["accessors"][0].update(count=2.0 ** 62), "accessor count"),
+    (lambda d: d["accessors"][0].update(count=2 ** 30 + 1), "accessor exceeds its bufferView"),
+    (lambda d: d["accessors"][0].update(count=-1), "accessor count"),
+    (lambda d: d["accessors"][0].update(byteOffset=2 ** 63), "accessor byteOffset"),
+    (lambda d: d["accessors"][0].update(byteOffset=8), "accessor exceeds its bufferView"),
+    (lambda d: d["bufferViews"][0].update(byteStride=4), "byteStride below"),
+    (lambda d: d["bufferViews"][0].update(byteStride=256), "bufferView byteStride"),
+    (lambda d: d["accessors"][2].update(componentType=-5123), "accessor componentType"),
+    (lambda d: d["buffers"][0].update(byteLength=2.0 ** 70), "buffer byteLength"),
+])
+def test_gltf_rejects_crafted_accessors(tmp_path, edit, msg):
+    """Accessor and bufferView fields that are negative, fractional, huge or
+    point outside their buffer are refused before any read."""
+    with pytest.raises(PyreError, match=msg):
+        pyre_amd.Preset.from_gltf(_crafted(tmp_path, "x.gltf", edit))
+
+
+def test_gltf_crafted_baseline_loads(tmp_path):
+    p = pyre_amd.Preset.from_gltf(_crafted(tmp_path, "ok.gltf", lambda d: None))
+    m = p.arrays()["meshes"][0]
+    assert m["positions"].tolist() == [[0, 0, 0], [1, 0, 0], [0, 1, 0]]
+    assert m["indices"].reshape(-1).tolist() == [0, 1, 2]
+    # an explicit stride equal to the element size, and a strided view
+    p = pyre_amd.Preset.from_gltf(_crafted(tmp_path, "s.gltf", lambda d: d["bufferViews"][0].update(byteStride=12)))
+    assert p.arrays()["meshes"][0]["positions"].tolist() == [[0, 0, 0], [1, 0, 0], [0, 1, 0]]
