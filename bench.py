@@ -354,8 +354,13 @@ def _crossing(curve, target):
 NRC_TRAIN_PATHS = 1 << 18
 NRC_BATCH = 1 << 14
 NRC_STEPS = 4
-# Frames of at most this many spp (a 1080p frame of 16 spp is one pass).
-NRC_MAX_SPF = 16
+# Frames of at most this many spp, after NRC_WARMUP training-only frames
+# (measured best of frames of 1, 2, 4 and 16 spp and 0, 8 or 16 warm-up
+# frames for a cache that starts untrained).
+NRC_MAX_SPF = 4
+NRC_WARMUP = 8
+# Untimed training-only frames of the warm-cache comparison.
+NRC_PRETRAIN = 64
 
 
 def _at_time(curve, t):
@@ -444,7 +449,7 @@ def render_reference(ctx, scene, cam, ip, ref_spp):
     return film.accum64 / ref_spp
 
 
-def nrc_comparison(ctx, scene, cam, ip, ref, max_spp=128, warmup=0):
+def nrc_comparison(ctx, scene, cam, ip, ref, max_spp=128, warmup=0, pre_frames=0, spf_max=None):
     """Seconds to relMSE 0.01 against `ref`, plain path tracer vs radiance
     cache (paths ending in the cache after the first diffuse bounce). Every
     checkpoint N spp is its own render from a cold start, timed whole from
@@ -453,20 +458,30 @@ def nrc_comparison(ctx, scene, cam, ip, ref, max_spp=128, warmup=0):
     cache and frames of min(N, 16) spp, each tracing its samples and 2^18
     training paths in one pass, taking 4 Adam steps of 2^14 of the frame's
     training samples, then adding the updated cache's radiance to the paths
-    that ended in it. relMSE is taken outside the timed region."""
+    that ended in it; `warmup` training-only frames come first, timed.
+    pre_frames > 0: the cache is first trained, untimed, for that many
+    training-only frames (a warm cache, as a renderer that keeps its cache
+    across frames of a still view has one), then the timed render starts
+    from it. relMSE is taken outside the timed region."""
     import pyre_amd
     lo, hi = scene.bounds()
     target = 0.01
+    spf_max = spf_max or NRC_MAX_SPF
     checkpoints = [c for c in (4, 6, 8, 10, 12, 16, 24, 32, 48, 64, 96, 128) if c <= max_spp]
     stats = {}
 
     def render(kind, n):
         cache = pyre_amd.RadianceCache(ctx, lo, hi, seed=5) if kind == "nrc" else None
-        sess = pyre_amd.Session(scene, cam, ip, PROD_W, PROD_H, nrc_train_paths=NRC_TRAIN_PATHS if cache else 0)
-        spf = min(n, NRC_MAX_SPF)
-        sess.reserve(n if cache is None else spf)
         if cache is not None:
             cache.reserve(NRC_BATCH)
+            if pre_frames:
+                pre = pyre_amd.Session(scene, cam, ip, PROD_W, PROD_H, nrc_train_paths=NRC_TRAIN_PATHS)
+                pre.render_nrc(cache, frames=pre_frames, spp_per_frame=1, train_paths=NRC_TRAIN_PATHS,
+                               train_batch=NRC_BATCH, train_steps=NRC_STEPS, warmup_frames=pre_frames)
+                pre.close()
+        sess = pyre_amd.Session(scene, cam, ip, PROD_W, PROD_H, nrc_train_paths=NRC_TRAIN_PATHS if cache else 0)
+        spf = min(n, spf_max)
+        sess.reserve(n if cache is None else spf)
         t0 = time.perf_counter()
         if cache is None:
             sess.render(n)
@@ -488,7 +503,7 @@ def nrc_comparison(ctx, scene, cam, ip, ref, max_spp=128, warmup=0):
         render(kind, checkpoints[0])  # untimed: first-use costs (graph capture, code load)
         curve = []
         for n in checkpoints:
-            if kind == "nrc" and n > NRC_MAX_SPF and n % NRC_MAX_SPF:
+            if kind == "nrc" and n > spf_max and n % spf_max:
                 continue
             curve.append(render(kind, n))
             if curve[-1][1] <= target * 0.7:
@@ -505,10 +520,56 @@ def nrc_comparison(ctx, scene, cam, ip, ref, max_spp=128, warmup=0):
         "relmse_at_plain_seconds": {"plain": target, "nrc": round(eq, 5) if eq else None},
         "training": {"train_paths_per_frame": NRC_TRAIN_PATHS, "adam_steps_per_frame": NRC_STEPS,
                      "batch": NRC_BATCH, "samples_per_frame": NRC_STEPS * NRC_BATCH, "lr": 1e-2,
-                     "max_spp_per_frame": NRC_MAX_SPF, "warmup_frames": warmup},
+                     "max_spp_per_frame": spf_max, "timed_warmup_frames": warmup,
+                     "untimed_pretraining_frames": pre_frames},
         "plain_curve": [(round(t, 4), round(e, 5)) for t, e in plain],
         "nrc_curve": [(round(t, 4), round(e, 5)) for t, e in nrc], "path_stats_last": stats,
     }
+
+
+def cache_best_case(ctx, scene, cam, ip, ref, plain_seconds, n_spp=8, k_cont=256):
+    """The most any cache that takes over after the first diffuse bounce can
+    gain on this view. Its error: the mean of k_cont renders of the same
+    n_spp samples whose paths continue on k_cont independent streams from the
+    vertex a cache render queries the cache at (pyre_render_split) is what a
+    perfect cache gives those samples (up to the suffix variance / k_cont);
+    an unbiased render's relMSE falls as 1/spp, so a perfect cache needs
+    relMSE * n_spp / 0.01 spp. Its cost: a cache render that does not train
+    (paths end at the query vertex, the cache is evaluated there), per spp,
+    measured on a 16-spp frame. Bound = that many spp at that cost, against
+    the plain path tracer's seconds to relMSE 0.01."""
+    import pyre_amd
+    t0 = time.perf_counter()
+    film = pyre_amd.Film(PROD_W, PROD_H)
+    for k in range(k_cont):
+        film.render_split(scene, cam, ip, n_spp, continuation=k)
+    perfect = film.accum64 / (n_spp * k_cont)
+    e_perfect = rel_mse(perfect, ref)
+    plain = pyre_amd.Film(PROD_W, PROD_H)
+    plain.render(scene, cam, ip, n_spp, accum64=True)
+    e_plain = rel_mse(plain.accum64 / n_spp, ref)
+    need = e_perfect * n_spp / 0.01
+    lo, hi = scene.bounds()
+    cache = pyre_amd.RadianceCache(ctx, lo, hi, seed=5)
+    sess = pyre_amd.Session(scene, cam, ip, PROD_W, PROD_H, nrc_train_paths=1 << 10)
+    sess.reserve(16)
+    kw = dict(frames=1, spp_per_frame=16, train_paths=1 << 10, train_batch=NRC_BATCH, train_steps=0)
+    sess.render_nrc(cache, **kw)  # untimed: first use
+    t1 = time.perf_counter()
+    sess.render_nrc(cache, **kw)
+    per_spp = (time.perf_counter() - t1) / 16
+    sess.close()
+    cache.close()
+    bound = need * per_spp
+    out = {"continuations": k_cont, "spp": n_spp, "relmse_perfect_cache": round(e_perfect, 6),
+           "mean_radiance_perfect_cache": round(float(perfect.mean()), 5),
+           "mean_radiance_reference": round(float(ref.mean()), 5),
+           "relmse_plain": round(e_plain, 6), "spp_needed_perfect_cache": round(need, 2),
+           "seconds_per_spp_cache_render_without_training": round(per_spp, 6), "seconds": round(bound, 5),
+           "over_plain": round(bound / plain_seconds, 4) if plain_seconds else None,
+           "wall_s": round(time.perf_counter() - t0, 1)}
+    log(f"cache best case: {out}")
+    return out
 
 
 def nrc_block(ctx, args):
@@ -521,6 +582,11 @@ def nrc_block(ctx, args):
     ip = pyre_amd.PathIntegrator(12, 3)
     ref = render_reference(ctx, scene, cam, ip, args.ref_spp)
     out = nrc_comparison(ctx, scene, cam, ip, ref, max_spp=args.max_spp, warmup=args.nrc_warmup)
+    warm = nrc_comparison(ctx, scene, cam, ip, ref, max_spp=args.max_spp, warmup=0, pre_frames=NRC_PRETRAIN,
+                          spf_max=16)
+    out["warm_cache"] = {k: warm[k] for k in ("nrc_seconds", "nrc_over_plain", "relmse_at_plain_seconds",
+                                              "training", "nrc_curve")}
+    out["best_case"] = cache_best_case(ctx, scene, cam, ip, ref, out["plain_seconds"], n_spp=16, k_cont=256)
     lo, hi = scene.bounds()
     out["cache_mfma"] = cache_mfma(ctx, lo, hi, args)
     out["config"] = {"workload": "production scene 1920x1080 (64 level-7 icospheres, 2048x1024 sky/sun HDRI), "
@@ -692,7 +758,8 @@ def main():
     ap.add_argument("--ref-spp", type=int, default=16384,
                     help="spp of the FP64 references (0 = skip the relMSE figures of cornell / batch)")
     ap.add_argument("--max-spp", type=int, default=128, help="nrc: largest spp tried")
-    ap.add_argument("--nrc-warmup", type=int, default=0, help="nrc: training-only frames before the first sample")
+    ap.add_argument("--nrc-warmup", type=int, default=NRC_WARMUP,
+                    help="nrc: timed training-only frames before the first sample")
     ap.add_argument("--nrc-scene", default="production", choices=["production", "cornell"],
                     help="nrc / --make-reference: the view of the cache comparison")
     args = ap.parse_args()
@@ -819,6 +886,8 @@ def main():
             line["plain_seconds_to_relmse_0.01"] = nrc["plain_seconds"]
             line["nrc_seconds_to_relmse_0.01"] = nrc["nrc_seconds"]
             line["nrc_over_plain"] = nrc["nrc_over_plain"]
+            line["nrc_warm_cache_over_plain"] = nrc["warm_cache"]["nrc_over_plain"]
+            line["nrc_best_case_over_plain"] = nrc["best_case"]["over_plain"]
             line["nrc"] = nrc
         except Exception as e:  # the traversal line is still reported
             line["nrc_error"] = f"{type(e).__name__}: {e}"

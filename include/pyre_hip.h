@@ -278,6 +278,19 @@ int pyre_render(pyre_ctx* ctx, pyre_scene* scene, const pyre_camera_desc* camera
                 const pyre_integrator_params* integrator, uint32_t width, uint32_t height, uint32_t spp,
                 uint32_t spp_offset, pyre_pixel_sample* out, double* accum64);
 
+/* The radiance cache's best case (no counterpart in the reference): the
+ * samples of pyre_render, except that a path reaching the vertex where a
+ * cache render would end it and query the cache (the first surface reached
+ * after a bounce off a diffuse material: Lambertian, or Disney roughness >=
+ * diffuse_roughness) continues from there on a stream of its own, numbered
+ * `continuation`. Paths are identical up to that vertex for every
+ * continuation, so the mean of K renders with continuations 0..K-1 is the
+ * image a perfect cache would give these samples, up to the suffix variance
+ * / K. accum64 (3 doubles per pixel, required) is added to. */
+int pyre_render_split(pyre_ctx* ctx, pyre_scene* scene, const pyre_camera_desc* camera,
+                      const pyre_integrator_params* integrator, uint32_t width, uint32_t height, uint32_t spp,
+                      uint32_t spp_offset, float diffuse_roughness, uint32_t continuation, double* accum64);
+
 /* Rays traced by the last pyre_render (or session render call) on this
  * scene: closest-hit extension rays (camera rays included) and any-hit
  * shadow rays. */

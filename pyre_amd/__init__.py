@@ -557,6 +557,20 @@ class Film(_Sinks):
                                               _ptr(self.accum64) if accum64 else None), "pyre_render")
         return self.samples
 
+    def render_split(self, scene: Scene, camera: Camera, integrator: PathIntegrator, spp: int, continuation: int,
+                     spp_offset: int = 0, diffuse_roughness: float = 0.25):
+        """pyre_render_split: adds to accum64 the radiance sums of `spp`
+        samples whose paths continue on stream `continuation` from the vertex
+        a cache render would query the radiance cache at."""
+        cam = camera.to_c()
+        ip = L.IntegratorParams(integrator.max_depth, integrator.min_rr_depth)
+        if self.accum64 is None:
+            self.accum64 = np.zeros((self.height * self.width, 3), np.float64)
+        scene.ctx.check(scene.lib.pyre_render_split(scene.ctx.h, scene.h, C.byref(cam), C.byref(ip), self.width,
+                                                    self.height, spp, spp_offset, diffuse_roughness, continuation,
+                                                    _ptr(self.accum64)), "pyre_render_split")
+        return self.accum64
+
     def pack(self, what: int) -> np.ndarray:
         """The film packed for a sink on the host (pyre_film_pack)."""
         shape, dt = _pack_shape(what, self.width, self.height)

@@ -124,6 +124,8 @@ SIGNATURES = {
     "pyre_light_pdf_batch": (C.c_int, [_vp, _vp, _u32, _vp, _vp, C.c_size_t, _vp]),
     "pyre_render": (C.c_int, [_vp, _vp, C.POINTER(CameraDesc), C.POINTER(IntegratorParams), _u32, _u32, _u32,
                               _u32, _vp, _vp]),
+    "pyre_render_split": (C.c_int, [_vp, _vp, C.POINTER(CameraDesc), C.POINTER(IntegratorParams), _u32, _u32, _u32,
+                                    _u32, _f, _u32, _vp]),
     # pyre_gen.h
     "pyre_gen_random_triangles": (C.c_int, [C.c_uint64, _u32, _f, _vp]),
     "pyre_gen_random_rays": (C.c_int, [C.c_uint64, _u32, _u32, _vp, _vp]),

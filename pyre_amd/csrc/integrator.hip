@@ -48,8 +48,10 @@ using namespace pyre;
 namespace {
 
 constexpr int kThreads = 256;
-constexpr uint32_t FL_SPECULAR = 1u, FL_AOV = 2u, FL_DIFFUSE = 4u, FL_TRAIN = 8u;
-constexpr uint32_t MODE_PLAIN = 0u, MODE_CACHE = 2u;
+constexpr uint32_t FL_SPECULAR = 1u, FL_AOV = 2u, FL_DIFFUSE = 4u, FL_TRAIN = 8u, FL_SPLIT = 16u;
+// MODE_SPLIT: a path reaching the vertex where a CACHE pass would query the
+// cache continues there on a stream of its own (the cache's best case)
+constexpr uint32_t MODE_PLAIN = 0u, MODE_CACHE = 2u, MODE_SPLIT = 3u;
 constexpr float kRayEps = 1e-3f;  // integrator.rs:34
 constexpr uint32_t kVrec = 5;     // float4 per training vertex record
 constexpr uint32_t kTrainStreamBase = 0x80000000u;  // sample indices of training paths
@@ -98,6 +100,7 @@ struct PassDev {
   uint32_t mode, pix_mul, pix_add, vmax;
   float diffuse_rough;
   uint32_t train0, t_sbase;
+  uint32_t split;  // MODE_SPLIT: the continuation's number
 };
 
 // What every stage reads from device memory: the pass being run is
@@ -288,6 +291,13 @@ __global__ __launch_bounds__(kThreads) void k_generate(Paths P, Sched sc) {
     P.pjob[p] = ps.job;
     if (ps.train) P.vcount[ps.local] = 0;
   }
+}
+
+// MODE_SPLIT: the stream a path continues on from its split vertex,
+// continuation number pp.split of the path's sample.
+__device__ __forceinline__ Rng split_stream(const Sched& sc, const PassDev& pp, uint32_t p) {
+  const PathSample ps = path_sample(sc, pp, p);
+  return rng_seed(pixel_seed(ps.x, ps.y, ps.sample) ^ (0xD1B54A32D192ED03ull * ((uint64_t)pp.split + 1ull)));
 }
 
 // The stream of path p as the first bounce finds it: seeded by pixel_seed
@@ -552,7 +562,14 @@ __device__ void shade_path(const Paths& P, const Sched& sc, const JobDev& J, con
     P.lum[p] = f4(l, __builtin_bit_cast(float, flags));
     return;
   }
-  if (pp.mode != MODE_PLAIN) {
+  bool reseed = false;
+  if (pp.mode == MODE_SPLIT) {
+    // the vertex where a cache render would end the path and query the cache
+    if (depth >= 1 && (flags & FL_DIFFUSE) && !(flags & FL_SPLIT)) {
+      reseed = true;
+      flags |= FL_SPLIT;
+    }
+  } else if (pp.mode != MODE_PLAIN) {
     const float rough = mat.kind == PYRE_MAT_LAMBERTIAN ? 1.0f : mat.roughness;
     const float qf[13] = {pos.x, pos.y, pos.z, -rd.x, -rd.y, -rd.z, ns.x, ns.y, ns.z, rough,
                           mat.base.x, mat.base.y, mat.base.z};
@@ -576,7 +593,7 @@ __device__ void shade_path(const Paths& P, const Sched& sc, const JobDev& J, con
       }
     }
   }
-  Rng rng = depth == 0 ? camera_stream(sc, pp, p) : load_rng(P, p);
+  Rng rng = depth == 0 ? camera_stream(sc, pp, p) : (reseed ? split_stream(sc, pp, p) : load_rng(P, p));
   uint32_t ns_count = 0;
   const size_t slot0 = (size_t)p * P.slots;
   // NEE toward each quad light (integrator.rs:142-167)
@@ -1344,6 +1361,7 @@ struct pyre_session {
   float* welford = nullptr;
   unsigned long long* stats = nullptr;  // ray / cache counters over the session's lifetime
   bool bound = false;                   // W's job table and counters are this session's
+  uint32_t split = 0;                   // MODE_SPLIT passes: continuation number
   uint32_t spp_done = 0, next_sample = 0, train_frames = 0;
   uint64_t stats_seen[2] = {0, 0};  // ray counters already reported
   bool use_graph = true;
@@ -1383,6 +1401,7 @@ void set_pass(pyre_session* S, uint32_t s_base, uint32_t n_samples, uint32_t mod
   pp.t_sbase = tr.s_base;
   pp.vmax = S->W.vmax;
   pp.diffuse_rough = diffuse_rough;
+  pp.split = S->split;
   SliceDev sl{0u, s_base, n_samples, 0u, 0u, S->spp_done};
   hipLaunchKernelGGL(k_set_pass, dim3(1), dim3(1), 0, S->ctx->stream, S->W.passes, S->W.slices, pp, sl);
 }
@@ -1536,12 +1555,12 @@ void session_reserve(pyre_session* S, uint32_t want) {
   PYRE_HIP_CHECK(hipGetLastError());
 }
 
-void session_render(pyre_session* S, uint32_t spp) {
+void session_render(pyre_session* S, uint32_t spp, uint32_t mode = MODE_PLAIN, float diffuse_rough = 0.25f) {
   if (!spp) return;
   session_reserve(S, spp);
   for (uint32_t s0 = 0; s0 < spp; s0 += S->per_pass) {
     uint32_t ns = std::min(S->per_pass, spp - s0);
-    session_pass(S, S->next_sample, ns, MODE_PLAIN, nullptr, 0.25f, true);
+    session_pass(S, S->next_sample, ns, mode, nullptr, diffuse_rough, true);
     S->next_sample += ns;
     S->spp_done += ns;
   }
@@ -1778,9 +1797,10 @@ int pyre_render_stats(pyre_ctx* ctx, pyre_scene* s, uint64_t* extension_rays, ui
   return PYRE_OK;
 }
 
-int pyre_render(pyre_ctx* ctx, pyre_scene* s, const pyre_camera_desc* camera, const pyre_integrator_params* integrator,
-                uint32_t width, uint32_t height, uint32_t spp, uint32_t spp_offset, pyre_pixel_sample* out,
-                double* accum64) {
+static int render_one_shot(pyre_ctx* ctx, pyre_scene* s, const pyre_camera_desc* camera,
+                           const pyre_integrator_params* integrator, uint32_t width, uint32_t height, uint32_t spp,
+                           uint32_t spp_offset, pyre_pixel_sample* out, double* accum64, uint32_t mode,
+                           float diffuse_rough, uint32_t split) {
   if (!ctx || !s || !camera || !integrator) return PYRE_EINVAL;
   return sguard(ctx, [&] {
     pyre_session S;
@@ -1794,7 +1814,8 @@ int pyre_render(pyre_ctx* ctx, pyre_scene* s, const pyre_camera_desc* camera, co
     const size_t n3 = 3 * (size_t)S.npix;
     if (accum64) PYRE_HIP_CHECK(hipMemcpyAsync(S.acc64, accum64, n3 * sizeof(double), hipMemcpyHostToDevice, st));
     S.next_sample = spp_offset;
-    session_render(&S, spp);
+    S.split = split;
+    session_render(&S, spp, mode, diffuse_rough);
     if (out) read_mean(ctx, S.acc, S.npix, out, 1.0f / (float)spp);  // main.rs:255
     if (accum64) PYRE_HIP_CHECK(hipMemcpyAsync(accum64, S.acc64, n3 * sizeof(double), hipMemcpyDeviceToHost, st));
     finish_stats(&S);
@@ -1803,6 +1824,21 @@ int pyre_render(pyre_ctx* ctx, pyre_scene* s, const pyre_camera_desc* camera, co
     *keep = std::move(S.W);
     ctx->render_cache = std::static_pointer_cast<void>(keep);
   });
+}
+
+int pyre_render(pyre_ctx* ctx, pyre_scene* s, const pyre_camera_desc* camera, const pyre_integrator_params* integrator,
+                uint32_t width, uint32_t height, uint32_t spp, uint32_t spp_offset, pyre_pixel_sample* out,
+                double* accum64) {
+  return render_one_shot(ctx, s, camera, integrator, width, height, spp, spp_offset, out, accum64, MODE_PLAIN, 0.25f,
+                         0u);
+}
+
+int pyre_render_split(pyre_ctx* ctx, pyre_scene* s, const pyre_camera_desc* camera,
+                      const pyre_integrator_params* integrator, uint32_t width, uint32_t height, uint32_t spp,
+                      uint32_t spp_offset, float diffuse_roughness, uint32_t continuation, double* accum64) {
+  if (!accum64) return PYRE_EINVAL;
+  return render_one_shot(ctx, s, camera, integrator, width, height, spp, spp_offset, nullptr, accum64, MODE_SPLIT,
+                         diffuse_roughness, continuation);
 }
 
 int pyre_render_batch(pyre_ctx* ctx, const pyre_job* jobs, size_t n, const pyre_batch_params* params,
