@@ -246,6 +246,19 @@ int pyre_intersect_any(pyre_ctx* ctx, pyre_scene* scene, const pyre_ray* rays, s
  * math.rs:16-24); pyre_intersect_* honour pyre_ray.time. */
 int pyre_trace_closest_dev(pyre_ctx* ctx, pyre_scene* scene, const void* rays, size_t n, void* hits, void* stream);
 int pyre_trace_any_dev(pyre_ctx* ctx, pyre_scene* scene, const void* rays, size_t n, void* occluded, void* stream);
+/* No counterpart in the reference: pyre_trace_closest_dev for callers that
+ * cannot allocate device memory (code under hipGraph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). The same kernel and results, bit
+ * for bit; its traversal scratch (work counter, stack spill columns) is a
+ * fixed 64 MiB array in the library's device code, one per GPU, held until
+ * the process ends, so the call allocates and frees nothing. The persistent
+ * grid has as many 256-thread blocks (at most the tuning's blocks per CU) as
+ * the array holds spill columns for on the scene's tree; a tree too deep for
+ * one block is refused with PYRE_EINVAL. Every call on one GPU shares that
+ * GPU's array: calls that overlap in time on one GPU (other streams,
+ * contexts or threads) race, and the caller must order them. */
+int pyre_trace_closest_dev_noalloc(pyre_ctx* ctx, pyre_scene* scene, const void* rays, size_t n, void* hits,
+                                   void* stream);
 
 /* -------------------------------------------------- shade-stage probes */
 /* Bsdf::eval + Bsdf::pdf (material.rs:23-36) on local-frame directions:

@@ -402,6 +402,12 @@ class Scene:
                                                        C.c_void_p(hits_ptr), C.c_void_p(stream or None)),
                        "pyre_trace_closest_dev")
 
+    def trace_closest_dev_noalloc(self, rays_ptr: int, n: int, hits_ptr: int, stream: int = 0):
+        """pyre_trace_closest_dev on the library's fixed scratch (allocates nothing)."""
+        self.ctx.check(self.lib.pyre_trace_closest_dev_noalloc(self.ctx.h, self.h, C.c_void_p(rays_ptr), n,
+                                                               C.c_void_p(hits_ptr), C.c_void_p(stream or None)),
+                       "pyre_trace_closest_dev_noalloc")
+
     def trace_any_dev(self, rays_ptr: int, n: int, occ_ptr: int, stream: int = 0):
         self.ctx.check(self.lib.pyre_trace_any_dev(self.ctx.h, self.h, C.c_void_p(rays_ptr), n,
                                                    C.c_void_p(occ_ptr), C.c_void_p(stream or None)),

@@ -115,6 +115,7 @@ SIGNATURES = {
     "pyre_intersect_any": (C.c_int, [_vp, _vp, _vp, C.c_size_t, _vp]),
     "pyre_trace_closest_dev": (C.c_int, [_vp, _vp, _vp, C.c_size_t, _vp, _vp]),
     "pyre_trace_any_dev": (C.c_int, [_vp, _vp, _vp, C.c_size_t, _vp, _vp]),
+    "pyre_trace_closest_dev_noalloc": (C.c_int, [_vp, _vp, _vp, C.c_size_t, _vp, _vp]),
     "pyre_bsdf_eval_batch": (C.c_int, [_vp, C.POINTER(Material), _vp, _vp, C.c_size_t, _vp, _vp]),
     "pyre_bsdf_sample_batch": (C.c_int, [_vp, C.POINTER(Material), _vp, _vp, C.c_size_t, _vp, _vp, _vp, _vp]),
     "pyre_env_sample_batch": (C.c_int, [_vp, _vp, _vp, C.c_size_t, _vp, _vp, _vp]),

@@ -453,6 +453,7 @@ int pyre_ctx_create(int device, pyre_ctx** out) {
     c->name = prop.gcnArchName;
     c->cus = prop.multiProcessorCount;
     PYRE_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    noalloc_scratch_init(c);
     const HostPhiTable& T = host_phi_table();
     c->d_phi_rank = dev_alloc<uint4>(T.rank.size());
     c->d_phi_val = dev_alloc<float2>(T.val.size());
@@ -749,6 +750,15 @@ int pyre_trace_closest_dev(pyre_ctx* ctx, pyre_scene* s, const void* rays, size_
   return guarded(ctx, [&] {
     hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
     trace_closest(s, (const float4*)rays, n, (pyre_hit8*)hits, nullptr, st, false);
+  });
+}
+
+int pyre_trace_closest_dev_noalloc(pyre_ctx* ctx, pyre_scene* s, const void* rays, size_t n, void* hits,
+                                   void* stream) {
+  if (!ctx || !s || (n && (!rays || !hits))) return PYRE_EINVAL;
+  return guarded(ctx, [&] {
+    hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+    trace_closest_noalloc(s, (const float4*)rays, n, (pyre_hit8*)hits, st);
   });
 }
 

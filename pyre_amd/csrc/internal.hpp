@@ -28,6 +28,9 @@ struct pyre_ctx {
   int2* trace_gstack = nullptr;
   uint32_t trace_depth = 0;
   hipEvent_t trace_ev = nullptr;
+  // the device address of the no-allocation trace scratch (a fixed array in
+  // trace.hip) on the context's device
+  void* noalloc_scratch = nullptr;
   // path buffers kept from the last one-shot render (owned; freed with the
   // context, on its device)
   std::shared_ptr<void> render_cache;
@@ -200,6 +203,9 @@ void trace_any(pyre_scene* s, const float4* rays, size_t n, uint32_t* occluded, 
 // (synchronises the device when it has to grow; called by pyre_bvh_build).
 void trace_scratch_reserve(pyre_ctx* ctx, uint32_t depth);
 void trace_scratch_release(pyre_ctx* ctx);
+// pyre_trace_closest_dev_noalloc: the same kernel on the fixed scratch array.
+void noalloc_scratch_init(pyre_ctx* ctx);
+void trace_closest_noalloc(pyre_scene* s, const float4* rays, size_t n, pyre_hit8* hits, hipStream_t st);
 
 // Resolved tuning values.
 inline uint32_t tune_blocks_per_cu(const pyre_ctx* c) {

@@ -7,6 +7,9 @@
 // form of the 4-wide kernel remain selectable through the context's tuning
 // (pyre_tuning.traversal) for comparison.
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <string>
 
 #include "traverse.hpp"
 
@@ -232,6 +235,61 @@ void trace_scratch_release(pyre_ctx* ctx) {
   ctx->trace_depth = 0;
   if (ctx->trace_ev) (void)hipEventDestroy(ctx->trace_ev);
   ctx->trace_ev = nullptr;
+}
+
+// ------------------------------------------------ no-allocation variant
+// Traversal scratch of pyre_trace_closest_dev_noalloc: a fixed array in the
+// device code, one copy per GPU, held until the process ends. Its first 256
+// bytes hold the persistent kernel's work counter (cleared by a fill kernel
+// before every launch), the rest the stack spill columns of as many
+// 256-thread blocks as fit. On the host `scratch` names no device address:
+// only the pointer hipGetSymbolAddress returns for the current device is
+// used (noalloc_scratch_init, at context creation).
+constexpr size_t kNoallocScratchBytes = 64ull << 20;
+constexpr size_t kNoallocCounterBytes = 256;
+static __device__ __align__(256) int2 scratch[kNoallocScratchBytes / sizeof(int2)];
+
+void noalloc_scratch_init(pyre_ctx* ctx) {
+  // one lookup per device (the context's device is current here)
+  static std::mutex mu;
+  static std::map<int, void*> per_device;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = per_device.find(ctx->device);
+  if (it == per_device.end()) {
+    void* ptr = nullptr;
+    PYRE_HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    it = per_device.emplace(ctx->device, ptr).first;
+  }
+  ctx->noalloc_scratch = it->second;
+}
+
+// Persistent blocks the fixed scratch has spill columns for (at most the
+// tuning's blocks per CU); 0 when not even one block fits.
+unsigned noalloc_blocks(const pyre_ctx* ctx, uint32_t depth) {
+  const size_t per_block = (size_t)kTraceThreads * spill_entries(depth) * sizeof(int2);
+  const size_t fit = (kNoallocScratchBytes - kNoallocCounterBytes) / per_block;
+  return (unsigned)std::min<size_t>(fit, trace_grid(ctx));
+}
+
+void trace_closest_noalloc(pyre_scene* s, const float4* rays, size_t n, pyre_hit8* hits, hipStream_t st) {
+  require(s->built, "trace before pyre_bvh_build");
+  require(n <= 0xFFFFFFFFull, "too many rays in one call");
+  pyre_ctx* ctx = s->ctx;
+  require(ctx->noalloc_scratch != nullptr, "no-allocation scratch not initialised");
+  const unsigned grid = noalloc_blocks(ctx, s->depth);
+  require(grid >= 1, "tree depth " + std::to_string(s->depth) + " needs more stack than the " +
+                         std::to_string(kNoallocScratchBytes >> 20) + " MiB no-allocation scratch holds");
+  if (n == 0) return;
+  uint32_t* fetch = static_cast<uint32_t*>(ctx->noalloc_scratch);
+  int2* gst = reinterpret_cast<int2*>(static_cast<char*>(ctx->noalloc_scratch) + kNoallocCounterBytes);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, fetch, 1u);
+  if (s->has_motion)
+    hipLaunchKernelGGL((k_trace4<false, 8, true>), dim3(grid), dim3(kTraceThreads), 0, st, s->d_nodes4, s->d_tris,
+                       rays, (uint32_t)n, hits, nullptr, nullptr, gst, fetch, nullptr, s->d_motion);
+  else
+    hipLaunchKernelGGL((k_trace4<false, 8>), dim3(grid), dim3(kTraceThreads), 0, st, s->d_nodes4, s->d_tris, rays,
+                       (uint32_t)n, hits, nullptr, nullptr, gst, fetch, nullptr, nullptr);
+  PYRE_HIP_CHECK(hipGetLastError());
 }
 
 void trace_closest(pyre_scene* s, const float4* rays, size_t n, pyre_hit8* hits, float2* uv, hipStream_t st,
