@@ -632,7 +632,8 @@ def batch_jobs(ctx, n_jobs=96, seed=4):
     1024^2 pixels), spp from {4, 16, 64, 256}, scenes of log-uniform
     10k..2M triangles built from instanced icospheres (level and count
     chosen to hit the triangle count), half Lambertian / half Disney
-    materials, procedural 512x256 sky + sun HDRI; pinhole, max depth 8."""
+    materials, the production scene's procedural 2048x1024 sky + sun HDRI;
+    pinhole, max depth 8."""
     import pyre_amd
     rng = np.random.default_rng(seed)
     jobs, scenes = [], []
@@ -645,7 +646,7 @@ def batch_jobs(ctx, n_jobs=96, seed=4):
         while tris / (20 * 4 ** level) > 64:
             level += 1
         n_inst = int(min(64, max(1, round(tris / (20 * 4 ** level)))))
-        pre = pyre_amd.Preset.instances(1000 + j, n_inst, level, 512, 256, 0.5)
+        pre = pyre_amd.Preset.instances(1000 + j, n_inst, level, 2048, 1024, 0.5)
         sc = pyre_amd.Scene.from_preset(ctx, pre)
         sc.build_bvh()
         scenes.append((pre, sc, n_inst * 20 * 4 ** level + 2))
@@ -683,7 +684,7 @@ def bench_batch(args):
         "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": round(per * 1e3, 1), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (job list seed 4)",
         "config": {"workload": "96 jobs: 128^2..1024^2 px (log-uniform), spp {4,16,64,256}, 10k..2M-triangle "
-                               "icosphere-instance scenes (log-uniform), Lambertian/Disney 50/50, 512x256 sky+sun "
+                               "icosphere-instance scenes (log-uniform), Lambertian/Disney 50/50, 2048x1024 sky+sun "
                                "HDRI, pinhole, depth 8", "jobs": len(jobs), "samples": total,
                    "triangles": int(sum(t for _, _, t in scenes))},
         "tuning": tune, "passes": stats["passes"], "gpu_ms": round(stats["gpu_ms"], 1),

@@ -1058,6 +1058,10 @@ struct Work {
   size_t gstack_cap = 0;
   std::vector<void*> bufs;
   size_t bytes = 0;  // device memory held
+  // the pool's own ray / cache counters; a session points P.stats at its
+  // own while it uses the pool, and unbind() points it back before the pool
+  // is handed to another render (the session's buffer dies with it)
+  unsigned long long* own_stats = nullptr;
   float* tfeat = nullptr;
   float* ttarget = nullptr;
   template <class T>
@@ -1093,8 +1097,10 @@ struct Work {
     ttarget = o.ttarget;
     bufs = std::move(o.bufs);
     bytes = o.bytes;
+    own_stats = o.own_stats;
     o.bufs.clear();
     o.bytes = 0;
+    o.own_stats = nullptr;
     o.cap = 0;
     o.gstack = nullptr;
     o.gstack_cap = 0;
@@ -1104,6 +1110,8 @@ struct Work {
     for (void* p : bufs) (void)hipFree(p);
     bufs.clear();
     bytes = 0;
+    own_stats = nullptr;
+    P.stats = nullptr;
     cap = 0;
     gstack = nullptr;
     gstack_cap = 0;
@@ -1142,7 +1150,7 @@ struct Work {
     P.q_shadow = get<uint32_t>(cap);
     P.pjob = get<uint32_t>(cap);
     P.counters = get<uint32_t>(kCounters);
-    P.stats = get<unsigned long long>(4);
+    P.stats = own_stats = get<unsigned long long>(4);
     P.slots = slots;
     if (vmax) {
       P.vrec = get<float4>((size_t)tpaths * vmax * kVrec);
@@ -1175,6 +1183,7 @@ struct Work {
     gstack_cap = need;
   }
   Sched sched() const { return Sched{jobs, slices, passes, pass_idx}; }
+  void unbind() { P.stats = own_stats; }
   ~Work() { release(); }
 };
 
@@ -1689,6 +1698,7 @@ float run_batch(Batch& B, const Tuning& t, bool accumulate, uint32_t max_passes,
     W = std::move(*static_cast<Work*>(ctx->render_cache.get()));
     ctx->render_cache.reset();
   }
+  W.unbind();
   const uint32_t pool = batch_pool(ctx, t, B, W.bytes);
   if (pool_used) *pool_used = pool;
   Schedule sc = make_schedule(B.jobs, B.n, pool);
@@ -1819,7 +1829,9 @@ static int render_one_shot(pyre_ctx* ctx, pyre_scene* s, const pyre_camera_desc*
     if (out) read_mean(ctx, S.acc, S.npix, out, 1.0f / (float)spp);  // main.rs:255
     if (accum64) PYRE_HIP_CHECK(hipMemcpyAsync(accum64, S.acc64, n3 * sizeof(double), hipMemcpyDeviceToHost, st));
     finish_stats(&S);
-    // keep the path buffers for the next one-shot render on this context
+    // keep the path buffers for the next one-shot render on this context,
+    // with their own counters again (this session's die with it)
+    S.W.unbind();
     auto keep = std::make_shared<Work>();
     *keep = std::move(S.W);
     ctx->render_cache = std::static_pointer_cast<void>(keep);

@@ -73,3 +73,48 @@ def rel_mse(img, ref, eps=1e-2):
     img = np.asarray(img, np.float64)
     ref = np.asarray(ref, np.float64)
     return float(np.mean((img - ref) ** 2 / (ref ** 2 + eps)))
+
+
+def oracle_camera(oracle, cam):
+    """The oracle's camera for a pyre_amd.Camera."""
+    return oracle.camera(cam.origin, cam.target, cam.up, cam.vfov_deg, cam.aspect, cam.aperture_radius,
+                         cam.focus_distance if cam.focus_distance > 0 else None)
+
+
+def oracle_scene(oracle, pre):
+    """The oracle's Scene built from a preset's arrays: its meshes (with
+    their motion), materials, quad lights and environment map."""
+    arr = pre.arrays()
+    s = oracle.Scene()
+    for m in arr["materials"]:
+        s.add_material(oracle.material(m.kind, m.base_color, m.metallic, m.roughness, m.specular))
+    for m in arr["meshes"]:
+        prim = s.add_mesh(m["positions"], m["indices"], m["normals"], m["material_id"])
+        if m.get("motion") is not None:
+            s.set_motion(prim, m["motion"][0], m["motion"][1])
+    for l in arr["lights"]:
+        s.add_light(l)
+    if arr["env"] is not None:
+        e = arr["env"]
+        s.set_env(oracle.Hdri(e["width"], e["height"], e["rgb"], e["intensity"]))
+    return s
+
+
+def assert_render_parity(g, o, exact):
+    """GPU vs oracle PixelSample images (h, w, 10) of the same samples:
+    exact -> every field of >= 99 % of the pixels bit-equal (scenes without
+    an environment map or thin lens); otherwise first-hit AOVs bit-equal and
+    radiance within 1e-5 relative on >= 99 % of the pixels. Returns the
+    fractions."""
+    same = np.all(g.view(np.uint32) == o.view(np.uint32), axis=-1).mean()
+    aov = np.all(g[..., 3:].view(np.uint32) == o[..., 3:].view(np.uint32), axis=-1).mean()
+    rel = (np.abs(g[..., :3] - o[..., :3]) / np.maximum(np.abs(o[..., :3]), 1e-6)).max(-1)
+    close = float(np.mean(rel <= 1e-5))
+    if exact:
+        assert same >= 0.99, f"bit-equal pixels {same:.4f}"
+    else:
+        assert aov >= 0.95, f"AOV bit-equal pixels {aov:.4f}"
+        assert close >= 0.99, f"radiance within 1e-5 on {close:.4f} of the pixels"
+    assert rel_mse(g[..., :3], o[..., :3]) < 1e-8
+    return dict(bit_equal=float(same), aov_bit_equal=float(aov), radiance_1e5=close)
+

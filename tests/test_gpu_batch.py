@@ -88,3 +88,60 @@ def test_batch_rejects_bad_jobs(ctx):
         pyre_amd.render_batch(ctx, zero)
     outs, _, st = pyre_amd.render_batch(ctx, [])
     assert outs == [] and st["passes"] == 0
+
+
+def test_batch_of_the_96_job_workload(ctx):
+    """The batched-jobs workload itself (bench.py's job list, seed 4: 96
+    jobs of 128^2..1024^2 pixels, 4..256 spp, 10k..2M-triangle scenes with
+    2048x1024 sky/sun maps) in one pyre_render_batch: every output finite
+    and non-negative, and the four smallest jobs' FP64 sums bit-equal to
+    their own pyre_render."""
+    import bench
+    print("[batch] the 96-job workload", flush=True)
+    jobs, scenes = bench.batch_jobs(ctx)
+    assert len(jobs) == 96
+    outs, accs, st = pyre_amd.render_batch(ctx, jobs, accum64=True)
+    print(f"  {st['passes']} passes, gpu {st['gpu_ms']:.1f} ms, pool {st['pool_paths']}", flush=True)
+    for k, (o, a) in enumerate(zip(outs, accs)):
+        assert np.all(np.isfinite(o)) and np.all(np.isfinite(a)), f"job {k}"
+        assert o[:, :3].min() >= 0.0 and a.min() >= 0.0, f"job {k}"
+    small = sorted(range(len(jobs)), key=lambda k: jobs[k].width * jobs[k].height * jobs[k].spp)[:4]
+    for k in small:
+        j = jobs[k]
+        f = pyre_amd.Film(j.width, j.height)
+        f.render(j.scene, j.camera, j.integrator, j.spp, spp_offset=j.spp_offset, accum64=True)
+        assert np.array_equal(accs[k].view(np.uint64), f.accum64.view(np.uint64)), f"job {k}"
+    for _, sc, _ in scenes:
+        sc.close()
+
+
+def test_mixed_batch_against_oracle(ctx, oracle):
+    """A batch of different scenes -- the Cornell box, instanced icospheres
+    under a sky/sun map with a thin lens, and the studio preset with its
+    moving sphere and gradient sky -- every job against the oracle's render
+    of the same samples (correlated samples; Cornell to the bit)."""
+    from tests.helpers import assert_render_parity, oracle_camera, oracle_scene
+    cb = pyre_amd.Preset.cornell()
+    pi = pyre_amd.Preset.instances(43, 4, 2, 128, 64, 0.5)
+    st_ = pyre_amd.Preset.studio(motion_blur=True, env_intensity=1.0)
+    scs = []
+    for pre in (cb, pi, st_):
+        sc = pyre_amd.Scene.from_preset(ctx, pre)
+        sc.build_bvh()
+        scs.append(sc)
+    J = pyre_amd.Job
+    jobs = [J(scs[0], cb.camera(1.0), pyre_amd.PathIntegrator(8, 3), 24, 24, 6, spp_offset=2),
+            J(scs[1], pi.camera(16 / 9, aperture=0.05), pyre_amd.PathIntegrator(12, 3), 32, 18, 5),
+            J(scs[2], st_.camera(1.5), pyre_amd.PathIntegrator(8, 3), 24, 16, 4, spp_offset=7)]
+    outs, _, st = pyre_amd.render_batch(ctx, jobs, pool_paths=1500)
+    assert st["passes"] > 1
+    for k, (pre, j, exact) in enumerate(zip((cb, pi, st_), jobs, (True, False, False))):
+        o = oracle_scene(oracle, pre)
+        if k == 2:  # the studio preset's sky is HdriEnvironmentLight::gradient (light.rs:252-272)
+            o.set_env(oracle.Hdri(gradient=((0.20, 0.35, 0.65), (0.95, 0.85, 0.70)), intensity=1.0))
+        ref = o.render(oracle_camera(oracle, j.camera), j.width, j.height, j.spp, spp_offset=j.spp_offset,
+                       max_depth=j.integrator.max_depth, min_rr=j.integrator.min_rr_depth)
+        fr = assert_render_parity(outs[k].reshape(j.height, j.width, 10), ref.reshape(j.height, j.width, 10), exact)
+        print(f"  job {k}: {fr}", flush=True)
+    for sc in scs:
+        sc.close()

@@ -157,3 +157,45 @@ def test_production_render_full_size(ctx):
     assert (ext64, sh64) == (ext8, sh8) and ext64 >= 64 * w * h
     assert np.all(np.isfinite(m64)) and m64[:, :3].min() >= 0.0
     sc.close()
+
+
+def test_radiance_cache_on_the_production_view(ctx):
+    """The radiance-cache configuration at full size: the production scene
+    at 1920x1080, depth 12, thin lens 0.05, paths ending in the cache after
+    the first diffuse bounce, the cache trained online on 2^16 samples per
+    frame (4 Adam steps of 2^14 from 2^18 training paths). Properties: the
+    image is finite and non-negative, paths do end in the cache and training
+    samples are produced, and a trained cache renders closer to a 256-spp
+    FP64 reference than an untrained one at equal spp."""
+    from tests.helpers import rel_mse
+    print("[full size] radiance cache, production 1920x1080", flush=True)
+    pre = pyre_amd.Preset.production(seed=3, n_instances=64, level=7, env_w=2048, env_h=1024)
+    sc = pyre_amd.Scene.from_preset(ctx, pre)
+    sc.build_bvh()
+    w, h = 1920, 1080
+    cam = pre.camera(w / h, aperture=0.05)
+    ip = pyre_amd.PathIntegrator(12, 3)
+    ref = pyre_amd.Film(w, h)
+    ref.render(sc, cam, ip, 256, spp_offset=1 << 20, accum64=True)
+    ref = ref.accum64 / 256
+    lo, hi = sc.bounds()
+    errs = {}
+    for kind, steps in (("trained", 4), ("untrained", 0)):
+        cache = pyre_amd.RadianceCache(ctx, lo, hi, seed=5)
+        sess = pyre_amd.Session(sc, cam, ip, w, h, nrc_train_paths=1 << 18)
+        kw = dict(train_paths=1 << 18, train_batch=1 << 14, train_steps=steps)
+        sess.render_nrc(cache, frames=32, spp_per_frame=0, warmup_frames=32, **kw)  # training only
+        losses = sess.render_nrc(cache, frames=4, spp_per_frame=4, **kw)
+        mean, r64, spp = sess.read(mean=True, radiance64=True)
+        st = sess.stats()
+        assert spp == 16
+        assert np.all(np.isfinite(mean)) and mean[:, :3].min() >= 0.0
+        assert st["cache_queries"] > 0 and st["training_samples"] > 0
+        if steps:
+            assert np.all(np.isfinite(losses)) and cache.stats()[0] > 100
+        errs[kind] = rel_mse(r64 / spp, ref)
+        sess.close()
+        cache.close()
+    print(f"  relMSE at 16 spp vs 256 spp: {errs}", flush=True)
+    assert errs["trained"] < errs["untrained"]
+    sc.close()
