@@ -383,14 +383,15 @@ int pyre_nrc_train_step(pyre_ctx* ctx, pyre_nrc* nrc, const float* feats, const 
                         float* loss);
 /* Device-resident forms (no counterpart in the reference): feats, rgb,
  * targets and loss are device arrays; the work is queued on stream
- * (hipStream_t, or NULL for the context's stream) without synchronising.
- * loss, when not NULL, has the batch's mean loss added to it. A batch larger
- * than any before grows the training buffers (an allocation). */
+ * (hipStream_t, or NULL for the context's stream) without synchronising or
+ * allocating (a training step's scratch is fixed at pyre_nrc_create). loss,
+ * when not NULL, has the batch's mean loss added to it. */
 int pyre_nrc_infer_dev(pyre_ctx* ctx, pyre_nrc* nrc, const float* feats, size_t n, float* rgb, void* stream);
 int pyre_nrc_train_step_dev(pyre_ctx* ctx, pyre_nrc* nrc, const float* feats, const float* targets, size_t n,
                             float* loss, void* stream);
-/* Sizes the training buffers for batches of up to max_batch samples now, so
- * that no later training step of at most that many allocates. */
+/* Sizes the staging buffers of the host-array entry points
+ * (pyre_nrc_gradients, pyre_nrc_train_step) for batches of up to max_batch
+ * samples now, so that no later call of at most that many allocates. */
 int pyre_nrc_reserve(pyre_ctx* ctx, pyre_nrc* nrc, size_t max_batch);
 /* Adam steps taken so far and the loss of the last training batch. */
 int pyre_nrc_stats(pyre_ctx* ctx, pyre_nrc* nrc, uint64_t* steps, float* last_loss);

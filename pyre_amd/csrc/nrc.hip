@@ -274,13 +274,12 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_infer(const bf16x8* __rest
 }
 
 // ---------------------------------------------------------- training
-// Weight gradients are summed over chunks of kDwChunk samples per block.
-constexpr int kDwChunk = 128;
-
-// Four launches per step: fused encode + forward + loss gradient, fused
-// backward dX through all layers, weight gradients of all layers, and their
-// deterministic reduction fused with the Adam update.
-constexpr int kTrainWaves = 4;
+// Two launches per step: the fused forward / loss / backward / weight
+// gradients (k_train_fused), then the partials' deterministic reduction
+// fused with the Adam update (k_train_update).
+static_assert(param_offset(1) % 64 == 0 && param_offset(2) % 64 == 0 && param_offset(5) % 64 == 0 &&
+                  kParams % 64 == 0,
+              "parameter blocks of 64 stay within one layer");
 
 // Size of a training batch: n given by the host, or, with count, the samples
 // [off, off + n) of the *count (at most cap) a device stage produced; none
@@ -295,275 +294,219 @@ __device__ __forceinline__ uint32_t batch_size(const BatchN& b) {
   return avail >= b.off + 16u ? min(b.n, avail - b.off) : 0u;
 }
 
-// Encode + forward of every layer + relative-L2 loss gradient for tiles of 16
-// samples. Writes each layer's input (bf16, sample-major), the loss gradient
-// dP (16 wide, zero past RGB and for padding rows q >= n_real) and per-block
-// loss sums; counts the Adam step in *step when given and the batch is not
-// empty.
-__global__ __launch_bounds__(64 * kTrainWaves) void k_train_fwd(const bf16x8* __restrict__ wfrag,
-                                                                const float* __restrict__ feats,
-                                                                const float* __restrict__ targets, BatchN bn,
-                                                                NrcBounds b, __bf16* acts, uint32_t cap, __bf16* dp,
-                                                                float* loss_part, uint32_t* step) {
-  const uint32_t n_real = batch_size(bn), np = (n_real + 15u) / 16u * 16u;
+// ------------------------------------------------- fused training step
+// One launch computes a step's loss, its backward pass and its weight
+// gradients: a block of 4 waves walks chunks of 64 samples (the batch's
+// chunks c = blockIdx, blockIdx + P, ...) and keeps everything a chunk needs
+// in LDS -- every layer's input, in both the sample-major layout the forward
+// and backward MFMAs read and the feature-major layout of the weight
+// gradients (whose MFMA K dimension is the samples), and the current output
+// gradient, double-buffered -- so no activation leaves the CU. Each wave
+// accumulates 25 of the 100 16x16 weight-gradient tiles in registers over all
+// of the block's chunks and writes them once as the block's partial (the
+// partial layout of k_train_update, which sums the P partials in a fixed
+// order and takes the Adam step). Layer 0's forward and the loss run one
+// sample tile per wave; the other layers run one 16-feature output tile per
+// wave over the chunk's four sample tiles, so each weight fragment is read
+// from L2 once per chunk per wave.
+constexpr int kFChunk = 64;             // samples per chunk
+constexpr int kFStrideS = kFChunk + 8;  // feature-major rows: 64 samples + pad (bf16)
+constexpr int kFStrideF = 64 + 8;       // sample-major rows: 64 features + pad (bf16)
+constexpr int kFTiles = 25;             // weight-gradient tiles per wave
+
+struct FusedLds {
+  __bf16 xt0[128 * kFStrideS];        // X_0^T (encoding), feature-major
+  __bf16 xs[5][kFChunk * kFStrideF];  // X_1..X_5 (inputs of layers 1..5), sample-major
+  __bf16 xt[5][64 * kFStrideS];       // X_1..X_5, feature-major
+  __bf16 zs[2][kFChunk * kFStrideF];  // current / next output gradient, sample-major
+  __bf16 zt[2][64 * kFStrideS];       // the same, feature-major
+};
+static_assert(sizeof(FusedLds) <= 160 * 1024, "fused training LDS exceeds the CU's 160 KB");
+
+__device__ __forceinline__ bf16x8 lds8(const __bf16* p) { return *(const bf16x8*)p; }
+
+__global__ __launch_bounds__(256, 1) void k_train_fused(const bf16x8* __restrict__ wf, const bf16x8* __restrict__ wb,
+                                                        const float* __restrict__ feats,
+                                                        const float* __restrict__ targets, BatchN bn, NrcBounds b,
+                                                        float* __restrict__ partial, float* __restrict__ loss_part,
+                                                        uint32_t* step) {
+  __shared__ __attribute__((aligned(16))) FusedLds L;
+  __shared__ float red[4];
+  const uint32_t n_real = batch_size(bn);
   if (step && n_real && blockIdx.x == 0 && threadIdx.x == 0) *step += 1u;
-  __shared__ bf16x8 sw[kFwdFrags * 64];
-  __shared__ __attribute__((aligned(16))) __bf16 sact[kTrainWaves][16][64 + 8];
-  __shared__ float red[kTrainWaves];
-  stage_frags<kFwdFrags * 64, 64 * kTrainWaves>(sw, wfrag);
-  __syncthreads();
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int g = lane >> 4, r = lane & 15;
-  const uint32_t ntiles = np / 16;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+  const uint32_t nch = (n_real + kFChunk - 1) / kFChunk;
+  f32x4 dw[kFTiles];
+#pragma unroll
+  for (int k = 0; k < kFTiles; ++k) dw[k] = f32x4{0, 0, 0, 0};
   float loss = 0.0f;
-  for (uint32_t tile = blockIdx.x * kTrainWaves + wave; tile < ntiles; tile += gridDim.x * kTrainWaves) {
-    const uint32_t q = tile * 16 + r;
-    const uint32_t row = q < n_real ? q : n_real - 1;  // padding rows repeat a sample; their gradient is zero
-    Query qq = load_query(feats + 13 * (size_t)row, b);
-    f32x4 acc[4];
+  for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
+    const uint32_t s0 = c * kFChunk;
+    // encode + layer 0 (this wave's sample tile w): Z_0^T = W_0^T X_0^T
+    {
+      const uint32_t q = s0 + 16 * w + r;
+      const uint32_t row = q < n_real ? q : n_real - 1;  // padding repeats a sample; its gradient is zero
+      const Query qq = load_query(feats + 13 * (size_t)row, b);
+      f32x4 acc[4];
 #pragma unroll
-    for (int ot = 0; ot < 4; ++ot) acc[ot] = f32x4{0, 0, 0, 0};
+      for (int ot = 0; ot < 4; ++ot) acc[ot] = f32x4{0, 0, 0, 0};
 #pragma unroll
-    for (int kb = 0; kb < 4; ++kb) {
-      bf16x8 bx = encode_frag(qq, kb, g);
-      *(bf16x8*)(acts + (size_t)q * 128 + 32 * kb + 8 * g) = bx;
+      for (int kb = 0; kb < 4; ++kb) {
+        const bf16x8 bx = encode_frag(qq, kb, g);
 #pragma unroll
-      for (int ot = 0; ot < 4; ++ot) acc[ot] = mfma(sw[(fwd_frag_offset(0) + ot * 4 + kb) * 64 + lane], bx, acc[ot]);
-    }
+        for (int j = 0; j < 8; ++j) L.xt0[(32 * kb + 8 * g + j) * kFStrideS + 16 * w + r] = bx[j];
 #pragma unroll
-    for (int l = 1; l < kLayers; ++l) {
-      __bf16* xl = acts + (size_t)cap * (128 + (l - 1) * 64) + (size_t)q * 64;
+        for (int ot = 0; ot < 4; ++ot) acc[ot] = mfma(wf[(fwd_frag_offset(0) + ot * 4 + kb) * 64 + lane], bx, acc[ot]);
+      }
+      // C[4g + i][r] = Z_0^T[16 ot + 4g + i][16 w + r] -> X_1 = ReLU
 #pragma unroll
       for (int ot = 0; ot < 4; ++ot)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) sact[wave][r][16 * ot + 4 * g + i] = to_bf16(fmaxf(acc[ot][i], 0.0f));
-      wave_lds_sync();
-      // B-operand layout: lane (g, r) holds features 32 kb + 8 g .. + 7 of
-      // sample q, which is also one 16-byte piece of its activation row
-      bf16x8 bx[2];
+        for (int i = 0; i < 4; ++i) {
+          const __bf16 v = to_bf16(fmaxf(acc[ot][i], 0.0f));
+          const int f = 16 * ot + 4 * g + i, sl = 16 * w + r;
+          L.xs[0][sl * kFStrideF + f] = v;
+          L.xt[0][f * kFStrideS + sl] = v;
+        }
+    }
+    __syncthreads();
+    // layers 1..4: this wave's output tile w over the chunk's 4 sample tiles
+#pragma unroll
+    for (int l = 1; l < kLayers - 1; ++l) {
+      f32x4 acc[4];
+#pragma unroll
+      for (int st = 0; st < 4; ++st) acc[st] = f32x4{0, 0, 0, 0};
 #pragma unroll
       for (int kb = 0; kb < 2; ++kb) {
-        bx[kb] = *(const bf16x8*)&sact[wave][r][32 * kb + 8 * g];
-        *(bf16x8*)(xl + 32 * kb + 8 * g) = bx[kb];
+        const bf16x8 a = wf[(fwd_frag_offset(l) + w * 2 + kb) * 64 + lane];
+#pragma unroll
+        for (int st = 0; st < 4; ++st)
+          acc[st] = mfma(a, lds8(&L.xs[l - 1][(16 * st + r) * kFStrideF + 32 * kb + 8 * g]), acc[st]);
       }
-      wave_lds_sync();
-      const int nout = kOut[l] / 16;
 #pragma unroll
-      for (int ot = 0; ot < 4; ++ot) {
-        acc[ot] = f32x4{0, 0, 0, 0};
-        if (ot < nout) {
+      for (int st = 0; st < 4; ++st)
 #pragma unroll
-          for (int kb = 0; kb < 2; ++kb)
-            acc[ot] = mfma(sw[(fwd_frag_offset(l) + ot * 2 + kb) * 64 + lane], bx[kb], acc[ot]);
+        for (int i = 0; i < 4; ++i) {
+          const __bf16 v = to_bf16(fmaxf(acc[st][i], 0.0f));
+          const int f = 16 * w + 4 * g + i, sl = 16 * st + r;
+          L.xs[l][sl * kFStrideF + f] = v;
+          L.xt[l][f * kFStrideS + sl] = v;
         }
+      __syncthreads();
+    }
+    // layer 5 (16 outputs, RGB used) for sample tile w, and the relative-L2
+    // loss gradient (p^2 held constant): dZ_5, zero past RGB and for padding
+    {
+      f32x4 p = {0, 0, 0, 0};
+#pragma unroll
+      for (int kb = 0; kb < 2; ++kb)
+        p = mfma(wf[(fwd_frag_offset(5) + kb) * 64 + lane], lds8(&L.xs[4][(16 * w + r) * kFStrideF + 32 * kb + 8 * g]),
+                 p);
+      const int sl = 16 * w + r;
+      const uint32_t q = s0 + sl;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int ch = 4 * g + i;
+        float gval = 0.0f;
+        if (ch < 3 && q < n_real) {
+          const float pv = p[i], t = targets[3 * (size_t)q + ch];
+          const float denom = pv * pv + 0.01f;
+          const float d = pv - t;
+          loss += d * d / denom;
+          gval = 2.0f * d / denom / (3.0f * (float)n_real);
+        }
+        const __bf16 v = to_bf16(gval);
+        L.zs[0][sl * kFStrideF + ch] = v;
+        L.zt[0][ch * kFStrideS + sl] = v;
       }
     }
-    // outputs 4g + i of sample q; relative L2 on RGB: L = (p - t)^2 / (p^2 + 0.01)
-    // with p^2 held constant in the gradient (the stop-gradient form)
+    __syncthreads();
+    // backward, layers 5..0: dW_l += X_l^T dZ_l over the chunk, then
+    // dZ_{l-1} = (W_l dZ_l^T) * (X_l > 0)
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int c = 4 * g + i;
-      float gval = 0.0f;
-      if (c < 3 && q < n_real) {
-        const float p = acc[0][i], t = targets[3 * (size_t)q + c];
-        const float denom = p * p + 0.01f;
-        const float d = p - t;
-        loss += d * d / denom;
-        gval = 2.0f * d / denom / (3.0f * (float)n_real);
+    for (int l = kLayers - 1; l >= 0; --l) {
+      const int cur = (kLayers - 1 - l) & 1;
+      const __bf16* xt = l == 0 ? L.xt0 : L.xt[l - 1];
+      // this wave's tiles of layer l: k in [base, end); layers 0..4: output
+      // tile w, input tiles k - base; layer 5: input tile w
+      constexpr int kBase[kLayers + 1] = {0, 8, 12, 16, 20, 24, 25};
+#pragma unroll
+      for (int k = kBase[l]; k < kBase[l + 1]; ++k) {
+        const int it = l == kLayers - 1 ? w : k - kBase[l], ot = l == kLayers - 1 ? 0 : w;
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb)
+          dw[k] = mfma(lds8(&xt[(16 * it + r) * kFStrideS + 32 * kb + 8 * g]),
+                       lds8(&L.zt[cur][(16 * ot + r) * kFStrideS + 32 * kb + 8 * g]), dw[k]);
       }
-      dp[(size_t)q * 16 + c] = to_bf16(gval);
+      if (l >= 1) {
+        const int nkb = (kOut[l] + 31) / 32;
+        f32x4 acc[4];
+#pragma unroll
+        for (int st = 0; st < 4; ++st) acc[st] = f32x4{0, 0, 0, 0};
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb) {
+          if (kb >= nkb) break;
+          const bf16x8 a = wb[(bwd_frag_offset(l) + w * nkb + kb) * 64 + lane];
+#pragma unroll
+          for (int st = 0; st < 4; ++st) {
+            bf16x8 bz;
+            if (kOut[l] >= 32 * kb + 8 * g + 8) {
+              bz = lds8(&L.zs[cur][(16 * st + r) * kFStrideF + 32 * kb + 8 * g]);
+            } else {
+#pragma unroll
+              for (int j = 0; j < 8; ++j) bz[j] = to_bf16(0.0f);  // K past the layer's 16 outputs
+            }
+            acc[st] = mfma(a, bz, acc[st]);
+          }
+        }
+        // C[4g + i][r] = dX_l^T[16 w + 4g + i][16 st + r]; mask by X_l > 0
+        const __bf16* xs = L.xs[l - 1];
+#pragma unroll
+        for (int st = 0; st < 4; ++st)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int f = 16 * w + 4 * g + i, sl = 16 * st + r;
+            const __bf16 v = from_bf16(xs[sl * kFStrideF + f]) > 0.0f ? to_bf16(acc[st][i]) : to_bf16(0.0f);
+            L.zs[cur ^ 1][sl * kFStrideF + f] = v;
+            L.zt[cur ^ 1][f * kFStrideS + sl] = v;
+          }
+      }
+      __syncthreads();
+    }
+  }
+  // this block's partial of every weight-gradient tile the wave owns
+  const uint32_t P = gridDim.x;
+#pragma unroll
+  for (int l = 0; l < kLayers; ++l) {
+    constexpr int kBase[kLayers + 1] = {0, 8, 12, 16, 20, 24, 25};
+    const int in = kIn[l], out = kOut[l];
+    float* dst = partial + (size_t)P * param_offset(l) + (size_t)blockIdx.x * in * out;
+#pragma unroll
+    for (int k = kBase[l]; k < kBase[l + 1]; ++k) {
+      const int it = l == kLayers - 1 ? w : k - kBase[l], ot = l == kLayers - 1 ? 0 : w;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) dst[(size_t)(16 * it + 4 * g + i) * out + 16 * ot + r] = dw[k][i];
     }
   }
   for (int o = 32; o > 0; o >>= 1) loss += __shfl_xor(loss, o);
-  if (lane == 0) red[wave] = loss;
+  if (lane == 0) red[w] = loss;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    float t = 0.0f;
-    for (int w = 0; w < kTrainWaves; ++w) t += red[w];
-    loss_part[blockIdx.x] = t;
-  }
+  if (threadIdx.x == 0) loss_part[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
 }
 
-// Backward dX through layers 5..1 for tiles of 16 samples:
-// dZ_{l-1} = (dZ_l W_l^T) * (X_l > 0), every dZ_l kept for the weight
-// gradients (dz + dz_offset(l)).
-__host__ __device__ constexpr size_t dz_offset(int l, uint32_t cap) { return (size_t)cap * 64 * (size_t)l; }
-
-__global__ __launch_bounds__(64 * kTrainWaves) void k_train_bwd(const bf16x8* __restrict__ bfrag,
-                                                                const __bf16* __restrict__ acts, BatchN bn,
-                                                                uint32_t cap, __bf16* dz) {
-  const uint32_t np = (batch_size(bn) + 15u) / 16u * 16u;
-  constexpr int kUsed = kBwdFrags - bwd_frags(0);  // layers 1..5
-  __shared__ bf16x8 sw[kUsed * 64];
-  __shared__ __attribute__((aligned(16))) __bf16 sdz[kTrainWaves][16][64 + 8];
-  stage_frags<kUsed * 64, 64 * kTrainWaves>(sw, bfrag + bwd_frag_offset(1) * 64);
-  __syncthreads();
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int g = lane >> 4, r = lane & 15;
-  const uint32_t ntiles = np / 16;
-  for (uint32_t tile = blockIdx.x * kTrainWaves + wave; tile < ntiles; tile += gridDim.x * kTrainWaves) {
-    const uint32_t q = tile * 16 + r;
-    // dZ_5 = dP (16 wide, zero-padded to the K block of 32)
-    bf16x8 bz[2];
-    {
-      const __bf16* d5 = dz + dz_offset(5, cap) + (size_t)q * 16;
-      if (g < 2) {
-        bz[0] = *(const bf16x8*)(d5 + 8 * g);
-      } else {
+// Sum of partials wave, wave + 4, ... (p[c * stride]) in a fixed order: 8
+// independent loads in flight, summed pairwise, then the groups in order.
+__device__ __forceinline__ float sum_partials(const float* p, size_t stride, uint32_t P, int wave) {
+  float s = 0.0f;
+  uint32_t c = wave;
+  for (; c + 28 < P; c += 32) {
+    float v[8];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) bz[0][j] = to_bf16(0.0f);
-      }
-    }
-#pragma unroll
-    for (int l = kLayers - 1; l >= 1; --l) {
-      const int nkb = (kOut[l] + 31) / 32;
-      const __bf16* xl = acts + (size_t)cap * (128 + (l - 1) * 64) + (size_t)q * 64;
-      __bf16* dzo = dz + dz_offset(l - 1, cap) + (size_t)q * 64;
-#pragma unroll
-      for (int it = 0; it < 4; ++it) {  // in = 64 for layers 1..5
-        f32x4 acc = {0, 0, 0, 0};
-#pragma unroll
-        for (int kb = 0; kb < 2; ++kb)
-          if (kb < nkb) acc = mfma(sw[(bwd_frag_offset(l) - bwd_frag_offset(1) + it * nkb + kb) * 64 + lane], bz[kb], acc);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) sdz[wave][r][16 * it + 4 * g + i] = to_bf16(acc[i]);
-      }
-      wave_lds_sync();
-      // ReLU mask in the B-operand layout (features 32 kb + 8 g .. + 7),
-      // 16-byte loads of X_l and stores of dZ_{l-1}
-#pragma unroll
-      for (int kb = 0; kb < 2; ++kb) {
-        const bf16x8 raw = *(const bf16x8*)&sdz[wave][r][32 * kb + 8 * g];
-        const bf16x8 xv = *(const bf16x8*)(xl + 32 * kb + 8 * g);
-#pragma unroll
-        for (int j = 0; j < 8; ++j) bz[kb][j] = from_bf16(xv[j]) > 0.0f ? raw[j] : to_bf16(0.0f);
-        *(bf16x8*)(dzo + 32 * kb + 8 * g) = bz[kb];
-      }
-      wave_lds_sync();
-    }
+    for (int k = 0; k < 8; ++k) v[k] = p[(size_t)(c + 4 * k) * stride];
+    s += ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
   }
-}
-
-// Weight gradients of every layer: block (p, layer) sums X_l^T dZ_l over the
-// chunks p, p + P, ... of kDwChunk samples (P = gridDim.x) in registers and
-// writes one partial. Each chunk is staged feature-major in LDS, so both MFMA
-// operands (A = X^T tile, B = dZ tile; K = samples) are single 16-byte reads;
-// wave w owns the 16x16 output tiles w, w + 4, ... of the layer.
-constexpr int kDwTiles = 8;               // tiles per wave: 128 x 64 / 256 / 4
-constexpr int kDwStride = kDwChunk + 8;   // bf16 per LDS row: 16-byte aligned, conflict-free 16-byte reads
-static_assert(param_offset(1) % 64 == 0 && param_offset(2) % 64 == 0 && param_offset(5) % 64 == 0 &&
-                  kParams % 64 == 0,
-              "parameter blocks of 64 stay within one layer");
-
-// One chunk's samples held in registers between its global loads and its
-// feature-major LDS copy (dst[f * kDwStride + s] = src[(q0 + s) * width + f],
-// zero past np). Work items are (sample pair, 8-feature group), sample pair
-// fastest, so each LDS store is one 32-bit word holding two samples.
-constexpr int kPairs = kDwChunk / 2;
-template <int W>
-struct ChunkRegs {
-  static constexpr int kTotal = kPairs * (W / 8);
-  static constexpr int kItems = (kTotal + 255) / 256;  // per thread (blocks of 256)
-  uint4 a[kItems], c[kItems];
-  __device__ __forceinline__ void load(const __bf16* __restrict__ src, uint32_t q0, uint32_t np) {
-#pragma unroll
-    for (int it = 0; it < kItems; ++it) {
-      const int i = threadIdx.x + 256 * it;
-      const int sp = i % kPairs, f8 = i / kPairs;
-      const uint32_t s = 2 * sp;
-      if (kTotal % 256 != 0 && i >= kTotal) break;
-      a[it] = q0 + s < np ? *(const uint4*)(src + (size_t)(q0 + s) * W + 8 * f8) : make_uint4(0, 0, 0, 0);
-      c[it] = q0 + s + 1 < np ? *(const uint4*)(src + (size_t)(q0 + s + 1) * W + 8 * f8) : make_uint4(0, 0, 0, 0);
-    }
-  }
-  __device__ __forceinline__ void store(__bf16* dst) const {
-    uint32_t* d = reinterpret_cast<uint32_t*>(dst);
-#pragma unroll
-    for (int it = 0; it < kItems; ++it) {
-      const int i = threadIdx.x + 256 * it;
-      const int sp = i % kPairs, f8 = i / kPairs;
-      const uint32_t s = 2 * sp;
-      if (kTotal % 256 != 0 && i >= kTotal) break;
-      const uint32_t wa[4] = {a[it].x, a[it].y, a[it].z, a[it].w}, wc[4] = {c[it].x, c[it].y, c[it].z, c[it].w};
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const uint32_t lo = (wa[j >> 1] >> (16 * (j & 1))) & 0xffffu;
-        const uint32_t hi = (wc[j >> 1] >> (16 * (j & 1))) & 0xffffu;
-        d[((8 * f8 + j) * kDwStride + s) >> 1] = lo | (hi << 16);
-      }
-    }
-  }
-};
-
-// Block (p, layer) of k_train_dw for a layer of IN inputs and OUT outputs:
-// chunk c + P's loads are in flight while chunk c's MFMAs run.
-template <int IN, int OUT>
-__device__ __forceinline__ void dw_layer(const __bf16* __restrict__ xl, const __bf16* __restrict__ dzl, uint32_t np,
-                                         __bf16* sx, __bf16* sz, f32x4 (&acc)[kDwTiles]) {
-  const uint32_t nch = (np + kDwChunk - 1) / kDwChunk;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
-  constexpr int kNot = OUT / 16, kTiles = (IN / 16) * kNot;
-  ChunkRegs<IN> rx;
-  ChunkRegs<OUT> rz;
-  uint32_t c = blockIdx.x;
-  if (c < nch) {
-    rx.load(xl, c * kDwChunk, np);
-    rz.load(dzl, c * kDwChunk, np);
-  }
-  for (; c < nch; c += gridDim.x) {
-    __syncthreads();  // the previous chunk's LDS reads are done
-    rx.store(sx);
-    rz.store(sz);
-    __syncthreads();
-    if (c + gridDim.x < nch) {
-      rx.load(xl, (c + gridDim.x) * kDwChunk, np);
-      rz.load(dzl, (c + gridDim.x) * kDwChunk, np);
-    }
-#pragma unroll
-    for (int k = 0; k < kDwTiles; ++k) {
-      const int t = wave + 4 * k;
-      if (t < kTiles) {
-        const int it = t / kNot, ot = t % kNot;
-        const __bf16* ra = sx + (16 * it + r) * kDwStride + 8 * g;
-        const __bf16* rb = sz + (16 * ot + r) * kDwStride + 8 * g;
-#pragma unroll
-        for (int kb = 0; kb < kDwChunk / 32; ++kb)
-          acc[k] = mfma(*(const bf16x8*)(ra + 32 * kb), *(const bf16x8*)(rb + 32 * kb), acc[k]);
-      }
-    }
-  }
-}
-
-__global__ __launch_bounds__(256) void k_train_dw(const __bf16* __restrict__ acts, const __bf16* __restrict__ dz,
-                                                  BatchN bn, uint32_t cap, float* partial) {
-  const uint32_t np = (batch_size(bn) + 15u) / 16u * 16u;
-  __shared__ __attribute__((aligned(16))) __bf16 sx[128 * kDwStride];
-  __shared__ __attribute__((aligned(16))) __bf16 sz[64 * kDwStride];
-  const int layer = blockIdx.y;
-  const int in = kIn[layer], out = kOut[layer];
-  const __bf16* xl = acts + (layer == 0 ? 0 : (size_t)cap * (128 + (layer - 1) * 64));
-  const __bf16* dzl = dz + dz_offset(layer, cap);
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
-  const int not_ = out / 16, ntile = (in / 16) * not_;
-  f32x4 acc[kDwTiles];
-#pragma unroll
-  for (int k = 0; k < kDwTiles; ++k) acc[k] = f32x4{0, 0, 0, 0};
-  if (layer == 0)
-    dw_layer<128, 64>(xl, dzl, np, sx, sz, acc);
-  else if (layer == kLayers - 1)
-    dw_layer<64, 16>(xl, dzl, np, sx, sz, acc);
-  else
-    dw_layer<64, 64>(xl, dzl, np, sx, sz, acc);
-  float* dst = partial + (size_t)gridDim.x * param_offset(layer) + (size_t)blockIdx.x * in * out;
-#pragma unroll
-  for (int k = 0; k < kDwTiles; ++k) {
-    const int t = wave + 4 * k;
-    if (t < ntile) {
-      const int it = t / not_, ot = t % not_;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) dst[(size_t)(16 * it + 4 * g + i) * out + 16 * ot + r] = acc[k][i];
-    }
-  }
+  for (; c < P; c += 4) s += p[(size_t)c * stride];
+  return s;
 }
 
 // grad[i] = sum of the P partials: block of 64 parameters (never straddling a
@@ -577,8 +520,7 @@ __global__ __launch_bounds__(256) void k_train_reduce(const float* partial, uint
   while (l + 1 < kLayers && blockIdx.x * 64 >= param_offset(l + 1)) ++l;
   const int size = kIn[l] * kOut[l], j = i - param_offset(l);
   const float* p = partial + (size_t)P * param_offset(l) + j;
-  float s = 0.0f;
-  for (uint32_t c = wave; c < P; c += 4) s += p[(size_t)c * size];
+  const float s = sum_partials(p, (size_t)size, P, wave);
   red[wave][lane] = s;
   __syncthreads();
   if (wave == 0) grad[i] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
@@ -603,7 +545,7 @@ __device__ __forceinline__ float adam_update(float& m, float& v, float w, float 
 // fragment slots (the inverse of k_make_frags' gather; the backward
 // fragments' zero padding is never written). The non-RGB output columns and
 // the padding rows of layer 0 have zero gradient and stay fixed. The step
-// number t is *step (counted by k_train_fwd); an empty batch changes nothing.
+// number t is *step (counted by k_train_fused); an empty batch changes nothing.
 // Block 0 also adds the batch's mean loss to *loss_dev and counts the step
 // in *steps_dev when they are given.
 __global__ __launch_bounds__(256) void k_train_update(const float* partial, uint32_t P, float* grad, float* w,
@@ -622,8 +564,7 @@ __global__ __launch_bounds__(256) void k_train_update(const float* partial, uint
   while (l + 1 < kLayers && blockIdx.x * 64 >= param_offset(l + 1)) ++l;
   const int in = kIn[l], out = kOut[l], j = i - param_offset(l);
   const float* p = partial + (size_t)P * param_offset(l) + j;
-  float s = 0.0f;
-  for (uint32_t c = wave; c < P; c += 4) s += p[(size_t)c * (in * out)];
+  const float s = sum_partials(p, (size_t)in * out, P, wave);
   red[wave][lane] = s;
   __syncthreads();
   if (wave == 0) {
@@ -664,10 +605,12 @@ __global__ __launch_bounds__(256) void k_train_update(const float* partial, uint
 
 inline unsigned blocks(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
-// Weight-gradient partials per layer: at least four chunks per block, at most
-// 256 partials (26 MB): large batches get more blocks for the (memory-bound)
-// weight gradients at the price of a longer reduction.
-inline unsigned dw_partials(uint32_t np) { return std::max(1u, std::min(256u, blocks(np, 4 * kDwChunk))); }
+// Blocks of k_train_fused for a batch of at most n samples: one per CU
+// (its LDS fills one), no more than the batch's 64-sample chunks. Each
+// writes one weight-gradient partial (100 KB).
+inline unsigned fused_grid(int cus, uint32_t n) {
+  return std::max(1u, std::min(blocks(n, kFChunk), (unsigned)std::max(1, cus)));
+}
 
 }  // namespace
 
@@ -683,10 +626,9 @@ struct pyre_nrc {
   bf16x8* wb = nullptr;  // backward fragments
   uint32_t* d_step = nullptr;  // Adam steps taken (device counter)
   float last_loss = 0.0f;
-  // training scratch, sized for the largest batch seen
+  // training scratch: weight-gradient partials and loss sums (one per CU),
+  // and the host entry points' staging of feats / targets (the largest batch)
   uint32_t cap = 0;
-  __bf16* acts = nullptr;  // layer inputs: 128 + 5 * 64 features per query
-  __bf16* dz = nullptr;  // every layer's output gradient (dz_offset)
   float* part = nullptr;
   float* loss_part = nullptr;
   float* feats = nullptr;
@@ -704,26 +646,13 @@ void nrc_refresh_frags(pyre_nrc* N, hipStream_t st) {
 static void ensure_cap(pyre_nrc* N, uint32_t n) {
   uint32_t need = (n + 15) / 16 * 16;
   if (need <= N->cap) return;
-  dev_free(N->acts);
-  dev_free(N->dz);
-  dev_free(N->part);
-  dev_free(N->loss_part);
   dev_free(N->feats);
   dev_free(N->targets);
   N->cap = need;
-  N->acts = dev_alloc<__bf16>((size_t)need * (128 + 5 * 64));
-  N->dz = dev_alloc<__bf16>((size_t)need * (5 * 64 + 16));
-  N->part = dev_alloc<float>((size_t)dw_partials(need) * kParams);
-  N->loss_part = dev_alloc<float>(std::max<size_t>(blocks(need, 64), (size_t)std::max(1, N->ctx->cus) * 2));
   N->feats = dev_alloc<float>((size_t)need * 13);
   N->targets = dev_alloc<float>((size_t)need * 3);
 }
 
-static size_t act_offset(int layer, uint32_t cap) {
-  return layer == 0 ? 0 : (size_t)cap * (128 + (layer - 1) * 64);
-}
-
-// Forward + loss + backward into N->grad. feats/targets are device arrays.
 // *out += scale * sum(part[0..n)) (one block).
 __global__ void k_sum_into(const float* part, uint32_t n, float scale, float* out) {
   __shared__ float red[256];
@@ -739,21 +668,15 @@ __global__ void k_sum_into(const float* part, uint32_t n, float scale, float* ou
 }
 
 // Forward + loss + backward + weight-gradient partials of one batch of at
-// most bn.n samples (feats/targets are device arrays); step counts the Adam
-// step on the device. Returns the number of per-block loss sums in
-// N->loss_part; the partials' count is dw_partials of the padded bn.n.
+// most bn.n samples (feats/targets are device arrays) in one launch; step
+// counts the Adam step on the device. Returns the number of partials (and
+// of per-block loss sums in N->loss_part).
 static unsigned train_pass(pyre_nrc* N, const float* feats, const float* targets, const BatchN& bn, uint32_t* step,
                            hipStream_t st) {
   require(bn.n > 0, "empty training batch");
-  ensure_cap(N, bn.n);
-  const uint32_t np = (bn.n + 15) / 16 * 16;
-  const uint32_t cap = N->cap;
-  const unsigned tiles = np / 16;
-  const unsigned grid = std::max(1u, std::min(blocks(tiles, kTrainWaves), (unsigned)std::max(1, N->ctx->cus) * 2));
-  hipLaunchKernelGGL(k_train_fwd, dim3(grid), dim3(64 * kTrainWaves), 0, st, N->wf, feats, targets, bn, N->bounds,
-                     N->acts, cap, N->dz + dz_offset(5, cap), N->loss_part, step);
-  hipLaunchKernelGGL(k_train_bwd, dim3(grid), dim3(64 * kTrainWaves), 0, st, N->wb, N->acts, bn, cap, N->dz);
-  hipLaunchKernelGGL(k_train_dw, dim3(dw_partials(np), kLayers), dim3(256), 0, st, N->acts, N->dz, bn, cap, N->part);
+  const unsigned grid = fused_grid(N->ctx->cus, bn.n);
+  hipLaunchKernelGGL(k_train_fused, dim3(grid), dim3(256), 0, st, N->wf, N->wb, feats, targets, bn, N->bounds,
+                     N->part, N->loss_part, step);
   return grid;
 }
 
@@ -776,8 +699,7 @@ float nrc_gradients_dev(pyre_nrc* N, const float* feats, const float* targets, u
                         float* loss_dev) {
   const BatchN bn{nullptr, 0, 0, n};
   const unsigned nlp = train_pass(N, feats, targets, bn, nullptr, st);
-  const uint32_t np = (n + 15) / 16 * 16;
-  hipLaunchKernelGGL(k_train_reduce, dim3(kParams / 64), dim3(256), 0, st, N->part, dw_partials(np), N->grad);
+  hipLaunchKernelGGL(k_train_reduce, dim3(kParams / 64), dim3(256), 0, st, N->part, nlp, N->grad);
   if (loss_dev) {
     hipLaunchKernelGGL(k_sum_into, dim3(1), dim3(256), 0, st, N->loss_part, nlp, 1.0f / (3.0f * n), loss_dev);
     return 0.0f;
@@ -789,8 +711,7 @@ float nrc_train_step_dev(pyre_nrc* N, const float* feats, const float* targets, 
                          float* loss_dev, bool read_back) {
   const BatchN bn{nullptr, 0, 0, n};
   const unsigned nlp = train_pass(N, feats, targets, bn, N->d_step, st);
-  const uint32_t np = (n + 15) / 16 * 16;
-  hipLaunchKernelGGL(k_train_update, dim3(kParams / 64), dim3(256), 0, st, N->part, dw_partials(np), N->grad, N->w,
+  hipLaunchKernelGGL(k_train_update, dim3(kParams / 64), dim3(256), 0, st, N->part, nlp, N->grad, N->w,
                      N->m, N->v, adam_args(N), (const uint32_t*)N->d_step, bn, N->wf, N->wb, N->loss_part, nlp,
                      loss_dev, (uint32_t*)nullptr);
   return loss_dev || !read_back ? 0.0f : read_loss(N, nlp, n, st);
@@ -801,8 +722,7 @@ void nrc_train_step_counted(pyre_nrc* N, const float* feats, const float* target
                             uint32_t* steps_dev) {
   const BatchN bn{count, cap, off, batch};
   const unsigned nlp = train_pass(N, feats + 13 * (size_t)off, targets + 3 * (size_t)off, bn, N->d_step, st);
-  const uint32_t np = (batch + 15) / 16 * 16;
-  hipLaunchKernelGGL(k_train_update, dim3(kParams / 64), dim3(256), 0, st, N->part, dw_partials(np), N->grad, N->w,
+  hipLaunchKernelGGL(k_train_update, dim3(kParams / 64), dim3(256), 0, st, N->part, nlp, N->grad, N->w,
                      N->m, N->v, adam_args(N), (const uint32_t*)N->d_step, bn, N->wf, N->wb, N->loss_part, nlp,
                      loss_dev, steps_dev);
 }
@@ -881,6 +801,9 @@ int pyre_nrc_create(pyre_ctx* ctx, const pyre_nrc_params* p, pyre_nrc** out) {
     N->wf = dev_alloc<bf16x8>((size_t)kFwdFrags * 64);
     N->wb = dev_alloc<bf16x8>((size_t)kBwdFrags * 64);
     N->d_step = dev_alloc<uint32_t>(1);
+    // weight-gradient partials and loss sums of k_train_fused: one per CU
+    N->part = dev_alloc<float>((size_t)std::max(1, ctx->cus) * kParams);
+    N->loss_part = dev_alloc<float>((size_t)std::max(1, ctx->cus));
     PYRE_HIP_CHECK(hipMemcpyAsync(N->w, w.data(), kParams * 4, hipMemcpyHostToDevice, st));
     PYRE_HIP_CHECK(hipMemsetAsync(N->m, 0, kParams * 4, st));
     PYRE_HIP_CHECK(hipMemsetAsync(N->v, 0, kParams * 4, st));
@@ -906,8 +829,6 @@ void pyre_nrc_destroy(pyre_nrc* N) {
   dev_free(N->wf);
   dev_free(N->wb);
   dev_free(N->d_step);
-  dev_free(N->acts);
-  dev_free(N->dz);
   dev_free(N->part);
   dev_free(N->loss_part);
   dev_free(N->feats);
