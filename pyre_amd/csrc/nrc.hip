@@ -26,6 +26,7 @@ using namespace pyre;
 namespace {
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
 
 constexpr int kLayers = 6;
@@ -90,23 +91,37 @@ struct Query {
   float rough, ar, ag, ab;
 };
 
-__device__ __forceinline__ Query load_query(const float* f, const NrcBounds& b) {
+// A query's 13 input floats, fetched ahead of their use.
+struct RawQuery {
+  float f[13];
+};
+
+__device__ __forceinline__ RawQuery fetch_query(const float* f) {
+  RawQuery r;
+#pragma unroll
+  for (int k = 0; k < 13; ++k) r.f[k] = f[k];
+  return r;
+}
+
+__device__ __forceinline__ Query query_of(const RawQuery& r, const NrcBounds& b) {
   Query q;
-  q.px = (f[0] - b.lo[0]) * b.inv_ext[0];
-  q.py = (f[1] - b.lo[1]) * b.inv_ext[1];
-  q.pz = (f[2] - b.lo[2]) * b.inv_ext[2];
-  q.dx = f[3];
-  q.dy = f[4];
-  q.dz = f[5];
-  q.nx = f[6];
-  q.ny = f[7];
-  q.nz = f[8];
-  q.rough = f[9];
-  q.ar = f[10];
-  q.ag = f[11];
-  q.ab = f[12];
+  q.px = (r.f[0] - b.lo[0]) * b.inv_ext[0];
+  q.py = (r.f[1] - b.lo[1]) * b.inv_ext[1];
+  q.pz = (r.f[2] - b.lo[2]) * b.inv_ext[2];
+  q.dx = r.f[3];
+  q.dy = r.f[4];
+  q.dz = r.f[5];
+  q.nx = r.f[6];
+  q.ny = r.f[7];
+  q.nz = r.f[8];
+  q.rough = r.f[9];
+  q.ar = r.f[10];
+  q.ag = r.f[11];
+  q.ab = r.f[12];
   return q;
 }
+
+__device__ __forceinline__ Query load_query(const float* f, const NrcBounds& b) { return query_of(fetch_query(f), b); }
 
 // Feature f of the 128-wide encoding (selects instead of indexed arrays, so
 // the query stays in registers when f differs across lanes).
@@ -315,6 +330,7 @@ constexpr int kFStrideF = 64 + 8;       // sample-major rows: 64 features + pad 
 constexpr int kFTiles = 25;             // weight-gradient tiles per wave
 
 struct FusedLds {
+  bf16x8 w0[16 * 64];                 // layer 0's forward weight fragments (16 per lane group)
   __bf16 xt0[128 * kFStrideS];        // X_0^T (encoding), feature-major
   __bf16 xs[5][kFChunk * kFStrideF];  // X_1..X_5 (inputs of layers 1..5), sample-major
   __bf16 xt[5][64 * kFStrideS];       // X_1..X_5, feature-major
@@ -331,7 +347,9 @@ __global__ __launch_bounds__(256, 1) void k_train_fused(const bf16x8* __restrict
                                                         float* __restrict__ partial, float* __restrict__ loss_part,
                                                         uint32_t* step) {
   __shared__ __attribute__((aligned(16))) FusedLds L;
-  __shared__ float red[4];
+  // the loss reduction's 4 floats sit in X_0^T's row padding (columns 64..71
+  // of row 0, never read by the products); used after the last chunk
+  float* const red = reinterpret_cast<float*>(&L.xt0[64]);
   const uint32_t n_real = batch_size(bn);
   if (step && n_real && blockIdx.x == 0 && threadIdx.x == 0) *step += 1u;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
@@ -340,34 +358,64 @@ __global__ __launch_bounds__(256, 1) void k_train_fused(const bf16x8* __restrict
 #pragma unroll
   for (int k = 0; k < kFTiles; ++k) dw[k] = f32x4{0, 0, 0, 0};
   float loss = 0.0f;
+  // layer 0's fragments are read by every wave for every chunk: from LDS,
+  // which leaves the registers they would hold to the products' operands
+  stage_frags<16 * 64, 256>(L.w0, wf + fwd_frag_offset(0) * 64);
+  // this lane's sample of a chunk (16 w + r; padding repeats a sample, whose
+  // gradient is zero) and its targets (lanes of group 0 use channels 0..2):
+  // fetched one chunk ahead, so their memory latency overlaps a chunk's work
+  // (loads without branches: a branch around them would make the compiler
+  // wait for them where it rejoins; past the last chunk the last sample is
+  // fetched again and not used)
+  auto fetch = [&](uint32_t c, RawQuery& rq, float (&tg)[3]) {
+    const uint32_t q = c * kFChunk + 16 * w + r;
+    const uint32_t row = q < n_real ? q : n_real - 1;
+    rq = fetch_query(feats + 13 * (size_t)row);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) tg[i] = targets[3 * (size_t)row + i];
+  };
+  RawQuery rq_next;
+  float tg_next[3];
+  if (n_real) fetch(blockIdx.x, rq_next, tg_next);
+  __syncthreads();
   for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
     const uint32_t s0 = c * kFChunk;
+    const RawQuery rq = rq_next;
+    float tg[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) tg[i] = tg_next[i];
+    fetch(c + gridDim.x, rq_next, tg_next);
     // encode + layer 0 (this wave's sample tile w): Z_0^T = W_0^T X_0^T
     {
-      const uint32_t q = s0 + 16 * w + r;
-      const uint32_t row = q < n_real ? q : n_real - 1;  // padding repeats a sample; its gradient is zero
-      const Query qq = load_query(feats + 13 * (size_t)row, b);
+      const Query qq = query_of(rq, b);
       f32x4 acc[4];
 #pragma unroll
       for (int ot = 0; ot < 4; ++ot) acc[ot] = f32x4{0, 0, 0, 0};
 #pragma unroll
       for (int kb = 0; kb < 4; ++kb) {
+        // the 4 weight fragments first: their LDS latency overlaps the encoding
+        bf16x8 wa[4];
+#pragma unroll
+        for (int ot = 0; ot < 4; ++ot) wa[ot] = L.w0[(ot * 4 + kb) * 64 + lane];
+        __builtin_amdgcn_sched_barrier(0);
         const bf16x8 bx = encode_frag(qq, kb, g);
 #pragma unroll
         for (int j = 0; j < 8; ++j) L.xt0[(32 * kb + 8 * g + j) * kFStrideS + 16 * w + r] = bx[j];
 #pragma unroll
-        for (int ot = 0; ot < 4; ++ot) acc[ot] = mfma(wf[(fwd_frag_offset(0) + ot * 4 + kb) * 64 + lane], bx, acc[ot]);
+        for (int ot = 0; ot < 4; ++ot) acc[ot] = mfma(wa[ot], bx, acc[ot]);
       }
-      // C[4g + i][r] = Z_0^T[16 ot + 4g + i][16 w + r] -> X_1 = ReLU
+      // C[4g + i][r] = Z_0^T[16 ot + 4g + i][16 w + r] -> X_1 = ReLU (the
+      // lane's 4 features are adjacent in the sample-major row: one store)
+      const int sl = 16 * w + r;
 #pragma unroll
-      for (int ot = 0; ot < 4; ++ot)
+      for (int ot = 0; ot < 4; ++ot) {
+        bf16x4 v;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const __bf16 v = to_bf16(fmaxf(acc[ot][i], 0.0f));
-          const int f = 16 * ot + 4 * g + i, sl = 16 * w + r;
-          L.xs[0][sl * kFStrideF + f] = v;
-          L.xt[0][f * kFStrideS + sl] = v;
-        }
+        for (int i = 0; i < 4; ++i) v[i] = to_bf16(fmaxf(acc[ot][i], 0.0f));
+        *(bf16x4*)&L.xs[0][sl * kFStrideF + 16 * ot + 4 * g] = v;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) L.xt[0][(16 * ot + 4 * g + i) * kFStrideS + sl] = v[i];
+      }
     }
     __syncthreads();
     // layers 1..4: this wave's output tile w over the chunk's 4 sample tiles
@@ -376,22 +424,30 @@ __global__ __launch_bounds__(256, 1) void k_train_fused(const bf16x8* __restrict
       f32x4 acc[4];
 #pragma unroll
       for (int st = 0; st < 4; ++st) acc[st] = f32x4{0, 0, 0, 0};
+      // all 8 operand tiles are read before the first product, so the
+      // products wait for one LDS round trip, not eight
+      bf16x8 bx[2][4];
+#pragma unroll
+      for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+        for (int st = 0; st < 4; ++st) bx[kb][st] = lds8(&L.xs[l - 1][(16 * st + r) * kFStrideF + 32 * kb + 8 * g]);
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int kb = 0; kb < 2; ++kb) {
         const bf16x8 a = wf[(fwd_frag_offset(l) + w * 2 + kb) * 64 + lane];
 #pragma unroll
-        for (int st = 0; st < 4; ++st)
-          acc[st] = mfma(a, lds8(&L.xs[l - 1][(16 * st + r) * kFStrideF + 32 * kb + 8 * g]), acc[st]);
+        for (int st = 0; st < 4; ++st) acc[st] = mfma(a, bx[kb][st], acc[st]);
       }
 #pragma unroll
-      for (int st = 0; st < 4; ++st)
+      for (int st = 0; st < 4; ++st) {
+        bf16x4 v;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const __bf16 v = to_bf16(fmaxf(acc[st][i], 0.0f));
-          const int f = 16 * w + 4 * g + i, sl = 16 * st + r;
-          L.xs[l][sl * kFStrideF + f] = v;
-          L.xt[l][f * kFStrideS + sl] = v;
-        }
+        for (int i = 0; i < 4; ++i) v[i] = to_bf16(fmaxf(acc[st][i], 0.0f));
+        const int sl = 16 * st + r;
+        *(bf16x4*)&L.xs[l][sl * kFStrideF + 16 * w + 4 * g] = v;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) L.xt[l][(16 * w + 4 * g + i) * kFStrideS + sl] = v[i];
+      }
       __syncthreads();
     }
     // layer 5 (16 outputs, RGB used) for sample tile w, and the relative-L2
@@ -404,21 +460,23 @@ __global__ __launch_bounds__(256, 1) void k_train_fused(const bf16x8* __restrict
                  p);
       const int sl = 16 * w + r;
       const uint32_t q = s0 + sl;
+      bf16x4 v;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int ch = 4 * g + i;
         float gval = 0.0f;
         if (ch < 3 && q < n_real) {
-          const float pv = p[i], t = targets[3 * (size_t)q + ch];
+          const float pv = p[i], t = tg[i];  // ch == i on the lanes of group 0
           const float denom = pv * pv + 0.01f;
           const float d = pv - t;
           loss += d * d / denom;
           gval = 2.0f * d / denom / (3.0f * (float)n_real);
         }
-        const __bf16 v = to_bf16(gval);
-        L.zs[0][sl * kFStrideF + ch] = v;
-        L.zt[0][ch * kFStrideS + sl] = v;
+        v[i] = to_bf16(gval);
       }
+      *(bf16x4*)&L.zs[0][sl * kFStrideF + 4 * g] = v;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) L.zt[0][(4 * g + i) * kFStrideS + sl] = v[i];
     }
     __syncthreads();
     // backward, layers 5..0: dW_l += X_l^T dZ_l over the chunk, then
@@ -430,16 +488,50 @@ __global__ __launch_bounds__(256, 1) void k_train_fused(const bf16x8* __restrict
       // this wave's tiles of layer l: k in [base, end); layers 0..4: output
       // tile w, input tiles k - base; layer 5: input tile w
       constexpr int kBase[kLayers + 1] = {0, 8, 12, 16, 20, 24, 25};
+      const int kT = kBase[l + 1] - kBase[l];  // 8, 4 or 1 (a constant once unrolled)
+      const int ot = l == kLayers - 1 ? 0 : w;
+      // weight gradients: this wave's dZ tile (B) and its input tiles (A),
+      // all read before the first product
+      bf16x8 zb[2], xa[8][2];
 #pragma unroll
-      for (int k = kBase[l]; k < kBase[l + 1]; ++k) {
-        const int it = l == kLayers - 1 ? w : k - kBase[l], ot = l == kLayers - 1 ? 0 : w;
+      for (int kb = 0; kb < 2; ++kb) zb[kb] = lds8(&L.zt[cur][(16 * ot + r) * kFStrideS + 32 * kb + 8 * g]);
 #pragma unroll
-        for (int kb = 0; kb < 2; ++kb)
-          dw[k] = mfma(lds8(&xt[(16 * it + r) * kFStrideS + 32 * kb + 8 * g]),
-                       lds8(&L.zt[cur][(16 * ot + r) * kFStrideS + 32 * kb + 8 * g]), dw[k]);
+      for (int t = 0; t < 8; ++t) {
+        if (t >= kT) break;
+        const int it = l == kLayers - 1 ? w : t;
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb) xa[t][kb] = lds8(&xt[(16 * it + r) * kFStrideS + 32 * kb + 8 * g]);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        if (t >= kT) break;
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb) dw[kBase[l] + t] = mfma(xa[t][kb], zb[kb], dw[kBase[l] + t]);
       }
       if (l >= 1) {
         const int nkb = (kOut[l] + 31) / 32;
+        // dZ tiles (B) and the ReLU mask of this lane's outputs
+        // (X_l[16 st + r][16 w + 4g .. + 3], one load each), all read before
+        // the first product
+        const __bf16* xs = L.xs[l - 1];
+        bf16x4 mask[4];
+        bf16x8 bz[2][4];
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+          for (int st = 0; st < 4; ++st) {
+            if (kb >= nkb) break;
+            if (kOut[l] >= 32 * kb + 8 * g + 8) {
+              bz[kb][st] = lds8(&L.zs[cur][(16 * st + r) * kFStrideF + 32 * kb + 8 * g]);
+            } else {
+#pragma unroll
+              for (int j = 0; j < 8; ++j) bz[kb][st][j] = to_bf16(0.0f);  // K past the layer's 16 outputs
+            }
+          }
+#pragma unroll
+        for (int st = 0; st < 4; ++st) mask[st] = *(const bf16x4*)&xs[(16 * st + r) * kFStrideF + 16 * w + 4 * g];
+        __builtin_amdgcn_sched_barrier(0);
         f32x4 acc[4];
 #pragma unroll
         for (int st = 0; st < 4; ++st) acc[st] = f32x4{0, 0, 0, 0};
@@ -448,28 +540,19 @@ __global__ __launch_bounds__(256, 1) void k_train_fused(const bf16x8* __restrict
           if (kb >= nkb) break;
           const bf16x8 a = wb[(bwd_frag_offset(l) + w * nkb + kb) * 64 + lane];
 #pragma unroll
-          for (int st = 0; st < 4; ++st) {
-            bf16x8 bz;
-            if (kOut[l] >= 32 * kb + 8 * g + 8) {
-              bz = lds8(&L.zs[cur][(16 * st + r) * kFStrideF + 32 * kb + 8 * g]);
-            } else {
-#pragma unroll
-              for (int j = 0; j < 8; ++j) bz[j] = to_bf16(0.0f);  // K past the layer's 16 outputs
-            }
-            acc[st] = mfma(a, bz, acc[st]);
-          }
+          for (int st = 0; st < 4; ++st) acc[st] = mfma(a, bz[kb][st], acc[st]);
         }
         // C[4g + i][r] = dX_l^T[16 w + 4g + i][16 st + r]; mask by X_l > 0
-        const __bf16* xs = L.xs[l - 1];
 #pragma unroll
-        for (int st = 0; st < 4; ++st)
+        for (int st = 0; st < 4; ++st) {
+          bf16x4 v;
 #pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const int f = 16 * w + 4 * g + i, sl = 16 * st + r;
-            const __bf16 v = from_bf16(xs[sl * kFStrideF + f]) > 0.0f ? to_bf16(acc[st][i]) : to_bf16(0.0f);
-            L.zs[cur ^ 1][sl * kFStrideF + f] = v;
-            L.zt[cur ^ 1][f * kFStrideS + sl] = v;
-          }
+          for (int i = 0; i < 4; ++i) v[i] = from_bf16(mask[st][i]) > 0.0f ? to_bf16(acc[st][i]) : to_bf16(0.0f);
+          const int sl = 16 * st + r;
+          *(bf16x4*)&L.zs[cur ^ 1][sl * kFStrideF + 16 * w + 4 * g] = v;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) L.zt[cur ^ 1][(16 * w + 4 * g + i) * kFStrideS + sl] = v[i];
+        }
       }
       __syncthreads();
     }
