@@ -324,6 +324,8 @@ typedef struct pyre_job {
   pyre_pixel_sample* out; /* width*height means, or NULL */
   double* accum64;        /* FP64 radiance sums, added to (as pyre_render), or NULL */
 } pyre_job;
+/* A job's out and accum64 may be host memory or device memory on the
+ * context's device (device buffers keep the results in HBM: no PCIe copy). */
 
 /* Launch configuration; 0 fields keep the context's pyre_tuning. A batch
  * never uses more paths per pass than the sum of its jobs' width * height *

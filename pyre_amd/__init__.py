@@ -794,11 +794,16 @@ class Job:
 
 
 def render_batch(ctx: Context, jobs, pool_paths=0, trace_blocks_per_cu=0, stack_ring=0, autotune=False,
-                 accum64=False):
+                 accum64=False, device_out=None):
     """pyre_render_batch: every job's PixelSample means (npix, 10), its FP64
-    radiance sums (npix, 3) if accum64, and the batch statistics."""
+    radiance sums (npix, 3) if accum64, and the batch statistics.
+    device_out: one device address per job (npix * 10 floats on the
+    context's device, e.g. a tensor's data_ptr()) that receives the job's
+    means in place of a host array; the returned means are then None."""
     n = len(jobs)
-    outs = [np.zeros((j.width * j.height, 10), np.float32) for j in jobs]
+    if device_out is not None and len(device_out) != n:
+        raise ValueError("device_out needs one address per job")
+    outs = None if device_out is not None else [np.zeros((j.width * j.height, 10), np.float32) for j in jobs]
     accs = [np.zeros((j.width * j.height, 3), np.float64) for j in jobs] if accum64 else None
     arr = (L.Job * max(n, 1))()
     for i, j in enumerate(jobs):
@@ -806,7 +811,7 @@ def render_batch(ctx: Context, jobs, pool_paths=0, trace_blocks_per_cu=0, stack_
         arr[i].camera = j.camera.to_c()
         arr[i].integrator = L.IntegratorParams(j.integrator.max_depth, j.integrator.min_rr_depth)
         arr[i].width, arr[i].height, arr[i].spp, arr[i].spp_offset = j.width, j.height, j.spp, j.spp_offset
-        arr[i].out = _ptr(outs[i])
+        arr[i].out = C.c_void_p(int(device_out[i])) if device_out is not None else _ptr(outs[i])
         arr[i].accum64 = _ptr(accs[i]) if accum64 else None
     p = L.BatchParams(pool_paths, trace_blocks_per_cu, stack_ring, 1 if autotune else 0)
     st = L.BatchStats()

@@ -474,6 +474,8 @@ void pyre_ctx_destroy(pyre_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   ctx->render_cache.reset();
   trace_scratch_release(ctx);
+  dev_free(ctx->batch_acc);
+  dev_free(ctx->batch_acc64);
   dev_free(ctx->d_phi_rank);
   dev_free(ctx->d_phi_val);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

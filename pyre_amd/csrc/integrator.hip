@@ -1663,15 +1663,9 @@ struct Batch {
   size_t n;
   PassShape shape;
   uint32_t slots = 1, max_npix = 0;
-  uint64_t total_paths = 0;     // sum of width * height * spp over the jobs
-  float* acc_all = nullptr;     // every job's film sums, job after job
-  double* acc64_all = nullptr;  // every job's FP64 radiance sums
-  std::vector<float*> acc;      // job j's part of acc_all
+  uint64_t total_paths = 0;  // sum of width * height * spp over the jobs
+  std::vector<float*> acc;   // job j's film sums (in the context's batch sums)
   std::vector<double*> acc64;
-  ~Batch() {
-    dev_free(acc_all);
-    dev_free(acc64_all);
-  }
 };
 
 // Paths per pass of a batch: the tuning's pool within its share of free
@@ -1860,7 +1854,7 @@ int pyre_render_batch(pyre_ctx* ctx, const pyre_job* jobs, size_t n, const pyre_
     if (stats) *stats = pyre_batch_stats{};
     if (n == 0) return;
     require(n < (1u << 20), "too many jobs");
-    Batch B{ctx, jobs, n, {}, 1, 0, {}, {}};
+    Batch B{ctx, jobs, n, {}, 1, 0, 0, {}, {}};
     for (size_t j = 0; j < n; ++j) {
       const pyre_job& J = jobs[j];
       require(J.scene && J.scene->built, "job " + std::to_string(j) + ": scene missing or BVH not built");
@@ -1878,21 +1872,30 @@ int pyre_render_batch(pyre_ctx* ctx, const pyre_job* jobs, size_t n, const pyre_
       B.shape.motion = B.shape.motion || J.scene->has_motion;
       B.slots = std::max(B.slots, sl);
     }
-    // one allocation for every job's sums
+    // every job's sums in the context's batch buffers, job after job (grown
+    // here when the batch has more pixels than any before it; the previous
+    // batch has finished with them: a batch returns after its copies)
     std::vector<size_t> off(n + 1, 0);
     for (size_t j = 0; j < n; ++j) off[j + 1] = off[j] + (size_t)jobs[j].width * jobs[j].height;
-    B.acc_all = dev_alloc<float>(10 * off[n]);
-    B.acc64_all = dev_alloc<double>(3 * off[n]);
-    clear_sums(ctx, B.acc_all, B.acc64_all, off[n]);
+    if (ctx->batch_npix < off[n]) {
+      ctx->batch_npix = 0;
+      dev_free(ctx->batch_acc);
+      dev_free(ctx->batch_acc64);
+      ctx->batch_acc = dev_alloc<float>(10 * off[n]);
+      ctx->batch_acc64 = dev_alloc<double>(3 * off[n]);
+      ctx->batch_npix = off[n];
+    }
+    clear_sums(ctx, ctx->batch_acc, ctx->batch_acc64, off[n]);
     B.acc.assign(n, nullptr);
     B.acc64.assign(n, nullptr);
     hipStream_t st = ctx->stream;
+    // job buffers may be host or device memory (hipMemcpyDefault finds out)
     for (size_t j = 0; j < n; ++j) {
       const size_t npix = off[j + 1] - off[j];
-      B.acc[j] = B.acc_all + 10 * off[j];
-      B.acc64[j] = B.acc64_all + 3 * off[j];
+      B.acc[j] = ctx->batch_acc + 10 * off[j];
+      B.acc64[j] = ctx->batch_acc64 + 3 * off[j];
       if (jobs[j].accum64)
-        PYRE_HIP_CHECK(hipMemcpyAsync(B.acc64[j], jobs[j].accum64, 3 * npix * sizeof(double), hipMemcpyHostToDevice, st));
+        PYRE_HIP_CHECK(hipMemcpyAsync(B.acc64[j], jobs[j].accum64, 3 * npix * sizeof(double), hipMemcpyDefault, st));
     }
     Tuning t = ctx_tuning(ctx);
     if (params) {
@@ -1912,10 +1915,10 @@ int pyre_render_batch(pyre_ctx* ctx, const pyre_job* jobs, size_t n, const pyre_
       if (jobs[j].out) {
         hipLaunchKernelGGL(k_scale, dim3(nblocks(10 * npix, 256)), dim3(256), 0, st, B.acc[j], 10 * npix,
                            1.0f / (float)jobs[j].spp, B.acc[j]);
-        PYRE_HIP_CHECK(hipMemcpyAsync(jobs[j].out, B.acc[j], 10 * npix * sizeof(float), hipMemcpyDeviceToHost, st));
+        PYRE_HIP_CHECK(hipMemcpyAsync(jobs[j].out, B.acc[j], 10 * npix * sizeof(float), hipMemcpyDefault, st));
       }
       if (jobs[j].accum64)
-        PYRE_HIP_CHECK(hipMemcpyAsync(jobs[j].accum64, B.acc64[j], 3 * npix * sizeof(double), hipMemcpyDeviceToHost, st));
+        PYRE_HIP_CHECK(hipMemcpyAsync(jobs[j].accum64, B.acc64[j], 3 * npix * sizeof(double), hipMemcpyDefault, st));
     }
     PYRE_HIP_CHECK(hipStreamSynchronize(st));
     if (stats) {

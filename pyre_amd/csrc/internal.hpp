@@ -34,6 +34,11 @@ struct pyre_ctx {
   // path buffers kept from the last one-shot render (owned; freed with the
   // context, on its device)
   std::shared_ptr<void> render_cache;
+  // every job's film sums of pyre_render_batch (10 floats and 3 doubles per
+  // pixel of the batch), kept for the next batch and grown, never shrunk
+  float* batch_acc = nullptr;
+  double* batch_acc64 = nullptr;
+  size_t batch_npix = 0;
   // the host libm's sinf/cosf on the BSDF samplers' angle grid (PhiTable in
   // shade.hpp), uploaded at context creation
   uint4* d_phi_rank = nullptr;
