@@ -49,16 +49,17 @@ BYTES_CLOSEST = 32 + 8 + 32 * N_NODE_CLOSEST + 36 * N_TRI_CLOSEST
 BYTES_ANY = 32 + 4 + 32 * N_NODE_ANY + 36 * N_TRI_ANY
 HBM_PEAK_GBS = 8000.0
 CPU_SAMPLE = 1 << 22
-TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                               "k_trace4_traversal_2e20tris.md")
+PROFILES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
+TRAFFIC_PROFILE = os.path.join(PROFILES, "k_trace4_traversal_2e20tris.md")
+PRODUCTION_PROFILE = os.path.join(PROFILES, "render_production.md")
 
 
-def measured_traffic(key):
-    """HBM bytes per ray of the traversal kernel from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE summary (counters scaled by a plain copy of known
-    size); None when the summary is absent."""
+def measured_traffic(key, path=TRAFFIC_PROFILE):
+    """A figure of a committed rocprofv3 summary (a "key: value" line), e.g.
+    the traversal kernel's HBM bytes per ray from FETCH_SIZE/WRITE_SIZE
+    (counters scaled by a plain copy of known size); None when absent."""
     try:
-        with open(TRAFFIC_PROFILE) as f:
+        with open(path) as f:
             for line in f:
                 if line.startswith(key + ":"):
                     return float(line.split(":", 1)[1])
@@ -264,26 +265,59 @@ def bench_production(args):
         line["relmse_eps"] = REL_EPS
     if not args.no_cpu:
         line["cpu_baseline"], osc, ocam = production_cpu_baseline(pre, cam, cspp=2)
-        line["traversal_roofline"] = production_traversal_roofline(osc, ocam, ext, sh, per)
+        line["roofline"] = production_roofline(osc, ocam, ext, sh, PROD_W * PROD_H * SPP)
     print(json.dumps(line), flush=True)
 
 
-def production_traversal_roofline(osc, ocam, ext, sh, per):
-    """C1's byte model on the production frame: per query, 32 B of ray in, 8 B
-    (closest) or 4 B (shadow) out, 32 B per node fetch and 36 B per triangle
-    test of the reference traversal (Scene::intersect / occluded over every
-    primitive's BVH, bvh.rs:86-143, scene.rs:57-140), counted by the oracle on
-    every 8th pixel in x and y at 1 spp. The frame time bounds the traversal
-    kernels' time from above, so the fraction is a lower bound."""
+def shade_byte_model(st, ext, sh, samples):
+    """Bytes k_shade must move per frame by the path pool's layout (fields
+    read or written once per vertex; CDF searches and cache-line granularity
+    not counted). Vertex kinds: V = extension rays (every one is shaded),
+    C = V - samples (vertices past the camera ray), Sh = shadow rays, and
+    from the oracle's render of every 8th pixel at 1 spp the fraction of
+    extension queries that hit a surface (H = h V) and of camera rays that
+    hit something (A = a samples). Per vertex: job id 4, ray 32, hit 16,
+    radiance 16 read and 16 written; past the camera ray queue index 4 and
+    throughput 16; at a surface the shading normal 36, the sampler state 32
+    read (not on camera rays) and 32 written, the shadow count 4; the
+    environment 24 (a texel and two pdf values: the miss's emission or the
+    hit's light sample); first hits write the AOVs 32; a continuing path
+    writes ray 32, throughput 16 and queue index 4; a shadow ray writes ray
+    32, contribution 16 and queue index 4."""
+    h = st["extension_surface_hits"] / max(st["extension_queries"], 1)
+    n_cam = ((PROD_W + 7) // 8) * ((PROD_H + 7) // 8)
+    a = st["camera_hits"] / n_cam
+    V, S0 = float(ext), float(samples)
+    C, Sh = V - S0, float(sh)
+    H, A = h * V, a * S0
+    reads = V * (4 + 32 + 16 + 16) + C * (4 + 16) + H * 36 + max(H - A, 0.0) * 32 + V * 24
+    writes = V * 16 + A * 32 + H * (32 + 4) + C * (32 + 16 + 4) + Sh * (32 + 16 + 4)
+    return {"bytes_read": reads, "bytes_written": writes, "surface_hit_fraction": round(h, 4),
+            "camera_hit_fraction": round(a, 4)}
+
+
+def production_roofline(osc, ocam, ext, sh, samples):
+    """k_shade, the frame's longest stage, against HBM: the layout byte
+    model over the stage's per-frame time and its FETCH_SIZE / WRITE_SIZE
+    bytes, both from the committed kernel trace and counter runs of this
+    frame (the stage runs inside the library's graph-replayed passes, out of
+    reach of events on the caller's stream)."""
     st = osc.render_stats(ocam, PROD_W, PROD_H, spp=1, stride=8, max_depth=12, min_rr=3)
-    b_ext = 32 + 8 + 32 * st["n_node_extension"] + 36 * st["n_tri_extension"]
-    b_sh = 32 + 4 + 32 * st["n_node_shadow"] + 36 * st["n_tri_shadow"]
-    gbs = (ext * b_ext + sh * b_sh) / per / 1e9
-    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
-            "frac_lower_bound": round(gbs / 8000.0, 4), "time": "whole frame",
-            "bytes_per_extension_ray": round(b_ext, 1), "bytes_per_shadow_ray": round(b_sh, 1),
-            "sample": {k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()},
-            "sample_pixels": "every 8th pixel in x and y (240 x 135), 1 spp"}
+    m = shade_byte_model(st, ext, sh, samples)
+    ms = measured_traffic("k_shade_ms_per_frame", PRODUCTION_PROFILE)
+    fetch = measured_traffic("k_shade_fetch_bytes_per_frame", PRODUCTION_PROFILE)
+    write = measured_traffic("k_shade_write_bytes_per_frame", PRODUCTION_PROFILE)
+    model = m["bytes_read"] + m["bytes_written"]
+    out = {"bound": "hbm", "kernel": "k_shade (12 launches per frame, one per bounce; figures per frame)",
+           "peak": HBM_PEAK_GBS, "unit": "GB/s", "algorithmic_bytes": round(model),
+           "traffic": None if fetch is None or write is None else round(fetch + write), "model": m,
+           "sample_pixels": "oracle: every 8th pixel in x and y (240 x 135), 1 spp"}
+    if ms:
+        gbs = model / (ms * 1e-3) / 1e9
+        out.update({"achieved": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4), "kernel_ms": ms})
+        if out["traffic"]:
+            out["traffic_gbs"] = round(out["traffic"] / (ms * 1e-3) / 1e9, 1)
+    return out
 
 
 def production_cpu_baseline(pre, cam, cspp):
@@ -626,6 +660,29 @@ def bench_nrc(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_cache(args):
+    """The radiance cache's kernels alone: inference over 2^20 queries and
+    training steps of 2^14, 2^16 and 2^18 samples, on synthetic queries in
+    the production scene's box (the scene is built for its bounds only)."""
+    import torch  # noqa: F401  (first: see bench_nrc)
+    import pyre_amd
+    ctx = pyre_amd.Context(0)
+    dev_name, _ = ctx.device_info()
+    _, scene, _, _ = production_scene(ctx)
+    lo, hi = scene.bounds()
+    scene.close()
+    out = cache_mfma(ctx, lo, hi, args)
+    t = out["train_step_2e16"]
+    line = {
+        "metric": "radiance-cache training step, 2^16 samples", "value": t["achieved"], "unit": "TFLOP/s",
+        "n_gpus": 1, "steps": 20, "warmup": 3, "ms_per_step": t["ms"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16 (f32 accumulate, f32 Adam)", "data": "synthetic queries",
+        "config": {"workload": "5x64 ReLU MLP (128-wide encoding in, RGB out), relative-L2 loss, Adam"},
+        "device": dev_name, "cache_mfma": out,
+    }
+    print(json.dumps(line), flush=True)
+
+
 # ------------------------------------------------------------------- batch
 def batch_jobs(ctx, n_jobs=96, seed=4):
     """96 jobs: square images with side log-uniform in [128, 1024] (128^2 to
@@ -657,23 +714,39 @@ def batch_jobs(ctx, n_jobs=96, seed=4):
 
 def bench_batch(args):
     """All 96 jobs in one pyre_render_batch call (shared path pool, graph-
-    replayed passes, autotuned launch shape); step = the whole batch."""
+    replayed passes, autotuned launch shape); step = the whole batch, its
+    means written to device buffers (results resident in HBM). One more
+    call with host buffers gives the PCIe-inclusive time."""
+    # torch first: it and the library load a HIP runtime of the same soname,
+    # and the first one loaded serves both
+    import torch
     import pyre_amd
     ctx = pyre_amd.Context(0)
     dev_name, _ = ctx.device_info()
     jobs, scenes = batch_jobs(ctx)
     total = sum(j.width * j.height * j.spp for j in jobs)
-    _, _, st0 = pyre_amd.render_batch(ctx, jobs, autotune=True)  # warmup + tuning
+    dev_out = [torch.empty((j.width * j.height, 10), dtype=torch.float32, device="cuda") for j in jobs]
+    ptrs = [t.data_ptr() for t in dev_out]
+    torch.cuda.synchronize()
+    _, _, st0 = pyre_amd.render_batch(ctx, jobs, autotune=True, device_out=ptrs)  # warmup + tuning
     tune = dict(pool_paths=st0["pool_paths"], trace_blocks_per_cu=st0["trace_blocks_per_cu"],
                 stack_ring=st0["stack_ring"])
     log(f"autotuned: {tune}")
     walls, stats = [], None
     for k in range(args.steps):
         t0 = time.perf_counter()
-        _, _, stats = pyre_amd.render_batch(ctx, jobs, **tune)
+        _, _, stats = pyre_amd.render_batch(ctx, jobs, **tune, device_out=ptrs)
         walls.append(time.perf_counter() - t0)
         log(f"batch step {k}: {walls[-1]:.3f}s ({stats['passes']} passes, gpu {stats['gpu_ms']:.1f} ms)")
     per = float(np.mean(walls))
+    t0 = time.perf_counter()
+    host_outs, _, _ = pyre_amd.render_batch(ctx, jobs, **tune)
+    host_wall = time.perf_counter() - t0
+    # the device means are the host means
+    for k in (0, len(jobs) - 1):
+        assert np.array_equal(dev_out[k].cpu().numpy().view(np.uint32), host_outs[k].view(np.uint32))
+    out_bytes = sum(o.nbytes for o in host_outs)
+    del host_outs
     # the same jobs one pyre_render call after another
     t0 = time.perf_counter()
     for j in jobs:
@@ -688,9 +761,12 @@ def bench_batch(args):
                                "HDRI, pinhole, depth 8", "jobs": len(jobs), "samples": total,
                    "triangles": int(sum(t for _, _, t in scenes))},
         "tuning": tune, "passes": stats["passes"], "gpu_ms": round(stats["gpu_ms"], 1),
+        "host_output_ms_per_step": round(host_wall * 1e3, 1), "host_output_bytes": out_bytes,
         "extension_mrays_s": round(stats["extension_rays"] / per / 1e6, 1),
         "shadow_mrays_s": round(stats["shadow_rays"] / per / 1e6, 1),
-        "sequential_gpu_s": round(seq, 3), "batched_over_sequential_speedup": round(seq / per, 3), "device": dev_name,
+        "sequential_gpu_s": round(seq, 3),
+        # both legs return host arrays (pyre_render has host outputs only)
+        "batched_over_sequential_speedup": round(seq / host_wall, 3), "device": dev_name,
     }
     if args.ref_spp:
         line["relmse_subset"] = batch_relmse(ctx, jobs, args.ref_spp)
@@ -750,10 +826,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity spot check")
     ap.add_argument("--no-nrc", action="store_true",
                     help="traversal: skip the plain-vs-radiance-cache comparison at 1080p")
-    ap.add_argument("--config", default="traversal", choices=["traversal", "cornell", "production", "nrc", "batch"],
+    ap.add_argument("--config", default="traversal",
+                    choices=["traversal", "cornell", "production", "nrc", "cache", "batch"],
                     help="traversal: the headline microbench; cornell: the Cornell render; production: the "
                          "1080p production render; nrc: seconds to relMSE 0.01, plain vs radiance cache; "
-                         "batch: 96 uneven jobs in one batched call")
+                         "cache: the radiance cache's kernels alone; batch: 96 uneven jobs in one batched call")
     ap.add_argument("--reference", default=None, help="production reference .npy (relMSE)")
     ap.add_argument("--make-reference", default=None, help="render the production reference to this .npy")
     ap.add_argument("--ref-spp", type=int, default=16384,
@@ -777,6 +854,8 @@ def main():
         return bench_production(args)
     if args.config == "nrc":
         return bench_nrc(args)
+    if args.config == "cache":
+        return bench_cache(args)
     if args.config == "batch":
         return bench_batch(args)
 

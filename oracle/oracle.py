@@ -480,13 +480,15 @@ class Scene:
         """Reference traversal work per scene query in a render of every
         stride-th pixel in x and y: per extension and per shadow query, the
         mean node fetches and triangle tests (bvh.rs:86-143 over every
-        primitive, scene.rs:57-140)."""
-        out = (C.c_uint64 * 6)()
+        primitive, scene.rs:57-140); extension queries that hit a surface;
+        samples whose camera ray hit a surface or a light."""
+        out = (C.c_uint64 * 8)()
         lib().orc_render_stats(self.h, f32(cam13), width, height, spp, 0, max_depth, min_rr, stride,
                                threads or nthreads(), out)
-        ne, nn_e, nt_e, ns, nn_s, nt_s = (int(v) for v in out)
+        ne, nn_e, nt_e, ns, nn_s, nt_s, n_surf, n_first = (int(v) for v in out)
         return {"extension_queries": ne, "n_node_extension": nn_e / max(ne, 1), "n_tri_extension": nt_e / max(ne, 1),
-                "shadow_queries": ns, "n_node_shadow": nn_s / max(ns, 1), "n_tri_shadow": nt_s / max(ns, 1)}
+                "shadow_queries": ns, "n_node_shadow": nn_s / max(ns, 1), "n_tri_shadow": nt_s / max(ns, 1),
+                "extension_surface_hits": n_surf, "camera_hits": n_first}
 
 
 def camera(origin, target, up=(0, 1, 0), vfov=45.0, aspect=1.0, aperture=0.0, focus=None):

@@ -791,6 +791,11 @@ int Scene::intersect(const Ray& ray, SurfaceInteraction* it_out, uint32_t* id_ou
       }
     }
   }
+  if (g_render_stats) {
+    if (kind == 1) g_render_stats->ext_surface++;
+    if (g_render_stats->first && kind != 0) g_render_stats->first_hit++;
+    g_render_stats->first = false;
+  }
   return kind;
 }
 
@@ -1520,7 +1525,8 @@ void orc_render(void* s, const float* cam, uint32_t width, uint32_t height, uint
 // The reference traversal's work in a render of the pixels (x, y) with
 // x % stride == 0 and y % stride == 0, spp samples each: out = {extension
 // queries, their node fetches, their triangle tests, shadow queries, their
-// node fetches, their triangle tests}.
+// node fetches, their triangle tests, extension queries that hit a surface,
+// samples whose camera ray hit a surface or a light}.
 void orc_render_stats(void* s, const float* cam, uint32_t width, uint32_t height, uint32_t spp, uint32_t spp_offset,
                       uint32_t max_depth, uint32_t min_rr, uint32_t stride, uint32_t threads, uint64_t* out) {
   const Scene& sc = *(Scene*)s;
@@ -1535,7 +1541,10 @@ void orc_render_stats(void* s, const float* cam, uint32_t width, uint32_t height
     for (size_t ry = a; ry < b; ++ry) {
       const uint32_t y = (uint32_t)ry * stride;
       for (uint32_t x = 0; x < width; x += stride)
-        for (uint32_t si = 0; si < spp; ++si) (void)render_one(sc, c, R, x, y, spp_offset + si);
+        for (uint32_t si = 0; si < spp; ++si) {
+          local.first = true;
+          (void)render_one(sc, c, R, x, y, spp_offset + si);
+        }
     }
     g_render_stats = nullptr;
     std::lock_guard<std::mutex> lk(mu);
@@ -1545,6 +1554,8 @@ void orc_render_stats(void* s, const float* cam, uint32_t width, uint32_t height
     total.ext_st.tris += local.ext_st.tris;
     total.shadow_st.nodes += local.shadow_st.nodes;
     total.shadow_st.tris += local.shadow_st.tris;
+    total.ext_surface += local.ext_surface;
+    total.first_hit += local.first_hit;
   });
   out[0] = total.ext;
   out[1] = total.ext_st.nodes;
@@ -1552,6 +1563,8 @@ void orc_render_stats(void* s, const float* cam, uint32_t width, uint32_t height
   out[3] = total.shadow;
   out[4] = total.shadow_st.nodes;
   out[5] = total.shadow_st.tris;
+  out[6] = total.ext_surface;
+  out[7] = total.first_hit;
 }
 
 // film.rs:181-198 (beauty), :125-134 (albedo), :136-147 (normal), :149-173 (depth)

@@ -202,6 +202,9 @@ struct TraversalStats {
 struct RenderStats {
   uint64_t ext = 0, shadow = 0;
   TraversalStats ext_st, shadow_st;
+  uint64_t ext_surface = 0;  // extension queries that hit a surface (kind 1)
+  uint64_t first_hit = 0;    // samples whose camera ray hit a surface or a light
+  bool first = false;        // the next extension query is a sample's camera ray
 };
 extern thread_local RenderStats* g_render_stats;
 

@@ -362,3 +362,24 @@ def test_gradient_light_le_and_pdf(oracle):
     up = env.pdf(_f([[0.0, 0.2, 0.98]]))[0]
     eq = env.pdf(_f([[0.98, 0.0, 0.2]]))[0]
     assert eq > up
+
+
+def test_render_stats_vertex_counts(oracle):
+    """The oracle's per-render event counts (the shading stage's byte model
+    is sized by them): inside the Cornell box every camera ray hits a wall or
+    the light; a path continues only from a surface hit, so extension queries
+    minus samples <= surface hits <= extension queries; a camera facing away
+    from the box hits nothing and traces one query per sample."""
+    sc = oracle.Scene.cornell()
+    w = h = 24
+    spp = 2
+    n = w * h * spp
+    cam = oracle.camera((0, 0, 3), (0, 0, 0), (0, 1, 0), 45.0, 1.0)
+    st = sc.render_stats(cam, w, h, spp=spp, stride=1, max_depth=8, min_rr=3)
+    assert st["camera_hits"] == n
+    assert st["extension_queries"] - n <= st["extension_surface_hits"] <= st["extension_queries"]
+    assert st["extension_queries"] > n  # some paths bounce
+    away = oracle.camera((0, 0, 3), (0, 0, 10), (0, 1, 0), 45.0, 1.0)
+    st = sc.render_stats(away, w, h, spp=spp, stride=1, max_depth=8, min_rr=3)
+    assert st["camera_hits"] == 0 and st["extension_surface_hits"] == 0
+    assert st["extension_queries"] == n and st["shadow_queries"] == 0
