@@ -16,8 +16,10 @@ The metric's second half follows on the same line: seconds to relMSE 0.01
 at 1920x1080 on the production scene (64 level-7 icospheres, sky/sun HDRI,
 depth 12, thin lens 0.05), the plain path tracer vs the radiance cache
 trained online on 2^16 samples per frame, against a 16384-spp FP64
-reference rendered on the GPU in the same run (nrc_over_plain; --no-nrc
-skips it).
+reference rendered on the GPU in the same run: from a cold start
+(nrc_over_plain: the cache renderer's first frame is plain while its cache
+trains), with a pre-trained cache (nrc_warm_cache_over_plain), and the
+bound of a perfect cache (nrc_best_case_over_plain). --no-nrc skips it.
 
 Roofline: algorithmic bytes per closest-hit ray =
 32 (ray) + 8 (hit) + 32 * N_node + 36 * N_tri, with N_node / N_tri the node
@@ -388,11 +390,20 @@ def _crossing(curve, target):
 NRC_TRAIN_PATHS = 1 << 18
 NRC_BATCH = 1 << 14
 NRC_STEPS = 4
-# Frames of at most this many spp, after NRC_WARMUP training-only frames
-# (measured best of frames of 1, 2, 4 and 16 spp and 0, 8 or 16 warm-up
-# frames for a cache that starts untrained).
+# A cache that starts untrained renders its first NRC_COLD_PLAIN frames as
+# plain frames (image samples traced to the end while the frame's training
+# paths train the cache) and takes no training-only frames: on the
+# production view, 1 to 16 plain frames of 4 spp reached relMSE 0.01 in
+# 1.56x the plain time, 8 training-only frames then cache frames in 3.06x,
+# cache frames from the start in 6.1x; each extra frame is an extra pass
+# over the pool, whose deep-bounce tail costs about 2.4 ms, so the cold
+# render takes one frame per checkpoint (its spp up to NRC_COLD_SPF).
+NRC_COLD_PLAIN = 1
+NRC_COLD_SPF = 128
+# Frames of at most this many spp for a cache that starts untrained and uses
+# it from the start (the --nrc-warmup comparison).
 NRC_MAX_SPF = 4
-NRC_WARMUP = 8
+NRC_WARMUP = 0
 # Untimed training-only frames of the warm-cache comparison.
 NRC_PRETRAIN = 64
 
@@ -483,7 +494,7 @@ def render_reference(ctx, scene, cam, ip, ref_spp):
     return film.accum64 / ref_spp
 
 
-def nrc_comparison(ctx, scene, cam, ip, ref, max_spp=128, warmup=0, pre_frames=0, spf_max=None):
+def nrc_comparison(ctx, scene, cam, ip, ref, max_spp=128, warmup=0, pre_frames=0, spf_max=None, plain_frames=0):
     """Seconds to relMSE 0.01 against `ref`, plain path tracer vs radiance
     cache (paths ending in the cache after the first diffuse bounce). Every
     checkpoint N spp is its own render from a cold start, timed whole from
@@ -492,7 +503,10 @@ def nrc_comparison(ctx, scene, cam, ip, ref, max_spp=128, warmup=0, pre_frames=0
     cache and frames of min(N, 16) spp, each tracing its samples and 2^18
     training paths in one pass, taking 4 Adam steps of 2^14 of the frame's
     training samples, then adding the updated cache's radiance to the paths
-    that ended in it; `warmup` training-only frames come first, timed.
+    that ended in it; `warmup` training-only frames come first, timed, and
+    the first `plain_frames` frames after them trace their image samples to the end
+    (the plain renderer's samples) while their training paths train the
+    cache.
     pre_frames > 0: the cache is first trained, untimed, for that many
     training-only frames (a warm cache, as a renderer that keeps its cache
     across frames of a still view has one), then the timed render starts
@@ -521,7 +535,8 @@ def nrc_comparison(ctx, scene, cam, ip, ref, max_spp=128, warmup=0, pre_frames=0
             sess.render(n)
         else:
             sess.render_nrc(cache, frames=warmup + n // spf, spp_per_frame=spf, train_paths=NRC_TRAIN_PATHS,
-                            train_batch=NRC_BATCH, train_steps=NRC_STEPS, warmup_frames=warmup)
+                            train_batch=NRC_BATCH, train_steps=NRC_STEPS, warmup_frames=warmup,
+                            plain_frames=plain_frames)
         dt = time.perf_counter() - t0  # render calls return after the stream is idle
         _, r64, spp = sess.read(mean=False, radiance64=True)
         assert spp == n
@@ -554,7 +569,7 @@ def nrc_comparison(ctx, scene, cam, ip, ref, max_spp=128, warmup=0, pre_frames=0
         "relmse_at_plain_seconds": {"plain": target, "nrc": round(eq, 5) if eq else None},
         "training": {"train_paths_per_frame": NRC_TRAIN_PATHS, "adam_steps_per_frame": NRC_STEPS,
                      "batch": NRC_BATCH, "samples_per_frame": NRC_STEPS * NRC_BATCH, "lr": 1e-2,
-                     "max_spp_per_frame": spf_max, "timed_warmup_frames": warmup,
+                     "max_spp_per_frame": spf_max, "timed_warmup_frames": warmup, "plain_frames": plain_frames,
                      "untimed_pretraining_frames": pre_frames},
         "plain_curve": [(round(t, 4), round(e, 5)) for t, e in plain],
         "nrc_curve": [(round(t, 4), round(e, 5)) for t, e in nrc], "path_stats_last": stats,
@@ -615,7 +630,8 @@ def nrc_block(ctx, args):
     pre, scene, cam, info = production_scene(ctx)
     ip = pyre_amd.PathIntegrator(12, 3)
     ref = render_reference(ctx, scene, cam, ip, args.ref_spp)
-    out = nrc_comparison(ctx, scene, cam, ip, ref, max_spp=args.max_spp, warmup=args.nrc_warmup)
+    out = nrc_comparison(ctx, scene, cam, ip, ref, max_spp=args.max_spp, warmup=args.nrc_warmup,
+                         plain_frames=NRC_COLD_PLAIN, spf_max=NRC_COLD_SPF)
     warm = nrc_comparison(ctx, scene, cam, ip, ref, max_spp=args.max_spp, warmup=0, pre_frames=NRC_PRETRAIN,
                           spf_max=16)
     out["warm_cache"] = {k: warm[k] for k in ("nrc_seconds", "nrc_over_plain", "relmse_at_plain_seconds",
@@ -644,7 +660,8 @@ def bench_nrc(args):
     ref = load_reference(args.reference)
     if ref is None:
         ref = render_reference(ctx, scene, cam, ip, args.ref_spp)
-    out = nrc_comparison(ctx, scene, cam, ip, ref, max_spp=args.max_spp, warmup=args.nrc_warmup)
+    out = nrc_comparison(ctx, scene, cam, ip, ref, max_spp=args.max_spp, warmup=args.nrc_warmup,
+                         plain_frames=NRC_COLD_PLAIN, spf_max=NRC_COLD_SPF)
     lo, hi = scene.bounds()
     out["cache_mfma"] = cache_mfma(ctx, lo, hi, args)
     tn = out["nrc_seconds"]

@@ -433,7 +433,11 @@ int pyre_session_reserve(pyre_ctx* ctx, pyre_session* session, uint32_t spp_per_
  * radiance the rest of the path carried back to it). Then up to train_steps
  * Adam steps run on consecutive batches of train_batch of the frame's
  * samples, and the stopped paths add the updated cache's radiance there.
- * Frames before warmup_frames only train (no image samples). */
+ * Frames before warmup_frames only train (no image samples); the next
+ * plain_frames frames trace their image samples to the end, as
+ * pyre_session_render does (the same samples, bit for bit), while their
+ * training paths train the cache; the frames after them stop paths in the
+ * cache. */
 typedef struct pyre_nrc_render_params {
   uint32_t frames;
   uint32_t spp_per_frame;
@@ -442,6 +446,7 @@ typedef struct pyre_nrc_render_params {
   uint32_t train_steps;
   float diffuse_roughness;
   uint32_t warmup_frames;
+  uint32_t plain_frames;
 } pyre_nrc_render_params;
 
 /* losses: frames floats (mean training loss per frame) or NULL. */
@@ -467,9 +472,8 @@ int pyre_session_read(pyre_ctx* ctx, pyre_session* session, pyre_pixel_sample* m
 
 /* The session's film (its means so far) packed on the device for the sinks
  * of pyre_gen.h: what = PYRE_PACK_*, out = host memory of that size. The
- * conversions of pyre_film_pack on pyre_session_read's means; the device's
- * powf may round a gamma-encoded channel one unit differently from the
- * host's. */
+ * conversions of pyre_film_pack on pyre_session_read's means, byte for byte
+ * and float for float (gamma bytes from thresholds of the host's powf). */
 int pyre_session_pack(pyre_ctx* ctx, pyre_session* session, uint32_t what, void* out);
 /* The viewer's displayed image (viewer.rs:114-184): the per-pixel running
  * mean of sample radiance updated sample by sample as d += (s - d) / n, in

@@ -726,9 +726,13 @@ class Session(_Sinks):
 
     def render_nrc(self, cache: RadianceCache, frames: int, spp_per_frame: int = 1, train_paths: int = 1 << 14,
                    train_batch: int = 1 << 14, train_steps: int = 4, diffuse_roughness: float = 0.25,
-                   warmup_frames: int = 0):
+                   warmup_frames: int = 0, plain_frames: int = 0):
+        """pyre_session_render_nrc: `warmup_frames` training-only frames, then
+        `plain_frames` frames whose image samples run to the end (the plain
+        renderer's samples) while their training paths train the cache, then
+        frames whose paths stop in the cache. Returns each frame's mean loss."""
         p = L.NrcRenderParams(frames, spp_per_frame, train_paths, train_batch, train_steps, diffuse_roughness,
-                              warmup_frames)
+                              warmup_frames, plain_frames)
         losses = np.zeros(max(frames, 1), np.float32)
         self.ctx.check(self.lib.pyre_session_render_nrc(self.ctx.h, self.h, cache.h, C.byref(p), _ptr(losses)),
                        "pyre_session_render_nrc")

@@ -196,7 +196,7 @@ SIGNATURES.update({
 
 class NrcRenderParams(C.Structure):
     _fields_ = [("frames", _u32), ("spp_per_frame", _u32), ("train_paths", _u32), ("train_batch", _u32),
-                ("train_steps", _u32), ("diffuse_roughness", _f), ("warmup_frames", _u32)]
+                ("train_steps", _u32), ("diffuse_roughness", _f), ("warmup_frames", _u32), ("plain_frames", _u32)]
 
 
 SIGNATURES.update({

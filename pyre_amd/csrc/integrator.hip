@@ -100,7 +100,8 @@ struct PassDev {
   uint32_t mode, pix_mul, pix_add, vmax;
   float diffuse_rough;
   uint32_t train0, t_sbase;
-  uint32_t split;  // MODE_SPLIT: the continuation's number
+  uint32_t split;      // MODE_SPLIT: the continuation's number
+  uint32_t cache_off;  // MODE_CACHE: 1 = image paths run to the end (plain), training paths still record
 };
 
 // What every stage reads from device memory: the pass being run is
@@ -573,7 +574,7 @@ __device__ void shade_path(const Paths& P, const Sched& sc, const JobDev& J, con
     const float rough = mat.kind == PYRE_MAT_LAMBERTIAN ? 1.0f : mat.roughness;
     const float qf[13] = {pos.x, pos.y, pos.z, -rd.x, -rd.y, -rd.z, ns.x, ns.y, ns.z, rough,
                           mat.base.x, mat.base.y, mat.base.z};
-    if (!(flags & FL_TRAIN) && depth >= 1 && (flags & FL_DIFFUSE)) {
+    if (!(flags & FL_TRAIN) && depth >= 1 && (flags & FL_DIFFUSE) && !pp.cache_off) {
       for (int k = 0; k < 13; ++k) P.qfeat[13 * (size_t)p + k] = qf[k];
       o.query = true;
       P.lum[p] = f4(l, __builtin_bit_cast(float, flags));
@@ -1371,6 +1372,7 @@ struct pyre_session {
   unsigned long long* stats = nullptr;  // ray / cache counters over the session's lifetime
   bool bound = false;                   // W's job table and counters are this session's
   uint32_t split = 0;                   // MODE_SPLIT passes: continuation number
+  bool cache_off = false;               // CACHE passes: image paths run as plain paths
   uint32_t spp_done = 0, next_sample = 0, train_frames = 0;
   uint64_t stats_seen[2] = {0, 0};  // ray counters already reported
   bool use_graph = true;
@@ -1411,6 +1413,7 @@ void set_pass(pyre_session* S, uint32_t s_base, uint32_t n_samples, uint32_t mod
   pp.vmax = S->W.vmax;
   pp.diffuse_rough = diffuse_rough;
   pp.split = S->split;
+  pp.cache_off = S->cache_off ? 1u : 0u;
   SliceDev sl{0u, s_base, n_samples, 0u, 0u, S->spp_done};
   hipLaunchKernelGGL(k_set_pass, dim3(1), dim3(1), 0, S->ctx->stream, S->W.passes, S->W.slices, pp, sl);
 }
@@ -2003,8 +2006,10 @@ int pyre_session_render_nrc(pyre_ctx* ctx, pyre_session* S, pyre_nrc* nrc, const
       // the frame's training paths (full unbiased paths from scattered
       // pixels, their own sample streams). After tracing, the frame trains on
       // its own samples, then the paths that ended in the cache are given
-      // the updated cache's radiance.
+      // the updated cache's radiance. The plain frames after the warmup
+      // trace their image samples to the end, as the plain renderer does.
       const uint32_t spp = f < p->warmup_frames ? 0u : p->spp_per_frame;
+      S->cache_off = f >= p->warmup_frames && f - p->warmup_frames < p->plain_frames;
       TrainPaths tr;
       tr.paths = p->train_paths;
       tr.s_base = kTrainStreamBase + S->train_frames;
@@ -2030,6 +2035,7 @@ int pyre_session_render_nrc(pyre_ctx* ctx, pyre_session* S, pyre_nrc* nrc, const
       }
       S->train_frames++;
     }
+    S->cache_off = false;
     if (p->frames) {
       std::vector<float> h(2 * nf);
       PYRE_HIP_CHECK(hipMemcpyAsync(h.data(), dloss, 8 * nf, hipMemcpyDeviceToHost, st));

@@ -113,6 +113,35 @@ def test_nrc_render_trains_and_terminates(ctx):
     sess.close()
 
 
+def test_nrc_plain_frames_are_the_plain_renderers_samples(ctx):
+    """Plain frames of a cache render trace their image samples to the end:
+    with the frames' training paths riding in the same passes and the cache
+    taking its Adam steps, the film's FP64 sums equal a plain session's of
+    the same samples bit for bit and no path queries the cache; a cache
+    frame after them stops paths in the cache."""
+    w, h = 40, 32
+    pre, sc, cam, cache = _nrc_setup(ctx, w, h)
+    ip = pyre_amd.PathIntegrator(8, 3)
+    kw = dict(spp_per_frame=2, train_paths=1 << 13, train_batch=1 << 12, train_steps=2)
+    s = pyre_amd.Session(sc, cam, ip, w, h, nrc_train_paths=1 << 13)
+    losses = s.render_nrc(cache, frames=3, plain_frames=3, **kw)
+    _, r64, spp = s.read(mean=False, radiance64=True)
+    st = s.stats()
+    assert spp == 6
+    assert st["cache_queries"] == 0 and st["training_samples"] > 0
+    assert cache.stats()[0] == 3 * 2 and np.all(np.isfinite(losses))
+    p = pyre_amd.Session(sc, cam, ip, w, h)
+    p.render(6)
+    _, p64, pspp = p.read(mean=False, radiance64=True)
+    p.close()
+    assert pspp == 6
+    assert np.array_equal(r64.view(np.uint64), p64.view(np.uint64))
+    s.render_nrc(cache, frames=1, **kw)
+    assert s.stats()["cache_queries"] > 0
+    s.close()
+    cache.close()
+
+
 def test_nrc_gain_over_plain_at_equal_spp(ctx):
     """What the trained cache gains: at 4 spp, relMSE against a 2048-spp
     plain reference is lower with paths ending in the trained cache than
