@@ -198,7 +198,9 @@ typedef struct pyre_tuning {
   uint32_t stack_ring;          /* LDS traversal stack entries per thread: 4, 8 or 16; 0 = 8 */
   uint32_t shade_waves;         /* 0 = shade kernel built for 4 waves per SIMD; 1 = no occupancy target */
   uint32_t traversal;           /* trace entry points: 0 = 4-wide tree, persistent threads; 1 = binary
-                                   tree, LDS stack (trees of depth <= 47); 2 = 4-wide tree, grid-stride */
+                                   tree, LDS stack (trees of depth <= 47); 2 = 4-wide tree, grid-stride;
+                                   3 = as 0 with leaf triangles tested one per step beside the node
+                                   visits */
   uint32_t eager;               /* 1 = render passes launched kernel by kernel instead of replaying a
                                    captured hipGraph */
   float memory_fraction;        /* share of the device's free memory a path pool may take, (0, 1];

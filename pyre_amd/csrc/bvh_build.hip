@@ -1047,6 +1047,16 @@ __global__ void k_reorder_tris(const float* verts, const uint32_t* ref, uint32_t
   tris[3 * (size_t)k + 2] = make_float4(e2.x, e2.y, e2.z, 0.0f);
 }
 
+// e2.w of the last slot of every leaf holds kLeafEnd (the traversal walks a
+// leaf slot by slot until that mark).
+__global__ void k_mark_leaf_ends(const BNode* nodes, uint32_t nb, float4* tris) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  const BNode& nd = nodes[i];
+  if ((nd.flags & FLAG_USED) && (nd.flags & FLAG_LEAF) && nd.count > 0)
+    tris[3 * (size_t)(nd.start + nd.count - 1) + 2].w = __builtin_bit_cast(float, kLeafEnd);
+}
+
 // Leaf count: grid-stride per thread, then per wave and block, one atomic
 // per block.
 __global__ __launch_bounds__(256) void k_count_leaves(const BNode* nodes, uint32_t n, uint32_t* cnt) {
@@ -1334,6 +1344,7 @@ void bvh_build(pyre_scene* s, const pyre_bvh_params& prm, hipStream_t st) {
   }
   hipLaunchKernelGGL(k_reorder_tris, dim3(blocks(n, 256)), dim3(256), 0, st, s->d_verts, ref, n, s->d_tris, moving,
                      s->d_tri_offset, s->n_meshes);
+  hipLaunchKernelGGL(k_mark_leaf_ends, dim3(blocks(nb, 256)), dim3(256), 0, st, nodes, nb, s->d_tris);
   bvh_collapse4(s, S, st);
   PYRE_HIP_CHECK(hipGetLastError());
   PYRE_HIP_CHECK(hipStreamSynchronize(st));

@@ -98,6 +98,10 @@ struct alignas(16) Node4 {
 };
 static_assert(sizeof(Node4) == 128, "Node4 layout");
 
+// Mark in a triangle record's e2.w: the last slot of its leaf (set by the
+// build on every leaf), so a traversal can hold a pending leaf as one slot.
+constexpr uint32_t kLeafEnd = 1u;
+
 // Build node (binned-SAH builder). Bounds are kept as order-preserving int
 // keys so they can be accumulated with integer atomics.
 struct BNode {

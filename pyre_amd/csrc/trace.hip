@@ -3,13 +3,15 @@
 //
 // Default: the 4-wide tree, persistent threads with dynamic ray fetch, an
 // 8-entry LDS stack ring per thread spilling to a per-thread global column.
-// The binary-tree kernels (LDS stack as deep as the tree) and a grid-stride
-// form of the 4-wide kernel remain selectable through the context's tuning
+// The binary-tree kernels (LDS stack as deep as the tree), a grid-stride
+// form of the 4-wide kernel and a form that tests leaf triangles one per
+// step remain selectable through the context's tuning
 // (pyre_tuning.traversal) for comparison.
 #include <algorithm>
 #include <map>
 #include <mutex>
 #include <string>
+#include <type_traits>
 
 #include "traverse.hpp"
 
@@ -58,10 +60,13 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(const Node2* __restrict
 }
 
 // Persistent 4-wide traversal with dynamic ray fetch: a lane that finishes
-// its ray takes the next one of its wave's chunk as soon as a quarter of the
+// its ray takes the next one of its wave's chunk as soon as half of the
 // wave is idle, instead of waiting for the wave's slowest ray.
 // MOTION: scenes with moving meshes; times = per-ray shutter times (null: 0).
-template <bool ANY, int RING, bool MOTION = false>
+// LEAFQ: leaf triangles tested one per step beside the node visits
+// (Trav4Q, pyre_tuning.traversal = 3); false: all of a node's hit leaves
+// tested at its visit (Trav4, the default).
+template <bool ANY, int RING, bool MOTION = false, bool LEAFQ = false>
 __global__ __launch_bounds__(kTraceThreads, 8) void k_trace4(const Node4* __restrict__ nodes,
                                                           const float4* __restrict__ tris,
                                                           const float4* __restrict__ rays, uint32_t n,
@@ -75,7 +80,7 @@ __global__ __launch_bounds__(kTraceThreads, 8) void k_trace4(const Node4* __rest
   const uint32_t nthreads = gridDim.x * kTraceThreads;
   HybridStack<RING> st{ring + threadIdx.x, kTraceThreads, gstack + tid, nthreads, 0, 0};
   WaveFeed<256, 64> feed(n, fetch);
-  Trav4<ANY, RING, MOTION> tv;
+  std::conditional_t<LEAFQ, Trav4Q<ANY, RING, MOTION>, Trav4<ANY, RING, MOTION>> tv;
   uint32_t ray = 0;
   bool active = false;
   for (;;) {
@@ -159,7 +164,7 @@ void launch(pyre_scene* s, const float4* rays, size_t n, pyre_hit8* hit, float2*
   pyre_ctx* ctx = s->ctx;
   require(ctx->trace_gstack && ctx->trace_depth >= s->depth, "trace scratch smaller than the scene's tree");
   const uint32_t mode = ctx->tune.traversal;
-  require(mode <= 2, "pyre_tuning.traversal must be 0, 1 or 2");
+  require(mode <= 3, "pyre_tuning.traversal must be 0, 1, 2 or 3");
   require(mode != 1 || s->depth + 1 <= 48, "binary traversal supports trees of depth <= 47");
   // calls sharing the context's scratch run in the order they were made,
   // whatever streams they are on
@@ -172,6 +177,10 @@ void launch(pyre_scene* s, const float4* rays, size_t n, pyre_hit8* hit, float2*
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, ctx->trace_fetch, 1u);
     hipLaunchKernelGGL((k_trace4<ANY, 8, true>), dim3(grid), dim3(kTraceThreads), 0, st, s->d_nodes4, s->d_tris,
                        rays, (uint32_t)n, hit, uv, occ, gst, ctx->trace_fetch, times, s->d_motion);
+  } else if (mode == 3) {
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, ctx->trace_fetch, 1u);
+    hipLaunchKernelGGL((k_trace4<ANY, 8, false, true>), dim3(grid), dim3(kTraceThreads), 0, st, s->d_nodes4,
+                       s->d_tris, rays, (uint32_t)n, hit, uv, occ, gst, ctx->trace_fetch, nullptr, nullptr);
   } else if (mode == 2) {
     hipLaunchKernelGGL((k_trace4s<ANY>), dim3(grid), dim3(kTraceThreads), 0, st, s->d_nodes4, s->d_tris, rays,
                        (uint32_t)n, hit, uv, occ, gst);

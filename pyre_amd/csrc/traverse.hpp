@@ -322,6 +322,141 @@ struct Trav4 {
   }
 };
 
+// Traversal of the 4-wide tree with leaf tests decoupled from node visits:
+// each step() visits at most one node and tests at most one triangle. A
+// lane keeps one pending leaf (the next slot to test); further hit leaves
+// wait on the traversal stack beside the inner nodes, marked by a negative
+// reference (~slot). A wave thus runs the triangle test once per step for
+// all lanes that have one pending, instead of the longest leaf loop of any
+// lane at every node visit.
+//
+// Hit semantics as Trav4: the closest hit is the least (t, scene-global id)
+// over the triangles tested with t in (t_min, t_max), which does not depend
+// on the order triangles are tested in; children and stacked entries
+// (nodes and leaves) whose slab entry distance lies beyond the closest hit
+// found so far are dropped, as the reference's slab test against the
+// tightened t_max would reject them (bvh.rs:146-153).
+template <bool ANY, int SL, bool MOTION = false>
+struct Trav4Q {
+  TRay r;
+  float best_t, bu, bv;
+  uint32_t best_gid;
+  bool found;
+  int32_t cur;  // next inner node to visit; -1: none
+  int32_t tri;  // next triangle slot of the pending leaf; -1: none
+
+  __device__ __forceinline__ void start(const TRay& ray, HybridStack<SL>& st) {
+    r = ray;
+    best_t = ray.t_max;
+    bu = bv = 0.0f;
+    best_gid = PYRE_MISS;
+    found = false;
+    cur = 0;
+    tri = -1;
+    st.sp = 0;
+    st.lo = 0;
+  }
+
+  // One step; true when the ray is finished. motion: the scene's motion
+  // table (MOTION only).
+  __device__ __forceinline__ bool step(const Node4* __restrict__ nodes, const float4* __restrict__ tris,
+                                       HybridStack<SL>& st, const float4* __restrict__ motion = nullptr) {
+    if (cur < 0 && tri < 0) {
+      // nothing in hand: the nearest stacked entry still in range
+      bool got = false;
+      while (st.sp > 0) {
+        const int2 en = st.pop();
+        if (ANY || __builtin_bit_cast(float, en.y) <= best_t) {
+          if (en.x >= 0)
+            cur = en.x;
+          else
+            tri = ~en.x;
+          got = true;
+          break;
+        }
+      }
+      if (!got) return true;
+    }
+    if (tri >= 0) {
+      const float4 a = tris[3 * (size_t)tri];
+      const float4 b = tris[3 * (size_t)tri + 1];
+      const float4 c = tris[3 * (size_t)tri + 2];
+      V3 o = r.o;
+      if constexpr (MOTION) o = motion_origin(motion, __builtin_bit_cast(uint32_t, b.w), r);
+      float t, u, v;
+      if (ANY) {
+        if (mt_intersect(o, r.d, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), r.t_min, r.t_max, false,
+                         &t, &u, &v)) {
+          found = true;
+          return true;
+        }
+      } else {
+        const bool ok = mt_intersect(o, r.d, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), r.t_min,
+                                     best_t, true, &t, &u, &v);
+        const uint32_t gid = __builtin_bit_cast(uint32_t, a.w);
+        const bool take = ok && (t < best_t || (found && gid < best_gid));
+        best_t = take ? t : best_t;
+        best_gid = take ? gid : best_gid;
+        bu = take ? u : bu;
+        bv = take ? v : bv;
+        found = found || take;
+      }
+      tri = __builtin_bit_cast(uint32_t, c.w) == kLeafEnd ? -1 : tri + 1;
+    }
+    if (cur >= 0) {
+      const Node4 nd = nodes[cur];
+      float e0, e1, e2, e3;
+      const bool h0 = (nd.cnt.x >= 0) & slab_test(r, nd.lox.x, nd.hix.x, nd.loy.x, nd.hiy.x, nd.loz.x, nd.hiz.x, best_t, &e0);
+      const bool h1 = (nd.cnt.y >= 0) & slab_test(r, nd.lox.y, nd.hix.y, nd.loy.y, nd.hiy.y, nd.loz.y, nd.hiz.y, best_t, &e1);
+      const bool h2 = (nd.cnt.z >= 0) & slab_test(r, nd.lox.z, nd.hix.z, nd.loy.z, nd.hiy.z, nd.loz.z, nd.hiz.z, best_t, &e2);
+      const bool h3 = (nd.cnt.w >= 0) & slab_test(r, nd.lox.w, nd.hix.w, nd.loy.w, nd.hiy.w, nd.loz.w, nd.hiz.w, best_t, &e3);
+      float k0 = h0 ? e0 : kInf, k1 = h1 ? e1 : kInf, k2 = h2 ? e2 : kInf, k3 = h3 ? e3 : kInf;
+      int c0 = h0 ? nd.cnt.x : -1, c1 = h1 ? nd.cnt.y : -1, c2 = h2 ? nd.cnt.z : -1, c3 = h3 ? nd.cnt.w : -1;
+      int32_t r0 = nd.ref.x, r1 = nd.ref.y, r2 = nd.ref.z, r3 = nd.ref.w;
+      cswap(k0, r0, c0, k1, r1, c1);
+      cswap(k2, r2, c2, k3, r3, c3);
+      cswap(k0, r0, c0, k2, r2, c2);
+      cswap(k1, r1, c1, k3, r3, c3);
+      cswap(k1, r1, c1, k2, r2, c2);
+      // farthest first: the nearest inner child becomes the next node and
+      // the nearest leaf the pending one (or, with a leaf pending, the
+      // nearest stacked entry); the rest are stacked
+      int2 next = make_int2(-1, 0), leaf = make_int2(0, 0);  // leaf.x < 0: ~slot of a leaf
+      child(st, next, leaf, k3, r3, c3);
+      child(st, next, leaf, k2, r2, c2);
+      child(st, next, leaf, k1, r1, c1);
+      child(st, next, leaf, k0, r0, c0);
+      if (leaf.x < 0) {
+        if (tri < 0)
+          tri = ~leaf.x;
+        else
+          st.push(leaf);
+      }
+      cur = next.x;
+    }
+    return false;
+  }
+
+  __device__ __forceinline__ void child(HybridStack<SL>& st, int2& next, int2& leaf, float k, int32_t ref,
+                                        int cnt) const {
+    if (cnt < 0 || k > best_t) return;
+    if (cnt > 0) {
+      if (leaf.x < 0) st.push(leaf);
+      leaf = make_int2(~ref, __builtin_bit_cast(int32_t, k));
+    } else {
+      if (next.x >= 0) st.push(next);
+      next = make_int2(ref, __builtin_bit_cast(int32_t, k));
+    }
+  }
+
+  __device__ __forceinline__ void result(THit* out) const {
+    out->gid = ANY ? (found ? 0u : PYRE_MISS) : (found ? best_gid : PYRE_MISS);
+    out->t = found ? best_t : 0.0f;
+    out->u = bu;
+    out->v = bv;
+  }
+};
+
 // Whole traversal of one ray.
 template <bool ANY, int SL>
 __device__ __forceinline__ void traverse4(const Node4* __restrict__ nodes, const float4* __restrict__ tris,

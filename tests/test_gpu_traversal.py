@@ -168,12 +168,13 @@ def test_degenerate_coincident_triangles(ctx, oracle):
     assert h["t"][0] == np.float32(1.0)
 
 
-@pytest.mark.parametrize("switch", [dict(traversal=1), dict(traversal=2), dict(stack_ring=4), dict(stack_ring=16),
-                                    dict(trace_blocks_per_cu=2)])
+@pytest.mark.parametrize("switch", [dict(traversal=1), dict(traversal=2), dict(traversal=3), dict(stack_ring=4),
+                                    dict(stack_ring=16), dict(trace_blocks_per_cu=2)])
 def test_traversal_variants_bit_exact(ctx, oracle, switch):
     """Every selectable traversal form (binary tree with an LDS stack;
-    grid-stride persistent; LDS stack rings of 4 and 16; a small persistent
-    grid) gives the reference's hits bit for bit, closest and any."""
+    grid-stride persistent; leaf triangles tested one per step; LDS stack
+    rings of 4 and 16; a small persistent grid) gives the reference's hits
+    bit for bit, closest and any."""
     print(f"[traversal variant] {switch}", flush=True)
     with ctx.tuning(**switch):
         excluded, hits = _closest_parity(ctx, oracle, 30000, 20000, 32, 4, 41)
