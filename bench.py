@@ -857,6 +857,14 @@ def main():
                     help="nrc: timed training-only frames before the first sample")
     ap.add_argument("--nrc-scene", default="production", choices=["production", "cornell"],
                     help="nrc / --make-reference: the view of the cache comparison")
+    ap.add_argument("--tuning", action="append", default=[], metavar="FIELD=VALUE",
+                    help="traversal: a launch setting of the context (pyre_tuning field), e.g. traversal=3; "
+                         "repeatable (A/B runs of the kernel forms)")
+    ap.add_argument("--ray-order", default="generated", choices=["generated", "octant-morton"],
+                    help="traversal: order of the rays in HBM; octant-morton sorts the closest-hit rays on the "
+                         "host (untimed) by direction octant, then the Morton code of the point where each "
+                         "enters the scene box (an experiment on ray coherence; the workload's own order is "
+                         "'generated')")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: there is no multi-GPU path here "
@@ -882,6 +890,13 @@ def main():
     ctx = pyre_amd.Context(0)
     dev_name, cus = ctx.device_info()
     log(f"device {dev_name} cus={cus}")
+    tuning = {}
+    for kv in args.tuning:
+        k, v = kv.split("=", 1)
+        tuning[k] = float(v) if k == "memory_fraction" else int(v)
+    if tuning:
+        ctx.set_tuning(**tuning)
+        log(f"tuning {ctx.get_tuning()}")
     tris = pyre_amd.random_triangles(1, N_TRIS, 0.01)
     rays32 = pyre_amd.random_rays(2, N_RAYS, N_CLOSEST, "ray32")
     scene = pyre_amd.Scene.from_arrays(
@@ -893,6 +908,11 @@ def main():
         build_ms.append(scene.build_bvh(bins=BINS, max_leaf=MAX_LEAF)["build_ms"])
     info = scene.bvh_info()
     log(f"bvh nodes={info['n_nodes']} leaves={info['n_leaves']} depth={info['depth']} build_ms={build_ms}")
+    order = None
+    if args.ray_order == "octant-morton":
+        order = coherent_order(rays32[:N_CLOSEST], *scene.bounds())
+        rays32[:N_CLOSEST] = rays32[:N_CLOSEST][order]
+        log("closest-hit rays sorted by octant, then the Morton code of their entry point")
 
     stream = torch.cuda.Stream()
     with torch.cuda.stream(stream):
@@ -961,12 +981,21 @@ def main():
         "bvh": {"nodes": info["n_nodes"], "leaves": info["n_leaves"], "depth": info["depth"]},
         "device": dev_name,
     }
+    if tuning:
+        line["tuning"] = tuning
+    if order is not None:
+        line["ray_order"] = args.ray_order
     if not args.no_cpu:
         rays36 = pyre_amd.random_rays(2, N_CLOSEST, N_CLOSEST)
         osc, o, cb = cpu_baseline(tris, rays36, CPU_SAMPLE)
         line["cpu_baseline"] = cb
         # parity spot check on the first 2^16 rays: hit ids and t bit-exact
-        h = hits[: 1 << 16].cpu().numpy()
+        h = hits.cpu().numpy()
+        if order is not None:  # hits of ray order[k] sit at k
+            inv = np.empty_like(order)
+            inv[order] = np.arange(len(order))
+            h = h[inv]
+        h = h[: 1 << 16]
         g_prim = h[:, 0].view(np.uint32)
         g_t = h[:, 1].view(np.uint32)
         o_prim = np.where(o["kind"][: 1 << 16] == 1, o["tri"][: 1 << 16], 0xFFFFFFFF).astype(np.uint32)
@@ -990,6 +1019,33 @@ def main():
             line["nrc_error"] = f"{type(e).__name__}: {e}"
             log(f"nrc block failed: {line['nrc_error']}")
     print(json.dumps(line), flush=True)
+
+
+def coherent_order(rays, lo, hi):
+    """Permutation of ray32 records by (misses the scene box, direction
+    octant, 30-bit Morton code of the point where the ray enters the box):
+    neighbouring rays start near each other and point the same way."""
+    lo, hi = np.asarray(lo, np.float64), np.asarray(hi, np.float64)
+    o, d = rays[:, 0:3].astype(np.float64), rays[:, 4:7].astype(np.float64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        inv = 1.0 / d
+        t1, t2 = (lo - o) * inv, (hi - o) * inv
+    tn = np.maximum(np.nanmax(np.minimum(t1, t2), axis=1), 0.0)
+    tf = np.nanmin(np.maximum(t1, t2), axis=1)
+    miss = ~(tn <= tf)
+    p = o + d * np.where(miss, 0.0, tn)[:, None]
+    q = np.clip(((p - lo) / (hi - lo) * 1024).astype(np.int64), 0, 1023).astype(np.uint64)
+
+    def spread(v):
+        v = (v | (v << 16)) & 0x30000FF
+        v = (v | (v << 8)) & 0x300F00F
+        v = (v | (v << 4)) & 0x30C30C3
+        return (v | (v << 2)) & 0x9249249
+
+    octant = ((d[:, 0] < 0).astype(np.uint64) << 2) | ((d[:, 1] < 0).astype(np.uint64) << 1) | (d[:, 2] < 0)
+    key = (miss.astype(np.uint64) << 33) | (octant.astype(np.uint64) << 30) | \
+        (spread(q[:, 0]) << 2) | (spread(q[:, 1]) << 1) | spread(q[:, 2])
+    return np.argsort(key, kind="stable")
 
 
 def copy_probe():
