@@ -189,18 +189,22 @@ const char* pyre_last_error(const pyre_ctx* ctx);
 int pyre_ctx_device_info(pyre_ctx* ctx, char* name, size_t name_len, int* cu_count);
 
 /* Launch configuration of a context's kernels (no counterpart in the
- * reference). Results never depend on it: every setting gives the same bits.
- * A 0 field selects the default; settings apply to later calls. */
+ * reference). Results do not depend on it: every setting gives the same
+ * bits, except that the trace entry points' quantised and exact box forms
+ * (traversal 0 and 4) can differ where two triangles' hits tie within
+ * rounding (rays within rounding of a shared edge). A 0 field selects the
+ * default; settings apply to later calls. */
 typedef struct pyre_tuning {
   uint32_t pool_paths;          /* path-pool paths per render pass; 0 = 2^27. Never more than
                                    memory_fraction of the device's free memory holds */
   uint32_t trace_blocks_per_cu; /* persistent traversal blocks per CU, 1..8; 0 = 8 */
   uint32_t stack_ring;          /* LDS traversal stack entries per thread: 4, 8 or 16; 0 = 8 */
   uint32_t shade_waves;         /* 0 = shade kernel built for 4 waves per SIMD; 1 = no occupancy target */
-  uint32_t traversal;           /* trace entry points: 0 = 4-wide tree, persistent threads; 1 = binary
-                                   tree, LDS stack (trees of depth <= 47); 2 = 4-wide tree, grid-stride;
-                                   3 = as 0 with leaf triangles tested one per step beside the node
-                                   visits */
+  uint32_t traversal;           /* trace entry points: 0 = 4-wide tree with child boxes quantised to 16
+                                   bits, persistent threads; 1 = binary tree, LDS stack (trees of depth
+                                   <= 47); 2 = 4-wide tree, grid-stride; 3 = 4-wide tree, persistent, leaf
+                                   triangles tested one per step beside the node visits; 4 = as 0 on the
+                                   exact float boxes. Scenes with moving meshes take 4 for 0 */
   uint32_t eager;               /* 1 = render passes launched kernel by kernel instead of replaying a
                                    captured hipGraph */
   float memory_fraction;        /* share of the device's free memory a path pool may take, (0, 1];

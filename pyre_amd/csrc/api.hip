@@ -488,7 +488,7 @@ int pyre_ctx_set_tuning(pyre_ctx* ctx, const pyre_tuning* t) {
   if (!ctx || !t) return PYRE_EINVAL;
   const bool ok = t->trace_blocks_per_cu <= 8 && (t->stack_ring == 0 || t->stack_ring == 4 || t->stack_ring == 8 ||
                                                   t->stack_ring == 16) &&
-                  t->shade_waves <= 4 && t->traversal <= 3 && t->eager <= 1 && t->memory_fraction >= 0.0f &&
+                  t->shade_waves <= 4 && t->traversal <= 4 && t->eager <= 1 && t->memory_fraction >= 0.0f &&
                   t->memory_fraction <= 1.0f;
   if (!ok) {
     ctx->err = "pyre_ctx_set_tuning: a field is out of range";

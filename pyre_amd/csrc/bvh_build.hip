@@ -27,6 +27,8 @@
 //                reordered into leaf order as (v0, e1, e2) float4 triples.
 #include <algorithm>
 #include <chrono>
+#include <cmath>
+#include <cstddef>
 
 #include "internal.hpp"
 
@@ -1177,7 +1179,77 @@ __global__ void k_collapse_leaf_root(const BNode* nodes, Node4* out) {
   out[0] = o;
 }
 
+// Outward-rounded cell of plane v on one grid axis: lo planes down, hi
+// planes up, with a margin of 2^-20 cells, far above the double rounding of
+// (v - g) * 2^-e, so g + q * 2^e <= v (lo) and >= v (hi) in exact arithmetic.
+__device__ __forceinline__ uint32_t qcell(float v, float g, int e, bool up) {
+  const double x = ((double)v - (double)g) * ldexp(1.0, -e);
+  const double q = up ? ceil(x + 0x1p-20) : floor(x - 0x1p-20);
+  return (uint32_t)fmin(fmax(q, 0.0), 65535.0);
+}
+
+// Node4 -> Node4Q, one node per thread.
+__global__ void k_quantize4(const Node4* __restrict__ in, uint32_t n, QGrid grid, Node4Q* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Node4 nd = in[i];
+  const float lo[3][4] = {{nd.lox.x, nd.lox.y, nd.lox.z, nd.lox.w},
+                          {nd.loy.x, nd.loy.y, nd.loy.z, nd.loy.w},
+                          {nd.loz.x, nd.loz.y, nd.loz.z, nd.loz.w}};
+  const float hi[3][4] = {{nd.hix.x, nd.hix.y, nd.hix.z, nd.hix.w},
+                          {nd.hiy.x, nd.hiy.y, nd.hiy.z, nd.hiy.w},
+                          {nd.hiz.x, nd.hiz.y, nd.hiz.z, nd.hiz.w}};
+  const int cnt[4] = {nd.cnt.x, nd.cnt.y, nd.cnt.z, nd.cnt.w};
+  const int ref[4] = {nd.ref.x, nd.ref.y, nd.ref.z, nd.ref.w};
+  uint32_t ql[3][4], qh[3][4];
+  int r[4];
+  for (int c = 0; c < 4; ++c) {
+    for (int a = 0; a < 3; ++a) {
+      ql[a][c] = cnt[c] < 0 ? 65535u : qcell(lo[a][c], grid.lo[a], grid.e[a], false);
+      qh[a][c] = cnt[c] < 0 ? 0u : qcell(hi[a][c], grid.lo[a], grid.e[a], true);
+    }
+    r[c] = cnt[c] < 0 ? -1 : (cnt[c] > 0 ? (kQLeaf | ref[c]) : ref[c]);
+  }
+  Node4Q o;
+  uint32_t w[3][4];
+  for (int a = 0; a < 3; ++a) {
+    w[a][0] = ql[a][0] | (ql[a][1] << 16);
+    w[a][1] = ql[a][2] | (ql[a][3] << 16);
+    w[a][2] = qh[a][0] | (qh[a][1] << 16);
+    w[a][3] = qh[a][2] | (qh[a][3] << 16);
+  }
+  o.x = make_uint4(w[0][0], w[0][1], w[0][2], w[0][3]);
+  o.y = make_uint4(w[1][0], w[1][1], w[1][2], w[1][3]);
+  o.z = make_uint4(w[2][0], w[2][1], w[2][2], w[2][3]);
+  o.ref = make_int4(r[0], r[1], r[2], r[3]);
+  out[i] = o;
+}
+
 }  // namespace
+
+// The quantised copy of the 4-wide tree: a grid over the root box (read back
+// from the build), cells of 2^e with 65534 cells covering each axis' extent.
+void bvh_quantize4(pyre_scene* s, hipStream_t st) {
+  dev_free(s->d_nodes4q);
+  int32_t key[6];
+  PYRE_HIP_CHECK(hipMemcpyAsync(key, reinterpret_cast<const char*>(s->d_bnodes) + offsetof(BNode, bkey), sizeof(key),
+                                hipMemcpyDeviceToHost, st));
+  PYRE_HIP_CHECK(hipStreamSynchronize(st));
+  QGrid g{};
+  for (int a = 0; a < 3; ++a) {
+    const float lo = key2f(key[a]), hi = key2f(key[3 + a]);
+    const double ext = (double)hi - (double)lo;
+    int e = 0;
+    if (ext > 0.0) e = (int)std::ceil(std::log2(ext / 65534.0));
+    g.lo[a] = lo;
+    g.e[a] = std::min(std::max(e, -120), 120);
+  }
+  s->qgrid = g;
+  s->d_nodes4q = dev_alloc<Node4Q>(s->n_nodes4);
+  hipLaunchKernelGGL(k_quantize4, dim3(blocks(s->n_nodes4, 256)), dim3(256), 0, st, s->d_nodes4, s->n_nodes4, g,
+                     s->d_nodes4q);
+  PYRE_HIP_CHECK(hipGetLastError());
+}
 
 void bvh_collapse4(pyre_scene* s, Scratch& S, hipStream_t st) {
   const uint32_t nb = s->n_bnodes;
@@ -1213,6 +1285,7 @@ void bvh_collapse4(pyre_scene* s, Scratch& S, hipStream_t st) {
 
 void bvh_release(pyre_scene* s) {
   dev_free(s->d_nodes4);
+  dev_free(s->d_nodes4q);
   dev_free(s->d_nodes);
   dev_free(s->d_tris);
   dev_free(s->d_prim_order);
@@ -1346,6 +1419,7 @@ void bvh_build(pyre_scene* s, const pyre_bvh_params& prm, hipStream_t st) {
                      s->d_tri_offset, s->n_meshes);
   hipLaunchKernelGGL(k_mark_leaf_ends, dim3(blocks(nb, 256)), dim3(256), 0, st, nodes, nb, s->d_tris);
   bvh_collapse4(s, S, st);
+  bvh_quantize4(s, st);
   PYRE_HIP_CHECK(hipGetLastError());
   PYRE_HIP_CHECK(hipStreamSynchronize(st));
   s->built = true;

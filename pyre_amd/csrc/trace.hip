@@ -1,11 +1,12 @@
 // Batched closest-hit / any-hit kernels (Scene::intersect / occluded over a
 // ray array) built on the traversal cores in traverse.hpp.
 //
-// Default: the 4-wide tree, persistent threads with dynamic ray fetch, an
-// 8-entry LDS stack ring per thread spilling to a per-thread global column.
-// The binary-tree kernels (LDS stack as deep as the tree), a grid-stride
-// form of the 4-wide kernel and a form that tests leaf triangles one per
-// step remain selectable through the context's tuning
+// Default: the 4-wide tree with child boxes quantised to 16 bits (Node4Q),
+// persistent threads with dynamic ray fetch, an 8-entry LDS stack ring per
+// thread spilling to a per-thread global column. The same kernel on the
+// exact float boxes, the binary-tree kernels (LDS stack as deep as the
+// tree), a grid-stride form of the 4-wide kernel and a form that tests leaf
+// triangles one per step remain selectable through the context's tuning
 // (pyre_tuning.traversal) for comparison.
 #include <algorithm>
 #include <map>
@@ -66,21 +67,25 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(const Node2* __restrict
 // LEAFQ: leaf triangles tested one per step beside the node visits
 // (Trav4Q, pyre_tuning.traversal = 3); false: all of a node's hit leaves
 // tested at its visit (Trav4, the default).
-template <bool ANY, int RING, bool MOTION = false, bool LEAFQ = false>
-__global__ __launch_bounds__(kTraceThreads, 8) void k_trace4(const Node4* __restrict__ nodes,
+// QNODE: the tree of quantised nodes (nodes is a Node4Q array, grid its
+// quantisation grid); otherwise nodes is a Node4 array.
+template <bool ANY, int RING, bool MOTION = false, bool LEAFQ = false, bool QNODE = false>
+__global__ __launch_bounds__(kTraceThreads, 8) void k_trace4(const void* __restrict__ nodes_v,
                                                           const float4* __restrict__ tris,
                                                           const float4* __restrict__ rays, uint32_t n,
                                                           pyre_hit8* __restrict__ out_hit,
                                                           float2* __restrict__ out_uv,
                                                           uint32_t* __restrict__ out_occ, int2* gstack,
                                                           uint32_t* fetch, const float* __restrict__ times,
-                                                          const float4* __restrict__ motion) {
+                                                          const float4* __restrict__ motion, QGrid grid) {
+  using TV = std::conditional_t<LEAFQ, Trav4Q<ANY, RING, MOTION>, Trav4<ANY, RING, MOTION, QNODE>>;
+  const auto* __restrict__ nodes = static_cast<const typename TV::NodeT*>(nodes_v);
   __shared__ int2 ring[RING * kTraceThreads];
   const uint32_t tid = blockIdx.x * kTraceThreads + threadIdx.x;
   const uint32_t nthreads = gridDim.x * kTraceThreads;
   HybridStack<RING> st{ring + threadIdx.x, kTraceThreads, gstack + tid, nthreads, 0, 0};
   WaveFeed<256, 64> feed(n, fetch);
-  std::conditional_t<LEAFQ, Trav4Q<ANY, RING, MOTION>, Trav4<ANY, RING, MOTION>> tv;
+  TV tv;
   uint32_t ray = 0;
   bool active = false;
   for (;;) {
@@ -89,7 +94,7 @@ __global__ __launch_bounds__(kTraceThreads, 8) void k_trace4(const Node4* __rest
       if (feed.take(idle, !active, ray)) {
         const float4 ra = rays[2 * (size_t)ray], rb = rays[2 * (size_t)ray + 1];
         const float tm = MOTION && times ? times[ray] : 0.0f;
-        tv.start(make_tray(v3(ra.x, ra.y, ra.z), v3(rb.x, rb.y, rb.z), ra.w, rb.w, tm), st);
+        tv.start(make_tray(v3(ra.x, ra.y, ra.z), v3(rb.x, rb.y, rb.z), ra.w, rb.w, tm), st, &grid);
         active = true;
       }
       if (!__any(active)) break;  // slice exhausted and every lane idle
@@ -164,7 +169,7 @@ void launch(pyre_scene* s, const float4* rays, size_t n, pyre_hit8* hit, float2*
   pyre_ctx* ctx = s->ctx;
   require(ctx->trace_gstack && ctx->trace_depth >= s->depth, "trace scratch smaller than the scene's tree");
   const uint32_t mode = ctx->tune.traversal;
-  require(mode <= 3, "pyre_tuning.traversal must be 0, 1, 2 or 3");
+  require(mode <= 4, "pyre_tuning.traversal must be 0 to 4");
   require(mode != 1 || s->depth + 1 <= 48, "binary traversal supports trees of depth <= 47");
   // calls sharing the context's scratch run in the order they were made,
   // whatever streams they are on
@@ -172,30 +177,45 @@ void launch(pyre_scene* s, const float4* rays, size_t n, pyre_hit8* hit, float2*
   const unsigned grid = trace_grid(ctx);
   // spill columns are laid out for the grid in use (gstride = its threads)
   int2* gst = ctx->trace_gstack;
-  if (s->has_motion) {
-    // moving meshes: the persistent 4-wide kernel with shifted leaf tests
+  if (mode == 0 || mode == 4 || s->has_motion) {
+    // the persistent 4-wide kernel: mode 0 on the quantised tree, mode 4 on
+    // the exact float boxes; scenes with moving meshes shift the leaf tests
+    // and take the exact boxes (the quantised form's extra per-ray terms
+    // beside the motion state would spill registers)
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, ctx->trace_fetch, 1u);
-    hipLaunchKernelGGL((k_trace4<ANY, 8, true>), dim3(grid), dim3(kTraceThreads), 0, st, s->d_nodes4, s->d_tris,
-                       rays, (uint32_t)n, hit, uv, occ, gst, ctx->trace_fetch, times, s->d_motion);
+    const bool q = mode == 0 && !s->has_motion;
+    const void* nodes = q ? (const void*)s->d_nodes4q : (const void*)s->d_nodes4;
+    const QGrid g = q ? s->qgrid : QGrid{};
+    const uint32_t ring = tune_ring(ctx);
+#define PYRE_TRACE4(RING, MOT, QN)                                                                                  \
+  hipLaunchKernelGGL((k_trace4<ANY, RING, MOT, false, QN>), dim3(grid), dim3(kTraceThreads), 0, st, nodes, s->d_tris, \
+                     rays, (uint32_t)n, hit, uv, occ, gst, ctx->trace_fetch, times, s->d_motion, g)
+    if (s->has_motion) {
+      PYRE_TRACE4(8, true, false);
+    } else if (ring == 4) {
+      if (q)
+        PYRE_TRACE4(4, false, true);
+      else
+        PYRE_TRACE4(4, false, false);
+    } else if (ring == 16) {
+      if (q)
+        PYRE_TRACE4(16, false, true);
+      else
+        PYRE_TRACE4(16, false, false);
+    } else {
+      if (q)
+        PYRE_TRACE4(8, false, true);
+      else
+        PYRE_TRACE4(8, false, false);
+    }
+#undef PYRE_TRACE4
   } else if (mode == 3) {
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, ctx->trace_fetch, 1u);
     hipLaunchKernelGGL((k_trace4<ANY, 8, false, true>), dim3(grid), dim3(kTraceThreads), 0, st, s->d_nodes4,
-                       s->d_tris, rays, (uint32_t)n, hit, uv, occ, gst, ctx->trace_fetch, nullptr, nullptr);
+                       s->d_tris, rays, (uint32_t)n, hit, uv, occ, gst, ctx->trace_fetch, nullptr, nullptr, QGrid{});
   } else if (mode == 2) {
     hipLaunchKernelGGL((k_trace4s<ANY>), dim3(grid), dim3(kTraceThreads), 0, st, s->d_nodes4, s->d_tris, rays,
                        (uint32_t)n, hit, uv, occ, gst);
-  } else if (mode == 0) {
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, ctx->trace_fetch, 1u);
-    const uint32_t ring = tune_ring(ctx);
-    if (ring == 4)
-      hipLaunchKernelGGL((k_trace4<ANY, 4>), dim3(grid), dim3(kTraceThreads), 0, st, s->d_nodes4, s->d_tris, rays,
-                         (uint32_t)n, hit, uv, occ, gst, ctx->trace_fetch, nullptr, nullptr);
-    else if (ring == 16)
-      hipLaunchKernelGGL((k_trace4<ANY, 16>), dim3(grid), dim3(kTraceThreads), 0, st, s->d_nodes4, s->d_tris, rays,
-                         (uint32_t)n, hit, uv, occ, gst, ctx->trace_fetch, nullptr, nullptr);
-    else
-      hipLaunchKernelGGL((k_trace4<ANY, 8>), dim3(grid), dim3(kTraceThreads), 0, st, s->d_nodes4, s->d_tris, rays,
-                         (uint32_t)n, hit, uv, occ, gst, ctx->trace_fetch, nullptr, nullptr);
   } else {
     dim3 g((unsigned)((n + kTraceThreads - 1) / kTraceThreads)), block(kTraceThreads);
     const uint32_t need = s->depth + 1;
@@ -294,10 +314,11 @@ void trace_closest_noalloc(pyre_scene* s, const float4* rays, size_t n, pyre_hit
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, fetch, 1u);
   if (s->has_motion)
     hipLaunchKernelGGL((k_trace4<false, 8, true>), dim3(grid), dim3(kTraceThreads), 0, st, s->d_nodes4, s->d_tris,
-                       rays, (uint32_t)n, hits, nullptr, nullptr, gst, fetch, nullptr, s->d_motion);
+                       rays, (uint32_t)n, hits, nullptr, nullptr, gst, fetch, nullptr, s->d_motion, QGrid{});
   else
-    hipLaunchKernelGGL((k_trace4<false, 8>), dim3(grid), dim3(kTraceThreads), 0, st, s->d_nodes4, s->d_tris, rays,
-                       (uint32_t)n, hits, nullptr, nullptr, gst, fetch, nullptr, nullptr);
+    hipLaunchKernelGGL((k_trace4<false, 8, false, false, true>), dim3(grid), dim3(kTraceThreads), 0, st,
+                       s->d_nodes4q, s->d_tris, rays, (uint32_t)n, hits, nullptr, nullptr, gst, fetch, nullptr,
+                       nullptr, s->qgrid);
   PYRE_HIP_CHECK(hipGetLastError());
 }
 

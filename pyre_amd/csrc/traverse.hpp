@@ -17,6 +17,8 @@
 // of an LDS array (or of a global array for very deep trees).
 #pragma once
 
+#include <type_traits>
+
 #include "internal.hpp"
 
 namespace pyre {
@@ -179,11 +181,13 @@ __device__ __forceinline__ V3 motion_origin(const float4* __restrict__ motion, u
 // MOTION: the triangle records carry their mesh's motion slot in e1.w and a
 // moving mesh's triangles are tested with the shifted origin (the boxes
 // above them are swept, so the slab tests use the unshifted ray).
-template <bool ANY, bool MOTION = false>
+// ENDMARK: the leaf runs from slot s to the slot marked kLeafEnd (cnt unused).
+template <bool ANY, bool MOTION = false, bool ENDMARK = false>
 __device__ __forceinline__ bool leaf_hits(const float4* __restrict__ tris, const TRay& r, int32_t s, int cnt,
                                           float& best_t, uint32_t& best_gid, float& bu, float& bv, bool& found,
                                           const float4* __restrict__ motion = nullptr) {
-  for (int k = 0; k < cnt; ++k) {
+  for (int k = 0;; ++k) {
+    if (!ENDMARK && k >= cnt) break;
     const float4 a = tris[3 * (size_t)(s + k)];
     const float4 b = tris[3 * (size_t)(s + k) + 1];
     const float4 c = tris[3 * (size_t)(s + k) + 2];
@@ -208,8 +212,58 @@ __device__ __forceinline__ bool leaf_hits(const float4* __restrict__ tris, const
       bv = take ? v : bv;
       found = found || take;
     }
+    if (ENDMARK && __builtin_bit_cast(uint32_t, c.w) == kLeafEnd) break;
   }
   return false;
+}
+
+// Per-ray terms of the slab test on quantised nodes (Node4Q). A child plane
+// of grid cell q on axis k lies at t = q * a_k + b_k along the ray, with
+// a_k = 2^e_k * inv_k (exact) and b_k = (grid lo_k - o_k) * inv_k. That form
+// and the reference's (plane - o) * inv (bvh.rs:146-153) each round within a
+// few ulps of |q a_k| + |b_k| <= 65536 |a_k| + |b_k|; pad_k is 16 ulps of
+// that bound, and it is folded into the offsets of the axis' lo and hi planes
+// (moving the nearer plane nearer and the farther one farther, by the sign of
+// a_k), so on boxes already rounded outward to the grid every child interval
+// contains the reference's and the test accepts every box the reference's
+// test on the exact box accepts. An axis whose terms are not finite (a
+// direction component of 0 or near it) gets NaN planes, which the min/max
+// ignore: no bound from that axis, a wider test again.
+struct QRay {
+  V3 a, blo, bhi;
+};
+
+__device__ __forceinline__ QRay make_qray(const TRay& r, const QGrid& g) {
+  const float inv[3] = {r.inv.x, r.inv.y, r.inv.z}, o[3] = {r.o.x, r.o.y, r.o.z};
+  float av[3], lo[3], hi[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float a = ldexpf(inv[k], g.e[k]);
+    const float b = (g.lo[k] - o[k]) * inv[k];
+    const bool fin = __builtin_isfinite(a) && __builtin_isfinite(b);
+    const float pad = 0x1p-20f * fmaf(65536.0f, fabsf(a), fabsf(b));
+    const float sp = a >= 0.0f ? pad : -pad;  // lo plane nearer when a >= 0
+    av[k] = fin ? a : 0.0f;
+    lo[k] = fin ? b - sp : __builtin_nanf("");
+    hi[k] = fin ? b + sp : __builtin_nanf("");
+  }
+  QRay q;
+  q.a = v3(av[0], av[1], av[2]);
+  q.blo = v3(lo[0], lo[1], lo[2]);
+  q.bhi = v3(hi[0], hi[1], hi[2]);
+  return q;
+}
+
+// The slab test of one quantised child box (cells lx..hz), widened per axis.
+__device__ __forceinline__ bool qslab_test(const QRay& q, float t_min, uint32_t lx, uint32_t hx, uint32_t ly,
+                                           uint32_t hy, uint32_t lz, uint32_t hz, float t_hi, float* t_enter) {
+  const float t1x = fmaf((float)lx, q.a.x, q.blo.x), t2x = fmaf((float)hx, q.a.x, q.bhi.x);
+  const float t1y = fmaf((float)ly, q.a.y, q.blo.y), t2y = fmaf((float)hy, q.a.y, q.bhi.y);
+  const float t1z = fmaf((float)lz, q.a.z, q.blo.z), t2z = fmaf((float)hz, q.a.z, q.bhi.z);
+  const float tmin = rmax(rmax(rmin(t1x, t2x), rmax(rmin(t1y, t2y), rmin(t1z, t2z))), t_min);
+  const float tmax = rmin(rmin(rmax(t1x, t2x), rmin(rmax(t1y, t2y), rmax(t1z, t2z))), t_hi);
+  *t_enter = tmin;
+  return tmin <= tmax;
 }
 
 __device__ __forceinline__ void cswap(float& ka, int32_t& ra, int& ca, float& kb, int32_t& rb, int& cb) {
@@ -233,16 +287,25 @@ __device__ __forceinline__ void cswap(float& ka, int32_t& ra, int& ca, float& kb
 // a stacked child whose entry distance lies beyond the closest hit found
 // since it was pushed is dropped without fetching it (the reference's slab
 // test against the tightened t_max would reject it, bvh.rs:146-153).
-template <bool ANY, int SL, bool MOTION = false>
+//
+// QNODE: the tree of quantised nodes (Node4Q) with the widened slab test of
+// QRay: every box the exact test accepts is visited, so the set of
+// triangles that can hold the closest hit is tested as before and the hit
+// (least t, then least scene-global id) is the same.
+template <bool ANY, int SL, bool MOTION = false, bool QNODE = false>
 struct Trav4 {
+  using NodeT = std::conditional_t<QNODE, Node4Q, Node4>;
   TRay r;
+  QRay qr;  // QNODE only
   float best_t, bu, bv;
   uint32_t best_gid;
   bool found;
   int32_t cur;
 
-  __device__ __forceinline__ void start(const TRay& ray, HybridStack<SL>& st) {
+  // grid: the tree's quantisation grid (QNODE only)
+  __device__ __forceinline__ void start(const TRay& ray, HybridStack<SL>& st, const QGrid* grid = nullptr) {
     r = ray;
+    if constexpr (QNODE) qr = make_qray(ray, *grid);
     best_t = ray.t_max;
     bu = bv = 0.0f;
     best_gid = PYRE_MISS;
@@ -254,21 +317,52 @@ struct Trav4 {
 
   // Visits node `cur`; true when the ray is finished. motion: the scene's
   // motion table (MOTION only).
-  __device__ __forceinline__ bool step(const Node4* __restrict__ nodes, const float4* __restrict__ tris,
+  __device__ __forceinline__ bool step(const NodeT* __restrict__ nodes, const float4* __restrict__ tris,
                                        HybridStack<SL>& st, const float4* __restrict__ motion = nullptr) {
-    const Node4 nd = nodes[cur];
+    const NodeT nd = nodes[cur];
     // missed children: key +inf and count -1 (a hit child can have key +inf
     // only if its box is at infinity, so the count marks misses)
     // all four slab tests without branches (a missing child's box is
     // tested too and masked out), so the lanes of a wave stay converged
     float e0, e1, e2, e3;
-    const bool h0 = (nd.cnt.x >= 0) & slab_test(r, nd.lox.x, nd.hix.x, nd.loy.x, nd.hiy.x, nd.loz.x, nd.hiz.x, best_t, &e0);
-    const bool h1 = (nd.cnt.y >= 0) & slab_test(r, nd.lox.y, nd.hix.y, nd.loy.y, nd.hiy.y, nd.loz.y, nd.hiz.y, best_t, &e1);
-    const bool h2 = (nd.cnt.z >= 0) & slab_test(r, nd.lox.z, nd.hix.z, nd.loy.z, nd.hiy.z, nd.loz.z, nd.hiz.z, best_t, &e2);
-    const bool h3 = (nd.cnt.w >= 0) & slab_test(r, nd.lox.w, nd.hix.w, nd.loy.w, nd.hiy.w, nd.loz.w, nd.hiz.w, best_t, &e3);
+    bool h0, h1, h2, h3;
+    int c0, c1, c2, c3;
+    int32_t r0, r1, r2, r3;
+    if constexpr (QNODE) {
+      // child i's cells: lo/hi words (i / 2) and (2 + i / 2), half i % 2;
+      // leaves carry kQLeaf (count 1 marks a leaf, the end mark its end)
+      constexpr uint32_t m = 0xFFFFu;
+      h0 = (nd.ref.x >= 0) & qslab_test(qr, r.t_min, nd.x.x & m, nd.x.z & m, nd.y.x & m, nd.y.z & m, nd.z.x & m,
+                                        nd.z.z & m, best_t, &e0);
+      h1 = (nd.ref.y >= 0) & qslab_test(qr, r.t_min, nd.x.x >> 16, nd.x.z >> 16, nd.y.x >> 16, nd.y.z >> 16,
+                                        nd.z.x >> 16, nd.z.z >> 16, best_t, &e1);
+      h2 = (nd.ref.z >= 0) & qslab_test(qr, r.t_min, nd.x.y & m, nd.x.w & m, nd.y.y & m, nd.y.w & m, nd.z.y & m,
+                                        nd.z.w & m, best_t, &e2);
+      h3 = (nd.ref.w >= 0) & qslab_test(qr, r.t_min, nd.x.y >> 16, nd.x.w >> 16, nd.y.y >> 16, nd.y.w >> 16,
+                                        nd.z.y >> 16, nd.z.w >> 16, best_t, &e3);
+      c0 = h0 ? (nd.ref.x >> 30) : -1;
+      c1 = h1 ? (nd.ref.y >> 30) : -1;
+      c2 = h2 ? (nd.ref.z >> 30) : -1;
+      c3 = h3 ? (nd.ref.w >> 30) : -1;
+      r0 = nd.ref.x & (kQLeaf - 1);
+      r1 = nd.ref.y & (kQLeaf - 1);
+      r2 = nd.ref.z & (kQLeaf - 1);
+      r3 = nd.ref.w & (kQLeaf - 1);
+    } else {
+      h0 = (nd.cnt.x >= 0) & slab_test(r, nd.lox.x, nd.hix.x, nd.loy.x, nd.hiy.x, nd.loz.x, nd.hiz.x, best_t, &e0);
+      h1 = (nd.cnt.y >= 0) & slab_test(r, nd.lox.y, nd.hix.y, nd.loy.y, nd.hiy.y, nd.loz.y, nd.hiz.y, best_t, &e1);
+      h2 = (nd.cnt.z >= 0) & slab_test(r, nd.lox.z, nd.hix.z, nd.loy.z, nd.hiy.z, nd.loz.z, nd.hiz.z, best_t, &e2);
+      h3 = (nd.cnt.w >= 0) & slab_test(r, nd.lox.w, nd.hix.w, nd.loy.w, nd.hiy.w, nd.loz.w, nd.hiz.w, best_t, &e3);
+      c0 = h0 ? nd.cnt.x : -1;
+      c1 = h1 ? nd.cnt.y : -1;
+      c2 = h2 ? nd.cnt.z : -1;
+      c3 = h3 ? nd.cnt.w : -1;
+      r0 = nd.ref.x;
+      r1 = nd.ref.y;
+      r2 = nd.ref.z;
+      r3 = nd.ref.w;
+    }
     float k0 = h0 ? e0 : kInf, k1 = h1 ? e1 : kInf, k2 = h2 ? e2 : kInf, k3 = h3 ? e3 : kInf;
-    int c0 = h0 ? nd.cnt.x : -1, c1 = h1 ? nd.cnt.y : -1, c2 = h2 ? nd.cnt.z : -1, c3 = h3 ? nd.cnt.w : -1;
-    int32_t r0 = nd.ref.x, r1 = nd.ref.y, r2 = nd.ref.z, r3 = nd.ref.w;
     // sort the four children by entry distance (misses sort last as +inf)
     cswap(k0, r0, c0, k1, r1, c1);
     cswap(k2, r2, c2, k3, r3, c3);
@@ -305,7 +399,7 @@ struct Trav4 {
   __device__ __forceinline__ bool leaf(const float4* __restrict__ tris, float k, int32_t ref, int cnt,
                                        const float4* __restrict__ motion) {
     if (cnt <= 0 || k > best_t) return false;
-    return leaf_hits<ANY, MOTION>(tris, r, ref, cnt, best_t, best_gid, bu, bv, found, motion);
+    return leaf_hits<ANY, MOTION, QNODE>(tris, r, ref, cnt, best_t, best_gid, bu, bv, found, motion);
   }
 
   __device__ __forceinline__ void inner(HybridStack<SL>& st, int2& next, float k, int32_t ref, int cnt) const {
@@ -338,6 +432,7 @@ struct Trav4 {
 // tightened t_max would reject them (bvh.rs:146-153).
 template <bool ANY, int SL, bool MOTION = false>
 struct Trav4Q {
+  using NodeT = Node4;
   TRay r;
   float best_t, bu, bv;
   uint32_t best_gid;
@@ -345,7 +440,7 @@ struct Trav4Q {
   int32_t cur;  // next inner node to visit; -1: none
   int32_t tri;  // next triangle slot of the pending leaf; -1: none
 
-  __device__ __forceinline__ void start(const TRay& ray, HybridStack<SL>& st) {
+  __device__ __forceinline__ void start(const TRay& ray, HybridStack<SL>& st, const QGrid* = nullptr) {
     r = ray;
     best_t = ray.t_max;
     bu = bv = 0.0f;

@@ -136,6 +136,73 @@ def test_any_hit_matches_occluded(ctx, oracle):
     assert 0.05 < g.mean() < 0.95
 
 
+def _adversarial_rays(seed, n):
+    """Rays aimed at the quantised slab test's edge cases (n per kind): axis-
+    parallel directions (1/d infinite: an axis without a bound), directions
+    with one component below 1e-6 (1/d huge), origins 10^3..10^4 away (the
+    widest pads), origins on the root box's faces, and short any-hit ranges."""
+    rng = np.random.default_rng(seed)
+    out = []
+    axis = np.eye(3)[rng.integers(0, 3, n)] * rng.choice([-1.0, 1.0], (n, 1))
+    o = rng.uniform(-1.2, 1.2, (n, 3)) - 1.3 * axis
+    out.append(np.c_[o, axis, np.full(n, 1e-4), np.full(n, np.inf), np.zeros(n)])
+    d = axis + rng.uniform(-1e-6, 1e-6, (n, 3)) * (1 - np.abs(axis))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    out.append(np.c_[rng.uniform(-1.0, 1.0, (n, 3)) - 1.3 * axis, d, np.full(n, 1e-4), np.full(n, np.inf),
+                     np.zeros(n)])
+    u = rng.normal(size=(n, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    far = u * rng.uniform(1e3, 1e4, (n, 1))
+    aim = rng.uniform(-0.8, 0.8, (n, 3)) - far
+    aim /= np.linalg.norm(aim, axis=1, keepdims=True)
+    out.append(np.c_[far, aim, np.full(n, 1e-4), np.full(n, np.inf), np.zeros(n)])
+    face = rng.uniform(-1.0, 1.0, (n, 3))
+    k = rng.integers(0, 3, n)
+    face[np.arange(n), k] = rng.choice([-1.0, 1.0], n) * 1.0
+    v = rng.normal(size=(n, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    out.append(np.c_[face, v, np.zeros(n), rng.uniform(0.01, 0.5, n), np.zeros(n)])
+    return np.concatenate(out).astype(np.float32)
+
+
+def test_quantised_boxes_on_edge_case_rays(ctx, oracle):
+    """The default trace kernel tests 16-bit quantised child boxes widened by
+    a per-axis rounding pad; it must accept every box the exact test accepts.
+    On rays chosen for the pad's edge cases its hits equal the exact-box
+    kernel's on every ray and the reference's bit for bit (near-edge rays
+    excluded and counted), closest and any."""
+    print("[quantised boxes] edge-case rays", flush=True)
+    tris = pyre_amd.random_triangles(61, 30000, 0.01)
+    rays = _adversarial_rays(62, 4000)
+    sc, pos, idx = _soup_scene(ctx, tris)
+    sc.build_bvh(bins=32, max_leaf=4)
+    g = sc.intersect(rays)
+    g_occ = sc.occluded(rays)
+    with ctx.tuning(traversal=4):
+        e = sc.intersect(rays)
+        e_occ = sc.occluded(rays)
+    for key in ("inst", "tri", "t", "u", "v"):
+        assert np.array_equal(g[key].view(np.uint32), e[key].view(np.uint32)), key
+    assert np.array_equal(g_occ, e_occ)
+    o_sc = oracle.Scene()
+    o_sc.add_material(oracle.material(0, (0.5, 0.5, 0.5)))
+    o_sc.add_mesh(pos, idx, None, 0, 32, 4)
+    o = o_sc.intersect(rays)
+    excl = near_edge_hits(o_sc, rays, [tris.reshape(-1, 3, 3)])
+    o_hit = o["kind"] == 1
+    keep = ~excl
+    g_hit = g["inst"] != pyre_amd.MISS
+    assert np.array_equal(g_hit[keep], o_hit[keep])
+    both = keep & o_hit
+    assert np.array_equal(g["tri"][both], o["tri"][both])
+    assert np.array_equal(g["t"][both].view(np.uint32), o["t"][both].view(np.uint32))
+    mism = (g_occ.astype(bool) != o_sc.any_hit(rays).astype(bool)) & keep
+    print(f"  hits {int(o_hit.sum())} of {len(rays)}, excluded near an edge {int(excl.sum())}, "
+          f"any-hit mismatches {int(mism.sum())}", flush=True)
+    assert not mism.any()
+    assert o_hit.sum() > len(rays) // 10
+
+
 def test_empty_and_degenerate_inputs(ctx):
     tris = pyre_amd.random_triangles(31, 100, 0.01)
     sc, _, _ = _soup_scene(ctx, tris)
@@ -168,13 +235,14 @@ def test_degenerate_coincident_triangles(ctx, oracle):
     assert h["t"][0] == np.float32(1.0)
 
 
-@pytest.mark.parametrize("switch", [dict(traversal=1), dict(traversal=2), dict(traversal=3), dict(stack_ring=4),
-                                    dict(stack_ring=16), dict(trace_blocks_per_cu=2)])
+@pytest.mark.parametrize("switch", [dict(traversal=1), dict(traversal=2), dict(traversal=3), dict(traversal=4),
+                                    dict(stack_ring=4), dict(stack_ring=16), dict(trace_blocks_per_cu=2)])
 def test_traversal_variants_bit_exact(ctx, oracle, switch):
     """Every selectable traversal form (binary tree with an LDS stack;
-    grid-stride persistent; leaf triangles tested one per step; LDS stack
-    rings of 4 and 16; a small persistent grid) gives the reference's hits
-    bit for bit, closest and any."""
+    grid-stride persistent; leaf triangles tested one per step; the exact
+    float boxes instead of the default's quantised ones; LDS stack rings of 4
+    and 16; a small persistent grid) gives the reference's hits bit for bit,
+    closest and any."""
     print(f"[traversal variant] {switch}", flush=True)
     with ctx.tuning(**switch):
         excluded, hits = _closest_parity(ctx, oracle, 30000, 20000, 32, 4, 41)
