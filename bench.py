@@ -194,7 +194,7 @@ def production_scene(ctx):
     info = scene.build_bvh()
     cam = pre.camera(PROD_W / PROD_H, aperture=0.05)
     log(f"production scene: generated+uploaded in {gen_s:.1f}s, bvh {info['build_ms']:.1f} ms, "
-        f"nodes={info['n_nodes']} depth={info['depth']}")
+        f"nodes={2 * info['n_leaves'] - 1} depth={info['depth']}")
     return pre, scene, cam, info
 
 
@@ -907,7 +907,7 @@ def main():
     for _ in range(2):
         build_ms.append(scene.build_bvh(bins=BINS, max_leaf=MAX_LEAF)["build_ms"])
     info = scene.bvh_info()
-    log(f"bvh nodes={info['n_nodes']} leaves={info['n_leaves']} depth={info['depth']} build_ms={build_ms}")
+    log(f"bvh nodes={2 * info['n_leaves'] - 1} leaves={info['n_leaves']} depth={info['depth']} build_ms={build_ms}")
     order = None
     if args.ray_order == "octant-morton":
         order = coherent_order(rays32[:N_CLOSEST], *scene.bounds())
@@ -978,7 +978,7 @@ def main():
         "anyhit_mrays_s": round((N_RAYS - N_CLOSEST) * args.steps / wall_a / 1e6, 2),
         "anyhit_kernel_ms": round(kern_a, 4),
         "bvh_build_ms": round(min(build_ms), 2),
-        "bvh": {"nodes": info["n_nodes"], "leaves": info["n_leaves"], "depth": info["depth"]},
+        "bvh": {"nodes": 2 * info["n_leaves"] - 1, "leaves": info["n_leaves"], "depth": info["depth"]},
         "device": dev_name,
     }
     if tuning:

@@ -229,6 +229,9 @@ int pyre_scene_bounds(pyre_ctx* ctx, pyre_scene* scene, float out_min[3], float 
  * mesh.rs:77-87). Split decisions follow bvh.rs:180-346; nodes are emitted
  * breadth-first. */
 int pyre_bvh_build(pyre_ctx* ctx, pyre_scene* scene, const pyre_bvh_params* params);
+/* The built tree: n_nodes = build-node slots (the length of
+ * pyre_bvh_export's node arrays; slots the build reserved but did not use
+ * are exported as unused), n_leaves, depth, and the build's wall time. */
 int pyre_bvh_info(pyre_ctx* ctx, pyre_scene* scene, uint32_t* n_nodes, uint32_t* n_leaves, uint32_t* depth,
                   double* build_ms);
 /* Exports the built tree for checking against the reference build:
