@@ -1241,10 +1241,14 @@ void bvh_quantize4(pyre_scene* s, hipStream_t st) {
   for (int a = 0; a < 3; ++a) {
     const float lo = key2f(key[a]), hi = key2f(key[3 + a]);
     const double ext = (double)hi - (double)lo;
-    int e = 0;
-    if (ext > 0.0) e = (int)std::ceil(std::log2(ext / 65534.0));
+    // cells of 2^e with 65534 of them spanning the extent (clamped to the
+    // exponents whose cells are normal floats; a non-finite extent gets the
+    // largest cell, and the outward rounding still covers every box)
+    double le = ext > 0.0 ? std::ceil(std::log2(ext / 65534.0)) : 0.0;
+    if (!(le >= -120.0)) le = -120.0;
+    if (!(le <= 120.0)) le = 120.0;
     g.lo[a] = lo;
-    g.e[a] = std::min(std::max(e, -120), 120);
+    g.e[a] = (int)le;
   }
   s->qgrid = g;
   s->d_nodes4q = dev_alloc<Node4Q>(s->n_nodes4);
