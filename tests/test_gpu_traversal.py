@@ -203,6 +203,58 @@ def test_quantised_boxes_on_edge_case_rays(ctx, oracle):
     assert o_hit.sum() > len(rays) // 10
 
 
+def test_quantised_boxes_on_the_cornell_box(ctx, oracle):
+    """Axis-aligned, coplanar geometry (the Cornell preset: walls whose
+    leaf boxes are flat, blocks touching the floor, two uv spheres; main.rs:
+    353-496) through the trace entry points' quantised boxes: hits equal the
+    reference's bit for bit away from triangle edges, closest and any-hit,
+    and equal the exact-box kernel's on every ray counted here."""
+    print("[quantised boxes] cornell preset", flush=True)
+    pre = pyre_amd.Preset.cornell()
+    sc = pyre_amd.Scene.from_preset(ctx, pre)
+    sc.build_bvh()
+    o_sc = oracle.Scene.cornell()
+    rng = np.random.default_rng(63)
+    n = 20000
+    o = rng.uniform(-0.99, 0.99, (n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    closest = np.c_[o, d, np.full(n, 1e-4), np.full(n, np.inf), np.zeros(n)].astype(np.float32)
+    target = np.c_[rng.uniform(-0.3, 0.3, n), np.full(n, 0.999), rng.uniform(-0.3, 0.3, n)]
+    sd = target - o
+    dist = np.linalg.norm(sd, axis=1)
+    shadow = np.c_[o, sd / dist[:, None], np.full(n, 1e-3), dist - 1e-3, np.zeros(n)].astype(np.float32)
+    rays = np.concatenate([closest, shadow])
+    g = sc.intersect(rays)
+    g_occ = sc.occluded(rays)
+    with ctx.tuning(traversal=4):
+        e = sc.intersect(rays)
+        e_occ = sc.occluded(rays)
+    same = np.ones(len(rays), bool)
+    for key in ("inst", "tri", "t", "u", "v"):
+        same &= g[key].view(np.uint32) == e[key].view(np.uint32)
+    r = o_sc.intersect(rays)
+    tri_arrays = []
+    for p in range(int(r["id"][r["kind"] == 1].max()) + 1):
+        m = o_sc.mesh(p)
+        tri_arrays.append(m["positions"][m["indices"].reshape(-1)].reshape(-1, 3, 3))
+    excl = near_edge_hits(o_sc, rays, tri_arrays)
+    # the trace entry points see triangles only: rays whose reference hit is
+    # the quad light (HitKind::Light, scene.rs:36-45) are left out
+    keep = ~excl & (r["kind"] != 2)
+    r_hit = r["kind"] == 1
+    both = keep & r_hit
+    print(f"  hits {int(r_hit.sum())} of {len(rays)}, excluded near an edge {int(excl.sum())}, light hits "
+          f"{int((r['kind'] == 2).sum())}, differing from the exact boxes {int((~same).sum())}", flush=True)
+    assert np.array_equal((g["inst"] != pyre_amd.MISS)[keep], r_hit[keep])
+    assert np.array_equal(g["inst"][both], r["id"][both])
+    assert np.array_equal(g["tri"][both], r["tri"][both])
+    assert np.array_equal(g["t"][both].view(np.uint32), r["t"][both].view(np.uint32))
+    assert not ((g_occ.astype(bool) != o_sc.any_hit(rays).astype(bool)) & keep).any()
+    assert same[keep].all() and np.array_equal(g_occ[keep], e_occ[keep])
+    assert r_hit.sum() > n // 2
+
+
 def test_empty_and_degenerate_inputs(ctx):
     tris = pyre_amd.random_triangles(31, 100, 0.01)
     sc, _, _ = _soup_scene(ctx, tris)
