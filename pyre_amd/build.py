@@ -31,6 +31,11 @@ COMMON = [
     f"-I{os.path.join(ROOT, 'include')}",
 ]
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+# Per-source device flags. nrc.hip: MFMA results in VGPRs, since its kernels
+# round and ReLU every product, which would otherwise first copy each result
+# out of the accumulation registers (about 160 moves per 64-sample chunk of a
+# training step).
+EXTRA = {"nrc.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 
 
 def _compile(src: str, force: bool) -> str:
@@ -41,7 +46,7 @@ def _compile(src: str, force: bool) -> str:
     if not force and os.path.exists(obj) and os.path.getmtime(obj) >= max(os.path.getmtime(d) for d in deps):
         return obj
     if src.endswith(".hip"):
-        cmd = [HIPCC, *COMMON, *DEVICE, "-x", "hip", "-c", path, "-o", obj]
+        cmd = [HIPCC, *COMMON, *DEVICE, *EXTRA.get(src, []), "-x", "hip", "-c", path, "-o", obj]
     else:
         cmd = [HIPCC, *COMMON, "-c", path, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -142,26 +142,57 @@ __device__ __forceinline__ float encode_feature(int f, const Query& q) {
 }
 
 // B fragment of the encoding for K block kb: features 32kb + 8g + j.
-// Features 2i and 2i + 1 below 72 are sin and cos of one argument: one
-// sincospi per pair.
+// Features 2i and 2i + 1 below 72 are sin and cos of one argument, and the 8
+// features of a lane below 72 are the frequencies k0 .. k0 + 3 of one axis
+// (k0 = 0, 4 or 8): one sincospi at 2^k0 p, then three angle doublings
+// (sin 2x = 2 sin x cos x, cos 2x = 1 - 2 sin^2 x), whose rounding error,
+// below 1e-6, is far under the bf16 step the features are stored with.
 __device__ __forceinline__ bf16x8 encode_frag(const Query& q, int kb, int g) {
   bf16x8 v;
+  const int f0 = 32 * kb + 8 * g;
+  if (f0 < 72) {
+    const int a = f0 / 24, k0 = (f0 % 24) / 2;
+    const float pa = a == 0 ? q.px : (a == 1 ? q.py : q.pz);
+    float s, c;
+    sincospif(ldexpf(pa, k0), &s, &c);
 #pragma unroll
-  for (int j = 0; j < 8; j += 2) {
-    const int f = 32 * kb + 8 * g + j;
-    if (f < 72) {
-      const int a = f / 24, k = (f % 24) / 2;
-      const float pa = a == 0 ? q.px : (a == 1 ? q.py : q.pz);
-      float sv, cv;
-      sincospif(ldexpf(pa, k), &sv, &cv);
-      v[j] = to_bf16(sv);
-      v[j + 1] = to_bf16(cv);
-    } else {
-      v[j] = to_bf16(encode_feature(f, q));
-      v[j + 1] = to_bf16(encode_feature(f + 1, q));
+    for (int j = 0; j < 8; j += 2) {
+      v[j] = to_bf16(s);
+      v[j + 1] = to_bf16(c);
+      const float s2 = s + s;
+      const float sn = s2 * c;
+      c = fmaf(-s2, s, 1.0f);
+      s = sn;
     }
+  } else {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = to_bf16(encode_feature(f0 + j, q));
   }
   return v;
+}
+
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
+
+// ReLU of 4 accumulators rounded to bf16: the maximum with 0 of the rounded
+// values read as signed 16-bit integers (negative values and -0 become +0),
+// two packed integer operations instead of four float ones.
+__device__ __forceinline__ bf16x4 relu_bf16(f32x4 a) {
+  bf16x4 v;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) v[i] = to_bf16(a[i]);
+  return __builtin_bit_cast(bf16x4, __builtin_elementwise_max(__builtin_bit_cast(s16x4, v), (s16x4)0));
+}
+
+// Gradients through a ReLU: bf16(a) where the layer's stored output x is
+// positive, else 0. x comes from relu_bf16, so its bit patterns are +0 or a
+// positive number: min(bits, 1) is the 0/1 mask, applied by a 16-bit multiply.
+__device__ __forceinline__ bf16x4 relu_grad_bf16(f32x4 a, bf16x4 x) {
+  bf16x4 v;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) v[i] = to_bf16(a[i]);
+  const u16x4 keep = __builtin_elementwise_min(__builtin_bit_cast(u16x4, x), (u16x4)1);
+  return __builtin_bit_cast(bf16x4, __builtin_bit_cast(u16x4, v) * keep);
 }
 
 // ------------------------------------------------- weights -> fragments
@@ -259,11 +290,10 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_infer(const bf16x8* __rest
     }
 #pragma unroll
     for (int l = 1; l < kLayers; ++l) {
-      // ReLU(previous) -> LDS tile [query][feature]
+      // ReLU(previous) -> LDS tile [query][feature] (a lane's 4 outputs are
+      // adjacent features of its query: one store)
 #pragma unroll
-      for (int ot = 0; ot < 4; ++ot)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) sact[wave][r][16 * ot + 4 * g + i] = to_bf16(fmaxf(acc[ot][i], 0.0f));
+      for (int ot = 0; ot < 4; ++ot) *(bf16x4*)&sact[wave][r][16 * ot + 4 * g] = relu_bf16(acc[ot]);
       wave_lds_sync();
       bf16x8 bx[2];
 #pragma unroll
@@ -313,33 +343,48 @@ __device__ __forceinline__ uint32_t batch_size(const BatchN& b) {
 // One launch computes a step's loss, its backward pass and its weight
 // gradients: a block of 4 waves walks chunks of 64 samples (the batch's
 // chunks c = blockIdx, blockIdx + P, ...) and keeps everything a chunk needs
-// in LDS -- every layer's input, in both the sample-major layout the forward
-// and backward MFMAs read and the feature-major layout of the weight
-// gradients (whose MFMA K dimension is the samples), and the current output
-// gradient, double-buffered -- so no activation leaves the CU. Each wave
-// accumulates 25 of the 100 16x16 weight-gradient tiles in registers over all
-// of the block's chunks and writes them once as the block's partial (the
-// partial layout of k_train_update, which sums the P partials in a fixed
-// order and takes the Adam step). Layer 0's forward and the loss run one
-// sample tile per wave; the other layers run one 16-feature output tile per
-// wave over the chunk's four sample tiles, so each weight fragment is read
-// from L2 once per chunk per wave.
-constexpr int kFChunk = 64;             // samples per chunk
-constexpr int kFStrideS = kFChunk + 8;  // feature-major rows: 64 samples + pad (bf16)
-constexpr int kFStrideF = 64 + 8;       // sample-major rows: 64 features + pad (bf16)
-constexpr int kFTiles = 25;             // weight-gradient tiles per wave
+// in LDS -- every layer's input and the current output gradient
+// (double-buffered), all sample-major ([sample][feature]) -- so no activation
+// leaves the CU. The forward and backward products read 8 consecutive
+// features of a sample (one 16-byte LDS read); the weight gradients, whose
+// MFMA K dimension is the samples, read 8 consecutive samples of a feature
+// with two transposing LDS reads (ds_read_b64_tr_b16) from the same buffers.
+// Each wave accumulates 25 of the 100 16x16 weight-gradient tiles in
+// registers over all of the block's chunks and writes them once, in the
+// MFMA's own lane order, as the block's partial (k_train_update sums the P
+// partials in a fixed order and takes the Adam step). Layer 0's forward and
+// the loss run one sample tile per wave; the other layers run one 16-feature
+// output tile per wave over the chunk's four sample tiles, so each weight
+// fragment is read from L2 once per chunk per wave.
+constexpr int kFChunk = 64;            // samples per chunk
+constexpr int kFStride0 = 128 + 16;    // encoding rows: 128 features + pad (bf16)
+constexpr int kFStride = 64 + 8;       // hidden rows: 64 features + pad (bf16)
+constexpr int kFTiles = 25;            // weight-gradient tiles per wave
+constexpr int kTileBase[kLayers + 1] = {0, 8, 12, 16, 20, 24, 25};  // a wave's tiles of layer l
 
 struct FusedLds {
-  bf16x8 w0[16 * 64];                 // layer 0's forward weight fragments (16 per lane group)
-  __bf16 xt0[128 * kFStrideS];        // X_0^T (encoding), feature-major
-  __bf16 xs[5][kFChunk * kFStrideF];  // X_1..X_5 (inputs of layers 1..5), sample-major
-  __bf16 xt[5][64 * kFStrideS];       // X_1..X_5, feature-major
-  __bf16 zs[2][kFChunk * kFStrideF];  // current / next output gradient, sample-major
-  __bf16 zt[2][64 * kFStrideS];       // the same, feature-major
+  bf16x8 w0[16 * 64];                  // layer 0's forward weight fragments (16 per lane group)
+  __bf16 x0[kFChunk * kFStride0];      // X_0 (encoding)
+  __bf16 xs[5][kFChunk * kFStride];    // X_1..X_5 (inputs of layers 1..5)
+  __bf16 zs[2][kFChunk * kFStride];    // current / next output gradient
 };
 static_assert(sizeof(FusedLds) <= 160 * 1024, "fused training LDS exceeds the CU's 160 KB");
 
 __device__ __forceinline__ bf16x8 lds8(const __bf16* p) { return *(const bf16x8*)p; }
+
+typedef __attribute__((address_space(3))) bf16x4 lds_bf16x4;
+
+// MFMA operand with K along the rows of a sample-major LDS tile: lane (g, r)
+// receives buf[s0 + 8g + j][f0 + r], j = 0..7. Each ds_read_b64_tr_b16 has
+// lane (g, i) of a 16-lane group read 4 adjacent features of row
+// s0 + 8g + i/4 (+4 for the second read) at column f0 + 4(i%4), and hands lane
+// (g, r) column f0 + r of those 4 rows.
+__device__ __forceinline__ bf16x8 lds_tr8(const __bf16* buf, int stride, int s0, int f0, int g, int r) {
+  const __bf16* p = buf + (s0 + 8 * g + (r >> 2)) * stride + f0 + 4 * (r & 3);
+  const bf16x4 lo = __builtin_amdgcn_ds_read_tr16_b64_v4bf16((lds_bf16x4*)p);
+  const bf16x4 hi = __builtin_amdgcn_ds_read_tr16_b64_v4bf16((lds_bf16x4*)(p + 4 * stride));
+  return __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+}
 
 __global__ __launch_bounds__(256, 1) void k_train_fused(const bf16x8* __restrict__ wf, const bf16x8* __restrict__ wb,
                                                         const float* __restrict__ feats,
@@ -347,9 +392,9 @@ __global__ __launch_bounds__(256, 1) void k_train_fused(const bf16x8* __restrict
                                                         float* __restrict__ partial, float* __restrict__ loss_part,
                                                         uint32_t* step) {
   __shared__ __attribute__((aligned(16))) FusedLds L;
-  // the loss reduction's 4 floats sit in X_0^T's row padding (columns 64..71
-  // of row 0, never read by the products); used after the last chunk
-  float* const red = reinterpret_cast<float*>(&L.xt0[64]);
+  // the loss reduction's 4 floats sit in X_0's row padding (columns
+  // 128..135 of row 0, never read by the products); used after the last chunk
+  float* const red = reinterpret_cast<float*>(&L.x0[128]);
   const uint32_t n_real = batch_size(bn);
   if (step && n_real && blockIdx.x == 0 && threadIdx.x == 0) *step += 1u;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
@@ -388,6 +433,7 @@ __global__ __launch_bounds__(256, 1) void k_train_fused(const bf16x8* __restrict
     // encode + layer 0 (this wave's sample tile w): Z_0^T = W_0^T X_0^T
     {
       const Query qq = query_of(rq, b);
+      const int sl = 16 * w + r;
       f32x4 acc[4];
 #pragma unroll
       for (int ot = 0; ot < 4; ++ot) acc[ot] = f32x4{0, 0, 0, 0};
@@ -399,23 +445,14 @@ __global__ __launch_bounds__(256, 1) void k_train_fused(const bf16x8* __restrict
         for (int ot = 0; ot < 4; ++ot) wa[ot] = L.w0[(ot * 4 + kb) * 64 + lane];
         __builtin_amdgcn_sched_barrier(0);
         const bf16x8 bx = encode_frag(qq, kb, g);
-#pragma unroll
-        for (int j = 0; j < 8; ++j) L.xt0[(32 * kb + 8 * g + j) * kFStrideS + 16 * w + r] = bx[j];
+        *(bf16x8*)&L.x0[sl * kFStride0 + 32 * kb + 8 * g] = bx;
 #pragma unroll
         for (int ot = 0; ot < 4; ++ot) acc[ot] = mfma(wa[ot], bx, acc[ot]);
       }
       // C[4g + i][r] = Z_0^T[16 ot + 4g + i][16 w + r] -> X_1 = ReLU (the
-      // lane's 4 features are adjacent in the sample-major row: one store)
-      const int sl = 16 * w + r;
+      // lane's 4 features are adjacent in the sample's row: one store)
 #pragma unroll
-      for (int ot = 0; ot < 4; ++ot) {
-        bf16x4 v;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) v[i] = to_bf16(fmaxf(acc[ot][i], 0.0f));
-        *(bf16x4*)&L.xs[0][sl * kFStrideF + 16 * ot + 4 * g] = v;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) L.xt[0][(16 * ot + 4 * g + i) * kFStrideS + sl] = v[i];
-      }
+      for (int ot = 0; ot < 4; ++ot) *(bf16x4*)&L.xs[0][sl * kFStride + 16 * ot + 4 * g] = relu_bf16(acc[ot]);
     }
     __syncthreads();
     // layers 1..4: this wave's output tile w over the chunk's 4 sample tiles
@@ -430,7 +467,7 @@ __global__ __launch_bounds__(256, 1) void k_train_fused(const bf16x8* __restrict
 #pragma unroll
       for (int kb = 0; kb < 2; ++kb)
 #pragma unroll
-        for (int st = 0; st < 4; ++st) bx[kb][st] = lds8(&L.xs[l - 1][(16 * st + r) * kFStrideF + 32 * kb + 8 * g]);
+        for (int st = 0; st < 4; ++st) bx[kb][st] = lds8(&L.xs[l - 1][(16 * st + r) * kFStride + 32 * kb + 8 * g]);
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int kb = 0; kb < 2; ++kb) {
@@ -439,15 +476,8 @@ __global__ __launch_bounds__(256, 1) void k_train_fused(const bf16x8* __restrict
         for (int st = 0; st < 4; ++st) acc[st] = mfma(a, bx[kb][st], acc[st]);
       }
 #pragma unroll
-      for (int st = 0; st < 4; ++st) {
-        bf16x4 v;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) v[i] = to_bf16(fmaxf(acc[st][i], 0.0f));
-        const int sl = 16 * st + r;
-        *(bf16x4*)&L.xs[l][sl * kFStrideF + 16 * w + 4 * g] = v;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) L.xt[l][(16 * w + 4 * g + i) * kFStrideS + sl] = v[i];
-      }
+      for (int st = 0; st < 4; ++st)
+        *(bf16x4*)&L.xs[l][(16 * st + r) * kFStride + 16 * w + 4 * g] = relu_bf16(acc[st]);
       __syncthreads();
     }
     // layer 5 (16 outputs, RGB used) for sample tile w, and the relative-L2
@@ -456,7 +486,7 @@ __global__ __launch_bounds__(256, 1) void k_train_fused(const bf16x8* __restrict
       f32x4 p = {0, 0, 0, 0};
 #pragma unroll
       for (int kb = 0; kb < 2; ++kb)
-        p = mfma(wf[(fwd_frag_offset(5) + kb) * 64 + lane], lds8(&L.xs[4][(16 * w + r) * kFStrideF + 32 * kb + 8 * g]),
+        p = mfma(wf[(fwd_frag_offset(5) + kb) * 64 + lane], lds8(&L.xs[4][(16 * w + r) * kFStride + 32 * kb + 8 * g]),
                  p);
       const int sl = 16 * w + r;
       const uint32_t q = s0 + sl;
@@ -474,9 +504,7 @@ __global__ __launch_bounds__(256, 1) void k_train_fused(const bf16x8* __restrict
         }
         v[i] = to_bf16(gval);
       }
-      *(bf16x4*)&L.zs[0][sl * kFStrideF + 4 * g] = v;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) L.zt[0][(4 * g + i) * kFStrideS + sl] = v[i];
+      *(bf16x4*)&L.zs[0][sl * kFStride + 4 * g] = v;
     }
     __syncthreads();
     // backward, layers 5..0: dW_l += X_l^T dZ_l over the chunk, then
@@ -484,37 +512,36 @@ __global__ __launch_bounds__(256, 1) void k_train_fused(const bf16x8* __restrict
 #pragma unroll
     for (int l = kLayers - 1; l >= 0; --l) {
       const int cur = (kLayers - 1 - l) & 1;
-      const __bf16* xt = l == 0 ? L.xt0 : L.xt[l - 1];
+      const __bf16* xl = l == 0 ? L.x0 : L.xs[l - 1];
+      const int xstride = l == 0 ? kFStride0 : kFStride;
       // this wave's tiles of layer l: k in [base, end); layers 0..4: output
       // tile w, input tiles k - base; layer 5: input tile w
-      constexpr int kBase[kLayers + 1] = {0, 8, 12, 16, 20, 24, 25};
-      const int kT = kBase[l + 1] - kBase[l];  // 8, 4 or 1 (a constant once unrolled)
+      const int kT = kTileBase[l + 1] - kTileBase[l];  // 8, 4 or 1 (a constant once unrolled)
       const int ot = l == kLayers - 1 ? 0 : w;
       // weight gradients: this wave's dZ tile (B) and its input tiles (A),
-      // all read before the first product
+      // K = the chunk's samples, all read before the first product
       bf16x8 zb[2], xa[8][2];
 #pragma unroll
-      for (int kb = 0; kb < 2; ++kb) zb[kb] = lds8(&L.zt[cur][(16 * ot + r) * kFStrideS + 32 * kb + 8 * g]);
+      for (int kb = 0; kb < 2; ++kb) zb[kb] = lds_tr8(L.zs[cur], kFStride, 32 * kb, 16 * ot, g, r);
 #pragma unroll
       for (int t = 0; t < 8; ++t) {
         if (t >= kT) break;
         const int it = l == kLayers - 1 ? w : t;
 #pragma unroll
-        for (int kb = 0; kb < 2; ++kb) xa[t][kb] = lds8(&xt[(16 * it + r) * kFStrideS + 32 * kb + 8 * g]);
+        for (int kb = 0; kb < 2; ++kb) xa[t][kb] = lds_tr8(xl, xstride, 32 * kb, 16 * it, g, r);
       }
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int t = 0; t < 8; ++t) {
         if (t >= kT) break;
 #pragma unroll
-        for (int kb = 0; kb < 2; ++kb) dw[kBase[l] + t] = mfma(xa[t][kb], zb[kb], dw[kBase[l] + t]);
+        for (int kb = 0; kb < 2; ++kb) dw[kTileBase[l] + t] = mfma(xa[t][kb], zb[kb], dw[kTileBase[l] + t]);
       }
       if (l >= 1) {
         const int nkb = (kOut[l] + 31) / 32;
         // dZ tiles (B) and the ReLU mask of this lane's outputs
         // (X_l[16 st + r][16 w + 4g .. + 3], one load each), all read before
         // the first product
-        const __bf16* xs = L.xs[l - 1];
         bf16x4 mask[4];
         bf16x8 bz[2][4];
 #pragma unroll
@@ -523,14 +550,14 @@ __global__ __launch_bounds__(256, 1) void k_train_fused(const bf16x8* __restrict
           for (int st = 0; st < 4; ++st) {
             if (kb >= nkb) break;
             if (kOut[l] >= 32 * kb + 8 * g + 8) {
-              bz[kb][st] = lds8(&L.zs[cur][(16 * st + r) * kFStrideF + 32 * kb + 8 * g]);
+              bz[kb][st] = lds8(&L.zs[cur][(16 * st + r) * kFStride + 32 * kb + 8 * g]);
             } else {
 #pragma unroll
               for (int j = 0; j < 8; ++j) bz[kb][st][j] = to_bf16(0.0f);  // K past the layer's 16 outputs
             }
           }
 #pragma unroll
-        for (int st = 0; st < 4; ++st) mask[st] = *(const bf16x4*)&xs[(16 * st + r) * kFStrideF + 16 * w + 4 * g];
+        for (int st = 0; st < 4; ++st) mask[st] = *(const bf16x4*)&xl[(16 * st + r) * kFStride + 16 * w + 4 * g];
         __builtin_amdgcn_sched_barrier(0);
         f32x4 acc[4];
 #pragma unroll
@@ -542,71 +569,77 @@ __global__ __launch_bounds__(256, 1) void k_train_fused(const bf16x8* __restrict
 #pragma unroll
           for (int st = 0; st < 4; ++st) acc[st] = mfma(a, bz[kb][st], acc[st]);
         }
-        // C[4g + i][r] = dX_l^T[16 w + 4g + i][16 st + r]; mask by X_l > 0
+        // C[4g + i][r] = dX_l^T[16 w + 4g + i][16 st + r]; masked by X_l > 0
 #pragma unroll
-        for (int st = 0; st < 4; ++st) {
-          bf16x4 v;
-#pragma unroll
-          for (int i = 0; i < 4; ++i) v[i] = from_bf16(mask[st][i]) > 0.0f ? to_bf16(acc[st][i]) : to_bf16(0.0f);
-          const int sl = 16 * st + r;
-          *(bf16x4*)&L.zs[cur ^ 1][sl * kFStrideF + 16 * w + 4 * g] = v;
-#pragma unroll
-          for (int i = 0; i < 4; ++i) L.zt[cur ^ 1][(16 * w + 4 * g + i) * kFStrideS + sl] = v[i];
-        }
+        for (int st = 0; st < 4; ++st)
+          *(bf16x4*)&L.zs[cur ^ 1][(16 * st + r) * kFStride + 16 * w + 4 * g] = relu_grad_bf16(acc[st], mask[st]);
       }
       __syncthreads();
     }
   }
-  // this block's partial of every weight-gradient tile the wave owns
-  const uint32_t P = gridDim.x;
+  // this block's partial: tile k of wave w is tile 25 w + k, stored as the
+  // MFMA holds it (lane-major, 4 floats a lane: one 1 KB store per tile)
+  f32x4* dst = reinterpret_cast<f32x4*>(partial + (size_t)blockIdx.x * kParams) + (size_t)w * kFTiles * 64 + lane;
 #pragma unroll
-  for (int l = 0; l < kLayers; ++l) {
-    constexpr int kBase[kLayers + 1] = {0, 8, 12, 16, 20, 24, 25};
-    const int in = kIn[l], out = kOut[l];
-    float* dst = partial + (size_t)P * param_offset(l) + (size_t)blockIdx.x * in * out;
-#pragma unroll
-    for (int k = kBase[l]; k < kBase[l + 1]; ++k) {
-      const int it = l == kLayers - 1 ? w : k - kBase[l], ot = l == kLayers - 1 ? 0 : w;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) dst[(size_t)(16 * it + 4 * g + i) * out + 16 * ot + r] = dw[k][i];
-    }
-  }
+  for (int k = 0; k < kFTiles; ++k) dst[k * 64] = dw[k];
   for (int o = 32; o > 0; o >>= 1) loss += __shfl_xor(loss, o);
   if (lane == 0) red[w] = loss;
   __syncthreads();
   if (threadIdx.x == 0) loss_part[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
 }
 
-// Sum of partials wave, wave + 4, ... (p[c * stride]) in a fixed order: 8
-// independent loads in flight, summed pairwise, then the groups in order.
-__device__ __forceinline__ float sum_partials(const float* p, size_t stride, uint32_t P, int wave) {
-  float s = 0.0f;
-  uint32_t c = wave;
-  for (; c + 28 < P; c += 32) {
-    float v[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) v[k] = p[(size_t)(c + 4 * k) * stride];
-    s += ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
-  }
-  for (; c < P; c += 4) s += p[(size_t)c * stride];
-  return s;
+// Parameter index (W_l[ji][o] at param_offset(l) + ji * kOut[l] + o) of
+// position q of a partial: tile t = q / 256 (wave t / 25, its tile k = t % 25),
+// lane (g, r) = (q / 4) % 64, accumulator element i = q % 4, which holds
+// dW_l[16 it + 4g + i][16 ot + r].
+__device__ __forceinline__ int partial_param(int q, int* layer) {
+  const int t = q >> 8, lane = (q >> 2) & 63, i = q & 3;
+  const int wv = t / kFTiles, k = t % kFTiles;
+  int l = 0;
+  while (l + 1 < kLayers && k >= kTileBase[l + 1]) ++l;
+  const int it = l == kLayers - 1 ? wv : k - kTileBase[l], ot = l == kLayers - 1 ? 0 : wv;
+  *layer = l;
+  return param_offset(l) + (16 * it + 4 * (lane >> 4) + i) * kOut[l] + 16 * ot + (lane & 15);
 }
 
-// grad[i] = sum of the P partials: block of 64 parameters (never straddling a
-// layer: every layer size is a multiple of 64), wave w sums partials w, w + 4,
-// ... and the four wave sums are added in wave order (deterministic).
-__global__ __launch_bounds__(256) void k_train_reduce(const float* partial, uint32_t P, float* grad) {
-  __shared__ float red[4][64];
+// grad at partial position q (block of 64 positions, one tile) = the sum of
+// the P partials: wave w of 16 adds partials w, w + 16, ... (16 loads in
+// flight, summed pairwise, then the groups in order) and the 16 wave sums are
+// added in wave order -- a fixed order, so steps are reproducible.
+constexpr int kRedWaves = 16;
+
+__device__ __forceinline__ float sum_partials(const float* p, uint32_t P) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int i = blockIdx.x * 64 + lane;
-  int l = 0;
-  while (l + 1 < kLayers && blockIdx.x * 64 >= param_offset(l + 1)) ++l;
-  const int size = kIn[l] * kOut[l], j = i - param_offset(l);
-  const float* p = partial + (size_t)P * param_offset(l) + j;
-  const float s = sum_partials(p, (size_t)size, P, wave);
+  __shared__ float red[kRedWaves][64];
+  float s = 0.0f;
+  uint32_t c = wave;
+  for (; c + 15 * kRedWaves < P; c += 16 * kRedWaves) {
+    float v[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = p[(size_t)(c + kRedWaves * k) * kParams];
+#pragma unroll
+    for (int h = 8; h > 0; h >>= 1)
+#pragma unroll
+      for (int k = 0; k < h; ++k) v[k] = v[2 * k] + v[2 * k + 1];
+    s += v[0];
+  }
+  for (; c < P; c += kRedWaves) s += p[(size_t)c * kParams];
   red[wave][lane] = s;
   __syncthreads();
-  if (wave == 0) grad[i] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+  float t = 0.0f;
+  if (wave == 0) {
+    t = red[0][lane];
+#pragma unroll
+    for (int k = 1; k < kRedWaves; ++k) t += red[k][lane];
+  }
+  return t;  // valid on wave 0
+}
+
+__global__ __launch_bounds__(64 * kRedWaves) void k_train_reduce(const float* partial, uint32_t P, float* grad) {
+  const int q = blockIdx.x * 64 + (threadIdx.x & 63);
+  const float s = sum_partials(partial + q, P);
+  int l;
+  if (threadIdx.x < 64) grad[partial_param(q, &l)] = s;
 }
 
 // Adam hyper-parameters with a step's bias corrections c1 = 1 - b1^t,
@@ -631,27 +664,21 @@ __device__ __forceinline__ float adam_update(float& m, float& v, float w, float 
 // number t is *step (counted by k_train_fused); an empty batch changes nothing.
 // Block 0 also adds the batch's mean loss to *loss_dev and counts the step
 // in *steps_dev when they are given.
-__global__ __launch_bounds__(256) void k_train_update(const float* partial, uint32_t P, float* grad, float* w,
-                                                      float* m, float* v, AdamArgs a, const uint32_t* step,
-                                                      BatchN bn, bf16x8* wf, bf16x8* wb, const float* loss_part,
-                                                      uint32_t nlp, float* loss_dev, uint32_t* steps_dev) {
-  __shared__ float red[4][64];
+__global__ __launch_bounds__(64 * kRedWaves) void k_train_update(const float* partial, uint32_t P, float* grad,
+                                                                 float* w, float* m, float* v, AdamArgs a,
+                                                                 const uint32_t* step, BatchN bn, bf16x8* wf,
+                                                                 bf16x8* wb, const float* loss_part, uint32_t nlp,
+                                                                 float* loss_dev, uint32_t* steps_dev) {
   const uint32_t n = batch_size(bn);
   if (!n) return;
-  const float t = (float)*step;
-  a.c1 = 1.0f - powf(a.b1, t);
-  a.c2 = 1.0f - powf(a.b2, t);
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int i = blockIdx.x * 64 + lane;
-  int l = 0;
-  while (l + 1 < kLayers && blockIdx.x * 64 >= param_offset(l + 1)) ++l;
-  const int in = kIn[l], out = kOut[l], j = i - param_offset(l);
-  const float* p = partial + (size_t)P * param_offset(l) + j;
-  const float s = sum_partials(p, (size_t)in * out, P, wave);
-  red[wave][lane] = s;
-  __syncthreads();
-  if (wave == 0) {
-    const float gr = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+  const int q = blockIdx.x * 64 + (threadIdx.x & 63);
+  const float gr = sum_partials(partial + q, P);
+  if (threadIdx.x < 64) {
+    const float t = (float)*step;
+    a.c1 = 1.0f - powf(a.b1, t);
+    a.c2 = 1.0f - powf(a.b2, t);
+    int l;
+    const int i = partial_param(q, &l);
     grad[i] = gr;
     float mm = m[i], vv = v[i];
     const float wn = adam_update(mm, vv, w[i], gr, a);
@@ -659,6 +686,7 @@ __global__ __launch_bounds__(256) void k_train_update(const float* partial, uint
     v[i] = vv;
     w[i] = wn;
     const __bf16 hb = to_bf16(wn);
+    const int in = kIn[l], out = kOut[l], j = i - param_offset(l);
     const int ji = j / out, o = j % out;  // W_l[ji][o]
     {
       const int frag = fwd_frag_offset(l) + (o / 16) * (in / 32) + ji / 32;
@@ -674,8 +702,9 @@ __global__ __launch_bounds__(256) void k_train_update(const float* partial, uint
   if (loss_dev && blockIdx.x == 0) {
     __shared__ float lred[256];
     float t = 0.0f;
-    for (uint32_t k = threadIdx.x; k < nlp; k += 256) t += loss_part[k];
-    lred[threadIdx.x] = t;
+    if (threadIdx.x < 256)
+      for (uint32_t k = threadIdx.x; k < nlp; k += 256) t += loss_part[k];
+    if (threadIdx.x < 256) lred[threadIdx.x] = t;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
       if ((int)threadIdx.x < o) lred[threadIdx.x] += lred[threadIdx.x + o];
@@ -782,7 +811,7 @@ float nrc_gradients_dev(pyre_nrc* N, const float* feats, const float* targets, u
                         float* loss_dev) {
   const BatchN bn{nullptr, 0, 0, n};
   const unsigned nlp = train_pass(N, feats, targets, bn, nullptr, st);
-  hipLaunchKernelGGL(k_train_reduce, dim3(kParams / 64), dim3(256), 0, st, N->part, nlp, N->grad);
+  hipLaunchKernelGGL(k_train_reduce, dim3(kParams / 64), dim3(64 * kRedWaves), 0, st, N->part, nlp, N->grad);
   if (loss_dev) {
     hipLaunchKernelGGL(k_sum_into, dim3(1), dim3(256), 0, st, N->loss_part, nlp, 1.0f / (3.0f * n), loss_dev);
     return 0.0f;
@@ -794,7 +823,7 @@ float nrc_train_step_dev(pyre_nrc* N, const float* feats, const float* targets, 
                          float* loss_dev, bool read_back) {
   const BatchN bn{nullptr, 0, 0, n};
   const unsigned nlp = train_pass(N, feats, targets, bn, N->d_step, st);
-  hipLaunchKernelGGL(k_train_update, dim3(kParams / 64), dim3(256), 0, st, N->part, nlp, N->grad, N->w,
+  hipLaunchKernelGGL(k_train_update, dim3(kParams / 64), dim3(64 * kRedWaves), 0, st, N->part, nlp, N->grad, N->w,
                      N->m, N->v, adam_args(N), (const uint32_t*)N->d_step, bn, N->wf, N->wb, N->loss_part, nlp,
                      loss_dev, (uint32_t*)nullptr);
   return loss_dev || !read_back ? 0.0f : read_loss(N, nlp, n, st);
@@ -805,7 +834,7 @@ void nrc_train_step_counted(pyre_nrc* N, const float* feats, const float* target
                             uint32_t* steps_dev) {
   const BatchN bn{count, cap, off, batch};
   const unsigned nlp = train_pass(N, feats + 13 * (size_t)off, targets + 3 * (size_t)off, bn, N->d_step, st);
-  hipLaunchKernelGGL(k_train_update, dim3(kParams / 64), dim3(256), 0, st, N->part, nlp, N->grad, N->w,
+  hipLaunchKernelGGL(k_train_update, dim3(kParams / 64), dim3(64 * kRedWaves), 0, st, N->part, nlp, N->grad, N->w,
                      N->m, N->v, adam_args(N), (const uint32_t*)N->d_step, bn, N->wf, N->wb, N->loss_part, nlp,
                      loss_dev, steps_dev);
 }
