@@ -353,9 +353,9 @@ __device__ __forceinline__ uint32_t batch_size(const BatchN& b) {
 // registers over all of the block's chunks and writes them once, in the
 // MFMA's own lane order, as the block's partial (k_train_update sums the P
 // partials in a fixed order and takes the Adam step). Layer 0's forward and
-// the loss run one sample tile per wave; the other layers run one 16-feature
-// output tile per wave over the chunk's four sample tiles, so each weight
-// fragment is read from L2 once per chunk per wave.
+// the loss run one sample tile per wave; the hidden layers' products run a
+// 2 x 2 block of tiles per wave (two 16-feature tiles of two 16-sample
+// tiles), so each operand read from LDS or L2 feeds two products.
 constexpr int kFChunk = 64;            // samples per chunk
 constexpr int kFStride0 = 128 + 16;    // encoding rows: 128 features + pad (bf16)
 constexpr int kFStride = 64 + 8;       // hidden rows: 64 features + pad (bf16)
@@ -398,6 +398,8 @@ __global__ __launch_bounds__(256, 1) void k_train_fused(const bf16x8* __restrict
   const uint32_t n_real = batch_size(bn);
   if (step && n_real && blockIdx.x == 0 && threadIdx.x == 0) *step += 1u;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+  // this wave's 2 x 2 block of 16 x 16 tiles in the hidden layers' products
+  const int ot0 = 2 * (w & 1), st0 = 2 * (w >> 1);
   const uint32_t nch = (n_real + kFChunk - 1) / kFChunk;
   f32x4 dw[kFTiles];
 #pragma unroll
@@ -455,29 +457,38 @@ __global__ __launch_bounds__(256, 1) void k_train_fused(const bf16x8* __restrict
       for (int ot = 0; ot < 4; ++ot) *(bf16x4*)&L.xs[0][sl * kFStride + 16 * ot + 4 * g] = relu_bf16(acc[ot]);
     }
     __syncthreads();
-    // layers 1..4: this wave's output tile w over the chunk's 4 sample tiles
+    // layers 1..4: this wave's 2 x 2 block of tiles -- output tiles ot0,
+    // ot0 + 1 of sample tiles st0, st0 + 1 -- so each operand read from LDS
+    // feeds two products
 #pragma unroll
     for (int l = 1; l < kLayers - 1; ++l) {
-      f32x4 acc[4];
+      f32x4 acc[2][2];
 #pragma unroll
-      for (int st = 0; st < 4; ++st) acc[st] = f32x4{0, 0, 0, 0};
-      // all 8 operand tiles are read before the first product, so the
-      // products wait for one LDS round trip, not eight
-      bf16x8 bx[2][4];
+      for (int o = 0; o < 2; ++o)
+#pragma unroll
+        for (int s = 0; s < 2; ++s) acc[o][s] = f32x4{0, 0, 0, 0};
+      // all 4 operand tiles are read before the first product, so the
+      // products wait for one LDS round trip, not four
+      bf16x8 bx[2][2];
 #pragma unroll
       for (int kb = 0; kb < 2; ++kb)
 #pragma unroll
-        for (int st = 0; st < 4; ++st) bx[kb][st] = lds8(&L.xs[l - 1][(16 * st + r) * kFStride + 32 * kb + 8 * g]);
+        for (int s = 0; s < 2; ++s)
+          bx[kb][s] = lds8(&L.xs[l - 1][(16 * (st0 + s) + r) * kFStride + 32 * kb + 8 * g]);
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-      for (int kb = 0; kb < 2; ++kb) {
-        const bf16x8 a = wf[(fwd_frag_offset(l) + w * 2 + kb) * 64 + lane];
+      for (int kb = 0; kb < 2; ++kb)
 #pragma unroll
-        for (int st = 0; st < 4; ++st) acc[st] = mfma(a, bx[kb][st], acc[st]);
-      }
+        for (int o = 0; o < 2; ++o) {
+          const bf16x8 a = wf[(fwd_frag_offset(l) + (ot0 + o) * 2 + kb) * 64 + lane];
 #pragma unroll
-      for (int st = 0; st < 4; ++st)
-        *(bf16x4*)&L.xs[l][(16 * st + r) * kFStride + 16 * w + 4 * g] = relu_bf16(acc[st]);
+          for (int s = 0; s < 2; ++s) acc[o][s] = mfma(a, bx[kb][s], acc[o][s]);
+        }
+#pragma unroll
+      for (int o = 0; o < 2; ++o)
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+          *(bf16x4*)&L.xs[l][(16 * (st0 + s) + r) * kFStride + 16 * (ot0 + o) + 4 * g] = relu_bf16(acc[o][s]);
       __syncthreads();
     }
     // layer 5 (16 outputs, RGB used) for sample tile w, and the relative-L2
@@ -514,65 +525,88 @@ __global__ __launch_bounds__(256, 1) void k_train_fused(const bf16x8* __restrict
       const int cur = (kLayers - 1 - l) & 1;
       const __bf16* xl = l == 0 ? L.x0 : L.xs[l - 1];
       const int xstride = l == 0 ? kFStride0 : kFStride;
-      // this wave's tiles of layer l: k in [base, end); layers 0..4: output
-      // tile w, input tiles k - base; layer 5: input tile w
-      const int kT = kTileBase[l + 1] - kTileBase[l];  // 8, 4 or 1 (a constant once unrolled)
-      const int ot = l == kLayers - 1 ? 0 : w;
-      // weight gradients: this wave's dZ tile (B) and its input tiles (A),
-      // K = the chunk's samples, all read before the first product
-      bf16x8 zb[2], xa[8][2];
+      // this wave's weight-gradient tiles of layer l (output tile ot_o, input
+      // tile it_t, held at dw[kTileBase[l] + o * nI + t]): layer 0 output
+      // tiles ot0, ot0 + 1 by input tiles 4 (w / 2) .. + 3; layers 1..4 output
+      // tiles ot0, ot0 + 1 by input tiles st0, st0 + 1; layer 5 input tile w
+      const int nO = l == kLayers - 1 ? 1 : 2, nI = l == 0 ? 4 : (l == kLayers - 1 ? 1 : 2);
+      // weight gradients: dZ tiles (B) and input tiles (A), K = the chunk's
+      // samples, all read before the first product
+      bf16x8 zb[2][2], xa[4][2];
 #pragma unroll
-      for (int kb = 0; kb < 2; ++kb) zb[kb] = lds_tr8(L.zs[cur], kFStride, 32 * kb, 16 * ot, g, r);
+      for (int o = 0; o < 2; ++o) {
+        if (o >= nO) break;
+        const int ot = l == kLayers - 1 ? 0 : ot0 + o;
 #pragma unroll
-      for (int t = 0; t < 8; ++t) {
-        if (t >= kT) break;
-        const int it = l == kLayers - 1 ? w : t;
+        for (int kb = 0; kb < 2; ++kb) zb[o][kb] = lds_tr8(L.zs[cur], kFStride, 32 * kb, 16 * ot, g, r);
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        if (t >= nI) break;
+        const int it = l == 0 ? 4 * (w >> 1) + t : (l == kLayers - 1 ? w : st0 + t);
 #pragma unroll
         for (int kb = 0; kb < 2; ++kb) xa[t][kb] = lds_tr8(xl, xstride, 32 * kb, 16 * it, g, r);
       }
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-      for (int t = 0; t < 8; ++t) {
-        if (t >= kT) break;
+      for (int o = 0; o < 2; ++o) {
+        if (o >= nO) break;
 #pragma unroll
-        for (int kb = 0; kb < 2; ++kb) dw[kTileBase[l] + t] = mfma(xa[t][kb], zb[kb], dw[kTileBase[l] + t]);
+        for (int t = 0; t < 4; ++t) {
+          if (t >= nI) break;
+          f32x4& d = dw[kTileBase[l] + o * nI + t];
+#pragma unroll
+          for (int kb = 0; kb < 2; ++kb) d = mfma(xa[t][kb], zb[o][kb], d);
+        }
       }
       if (l >= 1) {
+        // dX_l^T = W_l dZ_l^T for this wave's 2 x 2 block (input tiles ot0,
+        // ot0 + 1 of sample tiles st0, st0 + 1), masked by X_l > 0: the dZ
+        // tiles (B) and the ReLU masks (X_l[16 s + r][16 i + 4g .. + 3], one
+        // load each) are read before the first product
         const int nkb = (kOut[l] + 31) / 32;
-        // dZ tiles (B) and the ReLU mask of this lane's outputs
-        // (X_l[16 st + r][16 w + 4g .. + 3], one load each), all read before
-        // the first product
-        bf16x4 mask[4];
-        bf16x8 bz[2][4];
+        bf16x4 mask[2][2];
+        bf16x8 bz[2][2];
 #pragma unroll
         for (int kb = 0; kb < 2; ++kb)
 #pragma unroll
-          for (int st = 0; st < 4; ++st) {
+          for (int s = 0; s < 2; ++s) {
             if (kb >= nkb) break;
             if (kOut[l] >= 32 * kb + 8 * g + 8) {
-              bz[kb][st] = lds8(&L.zs[cur][(16 * st + r) * kFStride + 32 * kb + 8 * g]);
+              bz[kb][s] = lds8(&L.zs[cur][(16 * (st0 + s) + r) * kFStride + 32 * kb + 8 * g]);
             } else {
 #pragma unroll
-              for (int j = 0; j < 8; ++j) bz[kb][st][j] = to_bf16(0.0f);  // K past the layer's 16 outputs
+              for (int j = 0; j < 8; ++j) bz[kb][s][j] = to_bf16(0.0f);  // K past the layer's 16 outputs
             }
           }
 #pragma unroll
-        for (int st = 0; st < 4; ++st) mask[st] = *(const bf16x4*)&xl[(16 * st + r) * kFStride + 16 * w + 4 * g];
-        __builtin_amdgcn_sched_barrier(0);
-        f32x4 acc[4];
+        for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int st = 0; st < 4; ++st) acc[st] = f32x4{0, 0, 0, 0};
+          for (int s = 0; s < 2; ++s)
+            mask[i][s] = *(const bf16x4*)&xl[(16 * (st0 + s) + r) * kFStride + 16 * (ot0 + i) + 4 * g];
+        __builtin_amdgcn_sched_barrier(0);
+        f32x4 acc[2][2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int s = 0; s < 2; ++s) acc[i][s] = f32x4{0, 0, 0, 0};
 #pragma unroll
         for (int kb = 0; kb < 2; ++kb) {
           if (kb >= nkb) break;
-          const bf16x8 a = wb[(bwd_frag_offset(l) + w * nkb + kb) * 64 + lane];
 #pragma unroll
-          for (int st = 0; st < 4; ++st) acc[st] = mfma(a, bz[kb][st], acc[st]);
+          for (int i = 0; i < 2; ++i) {
+            const bf16x8 a = wb[(bwd_frag_offset(l) + (ot0 + i) * nkb + kb) * 64 + lane];
+#pragma unroll
+            for (int s = 0; s < 2; ++s) acc[i][s] = mfma(a, bz[kb][s], acc[i][s]);
+          }
         }
-        // C[4g + i][r] = dX_l^T[16 w + 4g + i][16 st + r]; masked by X_l > 0
+        // C[4g + j][r] = dX_l^T[16 i + 4g + j][16 s + r]
 #pragma unroll
-        for (int st = 0; st < 4; ++st)
-          *(bf16x4*)&L.zs[cur ^ 1][(16 * st + r) * kFStride + 16 * w + 4 * g] = relu_grad_bf16(acc[st], mask[st]);
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int s = 0; s < 2; ++s)
+            *(bf16x4*)&L.zs[cur ^ 1][(16 * (st0 + s) + r) * kFStride + 16 * (ot0 + i) + 4 * g] =
+                relu_grad_bf16(acc[i][s], mask[i][s]);
       }
       __syncthreads();
     }
@@ -591,13 +625,17 @@ __global__ __launch_bounds__(256, 1) void k_train_fused(const bf16x8* __restrict
 // Parameter index (W_l[ji][o] at param_offset(l) + ji * kOut[l] + o) of
 // position q of a partial: tile t = q / 256 (wave t / 25, its tile k = t % 25),
 // lane (g, r) = (q / 4) % 64, accumulator element i = q % 4, which holds
-// dW_l[16 it + 4g + i][16 ot + r].
+// dW_l[16 it + 4g + i][16 ot + r] for the wave's tile assignment in
+// k_train_fused.
 __device__ __forceinline__ int partial_param(int q, int* layer) {
   const int t = q >> 8, lane = (q >> 2) & 63, i = q & 3;
   const int wv = t / kFTiles, k = t % kFTiles;
   int l = 0;
   while (l + 1 < kLayers && k >= kTileBase[l + 1]) ++l;
-  const int it = l == kLayers - 1 ? wv : k - kTileBase[l], ot = l == kLayers - 1 ? 0 : wv;
+  const int nI = l == 0 ? 4 : (l == kLayers - 1 ? 1 : 2);
+  const int o = (k - kTileBase[l]) / nI, tt = (k - kTileBase[l]) % nI;
+  const int ot = l == kLayers - 1 ? 0 : 2 * (wv & 1) + o;
+  const int it = l == 0 ? 4 * (wv >> 1) + tt : (l == kLayers - 1 ? wv : 2 * (wv >> 1) + tt);
   *layer = l;
   return param_offset(l) + (16 * it + 4 * (lane >> 4) + i) * kOut[l] + 16 * ot + (lane & 15);
 }
