@@ -28,6 +28,7 @@ namespace {
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+typedef __attribute__((__vector_size__(8 * sizeof(float)))) float f32x8;
 
 constexpr int kLayers = 6;
 constexpr int kIn[kLayers] = {128, 64, 64, 64, 64, 64};
@@ -148,7 +149,7 @@ __device__ __forceinline__ float encode_feature(int f, const Query& q) {
 // (sin 2x = 2 sin x cos x, cos 2x = 1 - 2 sin^2 x), whose rounding error,
 // below 1e-6, is far under the bf16 step the features are stored with.
 __device__ __forceinline__ bf16x8 encode_frag(const Query& q, int kb, int g) {
-  bf16x8 v;
+  f32x8 v;
   const int f0 = 32 * kb + 8 * g;
   if (f0 < 72) {
     const int a = f0 / 24, k0 = (f0 % 24) / 2;
@@ -157,8 +158,8 @@ __device__ __forceinline__ bf16x8 encode_frag(const Query& q, int kb, int g) {
     sincospif(ldexpf(pa, k0), &s, &c);
 #pragma unroll
     for (int j = 0; j < 8; j += 2) {
-      v[j] = to_bf16(s);
-      v[j + 1] = to_bf16(c);
+      v[j] = s;
+      v[j + 1] = c;
       const float s2 = s + s;
       const float sn = s2 * c;
       c = fmaf(-s2, s, 1.0f);
@@ -166,33 +167,52 @@ __device__ __forceinline__ bf16x8 encode_frag(const Query& q, int kb, int g) {
     }
   } else {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) v[j] = to_bf16(encode_feature(f0 + j, q));
+    for (int j = 0; j < 8; ++j) v[j] = encode_feature(f0 + j, q);
   }
-  return v;
+  return __builtin_convertvector(v, bf16x8);  // round to nearest even, two values per instruction
 }
 
-typedef short s16x4 __attribute__((ext_vector_type(4)));
-typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((__vector_size__(2 * sizeof(__bf16)))) __bf16 bf16x2;
+typedef __attribute__((__vector_size__(2 * sizeof(float)))) float f32x2;
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+// Two floats rounded to bf16 (nearest even) in one v_cvt_pk_bf16_f32, lo in
+// bits 0..15 (converting pairs: from wider vectors the compiler rounds one
+// value per instruction and merges the halves).
+__device__ __forceinline__ unsigned cvt_pk_bf16(float lo, float hi) {
+  return __builtin_bit_cast(unsigned, __builtin_convertvector((f32x2){lo, hi}, bf16x2));
+}
 
 // ReLU of 4 accumulators rounded to bf16: the maximum with 0 of the rounded
 // values read as signed 16-bit integers (negative values and -0 become +0),
-// two packed integer operations instead of four float ones.
+// one packed integer instruction per pair.
 __device__ __forceinline__ bf16x4 relu_bf16(f32x4 a) {
-  bf16x4 v;
+  unsigned out[2];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) v[i] = to_bf16(a[i]);
-  return __builtin_bit_cast(bf16x4, __builtin_elementwise_max(__builtin_bit_cast(s16x4, v), (s16x4)0));
+  for (int h = 0; h < 2; ++h)
+    out[h] = __builtin_bit_cast(
+        unsigned, __builtin_elementwise_max(__builtin_bit_cast(s16x2, cvt_pk_bf16(a[2 * h], a[2 * h + 1])), (s16x2)0));
+  return __builtin_bit_cast(bf16x4, out);
 }
 
 // Gradients through a ReLU: bf16(a) where the layer's stored output x is
 // positive, else 0. x comes from relu_bf16, so its bit patterns are +0 or a
-// positive number: min(bits, 1) is the 0/1 mask, applied by a 16-bit multiply.
+// positive number: min(bits, 1) per 16-bit half is the 0/1 mask, applied by a
+// 16-bit multiply -- two packed instructions per pair of values. (The 1 goes
+// through an empty asm statement, which emits nothing: a multiply by a value
+// known to be 0 or 1 would be turned into a compare and a select per value.)
 __device__ __forceinline__ bf16x4 relu_grad_bf16(f32x4 a, bf16x4 x) {
-  bf16x4 v;
+  unsigned one = 0x00010001u;
+  asm("" : "+v"(one));
+  const unsigned* xb = reinterpret_cast<const unsigned*>(&x);
+  unsigned out[2];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) v[i] = to_bf16(a[i]);
-  const u16x4 keep = __builtin_elementwise_min(__builtin_bit_cast(u16x4, x), (u16x4)1);
-  return __builtin_bit_cast(bf16x4, __builtin_bit_cast(u16x4, v) * keep);
+  for (int h = 0; h < 2; ++h) {
+    const u16x2 k = __builtin_elementwise_min(__builtin_bit_cast(u16x2, xb[h]), __builtin_bit_cast(u16x2, one));
+    out[h] = __builtin_bit_cast(unsigned, __builtin_bit_cast(u16x2, cvt_pk_bf16(a[2 * h], a[2 * h + 1])) * k);
+  }
+  return __builtin_bit_cast(bf16x4, out);
 }
 
 // ------------------------------------------------- weights -> fragments
@@ -432,6 +452,32 @@ __global__ __launch_bounds__(256, 1) void k_train_fused(const bf16x8* __restrict
 #pragma unroll
     for (int i = 0; i < 3; ++i) tg[i] = tg_next[i];
     fetch(c + gridDim.x, rq_next, tg_next);
+    // this wave's weight fragments for the next product phase are loaded one
+    // phase ahead, so their L2 latency overlaps the current phase. The
+    // fragment pointers go through an empty asm statement once per chunk,
+    // which keeps the loads where they are written (otherwise they are
+    // hoisted out of the loop into registers the products need, or sunk to
+    // their use).
+    typedef const __attribute__((address_space(1))) bf16x8* gptr;
+    gptr wfc = (gptr)wf;
+    gptr wbc = (gptr)wb;
+    asm("" : "+s"(wfc), "+s"(wbc));
+    auto fwd_frags = [&](int l, bf16x8(&a)[4]) {  // a[o * 2 + kb]: output tile ot0 + o, K block kb
+#pragma unroll
+      for (int o = 0; o < 2; ++o)
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb) a[o * 2 + kb] = wfc[(fwd_frag_offset(l) + (ot0 + o) * 2 + kb) * 64 + lane];
+    };
+    auto bwd_frags = [&](int l, bf16x8(&a)[4]) {  // a[i * nkb + kb]: input tile ot0 + i, K block kb
+      const int nkb = (kOut[l] + 31) / 32;
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb)
+          if (kb < nkb) a[i * nkb + kb] = wbc[(bwd_frag_offset(l) + (ot0 + i) * nkb + kb) * 64 + lane];
+    };
+    bf16x8 wn[4];
+    fwd_frags(1, wn);
     // encode + layer 0 (this wave's sample tile w): Z_0^T = W_0^T X_0^T
     {
       const Query qq = query_of(rq, b);
@@ -475,15 +521,22 @@ __global__ __launch_bounds__(256, 1) void k_train_fused(const bf16x8* __restrict
 #pragma unroll
         for (int s = 0; s < 2; ++s)
           bx[kb][s] = lds8(&L.xs[l - 1][(16 * (st0 + s) + r) * kFStride + 32 * kb + 8 * g]);
+      bf16x8 wc[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) wc[k] = wn[k];
+      if (l + 1 < kLayers - 1) {
+        fwd_frags(l + 1, wn);
+      } else {  // layer 5's two fragments (one 16-output tile)
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb) wn[kb] = wfc[(fwd_frag_offset(5) + kb) * 64 + lane];
+      }
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int kb = 0; kb < 2; ++kb)
 #pragma unroll
-        for (int o = 0; o < 2; ++o) {
-          const bf16x8 a = wf[(fwd_frag_offset(l) + (ot0 + o) * 2 + kb) * 64 + lane];
+        for (int o = 0; o < 2; ++o)
 #pragma unroll
-          for (int s = 0; s < 2; ++s) acc[o][s] = mfma(a, bx[kb][s], acc[o][s]);
-        }
+          for (int s = 0; s < 2; ++s) acc[o][s] = mfma(wc[o * 2 + kb], bx[kb][s], acc[o][s]);
 #pragma unroll
       for (int o = 0; o < 2; ++o)
 #pragma unroll
@@ -495,10 +548,13 @@ __global__ __launch_bounds__(256, 1) void k_train_fused(const bf16x8* __restrict
     // loss gradient (p^2 held constant): dZ_5, zero past RGB and for padding
     {
       f32x4 p = {0, 0, 0, 0};
+      bf16x8 x5[2];
 #pragma unroll
-      for (int kb = 0; kb < 2; ++kb)
-        p = mfma(wf[(fwd_frag_offset(5) + kb) * 64 + lane], lds8(&L.xs[4][(16 * w + r) * kFStride + 32 * kb + 8 * g]),
-                 p);
+      for (int kb = 0; kb < 2; ++kb) x5[kb] = lds8(&L.xs[4][(16 * w + r) * kFStride + 32 * kb + 8 * g]);
+      const bf16x8 w5a = wn[0], w5b = wn[1];
+      bwd_frags(5, wn);
+      p = mfma(w5a, x5[0], p);
+      p = mfma(w5b, x5[1], p);
       const int sl = 16 * w + r;
       const uint32_t q = s0 + sl;
       bf16x4 v;
@@ -530,6 +586,10 @@ __global__ __launch_bounds__(256, 1) void k_train_fused(const bf16x8* __restrict
       // tiles ot0, ot0 + 1 by input tiles 4 (w / 2) .. + 3; layers 1..4 output
       // tiles ot0, ot0 + 1 by input tiles st0, st0 + 1; layer 5 input tile w
       const int nO = l == kLayers - 1 ? 1 : 2, nI = l == 0 ? 4 : (l == kLayers - 1 ? 1 : 2);
+      bf16x8 wc[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) wc[k] = wn[k];
+      if (l >= 2) bwd_frags(l - 1, wn);
       // weight gradients: dZ tiles (B) and input tiles (A), K = the chunk's
       // samples, all read before the first product
       bf16x8 zb[2][2], xa[4][2];
@@ -594,11 +654,9 @@ __global__ __launch_bounds__(256, 1) void k_train_fused(const bf16x8* __restrict
         for (int kb = 0; kb < 2; ++kb) {
           if (kb >= nkb) break;
 #pragma unroll
-          for (int i = 0; i < 2; ++i) {
-            const bf16x8 a = wb[(bwd_frag_offset(l) + (ot0 + i) * nkb + kb) * 64 + lane];
+          for (int i = 0; i < 2; ++i)
 #pragma unroll
-            for (int s = 0; s < 2; ++s) acc[i][s] = mfma(a, bz[kb][s], acc[i][s]);
-          }
+            for (int s = 0; s < 2; ++s) acc[i][s] = mfma(wc[i * nkb + kb], bz[kb][s], acc[i][s]);
         }
         // C[4g + j][r] = dX_l^T[16 i + 4g + j][16 s + r]
 #pragma unroll
