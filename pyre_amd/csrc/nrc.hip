@@ -409,7 +409,7 @@ __device__ __forceinline__ bf16x8 lds_tr8(const __bf16* buf, int stride, int s0,
 __global__ __launch_bounds__(256, 1) void k_train_fused(const bf16x8* __restrict__ wf, const bf16x8* __restrict__ wb,
                                                         const float* __restrict__ feats,
                                                         const float* __restrict__ targets, BatchN bn, NrcBounds b,
-                                                        float* __restrict__ partial, float* __restrict__ loss_part,
+                                                        __bf16* __restrict__ partial, float* __restrict__ loss_part,
                                                         uint32_t* step) {
   __shared__ __attribute__((aligned(16))) FusedLds L;
   // the loss reduction's 4 floats sit in X_0's row padding (columns
@@ -670,10 +670,15 @@ __global__ __launch_bounds__(256, 1) void k_train_fused(const bf16x8* __restrict
     }
   }
   // this block's partial: tile k of wave w is tile 25 w + k, stored as the
-  // MFMA holds it (lane-major, 4 floats a lane: one 1 KB store per tile)
-  f32x4* dst = reinterpret_cast<f32x4*>(partial + (size_t)blockIdx.x * kParams) + (size_t)w * kFTiles * 64 + lane;
+  // MFMA holds it (lane-major, 4 values a lane: one 512-byte store per tile)
+  // and rounded to bf16 like the products' operands (half the bytes of the
+  // partials' round trip through memory, the largest fixed cost of a step)
+  bf16x4* dst = reinterpret_cast<bf16x4*>(partial + (size_t)blockIdx.x * kParams) + (size_t)w * kFTiles * 64 + lane;
 #pragma unroll
-  for (int k = 0; k < kFTiles; ++k) dst[k * 64] = dw[k];
+  for (int k = 0; k < kFTiles; ++k) {
+    const unsigned pk[2] = {cvt_pk_bf16(dw[k][0], dw[k][1]), cvt_pk_bf16(dw[k][2], dw[k][3])};
+    dst[k * 64] = __builtin_bit_cast(bf16x4, pk);
+  }
   for (int o = 32; o > 0; o >>= 1) loss += __shfl_xor(loss, o);
   if (lane == 0) red[w] = loss;
   __syncthreads();
@@ -698,13 +703,13 @@ __device__ __forceinline__ int partial_param(int q, int* layer) {
   return param_offset(l) + (16 * it + 4 * (lane >> 4) + i) * kOut[l] + 16 * ot + (lane & 15);
 }
 
-// grad at partial position q (block of 64 positions, one tile) = the sum of
-// the P partials: wave w of 16 adds partials w, w + 16, ... (16 loads in
-// flight, summed pairwise, then the groups in order) and the 16 wave sums are
-// added in wave order -- a fixed order, so steps are reproducible.
+// grad at partial position q (block of 64 positions, one tile) = the FP32 sum
+// of the P bf16 partials: wave w of 16 adds partials w, w + 16, ... (16 loads
+// in flight, summed pairwise, then the groups in order) and the 16 wave sums
+// are added in wave order -- a fixed order, so steps are reproducible.
 constexpr int kRedWaves = 16;
 
-__device__ __forceinline__ float sum_partials(const float* p, uint32_t P) {
+__device__ __forceinline__ float sum_partials(const __bf16* p, uint32_t P) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   __shared__ float red[kRedWaves][64];
   float s = 0.0f;
@@ -712,14 +717,14 @@ __device__ __forceinline__ float sum_partials(const float* p, uint32_t P) {
   for (; c + 15 * kRedWaves < P; c += 16 * kRedWaves) {
     float v[16];
 #pragma unroll
-    for (int k = 0; k < 16; ++k) v[k] = p[(size_t)(c + kRedWaves * k) * kParams];
+    for (int k = 0; k < 16; ++k) v[k] = from_bf16(p[(size_t)(c + kRedWaves * k) * kParams]);
 #pragma unroll
     for (int h = 8; h > 0; h >>= 1)
 #pragma unroll
       for (int k = 0; k < h; ++k) v[k] = v[2 * k] + v[2 * k + 1];
     s += v[0];
   }
-  for (; c < P; c += kRedWaves) s += p[(size_t)c * kParams];
+  for (; c < P; c += kRedWaves) s += from_bf16(p[(size_t)c * kParams]);
   red[wave][lane] = s;
   __syncthreads();
   float t = 0.0f;
@@ -731,7 +736,7 @@ __device__ __forceinline__ float sum_partials(const float* p, uint32_t P) {
   return t;  // valid on wave 0
 }
 
-__global__ __launch_bounds__(64 * kRedWaves) void k_train_reduce(const float* partial, uint32_t P, float* grad) {
+__global__ __launch_bounds__(64 * kRedWaves) void k_train_reduce(const __bf16* partial, uint32_t P, float* grad) {
   const int q = blockIdx.x * 64 + (threadIdx.x & 63);
   const float s = sum_partials(partial + q, P);
   int l;
@@ -760,7 +765,7 @@ __device__ __forceinline__ float adam_update(float& m, float& v, float w, float 
 // number t is *step (counted by k_train_fused); an empty batch changes nothing.
 // Block 0 also adds the batch's mean loss to *loss_dev and counts the step
 // in *steps_dev when they are given.
-__global__ __launch_bounds__(64 * kRedWaves) void k_train_update(const float* partial, uint32_t P, float* grad,
+__global__ __launch_bounds__(64 * kRedWaves) void k_train_update(const __bf16* partial, uint32_t P, float* grad,
                                                                  float* w, float* m, float* v, AdamArgs a,
                                                                  const uint32_t* step, BatchN bn, bf16x8* wf,
                                                                  bf16x8* wb, const float* loss_part, uint32_t nlp,
@@ -815,7 +820,7 @@ inline unsigned blocks(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t
 
 // Blocks of k_train_fused for a batch of at most n samples: one per CU
 // (its LDS fills one), no more than the batch's 64-sample chunks. Each
-// writes one weight-gradient partial (100 KB).
+// writes one weight-gradient partial (50 KB).
 inline unsigned fused_grid(int cus, uint32_t n) {
   return std::max(1u, std::min(blocks(n, kFChunk), (unsigned)std::max(1, cus)));
 }
@@ -837,7 +842,7 @@ struct pyre_nrc {
   // training scratch: weight-gradient partials and loss sums (one per CU),
   // and the host entry points' staging of feats / targets (the largest batch)
   uint32_t cap = 0;
-  float* part = nullptr;
+  __bf16* part = nullptr;
   float* loss_part = nullptr;
   float* feats = nullptr;
   float* targets = nullptr;
@@ -1010,7 +1015,7 @@ int pyre_nrc_create(pyre_ctx* ctx, const pyre_nrc_params* p, pyre_nrc** out) {
     N->wb = dev_alloc<bf16x8>((size_t)kBwdFrags * 64);
     N->d_step = dev_alloc<uint32_t>(1);
     // weight-gradient partials and loss sums of k_train_fused: one per CU
-    N->part = dev_alloc<float>((size_t)std::max(1, ctx->cus) * kParams);
+    N->part = dev_alloc<__bf16>((size_t)std::max(1, ctx->cus) * kParams);
     N->loss_part = dev_alloc<float>((size_t)std::max(1, ctx->cus));
     PYRE_HIP_CHECK(hipMemcpyAsync(N->w, w.data(), kParams * 4, hipMemcpyHostToDevice, st));
     PYRE_HIP_CHECK(hipMemsetAsync(N->m, 0, kParams * 4, st));
