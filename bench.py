@@ -428,8 +428,10 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16
 
 def cache_mfma(ctx, lo, hi, args):
     """Inference and training throughput of the cache kernels on device
-    arrays (synthetic queries in the scene box), timed with HIP events on
-    the stream the kernels run on; TFLOP/s from the algorithmic FLOPs."""
+    arrays (synthetic queries in the scene box): k calls queued back to back
+    on one stream between two HIP events, as the renderer queues its training
+    steps (the host enqueues a call in about 12 us, less than any call's GPU
+    time, so the queue never runs dry); TFLOP/s from the algorithmic FLOPs."""
     import torch
     import pyre_amd
     rng = np.random.default_rng(8)
@@ -450,14 +452,14 @@ def cache_mfma(ctx, lo, hi, args):
     def timed(fn, k=20, warm=3):
         for _ in range(warm):
             fn()
-        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(k)]
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
-        for a, b in evs:
-            a.record(stream)
+        a.record(stream)
+        for _ in range(k):
             fn()
-            b.record(stream)
+        b.record(stream)
         torch.cuda.synchronize()
-        return float(np.mean([a.elapsed_time(b) for a, b in evs]))
+        return a.elapsed_time(b) / k
 
     n = 1 << 20
     with torch.cuda.stream(stream):

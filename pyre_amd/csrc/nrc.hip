@@ -421,6 +421,7 @@ __global__ __launch_bounds__(256, 1) void k_train_fused(const bf16x8* __restrict
   // this wave's 2 x 2 block of 16 x 16 tiles in the hidden layers' products
   const int ot0 = 2 * (w & 1), st0 = 2 * (w >> 1);
   const uint32_t nch = (n_real + kFChunk - 1) / kFChunk;
+  const float inv_3n = 1.0f / (3.0f * (float)n_real);  // the loss is the mean over 3 n values
   f32x4 dw[kFTiles];
 #pragma unroll
   for (int k = 0; k < kFTiles; ++k) dw[k] = f32x4{0, 0, 0, 0};
@@ -564,10 +565,10 @@ __global__ __launch_bounds__(256, 1) void k_train_fused(const bf16x8* __restrict
         float gval = 0.0f;
         if (ch < 3 && q < n_real) {
           const float pv = p[i], t = tg[i];  // ch == i on the lanes of group 0
-          const float denom = pv * pv + 0.01f;
+          const float inv = 1.0f / (pv * pv + 0.01f);  // one division per value
           const float d = pv - t;
-          loss += d * d / denom;
-          gval = 2.0f * d / denom / (3.0f * (float)n_real);
+          loss += d * d * inv;
+          gval = 2.0f * d * inv * inv_3n;
         }
         v[i] = to_bf16(gval);
       }
