@@ -144,8 +144,9 @@ struct Paths {
   uint32_t* q_cur;
   uint32_t* q_next;
   uint32_t* q_shadow;
+  uint32_t* q_hit;     // paths of the current queue whose ray hit a surface (k_shade_split -> k_shade)
   uint32_t* counters;  // [0] current, [1] next, [2] shadow, [3] cache queries, [4] training samples,
-                       // [8] extend work, [16] shadow work
+                       // [5] surface hits, [8] extend work, [16] shadow work
   unsigned long long* stats;  // [0] extension rays, [1] shadow rays, [2] cache queries, [3] training samples
   uint32_t* pjob;    // job of each path
   float4* vrec;      // training paths: kVrec float4 per (training path, vertex)
@@ -221,8 +222,8 @@ __global__ void k_counters(uint32_t* c, Sched sc) {
   if (i < kCounters) c[i] = i == 0 ? cur_pass(sc).paths : 0u;
 }
 
-// next queue becomes current; next and shadow queues and the work counters
-// are emptied
+// next queue becomes current; next, shadow and surface-hit queues and the
+// work counters are emptied
 __global__ void k_rotate(uint32_t* c, unsigned long long* stats) {
   const uint32_t i = threadIdx.x;
   const uint32_t cur = c[0], next = c[1];
@@ -230,7 +231,7 @@ __global__ void k_rotate(uint32_t* c, unsigned long long* stats) {
   if (i == 0) {
     stats[0] += cur;
     c[0] = next;
-  } else if (i == 1 || i == 2 || (i >= 8 && i < kCounters)) {
+  } else if (i == 1 || i == 2 || i == 5 || (i >= 8 && i < kCounters)) {
     c[i] = 0u;
   }
 }
@@ -467,8 +468,17 @@ struct ShadeOut {
   uint32_t n_shadow;
 };
 
-// One bounce of integrator.rs:76-222 for path p.
-__device__ void shade_path(const Paths& P, const Sched& sc, const JobDev& J, const PassDev& pp, uint32_t depth,
+// One bounce of integrator.rs:76-222 for path p: whole (PART = kShadeAll),
+// or in two halves. kShadeSplit (k_shade_split): the ray's nearest hit among
+// the BVH's and the quad lights'; an escaped ray or a light hit ends the path
+// here (and the call returns false); a surface hit returns true, writing
+// nothing, to be shaded by the other half. kShadeHits (k_shade over the
+// surface-hit queue): path p's ray hit a surface and no light is nearer, so
+// the light loop is not run again; the surface's shading, light sampling and
+// BSDF sampling follow.
+constexpr int kShadeAll = 0, kShadeSplit = 1, kShadeHits = 2;
+template <int PART>
+__device__ bool shade_path(const Paths& P, const Sched& sc, const JobDev& J, const PassDev& pp, uint32_t depth,
                            uint32_t p, ShadeOut& o) {
   const SceneDev& S = J.S;
   float4 ra = P.ray[2 * (size_t)p], rb = P.ray[2 * (size_t)p + 1];
@@ -491,18 +501,21 @@ __device__ void shade_path(const Paths& P, const Sched& sc, const JobDev& J, con
   float t_max = kind ? hv.y : rb.w;
   uint32_t light_id = 0;
   V3 lpos = splat(0.0f);
-  for (uint32_t li = 0; li < S.n_lights; ++li) {
-    float t;
-    V3 pos;
-    if (quad_intersect_plane(S.lights[li], ro, rd, r_tmin, t_max, &t, &pos) && t < t_max) {
-      t_max = t;
-      kind = 2;
-      light_id = li;
-      lpos = pos;
+  if (PART != kShadeHits) {
+    for (uint32_t li = 0; li < S.n_lights; ++li) {
+      float t;
+      V3 pos;
+      if (quad_intersect_plane(S.lights[li], ro, rd, r_tmin, t_max, &t, &pos) && t < t_max) {
+        t_max = t;
+        kind = 2;
+        light_id = li;
+        lpos = pos;
+      }
     }
+    if (PART == kShadeSplit && kind == 1) return true;
   }
 
-  if (kind == 0) {  // integrator.rs:77-94
+  if (PART != kShadeHits && kind == 0) {  // integrator.rs:77-94
     if (S.has_env) {
       V3 le = env_le(S.env, rd);
       if (!is_zero(le)) {
@@ -516,12 +529,12 @@ __device__ void shade_path(const Paths& P, const Sched& sc, const JobDev& J, con
       }
     }
     P.lum[p] = f4(l, lf.w);
-    return;
+    return false;
   }
 
   uint32_t mesh = 0;
   V3 pos, ns;
-  if (kind == 1) {
+  if (PART == kShadeHits || (PART == kShadeAll && kind == 1)) {
     mesh = mesh_of(S, gid);
     pos = ro + rd * hv.y;  // Ray::at (math.rs:26-28)
     if (S.motion) {
@@ -538,11 +551,12 @@ __device__ void shade_path(const Paths& P, const Sched& sc, const JobDev& J, con
   }
   if (!(flags & FL_AOV)) {  // integrator.rs:96-110
     flags |= FL_AOV;
-    V3 alb = kind == 1 ? S.mats[S.mesh_info[mesh] & 0x7FFFFFFFu].base : splat(1.0f);
+    V3 alb = PART == kShadeHits || (PART == kShadeAll && kind == 1) ? S.mats[S.mesh_info[mesh] & 0x7FFFFFFFu].base
+                                                                     : splat(1.0f);
     P.aov0[p] = f4(alb, t_max);
     P.aov1[p] = f4(ns, 0.0f);
   }
-  if (kind == 2) {  // integrator.rs:113-129
+  if (PART != kShadeHits && kind == 2) {  // a light hit: integrator.rs:113-129
     const QuadLightDev& L = S.lights[light_id];
     V3 le = dot(ns, -rd) > 0.0f ? L.emission : splat(0.0f);  // light.rs:146-152
     if (specular) {
@@ -553,15 +567,16 @@ __device__ void shade_path(const Paths& P, const Sched& sc, const JobDev& J, con
       l = l + beta * le * w;
     }
     P.lum[p] = f4(l, __builtin_bit_cast(float, flags));
-    return;
+    return false;
   }
+  if constexpr (PART == kShadeSplit) return false;  // not reached: every kind returned above
 
   const MatDev mat = S.mats[S.mesh_info[mesh] & 0x7FFFFFFFu];
   Frame frame = frame_from_normal(ns);
   V3 wo = to_local(frame, -rd);
   if (wo.z <= 0.0f) {  // back face: integrator.rs:136-139
     P.lum[p] = f4(l, __builtin_bit_cast(float, flags));
-    return;
+    return false;
   }
   bool reseed = false;
   if (pp.mode == MODE_SPLIT) {
@@ -578,7 +593,7 @@ __device__ void shade_path(const Paths& P, const Sched& sc, const JobDev& J, con
       for (int k = 0; k < 13; ++k) P.qfeat[13 * (size_t)p + k] = qf[k];
       o.query = true;
       P.lum[p] = f4(l, __builtin_bit_cast(float, flags));
-      return;
+      return false;
     }
     if (flags & FL_TRAIN) {
       const uint32_t tp = p - pp.train0;
@@ -679,15 +694,49 @@ __device__ void shade_path(const Paths& P, const Sched& sc, const JobDev& J, con
   }
   store_rng(P, p, rng);
   P.lum[p] = f4(l, __builtin_bit_cast(float, flags));
+  return false;
 }
 
+// The first half of a bounce for every path of the current queue: escaped
+// rays and light hits end here, and surface hits are queued for k_shade, so
+// that its waves run surface shading (material, light and environment
+// sampling, BSDF sampling) on full waves instead of on the quarter or less of
+// a wave's lanes whose rays hit a surface.
+__global__ __launch_bounds__(kThreads) void k_shade_split(Paths P, Sched sc, uint32_t depth) {
+  __shared__ BlockQueues<1> qs;
+  const PassDev pp = cur_pass(sc);
+  const uint32_t n = P.counters[0];
+  const uint32_t stride = gridDim.x * kThreads;
+  uint32_t* const ctr[1] = {&P.counters[5]};
+  // block-uniform trip count: the appends below synchronise the block
+  for (uint32_t b0 = blockIdx.x * kThreads; b0 < n; b0 += stride) {
+    const uint32_t i = b0 + threadIdx.x;
+    bool hit = false;
+    uint32_t p = 0;
+    if (i < n) {
+      p = depth == 0 ? i : P.q_cur[i];
+      ShadeOut o;
+      hit = shade_path<kShadeSplit>(P, sc, sc.jobs[P.pjob[p]], pp, depth, p, o);
+    }
+    const bool flag[1] = {hit};
+    uint32_t slot[1];
+    block_append<1>(qs, ctr, flag, slot);
+    if (hit) P.q_hit[slot[0]] = p;
+  }
+}
+
+// A whole bounce for every path of the current queue (HITS = false), or the
+// second half for the surface hits of k_shade_split's queue (HITS = true).
+// The first bounce runs whole: a quarter or more of its paths hit a surface,
+// and one kernel overlaps their shading with the escaped rays' work; later
+// bounces, where few paths hit, run in two halves.
 // MINW: minimum waves per SIMD asked of the register allocator (0 = none).
-template <int MINW>
+template <int MINW, bool HITS>
 __global__ __launch_bounds__(kThreads, MINW) void k_shade(Paths P, Sched sc, uint32_t depth) {
   __shared__ BlockQueues<3> qs;
   __shared__ unsigned long long s_nsh[kWaves];
   const PassDev pp = cur_pass(sc);
-  const uint32_t n = P.counters[0];
+  const uint32_t n = HITS ? P.counters[5] : P.counters[0];
   const uint32_t stride = gridDim.x * kThreads;
   uint32_t* const ctr[3] = {&P.counters[1], &P.counters[2], pp.mode == MODE_CACHE ? &P.counters[3] : nullptr};
   unsigned long long nsh = 0;
@@ -699,8 +748,11 @@ __global__ __launch_bounds__(kThreads, MINW) void k_shade(Paths P, Sched sc, uin
     o.n_shadow = 0;
     uint32_t p = 0;
     if (i < n) {
-      p = depth == 0 ? i : P.q_cur[i];
-      shade_path(P, sc, sc.jobs[P.pjob[p]], pp, depth, p, o);
+      p = HITS ? P.q_hit[i] : (depth == 0 ? i : P.q_cur[i]);
+      if (HITS)
+        shade_path<kShadeHits>(P, sc, sc.jobs[P.pjob[p]], pp, depth, p, o);
+      else
+        shade_path<kShadeAll>(P, sc, sc.jobs[P.pjob[p]], pp, depth, p, o);
     }
     nsh += o.n_shadow;
     const bool flag[3] = {o.cont, o.shadow, o.query};
@@ -1025,11 +1077,11 @@ Tuning ctx_tuning(const pyre_ctx* ctx) {
 }
 
 // Device bytes per pool path (Work::alloc): ray 32, stream 32, throughput and
-// pdf 16, radiance and flags 16, AOVs 32, hit 16, shutter time 4, the three
-// queues, the job id and the shadow-slot count 20, and per shadow slot a ray
+// pdf 16, radiance and flags 16, AOVs 32, hit 16, shutter time 4, the four
+// queues, the job id and the shadow-slot count 24, and per shadow slot a ray
 // (32) and its contribution (16); a radiance-cache pool adds a query (52),
 // its path (4), the cache output (12) and a vertex count (4).
-size_t path_bytes(uint32_t slots, bool cache) { return 168 + 48 * (size_t)slots + (cache ? 72 : 0); }
+size_t path_bytes(uint32_t slots, bool cache) { return 172 + 48 * (size_t)slots + (cache ? 72 : 0); }
 
 // Paths a pool may hold: the tuning's pool size, bounded by the tuning's
 // share of the device's free memory. `reclaimable` = bytes of a pool that is
@@ -1149,6 +1201,7 @@ struct Work {
     P.q_cur = get<uint32_t>(cap);
     P.q_next = get<uint32_t>(cap);
     P.q_shadow = get<uint32_t>(cap);
+    P.q_hit = get<uint32_t>(cap);
     P.pjob = get<uint32_t>(cap);
     P.counters = get<uint32_t>(kCounters);
     P.stats = own_stats = get<unsigned long long>(4);
@@ -1243,7 +1296,7 @@ unsigned resident_blocks(K kernel, unsigned want) {
 // so no runtime query is made while a pass is being captured).
 struct StageOcc {
   unsigned extend[3][2], shadow[3][2];  // [ring 4 / 8 / 16][static / moving]
-  unsigned generate, accumulate, shade1, shade4, train_targets, cache_apply;
+  unsigned generate, accumulate, shade_split, shade1[2], shade4[2], train_targets, cache_apply;  // [whole / hits]
 };
 
 const StageOcc& stage_occ() {
@@ -1259,8 +1312,11 @@ const StageOcc& stage_occ() {
     r.shadow[1][1] = resident_blocks(k_shadow<8, true>, 64);
     r.generate = resident_blocks(k_generate, 8);
     r.accumulate = resident_blocks(k_accumulate, 8);
-    r.shade1 = resident_blocks(k_shade<1>, 8);
-    r.shade4 = resident_blocks(k_shade<4>, 8);
+    r.shade_split = resident_blocks(k_shade_split, 8);
+    r.shade1[0] = resident_blocks(k_shade<1, false>, 8);
+    r.shade4[0] = resident_blocks(k_shade<4, false>, 8);
+    r.shade1[1] = resident_blocks(k_shade<1, true>, 8);
+    r.shade4[1] = resident_blocks(k_shade<4, true>, 8);
     r.train_targets = resident_blocks(k_train_targets, 8);
     r.cache_apply = resident_blocks(k_cache_apply, 8);
     return r;
@@ -1304,7 +1360,7 @@ void launch_pass(pyre_ctx* ctx, Work& W, const PassShape& shape, const Tuning& t
   // second, partial round of blocks would leave most of the GPU idle)
   const unsigned cus = (unsigned)std::max(1, ctx->cus);
   const StageOcc& occ = stage_occ();
-  const unsigned gen_bpc = occ.generate, acc_bpc = occ.accumulate, shade1_bpc = occ.shade1, shade4_bpc = occ.shade4;
+  const unsigned gen_bpc = occ.generate, acc_bpc = occ.accumulate;
   const unsigned tgrid = trace_grid_of(ctx, t);
   const Sched sc = W.sched();
   Paths P = W.P;
@@ -1312,10 +1368,19 @@ void launch_pass(pyre_ctx* ctx, Work& W, const PassShape& shape, const Tuning& t
   hipLaunchKernelGGL(k_generate, dim3(cus * gen_bpc), dim3(kThreads), 0, st, P, sc);
   for (uint32_t d = 0; d < shape.max_depth; ++d) {
     launch_trace(W, P, t, tgrid, true, false, st, shape.one_job, shape.motion, d == 0);
-    if (t.shade_waves >= 4)
-      hipLaunchKernelGGL(k_shade<4>, dim3(cus * shade4_bpc), dim3(kThreads), 0, st, P, sc, d);
-    else
-      hipLaunchKernelGGL(k_shade<1>, dim3(cus * shade1_bpc), dim3(kThreads), 0, st, P, sc, d);
+    const bool split = d > 0;  // see k_shade
+    if (split) hipLaunchKernelGGL(k_shade_split, dim3(cus * occ.shade_split), dim3(kThreads), 0, st, P, sc, d);
+    if (t.shade_waves >= 4) {
+      if (split)
+        hipLaunchKernelGGL((k_shade<4, true>), dim3(cus * occ.shade4[1]), dim3(kThreads), 0, st, P, sc, d);
+      else
+        hipLaunchKernelGGL((k_shade<4, false>), dim3(cus * occ.shade4[0]), dim3(kThreads), 0, st, P, sc, d);
+    } else {
+      if (split)
+        hipLaunchKernelGGL((k_shade<1, true>), dim3(cus * occ.shade1[1]), dim3(kThreads), 0, st, P, sc, d);
+      else
+        hipLaunchKernelGGL((k_shade<1, false>), dim3(cus * occ.shade1[0]), dim3(kThreads), 0, st, P, sc, d);
+    }
     if (shape.nee) launch_trace(W, P, t, tgrid, false, true, st, shape.one_job, shape.motion, false);
     hipLaunchKernelGGL(k_rotate, dim3(1), dim3(kCounters), 0, st, P.counters, P.stats);
     std::swap(P.q_cur, P.q_next);
