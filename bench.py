@@ -272,7 +272,8 @@ def bench_production(args):
 
 
 def shade_byte_model(st, ext, sh, samples):
-    """Bytes k_shade must move per frame by the path pool's layout (fields
+    """Bytes the shade stage (k_shade_split + k_shade) must move per frame
+    by the path pool's layout (fields
     read or written once per vertex; CDF searches and cache-line granularity
     not counted). Vertex kinds: V = extension rays (every one is shaded),
     C = V - samples (vertices past the camera ray), Sh = shadow rays, and
@@ -299,7 +300,8 @@ def shade_byte_model(st, ext, sh, samples):
 
 
 def production_roofline(osc, ocam, ext, sh, samples):
-    """k_shade, the frame's longest stage, against HBM: the layout byte
+    """The shade stage (k_shade, and k_shade_split from the second bounce
+    on), the frame's longest, against HBM: the layout byte
     model over the stage's per-frame time and its FETCH_SIZE / WRITE_SIZE
     bytes, both from the committed kernel trace and counter runs of this
     frame (the stage runs inside the library's graph-replayed passes, out of
@@ -310,7 +312,8 @@ def production_roofline(osc, ocam, ext, sh, samples):
     fetch = measured_traffic("k_shade_fetch_bytes_per_frame", PRODUCTION_PROFILE)
     write = measured_traffic("k_shade_write_bytes_per_frame", PRODUCTION_PROFILE)
     model = m["bytes_read"] + m["bytes_written"]
-    out = {"bound": "hbm", "kernel": "k_shade (12 launches per frame, one per bounce; figures per frame)",
+    out = {"bound": "hbm", "kernel": "shade stage: k_shade (12 launches per frame, one per bounce) + k_shade_split "
+                                     "(11, from the second bounce); figures per frame",
            "peak": HBM_PEAK_GBS, "unit": "GB/s", "algorithmic_bytes": round(model),
            "traffic": None if fetch is None or write is None else round(fetch + write), "model": m,
            "sample_pixels": "oracle: every 8th pixel in x and y (240 x 135), 1 spp"}
