@@ -146,7 +146,7 @@ struct Paths {
   uint32_t* q_shadow;
   uint32_t* q_hit;     // paths of the current queue whose ray hit a surface (k_shade_split -> k_shade)
   uint32_t* counters;  // [0] current, [1] next, [2] shadow, [3] cache queries, [4] training samples,
-                       // [5] surface hits, [8] extend work, [16] shadow work
+                       // [5] surface-hit queue, [6] BVH hits of the bounce, [8] extend work, [16] shadow work
   unsigned long long* stats;  // [0] extension rays, [1] shadow rays, [2] cache queries, [3] training samples
   uint32_t* pjob;    // job of each path
   float4* vrec;      // training paths: kVrec float4 per (training path, vertex)
@@ -222,8 +222,8 @@ __global__ void k_counters(uint32_t* c, Sched sc) {
   if (i < kCounters) c[i] = i == 0 ? cur_pass(sc).paths : 0u;
 }
 
-// next queue becomes current; next, shadow and surface-hit queues and the
-// work counters are emptied
+// next queue becomes current; next, shadow and surface-hit queues, the hit
+// count and the work counters are emptied
 __global__ void k_rotate(uint32_t* c, unsigned long long* stats) {
   const uint32_t i = threadIdx.x;
   const uint32_t cur = c[0], next = c[1];
@@ -231,7 +231,7 @@ __global__ void k_rotate(uint32_t* c, unsigned long long* stats) {
   if (i == 0) {
     stats[0] += cur;
     c[0] = next;
-  } else if (i == 1 || i == 2 || i == 5 || (i >= 8 && i < kCounters)) {
+  } else if (i == 1 || i == 2 || i == 5 || i == 6 || (i >= 8 && i < kCounters)) {
     c[i] = 0u;
   }
 }
@@ -333,6 +333,7 @@ __global__ __launch_bounds__(kThreads) void k_extend(Paths P, const JobDev* __re
   const float4* tris = one_job ? jobs[0].S.tris : nullptr;
   const float4* motion = MOTION && one_job ? jobs[0].S.motion : nullptr;
   bool active = false;
+  uint32_t nhit = 0;  // rays of this lane that hit the BVH (the shade stage's choice of form)
   for (;;) {
     const uint64_t idle = __ballot(!active);
     if (__popcll(idle) >= kRefillLanes) {
@@ -354,9 +355,13 @@ __global__ __launch_bounds__(kThreads) void k_extend(Paths P, const JobDev* __re
       THit h;
       tv.result(&h);
       P.hit[p] = make_float4(__builtin_bit_cast(float, h.gid), h.t, h.u, h.v);
+      nhit += h.gid != PYRE_MISS ? 1u : 0u;
       active = false;
     }
   }
+  // the loop ends for the whole wave at once
+  for (int k = 32; k > 0; k >>= 1) nhit += __shfl_xor(nhit, k);
+  if (__lane_id() == 0 && nhit) atomicAdd(&P.counters[6], nhit);
 }
 
 // Any-hit for every shadow slot of the queued paths; a lane keeps a path
@@ -468,15 +473,13 @@ struct ShadeOut {
   uint32_t n_shadow;
 };
 
-// One bounce of integrator.rs:76-222 for path p: whole (PART = kShadeAll),
-// or in two halves. kShadeSplit (k_shade_split): the ray's nearest hit among
-// the BVH's and the quad lights'; an escaped ray or a light hit ends the path
+// One bounce of integrator.rs:76-222 for path p (PART = kShadeAll), or its
+// first part (kShadeSplit, k_shade_split): the ray's nearest hit among the
+// BVH's and the quad lights'; an escaped ray or a light hit ends the path
 // here (and the call returns false); a surface hit returns true, writing
-// nothing, to be shaded by the other half. kShadeHits (k_shade over the
-// surface-hit queue): path p's ray hit a surface and no light is nearer, so
-// the light loop is not run again; the surface's shading, light sampling and
-// BSDF sampling follow.
-constexpr int kShadeAll = 0, kShadeSplit = 1, kShadeHits = 2;
+// nothing, to be shaded whole by k_shade from the surface-hit queue (its
+// light loop finds no nearer light again).
+constexpr int kShadeAll = 0, kShadeSplit = 1;
 template <int PART>
 __device__ bool shade_path(const Paths& P, const Sched& sc, const JobDev& J, const PassDev& pp, uint32_t depth,
                            uint32_t p, ShadeOut& o) {
@@ -501,7 +504,7 @@ __device__ bool shade_path(const Paths& P, const Sched& sc, const JobDev& J, con
   float t_max = kind ? hv.y : rb.w;
   uint32_t light_id = 0;
   V3 lpos = splat(0.0f);
-  if (PART != kShadeHits) {
+  {
     for (uint32_t li = 0; li < S.n_lights; ++li) {
       float t;
       V3 pos;
@@ -515,7 +518,7 @@ __device__ bool shade_path(const Paths& P, const Sched& sc, const JobDev& J, con
     if (PART == kShadeSplit && kind == 1) return true;
   }
 
-  if (PART != kShadeHits && kind == 0) {  // integrator.rs:77-94
+  if (kind == 0) {  // integrator.rs:77-94
     if (S.has_env) {
       V3 le = env_le(S.env, rd);
       if (!is_zero(le)) {
@@ -534,7 +537,7 @@ __device__ bool shade_path(const Paths& P, const Sched& sc, const JobDev& J, con
 
   uint32_t mesh = 0;
   V3 pos, ns;
-  if (PART == kShadeHits || (PART == kShadeAll && kind == 1)) {
+  if (PART == kShadeAll && kind == 1) {
     mesh = mesh_of(S, gid);
     pos = ro + rd * hv.y;  // Ray::at (math.rs:26-28)
     if (S.motion) {
@@ -551,12 +554,11 @@ __device__ bool shade_path(const Paths& P, const Sched& sc, const JobDev& J, con
   }
   if (!(flags & FL_AOV)) {  // integrator.rs:96-110
     flags |= FL_AOV;
-    V3 alb = PART == kShadeHits || (PART == kShadeAll && kind == 1) ? S.mats[S.mesh_info[mesh] & 0x7FFFFFFFu].base
-                                                                     : splat(1.0f);
+    V3 alb = PART == kShadeAll && kind == 1 ? S.mats[S.mesh_info[mesh] & 0x7FFFFFFFu].base : splat(1.0f);
     P.aov0[p] = f4(alb, t_max);
     P.aov1[p] = f4(ns, 0.0f);
   }
-  if (PART != kShadeHits && kind == 2) {  // a light hit: integrator.rs:113-129
+  if (kind == 2) {  // a light hit: integrator.rs:113-129
     const QuadLightDev& L = S.lights[light_id];
     V3 le = dot(ns, -rd) > 0.0f ? L.emission : splat(0.0f);  // light.rs:146-152
     if (specular) {
@@ -697,13 +699,23 @@ __device__ bool shade_path(const Paths& P, const Sched& sc, const JobDev& J, con
   return false;
 }
 
-// The first half of a bounce for every path of the current queue: escaped
-// rays and light hits end here, and surface hits are queued for k_shade, so
-// that its waves run surface shading (material, light and environment
-// sampling, BSDF sampling) on full waves instead of on the quarter or less of
-// a wave's lanes whose rays hit a surface.
+// The shade stage's form for a bounce, from the queue's size and its BVH hits
+// (counted by k_extend; both unchanged until k_rotate): when at most a fifth
+// of the paths hit a surface, the bounce is shaded in two kernels --
+// k_shade_split ends escaped rays and light hits and queues the surface hits,
+// k_shade shades the queued hits on full waves -- so that the rare hits'
+// chains of dependent loads do not stretch every wave. With more hits one
+// kernel over every path is faster (it overlaps the hits' chains with the
+// escaped rays' work): a bounce with a quarter of its paths hitting took 7.7
+// ms split and 7.1 ms whole on the 1080p production frame, and the closed
+// Cornell box, where almost every ray hits, lost 2.9 ms per frame split.
+__device__ __forceinline__ bool shade_split(const Paths& P) { return 5u * P.counters[6] <= P.counters[0]; }
+
+// The first part of a split bounce for every path of the current queue (the
+// kernel exits at once when the bounce is shaded whole).
 __global__ __launch_bounds__(kThreads) void k_shade_split(Paths P, Sched sc, uint32_t depth) {
   __shared__ BlockQueues<1> qs;
+  if (!shade_split(P)) return;
   const PassDev pp = cur_pass(sc);
   const uint32_t n = P.counters[0];
   const uint32_t stride = gridDim.x * kThreads;
@@ -725,18 +737,16 @@ __global__ __launch_bounds__(kThreads) void k_shade_split(Paths P, Sched sc, uin
   }
 }
 
-// A whole bounce for every path of the current queue (HITS = false), or the
-// second half for the surface hits of k_shade_split's queue (HITS = true).
-// The first bounce runs whole: a quarter or more of its paths hit a surface,
-// and one kernel overlaps their shading with the escaped rays' work; later
-// bounces, where few paths hit, run in two halves.
+// A whole bounce for every path of the current queue, or, when the bounce is
+// split (shade_split), for the surface hits that k_shade_split queued.
 // MINW: minimum waves per SIMD asked of the register allocator (0 = none).
-template <int MINW, bool HITS>
+template <int MINW>
 __global__ __launch_bounds__(kThreads, MINW) void k_shade(Paths P, Sched sc, uint32_t depth) {
   __shared__ BlockQueues<3> qs;
   __shared__ unsigned long long s_nsh[kWaves];
   const PassDev pp = cur_pass(sc);
-  const uint32_t n = HITS ? P.counters[5] : P.counters[0];
+  const bool split = shade_split(P);
+  const uint32_t n = split ? P.counters[5] : P.counters[0];
   const uint32_t stride = gridDim.x * kThreads;
   uint32_t* const ctr[3] = {&P.counters[1], &P.counters[2], pp.mode == MODE_CACHE ? &P.counters[3] : nullptr};
   unsigned long long nsh = 0;
@@ -748,11 +758,8 @@ __global__ __launch_bounds__(kThreads, MINW) void k_shade(Paths P, Sched sc, uin
     o.n_shadow = 0;
     uint32_t p = 0;
     if (i < n) {
-      p = HITS ? P.q_hit[i] : (depth == 0 ? i : P.q_cur[i]);
-      if (HITS)
-        shade_path<kShadeHits>(P, sc, sc.jobs[P.pjob[p]], pp, depth, p, o);
-      else
-        shade_path<kShadeAll>(P, sc, sc.jobs[P.pjob[p]], pp, depth, p, o);
+      p = split ? P.q_hit[i] : (depth == 0 ? i : P.q_cur[i]);
+      shade_path<kShadeAll>(P, sc, sc.jobs[P.pjob[p]], pp, depth, p, o);
     }
     nsh += o.n_shadow;
     const bool flag[3] = {o.cont, o.shadow, o.query};
@@ -1296,7 +1303,7 @@ unsigned resident_blocks(K kernel, unsigned want) {
 // so no runtime query is made while a pass is being captured).
 struct StageOcc {
   unsigned extend[3][2], shadow[3][2];  // [ring 4 / 8 / 16][static / moving]
-  unsigned generate, accumulate, shade_split, shade1[2], shade4[2], train_targets, cache_apply;  // [whole / hits]
+  unsigned generate, accumulate, shade_split, shade1, shade4, train_targets, cache_apply;
 };
 
 const StageOcc& stage_occ() {
@@ -1313,10 +1320,8 @@ const StageOcc& stage_occ() {
     r.generate = resident_blocks(k_generate, 8);
     r.accumulate = resident_blocks(k_accumulate, 8);
     r.shade_split = resident_blocks(k_shade_split, 8);
-    r.shade1[0] = resident_blocks(k_shade<1, false>, 8);
-    r.shade4[0] = resident_blocks(k_shade<4, false>, 8);
-    r.shade1[1] = resident_blocks(k_shade<1, true>, 8);
-    r.shade4[1] = resident_blocks(k_shade<4, true>, 8);
+    r.shade1 = resident_blocks(k_shade<1>, 8);
+    r.shade4 = resident_blocks(k_shade<4>, 8);
     r.train_targets = resident_blocks(k_train_targets, 8);
     r.cache_apply = resident_blocks(k_cache_apply, 8);
     return r;
@@ -1368,19 +1373,12 @@ void launch_pass(pyre_ctx* ctx, Work& W, const PassShape& shape, const Tuning& t
   hipLaunchKernelGGL(k_generate, dim3(cus * gen_bpc), dim3(kThreads), 0, st, P, sc);
   for (uint32_t d = 0; d < shape.max_depth; ++d) {
     launch_trace(W, P, t, tgrid, true, false, st, shape.one_job, shape.motion, d == 0);
-    const bool split = d > 0;  // see k_shade
-    if (split) hipLaunchKernelGGL(k_shade_split, dim3(cus * occ.shade_split), dim3(kThreads), 0, st, P, sc, d);
-    if (t.shade_waves >= 4) {
-      if (split)
-        hipLaunchKernelGGL((k_shade<4, true>), dim3(cus * occ.shade4[1]), dim3(kThreads), 0, st, P, sc, d);
-      else
-        hipLaunchKernelGGL((k_shade<4, false>), dim3(cus * occ.shade4[0]), dim3(kThreads), 0, st, P, sc, d);
-    } else {
-      if (split)
-        hipLaunchKernelGGL((k_shade<1, true>), dim3(cus * occ.shade1[1]), dim3(kThreads), 0, st, P, sc, d);
-      else
-        hipLaunchKernelGGL((k_shade<1, false>), dim3(cus * occ.shade1[0]), dim3(kThreads), 0, st, P, sc, d);
-    }
+    // the shade stage's form is chosen on the device (shade_split)
+    hipLaunchKernelGGL(k_shade_split, dim3(cus * occ.shade_split), dim3(kThreads), 0, st, P, sc, d);
+    if (t.shade_waves >= 4)
+      hipLaunchKernelGGL(k_shade<4>, dim3(cus * occ.shade4), dim3(kThreads), 0, st, P, sc, d);
+    else
+      hipLaunchKernelGGL(k_shade<1>, dim3(cus * occ.shade1), dim3(kThreads), 0, st, P, sc, d);
     if (shape.nee) launch_trace(W, P, t, tgrid, false, true, st, shape.one_job, shape.motion, false);
     hipLaunchKernelGGL(k_rotate, dim3(1), dim3(kCounters), 0, st, P.counters, P.stats);
     std::swap(P.q_cur, P.q_next);
