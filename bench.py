@@ -640,7 +640,7 @@ def nrc_block(ctx, args):
     warm = nrc_comparison(ctx, scene, cam, ip, ref, max_spp=args.max_spp, warmup=0, pre_frames=NRC_PRETRAIN,
                           spf_max=16)
     out["warm_cache"] = {k: warm[k] for k in ("nrc_seconds", "nrc_over_plain", "relmse_at_plain_seconds",
-                                              "training", "nrc_curve")}
+                                              "training", "nrc_curve", "path_stats_last")}
     out["best_case"] = cache_best_case(ctx, scene, cam, ip, ref, out["plain_seconds"], n_spp=16, k_cont=256)
     lo, hi = scene.bounds()
     out["cache_mfma"] = cache_mfma(ctx, lo, hi, args)
