@@ -386,7 +386,9 @@ int pyre_nrc_get_weights(pyre_ctx* ctx, pyre_nrc* nrc, float* out);
 int pyre_nrc_set_weights(pyre_ctx* ctx, pyre_nrc* nrc, const float* weights);
 /* rgb = 3n floats for feats = 13n floats. */
 int pyre_nrc_infer(pyre_ctx* ctx, pyre_nrc* nrc, const float* feats, size_t n, float* rgb);
-/* Loss and FP32 gradient (param_count floats) of one batch, no update. */
+/* Loss and FP32 gradient (param_count floats) of one batch, no update: the
+ * sum, in FP32 and in a fixed order, of per-CU partial gradients stored in
+ * bf16 (the products' operands are bf16); the Adam step uses the same sum. */
 int pyre_nrc_gradients(pyre_ctx* ctx, pyre_nrc* nrc, const float* feats, const float* targets, size_t n,
                        float* grad, float* loss);
 /* One Adam step on the batch; loss receives the pre-step loss. */
