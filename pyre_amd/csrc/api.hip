@@ -160,6 +160,7 @@ void scene_free(pyre_scene* s) {
   dev_free(s->d_normals);
   dev_free(s->d_motion);
   dev_free(s->d_tri_offset);
+  dev_free(s->d_mesh_blk);
   dev_free(s->d_mesh_info);
   dev_free(s->d_mats);
   dev_free(s->d_lights);
@@ -575,6 +576,21 @@ int pyre_scene_upload(pyre_ctx* ctx, const pyre_scene_desc* desc, pyre_scene** o
     }
     s->d_tri_offset = dev_alloc<uint32_t>(d.n_meshes + 1);
     h2d(s->d_tri_offset, s->h_tri_offset.data(), d.n_meshes + 1, st);
+    {
+      // entry k: the largest m with tri_offset[m] <= k 2^kMeshBlkShift, so the
+      // mesh of triangle t lies in [entry t >> shift, entry (t >> shift) + 1]
+      const size_t nblk = (tri_total >> kMeshBlkShift) + 2;
+      std::vector<uint32_t> blk(nblk, 0);
+      uint32_t m = 0;
+      for (size_t k = 0; k < nblk; ++k) {
+        const size_t t = k << kMeshBlkShift;
+        while (m + 1 < d.n_meshes && s->h_tri_offset[m + 1] <= t) ++m;
+        blk[k] = m;
+      }
+      s->d_mesh_blk = dev_alloc<uint32_t>(nblk);
+      h2d(s->d_mesh_blk, blk.data(), nblk, st);
+      PYRE_HIP_CHECK(hipStreamSynchronize(st));  // blk is freed at the end of this scope
+    }
     s->d_mesh_info = dev_alloc<uint32_t>(std::max<size_t>(d.n_meshes, 1));
     h2d(s->d_mesh_info, s->h_mesh_info.data(), d.n_meshes, st);
     if (s->has_motion) {

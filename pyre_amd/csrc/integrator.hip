@@ -63,6 +63,7 @@ struct SceneDev {
   const float* verts;
   const float* normals;
   const uint32_t* tri_offset;
+  const uint32_t* mesh_blk;  // mesh lookup table (kMeshBlkShift)
   const uint32_t* mesh_info;
   const MatDev* mats;
   const QuadLightDev* lights;
@@ -456,8 +457,13 @@ __device__ __forceinline__ V3 surface_normal(const SceneDev& S, uint32_t gid, ui
   return normalize(cross(v1 - v0, v2 - v0));
 }
 
+// The mesh of triangle gid: the largest m with tri_offset[m] <= gid. The
+// lookup table brackets it within [mesh_blk[b], mesh_blk[b + 1]] for gid's
+// block b (one cache line; the two entries are equal unless a mesh starts
+// inside the block), which the search then narrows.
 __device__ __forceinline__ uint32_t mesh_of(const SceneDev& S, uint32_t gid) {
-  uint32_t lo = 0, hi = S.n_meshes;  // largest m with tri_offset[m] <= gid
+  const uint32_t b = gid >> kMeshBlkShift;
+  uint32_t lo = S.mesh_blk[b], hi = S.mesh_blk[b + 1] + 1;
   while (hi - lo > 1) {
     uint32_t mid = (lo + hi) / 2;
     if (S.tri_offset[mid] <= gid)
@@ -1252,7 +1258,7 @@ SceneDev scene_dev(pyre_scene* s) {
   PhiTable phi;
   phi.rank = s->ctx->d_phi_rank;
   phi.val = s->ctx->d_phi_val;
-  return SceneDev{s->d_nodes4, s->d_tris, s->d_verts, s->d_normals, s->d_tri_offset, s->d_mesh_info, s->d_mats,
+  return SceneDev{s->d_nodes4, s->d_tris, s->d_verts, s->d_normals, s->d_tri_offset, s->d_mesh_blk, s->d_mesh_info, s->d_mats,
                   s->d_lights, s->n_meshes, s->n_lights, s->has_env ? 1u : 0u, s->env, s->d_motion, phi};
 }
 

@@ -101,6 +101,8 @@ static_assert(sizeof(Node4) == 128, "Node4 layout");
 // Mark in a triangle record's e2.w: the last slot of its leaf (set by the
 // build on every leaf), so a traversal can hold a pending leaf as one slot.
 constexpr uint32_t kLeafEnd = 1u;
+// Triangles per entry of a scene's mesh lookup table (d_mesh_blk), log2.
+constexpr uint32_t kMeshBlkShift = 12;
 
 // 4-wide node with its child boxes quantised to a scene-wide grid, 64 bytes
 // (four dwordx4 loads instead of eight). Each child plane is a 16-bit cell
@@ -193,6 +195,7 @@ struct pyre_scene {
   std::vector<uint32_t> h_moving;  // per mesh: 1 = moving
   float4* d_motion = nullptr;    // h_motion on the device (null for static scenes)
   uint32_t* d_tri_offset = nullptr;
+  uint32_t* d_mesh_blk = nullptr;  // per block of 2^kMeshBlkShift triangles: the last mesh starting at or before it
   uint32_t* d_mesh_info = nullptr;
   pyre::MatDev* d_mats = nullptr;
   pyre::QuadLightDev* d_lights = nullptr;
