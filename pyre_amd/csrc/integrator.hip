@@ -719,7 +719,7 @@ __device__ __forceinline__ bool shade_split(const Paths& P) { return 5u * P.coun
 
 // The first part of a split bounce for every path of the current queue (the
 // kernel exits at once when the bounce is shaded whole).
-__global__ __launch_bounds__(kThreads) void k_shade_split(Paths P, Sched sc, uint32_t depth) {
+__global__ __launch_bounds__(kThreads) void k_shade_split(Paths P, Sched sc, uint32_t depth, bool one_job) {
   __shared__ BlockQueues<1> qs;
   if (!shade_split(P)) return;
   const PassDev pp = cur_pass(sc);
@@ -734,7 +734,7 @@ __global__ __launch_bounds__(kThreads) void k_shade_split(Paths P, Sched sc, uin
     if (i < n) {
       p = depth == 0 ? i : P.q_cur[i];
       ShadeOut o;
-      hit = shade_path<kShadeSplit>(P, sc, sc.jobs[P.pjob[p]], pp, depth, p, o);
+      hit = shade_path<kShadeSplit>(P, sc, sc.jobs[one_job ? 0u : P.pjob[p]], pp, depth, p, o);
     }
     const bool flag[1] = {hit};
     uint32_t slot[1];
@@ -747,7 +747,7 @@ __global__ __launch_bounds__(kThreads) void k_shade_split(Paths P, Sched sc, uin
 // split (shade_split), for the surface hits that k_shade_split queued.
 // MINW: minimum waves per SIMD asked of the register allocator (0 = none).
 template <int MINW>
-__global__ __launch_bounds__(kThreads, MINW) void k_shade(Paths P, Sched sc, uint32_t depth) {
+__global__ __launch_bounds__(kThreads, MINW) void k_shade(Paths P, Sched sc, uint32_t depth, bool one_job) {
   __shared__ BlockQueues<3> qs;
   __shared__ unsigned long long s_nsh[kWaves];
   const PassDev pp = cur_pass(sc);
@@ -765,7 +765,7 @@ __global__ __launch_bounds__(kThreads, MINW) void k_shade(Paths P, Sched sc, uin
     uint32_t p = 0;
     if (i < n) {
       p = split ? P.q_hit[i] : (depth == 0 ? i : P.q_cur[i]);
-      shade_path<kShadeAll>(P, sc, sc.jobs[P.pjob[p]], pp, depth, p, o);
+      shade_path<kShadeAll>(P, sc, sc.jobs[one_job ? 0u : P.pjob[p]], pp, depth, p, o);
     }
     nsh += o.n_shadow;
     const bool flag[3] = {o.cont, o.shadow, o.query};
@@ -1380,11 +1380,11 @@ void launch_pass(pyre_ctx* ctx, Work& W, const PassShape& shape, const Tuning& t
   for (uint32_t d = 0; d < shape.max_depth; ++d) {
     launch_trace(W, P, t, tgrid, true, false, st, shape.one_job, shape.motion, d == 0);
     // the shade stage's form is chosen on the device (shade_split)
-    hipLaunchKernelGGL(k_shade_split, dim3(cus * occ.shade_split), dim3(kThreads), 0, st, P, sc, d);
+    hipLaunchKernelGGL(k_shade_split, dim3(cus * occ.shade_split), dim3(kThreads), 0, st, P, sc, d, shape.one_job);
     if (t.shade_waves >= 4)
-      hipLaunchKernelGGL(k_shade<4>, dim3(cus * occ.shade4), dim3(kThreads), 0, st, P, sc, d);
+      hipLaunchKernelGGL(k_shade<4>, dim3(cus * occ.shade4), dim3(kThreads), 0, st, P, sc, d, shape.one_job);
     else
-      hipLaunchKernelGGL(k_shade<1>, dim3(cus * occ.shade1), dim3(kThreads), 0, st, P, sc, d);
+      hipLaunchKernelGGL(k_shade<1>, dim3(cus * occ.shade1), dim3(kThreads), 0, st, P, sc, d, shape.one_job);
     if (shape.nee) launch_trace(W, P, t, tgrid, false, true, st, shape.one_job, shape.motion, false);
     hipLaunchKernelGGL(k_rotate, dim3(1), dim3(kCounters), 0, st, P.counters, P.stats);
     std::swap(P.q_cur, P.q_next);
