@@ -94,6 +94,47 @@ def test_production_scene_small(ctx, oracle):
     assert e2 < 1e-8
 
 
+def test_many_small_meshes_first_hit_aovs(ctx, oracle):
+    """2,500 meshes of 0..9 triangles (every 97th empty), each with its own
+    material, so mesh boundaries fall inside the blocks of 4096 triangles the
+    shade stage's mesh lookup table brackets: the first hit's albedo (its
+    mesh's material), normal and depth AOVs equal the oracle's bit for bit
+    (pinhole camera, no lights: the radiance is zero on both sides)."""
+    print("[render many meshes] 2500 meshes, 128x128, 1 spp", flush=True)
+    rng = np.random.default_rng(12)
+    n_mesh = 2500
+    counts = rng.integers(0, 10, n_mesh)
+    counts[::97] = 0
+    meshes, mats = [], []
+    os_ = oracle.Scene()
+    for m in range(n_mesh):
+        base = (((m % 13) + 1) / 14.0, ((m % 7) + 1) / 8.0, ((m % 5) + 1) / 6.0)
+        mats.append(pyre_amd.Material(pyre_amd.MAT_LAMBERTIAN, base))
+        os_.add_material(oracle.material(0, base))
+        # the mesh's triangles side by side in cell m of a 50 x 50 grid on z = 0
+        cx, cy = (m % 50) - 24.5, (m // 50) - 24.5
+        pos = np.array([[(cx - 0.45 + 0.1 * k, cy - 0.4, 0.0), (cx - 0.35 + 0.1 * k, cy - 0.4, 0.0),
+                         (cx - 0.45 + 0.1 * k, cy + 0.4, 0.0)] for k in range(int(counts[m]))],
+                       np.float32).reshape(-1, 3)
+        idx = np.arange(pos.shape[0], dtype=np.uint32).reshape(-1, 3)
+        meshes.append(dict(positions=pos, indices=idx, material_id=m))
+        if counts[m]:
+            os_.add_mesh(pos, idx, None, m)
+    assert int(counts.sum()) > 2 * 4096
+    sc = pyre_amd.Scene.from_arrays(ctx, meshes, mats)
+    sc.build_bvh(bins=12, max_leaf=8)
+    w = h = 128
+    cam = pyre_amd.Camera((0.0, 0.0, 60.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 45.0, 1.0)
+    g = pyre_amd.Film(w, h).render(sc, cam, pyre_amd.PathIntegrator(8, 3), 1).reshape(h, w, 10)
+    o = os_.render(_ocam(oracle, cam), w, h, 1).reshape(h, w, 10)
+    hit = o[..., 9] > 0
+    same = np.all(g[..., 3:].view(np.uint32) == o[..., 3:].view(np.uint32), axis=-1)
+    n_alb = len(np.unique(o[hit][:, 3:6], axis=0))
+    print(f"  pixels on a triangle {hit.mean():.3f}, materials seen {n_alb}, AOV bit-equal {same.mean():.4f}")
+    assert hit.mean() > 0.15 and n_alb > 100
+    assert same.all()
+
+
 def test_ray_counts_match_reference_queries(ctx, oracle):
     """The device's extension and shadow ray counts are the reference's
     Scene::intersect and Scene::occluded calls for the same samples (paths of
