@@ -280,6 +280,19 @@ __device__ __forceinline__ void cswap(float& ka, int32_t& ra, int& ca, float& kb
   }
 }
 
+// Compare-exchange of (key, reference) pairs: the pair with the smaller key
+// first, ties kept in place.
+__device__ __forceinline__ void cswap2(float& ka, int32_t& ra, float& kb, int32_t& rb) {
+  if (kb < ka) {
+    const float tk = ka;
+    ka = kb;
+    kb = tk;
+    const int32_t tr = ra;
+    ra = rb;
+    rb = tr;
+  }
+}
+
 // Traversal of the 4-wide tree as a resumable state machine, one node visit
 // per step() so a persistent kernel can hand a finished lane a new ray at
 // any step. Same hit semantics as traverse(): children are visited
@@ -326,11 +339,11 @@ struct Trav4 {
     // tested too and masked out), so the lanes of a wave stay converged
     float e0, e1, e2, e3;
     bool h0, h1, h2, h3;
-    int c0, c1, c2, c3;
     int32_t r0, r1, r2, r3;
     if constexpr (QNODE) {
       // child i's cells: lo/hi words (i / 2) and (2 + i / 2), half i % 2;
-      // leaves carry kQLeaf (count 1 marks a leaf, the end mark its end)
+      // the reference word tells leaves (kQLeaf set; the end mark in the
+      // triangle records ends them) from inner nodes, and -1 marks a miss
       constexpr uint32_t m = 0xFFFFu;
       h0 = (nd.ref.x >= 0) & qslab_test(qr, r.t_min, nd.x.x & m, nd.x.z & m, nd.y.x & m, nd.y.z & m, nd.z.x & m,
                                         nd.z.z & m, best_t, &e0);
@@ -340,15 +353,27 @@ struct Trav4 {
                                         nd.z.w & m, best_t, &e2);
       h3 = (nd.ref.w >= 0) & qslab_test(qr, r.t_min, nd.x.y >> 16, nd.x.w >> 16, nd.y.y >> 16, nd.y.w >> 16,
                                         nd.z.y >> 16, nd.z.w >> 16, best_t, &e3);
-      c0 = h0 ? (nd.ref.x >> 30) : -1;
-      c1 = h1 ? (nd.ref.y >> 30) : -1;
-      c2 = h2 ? (nd.ref.z >> 30) : -1;
-      c3 = h3 ? (nd.ref.w >> 30) : -1;
-      r0 = nd.ref.x & (kQLeaf - 1);
-      r1 = nd.ref.y & (kQLeaf - 1);
-      r2 = nd.ref.z & (kQLeaf - 1);
-      r3 = nd.ref.w & (kQLeaf - 1);
+      float k0 = h0 ? e0 : kInf, k1 = h1 ? e1 : kInf, k2 = h2 ? e2 : kInf, k3 = h3 ? e3 : kInf;
+      r0 = h0 ? nd.ref.x : -1;
+      r1 = h1 ? nd.ref.y : -1;
+      r2 = h2 ? nd.ref.z : -1;
+      r3 = h3 ? nd.ref.w : -1;
+      cswap2(k0, r0, k1, r1);
+      cswap2(k2, r2, k3, r3);
+      cswap2(k0, r0, k2, r2);
+      cswap2(k1, r1, k3, r3);
+      cswap2(k1, r1, k2, r2);
+      if (qleaf(tris, k0, r0, motion) || qleaf(tris, k1, r1, motion) || qleaf(tris, k2, r2, motion) ||
+          qleaf(tris, k3, r3, motion))
+        return true;
+      int2 next = make_int2(-1, 0);
+      qinner(st, next, k3, r3);
+      qinner(st, next, k2, r2);
+      qinner(st, next, k1, r1);
+      qinner(st, next, k0, r0);
+      return advance(st, next);
     } else {
+      int c0, c1, c2, c3;
       h0 = (nd.cnt.x >= 0) & slab_test(r, nd.lox.x, nd.hix.x, nd.loy.x, nd.hiy.x, nd.loz.x, nd.hiz.x, best_t, &e0);
       h1 = (nd.cnt.y >= 0) & slab_test(r, nd.lox.y, nd.hix.y, nd.loy.y, nd.hiy.y, nd.loz.y, nd.hiz.y, best_t, &e1);
       h2 = (nd.cnt.z >= 0) & slab_test(r, nd.lox.z, nd.hix.z, nd.loy.z, nd.hiy.z, nd.loz.z, nd.hiz.z, best_t, &e2);
@@ -361,26 +386,32 @@ struct Trav4 {
       r1 = nd.ref.y;
       r2 = nd.ref.z;
       r3 = nd.ref.w;
+      float k0 = h0 ? e0 : kInf, k1 = h1 ? e1 : kInf, k2 = h2 ? e2 : kInf, k3 = h3 ? e3 : kInf;
+      // sort the four children by entry distance (misses sort last as +inf)
+      cswap(k0, r0, c0, k1, r1, c1);
+      cswap(k2, r2, c2, k3, r3, c3);
+      cswap(k0, r0, c0, k2, r2, c2);
+      cswap(k1, r1, c1, k3, r3, c3);
+      cswap(k1, r1, c1, k2, r2, c2);
+      // leaves first, nearest first; a child entered beyond the closest hit
+      // found meanwhile is skipped
+      if (leaf(tris, k0, r0, c0, motion) || leaf(tris, k1, r1, c1, motion) || leaf(tris, k2, r2, c2, motion) ||
+          leaf(tris, k3, r3, c3, motion))
+        return true;
+      // inner children: the nearest becomes the next node, the others are
+      // stacked farthest first
+      int2 next = make_int2(-1, 0);
+      inner(st, next, k3, r3, c3);
+      inner(st, next, k2, r2, c2);
+      inner(st, next, k1, r1, c1);
+      inner(st, next, k0, r0, c0);
+      return advance(st, next);
     }
-    float k0 = h0 ? e0 : kInf, k1 = h1 ? e1 : kInf, k2 = h2 ? e2 : kInf, k3 = h3 ? e3 : kInf;
-    // sort the four children by entry distance (misses sort last as +inf)
-    cswap(k0, r0, c0, k1, r1, c1);
-    cswap(k2, r2, c2, k3, r3, c3);
-    cswap(k0, r0, c0, k2, r2, c2);
-    cswap(k1, r1, c1, k3, r3, c3);
-    cswap(k1, r1, c1, k2, r2, c2);
-    // leaves first, nearest first; a child entered beyond the closest hit
-    // found meanwhile is skipped
-    if (leaf(tris, k0, r0, c0, motion) || leaf(tris, k1, r1, c1, motion) || leaf(tris, k2, r2, c2, motion) ||
-        leaf(tris, k3, r3, c3, motion))
-      return true;
-    // inner children: the nearest becomes the next node, the others are
-    // stacked farthest first
-    int2 next = make_int2(-1, 0);
-    inner(st, next, k3, r3, c3);
-    inner(st, next, k2, r2, c2);
-    inner(st, next, k1, r1, c1);
-    inner(st, next, k0, r0, c0);
+  }
+
+  // The next node: the nearest hit inner child, else the nearest stacked
+  // entry still in range; true when there is none (the ray is finished).
+  __device__ __forceinline__ bool advance(HybridStack<SL>& st, int2 next) {
     if (next.x >= 0) {
       cur = next.x;
       return false;
@@ -393,6 +424,21 @@ struct Trav4 {
       }
     }
     return true;
+  }
+
+  // QNODE: a hit leaf child (reference word with kQLeaf set); true when an
+  // any-hit query is answered.
+  __device__ __forceinline__ bool qleaf(const float4* __restrict__ tris, float k, int32_t ref,
+                                        const float4* __restrict__ motion) {
+    if (ref < kQLeaf || k > best_t) return false;
+    return leaf_hits<ANY, MOTION, true>(tris, r, ref & (kQLeaf - 1), 0, best_t, best_gid, bu, bv, found, motion);
+  }
+
+  // QNODE: a hit inner child (reference word in [0, kQLeaf)).
+  __device__ __forceinline__ void qinner(HybridStack<SL>& st, int2& next, float k, int32_t ref) const {
+    if ((uint32_t)ref >= kQLeaf || k > best_t) return;
+    if (next.x >= 0) st.push(next);
+    next = make_int2(ref, __builtin_bit_cast(int32_t, k));
   }
 
   // A hit leaf child: its triangles; true when an any-hit query is answered.
