@@ -109,6 +109,15 @@ typedef struct pyre_view {
  * scene bounds (main.rs:184-194). */
 int pyre_preset_load_pbrt(const char* path, pyre_preset** out, char* err, size_t err_len);
 
+/* pyre_preset_load_pbrt with flags. PYRE_PBRT_MESH_EMITTERS: an
+ * AreaLightSource on a shape that is not a four-vertex two-triangle
+ * trianglemesh (which stays a quad light) makes the shape an emissive mesh
+ * light (pyre_mesh_light) instead of the non-emissive primitive the
+ * reference's builder makes of it (io/pbrt/builder.rs:463-485,513-516).
+ * flags = 0 is pyre_preset_load_pbrt; unknown flags give PYRE_EINVAL. */
+#define PYRE_PBRT_MESH_EMITTERS 1u
+int pyre_preset_load_pbrt_ex(const char* path, uint32_t flags, pyre_preset** out, char* err, size_t err_len);
+
 /* main.rs:128-141 over load_gltf (io/gltf.rs:20-92): every triangle
  * primitive of the default scene (else the first scene), node transforms
  * accumulated parent * local and baked into the positions, normals by the

@@ -127,6 +127,24 @@ typedef struct pyre_mesh_desc {
   const float* motion; /* 6 floats or NULL */
 } pyre_mesh_desc;
 
+/* An emissive triangle mesh as one light: the general emitter the reference's
+ * PBRT builder drops (io/pbrt/builder.rs:463-485,513-516), behind the Light
+ * trait (light.rs:36-42). World-space positions and indices; each triangle
+ * emits `emission` along normalize(e1 x e2) (one-sided, as
+ * DiffuseAreaQuadLight). NEE draws one point per mesh: a triangle by area
+ * (Distribution1D, distribution.rs:39-101), then a uniform point on it; its
+ * pdf is dist^2 / (cos_light * total area). Like the quads, its triangles are
+ * not occluders (scene.rs:118-119) and are hit after the meshes with a strict
+ * t < t_max; light ids run over the quads, then these. */
+typedef struct pyre_mesh_light {
+  const float* positions;  /* 3 * n_vertices */
+  const uint32_t* indices; /* 3 * n_triangles */
+  uint32_t n_vertices;
+  uint32_t n_triangles; /* 1 .. PYRE_MESH_LIGHT_MAX_TRIS over the scene */
+  float emission[3];
+} pyre_mesh_light;
+#define PYRE_MESH_LIGHT_MAX_TRIS 65536u
+
 /* HdriEnvironmentLight::new arguments (light.rs:214-242): row-major RGB32F. */
 typedef struct pyre_env_desc {
   uint32_t width;
@@ -143,6 +161,8 @@ typedef struct pyre_scene_desc {
   const pyre_quad_light* lights;
   uint32_t n_lights;
   const pyre_env_desc* env; /* NULL = no environment (scene.rs:33) */
+  const pyre_mesh_light* mesh_lights; /* may be NULL when n_mesh_lights = 0 */
+  uint32_t n_mesh_lights;
 } pyre_scene_desc;
 
 /* Binned-SAH parameters (bvh.rs:13-16): bins (2..64), max primitives per

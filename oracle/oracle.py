@@ -75,6 +75,12 @@ def lib():
             "orc_scene_add_mesh": (_u32, [_vp, _f32p, _u32, _u32p, _u32, _vp, _u32, _u32, _u32]),
             "orc_scene_set_motion": (None, [_vp, _u32, _f32p, _f32p]),
             "orc_scene_add_light": (None, [_vp, _f32p]),
+            "orc_scene_add_mesh_light": (None, [_vp, _f32p, _u32, _u32p, _u32, _f32p]),
+            "orc_mesh_light_new": (_vp, [_f32p, _u32, _u32p, _u32, _f32p]),
+            "orc_mesh_light_free": (None, [_vp]),
+            "orc_mesh_light_area": (C.c_float, [_vp]),
+            "orc_mesh_light_sample": (None, [_vp, _f32p, _f32p, _u32, _u8p, _f32p, _f32p, _f32p, _f32p, _u32p]),
+            "orc_mesh_light_pdf": (None, [_vp, _f32p, _f32p, _u32, _f32p]),
             "orc_scene_set_env": (None, [_vp, _vp]),
             "orc_scene_num_prims": (_u32, [_vp]),
             "orc_scene_num_materials": (_u32, [_vp]),
@@ -266,6 +272,46 @@ def quad_pdf(light12, frm, wi):
     return out
 
 
+class MeshLight:
+    """Emissive triangle mesh behind the Light trait (light.rs:36-42); the
+    general emitter the reference's PBRT builder drops (io/pbrt/builder.rs:463-485)."""
+
+    def __init__(self, positions, indices, emission):
+        self._pos = f32(positions).reshape(-1)
+        self._idx = u32(indices).reshape(-1)
+        self.h = lib().orc_mesh_light_new(self._pos, self._pos.size // 3, self._idx, self._idx.size // 3,
+                                          f32(emission))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_mesh_light_free(self.h)
+            self.h = None
+
+    @property
+    def area(self):
+        return float(lib().orc_mesh_light_area(self.h))
+
+    def sample(self, frm, u):
+        frm = f32(frm).reshape(-1, 3)
+        u = f32(u).reshape(-1, 2)
+        n = len(frm)
+        ok = np.empty(n, np.uint8)
+        pos = np.empty((n, 3), np.float32)
+        wi = np.empty((n, 3), np.float32)
+        dist = np.empty(n, np.float32)
+        pdf = np.empty(n, np.float32)
+        tri = np.empty(n, np.uint32)
+        lib().orc_mesh_light_sample(self.h, frm, u, n, ok, pos, wi, dist, pdf, tri)
+        return pos, wi, dist, pdf, ok.astype(bool), tri
+
+    def pdf(self, frm, wi):
+        frm = f32(frm).reshape(-1, 3)
+        wi = f32(wi).reshape(-1, 3)
+        out = np.empty(len(frm), np.float32)
+        lib().orc_mesh_light_pdf(self.h, frm, wi, len(frm), out)
+        return out
+
+
 class Hdri:
     def __init__(self, width=None, height=None, rgb=None, intensity=1.0, gradient=None):
         if gradient is not None:
@@ -367,6 +413,11 @@ class Scene:
 
     def add_light(self, light12):
         lib().orc_scene_add_light(self.h, f32(light12))
+
+    def add_mesh_light(self, positions, indices, emission):
+        pos = f32(positions).reshape(-1)
+        idx = u32(indices).reshape(-1)
+        lib().orc_scene_add_mesh_light(self.h, pos, pos.size // 3, idx, idx.size // 3, f32(emission))
 
     def set_env(self, hdri: Hdri):
         hdri.owned = False  # ownership moves to the scene

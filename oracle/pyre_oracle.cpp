@@ -622,6 +622,86 @@ float QuadLight::pdf(V3 from, V3 wi) const {  // :135-144
 
 V3 QuadLight::le(V3, V3 n, V3 wo) const { return dot(n, wo) > 0.0f ? emission : splat(0.0f); }  // :146-152
 
+// Emissive triangle mesh (no reference counterpart; the design is stated at
+// its declaration).
+MeshLight::MeshLight(const float* pos, uint32_t, const uint32_t* idx, uint32_t nt, V3 em) : emission(em) {
+  std::vector<float> w(nt);
+  for (uint32_t i = 0; i < nt; ++i) {
+    const float* pa = pos + 3 * (size_t)idx[3 * i];
+    const float* pb = pos + 3 * (size_t)idx[3 * i + 1];
+    const float* pc = pos + 3 * (size_t)idx[3 * i + 2];
+    V3 a = v3(pa[0], pa[1], pa[2]), b = v3(pb[0], pb[1], pb[2]), c = v3(pc[0], pc[1], pc[2]);
+    V3 x = b - a, y = c - a;
+    V3 cr = cross(x, y);
+    v0.push_back(a);
+    v1.push_back(b);
+    v2.push_back(c);
+    e1.push_back(x);
+    e2.push_back(y);
+    normal.push_back(normalize(cr));
+    w[i] = 0.5f * length(cr);
+  }
+  dist = Distribution1D(w.data(), nt);
+  area = dist.integral;
+}
+
+bool MeshLight::intersect(V3 o, V3 d, float tmin, float tmax, float* t_out, uint32_t* tri_out) const {
+  bool any = false;
+  for (size_t i = 0; i < v0.size(); ++i) {
+    Ray r{o, d, tmin, tmax, 0.0f};
+    float t, u, v;
+    if (moller_trumbore(r, v0[i], v1[i], v2[i], &t, &u, &v)) {
+      tmax = t;
+      *t_out = t;
+      *tri_out = (uint32_t)i;
+      any = true;
+    }
+  }
+  return any;
+}
+
+bool MeshLight::sample(V3 from, V2 u, V3* pos, V3* wi_out, float* dist_out, V3* li, float* pdf,
+                       uint32_t* tri_out) const {
+  size_t lo = 1, hi = dist.cdf.size() - 1;  // sample_continuous's bin search (distribution.rs:80-90)
+  while (lo < hi) {
+    size_t mid = (lo + hi) / 2;
+    if (dist.cdf[mid] <= u.x)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  size_t i = std::min(lo - 1, dist.n - 1);
+  if (tri_out) *tri_out = (uint32_t)i;
+  float du = (u.x - dist.cdf[i]) / rmax(dist.cdf[i + 1] - dist.cdf[i], 1e-20f);
+  du = rclamp(du, 0.0f, 1.0f);
+  float s = std::sqrt(du);
+  V3 p = (v0[i] + (s * (1.0f - u.y)) * e1[i]) + (s * u.y) * e2[i];
+  V3 to_light = p - from;
+  float dist2 = length_squared(to_light);
+  if (dist2 <= 0.0f) return false;
+  float dl = std::sqrt(dist2);
+  V3 wi = to_light / dl;
+  float cos_light = -dot(wi, normal[i]);
+  if (cos_light <= 0.0f) return false;
+  *pdf = dist2 / (cos_light * area);
+  *pos = p;
+  *wi_out = wi;
+  *dist_out = dl;
+  *li = emission;
+  return true;
+}
+
+float MeshLight::pdf(V3 from, V3 wi) const {
+  float t;
+  uint32_t i;
+  if (!intersect(from, wi, 1e-4f, INFINITY, &t, &i)) return 0.0f;
+  float cos_light = -dot(wi, normal[i]);
+  if (cos_light <= 0.0f) return 0.0f;
+  return t * t / (cos_light * area);
+}
+
+V3 MeshLight::le(V3, V3 n, V3 wo) const { return dot(n, wo) > 0.0f ? emission : splat(0.0f); }
+
 // light.rs:214-242
 Hdri::Hdri(uint32_t w, uint32_t h, std::vector<V3> px, float inten)
     : width(w), height(h), pixels(std::move(px)), intensity(inten) {
@@ -791,6 +871,21 @@ int Scene::intersect(const Ray& ray, SurfaceInteraction* it_out, uint32_t* id_ou
       }
     }
   }
+  // emissive meshes after the quads, with the same strict t < t_max
+  for (size_t id = 0; id < mesh_lights.size(); ++id) {
+    float t;
+    uint32_t tri;
+    if (mesh_lights[id].intersect(ray.o, ray.d, ray.t_min, t_max, &t, &tri) && t < t_max) {
+      t_max = t;
+      it_out->t = t;
+      it_out->position = ray_at(ray, t);
+      it_out->normal = mesh_lights[id].normal[tri];
+      it_out->tri = 0;
+      it_out->u = it_out->v = 0.0f;
+      *id_out = (uint32_t)(lights.size() + id);
+      kind = 2;
+    }
+  }
   if (g_render_stats) {
     if (kind == 1) g_render_stats->ext_surface++;
     if (g_render_stats->first && kind != 0) g_render_stats->first_hit++;
@@ -874,12 +969,13 @@ PixelSample integrate(const Scene& sc, Ray ray, Sampler& sampler, uint32_t max_d
       aov_albedo = kind == 1 ? bsdf_albedo(sc.materials[sc.material_id[id]]) : splat(1.0f);
     }
     if (kind == 2) {
-      const QuadLight& light = sc.lights[id];
-      V3 le = light.le(hit.position, hit.normal, -ray.d);
+      const bool quad = id < sc.lights.size();
+      V3 le = quad ? sc.lights[id].le(hit.position, hit.normal, -ray.d)
+                   : sc.mesh_lights[id - sc.lights.size()].le(hit.position, hit.normal, -ray.d);
       if (last_was_specular) {
         l = l + beta * le;
       } else {
-        float pdf_light = light.pdf(ray.o, ray.d);
+        float pdf_light = quad ? sc.lights[id].pdf(ray.o, ray.d) : sc.mesh_lights[id - sc.lights.size()].pdf(ray.o, ray.d);
         float w_bsdf = power_heuristic(last_pdf_bsdf, pdf_light);
         l = l + beta * le * w_bsdf;
       }
@@ -892,6 +988,23 @@ PixelSample integrate(const Scene& sc, Ray ray, Sampler& sampler, uint32_t max_d
     V3 wo_local = to_local(frame, -ray.d);
     if (wo_local.z <= 0.0f) break;
     for (const auto& light : sc.lights) {
+      V2 u = sampler.next_vec2();
+      V3 lpos, lwi, lli;
+      float ldist, lpdf;
+      if (!light.sample(p, u, &lpos, &lwi, &ldist, &lli, &lpdf)) continue;
+      if (lpdf <= 0.0f || lli == ZERO) continue;
+      V3 wi_local = to_local(frame, lwi);
+      if (wi_local.z <= 0.0f) continue;
+      V3 f = bsdf_eval(mat, wo_local, wi_local);
+      if (f == ZERO) continue;
+      V3 p_offset = p + RAY_EPS * ns;
+      V3 target = lpos - RAY_EPS * lwi;
+      if (sc.occluded(p_offset, target, ray.time)) continue;
+      float pdf_bsdf = bsdf_pdf(mat, wo_local, wi_local);
+      float w_light = power_heuristic(lpdf, pdf_bsdf);
+      l = l + beta * f * lli * wi_local.z * w_light / lpdf;
+    }
+    for (const auto& light : sc.mesh_lights) {  // the same arm for each emissive mesh
       V2 u = sampler.next_vec2();
       V3 lpos, lwi, lli;
       float ldist, lpdf;
@@ -1256,6 +1369,36 @@ void orc_scene_set_motion(void* s, uint32_t prim, const float* off0, const float
 }
 void orc_scene_add_light(void* s, const float* L) {
   ((Scene*)s)->lights.emplace_back(ld3(L), ld3(L + 3), ld3(L + 6), ld3(L + 9));
+}
+void orc_scene_add_mesh_light(void* s, const float* pos, uint32_t nv, const uint32_t* idx, uint32_t nt,
+                              const float* em) {
+  ((Scene*)s)->mesh_lights.emplace_back(pos, nv, idx, nt, ld3(em));
+}
+// --- Emissive mesh probes: sample / pdf / intersect of a free-standing mesh light
+void* orc_mesh_light_new(const float* pos, uint32_t nv, const uint32_t* idx, uint32_t nt, const float* em) {
+  return new MeshLight(pos, nv, idx, nt, ld3(em));
+}
+void orc_mesh_light_free(void* m) { delete (MeshLight*)m; }
+float orc_mesh_light_area(void* m) { return ((MeshLight*)m)->area; }
+// per query: ok, pos[3], wi[3], dist, pdf, tri
+void orc_mesh_light_sample(void* m, const float* from, const float* u, uint32_t n, uint8_t* ok, float* pos,
+                           float* wi, float* dist, float* pdf, uint32_t* tri) {
+  const MeshLight& L = *(MeshLight*)m;
+  for (uint32_t i = 0; i < n; ++i) {
+    V3 p, w, li;
+    float d = 0, pd = 0;
+    uint32_t t = 0;
+    ok[i] = L.sample(ld3(from + 3 * i), V2{u[2 * i], u[2 * i + 1]}, &p, &w, &d, &li, &pd, &t);
+    st3(pos + 3 * i, ok[i] ? p : splat(0));
+    st3(wi + 3 * i, ok[i] ? w : splat(0));
+    dist[i] = ok[i] ? d : 0;
+    pdf[i] = ok[i] ? pd : 0;
+    tri[i] = t;
+  }
+}
+void orc_mesh_light_pdf(void* m, const float* from, const float* wi, uint32_t n, float* pdf) {
+  const MeshLight& L = *(MeshLight*)m;
+  for (uint32_t i = 0; i < n; ++i) pdf[i] = L.pdf(ld3(from + 3 * i), ld3(wi + 3 * i));
 }
 void orc_scene_set_env(void* s, void* hdri) {
   auto* sc = (Scene*)s;

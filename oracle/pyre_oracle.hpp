@@ -270,6 +270,37 @@ struct QuadLight {
   V3 le(V3 pos, V3 n, V3 wo) const;
 };
 
+// Emissive triangle mesh behind the Light trait (light.rs:36-42): the
+// general emitter that the reference's PBRT builder drops
+// (io/pbrt/builder.rs:463-485,513-516). The reference has no implementation,
+// so this states the design the device follows operation for operation:
+//  - triangle i: v0, e1 = v1 - v0, e2 = v2 - v0 (mesh.rs:154-155),
+//    normal = normalize(e1 x e2) (one-sided, emitting along +normal as
+//    DiffuseAreaQuadLight does), weight 0.5 |e1 x e2|;
+//  - dist: Distribution1D over the weights (distribution.rs:39-62); A = its
+//    integral (the total area);
+//  - sample(from, u): the triangle of sample_continuous's bin search on u.x
+//    (distribution.rs:78-101), du = (u.x - cdf[i]) / max(cdf[i+1] - cdf[i],
+//    1e-20) clamped to [0, 1], s = sqrt(du), point (v0 + (s (1 - u.y)) e1) +
+//    (s u.y) e2 (uniform in area); then DiffuseAreaQuadLight::sample's steps
+//    with pdf = dist^2 / (cos_light A) (light.rs:111-133);
+//  - intersect: Moller-Trumbore (mesh.rs:153-178) over the triangles in
+//    order, t in (t_min, t_max) with t_max tightened by each hit (the first
+//    of equal t wins);
+//  - pdf(from, wi): the hit of Ray{from, wi, 1e-4, inf}, t^2 / (cos_light A);
+//  - le: emission when normal . wo > 0 (light.rs:146-152).
+struct MeshLight {
+  std::vector<V3> v0, v1, v2, e1, e2, normal;
+  Distribution1D dist;
+  float area = 0.0f;
+  V3 emission;
+  MeshLight(const float* pos, uint32_t nv, const uint32_t* idx, uint32_t nt, V3 em);
+  bool intersect(V3 o, V3 d, float tmin, float tmax, float* t, uint32_t* tri) const;
+  bool sample(V3 from, V2 u, V3* pos, V3* wi, float* dist, V3* li, float* pdf, uint32_t* tri = nullptr) const;
+  float pdf(V3 from, V3 wi) const;
+  V3 le(V3 pos, V3 n, V3 wo) const;
+};
+
 // light.rs:205-350
 struct Hdri {
   uint32_t width = 0, height = 0;
@@ -301,6 +332,7 @@ struct Scene {
   std::vector<uint32_t> material_id;
   std::vector<Material> materials;
   std::vector<QuadLight> lights;
+  std::vector<MeshLight> mesh_lights;  // light ids after the quads'
   Hdri* env = nullptr;
   ~Scene();
   // kind: 0 miss, 1 surface, 2 light

@@ -233,10 +233,20 @@ class Preset:
         return cls(h, lib)
 
     @classmethod
-    def from_pbrt(cls, path) -> "Preset":
+    def from_pbrt(cls, path, mesh_emitters: bool = False) -> "Preset":
         """load_pbrt (io/pbrt/mod.rs:56-76): the reference's PBRT subset; the
-        file's LookAt/fov and Film resolution are in .view."""
-        return cls._load("pyre_preset_load_pbrt", path)
+        file's LookAt/fov and Film resolution are in .view. mesh_emitters:
+        area lights on non-quad shapes become emissive mesh lights instead of
+        being dropped (PYRE_PBRT_MESH_EMITTERS)."""
+        if not mesh_emitters:
+            return cls._load("pyre_preset_load_pbrt", path)
+        lib = L.load()
+        h = C.c_void_p()
+        err = C.create_string_buffer(512)
+        rc = lib.pyre_preset_load_pbrt_ex(os.fsencode(path), 1, C.byref(h), err, 512)
+        if rc:
+            raise PyreError(rc, f"pyre_preset_load_pbrt_ex({path}): {err.value.decode(errors='replace')}")
+        return cls(h, lib)
 
     @classmethod
     def from_gltf(cls, path) -> "Preset":
@@ -292,7 +302,13 @@ class Preset:
             e = d.env.contents
             env = dict(width=e.width, height=e.height, intensity=e.intensity,
                        rgb=np.ctypeslib.as_array(e.rgb, (e.height, e.width, 3)).copy())
-        return dict(meshes=meshes, materials=mats, lights=lights, env=env)
+        mesh_lights = []
+        for i in range(d.n_mesh_lights):
+            m = d.mesh_lights[i]
+            mesh_lights.append(dict(positions=np.ctypeslib.as_array(m.positions, (m.n_vertices, 3)).copy(),
+                                    indices=np.ctypeslib.as_array(m.indices, (m.n_triangles, 3)).copy(),
+                                    emission=tuple(m.emission)))
+        return dict(meshes=meshes, materials=mats, lights=lights, env=env, mesh_lights=mesh_lights)
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -317,9 +333,11 @@ class Scene:
         return cls(ctx, preset.desc(), keep=preset)
 
     @classmethod
-    def from_arrays(cls, ctx: Context, meshes, materials, lights=(), env=None) -> "Scene":
+    def from_arrays(cls, ctx: Context, meshes, materials, lights=(), env=None, mesh_lights=()) -> "Scene":
         """meshes: dicts with positions (n,3), indices (m,3), optional normals,
-        material_id and motion ((offset_t0, offset_t1), InstanceMotion)."""
+        material_id and motion ((offset_t0, offset_t1), InstanceMotion).
+        mesh_lights: dicts with positions, indices and emission (emissive
+        triangle meshes, lit after the quad lights)."""
         keep = []
         md = (L.MeshDesc * max(1, len(meshes)))()
         for i, m in enumerate(meshes):
@@ -345,8 +363,15 @@ class Scene:
             ed = L.EnvDesc(env["width"], env["height"], rgb.ctypes.data_as(L._fp), env.get("intensity", 1.0))
             keep.append(ed)
             envp = C.pointer(ed)
-        d = L.SceneDesc(md, len(meshes), mats, len(materials), lts, len(lights), envp)
-        keep += [md, mats, lts]
+        mls = (L.MeshLight * max(1, len(mesh_lights)))()
+        for i, m in enumerate(mesh_lights):
+            pos = _f32(m["positions"]).reshape(-1)
+            idx = _u32(m["indices"]).reshape(-1)
+            keep += [pos, idx]
+            mls[i] = L.MeshLight(pos.ctypes.data_as(L._fp), idx.ctypes.data_as(L._u32p), pos.size // 3, idx.size // 3,
+                                 (C.c_float * 3)(*[float(x) for x in m["emission"]]))
+        d = L.SceneDesc(md, len(meshes), mats, len(materials), lts, len(lights), envp, mls, len(mesh_lights))
+        keep += [md, mats, lts, mls]
         return cls(ctx, d, keep=keep)
 
     def bounds(self):

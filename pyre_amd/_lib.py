@@ -64,10 +64,15 @@ class EnvDesc(C.Structure):
     _fields_ = [("width", _u32), ("height", _u32), ("rgb", _fp), ("intensity", _f)]
 
 
+class MeshLight(C.Structure):
+    _fields_ = [("positions", _fp), ("indices", _u32p), ("n_vertices", _u32), ("n_triangles", _u32),
+                ("emission", _f * 3)]
+
+
 class SceneDesc(C.Structure):
     _fields_ = [("meshes", C.POINTER(MeshDesc)), ("n_meshes", _u32), ("materials", C.POINTER(Material)),
                 ("n_materials", _u32), ("lights", C.POINTER(QuadLight)), ("n_lights", _u32),
-                ("env", C.POINTER(EnvDesc))]
+                ("env", C.POINTER(EnvDesc)), ("mesh_lights", C.POINTER(MeshLight)), ("n_mesh_lights", _u32)]
 
 
 class BvhParams(C.Structure):
@@ -139,6 +144,7 @@ SIGNATURES = {
     "pyre_preset_destroy": (None, [_vp]),
     "pyre_auto_frame": (None, [_fp, _fp, _f, _f, _fp, _fp]),
     "pyre_preset_load_pbrt": (C.c_int, [C.c_char_p, C.POINTER(_vp), C.c_char_p, C.c_size_t]),
+    "pyre_preset_load_pbrt_ex": (C.c_int, [C.c_char_p, _u32, C.POINTER(_vp), C.c_char_p, C.c_size_t]),
     "pyre_preset_load_gltf": (C.c_int, [C.c_char_p, C.POINTER(_vp), C.c_char_p, C.c_size_t]),
     "pyre_load_hdr": (C.c_int, [C.c_char_p, C.POINTER(_u32), C.POINTER(_u32), _vp, C.c_size_t, C.c_char_p,
                                 C.c_size_t]),

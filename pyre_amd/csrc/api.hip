@@ -164,6 +164,9 @@ void scene_free(pyre_scene* s) {
   dev_free(s->d_mesh_info);
   dev_free(s->d_mats);
   dev_free(s->d_lights);
+  dev_free(s->d_mlights);
+  dev_free(s->d_etris);
+  dev_free(s->d_ecdf);
   dev_free(s->d_env_texels);
   dev_free(s->d_env_cond);
   dev_free(s->d_env_marg);
@@ -436,6 +439,63 @@ static const HostPhiTable& host_phi_table() {
   return table;
 }
 
+// Emissive meshes (pyre_mesh_light): per triangle v0, e1 = v1 - v0,
+// e2 = v2 - v0 (the f32 operations of mesh.rs:154-155), normalize(e1 x e2)
+// and the weight 0.5 |e1 x e2|; per mesh Distribution1D::new over the
+// weights (distribution.rs:39-62). A mesh whose total area is not a positive
+// finite number is refused (its pdf would divide by it).
+void upload_mesh_lights(pyre_scene* s, const pyre_scene_desc& d, hipStream_t st) {
+  s->n_mlights = d.n_mesh_lights;
+  if (!d.n_mesh_lights) return;
+  require(d.mesh_lights != nullptr, "mesh_lights pointer is null");
+  std::vector<float4> tris;
+  std::vector<float> cdf;
+  size_t total = 0;
+  for (uint32_t i = 0; i < d.n_mesh_lights; ++i) {
+    const pyre_mesh_light& m = d.mesh_lights[i];
+    require(m.n_triangles > 0 && m.positions && m.indices, "mesh light without triangles");
+    total += m.n_triangles;
+    require(total <= PYRE_MESH_LIGHT_MAX_TRIS, "mesh lights hold more than PYRE_MESH_LIGHT_MAX_TRIS triangles");
+    MeshLightDev L{};
+    L.tri0 = (uint32_t)(tris.size() / 4);
+    L.ntri = m.n_triangles;
+    L.cdf0 = (uint32_t)cdf.size();
+    L.emission = v3(m.emission[0], m.emission[1], m.emission[2]);
+    std::vector<float> w(m.n_triangles);
+    for (uint32_t t = 0; t < m.n_triangles; ++t) {
+      uint32_t ia = m.indices[3 * t], ib = m.indices[3 * t + 1], ic = m.indices[3 * t + 2];
+      require(ia < m.n_vertices && ib < m.n_vertices && ic < m.n_vertices, "mesh light index out of range");
+      V3 a = ld3(m.positions + 3 * (size_t)ia), b = ld3(m.positions + 3 * (size_t)ib),
+         c = ld3(m.positions + 3 * (size_t)ic);
+      V3 e1 = b - a, e2 = c - a;
+      V3 cr = cross(e1, e2);
+      V3 n = normalize(cr);
+      tris.push_back(make_float4(a.x, a.y, a.z, 0.0f));
+      tris.push_back(make_float4(e1.x, e1.y, e1.z, 0.0f));
+      tris.push_back(make_float4(e2.x, e2.y, e2.z, 0.0f));
+      tris.push_back(make_float4(n.x, n.y, n.z, 0.0f));
+      w[t] = 0.5f * length(cr);
+    }
+    float acc = 0.0f;
+    cdf.push_back(0.0f);
+    for (float x : w) {
+      acc += std::fmax(x, 0.0f);
+      cdf.push_back(acc);
+    }
+    require(acc > 0.0f && std::isfinite(acc), "mesh light with no area");
+    for (size_t k = L.cdf0; k < cdf.size(); ++k) cdf[k] /= acc;
+    L.area = acc;
+    s->h_mlights.push_back(L);
+  }
+  s->n_etris = (uint32_t)total;
+  s->d_mlights = dev_alloc<MeshLightDev>(s->h_mlights.size());
+  h2d(s->d_mlights, s->h_mlights.data(), s->h_mlights.size(), st);
+  s->d_etris = dev_alloc<float4>(tris.size());
+  h2d(s->d_etris, tris.data(), tris.size(), st);
+  s->d_ecdf = dev_alloc<float>(cdf.size());
+  h2d(s->d_ecdf, cdf.data(), cdf.size(), st);
+}
+
 extern "C" {
 
 int pyre_ctx_create(int device, pyre_ctx** out) {
@@ -606,6 +666,7 @@ int pyre_scene_upload(pyre_ctx* ctx, const pyre_scene_desc* desc, pyre_scene** o
     }
     s->d_lights = dev_alloc<QuadLightDev>(std::max<size_t>(d.n_lights, 1));
     h2d(s->d_lights, s->h_lights.data(), d.n_lights, st);
+    upload_mesh_lights(s, d, st);
     if (d.env) upload_env(s, *d.env, st);
     // Scene::bounds over the referenced vertices
     DevVec<int32_t> keys(6);

@@ -138,6 +138,16 @@ void preset_finish(pyre_preset* p) {
     d.motion = m.motion.size() == 6 ? m.motion.data() : nullptr;
     p->mdesc.push_back(d);
   }
+  p->mldesc.clear();
+  for (auto& m : p->mesh_lights) {
+    pyre_mesh_light d{};
+    d.positions = m.pos.data();
+    d.indices = m.idx.data();
+    d.n_vertices = (uint32_t)(m.pos.size() / 3);
+    d.n_triangles = (uint32_t)(m.idx.size() / 3);
+    for (int k = 0; k < 3; ++k) d.emission[k] = m.emission[k];
+    p->mldesc.push_back(d);
+  }
   p->edesc.width = p->env_w;
   p->edesc.height = p->env_h;
   p->edesc.rgb = p->env_rgb.data();
@@ -406,6 +416,8 @@ int pyre_preset_desc(const pyre_preset* p, pyre_scene_desc* d) {
   d->n_materials = (uint32_t)p->materials.size();
   d->lights = p->lights.data();
   d->n_lights = (uint32_t)p->lights.size();
+  d->mesh_lights = p->mldesc.empty() ? nullptr : p->mldesc.data();
+  d->n_mesh_lights = (uint32_t)p->mldesc.size();
   d->env = p->env_w ? &p->edesc : nullptr;
   return PYRE_OK;
 }

@@ -21,7 +21,7 @@
 // parameters live in device memory, so a pass needs no host synchronisation
 // and is captured once into a hipGraph and replayed. RNG draws happen in
 // exactly the reference's order per path (jitter x/y, lens x/y, time; per
-// surface bounce: 2 per light, 2 for the environment, 2 for the BSDF, 1 for
+// surface bounce: 2 per light (quads, then emissive meshes), 2 for the environment, 2 for the BSDF, 1 for
 // Russian roulette), so a GPU path consumes the same stream as the CPU one.
 //
 // Radiance cache (no counterpart in the reference): a CACHE pass ends a
@@ -71,6 +71,11 @@ struct SceneDev {
   EnvDev env;
   const float4* motion;  // 2 per mesh (offset_t0, offset_t1) when some mesh moves, else null
   PhiTable phi;          // the host libm's sinf/cosf on the BSDF samplers' grid
+  // emissive meshes (pyre_mesh_light): light ids n_lights .. n_lights + n_mlights - 1
+  const MeshLightDev* mlights = nullptr;
+  const float4* etris = nullptr;
+  const float* ecdf = nullptr;
+  uint32_t n_mlights = 0;
 };
 
 // One render job: scene, camera, image and its sums (film.rs:38-109).
@@ -508,7 +513,7 @@ __device__ bool shade_path(const Paths& P, const Sched& sc, const JobDev& J, con
   // Scene::intersect light loop (scene.rs:87-95): strict t < t_max.
   int kind = gid != PYRE_MISS ? 1 : 0;
   float t_max = kind ? hv.y : rb.w;
-  uint32_t light_id = 0;
+  uint32_t light_id = 0, light_tri = 0;
   V3 lpos = splat(0.0f);
   {
     for (uint32_t li = 0; li < S.n_lights; ++li) {
@@ -519,6 +524,18 @@ __device__ bool shade_path(const Paths& P, const Sched& sc, const JobDev& J, con
         kind = 2;
         light_id = li;
         lpos = pos;
+      }
+    }
+    // emissive meshes after the quads, with the same strict t < t_max
+    for (uint32_t mi = 0; mi < S.n_mlights; ++mi) {
+      float t;
+      uint32_t tri;
+      if (mlight_intersect(S.mlights[mi], S.etris, ro, rd, r_tmin, t_max, &t, &tri) && t < t_max) {
+        t_max = t;
+        kind = 2;
+        light_id = S.n_lights + mi;
+        light_tri = tri;
+        lpos = ro + rd * t;
       }
     }
     if (PART == kShadeSplit && kind == 1) return true;
@@ -556,7 +573,8 @@ __device__ bool shade_path(const Paths& P, const Sched& sc, const JobDev& J, con
     ns = surface_normal(S, gid, mesh, hv.z, hv.w);
   } else {
     pos = lpos;
-    ns = S.lights[light_id].normal;
+    ns = light_id < S.n_lights ? S.lights[light_id].normal
+                               : mlight_normal(S.mlights[light_id - S.n_lights], S.etris, light_tri);
   }
   if (!(flags & FL_AOV)) {  // integrator.rs:96-110
     flags |= FL_AOV;
@@ -565,12 +583,14 @@ __device__ bool shade_path(const Paths& P, const Sched& sc, const JobDev& J, con
     P.aov1[p] = f4(ns, 0.0f);
   }
   if (kind == 2) {  // a light hit: integrator.rs:113-129
-    const QuadLightDev& L = S.lights[light_id];
-    V3 le = dot(ns, -rd) > 0.0f ? L.emission : splat(0.0f);  // light.rs:146-152
+    const bool quad = light_id < S.n_lights;
+    const V3 em = quad ? S.lights[light_id].emission : S.mlights[light_id - S.n_lights].emission;
+    V3 le = dot(ns, -rd) > 0.0f ? em : splat(0.0f);  // light.rs:146-152
     if (specular) {
       l = l + beta * le;
     } else {
-      float pdf_light = quad_pdf(L, ro, rd);
+      float pdf_light = quad ? quad_pdf(S.lights[light_id], ro, rd)
+                             : mlight_pdf(S.mlights[light_id - S.n_lights], S.etris, ro, rd);
       float w = power_heuristic(last_pdf, pdf_light);
       l = l + beta * le * w;
     }
@@ -620,18 +640,13 @@ __device__ bool shade_path(const Paths& P, const Sched& sc, const JobDev& J, con
   Rng rng = depth == 0 ? camera_stream(sc, pp, p) : (reseed ? split_stream(sc, pp, p) : load_rng(P, p));
   uint32_t ns_count = 0;
   const size_t slot0 = (size_t)p * P.slots;
-  // NEE toward each quad light (integrator.rs:142-167)
-  for (uint32_t li = 0; li < S.n_lights; ++li) {
-    V2 u = rng_vec2(rng);
-    const QuadLightDev& L = S.lights[li];
-    V3 lp, lwi;
-    float ldist, lpdf;
-    if (!quad_sample(L, pos, u, &lp, &lwi, &ldist, &lpdf)) continue;
-    if (lpdf <= 0.0f || is_zero(L.emission)) continue;
+  // NEE toward a point sampled on a light (integrator.rs:142-167)
+  auto nee_point = [&](V3 lp, V3 lwi, float lpdf, V3 li) {
+    if (lpdf <= 0.0f || is_zero(li)) return;
     V3 wi = to_local(frame, lwi);
-    if (wi.z <= 0.0f) continue;
+    if (wi.z <= 0.0f) return;
     V3 f = bsdf_eval(mat, wo, wi);
-    if (is_zero(f)) continue;
+    if (is_zero(f)) return;
     V3 p_off = pos + kRayEps * ns;
     V3 target = lp - kRayEps * lwi;
     // Scene::occluded ray (scene.rs:120-135)
@@ -640,12 +655,26 @@ __device__ bool shade_path(const Paths& P, const Sched& sc, const JobDev& J, con
     V3 dir = to_from / dist;
     float pdf_bsdf = bsdf_pdf(mat, wo, wi);
     float w = power_heuristic(lpdf, pdf_bsdf);
-    V3 c = beta * f * L.emission * wi.z * w / lpdf;
+    V3 c = beta * f * li * wi.z * w / lpdf;
     size_t sl = slot0 + ns_count++;
     bool empty = !(dist > 0.0f);
     P.sray[2 * sl] = f4(p_off, 1e-3f);
     P.sray[2 * sl + 1] = f4(empty ? v3(0.0f, 0.0f, 1.0f) : dir, empty ? -1.0f : dist - 1e-3f);
     P.scontrib[sl] = f4(c, 0.0f);
+  };
+  for (uint32_t li = 0; li < S.n_lights; ++li) {  // each quad light
+    V2 u = rng_vec2(rng);
+    const QuadLightDev& L = S.lights[li];
+    V3 lp, lwi;
+    float ldist, lpdf;
+    if (quad_sample(L, pos, u, &lp, &lwi, &ldist, &lpdf)) nee_point(lp, lwi, lpdf, L.emission);
+  }
+  for (uint32_t mi = 0; mi < S.n_mlights; ++mi) {  // then each emissive mesh
+    V2 u = rng_vec2(rng);
+    const MeshLightDev& L = S.mlights[mi];
+    V3 lp, lwi;
+    float ldist, lpdf;
+    if (mlight_sample(L, S.etris, S.ecdf, pos, u, &lp, &lwi, &ldist, &lpdf)) nee_point(lp, lwi, lpdf, L.emission);
   }
   // environment NEE arm (integrator.rs:172-188)
   if (S.has_env) {
@@ -1258,8 +1287,13 @@ SceneDev scene_dev(pyre_scene* s) {
   PhiTable phi;
   phi.rank = s->ctx->d_phi_rank;
   phi.val = s->ctx->d_phi_val;
-  return SceneDev{s->d_nodes4, s->d_tris, s->d_verts, s->d_normals, s->d_tri_offset, s->d_mesh_blk, s->d_mesh_info, s->d_mats,
-                  s->d_lights, s->n_meshes, s->n_lights, s->has_env ? 1u : 0u, s->env, s->d_motion, phi};
+  SceneDev S{s->d_nodes4, s->d_tris, s->d_verts, s->d_normals, s->d_tri_offset, s->d_mesh_blk, s->d_mesh_info, s->d_mats,
+             s->d_lights, s->n_meshes, s->n_lights, s->has_env ? 1u : 0u, s->env, s->d_motion, phi};
+  S.mlights = s->d_mlights;
+  S.etris = s->d_etris;
+  S.ecdf = s->d_ecdf;
+  S.n_mlights = s->n_mlights;
+  return S;
 }
 
 JobDev job_dev(pyre_scene* s, const pyre_camera_desc& cam, const pyre_integrator_params& ip, uint32_t w, uint32_t h,
@@ -1571,13 +1605,13 @@ void session_init(pyre_session* S, pyre_ctx* ctx, pyre_scene* s, const pyre_came
   S->npix = w * h;
   S->tune = ctx_tuning(ctx);
   S->train_paths = train_paths;
-  S->slots = std::max(1u, s->n_lights + (s->has_env ? 1u : 0u));
+  S->slots = std::max(1u, s->n_lights + s->n_mlights + (s->has_env ? 1u : 0u));
   // vertices recorded per training path (at most 64: one bit each in the target pass)
   S->vmax = train_paths ? std::min(64u, std::max(1u, ip.max_depth)) : 0;
   S->tcap = train_paths ? std::min<uint32_t>(train_paths * S->vmax, 1u << 20) : 0;
   S->shape.max_depth = ip.max_depth;
   S->shape.tree_depth = s->depth;
-  S->shape.nee = s->n_lights + (s->has_env ? 1u : 0u) > 0;
+  S->shape.nee = s->n_lights + s->n_mlights + (s->has_env ? 1u : 0u) > 0;
   S->shape.one_job = true;
   S->shape.motion = s->has_motion;
   S->acc = dev_alloc<float>(10 * (size_t)S->npix);
@@ -1938,7 +1972,7 @@ int pyre_render_batch(pyre_ctx* ctx, const pyre_job* jobs, size_t n, const pyre_
       B.total_paths += (uint64_t)J.width * J.height * J.spp;
       B.shape.max_depth = std::max(B.shape.max_depth, J.integrator.max_depth);
       B.shape.tree_depth = std::max(B.shape.tree_depth, J.scene->depth);
-      const uint32_t sl = J.scene->n_lights + (J.scene->has_env ? 1u : 0u);
+      const uint32_t sl = J.scene->n_lights + J.scene->n_mlights + (J.scene->has_env ? 1u : 0u);
       B.shape.nee = B.shape.nee || sl > 0;
       B.shape.one_job = n == 1;
       B.shape.motion = B.shape.motion || J.scene->has_motion;

@@ -155,6 +155,15 @@ struct QuadLightDev {
   float inv_cross_len_sq, area;
 };
 
+// An emissive triangle mesh (pyre_mesh_light): its triangles are tri0 ..
+// tri0 + ntri - 1 of the scene's emitter arrays (4 float4 each: v0, e1, e2,
+// normal), its Distribution1D cdf (ntri + 1 entries) starts at cdf0.
+struct MeshLightDev {
+  uint32_t tri0, ntri, cdf0;
+  float area;  // the cdf's integral: the mesh's total area
+  V3 emission;
+};
+
 struct MatDev {
   uint32_t kind;
   V3 base;
@@ -188,6 +197,11 @@ struct pyre_scene {
   std::vector<uint32_t> h_mesh_info;   // material id | has_normals << 31
   std::vector<pyre::MatDev> h_mats;
   std::vector<pyre::QuadLightDev> h_lights;
+  uint32_t n_mlights = 0, n_etris = 0;  // emissive meshes and their triangles
+  std::vector<pyre::MeshLightDev> h_mlights;
+  pyre::MeshLightDev* d_mlights = nullptr;
+  float4* d_etris = nullptr;  // 4 per emitter triangle
+  float* d_ecdf = nullptr;    // the meshes' cdfs, one after another
   float* d_verts = nullptr;    // 9 floats per scene triangle (v0, v1, v2)
   float* d_normals = nullptr;  // 9 floats per triangle (vertex normals) or null
   bool has_motion = false;     // some mesh moves over the shutter

@@ -17,6 +17,13 @@ struct PresetMesh {
   std::vector<float> motion;  // empty, or InstanceMotion {offset_t0, offset_t1}
 };
 
+// An emissive triangle mesh (pyre_mesh_light).
+struct PresetMeshLight {
+  std::vector<float> pos;
+  std::vector<uint32_t> idx;
+  float emission[3] = {0, 0, 0};
+};
+
 enum PresetKind : int {
   PRESET_CORNELL = 0,    // the CLI's fixed Cornell camera
   PRESET_INSTANCES = 1,  // auto-framed over the instance bounds
@@ -28,6 +35,8 @@ struct pyre_preset {
   std::vector<PresetMesh> meshes;
   std::vector<pyre_material> materials;
   std::vector<pyre_quad_light> lights;
+  std::vector<PresetMeshLight> mesh_lights;
+  std::vector<pyre_mesh_light> mldesc;
   std::vector<float> env_rgb;
   uint32_t env_w = 0, env_h = 0;
   float env_intensity = 1.0f;

@@ -776,11 +776,28 @@ PresetMesh uv_sphere_mesh(float radius, uint32_t us, uint32_t vs, const M4& ctm)
   return m;
 }
 
-void add_shape(const std::string& kind, const Directive& d, const Attr& cur, pyre_preset* p) {
+// An emissive shape that is not a quad, as an emissive mesh light (with
+// PYRE_PBRT_MESH_EMITTERS): like the quad conversion, the shape becomes a
+// light and not a primitive.
+void add_mesh_emitter(const PresetMesh& m, const Attr& cur, pyre_preset* p) {
+  PresetMeshLight l;
+  l.pos = m.pos;
+  l.idx = m.idx;
+  l.emission[0] = cur.emission.x;
+  l.emission[1] = cur.emission.y;
+  l.emission[2] = cur.emission.z;
+  p->mesh_lights.push_back(std::move(l));
+}
+
+void add_shape(const std::string& kind, const Directive& d, const Attr& cur, pyre_preset* p, bool mesh_emitters) {
   if (kind == "sphere") {
     float radius = 1.0f;
     param_float(d, "radius", &radius);
     PresetMesh m = uv_sphere_mesh(radius, 48, 24, cur.ctm);
+    if (cur.area && mesh_emitters) {
+      add_mesh_emitter(m, cur, p);
+      return;
+    }
     // an area light on a sphere is ignored (builder.rs:509-518)
     m.material = cur.material >= 0 ? (uint32_t)cur.material : default_material(p);
     p->meshes.push_back(std::move(m));
@@ -831,6 +848,10 @@ void add_shape(const std::string& kind, const Directive& d, const Attr& cur, pyr
     p->lights.push_back(q);
     return;
   }
+  if (cur.area && mesh_emitters && !m.idx.empty()) {
+    add_mesh_emitter(m, cur, p);
+    return;
+  }
   m.material = cur.material >= 0 ? (uint32_t)cur.material : default_material(p);
   p->meshes.push_back(std::move(m));
 }
@@ -855,7 +876,7 @@ void add_light_source(const std::string& kind, const Directive& d, const std::st
   set_env(p, std::move(c), intensity);
 }
 
-void build_pbrt(const std::vector<Directive>& dirs, const std::string& base, pyre_preset* p) {
+void build_pbrt(const std::vector<Directive>& dirs, const std::string& base, pyre_preset* p, bool mesh_emitters) {
   std::map<std::string, uint32_t> named;
   std::vector<Attr> attr_stack;
   std::vector<M4> xform_stack;
@@ -949,7 +970,7 @@ void build_pbrt(const std::vector<Directive>& dirs, const std::string& base, pyr
       add_light_source(expect_first_string(d, "LightSource"), d, base, p);
     } else if (k == "Shape") {
       if (!in_world) continue;  // shapes outside the world block are ignored
-      add_shape(expect_first_string(d, "Shape"), d, cur, p);
+      add_shape(expect_first_string(d, "Shape"), d, cur, p, mesh_emitters);
     }
     // Sampler, Integrator, Accelerator, PixelFilter and every other
     // directive are skipped
@@ -1495,12 +1516,17 @@ int guarded(char* err, size_t err_len, F&& f) {
 extern "C" {
 
 int pyre_preset_load_pbrt(const char* path, pyre_preset** out, char* err, size_t err_len) {
+  return pyre_preset_load_pbrt_ex(path, 0u, out, err, err_len);
+}
+
+int pyre_preset_load_pbrt_ex(const char* path, uint32_t flags, pyre_preset** out, char* err, size_t err_len) {
   if (!path || !out) return PYRE_EINVAL;
+  if (flags & ~(uint32_t)PYRE_PBRT_MESH_EMITTERS) return PYRE_EINVAL;
   *out = nullptr;
   return guarded(err, err_len, [&] {
     std::unique_ptr<pyre_preset> p(new pyre_preset());
     p->kind = PRESET_LOADED;
-    build_pbrt(parse(tokenize(read_bytes(path))), dir_of(path), p.get());
+    build_pbrt(parse(tokenize(read_bytes(path))), dir_of(path), p.get(), (flags & PYRE_PBRT_MESH_EMITTERS) != 0);
     preset_scene_bounds(p.get());
     preset_finish(p.get());
     *out = p.release();

@@ -294,6 +294,78 @@ PH float quad_pdf(const QuadLightDev& L, V3 from, V3 wi) {  // :135-144
   return t * t / (cos_light * L.area);
 }
 
+// Emissive triangle mesh (pyre_mesh_light; no reference counterpart, the
+// design is stated in pyre_hip.h). et holds 4 float4 per emitter triangle
+// (v0, e1, e2, normal). The nearest Moller-Trumbore hit over the mesh's
+// triangles in order, t in (t_min, t_max) tightened by each hit, so the
+// first of equal t wins.
+PH bool mlight_intersect(const MeshLightDev& L, const float4* et, V3 o, V3 d, float tmin, float tmax, float* t_out,
+                         uint32_t* tri_out) {
+  bool any = false;
+  for (uint32_t i = 0; i < L.ntri; ++i) {
+    const float4* q = et + 4 * (size_t)(L.tri0 + i);
+    const float4 a = q[0], b = q[1], c = q[2];
+    float t, u, v;
+    if (mt_intersect(o, d, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), tmin, tmax, false, &t, &u, &v)) {
+      tmax = t;
+      *t_out = t;
+      *tri_out = i;
+      any = true;
+    }
+  }
+  return any;
+}
+
+PH V3 mlight_normal(const MeshLightDev& L, const float4* et, uint32_t tri) {
+  const float4 n = et[4 * (size_t)(L.tri0 + tri) + 3];
+  return v3(n.x, n.y, n.z);
+}
+
+// A triangle by the bin search of Distribution1D::sample_continuous on u.x
+// (distribution.rs:78-101), a uniform point on it from the bin's remapped
+// u.x and u.y, then DiffuseAreaQuadLight::sample's steps (light.rs:111-133)
+// with the mesh's total area.
+PH bool mlight_sample(const MeshLightDev& L, const float4* et, const float* ecdf, V3 from, V2 u, V3* pos,
+                      V3* wi_out, float* dist_out, float* pdf) {
+  const float* cdf = ecdf + L.cdf0;
+  uint32_t lo = 1, hi = L.ntri;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) / 2;
+    if (cdf[mid] <= u.x)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  const uint32_t i = lo - 1 < L.ntri - 1 ? lo - 1 : L.ntri - 1;
+  float du = (u.x - cdf[i]) / rmax(cdf[i + 1] - cdf[i], 1e-20f);
+  du = rclamp(du, 0.0f, 1.0f);
+  const float s = sqrtf(du);
+  const float4* q = et + 4 * (size_t)(L.tri0 + i);
+  const float4 a = q[0], b = q[1], c = q[2], nn = q[3];
+  V3 p = (v3(a.x, a.y, a.z) + (s * (1.0f - u.y)) * v3(b.x, b.y, b.z)) + (s * u.y) * v3(c.x, c.y, c.z);
+  V3 to_light = p - from;
+  float dist2 = length_squared(to_light);
+  if (dist2 <= 0.0f) return false;
+  float dist = sqrtf(dist2);
+  V3 wi = to_light / dist;
+  float cos_light = -dot(wi, v3(nn.x, nn.y, nn.z));
+  if (cos_light <= 0.0f) return false;
+  *pdf = dist2 / (cos_light * L.area);
+  *pos = p;
+  *wi_out = wi;
+  *dist_out = dist;
+  return true;
+}
+
+PH float mlight_pdf(const MeshLightDev& L, const float4* et, V3 from, V3 wi) {
+  float t;
+  uint32_t i;
+  if (!mlight_intersect(L, et, from, wi, 1e-4f, kInf, &t, &i)) return 0.0f;
+  float cos_light = -dot(wi, mlight_normal(L, et, i));
+  if (cos_light <= 0.0f) return 0.0f;
+  return t * t / (cos_light * L.area);
+}
+
 // ----------------------------------------------------------- distribution.rs
 // Distribution1D::sample_continuous over a cdf of n + 1 entries (:78-101).
 // The reference's binary search finds the first i in [1, n) with cdf[i] > u
