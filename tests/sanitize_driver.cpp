@@ -27,6 +27,11 @@ static double touch_preset(pyre_preset* p) {
   }
   for (uint32_t i = 0; i < d.n_materials; ++i) sum += d.materials[i].roughness;
   for (uint32_t i = 0; i < d.n_lights; ++i) sum += d.lights[i].emission[0];
+  for (uint32_t i = 0; i < d.n_mesh_lights; ++i) {
+    const pyre_mesh_light& ml = d.mesh_lights[i];
+    for (size_t k = 0; k < 3 * (size_t)ml.n_vertices; ++k) sum += ml.positions[k];
+    for (size_t k = 0; k < 3 * (size_t)ml.n_triangles; ++k) sum += ml.indices[k];
+  }
   if (d.env)
     for (size_t k = 0; k < 3 * (size_t)d.env->width * d.env->height; ++k) sum += d.env->rgb[k];
   pyre_camera_desc cam{};
@@ -79,6 +84,19 @@ int main(int argc, char** argv) {
       if (rc == 0 && p) {
         sum += touch_preset(p);
         pyre_preset_destroy(p);
+      }
+      if (kind == "pbrt") {  // again with area lights on any shape kept as emissive meshes
+        pyre_preset* q = nullptr;
+        char err2[512] = {0};
+        const int rc2 = pyre_preset_load_pbrt_ex(path, PYRE_PBRT_MESH_EMITTERS, &q, err2, sizeof err2);
+        if (rc2 != rc) {
+          std::fprintf(stderr, "pyre_preset_load_pbrt_ex returned %d where pyre_preset_load_pbrt returned %d\n", rc2, rc);
+          return 3;
+        }
+        if (rc2 == 0 && q) {
+          sum += touch_preset(q);
+          pyre_preset_destroy(q);
+        }
       }
     } else if (kind == "hdr") {
       uint32_t w = 0, hh = 0;

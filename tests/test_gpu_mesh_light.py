@@ -132,3 +132,25 @@ def test_pbrt_mesh_emitters_render_to_the_oracles_bits(ctx, oracle, tmp_path):
     assert same.mean() >= 0.99
     assert rel_mse(g[..., :3], o[..., :3]) < 1e-9
     assert o[..., :3].mean() > 0.01
+
+
+@pytest.mark.parametrize("pool", [3000, 1 << 20])
+def test_mesh_light_jobs_in_a_batch(ctx, oracle, pool):
+    """Jobs of scenes with 2 and 1 light slots share one path pool; each job
+    equals its own pyre_render (compared with the oracle above)."""
+    g_sc, _ = _scenes(ctx, oracle, True, True)
+    pre = pyre_amd.Preset.cornell()
+    c_sc = pyre_amd.Scene.from_preset(ctx, pre)
+    c_sc.build_bvh(bins=12, max_leaf=8)
+    cam = pre.camera(1.0)
+    ip = pyre_amd.PathIntegrator(8, 3)
+    jobs = [pyre_amd.Job(g_sc, cam, ip, 24, 24, 8), pyre_amd.Job(c_sc, cam, ip, 16, 16, 4),
+            pyre_amd.Job(g_sc, cam, ip, 20, 20, 4, 3)]
+    print(f"[mesh light batch] pool={pool}", flush=True)
+    outs, accs, st = pyre_amd.render_batch(ctx, jobs, pool_paths=pool, accum64=True)
+    for k, j in enumerate(jobs):
+        f = pyre_amd.Film(j.width, j.height)
+        ref = f.render(j.scene, j.camera, j.integrator, j.spp, spp_offset=j.spp_offset, accum64=True)
+        assert np.array_equal(np.ascontiguousarray(outs[k], np.float32).view(np.uint32),
+                              np.ascontiguousarray(ref, np.float32).view(np.uint32)), f"job {k}"
+        assert np.array_equal(accs[k], f.accum64), f"job {k}: FP64 sums"

@@ -45,6 +45,8 @@ def _corpus(tmp_path):
         files.append((kind, str(p)))
 
     add("pbrt", "cornell.pbrt", T._cornell_pbrt())
+    from tests.test_scene_io_emitters import SCENE as EMITTERS
+    add("pbrt", "emitters.pbrt", EMITTERS)
     w, h = 16, 8
     px = (np.arange(w * h * 4) % 251).astype(np.uint8).reshape(h, w, 4)
     px[..., 3] = 130
@@ -104,7 +106,7 @@ def test_loaders_and_sinks_under_asan_ubsan(driver, tmp_path):
     assert len(rc) == len(files)
     name = lambda n: str(tmp_path / n)
     # the valid files load; the crafted and truncated ones fail with a status
-    for good in ("cornell.pbrt", "flat.hdr", "rle.hdr", "g0.gltf"):
+    for good in ("cornell.pbrt", "emitters.pbrt", "flat.hdr", "rle.hdr", "g0.gltf"):
         assert rc[name(good)] == 0, good
     for bad in ("short.hdr", "huge.hdr", "g1.gltf", "g2.gltf", "g3.gltf", "g4.gltf", "g5.gltf", "g7.gltf"):
         assert rc[name(bad)] < 0, bad
