@@ -462,6 +462,7 @@ void upload_mesh_lights(pyre_scene* s, const pyre_scene_desc& d, hipStream_t st)
     L.cdf0 = (uint32_t)cdf.size();
     L.emission = v3(m.emission[0], m.emission[1], m.emission[2]);
     std::vector<float> w(m.n_triangles);
+    V3 lo = v3(kInf, kInf, kInf), hi = v3(-kInf, -kInf, -kInf);
     for (uint32_t t = 0; t < m.n_triangles; ++t) {
       uint32_t ia = m.indices[3 * t], ib = m.indices[3 * t + 1], ic = m.indices[3 * t + 2];
       require(ia < m.n_vertices && ib < m.n_vertices && ic < m.n_vertices, "mesh light index out of range");
@@ -470,6 +471,10 @@ void upload_mesh_lights(pyre_scene* s, const pyre_scene_desc& d, hipStream_t st)
       V3 e1 = b - a, e2 = c - a;
       V3 cr = cross(e1, e2);
       V3 n = normalize(cr);
+      for (const V3& q : {a, b, c}) {
+        lo = v3(std::fmin(lo.x, q.x), std::fmin(lo.y, q.y), std::fmin(lo.z, q.z));
+        hi = v3(std::fmax(hi.x, q.x), std::fmax(hi.y, q.y), std::fmax(hi.z, q.z));
+      }
       tris.push_back(make_float4(a.x, a.y, a.z, 0.0f));
       tris.push_back(make_float4(e1.x, e1.y, e1.z, 0.0f));
       tris.push_back(make_float4(e2.x, e2.y, e2.z, 0.0f));
@@ -483,6 +488,21 @@ void upload_mesh_lights(pyre_scene* s, const pyre_scene_desc& d, hipStream_t st)
       cdf.push_back(acc);
     }
     require(acc > 0.0f && std::isfinite(acc), "mesh light with no area");
+    require(std::isfinite(lo.x) && std::isfinite(lo.y) && std::isfinite(lo.z) && std::isfinite(hi.x) &&
+                std::isfinite(hi.y) && std::isfinite(hi.z),
+            "mesh light with a non-finite vertex");
+    // A hit that Moller-Trumbore accepts lies within rounding of a triangle;
+    // the pad (1e-4 of the box's size and of its coordinates' magnitude) is
+    // orders of magnitude wider than that rounding and than the slab test's.
+    {
+      const float ext = std::fmax(hi.x - lo.x, std::fmax(hi.y - lo.y, hi.z - lo.z));
+      const float mag = std::fmax(std::fmax(std::fabs(lo.x), std::fabs(hi.x)),
+                                  std::fmax(std::fmax(std::fabs(lo.y), std::fabs(hi.y)),
+                                            std::fmax(std::fabs(lo.z), std::fabs(hi.z))));
+      const float pad = 1e-4f * (ext + mag) + 1e-20f;
+      L.bmin = v3(lo.x - pad, lo.y - pad, lo.z - pad);
+      L.bmax = v3(hi.x + pad, hi.y + pad, hi.z + pad);
+    }
     for (size_t k = L.cdf0; k < cdf.size(); ++k) cdf[k] /= acc;
     L.area = acc;
     s->h_mlights.push_back(L);

@@ -162,6 +162,7 @@ struct MeshLightDev {
   uint32_t tri0, ntri, cdf0;
   float area;  // the cdf's integral: the mesh's total area
   V3 emission;
+  V3 bmin, bmax;  // the triangles' box, padded far beyond rounding (a cull test only)
 };
 
 struct MatDev {

@@ -299,9 +299,29 @@ PH float quad_pdf(const QuadLightDev& L, V3 from, V3 wi) {  // :135-144
 // (v0, e1, e2, normal). The nearest Moller-Trumbore hit over the mesh's
 // triangles in order, t in (t_min, t_max) tightened by each hit, so the
 // first of equal t wins.
+// Whether the line o + t d, t >= 0, meets the mesh's padded box: a cull test
+// that no ray Moller-Trumbore could accept fails. The interval starts
+// unbounded, and an axis whose terms are NaN (o on a plane of the box, d = 0
+// there) leaves it so (fmin / fmax drop NaN).
+PH bool mlight_box(const MeshLightDev& L, V3 o, V3 d) {
+  const float ix = 1.0f / d.x, iy = 1.0f / d.y, iz = 1.0f / d.z;
+  const float x0 = (L.bmin.x - o.x) * ix, x1 = (L.bmax.x - o.x) * ix;
+  const float y0 = (L.bmin.y - o.y) * iy, y1 = (L.bmax.y - o.y) * iy;
+  const float z0 = (L.bmin.z - o.z) * iz, z1 = (L.bmax.z - o.z) * iz;
+  float tn = -kInf, tf = kInf;
+  tn = fmaxf(tn, fminf(x0, x1));
+  tf = fminf(tf, fmaxf(x0, x1));
+  tn = fmaxf(tn, fminf(y0, y1));
+  tf = fminf(tf, fmaxf(y0, y1));
+  tn = fmaxf(tn, fminf(z0, z1));
+  tf = fminf(tf, fmaxf(z0, z1));
+  return tn <= tf && tf >= 0.0f;
+}
+
 PH bool mlight_intersect(const MeshLightDev& L, const float4* et, V3 o, V3 d, float tmin, float tmax, float* t_out,
                          uint32_t* tri_out) {
   bool any = false;
+  if (!mlight_box(L, o, d)) return false;
   for (uint32_t i = 0; i < L.ntri; ++i) {
     const float4* q = et + 4 * (size_t)(L.tri0 + i);
     const float4 a = q[0], b = q[1], c = q[2];
