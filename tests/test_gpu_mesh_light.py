@@ -154,3 +154,34 @@ def test_mesh_light_jobs_in_a_batch(ctx, oracle, pool):
         assert np.array_equal(np.ascontiguousarray(outs[k], np.float32).view(np.uint32),
                               np.ascontiguousarray(ref, np.float32).view(np.uint32)), f"job {k}"
         assert np.array_equal(accs[k], f.accum64), f"job {k}: FP64 sums"
+
+
+def test_mesh_lights_in_cache_renders(ctx, oracle):
+    """A radiance-cache render of a scene with a quad and an emissive mesh:
+    its plain frames (training paths riding in the same passes) give the
+    plain session's FP64 sums bit for bit, and a cache frame after them
+    queries the cache and stays finite."""
+    w, h = 32, 32
+    g_sc, _ = _scenes(ctx, oracle, True, True)
+    cam = pyre_amd.Preset.cornell().camera(w / h)
+    lo, hi = g_sc.bounds()
+    cache = pyre_amd.RadianceCache(ctx, lo, hi, seed=5)
+    ip = pyre_amd.PathIntegrator(8, 3)
+    kw = dict(spp_per_frame=2, train_paths=1 << 13, train_batch=1 << 12, train_steps=2)
+    print("[mesh light cache render]", flush=True)
+    s = pyre_amd.Session(g_sc, cam, ip, w, h, nrc_train_paths=1 << 13)
+    losses = s.render_nrc(cache, frames=2, plain_frames=2, **kw)
+    _, r64, spp = s.read(mean=False, radiance64=True)
+    assert spp == 4 and np.all(np.isfinite(losses))
+    assert s.stats()["training_samples"] > 0
+    p = pyre_amd.Session(g_sc, cam, ip, w, h)
+    p.render(4)
+    _, p64, _ = p.read(mean=False, radiance64=True)
+    p.close()
+    assert np.array_equal(r64.view(np.uint64), p64.view(np.uint64))
+    s.render_nrc(cache, frames=1, **kw)
+    assert s.stats()["cache_queries"] > 0
+    m, _, _ = s.read()
+    assert np.all(np.isfinite(m))
+    s.close()
+    cache.close()
